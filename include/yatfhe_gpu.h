@@ -1,0 +1,170 @@
+/* yatfhe_gpu.h -- C ABI of the MI355X TFHE programmable-bootstrapping engine.
+ *
+ * This is the boundary a caller of the reference library's bootstrapping path
+ * binds to.  Each entry point names the reference interface it replaces
+ * (paths relative to the reference repository root, xsong93/yatfhe).
+ *
+ * Conventions
+ *  - Every function returns an int status (YGPU_OK == 0, negative on error);
+ *    nothing throws across this boundary.  ygpu_last_error() returns the
+ *    calling thread's last message.
+ *  - Torus values (test polynomial v, accumulator) use the reference's Torus
+ *    type: int32_t when params.torus_bits == 32 (TORUS32), int64_t holding the
+ *    sign-extended 56-bit value when torus_bits == 56 (TORUS56).  They are passed
+ *    as void* and sized by torus_bits.
+ *  - LWE ciphertexts are int32_t[n+1] (a[0..n-1], b): every supported
+ *    configuration has qLwe = 2^32, so LweTorus values fit 32 bits; callers
+ *    built with LweTorus = int64_t narrow them losslessly.
+ *  - Buffers whose name starts with d_ are device pointers (from ygpu_malloc);
+ *    all others are host pointers owned by the caller.  The library owns every
+ *    device buffer it allocates itself (keys, scratch).
+ *  - Bootstrapping keys cross this boundary in coefficient form: for every
+ *    NTT-form key polynomial of the reference, its coefficient-domain residues
+ *    mod qNtt in [0, qNtt) (the inverse transform without centring).  The
+ *    engine's own transform order never crosses the interface, so keys produced
+ *    by any correct NTT (the reference uses Intel HEXL) are interchangeable.
+ *    Layout: bsk[i][lvl][row][col][N], i < n, lvl < level, row < k+1 (row r < k
+ *    is the RGSW row of the mask digit a_r, row k the row of the body digit b),
+ *    col < k+1 (col c < k is the mask polynomial a_c, col k the body b).
+ *  - KSK layout: ksk[i][j][n+1] int32, i < k*N, j < ks_level, entries
+ *    a[0..n-1] then b (TlweKeySwitchingKey::decomposedKsk, keyswitching.h:14-31).
+ *  - One context per GPU; calls on one context are serialised by the library.
+ */
+#ifndef YATFHE_GPU_H
+#define YATFHE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YGPU_OK 0
+#define YGPU_ERR_INVALID_ARG (-1)
+#define YGPU_ERR_UNSUPPORTED (-2)
+#define YGPU_ERR_HIP (-3)
+#define YGPU_ERR_NO_DEVICE (-4)
+#define YGPU_ERR_OUT_OF_MEMORY (-5)
+#define YGPU_ERR_NOT_FOUND (-6)
+
+/* Parameter set.  Mirrors YatfheParameters (yatfhe_parameters.h:9-110) for the
+ * fields the bootstrapping path reads; `level` is the gadget length used by the
+ * external product (lApprox == l for the key formats supported). */
+typedef struct ygpu_params {
+    int32_t torus_bits;     /* 32 or 56 (torus.h:11-21) */
+    int32_t n;              /* LWE dimension */
+    int32_t k;              /* GLWE dimension (1) */
+    int32_t N;              /* ring degree (1024 or 2048) */
+    int32_t radix_bits;     /* gadget base log2 */
+    int32_t level;          /* gadget levels */
+    int32_t ks_radix_bits;  /* keyswitch base log2 */
+    int32_t ks_level;       /* keyswitch levels */
+    int32_t ks_width_bits;  /* keyswitch precision (32) */
+    int32_t lwe_q_bits;     /* log2 qLwe (32) */
+    uint64_t q_ntt;         /* NTT prime: Q_49P_T32 or Q_61P_T56 (torus.h:151,157) */
+} ygpu_params;
+
+typedef struct ygpu_ctx ygpu_ctx;
+
+/* Multi-key job: `count` ciphertexts starting at row `first` of the batch
+ * buffers, all bootstrapped under key `key_id`. */
+typedef struct ygpu_job {
+    int32_t key_id;
+    int32_t count;
+    int64_t first;
+} ygpu_job;
+
+const char* ygpu_last_error(void);
+int ygpu_device_count(int32_t* count);
+
+/* Replaces initYatfhe (initializer.cpp:50-85): validates the parameter set,
+ * builds the transform tables and binds the context to `device`.  Unlike the
+ * reference's process-global state, contexts of different parameter sets
+ * coexist in one process. */
+int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out);
+int ygpu_destroy(ygpu_ctx* ctx);
+int ygpu_get_params(const ygpu_ctx* ctx, ygpu_params* out);
+
+/* Makes a bootstrapping key (BootstrappingKeyMP / BootstrappingKey,
+ * bootstrapping.h:16-114) and its keyswitching key (keyswitching.h:14-31)
+ * resident on the device, the BSK transformed into the engine's NTT form. */
+int ygpu_key_upload(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* key_id);
+int ygpu_key_release(ygpu_ctx* ctx, int32_t key_id);
+/* Device bytes held by a key (BSK in NTT form + KSK). */
+int ygpu_key_bytes(const ygpu_ctx* ctx, int32_t key_id, int64_t* bytes);
+
+/* Plain drop-in form, one ciphertext per call, host buffers.
+ * Replaces functionalBootstrappingMPNtt (bootstrapping.cpp:71-80) and, for
+ * keys with level == lApprox, functionalBootstrappingNtt (bootstrapping.cpp:60-69).
+ * lwe_in, lwe_out: int32[n+1].  v: Torus[N].  Optional (may be NULL):
+ * acc_out Torus[(k+1)N] (the blind-rotated accumulator, blindRotateMP21Ntt,
+ * blind_rotate.cpp:1538-1565), extract_out int32[kN+1] (extractTlweFromTrlwe,
+ * trlwe.cpp:241-250). */
+int ygpu_pbs(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v,
+             int32_t* lwe_out, void* acc_out, int32_t* extract_out);
+
+/* Blind rotation alone, host buffers: rescaleTlweToNewMod (tlwe.cpp:75-81) +
+ * genNoiselessTrlweSample (trlwe.cpp:175-182) + blindRotateMP21Ntt
+ * (blind_rotate.cpp:1538-1565).  acc_out: Torus[(k+1)N]. */
+int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v, void* acc_out);
+
+/* Sample extract + keyswitch of an accumulator, host buffers:
+ * extractTlweFromTrlwe (trlwe.cpp:241-250, index 0) then switchKeyForTlwe
+ * (keyswitching.cpp:48-72).  acc: Torus[(k+1)N]; extract_out optional. */
+int ygpu_extract_keyswitch(ygpu_ctx* ctx, int32_t key_id, const void* acc, int32_t* extract_out, int32_t* lwe_out);
+
+/* Batched form on device buffers: `count` ciphertexts under one key in one
+ * persistent blind-rotation launch, then extract + keyswitch.
+ * d_lwe_in, d_lwe_out: int32[count][n+1].  d_v: Torus[N] shared by all, or
+ * Torus[count][N] when v_per_ct != 0.  Optional: d_acc_out Torus[count][(k+1)N],
+ * d_extract_out int32[count][kN+1].  Asynchronous on the context's stream. */
+int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const int32_t* d_lwe_in,
+                          const void* d_v, int32_t v_per_ct, int32_t* d_lwe_out,
+                          void* d_acc_out, int32_t* d_extract_out);
+
+/* Multi-key batched form: jobs grouped by key in one persistent launch;
+ * buffers as in ygpu_pbs_batch_device with `total` rows. */
+int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64_t total,
+                         const int32_t* d_lwe_in, const void* d_v, int32_t v_per_ct,
+                         int32_t* d_lwe_out, void* d_acc_out);
+
+/* Plain form on device buffers (one ciphertext, the per-call persistent
+ * kernel), for timing the drop-in path without host copies. */
+int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in, const void* d_v,
+                          int32_t* d_lwe_out, void* d_acc_out);
+
+/* Device memory and stream helpers. */
+int ygpu_malloc(ygpu_ctx* ctx, size_t bytes, void** d_ptr);
+int ygpu_free(ygpu_ctx* ctx, void* d_ptr);
+int ygpu_memcpy_htod(ygpu_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+int ygpu_memcpy_dtoh(ygpu_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+int ygpu_sync(ygpu_ctx* ctx);
+
+/* Kernel timing: when enabled, the library brackets each kernel it launches
+ * with HIP events on the stream the kernel runs on and accumulates the
+ * durations per kernel name. */
+int ygpu_timing_enable(ygpu_ctx* ctx, int32_t on);
+int ygpu_timing_get(ygpu_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
+int ygpu_timing_reset(ygpu_ctx* ctx);
+
+/* Seeded key generation on the host (genTlweKey, genTrlweKey,
+ * genBootstrappingKeyMP, genTlweKeySwitchingKey: tlwe.cpp, trlwe.cpp:70-81,
+ * bootstrapping.cpp:150-174, keyswitching.cpp:19-29), with Gaussian noise of
+ * standard deviation 2^sigma_log2_lwe / 2^sigma_log2_glwe (fractions of the
+ * torus) rounded to integers.  Mask polynomials of the BSK are uniform mod qNtt,
+ * as the reference's NTT-domain sampling makes them.  Outputs:
+ * lwe_key int32[n] (bits), glwe_key int32[kN] (bits), bsk_coeff (layout above),
+ * ksk (layout above).  Deterministic in `seed` for any thread count. */
+int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe,
+                int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads);
+
+/* LWE encryption of torus messages mu[count] (symEncTlwe, tlwe.cpp:20-31) under
+ * lwe_key, Gaussian noise 2^sigma_log2, seeded.  out: int32[count][n+1]. */
+int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count,
+                     const int32_t* mu, int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YATFHE_GPU_H */

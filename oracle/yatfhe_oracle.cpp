@@ -1,0 +1,704 @@
+// yatfhe_oracle.cpp -- CPU oracle for the TFHE programmable-bootstrapping path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, on the CPU, the arithmetic of
+// the reference library's bootstrapping path (xsong93/yatfhe) so that the HIP
+// engine in yatfhe_amd/ can be checked against it bit for bit.  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+// as the checker or as the timed CPU baseline; the product never calls it.
+//
+// Every function names the reference file:line it follows (paths relative to
+// the reference repository root).  Torus values are carried as int64_t holding
+// the sign-extended w-bit value (w = 32 or 56): the reference's TORUS32 int32
+// wrap-around arithmetic equals int64 arithmetic followed by sign extension from
+// bit 31, which is how every result here is formed.
+//
+// Pinning (what checks each part independently of this text):
+//   modswitch (R9)          exhaustive closed-form check over all 2^32 inputs
+//   rotation                the reference's POLY.ROT known-answer table
+//   digits                  two restatements (scalar zeroMeanDigits form and the
+//                           level-major row form) compared exhaustively, plus the
+//                           reference's own header compiled under oracle/_ref
+//   NTT / MAC / INTT        O(N^2) schoolbook negacyclic product mod q
+//   external product        schoolbook route vs NTT route, bit for bit
+//   blind rotate / extract  decryption identities (phase = X^{-b+sum a s} v)
+//   keyswitch               int64 and uint32-wrap accumulations cross-checked,
+//                           decryption of every message slot
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <thread>
+#include <atomic>
+#include <algorithm>
+#include <stdexcept>
+
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+extern "C" {
+typedef struct or_params {
+    int32_t torus_bits;     // w: 32 (TORUS32) or 56 (TORUS56)
+    int32_t n;              // LWE dimension
+    int32_t k;              // GLWE dimension
+    int32_t N;              // ring degree
+    int32_t radix_bits;     // gadget base log2 (param.radixBits)
+    int32_t level;          // gadget levels used (lApprox == l)
+    int32_t ks_radix_bits;  // param.ksRadixBits
+    int32_t ks_level;       // param.ksLevel
+    int32_t ks_width_bits;  // param.ksWidthBits
+    int32_t lwe_q_bits;     // log2 qLwe (32)
+    uint64_t q_ntt;         // param.qNtt
+} or_params;
+}
+
+namespace {
+
+// ---------------------------------------------------------------- scalar rules
+// longModPow2 (numeric.h:69-71): sign extension from the low (64-shift) bits.
+inline int64_t sext(int64_t x, int w) {
+    const int shift = 64 - w;
+    return static_cast<int64_t>(static_cast<uint64_t>(x) << shift) >> shift;
+}
+
+// longModP (numeric.h:53-66), general modulus branch included for completeness.
+inline int64_t long_mod_p(int64_t a, int64_t p) {
+    if (p > 1 && (p & (p - 1)) == 0) {
+        const int shift = 64 - __builtin_ctzll(static_cast<uint64_t>(p));
+        return static_cast<int64_t>(static_cast<uint64_t>(a) << shift) >> shift;
+    }
+    int64_t b = a % p;
+    if (b > (p - 1) / 2) b -= p;
+    else if (b < -p / 2) b += p;
+    return b;
+}
+
+// modSwitchFromTorusGeneral (numeric.cpp:189-208).
+inline int64_t mod_switch_from_torus(int64_t in, int64_t newMod, int64_t torusQ) {
+    if (torusQ >= newMod) {
+        const int64_t scale = torusQ / newMod;
+        const int64_t num = in;
+        const int64_t real = (num >= 0) ? (num + scale / 2) / scale : (num - scale / 2) / scale;
+        return long_mod_p(real, newMod);
+    }
+    const int64_t scaleUp = newMod / torusQ;
+    if (scaleUp * torusQ == newMod) return long_mod_p(in * scaleUp, newMod);
+    const double div = static_cast<double>(in) * static_cast<double>(newMod) / static_cast<double>(torusQ);
+    const auto real = static_cast<int32_t>(std::round(div));
+    return long_mod_p(static_cast<int64_t>(real), newMod);
+}
+
+// validateRotator (polynomial.cpp:22-29).
+inline void validate_rotator(int& aTrue, int& isWrap, int a, int N) {
+    aTrue = a % (2 * N);
+    if (aTrue < 0) aTrue += 2 * N;
+    isWrap = (aTrue < N) ? 1 : -1;
+    aTrue = (aTrue < N) ? aTrue : aTrue - N;
+}
+
+// rotateTorusPolynomial (polynomial.cpp:184-193): out = X^a * in.
+void rotate(int w, int N, int a, const int64_t* in, int64_t* out) {
+    int aTrue, isWrap;
+    validate_rotator(aTrue, isWrap, a, N);
+    for (int i = 0; i < N; i++) {
+        const int64_t tmp = (i < aTrue) ? (-in[i - aTrue + N] * isWrap) : (in[i - aTrue] * isWrap);
+        out[i] = sext(tmp, w);
+    }
+}
+
+// rotateTorusPolynomialMinusOne (polynomial.cpp:213-238): out = (X^a - 1) * in,
+// power-of-two torus branch (subTorusPow2, numeric.h:74-76).
+void rotate_minus_one(int w, int N, int a, const int64_t* in, int64_t* out) {
+    int aTrue, isWrap;
+    validate_rotator(aTrue, isWrap, a, N);
+    for (int i = 0; i < aTrue; i++) {
+        const int64_t tmp = sext(-in[i - aTrue + N] * isWrap, w);
+        out[i] = sext(tmp - in[i], w);
+    }
+    for (int i = aTrue; i < N; i++) {
+        const int64_t tmp = sext(in[i - aTrue] * isWrap, w);
+        out[i] = sext(tmp - in[i], w);
+    }
+}
+
+// ---------------------------------------------------------------- digits
+// roundingBias / maskToWidth / windowShift (gadget_decomposition.h:52-68).
+inline uint64_t width_mask(int w) { return w >= 64 ? ~0ULL : ((1ULL << w) - 1); }
+inline uint64_t rounding_bias(int b, int w, int l) {
+    const int dropped = w - l * b;
+    return dropped > 0 ? (1ULL << (dropped - 1)) : 0;
+}
+
+// Form 1: zeroMeanDigits (gadget_decomposition.h:75-95), scalar, digit per level.
+void digits_scalar(int64_t in, int l, int b, int w, int64_t* out) {
+    const uint64_t mask = width_mask(w);
+    const uint64_t u = static_cast<uint64_t>(in) & mask;
+    const bool negative = (u >> (w - 1)) & 1;
+    const uint64_t m = (negative ? (0ULL - u) & mask : u) + rounding_bias(b, w, l);
+    const int64_t B = int64_t{1} << b;
+    const int64_t halfB = B >> 1;
+    const int64_t sign = negative ? -1 : 1;
+    int64_t carry = 0;
+    for (int level = l - 1; level >= 0; --level) {
+        const int shift = w - (level + 1) * b;
+        const int64_t window = static_cast<int64_t>((m >> shift) & static_cast<uint64_t>(B - 1));
+        const int64_t digit = window + carry;
+        const int64_t tieBit = level > 0 ? static_cast<int64_t>((m >> (shift + 2 * b - 1)) & 1) : 0;
+        carry = (digit > halfB) | ((digit == halfB) & tieBit);
+        out[level] = sign * (carry ? digit - B : digit);
+    }
+}
+
+// Form 2: decomposeRowByLevel (gadget_decomposition.h:110-152), level-major over
+// a whole row with flat per-value state.  out is [l][N].
+void digits_row(const int64_t* in, int N, int l, int b, int w, int64_t* out) {
+    const int64_t B = int64_t{1} << b;
+    const int64_t halfB = B >> 1;
+    const uint64_t mask = width_mask(w);
+    const uint64_t round = rounding_bias(b, w, l);
+    std::vector<uint64_t> mag(N);
+    std::vector<int64_t> sgn(N), car(N);
+    for (int j = 0; j < N; j++) {
+        const uint64_t value = static_cast<uint64_t>(in[j]) & mask;
+        const bool negative = (value >> (w - 1)) & 1;
+        mag[j] = (negative ? (0ULL - value) & mask : value) + round;
+        sgn[j] = negative ? -1 : 1;
+        car[j] = 0;
+    }
+    for (int level = l - 1; level >= 0; --level) {
+        const int shift = w - (level + 1) * b;
+        const int tieShift = level > 0 ? shift + 2 * b - 1 : 0;
+        const int64_t tieMask = level > 0 ? 1 : 0;
+        int64_t* o = out + static_cast<size_t>(level) * N;
+        for (int j = 0; j < N; j++) {
+            const int64_t window = static_cast<int64_t>((mag[j] >> shift) & static_cast<uint64_t>(B - 1));
+            const int64_t digit = window + car[j];
+            const int64_t tieBit = static_cast<int64_t>((mag[j] >> tieShift) & 1) & tieMask;
+            car[j] = (digit > halfB) | ((digit == halfB) & tieBit);
+            o[j] = (digit - car[j] * B) * sgn[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- mod q
+inline uint64_t mulmod(uint64_t a, uint64_t b, uint64_t q) {
+    return static_cast<uint64_t>((static_cast<u128>(a) * b) % q);
+}
+inline uint64_t powmod(uint64_t a, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    a %= q;
+    while (e) {
+        if (e & 1) r = mulmod(r, a, q);
+        a = mulmod(a, a, q);
+        e >>= 1;
+    }
+    return r;
+}
+// Shoup's multiplication by a fixed operand w with wp = floor(w * 2^64 / q).
+inline uint64_t shoup(uint64_t x, uint64_t w, uint64_t wp, uint64_t q) {
+    const uint64_t hi = static_cast<uint64_t>((static_cast<u128>(x) * wp) >> 64);
+    uint64_t r = x * w - hi * q;
+    return r >= q ? r - q : r;
+}
+inline uint64_t shoup_pre(uint64_t w, uint64_t q) {
+    return static_cast<uint64_t>((static_cast<u128>(w) << 64) / q);
+}
+
+// Smallest generator of Z_q^*, q prime with q-1 = 2^e * (small odd part).
+uint64_t find_generator(uint64_t q) {
+    std::vector<uint64_t> f;
+    uint64_t m = q - 1;
+    if ((m & 1) == 0) { f.push_back(2); while ((m & 1) == 0) m >>= 1; }
+    for (uint64_t p = 3; p * p <= m; p += 2) {
+        if (m % p == 0) { f.push_back(p); while (m % p == 0) m /= p; }
+    }
+    if (m > 1) f.push_back(m);
+    for (uint64_t g = 2;; g++) {
+        bool ok = true;
+        for (auto p : f) if (powmod(g, (q - 1) / p, q) == 1) { ok = false; break; }
+        if (ok) return g;
+    }
+}
+
+inline int bitrev(int x, int bits) {
+    int r = 0;
+    for (int i = 0; i < bits; i++) { r = (r << 1) | (x & 1); x >>= 1; }
+    return r;
+}
+
+// Textbook negacyclic NTT mod q (Cooley-Tukey forward, Gentleman-Sande inverse,
+// psi powers merged).  The reference uses Intel HEXL here (ntt_hexl.h:35-44,
+// ntt_hexl.cpp:153-175); the transform order and root are internal to each
+// implementation and never cross the interface: only the negacyclic product mod
+// q, which is unique, does.
+struct Ntt {
+    uint64_t q = 0;
+    int N = 0, logN = 0;
+    std::vector<uint64_t> psi, psip, ipsi, ipsip;  // bit-reversed powers, Shoup constants
+    uint64_t ninv = 0, ninvp = 0;
+
+    Ntt(uint64_t q_, int N_) : q(q_), N(N_) {
+        while ((1 << logN) < N) logN++;
+        const uint64_t g = find_generator(q);
+        const uint64_t root = powmod(g, (q - 1) / (2 * static_cast<uint64_t>(N)), q);
+        if (powmod(root, N, q) != q - 1) throw std::runtime_error("no primitive 2N-th root");
+        const uint64_t iroot = powmod(root, q - 2, q);
+        psi.resize(N); psip.resize(N); ipsi.resize(N); ipsip.resize(N);
+        for (int i = 0; i < N; i++) {
+            const int e = bitrev(i, logN);
+            psi[i] = powmod(root, e, q);
+            psip[i] = shoup_pre(psi[i], q);
+            ipsi[i] = powmod(iroot, e, q);
+            ipsip[i] = shoup_pre(ipsi[i], q);
+        }
+        ninv = powmod(N, q - 2, q);
+        ninvp = shoup_pre(ninv, q);
+    }
+    void forward(uint64_t* a) const {
+        for (int m = 1, t = N / 2; m < N; m <<= 1, t >>= 1) {
+            for (int i = 0; i < m; i++) {
+                const uint64_t w = psi[m + i], wp = psip[m + i];
+                for (int j = 2 * i * t; j < 2 * i * t + t; j++) {
+                    const uint64_t U = a[j];
+                    const uint64_t V = shoup(a[j + t], w, wp, q);
+                    uint64_t s = U + V; if (s >= q) s -= q;
+                    a[j] = s;
+                    a[j + t] = U >= V ? U - V : U + q - V;
+                }
+            }
+        }
+    }
+    void inverse(uint64_t* a) const {
+        for (int m = N / 2, t = 1; m >= 1; m >>= 1, t <<= 1) {
+            for (int i = 0; i < m; i++) {
+                const uint64_t w = ipsi[m + i], wp = ipsip[m + i];
+                for (int j = 2 * i * t; j < 2 * i * t + t; j++) {
+                    const uint64_t U = a[j], V = a[j + t];
+                    uint64_t s = U + V; if (s >= q) s -= q;
+                    a[j] = s;
+                    a[j + t] = shoup(U >= V ? U - V : U + q - V, w, wp, q);
+                }
+            }
+        }
+        for (int j = 0; j < N; j++) a[j] = shoup(a[j], ninv, ninvp, q);
+    }
+};
+
+// applyNtt's lift (ntt_hexl.h:38-42): signed v -> v + q if v < 0.
+inline uint64_t lift(int64_t v, uint64_t q) {
+    return static_cast<uint64_t>(v) + (v < 0 ? q : 0);
+}
+// applyIntt's exit (ntt_hexl.cpp:156-167): x in [0,q); x >= (q+1)/2 -> x - q;
+// then sign-extend the low w bits.
+inline int64_t intt_exit(uint64_t x, uint64_t q, int w) {
+    const uint64_t halfQ = (q + 1) >> 1;
+    const uint64_t centred = x - (x >= halfQ ? q : 0);
+    return sext(static_cast<int64_t>(centred), w);
+}
+
+// Brute-force negacyclic product mod q: out = a * b in Z_q[X]/(X^N+1), a signed
+// (small or torus), b residues in [0,q).  O(N^2), exact, no transform.
+void schoolbook_modq(uint64_t q, int N, const int64_t* a, const uint64_t* b, uint64_t* out) {
+    std::vector<i128> acc(N, 0);
+    for (int i = 0; i < N; i++) {
+        if (a[i] == 0) continue;
+        const i128 ai = a[i];
+        for (int j = 0; j < N; j++) {
+            const i128 p = ai * static_cast<i128>(b[j]);
+            const int d = i + j;
+            if (d < N) acc[d] += p; else acc[d - N] -= p;
+        }
+    }
+    for (int i = 0; i < N; i++) {
+        i128 r = acc[i] % static_cast<i128>(q);
+        if (r < 0) r += q;
+        out[i] = static_cast<uint64_t>(r);
+    }
+}
+
+// ---------------------------------------------------------------- keys
+struct Key {
+    or_params p;
+    Ntt ntt;
+    // BSK in this oracle's NTT order, [n][l][k+1 rows][k+1 cols][N], with Shoup constants.
+    std::vector<uint64_t> bsk, bskp;
+    // Coefficient-domain residues, kept for the schoolbook route.
+    std::vector<uint64_t> coeff;
+    Key(const or_params& p_, const uint64_t* bsk_coeff) : p(p_), ntt(p_.q_ntt, p_.N) {
+        const size_t per = static_cast<size_t>(p.level) * (p.k + 1) * (p.k + 1) * p.N;
+        const size_t total = per * p.n;
+        coeff.assign(bsk_coeff, bsk_coeff + total);
+        bsk = coeff;
+        bskp.resize(total);
+        for (size_t o = 0; o < total; o += p.N) ntt.forward(&bsk[o]);
+        for (size_t i = 0; i < total; i++) bskp[i] = shoup_pre(bsk[i], p.q_ntt);
+    }
+};
+
+// externalProductTrgswMPNttImpl (trgsw.cpp:712-767) for one BSK row, k+1 torus
+// polynomials in, k+1 out.  Decomposition into exactly `level` digits
+// (gadgetDecomposeTrlwe, trlwe.cpp:185-198, with DecomposedTrlwe{param, level},
+// trgsw.cpp:700); row r < k is the digit of a_r, row k the digit of b; MAC order
+// is irrelevant mod q (HEXL Eltwise*Mod are exact); one INTT per output column.
+void external_product(const Key& key, size_t i, const int64_t* in, int64_t* out, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K1 = p.k + 1, L = p.level, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    std::vector<int64_t> dig(static_cast<size_t>(K1) * L * N);   // [row][lvl][N]
+    for (int r = 0; r < K1; r++) digits_row(in + static_cast<size_t>(r) * N, N, L, p.radix_bits, w, &dig[static_cast<size_t>(r) * L * N]);
+    std::vector<uint64_t> acc(static_cast<size_t>(K1) * N, 0), tmp(N);
+    const size_t rowBase = i * static_cast<size_t>(L) * K1 * K1 * N;
+    for (int lvl = 0; lvl < L; lvl++) {
+        for (int r = 0; r < K1; r++) {
+            const int64_t* d = &dig[(static_cast<size_t>(r) * L + lvl) * N];
+            const size_t keyOff = rowBase + (static_cast<size_t>(lvl) * K1 + r) * K1 * N;
+            if (schoolbook) {
+                for (int c = 0; c < K1; c++) {
+                    schoolbook_modq(q, N, d, &key.coeff[keyOff + static_cast<size_t>(c) * N], tmp.data());
+                    for (int j = 0; j < N; j++) {
+                        uint64_t s = acc[static_cast<size_t>(c) * N + j] + tmp[j];
+                        acc[static_cast<size_t>(c) * N + j] = s >= q ? s - q : s;
+                    }
+                }
+            } else {
+                for (int j = 0; j < N; j++) tmp[j] = lift(d[j], q);
+                key.ntt.forward(tmp.data());
+                for (int c = 0; c < K1; c++) {
+                    const uint64_t* kb = &key.bsk[keyOff + static_cast<size_t>(c) * N];
+                    const uint64_t* kp = &key.bskp[keyOff + static_cast<size_t>(c) * N];
+                    uint64_t* a = &acc[static_cast<size_t>(c) * N];
+                    for (int j = 0; j < N; j++) {
+                        uint64_t s = a[j] + shoup(tmp[j], kb[j], kp[j], q);
+                        a[j] = s >= q ? s - q : s;
+                    }
+                }
+            }
+        }
+    }
+    for (int c = 0; c < K1; c++) {
+        uint64_t* a = &acc[static_cast<size_t>(c) * N];
+        if (!schoolbook) key.ntt.inverse(a);
+        for (int j = 0; j < N; j++) out[static_cast<size_t>(c) * N + j] = intt_exit(a[j], q, w);
+    }
+}
+
+// blindRotateMP21Ntt (blind_rotate.cpp:1538-1565), binary-key branch:
+// skip a_i == 0; tmp = (X^{a_i} - 1) acc; tmp = ExtProd(BSK_i, tmp); acc += tmp.
+void blind_rotate(const Key& key, const int32_t* abar, int64_t* acc, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K1 = p.k + 1, w = p.torus_bits;
+    std::vector<int64_t> tmp(static_cast<size_t>(K1) * N), ext(static_cast<size_t>(K1) * N);
+    for (int i = 0; i < p.n; i++) {
+        if (abar[i] == 0) continue;
+        for (int r = 0; r < K1; r++) rotate_minus_one(w, N, abar[i], acc + static_cast<size_t>(r) * N, &tmp[static_cast<size_t>(r) * N]);
+        external_product(key, i, tmp.data(), ext.data(), schoolbook);
+        // accumulateTrlwe -> accumulateTorusPolynomial (trlwe.h:268-274, polynomial.h:203-217)
+        for (size_t j = 0; j < tmp.size(); j++) acc[j] = sext(acc[j] + ext[j], w);
+    }
+}
+
+int64_t torus_q(const or_params& p) { return p.torus_bits >= 63 ? 0 : (int64_t{1} << p.torus_bits); }
+int64_t lwe_q(const or_params& p) { return int64_t{1} << p.lwe_q_bits; }
+
+// rescaleTlweToNewMod (tlwe.cpp:75-81) to mod 2N.
+void rescale(const or_params& p, const int32_t* lwe, int32_t* out) {
+    const int64_t m = 2 * static_cast<int64_t>(p.N);
+    for (int i = 0; i <= p.n; i++) out[i] = static_cast<int32_t>(mod_switch_from_torus(lwe[i], m, lwe_q(p)));
+}
+
+// genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
+void init_acc(const or_params& p, const int64_t* v, int32_t bbar, int64_t* acc) {
+    const int N = p.N;
+    std::fill(acc, acc + static_cast<size_t>(p.k) * N, 0);
+    rotate(p.torus_bits, N, -bbar, v, acc + static_cast<size_t>(p.k) * N);
+}
+
+// extractTlweFromTrlwe (trlwe.cpp:241-250), index 0, with the torus -> LWE_Q modswitch.
+void extract(const or_params& p, const int64_t* acc, int32_t* out) {
+    const int N = p.N, w = p.torus_bits;
+    const int64_t tq = torus_q(p), lq = lwe_q(p);
+    for (int i = 0; i < p.k; i++) {
+        const int64_t* a = acc + static_cast<size_t>(i) * N;
+        for (int j = 0; j < N; j++) {
+            const int64_t x = (j == 0) ? a[0] : sext(-a[N - j], w == 32 ? 32 : 64);
+            out[i * N + j] = static_cast<int32_t>(mod_switch_from_torus(x, lq, tq));
+        }
+    }
+    out[p.k * N] = static_cast<int32_t>(mod_switch_from_torus(acc[static_cast<size_t>(p.k) * N], lq, tq));
+}
+
+// switchKeyForTlwe (keyswitching.cpp:48-72) with signedGadgetDecompositionKs
+// (gadget_decomposition.cpp:53-56).  KSK layout [kN][t][n+1] (a[0..n-1], b).
+void keyswitch(const or_params& p, const int32_t* ksk, const int32_t* in, int32_t* out) {
+    const int nOut = p.n, nIn = p.k * p.N, t = p.ks_level;
+    std::vector<int64_t> delta(nOut, 0);
+    int64_t delta_b = 0;
+    std::vector<int64_t> d(t);
+    for (int i = 0; i < nIn; i++) {
+        digits_scalar(in[i], t, p.ks_radix_bits, p.ks_width_bits, d.data());
+        for (int j = 0; j < t; j++) {
+            const int64_t coeff = d[j];
+            if (coeff == 0) continue;
+            const int32_t* row = ksk + (static_cast<size_t>(i) * t + j) * (nOut + 1);
+            for (int kk = 0; kk < nOut; kk++) delta[kk] += coeff * row[kk];
+            delta_b += coeff * row[nOut];
+        }
+    }
+    const int64_t lq = lwe_q(p);
+    for (int kk = 0; kk < nOut; kk++) out[kk] = static_cast<int32_t>(long_mod_p(-delta[kk], lq));
+    out[nOut] = static_cast<int32_t>(long_mod_p(static_cast<int64_t>(in[nIn]) - delta_b, lq));
+}
+
+void pbs_one(const Key& key, const int32_t* ksk, const int32_t* lwe, const int64_t* v,
+             int64_t* acc, int32_t* ext, int32_t* out, bool schoolbook) {
+    const or_params& p = key.p;
+    std::vector<int32_t> scaled(p.n + 1);
+    rescale(p, lwe, scaled.data());
+    init_acc(p, v, scaled[p.n], acc);
+    blind_rotate(key, scaled.data(), acc, schoolbook);
+    extract(p, acc, ext);
+    keyswitch(p, ksk, ext, out);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int64_t or_sext(int64_t x, int32_t w) { return sext(x, w); }
+
+int64_t or_mod_switch_from_torus(int64_t in, int64_t new_mod, int64_t torus_q_) {
+    return mod_switch_from_torus(in, new_mod, torus_q_);
+}
+
+// Exhaustive check of the 2N modswitch against its closed form over [lo, hi):
+// returns the number of mismatches.  Closed form: round-half-away-from-zero of
+// x / scale with C truncating division, then sign extension mod 2N.
+int64_t or_modswitch_closed_form_check(int32_t log2_2N, int64_t lo, int64_t hi) {
+    const int64_t m = int64_t{1} << log2_2N;
+    const int64_t scale = (int64_t{1} << 32) / m;
+    int64_t bad = 0;
+    for (int64_t u = lo; u < hi; u++) {
+        const int64_t x = static_cast<int32_t>(static_cast<uint32_t>(u));
+        const int64_t a = x >= 0 ? x : -x;
+        const int64_t r = (a + scale / 2) / scale;        // |x| rounded half up
+        const int64_t s = x >= 0 ? r : -r;
+        const int64_t want = sext(s, log2_2N);
+        if (mod_switch_from_torus(x, m, int64_t{1} << 32) != want) bad++;
+    }
+    return bad;
+}
+
+void or_rotate(int32_t w, int32_t N, int32_t a, const int64_t* in, int64_t* out) { rotate(w, N, a, in, out); }
+void or_rotate_minus_one(int32_t w, int32_t N, int32_t a, const int64_t* in, int64_t* out) { rotate_minus_one(w, N, a, in, out); }
+
+void or_digits_scalar(int64_t in, int32_t l, int32_t b, int32_t w, int64_t* out) { digits_scalar(in, l, b, w, out); }
+void or_digits_row(const int64_t* in, int32_t N, int32_t l, int32_t b, int32_t w, int64_t* out) { digits_row(in, N, l, b, w, out); }
+
+// Compare the two digit forms on the values x = lo + i*stride (i < count, low w
+// bits), threaded.  Returns the number of mismatching values.
+int64_t or_digits_cross_check(int32_t l, int32_t b, int32_t w, uint64_t lo, uint64_t stride, uint64_t count, int32_t threads) {
+    std::atomic<int64_t> bad{0};
+    const uint64_t chunk = 4096;
+    std::atomic<uint64_t> next{0};
+    auto work = [&]() {
+        std::vector<int64_t> in(chunk), rowOut(static_cast<size_t>(l) * chunk), one(l);
+        for (;;) {
+            const uint64_t s = next.fetch_add(chunk);
+            if (s >= count) break;
+            const uint64_t e = std::min(count, s + chunk);
+            const int n = static_cast<int>(e - s);
+            for (int i = 0; i < n; i++) in[i] = sext(static_cast<int64_t>(lo + (s + i) * stride), w);
+            digits_row(in.data(), n, l, b, w, rowOut.data());
+            int64_t local = 0;
+            for (int i = 0; i < n; i++) {
+                digits_scalar(in[i], l, b, w, one.data());
+                for (int lv = 0; lv < l; lv++) if (one[lv] != rowOut[static_cast<size_t>(lv) * n + i]) { local++; break; }
+            }
+            bad += local;
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < std::max(1, threads); t++) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+    return bad.load();
+}
+
+// Hash of the oracle's scalar-form digits of x = lo + i*stride (i < count),
+// chunked and combined exactly as oracle/ref_digits.cpp hashes the reference's
+// decomposeRow output, so equal hashes mean equal digits on every input.
+uint64_t or_digits_hash(int32_t l, int32_t b, int32_t w, uint64_t lo, uint64_t stride, uint64_t count, int32_t threads) {
+    const uint64_t chunk = 4096;
+    const uint64_t nChunks = (count + chunk - 1) / chunk;
+    std::vector<uint64_t> part(nChunks, 0);
+    std::atomic<uint64_t> next{0};
+    auto work = [&]() {
+        std::vector<int64_t> d(l);
+        std::vector<int64_t> all(static_cast<size_t>(l) * chunk);
+        for (;;) {
+            const uint64_t c = next.fetch_add(1);
+            if (c >= nChunks) break;
+            const uint64_t s = c * chunk;
+            const int n = static_cast<int>(std::min(chunk, count - s));
+            uint64_t h = 0x9E3779B97F4A7C15ULL ^ c;
+            for (int i = 0; i < n; i++) {
+                digits_scalar(sext(static_cast<int64_t>(lo + (s + i) * stride), w), l, b, w, d.data());
+                for (int j = 0; j < l; j++) h ^= static_cast<uint64_t>(d[j]) + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+            }
+            part[c] = h;
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < std::max(1, threads); t++) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+    uint64_t h = 0;
+    for (auto x : part) h = h * 0x100000001B3ULL ^ x;
+    return h;
+}
+
+// Negacyclic product mod q via the oracle NTT (method 0) or schoolbook (1).
+void or_negacyclic_mul_modq(uint64_t q, int32_t N, const int64_t* a, const uint64_t* b, uint64_t* out, int32_t method) {
+    if (method == 1) { schoolbook_modq(q, N, a, b, out); return; }
+    Ntt ntt(q, N);
+    std::vector<uint64_t> x(N), y(b, b + N);
+    for (int i = 0; i < N; i++) x[i] = lift(a[i], q);
+    ntt.forward(x.data());
+    ntt.forward(y.data());
+    for (int i = 0; i < N; i++) x[i] = mulmod(x[i], y[i], q);
+    ntt.inverse(x.data());
+    std::memcpy(out, x.data(), sizeof(uint64_t) * N);
+}
+
+// Forward/inverse round trip of the oracle NTT (returns 1 on identity).
+int32_t or_ntt_roundtrip(uint64_t q, int32_t N, const uint64_t* a) {
+    Ntt ntt(q, N);
+    std::vector<uint64_t> x(a, a + N);
+    ntt.forward(x.data());
+    ntt.inverse(x.data());
+    return std::equal(x.begin(), x.end(), a) ? 1 : 0;
+}
+
+int64_t or_intt_exit(uint64_t x, uint64_t q, int32_t w) { return intt_exit(x, q, w); }
+
+void* or_key_create(const or_params* p, const uint64_t* bsk_coeff) {
+    try { return new Key(*p, bsk_coeff); } catch (...) { return nullptr; }
+}
+void or_key_free(void* k) { delete static_cast<Key*>(k); }
+
+void or_external_product(void* key, int32_t i, const int64_t* in, int64_t* out, int32_t schoolbook) {
+    external_product(*static_cast<Key*>(key), static_cast<size_t>(i), in, out, schoolbook != 0);
+}
+
+void or_rescale(const or_params* p, const int32_t* lwe, int32_t* out) { rescale(*p, lwe, out); }
+void or_init_acc(const or_params* p, const int64_t* v, int32_t bbar, int64_t* acc) { init_acc(*p, v, bbar, acc); }
+void or_blind_rotate(void* key, const int32_t* abar, int64_t* acc, int32_t schoolbook) {
+    blind_rotate(*static_cast<Key*>(key), abar, acc, schoolbook != 0);
+}
+void or_extract(const or_params* p, const int64_t* acc, int32_t* out) { extract(*p, acc, out); }
+void or_keyswitch(const or_params* p, const int32_t* ksk, const int32_t* in, int32_t* out) { keyswitch(*p, ksk, in, out); }
+
+// Keyswitch with wrapping uint32 accumulation (rule R11 cross-check): must equal
+// or_keyswitch for qLwe = 2^32.
+void or_keyswitch_u32(const or_params* p, const int32_t* ksk, const int32_t* in, int32_t* out) {
+    const int nOut = p->n, nIn = p->k * p->N, t = p->ks_level;
+    std::vector<uint32_t> acc(nOut + 1, 0);
+    std::vector<int64_t> d(t);
+    for (int i = 0; i < nIn; i++) {
+        digits_scalar(in[i], t, p->ks_radix_bits, p->ks_width_bits, d.data());
+        for (int j = 0; j < t; j++) {
+            const uint32_t c = static_cast<uint32_t>(d[j]);
+            const int32_t* row = ksk + (static_cast<size_t>(i) * t + j) * (nOut + 1);
+            for (int kk = 0; kk <= nOut; kk++) acc[kk] += c * static_cast<uint32_t>(row[kk]);
+        }
+    }
+    for (int kk = 0; kk < nOut; kk++) out[kk] = static_cast<int32_t>(0u - acc[kk]);
+    out[nOut] = static_cast<int32_t>(static_cast<uint32_t>(in[nIn]) - acc[nOut]);
+}
+
+// functionalBootstrappingMPNtt (bootstrapping.cpp:71-80) for one ciphertext;
+// acc (k+1)N torus, ext kN+1, out n+1 (any of the three may be null).
+void or_pbs(void* key, const int32_t* ksk, const int32_t* lwe, const int64_t* v,
+            int64_t* acc, int32_t* ext, int32_t* out, int32_t schoolbook) {
+    const Key& k = *static_cast<Key*>(key);
+    const or_params& p = k.p;
+    std::vector<int64_t> a(static_cast<size_t>(p.k + 1) * p.N);
+    std::vector<int32_t> e(static_cast<size_t>(p.k) * p.N + 1), o(p.n + 1);
+    pbs_one(k, ksk, lwe, v, a.data(), e.data(), o.data(), schoolbook != 0);
+    if (acc) std::memcpy(acc, a.data(), a.size() * sizeof(int64_t));
+    if (ext) std::memcpy(ext, e.data(), e.size() * sizeof(int32_t));
+    if (out) std::memcpy(out, o.data(), o.size() * sizeof(int32_t));
+}
+
+// Many independent ciphertexts, one per thread at a time (the CPU baseline's
+// throughput form).  lwe/out are [count][n+1]; v is one test polynomial.
+void or_pbs_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,
+                  int32_t* out, int32_t threads) {
+    const Key& k = *static_cast<Key*>(key);
+    const or_params& p = k.p;
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        std::vector<int64_t> a(static_cast<size_t>(p.k + 1) * p.N);
+        std::vector<int32_t> e(static_cast<size_t>(p.k) * p.N + 1);
+        for (;;) {
+            const int c = next.fetch_add(1);
+            if (c >= count) break;
+            pbs_one(k, ksk, lwe + static_cast<size_t>(c) * (p.n + 1), v, a.data(), e.data(),
+                    out + static_cast<size_t>(c) * (p.n + 1), false);
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < std::max(1, threads); t++) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+}
+
+// Decryption helpers for semantic gates.
+// LWE phase b - <a, s> mod 2^32 (symDecTlwe*, tlwe.cpp:33-53, before rounding).
+int32_t or_lwe_phase(const int32_t* ct, const int32_t* s, int32_t n) {
+    int64_t tmp = 0;
+    for (int i = 0; i < n; i++) if (s[i]) tmp += static_cast<int64_t>(ct[i]) * s[i];
+    return static_cast<int32_t>(static_cast<uint32_t>(ct[n]) - static_cast<uint32_t>(tmp));
+}
+
+// GLWE phase b - sum_i a_i * z_i mod 2^w (symDecTrlweWoRounding), binary key z [k][N].
+void or_glwe_phase(int32_t w, int32_t k, int32_t N, const int64_t* acc, const int32_t* z, int64_t* out) {
+    std::vector<i128> t(N, 0);
+    for (int i = 0; i < k; i++) {
+        const int64_t* a = acc + static_cast<size_t>(i) * N;
+        const int32_t* zi = z + static_cast<size_t>(i) * N;
+        for (int u = 0; u < N; u++) {
+            if (!zi[u]) continue;
+            for (int j = 0; j < N; j++) {
+                const int d = u + j;
+                if (d < N) t[d] += a[j]; else t[d - N] -= a[j];
+            }
+        }
+    }
+    const int64_t* b = acc + static_cast<size_t>(k) * N;
+    for (int j = 0; j < N; j++) out[j] = sext(static_cast<int64_t>(static_cast<uint64_t>(b[j]) - static_cast<uint64_t>(static_cast<int64_t>(t[j]))), w);
+}
+
+// GLWE phase mod q of a coefficient-domain key row: (b - sum a_i z_i) mod q,
+// centred to (-q/2, q/2).  rows: a[k][N] then b[N], residues in [0,q).
+void or_glwe_phase_modq(uint64_t q, int32_t k, int32_t N, const uint64_t* row, const int32_t* z, int64_t* out) {
+    std::vector<i128> t(N, 0);
+    for (int i = 0; i < k; i++) {
+        const uint64_t* a = row + static_cast<size_t>(i) * N;
+        const int32_t* zi = z + static_cast<size_t>(i) * N;
+        for (int u = 0; u < N; u++) {
+            if (!zi[u]) continue;
+            for (int j = 0; j < N; j++) {
+                const int d = u + j;
+                if (d < N) t[d] += a[j]; else t[d - N] -= a[j];
+            }
+        }
+    }
+    const uint64_t* b = row + static_cast<size_t>(k) * N;
+    for (int j = 0; j < N; j++) {
+        i128 r = (static_cast<i128>(b[j]) - t[j]) % static_cast<i128>(q);
+        if (r < 0) r += q;
+        int64_t c = static_cast<int64_t>(r);
+        if (static_cast<uint64_t>(c) > q / 2) c -= static_cast<int64_t>(q);
+        out[j] = c;
+    }
+}
+
+}  // extern "C"

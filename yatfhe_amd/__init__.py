@@ -1,0 +1,409 @@
+"""yatfhe_amd -- MI355X TFHE programmable-bootstrapping engine, Python binding.
+
+The engine is the C-ABI library ``libyatfhe_gpu.so`` (declared in
+``include/yatfhe_gpu.h``) built from ``yatfhe_amd/csrc``.  This module loads it
+with ctypes and offers thin wrappers used by the tests and the benchmark.  There
+is no CPU fallback: if the library is missing, importing the binding raises.
+
+Parameter presets follow the reference's ``YatfheParameters``
+(yatfhe_parameters.h:9-110); see ``PARAMS_T32_N1024`` and ``PARAMS_T56_N2048``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libyatfhe_gpu.so")
+
+Q_49P_T32 = 562941363486721  # torus.h:151
+Q_61P_T56 = 1945555039024054273  # torus.h:157
+
+YGPU_OK = 0
+
+
+class YgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ygpu error {code}: {msg}")
+        self.code = code
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("torus_bits", ctypes.c_int32),
+        ("n", ctypes.c_int32),
+        ("k", ctypes.c_int32),
+        ("N", ctypes.c_int32),
+        ("radix_bits", ctypes.c_int32),
+        ("level", ctypes.c_int32),
+        ("ks_radix_bits", ctypes.c_int32),
+        ("ks_level", ctypes.c_int32),
+        ("ks_width_bits", ctypes.c_int32),
+        ("lwe_q_bits", ctypes.c_int32),
+        ("q_ntt", ctypes.c_uint64),
+    ]
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("key_id", ctypes.c_int32), ("count", ctypes.c_int32), ("first", ctypes.c_int64)]
+
+
+@dataclass(frozen=True)
+class Params:
+    """Bootstrapping parameter set (the fields of YatfheParameters the path reads)."""
+
+    torus_bits: int
+    n: int
+    k: int
+    N: int
+    radix_bits: int
+    level: int
+    ks_radix_bits: int
+    ks_level: int
+    ks_width_bits: int = 32
+    lwe_q_bits: int = 32
+    q_ntt: int = Q_49P_T32
+
+    def c(self) -> _Params:
+        return _Params(self.torus_bits, self.n, self.k, self.N, self.radix_bits, self.level,
+                       self.ks_radix_bits, self.ks_level, self.ks_width_bits, self.lwe_q_bits, self.q_ntt)
+
+    @property
+    def torus_dtype(self):
+        return np.int32 if self.torus_bits == 32 else np.int64
+
+    @property
+    def bsk_shape(self):
+        k1 = self.k + 1
+        return (self.n, self.level, k1, k1, self.N)
+
+    @property
+    def ksk_shape(self):
+        return (self.k * self.N, self.ks_level, self.n + 1)
+
+
+# TORUS32, n=630, k=1, N=1024, gadget 2^7 x 3, keyswitch 2^2 x 8.
+PARAMS_T32_N1024 = Params(torus_bits=32, n=630, k=1, N=1024, radix_bits=7, level=3,
+                          ks_radix_bits=2, ks_level=8, q_ntt=Q_49P_T32)
+# TORUS56, n=742, k=1, N=2048, gadget 2^10 x 3, keyswitch 2^2 x 8.
+PARAMS_T56_N2048 = Params(torus_bits=56, n=742, k=1, N=2048, radix_bits=10, level=3,
+                          ks_radix_bits=2, ks_level=8, q_ntt=Q_61P_T56)
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load the engine library; raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(f"{p} is missing: build it with yatfhe_amd.build.build() (no CPU fallback exists)")
+    lib = ctypes.CDLL(p)
+    c_int, c_i32, c_i64, c_u64, c_vp, c_dbl = ctypes.c_int, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_double
+    P = ctypes.POINTER
+    sigs = {
+        "ygpu_last_error": (ctypes.c_char_p, []),
+        "ygpu_device_count": (c_int, [P(c_i32)]),
+        "ygpu_init": (c_int, [P(_Params), c_i32, P(c_vp)]),
+        "ygpu_destroy": (c_int, [c_vp]),
+        "ygpu_get_params": (c_int, [c_vp, P(_Params)]),
+        "ygpu_key_upload": (c_int, [c_vp, c_vp, c_vp, P(c_i32)]),
+        "ygpu_key_release": (c_int, [c_vp, c_i32]),
+        "ygpu_key_bytes": (c_int, [c_vp, c_i32, P(c_i64)]),
+        "ygpu_pbs": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+        "ygpu_blind_rotate": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_extract_keyswitch": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_pbs_batch_device": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_pbs_jobs_device": (c_int, [c_vp, P(_Job), c_i32, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp]),
+        "ygpu_pbs_plain_device": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
+        "ygpu_malloc": (c_int, [c_vp, ctypes.c_size_t, P(c_vp)]),
+        "ygpu_free": (c_int, [c_vp, c_vp]),
+        "ygpu_memcpy_htod": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
+        "ygpu_memcpy_dtoh": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
+        "ygpu_sync": (c_int, [c_vp]),
+        "ygpu_timing_enable": (c_int, [c_vp, c_i32]),
+        "ygpu_timing_get": (c_int, [c_vp, ctypes.c_char_p, P(c_i64), P(c_dbl)]),
+        "ygpu_timing_reset": (c_int, [c_vp]),
+        "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
+        "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def exported_symbols():
+    return [
+        "ygpu_last_error", "ygpu_device_count", "ygpu_init", "ygpu_destroy", "ygpu_get_params",
+        "ygpu_key_upload", "ygpu_key_release", "ygpu_key_bytes", "ygpu_pbs", "ygpu_blind_rotate",
+        "ygpu_extract_keyswitch", "ygpu_pbs_batch_device", "ygpu_pbs_jobs_device", "ygpu_pbs_plain_device",
+        "ygpu_malloc", "ygpu_free", "ygpu_memcpy_htod", "ygpu_memcpy_dtoh", "ygpu_sync",
+        "ygpu_timing_enable", "ygpu_timing_get", "ygpu_timing_reset", "ygpu_keygen", "ygpu_lwe_encrypt",
+    ]
+
+
+def _check(rc):
+    if rc != YGPU_OK:
+        msg = load_library().ygpu_last_error()
+        raise YgpuError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ----------------------------------------------------------------------- keys
+@dataclass
+class KeySet:
+    params: Params
+    lwe_key: np.ndarray   # int32[n]
+    glwe_key: np.ndarray  # int32[kN]
+    bsk: np.ndarray       # uint64[n][l][k+1][k+1][N], coefficient residues mod q
+    ksk: np.ndarray       # int32[kN][t][n+1]
+
+
+SIGMA_LOG2_LWE = -15.0
+SIGMA_LOG2_GLWE = -25.0
+
+
+def keygen(params: Params, seed: int, sigma_log2_lwe: float = SIGMA_LOG2_LWE,
+           sigma_log2_glwe: float = SIGMA_LOG2_GLWE, threads: int = 0) -> KeySet:
+    lib = load_library()
+    lk = np.zeros(params.n, np.int32)
+    gk = np.zeros(params.k * params.N, np.int32)
+    bsk = np.zeros(params.bsk_shape, np.uint64)
+    ksk = np.zeros(params.ksk_shape, np.int32)
+    cp = params.c()
+    _check(lib.ygpu_keygen(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe,
+                           _ptr(lk), _ptr(gk), _ptr(bsk), _ptr(ksk), threads))
+    return KeySet(params, lk, gk, bsk, ksk)
+
+
+def lwe_encrypt(lwe_key: np.ndarray, mu: np.ndarray, seed: int, sigma_log2: float = SIGMA_LOG2_LWE) -> np.ndarray:
+    lib = load_library()
+    mu = np.ascontiguousarray(mu, dtype=np.int32)
+    n = lwe_key.shape[0]
+    out = np.zeros((mu.shape[0], n + 1), np.int32)
+    key = np.ascontiguousarray(lwe_key, dtype=np.int32)
+    _check(lib.ygpu_lwe_encrypt(n, _ptr(key), seed, sigma_log2, mu.shape[0], _ptr(mu), _ptr(out)))
+    return out
+
+
+def lwe_decrypt_phase(ct: np.ndarray, lwe_key: np.ndarray) -> np.ndarray:
+    """b - <a, s> mod 2^32 as int32 (before rounding), rows of ct."""
+    ct = np.atleast_2d(ct).astype(np.int64)
+    s = lwe_key.astype(np.int64)
+    ph = (ct[:, -1] - (ct[:, :-1] * s).sum(axis=1)) & 0xFFFFFFFF
+    return ph.astype(np.uint32).astype(np.int32)
+
+
+# ------------------------------------------------------------- test polynomials
+def sign_test_polynomial(params: Params, value: int | None = None) -> np.ndarray:
+    """v_i = 1/8 of the torus for every i (the sign gate)."""
+    val = value if value is not None else (1 << (params.torus_bits - 3))
+    return np.full(params.N, val, dtype=params.torus_dtype)
+
+
+def lut_test_polynomial(params: Params, lut, p: int) -> np.ndarray:
+    """Programmable-bootstrap test polynomial for a p-entry LUT with one padding
+    bit (messages m in [0, p) encoded at m * 2^32 / (2p)), laid out like
+    generateTestPolynomialFR (polynomial.cpp:92-107): slot s covers 2N/(2p)
+    consecutive coefficients, each slot centred on its sample point, the left
+    half of slot 0 wrapped (negated) to the end.  Output values are
+    lut[s] mod p scaled to the torus (modSwitchToTorusGeneral)."""
+    N = params.N
+    box = 2 * N // p
+    half = box // 2
+    halfhalf = half // 2
+    w = params.torus_bits
+    scale = 1 << (w - int(np.log2(p)))
+    v = np.zeros(N, dtype=np.int64)
+    for i in range(N):
+        s = i // half
+        m = int(lut[s]) % p
+        m = m - p if m > (p - 1) // 2 else m  # intModP centring
+        v[i] = m * scale
+    v[:halfhalf] = -v[:halfhalf]
+    v = np.roll(v, -halfhalf)
+    mask = (1 << w) - 1
+    v = ((v & mask) ^ (1 << (w - 1))) - (1 << (w - 1))
+    return v.astype(params.torus_dtype)
+
+
+# ----------------------------------------------------------------- device side
+class DeviceBuffer:
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = nbytes
+        self.ptr = ctypes.c_void_p()
+        _check(ctx.lib.ygpu_malloc(ctx.h, nbytes, ctypes.byref(self.ptr)))
+
+    @property
+    def value(self):
+        return self.ptr.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(self.ctx.lib.ygpu_memcpy_htod(self.ctx.h, self.ptr, _ptr(a), a.nbytes))
+
+    def download(self, a: np.ndarray):
+        assert a.flags.c_contiguous and a.nbytes <= self.nbytes
+        _check(self.ctx.lib.ygpu_memcpy_dtoh(self.ctx.h, _ptr(a), self.ptr, a.nbytes))
+        return a
+
+    def free(self):
+        if self.ptr.value:
+            _check(self.ctx.lib.ygpu_free(self.ctx.h, self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+
+def device_count() -> int:
+    lib = load_library()
+    c = ctypes.c_int32(0)
+    rc = lib.ygpu_device_count(ctypes.byref(c))
+    return c.value if rc == YGPU_OK else 0
+
+
+class Context:
+    """One engine context on one GPU (replaces initYatfhe, initializer.cpp:50-85)."""
+
+    def __init__(self, params: Params, device: int = 0):
+        self.lib = load_library()
+        self.params = params
+        self.h = ctypes.c_void_p()
+        cp = params.c()
+        _check(self.lib.ygpu_init(ctypes.byref(cp), device, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h.value:
+            self.lib.ygpu_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload_key(self, keys: KeySet) -> int:
+        kid = ctypes.c_int32(-1)
+        bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
+        ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        _check(self.lib.ygpu_key_upload(self.h, _ptr(bsk), _ptr(ksk), ctypes.byref(kid)))
+        return kid.value
+
+    def release_key(self, key_id: int):
+        _check(self.lib.ygpu_key_release(self.h, key_id))
+
+    def key_bytes(self, key_id: int) -> int:
+        b = ctypes.c_int64(0)
+        _check(self.lib.ygpu_key_bytes(self.h, key_id, ctypes.byref(b)))
+        return b.value
+
+    # -- plain drop-in form (host buffers)
+    def pbs(self, key_id: int, lwe_in: np.ndarray, v: np.ndarray, want_acc=False, want_extract=False):
+        p = self.params
+        lwe = np.ascontiguousarray(lwe_in, dtype=np.int32)
+        vv = np.ascontiguousarray(v, dtype=p.torus_dtype)
+        out = np.zeros(p.n + 1, np.int32)
+        acc = np.zeros((p.k + 1, p.N), p.torus_dtype) if want_acc else None
+        ext = np.zeros(p.k * p.N + 1, np.int32) if want_extract else None
+        _check(self.lib.ygpu_pbs(self.h, key_id, _ptr(lwe), _ptr(vv), _ptr(out),
+                                 _ptr(acc) if acc is not None else None, _ptr(ext) if ext is not None else None))
+        return out, acc, ext
+
+    def blind_rotate(self, key_id: int, lwe_in: np.ndarray, v: np.ndarray):
+        p = self.params
+        lwe = np.ascontiguousarray(lwe_in, dtype=np.int32)
+        vv = np.ascontiguousarray(v, dtype=p.torus_dtype)
+        acc = np.zeros((p.k + 1, p.N), p.torus_dtype)
+        _check(self.lib.ygpu_blind_rotate(self.h, key_id, _ptr(lwe), _ptr(vv), _ptr(acc)))
+        return acc
+
+    def extract_keyswitch(self, key_id: int, acc: np.ndarray):
+        p = self.params
+        a = np.ascontiguousarray(acc, dtype=p.torus_dtype)
+        ext = np.zeros(p.k * p.N + 1, np.int32)
+        out = np.zeros(p.n + 1, np.int32)
+        _check(self.lib.ygpu_extract_keyswitch(self.h, key_id, _ptr(a), _ptr(ext), _ptr(out)))
+        return ext, out
+
+    # -- batched form (device buffers)
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def pbs_batch_device(self, key_id: int, count: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None, d_ext=None):
+        _check(self.lib.ygpu_pbs_batch_device(self.h, key_id, count, d_lwe.value, d_v.value, 1 if v_per_ct else 0,
+                                              d_out.value, d_acc.value if d_acc else None,
+                                              d_ext.value if d_ext else None))
+
+    def pbs_jobs_device(self, jobs, total: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None):
+        arr = (_Job * max(1, len(jobs)))()
+        for i, (kid, cnt, first) in enumerate(jobs):
+            arr[i] = _Job(kid, cnt, first)
+        _check(self.lib.ygpu_pbs_jobs_device(self.h, arr, len(jobs), total, d_lwe.value, d_v.value,
+                                             1 if v_per_ct else 0, d_out.value, d_acc.value if d_acc else None))
+
+    def pbs_plain_device(self, key_id: int, d_lwe, d_v, d_out, d_acc=None):
+        _check(self.lib.ygpu_pbs_plain_device(self.h, key_id, d_lwe.value, d_v.value, d_out.value,
+                                              d_acc.value if d_acc else None))
+
+    def pbs_batch(self, key_id: int, lwe_in: np.ndarray, v: np.ndarray, want_acc=False, want_extract=False):
+        """Host-buffer convenience over the batched device form."""
+        p = self.params
+        lwe = np.ascontiguousarray(lwe_in, dtype=np.int32)
+        count = lwe.shape[0]
+        vv = np.ascontiguousarray(v, dtype=p.torus_dtype)
+        v_per_ct = vv.ndim == 2
+        tb = np.dtype(p.torus_dtype).itemsize
+        bufs = []
+        try:
+            d_lwe = self.alloc(lwe.nbytes); bufs.append(d_lwe)
+            d_v = self.alloc(vv.nbytes); bufs.append(d_v)
+            d_out = self.alloc(count * (p.n + 1) * 4); bufs.append(d_out)
+            d_acc = self.alloc(count * (p.k + 1) * p.N * tb) if want_acc else None
+            d_ext = self.alloc(count * (p.k * p.N + 1) * 4) if want_extract else None
+            bufs += [b for b in (d_acc, d_ext) if b]
+            d_lwe.upload(lwe)
+            d_v.upload(vv)
+            self.pbs_batch_device(key_id, count, d_lwe, d_v, v_per_ct, d_out, d_acc, d_ext)
+            self.sync()
+            out = d_out.download(np.zeros((count, p.n + 1), np.int32))
+            acc = d_acc.download(np.zeros((count, p.k + 1, p.N), p.torus_dtype)) if d_acc else None
+            ext = d_ext.download(np.zeros((count, p.k * p.N + 1), np.int32)) if d_ext else None
+            return out, acc, ext
+        finally:
+            for b in bufs:
+                b.free()
+
+    def sync(self):
+        _check(self.lib.ygpu_sync(self.h))
+
+    def timing(self, on: bool):
+        _check(self.lib.ygpu_timing_enable(self.h, 1 if on else 0))
+
+    def timing_get(self, kernel: str):
+        n = ctypes.c_int64(0)
+        ms = ctypes.c_double(0)
+        _check(self.lib.ygpu_timing_get(self.h, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def timing_reset(self):
+        _check(self.lib.ygpu_timing_reset(self.h))
+
+
+__all__ = [
+    "Params", "PARAMS_T32_N1024", "PARAMS_T56_N2048", "Q_49P_T32", "Q_61P_T56", "KeySet", "keygen",
+    "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
+    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace",
+]

@@ -1,0 +1,228 @@
+// ygpu_device.h -- device arithmetic and the wave-level negacyclic NTT for gfx950.
+//
+// Exact modular arithmetic in binary64.  A residue mod a prime p < 2^49 is held
+// as a double that is an integer of magnitude < 16p <= 2^53, so every sum,
+// difference and fused product below is exact.  The 64-bit integer multiplier
+// on CDNA is built from quarter-rate 32-bit multiplies; FP64 FMA is full rate,
+// so a Barrett-style product in FP64 (a handful of full-rate ops) is the
+// fastest exact modmul this VALU offers.
+//
+// mulmod(x, w):  h = fl(x*w), l = x*w - h (exact, by FMA), t = rint(h / p),
+//                r = h - t*p (exact, by FMA), return r + l.
+//   x*w - (r + l) = t*p, so the result is congruent to x*w; with |x| <= 8p and
+//   |w| <= p/2 its magnitude is at most 1.25p (derivation in DESIGN.md).
+// reduce(x):     x - rint(x / p) * p, magnitude <= p/2 + 1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ygpu {
+
+struct PrimeF64 {
+    double p;      // the prime, exact
+    double pinv;   // fl(1/p)
+    double half;   // (p - 1) / 2, exact
+};
+
+__device__ __forceinline__ double mulmod(double x, double w, const PrimeF64& P) {
+    const double h = __dmul_rn(x, w);
+    const double l = __fma_rn(x, w, -h);
+    const double t = __builtin_rint(__dmul_rn(h, P.pinv));
+    const double r = __fma_rn(-t, P.p, h);
+    return __dadd_rn(r, l);
+}
+
+__device__ __forceinline__ double reduce(double x, const PrimeF64& P) {
+    const double t = __builtin_rint(__dmul_rn(x, P.pinv));
+    return __fma_rn(-t, P.p, x);
+}
+
+// Exact centred representative in [-(p-1)/2, (p-1)/2] of a value |x| < 8p.
+__device__ __forceinline__ double centre(double x, const PrimeF64& P) {
+    double r = reduce(x, P);
+    r = (r > P.half) ? __dadd_rn(r, -P.p) : r;
+    r = (r < -P.half) ? __dadd_rn(r, P.p) : r;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Register layout of the transform.  A team of T = N/16 lanes holds one
+// polynomial, 16 values per lane.  The log2(N) stages run in three passes of
+// 4, 4 and log2(N)-8 stages; within a pass every butterfly is lane-local.
+// Pass P covers the index bits [lo, lo+S) (lo = log2N - 4P - S); lane tl,
+// register r = g*2^S + e holds index  low(G) | e << lo | high(G) << (lo+S)
+// with G = tl + T*g.  Pass 0 is natural order j = tl + T*r.
+template <int LOGN, int P>
+struct Pass {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int T = N / 16;
+    static constexpr int S = (P < 2) ? 4 : (LOGN - 8);
+    static constexpr int S0 = 4 * P;
+    static constexpr int LO = LOGN - S0 - S;
+    __host__ __device__ __forceinline__ static int idx(int tl, int r) {
+        const int g = r >> S;
+        const int e = r & ((1 << S) - 1);
+        const int G = tl + T * g;
+        return (G & ((1 << LO) - 1)) | (e << LO) | ((G >> LO) << (LO + S));
+    }
+};
+
+// LDS addresses of the two exchanges (0: passes 0<->1, 1: passes 1<->2), padded
+// so that each 16-lane group of a 64-bit access touches 16 distinct 8-byte
+// slots on both the writing and the reading side.
+template <int LOGN, int X>
+struct Xch {
+    static constexpr int S2 = LOGN - 8;
+    static constexpr int P12 = 272 + (16 >> S2);
+    static constexpr int SIZE = (X == 0) ? ((1 << LOGN) + (1 << (LOGN - 8)) * 16) : ((1 << S2) * P12);
+    __device__ __forceinline__ static int addr(int j) {
+        if constexpr (X == 0) {
+            return j + (1 << (LOGN - 8)) * (j >> (LOGN - 4));
+        } else {
+            const int c = j & ((1 << S2) - 1);
+            const int G = j >> S2;
+            return c * P12 + G + (G >> 4);
+        }
+    }
+};
+
+template <int LOGN>
+struct Scratch {
+    static constexpr int A = Xch<LOGN, 0>::SIZE;
+    static constexpr int B = Xch<LOGN, 1>::SIZE;
+    static constexpr int DOUBLES = A > B ? A : B;
+};
+
+// Synchronisation of the lanes of one team: a single wavefront executes its
+// LDS instructions in order, so only the compiler needs fencing; a two-wave
+// team uses the workgroup barrier (every team of a workgroup runs the same
+// sequence of exchanges).
+template <int TEAM_WAVES>
+__device__ __forceinline__ void team_sync() {
+    if constexpr (TEAM_WAVES == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// Cooley-Tukey pass (forward): U, V -> U + wV, U - wV.
+template <int LOGN, int P>
+__device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* __restrict__ tw, const PrimeF64& pr) {
+    using PS = Pass<LOGN, P>;
+#pragma unroll
+    for (int ls = 0; ls < PS::S; ls++) {
+        const int st = PS::S0 + ls;
+        const int half = 1 << (PS::S - 1 - ls);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            if (r & half) continue;
+            const int j = PS::idx(tl, r);
+            const double w = tw[(1 << st) + (j >> (LOGN - st))];
+            const double U = x[r];
+            const double V = mulmod(x[r + half], w, pr);
+            x[r] = __dadd_rn(U, V);
+            x[r + half] = __dadd_rn(U, -V);
+        }
+    }
+}
+
+// Gentleman-Sande pass (inverse): U, V -> U + V, (U - V) w.
+template <int LOGN, int P>
+__device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* __restrict__ itw, const PrimeF64& pr) {
+    using PS = Pass<LOGN, P>;
+#pragma unroll
+    for (int ls = PS::S - 1; ls >= 0; ls--) {
+        const int st = PS::S0 + ls;
+        const int half = 1 << (PS::S - 1 - ls);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            if (r & half) continue;
+            const int j = PS::idx(tl, r);
+            const double w = itw[(1 << st) + (j >> (LOGN - st))];
+            const double U = x[r];
+            const double V = x[r + half];
+            x[r] = __dadd_rn(U, V);
+            x[r + half] = mulmod(__dadd_rn(U, -V), w, pr);
+        }
+    }
+}
+
+template <int LOGN, int TEAM_WAVES, int PW, int PR, int X>
+__device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl) {
+    team_sync<TEAM_WAVES>();
+#pragma unroll
+    for (int r = 0; r < 16; r++) scr[Xch<LOGN, X>::addr(Pass<LOGN, PW>::idx(tl, r))] = x[r];
+    team_sync<TEAM_WAVES>();
+#pragma unroll
+    for (int r = 0; r < 16; r++) x[r] = scr[Xch<LOGN, X>::addr(Pass<LOGN, PR>::idx(tl, r))];
+}
+
+// Forward negacyclic NTT: natural order in pass-0 layout -> bit-reversed order
+// in pass-2 layout.  Values are reduced once before the last pass.
+template <int LOGN, int TEAM_WAVES>
+__device__ __forceinline__ void ntt_forward(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr) {
+    fwd_pass<LOGN, 0>(x, tl, tw, pr);
+    exchange<LOGN, TEAM_WAVES, 0, 1, 0>(x, scr, tl);
+    fwd_pass<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, TEAM_WAVES, 1, 2, 1>(x, scr, tl);
+#pragma unroll
+    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
+    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+}
+
+// Inverse negacyclic NTT without the 1/N factor (folded into the key):
+// pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Values are reduced
+// at the start of every pass, which bounds them by 8.8p throughout.
+template <int LOGN, int TEAM_WAVES>
+__device__ __forceinline__ void ntt_inverse(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr) {
+#pragma unroll
+    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
+    inv_pass<LOGN, 2>(x, tl, itw, pr);
+    exchange<LOGN, TEAM_WAVES, 2, 1, 1>(x, scr, tl);
+#pragma unroll
+    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
+    inv_pass<LOGN, 1>(x, tl, itw, pr);
+    exchange<LOGN, TEAM_WAVES, 1, 0, 0>(x, scr, tl);
+#pragma unroll
+    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
+    inv_pass<LOGN, 0>(x, tl, itw, pr);
+}
+
+// ---------------------------------------------------------------------------
+// Torus helpers.
+template <int TBITS>
+__device__ __forceinline__ int64_t sext_t(int64_t x) {
+    return static_cast<int64_t>(static_cast<uint64_t>(x) << (64 - TBITS)) >> (64 - TBITS);
+}
+
+// Zero-mean balanced digits (gadget_decomposition.h:75-95): all `l` digits of
+// one torus value, least significant level first; out[lvl] for lvl < l.
+template <int TBITS>
+__device__ __forceinline__ void digits_of(int64_t x, int l, int rb, int (&out)[4]) {
+    const uint64_t mask = (TBITS >= 64) ? ~0ULL : ((1ULL << TBITS) - 1);
+    const uint64_t u = static_cast<uint64_t>(x) & mask;
+    const bool neg = (u >> (TBITS - 1)) & 1;
+    const int dropped = TBITS - l * rb;
+    const uint64_t bias = dropped > 0 ? (1ULL << (dropped - 1)) : 0;
+    const uint64_t m = (neg ? ((0ULL - u) & mask) : u) + bias;
+    const int B = 1 << rb;
+    const int halfB = B >> 1;
+    int carry = 0;
+#pragma unroll
+    for (int lvl = 3; lvl >= 0; lvl--) {
+        if (lvl < l) {
+            const int shift = TBITS - (lvl + 1) * rb;
+            const int window = static_cast<int>((m >> shift) & static_cast<uint64_t>(B - 1));
+            const int digit = window + carry;
+            const int tie = lvl > 0 ? static_cast<int>((m >> (shift + 2 * rb - 1)) & 1) : 0;
+            carry = (digit > halfB) | ((digit == halfB) & tie);
+            const int bal = carry ? digit - B : digit;
+            out[lvl] = neg ? -bal : bal;
+        }
+    }
+}
+
+}  // namespace ygpu

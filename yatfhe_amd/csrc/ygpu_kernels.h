@@ -1,0 +1,152 @@
+// ygpu_kernels.h -- kernel configurations, argument blocks and launch entry points.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <type_traits>
+#include <algorithm>
+#include "ygpu_device.h"
+
+namespace ygpu {
+
+// Blind-rotation kernel configuration (k = 1).
+//   TBITS  torus width (32 or 56)
+//   LOGN   log2 ring degree (10 or 11)
+//   NP     transform primes (1: the reference prime itself; 2: CRT pair)
+//   TPC    teams per ciphertext (1 or 2), each team T = N/16 lanes
+template <int TBITS_, int LOGN_, int NP_, int TPC_>
+struct BrCfg {
+    static constexpr int TBITS = TBITS_;
+    using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
+    static constexpr int LOGN = LOGN_;
+    static constexpr int N = 1 << LOGN;
+    static constexpr int T = N / 16;
+    static constexpr int TEAM_WAVES = T / 64;
+    static constexpr int NP = NP_;
+    static constexpr int TPC = TPC_;
+    static constexpr int THREADS = T * TPC;
+    static constexpr int ROWS_PER_TEAM = (TPC == 2) ? 1 : 2;
+    static constexpr int OWNED = (TPC == 2) ? 1 : 2;
+};
+
+// Garner reconstruction constants for NP == 2.
+struct CrtConst {
+    double p0;         // first prime (double, exact)
+    double inv01;      // p0^{-1} mod p1, centred
+    int64_t p0i;       // first prime (integer)
+    uint64_t q;        // target modulus (the reference NTT prime)
+    double qd;         // q as double (rounded)
+    double qinv;       // 1/q (rounded)
+    int64_t qhalf;     // (q - 1) / 2
+};
+
+struct BrArgs {
+    int n, level, radix_bits, count;
+    const int32_t* lwe_in;            // [count][n+1]
+    const void* v;                    // Torus[N] (v_stride 0) or Torus[count][N]
+    int v_stride;
+    void* acc_out;                    // Torus[count][2][N]
+    const double* const* bsk_table;   // device array: BSK base per key slot
+    const int32_t* ct_slot;           // per-ciphertext key slot (null: slot 0)
+    const double* tw;                 // [NP][2][N]: forward then inverse powers
+    PrimeF64 prime[2];
+    CrtConst crt;
+};
+
+constexpr int KS_THREADS = 256;
+constexpr int KS_CT = 8;
+constexpr int KS_COLS_PER_THREAD = 3;   // n + 1 <= 768
+constexpr int KS_MAX_LEVEL = 16;
+
+struct KsArgs {
+    int n, N, k, count, ks_level, ks_radix_bits, ks_width_bits;
+    const void* acc;                   // Torus[count][k+1][N]
+    const int32_t* const* ksk_table;   // device array: KSK base per key slot
+    const int32_t* ct_slot;
+    int32_t* lwe_out;                  // [count][n+1]
+    int32_t* partial;                  // [count][n+1] (slices > 1)
+};
+
+// ---------------------------------------------------------------- helpers
+// Workgroup b runs on XCD b % 8 (round-robin dispatch); map logical work so each
+// XCD gets a contiguous range (same keys share that XCD's L2).  The grid is a
+// multiple of 8.
+__device__ __forceinline__ int map_workgroup(int b, int grid) {
+    const int per = grid / 8;
+    return (b % 8) * per + b / 8;
+}
+
+// modSwitchFromTorusGeneral(x, 2N, 2^32) (numeric.cpp:189-197): round half away
+// from zero of x / 2^(32 - log2(2N)), then sign extension mod 2N.
+__device__ __forceinline__ int mod_switch_2n(int32_t x, int log2_2n) {
+    const int sb = 32 - log2_2n;
+    const int64_t h = int64_t{1} << (sb - 1);
+    const int64_t v = x;
+    const int64_t r = v >= 0 ? ((v + h) >> sb) : -((-v + h) >> sb);
+    return static_cast<int>(static_cast<int64_t>(static_cast<uint64_t>(r) << (64 - log2_2n)) >> (64 - log2_2n));
+}
+
+// validateRotator (polynomial.cpp:22-29).
+__device__ __forceinline__ void validate_rotator(int a, int N, int& aT, int& wrap) {
+    int t = a % (2 * N);
+    if (t < 0) t += 2 * N;
+    wrap = t < N ? 1 : -1;
+    aT = t < N ? t : t - N;
+}
+
+// Torus (2^TBITS) -> LWE (2^32) modswitch of extractTlweFromTrlwe
+// (trlwe.cpp:247,250 via numeric.cpp:189-197).
+template <int TBITS>
+__device__ __forceinline__ int32_t torus_to_lwe32(int64_t x) {
+    if constexpr (TBITS == 32) {
+        return static_cast<int32_t>(x);
+    } else {
+        constexpr int sb = TBITS - 32;
+        const int64_t h = int64_t{1} << (sb - 1);
+        const int64_t r = x >= 0 ? ((x + h) >> sb) : -((-x + h) >> sb);
+        return static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(r)));
+    }
+}
+
+// Keyswitch digits: zeroMeanDigits(x, t, b_ks, width) (gadget_decomposition.h:75-95
+// via signedGadgetDecompositionKs, gadget_decomposition.cpp:53-56).
+__device__ __forceinline__ void ks_digits(int32_t x, int t, int b, int width, int8_t* out) {
+    const uint64_t mask = width >= 64 ? ~0ULL : ((1ULL << width) - 1);
+    const uint64_t u = static_cast<uint64_t>(static_cast<int64_t>(x)) & mask;
+    const bool neg = (u >> (width - 1)) & 1;
+    const int dropped = width - t * b;
+    const uint64_t bias = dropped > 0 ? (1ULL << (dropped - 1)) : 0;
+    const uint64_t m = (neg ? ((0ULL - u) & mask) : u) + bias;
+    const int B = 1 << b, halfB = B >> 1;
+    int carry = 0;
+    for (int lvl = t - 1; lvl >= 0; lvl--) {
+        const int shift = width - (lvl + 1) * b;
+        const int window = static_cast<int>((m >> shift) & static_cast<uint64_t>(B - 1));
+        const int digit = window + carry;
+        const int tie = lvl > 0 ? static_cast<int>((m >> (shift + 2 * b - 1)) & 1) : 0;
+        carry = (digit > halfB) | ((digit == halfB) & tie);
+        const int bal = carry ? digit - B : digit;
+        out[lvl] = static_cast<int8_t>(neg ? -bal : bal);
+    }
+}
+
+// Exact value mod q, centred in [-(q-1)/2, (q-1)/2], of the integer C with
+// C = c0 mod p0, C = c1 mod p1 and |C| < p0 p1 / 2 (Garner), from the centred
+// residues c0, c1.  C = c0 + p0 * y with y = (c1 - c0) p0^{-1} mod p1 centred;
+// the quotient C / q is estimated in binary64 (off by at most one) and the
+// remainder formed exactly in 128-bit integers.
+__device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF64& P1, const CrtConst& k) {
+    const double y = centre(mulmod(__dadd_rn(c1, -c0), k.inv01, P1), P1);
+    const __int128 C = static_cast<__int128>(k.p0i) * static_cast<int64_t>(y) + static_cast<int64_t>(c0);
+    const double Cd = __fma_rn(k.p0, y, c0);
+    const int64_t qt = static_cast<int64_t>(__builtin_rint(__dmul_rn(Cd, k.qinv)));
+    __int128 R = C - static_cast<__int128>(qt) * static_cast<__int128>(k.q);
+    if (R > k.qhalf) R -= static_cast<__int128>(k.q);
+    if (R < -k.qhalf) R += static_cast<__int128>(k.q);
+    return static_cast<int64_t>(R);
+}
+
+hipError_t launch_blind_rotate(int tbits, int logn, int tpc, const BrArgs& a, hipStream_t s);
+hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s);
+hipError_t launch_extract(int tbits, const KsArgs& a, int32_t* out, hipStream_t s);
+
+}  // namespace ygpu

@@ -1,0 +1,152 @@
+// ygpu_keygen.cpp -- seeded key generation and LWE encryption on the host.
+//
+// The reference draws keys from an OS-seeded ChaCha20 (csprng.cpp:63-72) with
+// no seed API, so its key bits cannot be reproduced; this generator follows the
+// same encryption equations with a seeded, thread-count-independent stream per
+// ciphertext row, and Gaussian noise rounded to integers.
+//   genTlweKey / genTrlweKey        binary keys (tlwe.cpp, trlwe.cpp:70-81)
+//   encryptTrgswMPNtt               (trgsw.cpp:36-51): row r < k encrypts 0 and adds
+//                                   mu_lvl to its mask polynomial a_r; row k (cPrime)
+//                                   encrypts mu_lvl in the body; masks uniform mod q
+//                                   (trlwe.cpp:25-48 samples them in the NTT domain,
+//                                   which is the same distribution)
+//   genTlweKeySwitchingKey          (keyswitching.cpp:19-29): KSK[i][j] = LWE_s(z_i * 2^(w - (j+1) b))
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#include <thread>
+#include <atomic>
+#include <string>
+#include <algorithm>
+#include <functional>
+#include "yatfhe_gpu.h"
+#include "ygpu_math.h"
+
+using namespace ygpu;
+
+namespace {
+
+void par_for(int count, int threads, const std::function<void(int)>& fn) {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count) break;
+            fn(i);
+        }
+    };
+    std::vector<std::thread> ts;
+    const int nt = std::max(1, std::min(threads, count));
+    for (int t = 1; t < nt; t++) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
+}
+
+inline uint64_t lift(int64_t v, uint64_t q) {
+    int64_t m = v % static_cast<int64_t>(q);
+    if (m < 0) m += static_cast<int64_t>(q);
+    return static_cast<uint64_t>(m);
+}
+
+// stream identifiers
+constexpr uint64_t S_LWE_KEY = 1, S_GLWE_KEY = 2;
+constexpr uint64_t S_BSK = 1ULL << 40, S_KSK = 2ULL << 40, S_ENC = 3ULL << 40;
+
+}  // namespace
+
+extern "C" {
+
+int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t* lwe_key,
+                int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
+    if (!params || !lwe_key || !glwe_key || !bsk_coeff || !ksk) return YGPU_ERR_INVALID_ARG;
+    const ygpu_params p = *params;
+    if (p.k < 1 || p.N < 2 || p.n < 1 || p.level < 1 || p.ks_level < 1) return YGPU_ERR_INVALID_ARG;
+    const int N = p.N, K = p.k, K1 = p.k + 1, L = p.level, n = p.n, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    try {
+        {
+            Rng r(seed, S_LWE_KEY);
+            for (int i = 0; i < n; i++) lwe_key[i] = static_cast<int32_t>(r.next() & 1);
+        }
+        {
+            Rng r(seed, S_GLWE_KEY);
+            for (int i = 0; i < K * N; i++) glwe_key[i] = static_cast<int32_t>(r.next() & 1);
+        }
+        HostNtt ntt(q, N);
+        // transformed GLWE key
+        std::vector<std::vector<uint64_t>> zt(K, std::vector<uint64_t>(N));
+        for (int c = 0; c < K; c++) {
+            for (int j = 0; j < N; j++) zt[c][j] = static_cast<uint64_t>(glwe_key[c * N + j]);
+            ntt.forward(zt[c].data());
+        }
+        const double sg = std::pow(2.0, sigma_log2_glwe + w);
+        const int nt = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+        // BSK rows: [i][lvl][row][col][N]
+        par_for(n * L * K1, nt, [&](int idx) {
+            const int i = idx / (L * K1);
+            const int lvl = (idx / K1) % L;
+            const int row = idx % K1;
+            Rng r(seed, S_BSK + static_cast<uint64_t>(idx));
+            uint64_t* out = bsk_coeff + static_cast<size_t>(idx) * K1 * N;
+            std::vector<uint64_t> body(N, 0), tmp(N);
+            for (int c = 0; c < K; c++) {
+                uint64_t* a = out + static_cast<size_t>(c) * N;
+                for (int j = 0; j < N; j++) a[j] = r.below(q);
+                std::memcpy(tmp.data(), a, sizeof(uint64_t) * N);
+                ntt.forward(tmp.data());
+                for (int j = 0; j < N; j++) tmp[j] = mulmod(tmp[j], zt[c][j], q);
+                ntt.inverse(tmp.data());
+                for (int j = 0; j < N; j++) body[j] = (body[j] + tmp[j]) % q;
+            }
+            for (int j = 0; j < N; j++) body[j] = (body[j] + lift(r.gauss(sg), q)) % q;
+            // mu_lvl = s_i * 2^(w - (lvl+1) b) as a torus value, lifted mod q
+            const int64_t mu = sext(static_cast<int64_t>(lwe_key[i]) << (w - (lvl + 1) * p.radix_bits), w);
+            if (row == K) {
+                body[0] = (body[0] + lift(mu, q)) % q;
+            } else {
+                uint64_t* a = out + static_cast<size_t>(row) * N;
+                a[0] = (a[0] + lift(mu, q)) % q;
+            }
+            std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
+        });
+        // KSK rows: [i][j][n+1], i < kN
+        const double sl = std::pow(2.0, sigma_log2_lwe + 32);
+        const int t = p.ks_level;
+        par_for(K * N * t, nt, [&](int idx) {
+            const int i = idx / t, j = idx % t;
+            Rng r(seed, S_KSK + static_cast<uint64_t>(idx));
+            int32_t* out = ksk + static_cast<size_t>(idx) * (n + 1);
+            uint32_t dot = 0;
+            for (int u = 0; u < n; u++) {
+                out[u] = static_cast<int32_t>(static_cast<uint32_t>(r.next()));
+                if (lwe_key[u]) dot += static_cast<uint32_t>(out[u]);
+            }
+            const uint32_t m = static_cast<uint32_t>(glwe_key[i]) << (p.ks_width_bits - (j + 1) * p.ks_radix_bits);
+            const uint32_t e = static_cast<uint32_t>(static_cast<int64_t>(r.gauss(sl)));
+            out[n] = static_cast<int32_t>(dot + e + m);
+        });
+    } catch (...) {
+        return YGPU_ERR_INVALID_ARG;
+    }
+    return YGPU_OK;
+}
+
+int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count, const int32_t* mu,
+                     int32_t* out) {
+    if (n < 1 || !lwe_key || count < 0 || (!mu && count) || (!out && count)) return YGPU_ERR_INVALID_ARG;
+    const double s = std::pow(2.0, sigma_log2 + 32);
+    for (int c = 0; c < count; c++) {
+        Rng r(seed, S_ENC + static_cast<uint64_t>(c));
+        int32_t* o = out + static_cast<size_t>(c) * (n + 1);
+        uint32_t dot = 0;
+        for (int u = 0; u < n; u++) {
+            o[u] = static_cast<int32_t>(static_cast<uint32_t>(r.next()));
+            if (lwe_key[u]) dot += static_cast<uint32_t>(o[u]);
+        }
+        const uint32_t e = static_cast<uint32_t>(static_cast<int64_t>(r.gauss(s)));
+        o[n] = static_cast<int32_t>(dot + e + static_cast<uint32_t>(mu[c]));
+    }
+    return YGPU_OK;
+}
+
+}  // extern "C"
