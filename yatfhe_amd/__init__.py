@@ -173,11 +173,23 @@ class KeySet:
 
 SIGMA_LOG2_LWE = -15.0
 SIGMA_LOG2_GLWE = -25.0
+# TORUS56 / N=2048 / 2^10 x 3: GLWE noise 2^-25 would exceed a 4-bit slot after
+# 742 external products; the reference's TORUS56 default (TUniform bound 2^8 on
+# a 56-bit torus, yatfhe_parameters.h:51) is ~2^-48.
+SIGMA_LOG2_GLWE_T56 = -48.0
 
 
-def keygen(params: Params, seed: int, sigma_log2_lwe: float = SIGMA_LOG2_LWE,
-           sigma_log2_glwe: float = SIGMA_LOG2_GLWE, threads: int = 0) -> KeySet:
+def default_sigmas(params: "Params"):
+    return SIGMA_LOG2_LWE, (SIGMA_LOG2_GLWE if params.torus_bits == 32 else SIGMA_LOG2_GLWE_T56)
+
+
+def keygen(params: Params, seed: int, sigma_log2_lwe: float | None = None,
+           sigma_log2_glwe: float | None = None, threads: int = 0) -> KeySet:
+    """Seeded keys; noise defaults per torus width (default_sigmas)."""
     lib = load_library()
+    dl, dg = default_sigmas(params)
+    sigma_log2_lwe = dl if sigma_log2_lwe is None else sigma_log2_lwe
+    sigma_log2_glwe = dg if sigma_log2_glwe is None else sigma_log2_glwe
     lk = np.zeros(params.n, np.int32)
     gk = np.zeros(params.k * params.N, np.int32)
     bsk = np.zeros(params.bsk_shape, np.uint64)

@@ -8,12 +8,13 @@
 
 namespace ygpu {
 
-// Blind-rotation kernel configuration (k = 1).
+// Blind-rotation kernel configuration (k = 1, two teams per ciphertext).
 //   TBITS  torus width (32 or 56)
 //   LOGN   log2 ring degree (10 or 11)
 //   NP     transform primes (1: the reference prime itself; 2: CRT pair)
-//   TPC    teams per ciphertext (1 or 2), each team T = N/16 lanes
-template <int TBITS_, int LOGN_, int NP_, int TPC_>
+//   CPW    ciphertexts per workgroup
+//   PACK8  digits packed 8 bits each (radix_bits <= 8), else 16 bits
+template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_>
 struct BrCfg {
     static constexpr int TBITS = TBITS_;
     using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
@@ -22,10 +23,11 @@ struct BrCfg {
     static constexpr int T = N / 16;
     static constexpr int TEAM_WAVES = T / 64;
     static constexpr int NP = NP_;
-    static constexpr int TPC = TPC_;
-    static constexpr int THREADS = T * TPC;
-    static constexpr int ROWS_PER_TEAM = (TPC == 2) ? 1 : 2;
-    static constexpr int OWNED = (TPC == 2) ? 1 : 2;
+    static constexpr int CPW = CPW_;
+    static constexpr bool PACK8 = PACK8_;
+    static constexpr int THREADS = 2 * T * CPW;
+    // twiddles in LDS when they fit beside the per-ciphertext state
+    static constexpr bool TW_LDS = NP * 2 * N * 8 <= 16384;
 };
 
 // Garner reconstruction constants for NP == 2.
@@ -145,7 +147,7 @@ __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF6
     return static_cast<int64_t>(R);
 }
 
-hipError_t launch_blind_rotate(int tbits, int logn, int tpc, const BrArgs& a, hipStream_t s);
+hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, const BrArgs& a, hipStream_t s);
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s);
 hipError_t launch_extract(int tbits, const KsArgs& a, int32_t* out, hipStream_t s);
 

@@ -21,41 +21,57 @@ namespace ygpu {
 // polynomial, multiply-accumulate against the NTT-form BSK row, inverse NTT of
 // each output column, centre mod q, reduce to the torus (ntt_hexl.cpp:153-175).
 //
-// Work split: the workgroup holds TPC teams of T = N/16 lanes.  Team tm
-// transforms the digit polynomials d with d % TPC == tm (d = lvl*(k+1) + row)
-// and owns output columns c with c % TPC == tm.  With TPC == 2 team r handles
-// exactly the digits of row r and the output column r, so it reads and writes
-// only accumulator row r.
+// Work split: a workgroup holds CPW ciphertexts, each served by two teams of
+// T = N/16 lanes (T = 64: one wavefront).  Team r transforms the digit
+// polynomials of accumulator row r (d = lvl*2 + r, lvl < level) and owns output
+// column r, so it reads and writes only accumulator row r; the two teams swap
+// their partial sums for the other column once per step.  The workgroup's
+// ciphertexts run in lockstep (the swap is a workgroup barrier), so the BSK row
+// loads of one step hit in L1 after the first ciphertext; the transform
+// twiddles sit in LDS, shared by the workgroup.
 template <class C>
 __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     using TorusT = typename C::TorusT;
-    constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = C::NP, TPC = C::TPC, TW = C::TEAM_WAVES;
+    constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = C::NP, TW = C::TEAM_WAVES, CPW = C::CPW;
     constexpr int SCR = Scratch<LOGN>::DOUBLES;
+    constexpr int CT_THREADS = 2 * T;
+    constexpr int TWN = C::TW_LDS ? NP * 2 * N : 1;
 
-    __shared__ TorusT acc[K1 * N];
-    __shared__ double scratch[TPC * SCR];
+    __shared__ double tws[TWN];
+    __shared__ TorusT accs[CPW][K1 * N];
+    __shared__ double scratch[CPW][2 * SCR];
 
-    const int wg = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
-    if (wg >= a.count) return;
-    const int tm = static_cast<int>(threadIdx.x) / T;
-    const int tl = static_cast<int>(threadIdx.x) % T;
-    double* scr = scratch + tm * SCR;
+    const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    const int first = grp * CPW;
+    if (first >= a.count) return;  // whole workgroup
+    const int lc = static_cast<int>(threadIdx.x) / CT_THREADS;
+    const int ctid = static_cast<int>(threadIdx.x) % CT_THREADS;
+    const int tm = ctid / T;
+    const int tl = ctid % T;
+    // tail slots of the last workgroup repeat its last ciphertext and write nothing
+    const bool live = first + lc < a.count;
+    const int ct = live ? first + lc : a.count - 1;
+    TorusT* acc = accs[lc];
+    double* scr = scratch[lc] + tm * SCR;
 
     const int n = a.n;
     const int L = a.level;
     const int rb = a.radix_bits;
-    const int slot = a.ct_slot ? a.ct_slot[wg] : 0;
+    const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
     const double* __restrict__ bsk = a.bsk_table[slot];
-    const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(wg) * (n + 1);
-    const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * wg;
+    const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
+    const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
 
+    if constexpr (C::TW_LDS) {
+        for (int j = static_cast<int>(threadIdx.x); j < TWN; j += C::THREADS) tws[j] = a.tw[j];
+    }
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
     // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
     {
         const int bbar = mod_switch_2n(lwe[n], LOGN + 1);
         int aT, wrap;
         validate_rotator(-bbar, N, aT, wrap);
-        for (int j = static_cast<int>(threadIdx.x); j < N; j += C::THREADS) {
+        for (int j = ctid; j < N; j += CT_THREADS) {
             acc[j] = 0;
             const int src = j - aT;
             const int64_t val = src < 0 ? -static_cast<int64_t>(v[src + N]) * wrap : static_cast<int64_t>(v[src]) * wrap;
@@ -63,121 +79,120 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
         }
     }
     __syncthreads();
+    const double* twBase = C::TW_LDS ? static_cast<const double*>(tws) : a.tw;
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
+    TorusT* accRow = acc + tm * N;
     for (int i = 0; i < n; i++) {
+        // (blind_rotate.cpp:1557-1559 skips a_i == 0; ExtProd(0) = 0 exactly,
+        // so the lockstep workgroup simply computes it.)
         const int abar = mod_switch_2n(lwe[i], LOGN + 1);
-        if (abar == 0) continue;  // ExtProd of zero is zero (blind_rotate.cpp:1557-1559)
         int aT, wrap;
         validate_rotator(abar, N, aT, wrap);
 
-        // (X^a - 1) acc at this lane's pass-0 positions, then all digits,
-        // packed two 16-bit digits per word.
-        uint32_t dig[C::ROWS_PER_TEAM][16][2];
+        // (X^a - 1) acc_r at this lane's pass-0 positions (rotateTorusPolynomialMinusOne,
+        // polynomial.cpp:213-238) and its digits, packed 8 or 16 bits per digit.
+        uint32_t dg0[16], dg1[16];
 #pragma unroll
-        for (int rr = 0; rr < C::ROWS_PER_TEAM; rr++) {
-            const int row = (TPC == 2) ? tm : rr;
-            const TorusT* accRow = acc + row * N;
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int j = tl + T * r;
-                int src = j - aT;
-                int sgn = wrap;
-                if (src < 0) { src += N; sgn = -wrap; }
-                const int64_t rot = sext_t<C::TBITS>(static_cast<int64_t>(accRow[src]) * sgn - static_cast<int64_t>(accRow[j]));
-                int d4[4] = {0, 0, 0, 0};
-                digits_of<C::TBITS>(rot, L, rb, d4);
-                dig[rr][r][0] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
-                dig[rr][r][1] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+        for (int r = 0; r < 16; r++) {
+            const int j = tl + T * r;
+            int src = j - aT;
+            int sgn = wrap;
+            if (src < 0) { src += N; sgn = -wrap; }
+            const int64_t rot = sext_t<C::TBITS>(static_cast<int64_t>(accRow[src]) * sgn - static_cast<int64_t>(accRow[j]));
+            int d4[4] = {0, 0, 0, 0};
+            digits_of<C::TBITS>(rot, L, rb, d4);
+            if constexpr (C::PACK8) {
+                dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFu) | ((static_cast<uint32_t>(d4[1]) & 0xFFu) << 8) |
+                         ((static_cast<uint32_t>(d4[2]) & 0xFFu) << 16) | (static_cast<uint32_t>(d4[3]) << 24);
+                dg1[r] = 0;
+            } else {
+                dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
+                dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
             }
         }
 
         const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
-        double res[C::OWNED][NP > 1 ? 16 : 1];  // prime-0 residues of owned columns (NP == 2)
+        double res[NP > 1 ? 16 : 1];  // prime-0 residues of the owned column (NP == 2)
 
 #pragma unroll
         for (int np = 0; np < NP; np++) {
             const PrimeF64 pr = a.prime[np];
-            const double* __restrict__ tw = a.tw + static_cast<size_t>(np) * 2 * N;
-            const double* __restrict__ itw = tw + N;
-            double out[K1][16];
+            const double* tw = twBase + static_cast<size_t>(np) * 2 * N;
+            const double* itw = tw + N;
+            // own: output column tm (this team's); oth: column 1 - tm.  The
+            // team index only moves addresses, never register-array indices.
+            double own[16], oth[16];
 #pragma unroll
-            for (int c = 0; c < K1; c++)
-#pragma unroll
-                for (int r = 0; r < 16; r++) out[c][r] = 0.0;
+            for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
 
-            for (int d = tm; d < L * K1; d += TPC) {
-                const int lvl = d / K1;
-                const int row = d % K1;
-                const int rr = (TPC == 2) ? 0 : row;
+            for (int lvl = 0; lvl < L; lvl++) {
+                const int d = lvl * K1 + tm;
                 // BSK values of this digit polynomial, both columns, issued
                 // before the transform so their latency overlaps it.
-                double kb[K1][16];
+                double kbo[16], kbt[16];
                 const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + np) * N + tl;
+                const double* __restrict__ kpo = kp + static_cast<size_t>(tm) * NP * N;
+                const double* __restrict__ kpt = kp + static_cast<size_t>(1 - tm) * NP * N;
 #pragma unroll
-                for (int c = 0; c < K1; c++)
-#pragma unroll
-                    for (int r = 0; r < 16; r++) kb[c][r] = kp[static_cast<size_t>(c) * NP * N + r * T];
+                for (int r = 0; r < 16; r++) { kbo[r] = kpo[r * T]; kbt[r] = kpt[r * T]; }
                 double x[16];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    const uint32_t w = dig[rr][r][lvl >> 1];
-                    const int dv = (lvl & 1) ? (static_cast<int32_t>(w) >> 16) : (static_cast<int32_t>(w << 16) >> 16);
+                    int dv;
+                    if constexpr (C::PACK8) {
+                        dv = static_cast<int32_t>(dg0[r] << (24 - 8 * lvl)) >> 24;
+                    } else {
+                        const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
+                        dv = static_cast<int32_t>(w << (16 - 16 * (lvl & 1))) >> 16;
+                    }
                     x[r] = static_cast<double>(dv);
                 }
                 ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
 #pragma unroll
-                for (int c = 0; c < K1; c++)
-#pragma unroll
-                    for (int r = 0; r < 16; r++) out[c][r] = __dadd_rn(out[c][r], mulmod(x[r], kb[c][r], pr));
+                for (int r = 0; r < 16; r++) {
+                    own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
+                    oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
+                }
             }
 
-            if constexpr (TPC == 2) {
-                // partner sums: team tm hands column 1-tm to the other team
-                team_sync<TW>();
+            // partner sums: hand column 1-tm to the other team, gather the full
+            // sum of column tm
+            team_sync<TW>();
 #pragma unroll
-                for (int r = 0; r < 16; r++) scr[r * T + tl] = out[1 - tm][r];
-                __syncthreads();
-                const double* other = scratch + (1 - tm) * SCR;
+            for (int r = 0; r < 16; r++) scr[r * T + tl] = oth[r];
+            __syncthreads();
+            const double* other = scratch[lc] + (1 - tm) * SCR;
+            double x[16];
 #pragma unroll
-                for (int r = 0; r < 16; r++) out[tm][r] = __dadd_rn(out[tm][r], other[r * T + tl]);
-                __syncthreads();
-            }
+            for (int r = 0; r < 16; r++) x[r] = __dadd_rn(own[r], other[r * T + tl]);
+            __syncthreads();
 
+            ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+            if constexpr (NP == 1) {
+                // centre mod q, reduce to the torus, accumulate (mod 2^w)
 #pragma unroll
-            for (int oc = 0; oc < C::OWNED; oc++) {
-                const int c = (TPC == 2) ? tm : oc;
-                double x[16];
+                for (int r = 0; r < 16; r++) {
+                    const int j = tl + T * r;
+                    const int64_t val = static_cast<int64_t>(centre(x[r], pr));
+                    accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + val));
+                }
+            } else {
+                if (np == 0) {
 #pragma unroll
-                for (int r = 0; r < 16; r++) x[r] = out[c][r];
-                ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
-                if constexpr (NP == 1) {
-                    // centre mod q, reduce to the torus, accumulate (mod 2^w)
-                    TorusT* accRow = acc + c * N;
+                    for (int r = 0; r < 16; r++) res[r] = centre(x[r], pr);
+                } else {
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
                         const int j = tl + T * r;
-                        const int64_t val = static_cast<int64_t>(centre(x[r], pr));
+                        const int64_t val = crt_mod_q(res[r], centre(x[r], pr), pr, a.crt);
                         accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + val));
-                    }
-                } else {
-                    if (np == 0) {
-#pragma unroll
-                        for (int r = 0; r < 16; r++) res[oc][r] = centre(x[r], pr);
-                    } else {
-                        TorusT* accRow = acc + c * N;
-#pragma unroll
-                        for (int r = 0; r < 16; r++) {
-                            const int j = tl + T * r;
-                            const int64_t val = crt_mod_q(res[oc][r], centre(x[r], pr), pr, a.crt);
-                            accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + val));
-                        }
                     }
                 }
             }
         }
-        // each team reads back, at rotated positions, only the accumulator
-        // rows it wrote itself
+        // each team reads back, at rotated positions, only the accumulator row
+        // it wrote itself
         if constexpr (TW == 1) {
             team_sync<1>();
         } else {
@@ -186,8 +201,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     }
 
     __syncthreads();
-    TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(wg) * K1 * N;
-    for (int j = static_cast<int>(threadIdx.x); j < K1 * N; j += C::THREADS) outAcc[j] = acc[j];
+    if (live) {
+        TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N;
+        for (int j = ctid; j < K1 * N; j += CT_THREADS) outAcc[j] = acc[j];
+    }
 }
 
 // ============================================================ keyswitch
@@ -362,20 +379,22 @@ __global__ void k_extract(KsArgs a, int32_t* out) {
 // ============================================================ launchers
 template <class C>
 hipError_t launch_br(const BrArgs& a, hipStream_t s) {
-    const unsigned grid = static_cast<unsigned>((a.count + 7) / 8 * 8);  // map_workgroup needs a multiple of 8
+    const int groups = (a.count + C::CPW - 1) / C::CPW;
+    const unsigned grid = static_cast<unsigned>((groups + 7) / 8 * 8);  // map_workgroup needs a multiple of 8
     hipLaunchKernelGGL(k_blind_rotate<C>, dim3(grid), dim3(C::THREADS), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_blind_rotate(int tbits, int logn, int tpc, const BrArgs& a, hipStream_t s) {
-    if (tbits == 32 && logn == 10 && tpc == 1) return launch_br<BrCfg<32, 10, 1, 1>>(a, s);
-    if (tbits == 32 && logn == 10 && tpc == 2) return launch_br<BrCfg<32, 10, 1, 2>>(a, s);
-    if (tbits == 32 && logn == 11 && tpc == 1) return launch_br<BrCfg<32, 11, 1, 1>>(a, s);
-    if (tbits == 32 && logn == 11 && tpc == 2) return launch_br<BrCfg<32, 11, 1, 2>>(a, s);
-    if (tbits == 56 && logn == 10 && tpc == 1) return launch_br<BrCfg<56, 10, 2, 1>>(a, s);
-    if (tbits == 56 && logn == 10 && tpc == 2) return launch_br<BrCfg<56, 10, 2, 2>>(a, s);
-    if (tbits == 56 && logn == 11 && tpc == 1) return launch_br<BrCfg<56, 11, 2, 1>>(a, s);
-    if (tbits == 56 && logn == 11 && tpc == 2) return launch_br<BrCfg<56, 11, 2, 2>>(a, s);
+// cpw: ciphertexts per workgroup (1 for the plain form)
+hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, const BrArgs& a, hipStream_t s) {
+    const bool p8 = radix_bits <= 8;
+    if (tbits == 32 && logn == 10) {
+        if (cpw >= 4) return p8 ? launch_br<BrCfg<32, 10, 1, 4, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, false>>(a, s);
+        return p8 ? launch_br<BrCfg<32, 10, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, false>>(a, s);
+    }
+    if (tbits == 32 && logn == 11) return p8 ? launch_br<BrCfg<32, 11, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 11, 1, 1, false>>(a, s);
+    if (tbits == 56 && logn == 10) return p8 ? launch_br<BrCfg<56, 10, 2, 1, true>>(a, s) : launch_br<BrCfg<56, 10, 2, 1, false>>(a, s);
+    if (tbits == 56 && logn == 11) return p8 ? launch_br<BrCfg<56, 11, 2, 1, true>>(a, s) : launch_br<BrCfg<56, 11, 2, 1, false>>(a, s);
     return hipErrorInvalidValue;
 }
 

@@ -57,7 +57,7 @@ struct ygpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int logn = 0, np = 1, tbits = 32;
-    int tpc_batch = 2, tpc_plain = 2;
+    int cpw_batch = 4;
     uint64_t primes[2] = {0, 0};
     std::vector<HostNtt> ntt;
     double* d_tw = nullptr;
@@ -261,8 +261,8 @@ int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* ta
     return YGPU_OK;
 }
 
-int run_br(ygpu_ctx* ctx, int tpc, const BrArgs& a) {
-    YG_HIP(timed(ctx, "k_blind_rotate", [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, tpc, a, ctx->stream); }));
+int run_br(ygpu_ctx* ctx, int cpw, const BrArgs& a) {
+    YG_HIP(timed(ctx, "k_blind_rotate", [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, cpw, ctx->p.radix_bits, a, ctx->stream); }));
     return YGPU_OK;
 }
 
@@ -338,8 +338,7 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         ctx->primes[0] = P_AUX_0;
         ctx->primes[1] = P_AUX_1;
     }
-    if (const char* e = std::getenv("YGPU_TPC_BATCH")) ctx->tpc_batch = std::atoi(e) == 1 ? 1 : 2;
-    if (const char* e = std::getenv("YGPU_TPC_PLAIN")) ctx->tpc_plain = std::atoi(e) == 1 ? 1 : 2;
+    if (const char* e = std::getenv("YGPU_CPW")) ctx->cpw_batch = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const int N = params->N;
@@ -490,7 +489,7 @@ int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const in
             acc = ctx->d_acc;
         }
         BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, ctx->tpc_batch, a);
+        rc = run_br(ctx, ctx->cpw_batch, a);
         if (rc) return rc;
         return run_ks(ctx, count, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, d_extract_out);
     });
@@ -526,7 +525,7 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
         }
         const int count = static_cast<int>(total);
         BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table, ctx->d_slots);
-        rc = run_br(ctx, ctx->tpc_batch, a);
+        rc = run_br(ctx, ctx->cpw_batch, a);
         if (rc) return rc;
         rc = run_ks(ctx, count, acc, ctx->d_ksk_table, ctx->d_slots, d_lwe_out, nullptr);
         if (rc) return rc;
@@ -546,7 +545,7 @@ int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in
         if (rc) return rc;
         void* acc = d_acc_out ? d_acc_out : ctx->d_acc1;
         BrArgs a = br_args(ctx, 1, d_lwe_in, d_v, 0, acc, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, ctx->tpc_plain, a);
+        rc = run_br(ctx, 1, a);
         if (rc) return rc;
         return run_ks(ctx, 1, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, nullptr);
     });
@@ -565,7 +564,7 @@ int ygpu_pbs(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v
         YG_HIP(hipMemcpyAsync(ctx->d_lwe1, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, ctx->stream));
         YG_HIP(hipMemcpyAsync(ctx->d_v1, v, tb * p.N, hipMemcpyHostToDevice, ctx->stream));
         BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, ctx->tpc_plain, a);
+        rc = run_br(ctx, 1, a);
         if (rc) return rc;
         rc = run_ks(ctx, 1, ctx->d_acc1, ctx->d_ksk_table + key_id, nullptr, ctx->d_out1, extract_out ? ctx->d_ext1 : nullptr);
         if (rc) return rc;
@@ -589,7 +588,7 @@ int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, cons
         YG_HIP(hipMemcpyAsync(ctx->d_lwe1, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, ctx->stream));
         YG_HIP(hipMemcpyAsync(ctx->d_v1, v, tb * p.N, hipMemcpyHostToDevice, ctx->stream));
         BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, ctx->tpc_plain, a);
+        rc = run_br(ctx, 1, a);
         if (rc) return rc;
         YG_HIP(hipMemcpyAsync(acc_out, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
         YG_HIP(hipStreamSynchronize(ctx->stream));
