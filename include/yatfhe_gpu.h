@@ -148,6 +148,16 @@ int ygpu_timing_enable(ygpu_ctx* ctx, int32_t on);
 int ygpu_timing_get(ygpu_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
 int ygpu_timing_reset(ygpu_ctx* ctx);
 
+/* Measurement helpers (no counterpart in the reference).
+ * ygpu_modmul_peak: throughput of a microbenchmark running independent chains
+ * of modular products mod the context's first transform prime; kind 0 is the
+ * binary64 exact product the engine uses, kind 1 a 64-bit integer Shoup
+ * product.  The larger is the roofline denominator for the batched forms.
+ * ygpu_copy_device: a plain streaming copy (bytes % 16 == 0), the calibration
+ * kernel for HBM byte counters. */
+int ygpu_modmul_peak(ygpu_ctx* ctx, int32_t kind, double* modmul_per_sec);
+int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+
 /* Seeded key generation on the host (genTlweKey, genTrlweKey,
  * genBootstrappingKeyMP, genTlweKeySwitchingKey: tlwe.cpp, trlwe.cpp:70-81,
  * bootstrapping.cpp:150-174, keyswitching.cpp:19-29), with Gaussian noise of

@@ -129,6 +129,8 @@ def load_library(path: str | None = None):
         "ygpu_timing_enable": (c_int, [c_vp, c_i32]),
         "ygpu_timing_get": (c_int, [c_vp, ctypes.c_char_p, P(c_i64), P(c_dbl)]),
         "ygpu_timing_reset": (c_int, [c_vp]),
+        "ygpu_modmul_peak": (c_int, [c_vp, c_i32, P(c_dbl)]),
+        "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
     }
@@ -148,6 +150,7 @@ def exported_symbols():
         "ygpu_extract_keyswitch", "ygpu_pbs_batch_device", "ygpu_pbs_jobs_device", "ygpu_pbs_plain_device",
         "ygpu_malloc", "ygpu_free", "ygpu_memcpy_htod", "ygpu_memcpy_dtoh", "ygpu_sync",
         "ygpu_timing_enable", "ygpu_timing_get", "ygpu_timing_reset", "ygpu_keygen", "ygpu_lwe_encrypt",
+        "ygpu_modmul_peak", "ygpu_copy_device",
     ]
 
 
@@ -400,6 +403,15 @@ class Context:
 
     def sync(self):
         _check(self.lib.ygpu_sync(self.h))
+
+    def modmul_peak(self, kind: int = 0) -> float:
+        """Measured modmul/s of the microbenchmark (0: binary64 exact, 1: 64-bit Shoup)."""
+        r = ctypes.c_double(0)
+        _check(self.lib.ygpu_modmul_peak(self.h, kind, ctypes.byref(r)))
+        return r.value
+
+    def copy_device(self, d_dst, d_src, nbytes: int):
+        _check(self.lib.ygpu_copy_device(self.h, d_dst.value, d_src.value, nbytes))
 
     def timing(self, on: bool):
         _check(self.lib.ygpu_timing_enable(self.h, 1 if on else 0))

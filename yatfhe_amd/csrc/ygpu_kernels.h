@@ -150,5 +150,8 @@ __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF6
 hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, const BrArgs& a, hipStream_t s);
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s);
 hipError_t launch_extract(int tbits, const KsArgs& a, int32_t* out, hipStream_t s);
+hipError_t launch_modmul_peak(int kind, const PrimeF64& pr, uint64_t q, int blocks, int iters, void* sink, hipStream_t s);
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+constexpr int MODMUL_CHAINS = 8;
 
 }  // namespace ygpu

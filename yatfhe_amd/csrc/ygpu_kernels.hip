@@ -376,6 +376,70 @@ __global__ void k_extract(KsArgs a, int32_t* out) {
     }
 }
 
+// ============================================================ measurement
+// Modmul throughput microbenchmark (the roofline denominator): every lane runs
+// MM_CHAINS independent chains x <- x * w mod p, either with the binary64 exact
+// product the engine uses or with a 64-bit integer Shoup product.
+constexpr int MM_CHAINS = 8;
+
+__global__ void __launch_bounds__(256) k_modmul_peak_f64(PrimeF64 pr, int iters, double* sink) {
+    double x[MM_CHAINS], w[MM_CHAINS];
+#pragma unroll
+    for (int c = 0; c < MM_CHAINS; c++) {
+        x[c] = static_cast<double>((threadIdx.x * 131 + c * 977 + blockIdx.x) % 100000) + 1.0;
+        w[c] = static_cast<double>(123456789 + c * 1000003);
+    }
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int c = 0; c < MM_CHAINS; c++) x[c] = mulmod(x[c], w[c], pr);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < MM_CHAINS; c++) acc = __dadd_rn(acc, x[c]);
+    if (acc == 12345.0) sink[blockIdx.x] = acc;  // keeps the chains live
+}
+
+__global__ void __launch_bounds__(256) k_modmul_peak_u64(uint64_t q, int iters, uint64_t* sink) {
+    uint64_t x[MM_CHAINS], w[MM_CHAINS], wp[MM_CHAINS];
+#pragma unroll
+    for (int c = 0; c < MM_CHAINS; c++) {
+        x[c] = (threadIdx.x * 131 + c * 977 + blockIdx.x) % 100000 + 1;
+        w[c] = 123456789 + c * 1000003;
+        wp[c] = static_cast<uint64_t>((static_cast<unsigned __int128>(w[c]) << 64) / q);
+    }
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int c = 0; c < MM_CHAINS; c++) {
+            const uint64_t hi = __umul64hi(x[c], wp[c]);
+            uint64_t r = x[c] * w[c] - hi * q;
+            x[c] = r >= q ? r - q : r;
+        }
+    }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < MM_CHAINS; c++) acc += x[c];
+    if (acc == 12345) sink[blockIdx.x] = acc;
+}
+
+// Plain streaming copy, the calibration kernel for HBM byte counters.
+__global__ void __launch_bounds__(256) k_copy_u32x4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_modmul_peak(int kind, const PrimeF64& pr, uint64_t q, int blocks, int iters, void* sink, hipStream_t s) {
+    if (kind == 0) hipLaunchKernelGGL(k_modmul_peak_f64, dim3(blocks), dim3(256), 0, s, pr, iters, static_cast<double*>(sink));
+    else hipLaunchKernelGGL(k_modmul_peak_u64, dim3(blocks), dim3(256), 0, s, q, iters, static_cast<uint64_t*>(sink));
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n = bytes / 16;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_copy_u32x4, dim3(blocks ? blocks : 1), dim3(256), 0, s, static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
+
 // ============================================================ launchers
 template <class C>
 hipError_t launch_br(const BrArgs& a, hipStream_t s) {

@@ -655,6 +655,44 @@ int ygpu_sync(ygpu_ctx* ctx) {
     });
 }
 
+int ygpu_modmul_peak(ygpu_ctx* ctx, int32_t kind, double* modmul_per_sec) {
+    if (!modmul_per_sec || (kind != 0 && kind != 1)) return fail(YGPU_ERR_INVALID_ARG, "bad argument");
+    return with_ctx(ctx, [&]() -> int {
+        hipDeviceProp_t prop;
+        YG_HIP(hipGetDeviceProperties(&prop, ctx->device));
+        const int blocks = prop.multiProcessorCount * 16;
+        const int iters = kind == 0 ? 4096 : 1024;
+        void* sink = nullptr;
+        YG_HIP(hipMalloc(&sink, sizeof(double) * blocks));
+        const PrimeF64 pr = ctx->prime[0];
+        const uint64_t q = ctx->primes[0];
+        hipEvent_t a, b;
+        YG_HIP(hipEventCreate(&a));
+        YG_HIP(hipEventCreate(&b));
+        YG_HIP(launch_modmul_peak(kind, pr, q, blocks, 16, sink, ctx->stream));  // warm-up
+        YG_HIP(hipEventRecord(a, ctx->stream));
+        YG_HIP(launch_modmul_peak(kind, pr, q, blocks, iters, sink, ctx->stream));
+        YG_HIP(hipEventRecord(b, ctx->stream));
+        YG_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        YG_HIP(hipEventElapsedTime(&ms, a, b));
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        hipFree(sink);
+        const double ops = static_cast<double>(blocks) * 256.0 * MODMUL_CHAINS * iters;
+        *modmul_per_sec = ops / (ms * 1e-3);
+        return YGPU_OK;
+    });
+}
+
+int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
+    if (!d_dst || !d_src || bytes % 16) return fail(YGPU_ERR_INVALID_ARG, "copy needs device buffers and a multiple of 16 bytes");
+    return with_ctx(ctx, [&]() -> int {
+        YG_HIP(timed(ctx, "k_copy_u32x4", [&] { return launch_copy(d_src, d_dst, bytes, ctx->stream); }));
+        return YGPU_OK;
+    });
+}
+
 int ygpu_timing_enable(ygpu_ctx* ctx, int32_t on) {
     return with_ctx(ctx, [&]() -> int {
         ctx->timing = on != 0;
