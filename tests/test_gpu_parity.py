@@ -81,3 +81,106 @@ def test_batched_pbs_t56_lut_bit_exact(p56, keys56, ctx56):
     orc.close()
     ph = yg.lwe_decrypt_phase(out, keys56.lwe_key).astype(np.int64)
     assert np.array_equal(np.round(ph / (1 << 32) * p).astype(np.int64) % p, lut[msgs] % p)
+
+
+def edge_ciphertexts(p, keys):
+    """Inputs at the edges of the modswitch and rotation rules."""
+    n = p.n
+    scale = (1 << 32) // (2 * p.N)
+    rows = []
+    z = np.zeros(n + 1, np.int64)
+    rows.append(z.copy())                                    # every abar = 0: all steps skipped
+    r = z.copy(); r[-1] = -(1 << 31); rows.append(r)         # bbar = -N
+    r = np.full(n + 1, -(1 << 31), np.int64); rows.append(r) # INT32_MIN everywhere
+    r = np.full(n + 1, (1 << 31) - 1, np.int64); rows.append(r)
+    r = np.zeros(n + 1, np.int64); r[::2] = scale // 2; r[1::2] = -(scale // 2); rows.append(r)   # ties round away
+    r = np.zeros(n + 1, np.int64); r[::3] = scale // 2 - 1; r[1::3] = p.N * scale; rows.append(r)
+    rng = np.random.default_rng(99)
+    r = rng.integers(-(1 << 31), 1 << 31, n + 1); rows.append(r)
+    return np.array(rows).astype(np.int64).astype(np.int32)
+
+
+def test_edge_inputs_t32(p32, keys32, ctx32, orc32):
+    ctx, kid = ctx32
+    cts = edge_ciphertexts(p32, keys32)
+    rng = np.random.default_rng(5)
+    vs = [yg.sign_test_polynomial(p32),
+          rng.integers(-(1 << 31), 1 << 31, p32.N).astype(np.int32)]
+    vs[1][:3] = [-(1 << 31), (1 << 31) - 1, 0]
+    for v in vs:
+        out, acc, ext = ctx.pbs_batch(kid, cts, v, want_acc=True, want_extract=True)
+        for c in range(len(cts)):
+            o_acc, o_ext, o_out = orc32.pbs(cts[c], v)
+            assert np.array_equal(acc[c], o_acc), f"accumulator differs, edge ct {c}"
+            assert np.array_equal(ext[c], o_ext), f"extract differs, edge ct {c}"
+            assert np.array_equal(out[c], o_out), f"keyswitch differs, edge ct {c}"
+
+
+def test_extract_keyswitch_adversarial_t32(p32, keys32, ctx32, orc32):
+    ctx, kid = ctx32
+    rng = np.random.default_rng(6)
+    for trial in range(4):
+        acc = rng.integers(-(1 << 31), 1 << 31, (2, p32.N)).astype(np.int32)
+        if trial == 1:
+            acc[:] = -(1 << 31)
+        if trial == 2:
+            acc[:] = 0
+        ext, out = ctx.extract_keyswitch(kid, acc)
+        o_ext = orc32.extract(acc)
+        assert np.array_equal(ext, o_ext)
+        assert np.array_equal(out, orc32.keyswitch(o_ext))
+
+
+def test_multi_key_jobs_t32(p32):
+    keysets = [yg.keygen(p32, seed=100 + t) for t in range(3)]
+    counts = [5, 70, 1]
+    v = yg.sign_test_polynomial(p32)
+    cts, bits = [], []
+    for t, ks in enumerate(keysets):
+        b = np.random.default_rng(t).integers(0, 2, counts[t])
+        bits.append(b)
+        cts.append(yg.lwe_encrypt(ks.lwe_key, ((2 * b - 1) * (1 << 29)).astype(np.int32), seed=200 + t))
+    allc = np.concatenate(cts)
+    with yg.Context(p32, 0) as ctx:
+        kids = [ctx.upload_key(ks) for ks in keysets]
+        jobs, first = [], 0
+        for t in range(3):
+            jobs.append((kids[t], counts[t], first))
+            first += counts[t]
+        d_lwe = ctx.alloc(allc.nbytes); d_lwe.upload(allc)
+        d_v = ctx.alloc(v.nbytes); d_v.upload(v)
+        d_out = ctx.alloc(allc.shape[0] * (p32.n + 1) * 4)
+        ctx.pbs_jobs_device(jobs, allc.shape[0], d_lwe, d_v, False, d_out)
+        ctx.sync()
+        mixed = d_out.download(np.zeros((allc.shape[0], p32.n + 1), np.int32))
+        for b in (d_lwe, d_v, d_out):
+            b.free()
+        first = 0
+        for t in range(3):
+            alone, _, _ = ctx.pbs_batch(kids[t], cts[t], v)
+            assert np.array_equal(mixed[first:first + counts[t]], alone), f"tenant {t} differs alone vs in the mix"
+            ph = yg.lwe_decrypt_phase(alone, keysets[t].lwe_key)
+            assert np.array_equal((ph > 0).astype(int), bits[t])
+            first += counts[t]
+    orc = Oracle(p32, keysets[1])
+    o = orc.pbs(cts[1][7], v)[2]
+    assert np.array_equal(mixed[5 + 7], o)
+    orc.close()
+
+
+def test_full_batch_4096_t32(p32, keys32, ctx32, orc32):
+    """The benchmark's batch size: a sample against the oracle, every output
+    decrypted, determinism, and each result equal alone and in the batch."""
+    ctx, kid = ctx32
+    bits, cts = sign_cts(keys32, 4096, seed=1)
+    v = yg.sign_test_polynomial(p32)
+    out1, _, _ = ctx.pbs_batch(kid, cts, v)
+    out2, _, _ = ctx.pbs_batch(kid, cts, v)
+    assert np.array_equal(out1, out2)
+    ph = yg.lwe_decrypt_phase(out1, keys32.lwe_key)
+    assert np.array_equal((ph > 0).astype(int), bits)
+    idx = np.random.default_rng(3).choice(4096, 24, replace=False)
+    assert np.array_equal(out1[idx], orc32.pbs_batch(cts[idx], v))
+    for c in idx[:3]:
+        alone, _, _ = ctx.pbs(kid, cts[c], v)
+        assert np.array_equal(alone, out1[c])

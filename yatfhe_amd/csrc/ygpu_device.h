@@ -15,6 +15,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <type_traits>
 
 namespace ygpu {
 
@@ -221,6 +222,51 @@ __device__ __forceinline__ void digits_of(int64_t x, int l, int rb, int (&out)[4
             carry = (digit > halfB) | ((digit == halfB) & tie);
             const int bal = carry ? digit - B : digit;
             out[lvl] = neg ? -bal : bal;
+        }
+    }
+}
+
+// Exact conversions of an integer-valued double: adding 1.5 * 2^52 keeps the
+// sum inside one binade, where the mantissa field is the integer plus 2^51.
+__device__ __forceinline__ int64_t f64_to_i64(double v) {  // |v| < 2^51
+    return static_cast<int64_t>(__double_as_longlong(__dadd_rn(v, 0x1.8p52))) - 0x4338000000000000LL;
+}
+__device__ __forceinline__ uint32_t f64_low32(double v) {  // |v| < 2^51: v mod 2^32
+    return static_cast<uint32_t>(__double_as_longlong(__dadd_rn(v, 0x1.8p52)));
+}
+
+// All L balanced digits (gadget_decomposition.h:75-95) of a value of a
+// TBITS-bit torus, with the gadget shape known at compile time.  Packed in
+// 8-bit fields (RB <= 8) into w0, or 16-bit fields into w0 (levels 0, 1) and
+// w1 (levels 2, 3).
+template <int TBITS, int L, int RB>
+__device__ __forceinline__ void digits_packed(int64_t x, uint32_t& w0, uint32_t& w1) {
+    using U = typename std::conditional<TBITS == 32, uint32_t, uint64_t>::type;
+    constexpr int DROP = TBITS - L * RB;
+    constexpr U BIAS = DROP > 0 ? (U{1} << (DROP - 1)) : U{0};
+    constexpr U MASK = (TBITS >= static_cast<int>(sizeof(U) * 8)) ? ~U{0} : static_cast<U>((uint64_t{1} << (TBITS % 64)) - 1);
+    constexpr int B = 1 << RB, HB = B >> 1;
+    const U u = static_cast<U>(x) & MASK;
+    const bool neg = (u >> (TBITS - 1)) & 1;
+    const U m = (neg ? ((U{0} - u) & MASK) : u) + BIAS;
+    const int sneg = neg ? -1 : 0;
+    int carry = 0;
+    w0 = 0;
+    w1 = 0;
+#pragma unroll
+    for (int lvl = L - 1; lvl >= 0; lvl--) {
+        const int shift = TBITS - (lvl + 1) * RB;
+        const int window = static_cast<int>((m >> shift) & static_cast<U>(B - 1));
+        const int digit = window + carry;
+        const int tie = lvl > 0 ? static_cast<int>((m >> (shift + 2 * RB - 1)) & 1) : 0;
+        carry = (digit > HB) | ((digit == HB) & tie);
+        const int bal = digit - (carry << RB);
+        const int sd = (bal ^ sneg) - sneg;
+        if constexpr (RB <= 8) {
+            w0 |= (static_cast<uint32_t>(sd) & 0xFFu) << (8 * lvl);
+        } else {
+            if (lvl < 2) w0 |= (static_cast<uint32_t>(sd) & 0xFFFFu) << (16 * lvl);
+            else w1 |= (static_cast<uint32_t>(sd) & 0xFFFFu) << (16 * (lvl - 2));
         }
     }
 }

@@ -14,8 +14,11 @@ namespace ygpu {
 //   NP     transform primes (1: the reference prime itself; 2: CRT pair)
 //   CPW    ciphertexts per workgroup
 //   PACK8  digits packed 8 bits each (radix_bits <= 8), else 16 bits
-template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_>
+//   L, RB  gadget levels and base log2 when fixed at compile time (0: runtime)
+template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0>
 struct BrCfg {
+    static constexpr int L = L_;
+    static constexpr int RB = RB_;
     static constexpr int TBITS = TBITS_;
     using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
     static constexpr int LOGN = LOGN_;
@@ -55,14 +58,16 @@ struct BrArgs {
 };
 
 constexpr int KS_THREADS = 256;
-constexpr int KS_CT = 8;
-constexpr int KS_COLS_PER_THREAD = 3;   // n + 1 <= 768
+constexpr int KS_TC = 64;    // ciphertexts per keyswitch tile
+constexpr int KS_TK = 64;    // output columns per keyswitch tile
+constexpr int KS_RG = 16;    // row groups (4 KSK rows each) per LDS chunk
 constexpr int KS_MAX_LEVEL = 16;
+constexpr int KS_MAX_N = 1023;
 
 struct KsArgs {
     int n, N, k, count, ks_level, ks_radix_bits, ks_width_bits;
     const void* acc;                   // Torus[count][k+1][N]
-    const int32_t* const* ksk_table;   // device array: KSK base per key slot
+    const int32_t* const* ksk_table;   // device array: KSK byte planes per key slot
     const int32_t* ct_slot;
     int32_t* lwe_out;                  // [count][n+1]
     int32_t* partial;                  // [count][n+1] (slices > 1)
@@ -138,7 +143,7 @@ __device__ __forceinline__ void ks_digits(int32_t x, int t, int b, int width, in
 // remainder formed exactly in 128-bit integers.
 __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF64& P1, const CrtConst& k) {
     const double y = centre(mulmod(__dadd_rn(c1, -c0), k.inv01, P1), P1);
-    const __int128 C = static_cast<__int128>(k.p0i) * static_cast<int64_t>(y) + static_cast<int64_t>(c0);
+    const __int128 C = static_cast<__int128>(k.p0i) * f64_to_i64(y) + f64_to_i64(c0);
     const double Cd = __fma_rn(k.p0, y, c0);
     const int64_t qt = static_cast<int64_t>(__builtin_rint(__dmul_rn(Cd, k.qinv)));
     __int128 R = C - static_cast<__int128>(qt) * static_cast<__int128>(k.q);

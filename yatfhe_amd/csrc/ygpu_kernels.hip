@@ -93,22 +93,36 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
         // (X^a - 1) acc_r at this lane's pass-0 positions (rotateTorusPolynomialMinusOne,
         // polynomial.cpp:213-238) and its digits, packed 8 or 16 bits per digit.
         uint32_t dg0[16], dg1[16];
+        // negate the rotated source where j < aT (wrap-around), times the
+        // rotator's own sign
+        const bool flip = wrap < 0;
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             const int j = tl + T * r;
-            int src = j - aT;
-            int sgn = wrap;
-            if (src < 0) { src += N; sgn = -wrap; }
-            const int64_t rot = sext_t<C::TBITS>(static_cast<int64_t>(accRow[src]) * sgn - static_cast<int64_t>(accRow[j]));
-            int d4[4] = {0, 0, 0, 0};
-            digits_of<C::TBITS>(rot, L, rb, d4);
-            if constexpr (C::PACK8) {
-                dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFu) | ((static_cast<uint32_t>(d4[1]) & 0xFFu) << 8) |
-                         ((static_cast<uint32_t>(d4[2]) & 0xFFu) << 16) | (static_cast<uint32_t>(d4[3]) << 24);
-                dg1[r] = 0;
+            const int src = (j - aT) & (N - 1);
+            const bool negs = (j < aT) != flip;
+            if constexpr (C::TBITS == 32) {
+                const uint32_t av = static_cast<uint32_t>(accRow[src]);
+                const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(accRow[j]);
+                if constexpr (C::L > 0) {
+                    digits_packed<32, C::L, C::RB>(static_cast<int32_t>(rot), dg0[r], dg1[r]);
+                } else {
+                    int d4[4] = {0, 0, 0, 0};
+                    digits_of<32>(static_cast<int32_t>(rot), L, rb, d4);
+                    dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
+                    dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+                }
             } else {
-                dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
-                dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+                const int64_t av = static_cast<int64_t>(accRow[src]);
+                const int64_t rot = sext_t<C::TBITS>((negs ? -av : av) - static_cast<int64_t>(accRow[j]));
+                if constexpr (C::L > 0) {
+                    digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
+                } else {
+                    int d4[4] = {0, 0, 0, 0};
+                    digits_of<C::TBITS>(rot, L, rb, d4);
+                    dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
+                    dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+                }
             }
         }
 
@@ -140,11 +154,11 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     int dv;
-                    if constexpr (C::PACK8) {
-                        dv = static_cast<int32_t>(dg0[r] << (24 - 8 * lvl)) >> 24;
+                    if constexpr (C::L > 0 && C::RB <= 8) {
+                        dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
                     } else {
                         const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
-                        dv = static_cast<int32_t>(w << (16 - 16 * (lvl & 1))) >> 16;
+                        dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
                     }
                     x[r] = static_cast<double>(dv);
                 }
@@ -174,8 +188,12 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const int j = tl + T * r;
-                    const int64_t val = static_cast<int64_t>(centre(x[r], pr));
-                    accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + val));
+                    const double c = centre(x[r], pr);
+                    if constexpr (C::TBITS == 32) {
+                        accRow[j] = static_cast<TorusT>(static_cast<uint32_t>(accRow[j]) + f64_low32(c));
+                    } else {
+                        accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + f64_to_i64(c)));
+                    }
                 }
             } else {
                 if (np == 0) {
@@ -211,13 +229,16 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
 // switchKeyForTlwe (keyswitching.cpp:48-72) of the sample extracted at index 0
 // (extractTlweFromTrlwe, trlwe.cpp:241-250):
 //   a'_0 = a_0, a'_j = -a_{N-j}, b' = b_0, each modswitched 2^w -> 2^32;
-//   out.a[k] = -sum_{i,t} d_{i,t} KSK[i][t].a[k],  out.b = b' - sum d KSK[i][t].b
+//   out.a[k] = -sum_{i,j} d_{i,j} KSK[i][j].a[k],  out.b = b' - sum d KSK[i][j].b
 // (mod 2^32; the reference's int64 sum + longModP is the same residue).
-// Tile: KS_CT ciphertexts x all n+1 output columns per workgroup, rows of the
-// extracted LWE split into `slices` along grid.y.  Digits of a chunk of rows
-// are built once into LDS and broadcast; each KSK element is loaded once per
-// tile and split into signed 16-bit halves so that every product is a
-// full-rate 24-bit multiply-add: d*x = d*lo + (d*hi << 16) mod 2^32.
+//
+// Shaped as an integer GEMM: digits D[ct][row] (row = i*t + j) times KSK[row][col].
+// The KSK is stored as signed-byte planes: x = sum_b s_b 2^(8b) mod 2^32 with
+// s_b in [-128, 127], four consecutive rows of one plane packed per word, so
+// one v_dot4_i32_i8 performs four exact MACs of one plane.  Tile: KS_TC
+// ciphertexts x KS_TK columns per workgroup, 4 x 4 outputs per thread; row
+// groups stream through LDS in chunks of KS_RG; grid.z slices the rows for
+// small batches (partial sums then meet in an atomic add, exact mod 2^32).
 template <int TBITS>
 __device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base, int N, int j) {
     using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
@@ -228,113 +249,102 @@ __device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base
 
 template <int TBITS>
 __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
-    constexpr int CHUNK = 32;
-    __shared__ int8_t dig[KS_CT][CHUNK][KS_MAX_LEVEL];
+    __shared__ uint32_t sk[KS_RG][4][KS_TK];   // KSK planes: [group][plane][col]
+    __shared__ uint32_t sd[KS_RG][KS_TC];      // packed digits: [group][ct]
     const int n = a.n, N = a.N, t = a.ks_level, cols = n + 1;
-    const int ct0 = static_cast<int>(blockIdx.x) * KS_CT;
-    const int slices = static_cast<int>(gridDim.y);
-    const int rows = a.k * N;
-    const int per = (rows + slices - 1) / slices;
-    const int i0 = static_cast<int>(blockIdx.y) * per;
-    const int i1 = min(rows, i0 + per);
+    const int rows = a.k * N * t;
+    const int groups = (rows + 3) / 4;
+    const int ct0 = static_cast<int>(blockIdx.x) * KS_TC;
+    const int col0 = static_cast<int>(blockIdx.y) * KS_TK;
+    const int slices = static_cast<int>(gridDim.z);
+    const int per = ((groups + slices - 1) / slices + KS_RG - 1) / KS_RG * KS_RG;
+    const int g0 = static_cast<int>(blockIdx.z) * per;
+    const int g1 = min(groups, g0 + per);
     const int tid = static_cast<int>(threadIdx.x);
+    const int tx = tid % 16, ty = tid / 16;   // 4 columns x 4 ciphertexts per thread
 
-    uint32_t accLo[KS_CT][KS_COLS_PER_THREAD], accHi[KS_CT][KS_COLS_PER_THREAD];
-#pragma unroll
-    for (int c = 0; c < KS_CT; c++)
-#pragma unroll
-        for (int m = 0; m < KS_COLS_PER_THREAD; m++) { accLo[c][m] = 0; accHi[c][m] = 0; }
+    // one key per launch (the host launches a key-grouped batch job by job)
+    const uint32_t* __restrict__ planes = reinterpret_cast<const uint32_t*>(a.ksk_table[0]);
 
-    const int32_t* kskBase[KS_CT];
-    size_t accBase[KS_CT];
+    int32_t acc[4][4][4];  // [plane][ct][col]
 #pragma unroll
-    for (int c = 0; c < KS_CT; c++) {
-        const int ct = min(ct0 + c, a.count - 1);
-        const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
-        kskBase[c] = a.ksk_table[slot];
-        accBase[c] = static_cast<size_t>(ct) * (a.k + 1) * N;
-    }
-    // Key-grouped tiles share one KSK; a tile spanning two keys falls back to
-    // per-ciphertext rows.
-    bool oneKey = true;
+    for (int b = 0; b < 4; b++)
 #pragma unroll
-    for (int c = 1; c < KS_CT; c++) oneKey = oneKey && (kskBase[c] == kskBase[0]);
+        for (int c = 0; c < 4; c++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc[b][c][k] = 0;
 
-    for (int base = i0; base < i1; base += CHUNK) {
+    for (int gb = g0; gb < g1; gb += KS_RG) {
         __syncthreads();
-        for (int idx = tid; idx < KS_CT * CHUNK; idx += KS_THREADS) {
-            const int c = idx / CHUNK, ii = idx % CHUNK;
-            const int i = base + ii;
-            int8_t* dd = dig[c][ii];
-            if (i < i1 && ct0 + c < a.count) {
-                const int poly = i / N, j = i % N;
-                const int32_t x = extracted_coeff<TBITS>(a.acc, accBase[c] + static_cast<size_t>(poly) * N, N, j);
-                ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dd);
-            } else {
-                for (int l = 0; l < KS_MAX_LEVEL; l++) dd[l] = 0;
-            }
+        // KSK planes of this chunk (coalesced along columns)
+        for (int e = tid; e < KS_RG * 4 * KS_TK; e += KS_THREADS) {
+            const int k = e % KS_TK, b = (e / KS_TK) % 4, g = e / (4 * KS_TK);
+            const int gg = gb + g, col = col0 + k;
+            sk[g][b][k] = (gg < g1 && col < cols) ? planes[(static_cast<size_t>(gg) * 4 + b) * cols + col] : 0u;
         }
-        __syncthreads();
-        const int iEnd = min(CHUNK, i1 - base);
-        for (int ii = 0; ii < iEnd; ii++) {
-            const int i = base + ii;
-            for (int l = 0; l < t; l++) {
-                const size_t rowOff = (static_cast<size_t>(i) * t + l) * cols;
-                if (oneKey) {
-                    int32_t lo[KS_COLS_PER_THREAD], hi[KS_COLS_PER_THREAD];
-#pragma unroll
-                    for (int m = 0; m < KS_COLS_PER_THREAD; m++) {
-                        const int col = tid + m * KS_THREADS;
-                        const int32_t x = col < cols ? kskBase[0][rowOff + col] : 0;
-                        lo[m] = x & 0xFFFF;
-                        hi[m] = x >> 16;
+        // digits of this chunk: one (ct, group) word per entry; the digits of
+        // one extracted coefficient are computed once for the rows it spans
+        for (int e = tid; e < KS_RG * KS_TC; e += KS_THREADS) {
+            const int c = e % KS_TC, g = e / KS_TC;
+            const int gg = gb + g, ct = ct0 + c;
+            uint32_t w = 0;
+            if (gg < g1 && ct < a.count) {
+                const size_t accBase = static_cast<size_t>(ct) * (a.k + 1) * N;
+                int8_t dig[KS_MAX_LEVEL];
+                int lastI = -1;
+                for (int r = 0; r < 4; r++) {
+                    const int row = gg * 4 + r;
+                    if (row >= rows) break;
+                    const int i = row / t, j = row % t;
+                    if (i != lastI) {
+                        const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N);
+                        ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dig);
+                        lastI = i;
                     }
-#pragma unroll
-                    for (int c = 0; c < KS_CT; c++) {
-                        const int d = dig[c][ii][l];
-#pragma unroll
-                        for (int m = 0; m < KS_COLS_PER_THREAD; m++) {
-                            accLo[c][m] += static_cast<uint32_t>(__mul24(d, lo[m]));
-                            accHi[c][m] += static_cast<uint32_t>(__mul24(d, hi[m]));
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < KS_CT; c++) {
-                        const int d = dig[c][ii][l];
-#pragma unroll
-                        for (int m = 0; m < KS_COLS_PER_THREAD; m++) {
-                            const int col = tid + m * KS_THREADS;
-                            const int32_t x = col < cols ? kskBase[c][rowOff + col] : 0;
-                            accLo[c][m] += static_cast<uint32_t>(__mul24(d, x & 0xFFFF));
-                            accHi[c][m] += static_cast<uint32_t>(__mul24(d, x >> 16));
-                        }
-                    }
+                    w |= (static_cast<uint32_t>(dig[j]) & 0xFFu) << (8 * r);
                 }
             }
+            sd[g][c] = w;
+        }
+        __syncthreads();
+        const int gEnd = min(KS_RG, g1 - gb);
+        for (int g = 0; g < gEnd; g++) {
+            const uint4 dw = *reinterpret_cast<const uint4*>(&sd[g][ty * 4]);
+            const uint32_t d[4] = {dw.x, dw.y, dw.z, dw.w};
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint4 kw = *reinterpret_cast<const uint4*>(&sk[g][b][tx * 4]);
+                const uint32_t kv[4] = {kw.x, kw.y, kw.z, kw.w};
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        acc[b][c][k] = __builtin_amdgcn_sdot4(static_cast<int>(d[c]), static_cast<int>(kv[k]), acc[b][c][k], false);
+            }
         }
     }
 
 #pragma unroll
-    for (int c = 0; c < KS_CT; c++) {
-        const int ct = ct0 + c;
+    for (int c = 0; c < 4; c++) {
+        const int ct = ct0 + ty * 4 + c;
         if (ct >= a.count) continue;
 #pragma unroll
-        for (int m = 0; m < KS_COLS_PER_THREAD; m++) {
-            const int col = tid + m * KS_THREADS;
+        for (int k = 0; k < 4; k++) {
+            const int col = col0 + tx * 4 + k;
             if (col >= cols) continue;
-            const uint32_t s = accLo[c][m] + (accHi[c][m] << 16);
+            const uint32_t sum = static_cast<uint32_t>(acc[0][c][k]) + (static_cast<uint32_t>(acc[1][c][k]) << 8) +
+                                 (static_cast<uint32_t>(acc[2][c][k]) << 16) + (static_cast<uint32_t>(acc[3][c][k]) << 24);
             if (slices == 1) {
                 uint32_t o;
                 if (col < n) {
-                    o = 0u - s;
+                    o = 0u - sum;
                 } else {
-                    const int32_t b = extracted_coeff<TBITS>(a.acc, accBase[c] + static_cast<size_t>(a.k) * N, N, 0);
-                    o = static_cast<uint32_t>(b) - s;
+                    const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0);
+                    o = static_cast<uint32_t>(b) - sum;
                 }
                 a.lwe_out[static_cast<size_t>(ct) * cols + col] = static_cast<int32_t>(o);
             } else {
-                atomicAdd(reinterpret_cast<unsigned int*>(a.partial) + static_cast<size_t>(ct) * cols + col, s);
+                atomicAdd(reinterpret_cast<unsigned int*>(a.partial) + static_cast<size_t>(ct) * cols + col, sum);
             }
         }
     }
@@ -452,6 +462,11 @@ hipError_t launch_br(const BrArgs& a, hipStream_t s) {
 // cpw: ciphertexts per workgroup (1 for the plain form)
 hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, const BrArgs& a, hipStream_t s) {
     const bool p8 = radix_bits <= 8;
+    if (tbits == 32 && logn == 10 && a.level == 3 && radix_bits == 7) {
+        if (cpw >= 4) return launch_br<BrCfg<32, 10, 1, 4, true, 3, 7>>(a, s);
+        return launch_br<BrCfg<32, 10, 1, 1, true, 3, 7>>(a, s);
+    }
+    if (tbits == 56 && logn == 11 && a.level == 3 && radix_bits == 10) return launch_br<BrCfg<56, 11, 2, 1, false, 3, 10>>(a, s);
     if (tbits == 32 && logn == 10) {
         if (cpw >= 4) return p8 ? launch_br<BrCfg<32, 10, 1, 4, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, false>>(a, s);
         return p8 ? launch_br<BrCfg<32, 10, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, false>>(a, s);
@@ -463,7 +478,7 @@ hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, con
 }
 
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s) {
-    const dim3 grid((a.count + KS_CT - 1) / KS_CT, slices);
+    const dim3 grid((a.count + KS_TC - 1) / KS_TC, (a.n + 1 + KS_TK - 1) / KS_TK, slices);
     if (slices > 1) {
         const size_t total = static_cast<size_t>(a.count) * (a.n + 1);
         hipLaunchKernelGGL(k_zero_u32, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,

@@ -93,7 +93,7 @@ int check_params(const ygpu_params& p) {
     if (p.torus_bits != 32 && p.torus_bits != 56) return fail(YGPU_ERR_UNSUPPORTED, "torus_bits must be 32 or 56");
     if (p.k != 1) return fail(YGPU_ERR_UNSUPPORTED, "only GLWE dimension k = 1 is supported");
     if (p.N != 1024 && p.N != 2048) return fail(YGPU_ERR_UNSUPPORTED, "ring degree N must be 1024 or 2048");
-    if (p.n < 1 || p.n + 1 > KS_THREADS * KS_COLS_PER_THREAD) return fail(YGPU_ERR_UNSUPPORTED, "LWE dimension n must be in [1, 767]");
+    if (p.n < 1 || p.n > KS_MAX_N) return fail(YGPU_ERR_UNSUPPORTED, "LWE dimension n must be in [1, 1023]");
     if (p.level < 1 || p.level > 4) return fail(YGPU_ERR_UNSUPPORTED, "gadget level must be in [1, 4]");
     if (p.radix_bits < 1 || p.radix_bits > 16) return fail(YGPU_ERR_UNSUPPORTED, "radix_bits must be in [1, 16]");
     // decomposeRow's own checks (gadget_decomposition.h:194-206)
@@ -247,9 +247,9 @@ KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const
 // small batch still spreads over the device.
 int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
            int32_t* d_ext) {
-    const int tiles = (count + KS_CT - 1) / KS_CT;
+    const int tiles = (count + KS_TC - 1) / KS_TC * ((ctx->p.n + 1 + KS_TK - 1) / KS_TK);
     int slices = 1;
-    while (tiles * slices < 512 && slices < 64) slices *= 2;
+    while (tiles * slices < 256 && slices < 32) slices *= 2;
     if (slices > 1) {
         int rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap,
                         sizeof(int32_t) * static_cast<size_t>(count) * (ctx->p.n + 1));
@@ -432,16 +432,33 @@ int ygpu_key_upload(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk
             }
         });
         if (bad) return fail(YGPU_ERR_INVALID_ARG, "BSK transform failed");
+        // KSK as signed-byte planes: rows grouped by four, word = the four rows'
+        // byte b of the same column; x = sum_b s_b 2^(8b) (mod 2^32), s_b signed
+        const size_t cols = static_cast<size_t>(p.n) + 1;
+        const size_t kskRows = static_cast<size_t>(p.k) * N * p.ks_level;
+        const size_t groups = (kskRows + 3) / 4;
+        std::vector<uint32_t> planes(groups * 4 * cols, 0u);
+        for (size_t row = 0; row < kskRows; row++) {
+            const size_t g = row / 4, r = row % 4;
+            for (size_t c = 0; c < cols; c++) {
+                int64_t x = ksk[row * cols + c];
+                for (int b = 0; b < 4; b++) {
+                    const int8_t sb = static_cast<int8_t>(static_cast<uint8_t>(x & 0xFF));
+                    planes[(g * 4 + b) * cols + c] |= (static_cast<uint32_t>(static_cast<uint8_t>(sb))) << (8 * r);
+                    x = (x - sb) >> 8;
+                }
+            }
+        }
         KeyRec rec;
         rec.live = true;
-        const size_t kskElems = static_cast<size_t>(p.k) * N * p.ks_level * (p.n + 1);
+        const size_t kskElems = planes.size();
         rec.bytes = host.size() * sizeof(double) + kskElems * sizeof(int32_t);
         hipError_t e = hipMalloc(reinterpret_cast<void**>(&rec.d_bsk), host.size() * sizeof(double));
         if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc BSK: ") + hipGetErrorString(e));
         e = hipMalloc(reinterpret_cast<void**>(&rec.d_ksk), kskElems * sizeof(int32_t));
         if (e != hipSuccess) { hipFree(rec.d_bsk); return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc KSK: ") + hipGetErrorString(e)); }
         YG_HIP(hipMemcpy(rec.d_bsk, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice));
-        YG_HIP(hipMemcpy(rec.d_ksk, ksk, kskElems * sizeof(int32_t), hipMemcpyHostToDevice));
+        YG_HIP(hipMemcpy(rec.d_ksk, planes.data(), kskElems * sizeof(int32_t), hipMemcpyHostToDevice));
         int id = -1;
         for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
         if (id < 0) { id = static_cast<int>(ctx->keys.size()); ctx->keys.push_back(rec); }
@@ -527,8 +544,14 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
         BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table, ctx->d_slots);
         rc = run_br(ctx, ctx->cpw_batch, a);
         if (rc) return rc;
-        rc = run_ks(ctx, count, acc, ctx->d_ksk_table, ctx->d_slots, d_lwe_out, nullptr);
-        if (rc) return rc;
+        // keyswitch job by job: each job's rows are contiguous and share a KSK
+        const size_t accRow = ctx->torus_bytes() * (ctx->p.k + 1) * ctx->p.N;
+        for (int j = 0; j < n_jobs; j++) {
+            if (jobs[j].count == 0) continue;
+            rc = run_ks(ctx, jobs[j].count, static_cast<const char*>(acc) + accRow * jobs[j].first,
+                        ctx->d_ksk_table + jobs[j].key_id, nullptr, d_lwe_out + (ctx->p.n + 1) * jobs[j].first, nullptr);
+            if (rc) return rc;
+        }
         // the host slot vector dies with this call: finish the copy first
         YG_HIP(hipStreamSynchronize(ctx->stream));
         return YGPU_OK;
