@@ -9,8 +9,9 @@
 //
 // mulmod(x, w):  h = fl(x*w), l = x*w - h (exact, by FMA), t = rint(h / p),
 //                r = h - t*p (exact, by FMA), return r + l.
-//   x*w - (r + l) = t*p, so the result is congruent to x*w; with |x| <= 8p and
-//   |w| <= p/2 its magnitude is at most 1.25p (derivation in DESIGN.md).
+//   x*w - (r + l) = t*p exactly, so the result is congruent to x*w, and
+//   |r + l| = p |x*w/p - t| <= p (0.5 + 3 * 2^-53 |x*w| / p): with |w| <= p/2
+//   the magnitude is at most p (0.5 + 1.5 |x| 2^-53), e.g. 1.34p for |x| <= 9p.
 // reduce(x):     x - rint(x / p) * p, magnitude <= p/2 + 1.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -162,15 +163,15 @@ __device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl) {
 }
 
 // Forward negacyclic NTT: natural order in pass-0 layout -> bit-reversed order
-// in pass-2 layout.  Values are reduced once before the last pass.
+// in pass-2 layout.  Inputs are digits (|x| <= 2^15); no reduction is needed:
+// a stage adds at most p(0.5 + 1.5|x| 2^-53) to the bound (mulmod above), so
+// after 10 (11) stages every value is below 7.75p (8.98p) < 16p <= 2^53.
 template <int LOGN, int TEAM_WAVES>
 __device__ __forceinline__ void ntt_forward(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr) {
     fwd_pass<LOGN, 0>(x, tl, tw, pr);
     exchange<LOGN, TEAM_WAVES, 0, 1, 0>(x, scr, tl);
     fwd_pass<LOGN, 1>(x, tl, tw, pr);
     exchange<LOGN, TEAM_WAVES, 1, 2, 1>(x, scr, tl);
-#pragma unroll
-    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
     fwd_pass<LOGN, 2>(x, tl, tw, pr);
 }
 
@@ -259,8 +260,9 @@ __device__ __forceinline__ void digits_packed(int64_t x, uint32_t& w0, uint32_t&
         const int window = static_cast<int>((m >> shift) & static_cast<U>(B - 1));
         const int digit = window + carry;
         const int tie = lvl > 0 ? static_cast<int>((m >> (shift + 2 * RB - 1)) & 1) : 0;
-        carry = (digit > HB) | ((digit == HB) & tie);
-        const int bal = digit - (carry << RB);
+        // digit > B/2, or digit == B/2 with the tie bit set
+        carry = (digit + tie) > HB;
+        const int bal = carry ? digit - B : digit;
         const int sd = (bal ^ sneg) - sneg;
         if constexpr (RB <= 8) {
             w0 |= (static_cast<uint32_t>(sd) & 0xFFu) << (8 * lvl);

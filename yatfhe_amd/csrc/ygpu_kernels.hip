@@ -38,7 +38,6 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     constexpr int TWN = C::TW_LDS ? NP * 2 * N : 1;
 
     __shared__ double tws[TWN];
-    __shared__ TorusT accs[CPW][K1 * N];
     __shared__ double scratch[CPW][2 * SCR];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
@@ -51,8 +50,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     // tail slots of the last workgroup repeat its last ciphertext and write nothing
     const bool live = first + lc < a.count;
     const int ct = live ? first + lc : a.count - 1;
-    TorusT* acc = accs[lc];
     double* scr = scratch[lc] + tm * SCR;
+    // the team's accumulator row lives in registers at its pass-0 positions
+    // j = tl + T*r; it is staged through the team's scratch for the rotation
+    TorusT* stage = reinterpret_cast<TorusT*>(scr);
 
     const int n = a.n;
     const int L = a.level;
@@ -67,22 +68,23 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     }
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
     // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
+    TorusT accr[16];
     {
         const int bbar = mod_switch_2n(lwe[n], LOGN + 1);
         int aT, wrap;
         validate_rotator(-bbar, N, aT, wrap);
-        for (int j = ctid; j < N; j += CT_THREADS) {
-            acc[j] = 0;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int j = tl + T * r;
             const int src = j - aT;
             const int64_t val = src < 0 ? -static_cast<int64_t>(v[src + N]) * wrap : static_cast<int64_t>(v[src]) * wrap;
-            acc[N + j] = static_cast<TorusT>(sext_t<C::TBITS>(val));
+            accr[r] = tm == 0 ? TorusT(0) : static_cast<TorusT>(sext_t<C::TBITS>(val));
         }
     }
     __syncthreads();
     const double* twBase = C::TW_LDS ? static_cast<const double*>(tws) : a.tw;
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
-    TorusT* accRow = acc + tm * N;
     for (int i = 0; i < n; i++) {
         // (blind_rotate.cpp:1557-1559 skips a_i == 0; ExtProd(0) = 0 exactly,
         // so the lockstep workgroup simply computes it.)
@@ -93,6 +95,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
         // (X^a - 1) acc_r at this lane's pass-0 positions (rotateTorusPolynomialMinusOne,
         // polynomial.cpp:213-238) and its digits, packed 8 or 16 bits per digit.
         uint32_t dg0[16], dg1[16];
+        team_sync<TW>();
+#pragma unroll
+        for (int r = 0; r < 16; r++) stage[tl + T * r] = accr[r];
+        team_sync<TW>();
         // negate the rotated source where j < aT (wrap-around), times the
         // rotator's own sign
         const bool flip = wrap < 0;
@@ -102,8 +108,8 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
             const int src = (j - aT) & (N - 1);
             const bool negs = (j < aT) != flip;
             if constexpr (C::TBITS == 32) {
-                const uint32_t av = static_cast<uint32_t>(accRow[src]);
-                const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(accRow[j]);
+                const uint32_t av = static_cast<uint32_t>(stage[src]);
+                const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(accr[r]);
                 if constexpr (C::L > 0) {
                     digits_packed<32, C::L, C::RB>(static_cast<int32_t>(rot), dg0[r], dg1[r]);
                 } else {
@@ -113,8 +119,8 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                     dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
                 }
             } else {
-                const int64_t av = static_cast<int64_t>(accRow[src]);
-                const int64_t rot = sext_t<C::TBITS>((negs ? -av : av) - static_cast<int64_t>(accRow[j]));
+                const int64_t av = static_cast<int64_t>(stage[src]);
+                const int64_t rot = sext_t<C::TBITS>((negs ? -av : av) - static_cast<int64_t>(accr[r]));
                 if constexpr (C::L > 0) {
                     digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
                 } else {
@@ -142,14 +148,17 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
 
             for (int lvl = 0; lvl < L; lvl++) {
                 const int d = lvl * K1 + tm;
-                // BSK values of this digit polynomial, both columns, issued
-                // before the transform so their latency overlaps it.
+                // BSK values of this digit polynomial: the own column is issued
+                // before the transform so its latency overlaps it, the other
+                // column after it (registers are the binding resource; the
+                // workgroup's ciphertexts run in lockstep, so these loads hit
+                // in L1 after the first ciphertext of the workgroup)
                 double kbo[16], kbt[16];
                 const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + np) * N + tl;
                 const double* __restrict__ kpo = kp + static_cast<size_t>(tm) * NP * N;
                 const double* __restrict__ kpt = kp + static_cast<size_t>(1 - tm) * NP * N;
 #pragma unroll
-                for (int r = 0; r < 16; r++) { kbo[r] = kpo[r * T]; kbt[r] = kpt[r * T]; }
+                for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
                 double x[16];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
@@ -163,6 +172,8 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                     x[r] = static_cast<double>(dv);
                 }
                 ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+#pragma unroll
+                for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
@@ -187,12 +198,11 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                 // centre mod q, reduce to the torus, accumulate (mod 2^w)
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    const int j = tl + T * r;
                     const double c = centre(x[r], pr);
                     if constexpr (C::TBITS == 32) {
-                        accRow[j] = static_cast<TorusT>(static_cast<uint32_t>(accRow[j]) + f64_low32(c));
+                        accr[r] = static_cast<TorusT>(static_cast<uint32_t>(accr[r]) + f64_low32(c));
                     } else {
-                        accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + f64_to_i64(c)));
+                        accr[r] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accr[r]) + f64_to_i64(c)));
                     }
                 }
             } else {
@@ -202,26 +212,18 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                 } else {
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
-                        const int j = tl + T * r;
                         const int64_t val = crt_mod_q(res[r], centre(x[r], pr), pr, a.crt);
-                        accRow[j] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accRow[j]) + val));
+                        accr[r] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accr[r]) + val));
                     }
                 }
             }
         }
-        // each team reads back, at rotated positions, only the accumulator row
-        // it wrote itself
-        if constexpr (TW == 1) {
-            team_sync<1>();
-        } else {
-            __syncthreads();
-        }
     }
 
-    __syncthreads();
     if (live) {
-        TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N;
-        for (int j = ctid; j < K1 * N; j += CT_THREADS) outAcc[j] = acc[j];
+        TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + tm * N;
+#pragma unroll
+        for (int r = 0; r < 16; r++) outAcc[tl + T * r] = accr[r];
     }
 }
 
