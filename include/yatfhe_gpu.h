@@ -134,6 +134,25 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
 int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in, const void* d_v,
                           int32_t* d_lwe_out, void* d_acc_out);
 
+/* Batched-form tile: ciphertexts per workgroup (cpw) and whether the transform
+ * twiddles live in LDS (twl = 1) or are read through L1 (twl = 0).
+ * ygpu_autotune times every tile instantiated for the context's parameter set
+ * on a synthetic batch of `count` ciphertexts under `key_id` and keeps the
+ * fastest (no counterpart in the reference, which has no tiles). */
+int ygpu_get_tile(ygpu_ctx* ctx, int32_t* cpw, int32_t* twl);
+int ygpu_set_tile(ygpu_ctx* ctx, int32_t cpw, int32_t twl);
+int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, int32_t* twl, double* best_ms);
+
+/* hipGraph of the key-grouped pipeline (blind rotate -> extract -> keyswitch)
+ * over fixed device buffers, captured once and replayed with one launch.
+ * Arguments as ygpu_pbs_jobs_device; the graph owns its scratch (key slots,
+ * keyswitch partial sums, and the accumulators when d_acc_out is NULL).  Keys
+ * used by a graph must stay resident while it exists. */
+int ygpu_graph_create(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64_t total, const int32_t* d_lwe_in,
+                      const void* d_v, int32_t v_per_ct, int32_t* d_lwe_out, void* d_acc_out, int32_t* graph_id);
+int ygpu_graph_launch(ygpu_ctx* ctx, int32_t graph_id);
+int ygpu_graph_destroy(ygpu_ctx* ctx, int32_t graph_id);
+
 /* Device memory and stream helpers. */
 int ygpu_malloc(ygpu_ctx* ctx, size_t bytes, void** d_ptr);
 int ygpu_free(ygpu_ctx* ctx, void* d_ptr);

@@ -130,6 +130,12 @@ def load_library(path: str | None = None):
         "ygpu_timing_get": (c_int, [c_vp, ctypes.c_char_p, P(c_i64), P(c_dbl)]),
         "ygpu_timing_reset": (c_int, [c_vp]),
         "ygpu_modmul_peak": (c_int, [c_vp, c_i32, P(c_dbl)]),
+        "ygpu_get_tile": (c_int, [c_vp, P(c_i32), P(c_i32)]),
+        "ygpu_set_tile": (c_int, [c_vp, c_i32, c_i32]),
+        "ygpu_autotune": (c_int, [c_vp, c_i32, c_i32, P(c_i32), P(c_i32), P(c_dbl)]),
+        "ygpu_graph_create": (c_int, [c_vp, P(_Job), c_i32, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp, P(c_i32)]),
+        "ygpu_graph_launch": (c_int, [c_vp, c_i32]),
+        "ygpu_graph_destroy": (c_int, [c_vp, c_i32]),
         "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
@@ -150,7 +156,8 @@ def exported_symbols():
         "ygpu_extract_keyswitch", "ygpu_pbs_batch_device", "ygpu_pbs_jobs_device", "ygpu_pbs_plain_device",
         "ygpu_malloc", "ygpu_free", "ygpu_memcpy_htod", "ygpu_memcpy_dtoh", "ygpu_sync",
         "ygpu_timing_enable", "ygpu_timing_get", "ygpu_timing_reset", "ygpu_keygen", "ygpu_lwe_encrypt",
-        "ygpu_modmul_peak", "ygpu_copy_device",
+        "ygpu_modmul_peak", "ygpu_copy_device", "ygpu_get_tile", "ygpu_set_tile", "ygpu_autotune",
+        "ygpu_graph_create", "ygpu_graph_launch", "ygpu_graph_destroy",
     ]
 
 
@@ -409,6 +416,34 @@ class Context:
         r = ctypes.c_double(0)
         _check(self.lib.ygpu_modmul_peak(self.h, kind, ctypes.byref(r)))
         return r.value
+
+    def tile(self):
+        c, t = ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(self.lib.ygpu_get_tile(self.h, ctypes.byref(c), ctypes.byref(t)))
+        return c.value, t.value
+
+    def set_tile(self, cpw: int, twl: int):
+        _check(self.lib.ygpu_set_tile(self.h, cpw, twl))
+
+    def autotune(self, key_id: int, count: int):
+        c, t, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_double(0)
+        _check(self.lib.ygpu_autotune(self.h, key_id, count, ctypes.byref(c), ctypes.byref(t), ctypes.byref(ms)))
+        return c.value, t.value, ms.value
+
+    def graph_create(self, jobs, total: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None) -> int:
+        arr = (_Job * max(1, len(jobs)))()
+        for i, (kid, cnt, first) in enumerate(jobs):
+            arr[i] = _Job(kid, cnt, first)
+        gid = ctypes.c_int32(-1)
+        _check(self.lib.ygpu_graph_create(self.h, arr, len(jobs), total, d_lwe.value, d_v.value, 1 if v_per_ct else 0,
+                                          d_out.value, d_acc.value if d_acc else None, ctypes.byref(gid)))
+        return gid.value
+
+    def graph_launch(self, graph_id: int):
+        _check(self.lib.ygpu_graph_launch(self.h, graph_id))
+
+    def graph_destroy(self, graph_id: int):
+        _check(self.lib.ygpu_graph_destroy(self.h, graph_id))
 
     def copy_device(self, d_dst, d_src, nbytes: int):
         _check(self.lib.ygpu_copy_device(self.h, d_dst.value, d_src.value, nbytes))

@@ -15,7 +15,8 @@ namespace ygpu {
 //   CPW    ciphertexts per workgroup
 //   PACK8  digits packed 8 bits each (radix_bits <= 8), else 16 bits
 //   L, RB  gadget levels and base log2 when fixed at compile time (0: runtime)
-template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0>
+//   TWL    twiddles in LDS (1), global memory through L1 (0), or automatic (-1)
+template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0, int TWL_ = -1>
 struct BrCfg {
     static constexpr int L = L_;
     static constexpr int RB = RB_;
@@ -30,7 +31,7 @@ struct BrCfg {
     static constexpr bool PACK8 = PACK8_;
     static constexpr int THREADS = 2 * T * CPW;
     // twiddles in LDS when they fit beside the per-ciphertext state
-    static constexpr bool TW_LDS = NP * 2 * N * 8 <= 16384;
+    static constexpr bool TW_LDS = TWL_ < 0 ? (NP * 2 * N * 8 <= 16384) : (TWL_ == 1);
 };
 
 // Garner reconstruction constants for NP == 2.
@@ -152,7 +153,15 @@ __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF6
     return static_cast<int64_t>(R);
 }
 
-hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, const BrArgs& a, hipStream_t s);
+// Tile choice of the blind-rotation launch: ciphertexts per workgroup and
+// where the transform twiddles live.
+struct BrTile {
+    int cpw = 4;
+    int twl = 1;
+};
+hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& tile, int radix_bits, const BrArgs& a, hipStream_t s);
+// Tiles instantiated for a parameter set (the autotuner's candidates).
+int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* out, int max);
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s);
 hipError_t launch_extract(int tbits, const KsArgs& a, int32_t* out, hipStream_t s);
 hipError_t launch_modmul_peak(int kind, const PrimeF64& pr, uint64_t q, int blocks, int iters, void* sink, hipStream_t s);

@@ -461,22 +461,46 @@ hipError_t launch_br(const BrArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// cpw: ciphertexts per workgroup (1 for the plain form)
-hipError_t launch_blind_rotate(int tbits, int logn, int cpw, int radix_bits, const BrArgs& a, hipStream_t s) {
+// Specialised gadget shapes (the headline parameter sets) come in several
+// tiles; every other parameter set runs a runtime-gadget instantiation.
+hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& t, int radix_bits, const BrArgs& a, hipStream_t s) {
     const bool p8 = radix_bits <= 8;
     if (tbits == 32 && logn == 10 && a.level == 3 && radix_bits == 7) {
-        if (cpw >= 4) return launch_br<BrCfg<32, 10, 1, 4, true, 3, 7>>(a, s);
-        return launch_br<BrCfg<32, 10, 1, 1, true, 3, 7>>(a, s);
+        if (t.cpw >= 4) return t.twl ? launch_br<BrCfg<32, 10, 1, 4, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, true, 3, 7, 0>>(a, s);
+        if (t.cpw == 2) return t.twl ? launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 0>>(a, s);
+        return t.twl ? launch_br<BrCfg<32, 10, 1, 1, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, true, 3, 7, 0>>(a, s);
     }
-    if (tbits == 56 && logn == 11 && a.level == 3 && radix_bits == 10) return launch_br<BrCfg<56, 11, 2, 1, false, 3, 10>>(a, s);
+    if (tbits == 56 && logn == 11 && a.level == 3 && radix_bits == 10) {
+        if (t.cpw >= 2) return launch_br<BrCfg<56, 11, 2, 2, false, 3, 10, 0>>(a, s);
+        return launch_br<BrCfg<56, 11, 2, 1, false, 3, 10, 0>>(a, s);
+    }
     if (tbits == 32 && logn == 10) {
-        if (cpw >= 4) return p8 ? launch_br<BrCfg<32, 10, 1, 4, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, false>>(a, s);
+        if (t.cpw >= 4) return p8 ? launch_br<BrCfg<32, 10, 1, 4, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, false>>(a, s);
         return p8 ? launch_br<BrCfg<32, 10, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, false>>(a, s);
     }
     if (tbits == 32 && logn == 11) return p8 ? launch_br<BrCfg<32, 11, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 11, 1, 1, false>>(a, s);
     if (tbits == 56 && logn == 10) return p8 ? launch_br<BrCfg<56, 10, 2, 1, true>>(a, s) : launch_br<BrCfg<56, 10, 2, 1, false>>(a, s);
     if (tbits == 56 && logn == 11) return p8 ? launch_br<BrCfg<56, 11, 2, 1, true>>(a, s) : launch_br<BrCfg<56, 11, 2, 1, false>>(a, s);
     return hipErrorInvalidValue;
+}
+
+int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* out, int max) {
+    BrTile c[8];
+    int n = 0;
+    if (tbits == 32 && logn == 10 && level == 3 && radix_bits == 7) {
+        for (int cpw : {4, 2, 1})
+            for (int twl : {1, 0}) c[n++] = BrTile{cpw, twl};
+    } else if (tbits == 56 && logn == 11 && level == 3 && radix_bits == 10) {
+        c[n++] = BrTile{1, 0};
+        c[n++] = BrTile{2, 0};
+    } else if (tbits == 32 && logn == 10) {
+        c[n++] = BrTile{4, 1};
+        c[n++] = BrTile{1, 1};
+    } else {
+        c[n++] = BrTile{1, (tbits == 32 && logn == 10) ? 1 : 0};
+    }
+    for (int i = 0; i < n && i < max; i++) out[i] = c[i];
+    return n < max ? n : max;
 }
 
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s) {

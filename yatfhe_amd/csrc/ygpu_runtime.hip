@@ -50,6 +50,24 @@ struct Timing {
     hipEvent_t a, b;
 };
 
+// A captured blind-rotate -> extract -> keyswitch pipeline with its own scratch.
+struct GraphRec {
+    bool live = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* d_acc = nullptr;
+    int32_t* d_partial = nullptr;
+    int32_t* d_slots = nullptr;
+};
+
+void free_graph(GraphRec& g) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    for (void* p : {g.d_acc, static_cast<void*>(g.d_partial), static_cast<void*>(g.d_slots)}) if (p) hipFree(p);
+    g = GraphRec{};
+}
+
+
 }  // namespace
 
 struct ygpu_ctx {
@@ -57,7 +75,9 @@ struct ygpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int logn = 0, np = 1, tbits = 32;
-    int cpw_batch = 4;
+    BrTile tile{};          // batched-form tile (autotuned or default)
+    bool capturing = false;
+    std::vector<GraphRec> graphs;
     uint64_t primes[2] = {0, 0};
     std::vector<HostNtt> ntt;
     double* d_tw = nullptr;
@@ -139,7 +159,7 @@ hipEvent_t take_event(ygpu_ctx* ctx) {
 // Launch `f` bracketed by events on the context stream when timing is on.
 template <class F>
 hipError_t timed(ygpu_ctx* ctx, const char* name, F&& f) {
-    if (!ctx->timing) return f();
+    if (!ctx->timing || ctx->capturing) return f();
     Timing t{name, take_event(ctx), take_event(ctx)};
     hipEventRecord(t.a, ctx->stream);
     hipError_t e = f();
@@ -226,7 +246,8 @@ BrArgs br_args(ygpu_ctx* ctx, int count, const int32_t* d_lwe, const void* d_v, 
     return a;
 }
 
-KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out) {
+KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
+               int32_t* partial) {
     KsArgs a{};
     a.n = ctx->p.n;
     a.N = ctx->p.N;
@@ -239,30 +260,75 @@ KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const
     a.ksk_table = table;
     a.ct_slot = slots;
     a.lwe_out = d_out;
-    a.partial = ctx->d_partial;
+    a.partial = partial;
     return a;
 }
 
-// Extract + keyswitch of `count` accumulators; slices split the KSK rows so a
-// small batch still spreads over the device.
-int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
-           int32_t* d_ext) {
+// Keyswitch row slices for a batch of `count`: enough workgroups to cover the device.
+int ks_slices(const ygpu_ctx* ctx, int count) {
     const int tiles = (count + KS_TC - 1) / KS_TC * ((ctx->p.n + 1 + KS_TK - 1) / KS_TK);
     int slices = 1;
     while (tiles * slices < 256 && slices < 32) slices *= 2;
-    if (slices > 1) {
+    return slices;
+}
+
+// Extract + keyswitch of `count` accumulators.  `partial` (count x (n+1)
+// words) is needed when the rows are sliced; null selects the context's buffer.
+int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
+           int32_t* d_ext, int32_t* partial = nullptr) {
+    const int slices = ks_slices(ctx, count);
+    if (slices > 1 && !partial) {
         int rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap,
                         sizeof(int32_t) * static_cast<size_t>(count) * (ctx->p.n + 1));
         if (rc) return rc;
+        partial = ctx->d_partial;
     }
-    KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out);
+    KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out, partial);
     YG_HIP(timed(ctx, "k_extract_ks", [&] { return launch_extract_ks(ctx->tbits, a, slices, ctx->stream); }));
     if (d_ext) YG_HIP(timed(ctx, "k_extract", [&] { return launch_extract(ctx->tbits, a, d_ext, ctx->stream); }));
     return YGPU_OK;
 }
 
-int run_br(ygpu_ctx* ctx, int cpw, const BrArgs& a) {
-    YG_HIP(timed(ctx, "k_blind_rotate", [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, cpw, ctx->p.radix_bits, a, ctx->stream); }));
+int run_br(ygpu_ctx* ctx, const BrTile& tile, const BrArgs& a) {
+    YG_HIP(timed(ctx, "k_blind_rotate", [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, tile, ctx->p.radix_bits, a, ctx->stream); }));
+    return YGPU_OK;
+}
+
+BrTile plain_tile(const ygpu_ctx* ctx) { return BrTile{1, ctx->tile.twl}; }
+
+// Validated job list -> per-ciphertext key slots.
+int job_slots(const ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int64_t total, std::vector<int32_t>& slots) {
+    slots.assign(static_cast<size_t>(total), -1);
+    for (int j = 0; j < n_jobs; j++) {
+        int rc = key_live(ctx, jobs[j].key_id);
+        if (rc) return rc;
+        if (jobs[j].count < 0 || jobs[j].first < 0 || jobs[j].first + jobs[j].count > total)
+            return fail(YGPU_ERR_INVALID_ARG, "job " + std::to_string(j) + " out of range");
+        for (int64_t c = jobs[j].first; c < jobs[j].first + jobs[j].count; c++) {
+            if (slots[c] != -1) return fail(YGPU_ERR_INVALID_ARG, "ciphertext " + std::to_string(c) + " in two jobs");
+            slots[c] = jobs[j].key_id;
+        }
+    }
+    for (int64_t c = 0; c < total; c++)
+        if (slots[c] < 0) return fail(YGPU_ERR_INVALID_ARG, "ciphertext " + std::to_string(c) + " in no job");
+    return YGPU_OK;
+}
+
+// The key-grouped pipeline on device buffers (eager or under capture).
+int enqueue_jobs(ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int count, const int32_t* d_lwe_in, const void* d_v,
+                 int32_t v_per_ct, int32_t* d_lwe_out, void* acc, const int32_t* d_slots, int32_t* partial) {
+    BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table, d_slots);
+    int rc = run_br(ctx, ctx->tile, a);
+    if (rc) return rc;
+    // keyswitch job by job: each job's rows are contiguous and share a KSK
+    const size_t accRow = ctx->torus_bytes() * (ctx->p.k + 1) * ctx->p.N;
+    for (int j = 0; j < n_jobs; j++) {
+        if (jobs[j].count == 0) continue;
+        rc = run_ks(ctx, jobs[j].count, static_cast<const char*>(acc) + accRow * jobs[j].first, ctx->d_ksk_table + jobs[j].key_id,
+                    nullptr, d_lwe_out + (ctx->p.n + 1) * jobs[j].first, nullptr,
+                    partial ? partial + (ctx->p.n + 1) * jobs[j].first : nullptr);
+        if (rc) return rc;
+    }
     return YGPU_OK;
 }
 
@@ -338,7 +404,8 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         ctx->primes[0] = P_AUX_0;
         ctx->primes[1] = P_AUX_1;
     }
-    if (const char* e = std::getenv("YGPU_CPW")) ctx->cpw_batch = std::max(1, std::atoi(e));
+    ctx->tile = BrTile{params->torus_bits == 32 && params->N == 1024 ? 4 : 1, params->torus_bits == 32 && params->N == 1024 ? 1 : 0};
+    if (const char* e = std::getenv("YGPU_CPW")) ctx->tile.cpw = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const int N = params->N;
@@ -380,6 +447,7 @@ int ygpu_destroy(ygpu_ctx* ctx) {
             if (k.d_bsk) hipFree(k.d_bsk);
             if (k.d_ksk) hipFree(k.d_ksk);
         }
+        for (auto& g : ctx->graphs) if (g.live) free_graph(g);
         for (void* p : {static_cast<void*>(ctx->d_tw), static_cast<void*>(ctx->d_bsk_table), static_cast<void*>(ctx->d_ksk_table),
                         ctx->d_acc, static_cast<void*>(ctx->d_partial), static_cast<void*>(ctx->d_slots),
                         static_cast<void*>(ctx->d_lwe1), ctx->d_v1, static_cast<void*>(ctx->d_out1),
@@ -506,7 +574,7 @@ int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const in
             acc = ctx->d_acc;
         }
         BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, ctx->cpw_batch, a);
+        rc = run_br(ctx, ctx->tile, a);
         if (rc) return rc;
         return run_ks(ctx, count, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, d_extract_out);
     });
@@ -518,20 +586,10 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
     if (total == 0) return YGPU_OK;
     if (!d_lwe_in || !d_v || !d_lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null buffer");
     return with_ctx(ctx, [&]() -> int {
-        std::vector<int32_t> slots(static_cast<size_t>(total), -1);
-        for (int j = 0; j < n_jobs; j++) {
-            int rc = key_live(ctx, jobs[j].key_id);
-            if (rc) return rc;
-            if (jobs[j].count < 0 || jobs[j].first < 0 || jobs[j].first + jobs[j].count > total)
-                return fail(YGPU_ERR_INVALID_ARG, "job " + std::to_string(j) + " out of range");
-            for (int64_t c = jobs[j].first; c < jobs[j].first + jobs[j].count; c++) {
-                if (slots[c] != -1) return fail(YGPU_ERR_INVALID_ARG, "ciphertext " + std::to_string(c) + " in two jobs");
-                slots[c] = jobs[j].key_id;
-            }
-        }
-        for (int64_t c = 0; c < total; c++)
-            if (slots[c] < 0) return fail(YGPU_ERR_INVALID_ARG, "ciphertext " + std::to_string(c) + " in no job");
-        int rc = ensure(reinterpret_cast<void**>(&ctx->d_slots), &ctx->slots_cap, sizeof(int32_t) * slots.size());
+        std::vector<int32_t> slots;
+        int rc = job_slots(ctx, jobs, n_jobs, total, slots);
+        if (rc) return rc;
+        rc = ensure(reinterpret_cast<void**>(&ctx->d_slots), &ctx->slots_cap, sizeof(int32_t) * slots.size());
         if (rc) return rc;
         YG_HIP(hipMemcpyAsync(ctx->d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, ctx->stream));
         void* acc = d_acc_out;
@@ -540,18 +598,10 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
             if (rc) return rc;
             acc = ctx->d_acc;
         }
-        const int count = static_cast<int>(total);
-        BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table, ctx->d_slots);
-        rc = run_br(ctx, ctx->cpw_batch, a);
+        rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap, sizeof(int32_t) * static_cast<size_t>(total) * (ctx->p.n + 1));
         if (rc) return rc;
-        // keyswitch job by job: each job's rows are contiguous and share a KSK
-        const size_t accRow = ctx->torus_bytes() * (ctx->p.k + 1) * ctx->p.N;
-        for (int j = 0; j < n_jobs; j++) {
-            if (jobs[j].count == 0) continue;
-            rc = run_ks(ctx, jobs[j].count, static_cast<const char*>(acc) + accRow * jobs[j].first,
-                        ctx->d_ksk_table + jobs[j].key_id, nullptr, d_lwe_out + (ctx->p.n + 1) * jobs[j].first, nullptr);
-            if (rc) return rc;
-        }
+        rc = enqueue_jobs(ctx, jobs, n_jobs, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc, ctx->d_slots, ctx->d_partial);
+        if (rc) return rc;
         // the host slot vector dies with this call: finish the copy first
         YG_HIP(hipStreamSynchronize(ctx->stream));
         return YGPU_OK;
@@ -568,7 +618,7 @@ int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in
         if (rc) return rc;
         void* acc = d_acc_out ? d_acc_out : ctx->d_acc1;
         BrArgs a = br_args(ctx, 1, d_lwe_in, d_v, 0, acc, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, 1, a);
+        rc = run_br(ctx, plain_tile(ctx), a);
         if (rc) return rc;
         return run_ks(ctx, 1, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, nullptr);
     });
@@ -587,7 +637,7 @@ int ygpu_pbs(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v
         YG_HIP(hipMemcpyAsync(ctx->d_lwe1, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, ctx->stream));
         YG_HIP(hipMemcpyAsync(ctx->d_v1, v, tb * p.N, hipMemcpyHostToDevice, ctx->stream));
         BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, 1, a);
+        rc = run_br(ctx, plain_tile(ctx), a);
         if (rc) return rc;
         rc = run_ks(ctx, 1, ctx->d_acc1, ctx->d_ksk_table + key_id, nullptr, ctx->d_out1, extract_out ? ctx->d_ext1 : nullptr);
         if (rc) return rc;
@@ -611,7 +661,7 @@ int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, cons
         YG_HIP(hipMemcpyAsync(ctx->d_lwe1, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, ctx->stream));
         YG_HIP(hipMemcpyAsync(ctx->d_v1, v, tb * p.N, hipMemcpyHostToDevice, ctx->stream));
         BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, 1, a);
+        rc = run_br(ctx, plain_tile(ctx), a);
         if (rc) return rc;
         YG_HIP(hipMemcpyAsync(acc_out, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
         YG_HIP(hipStreamSynchronize(ctx->stream));
@@ -712,6 +762,147 @@ int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes
     if (!d_dst || !d_src || bytes % 16) return fail(YGPU_ERR_INVALID_ARG, "copy needs device buffers and a multiple of 16 bytes");
     return with_ctx(ctx, [&]() -> int {
         YG_HIP(timed(ctx, "k_copy_u32x4", [&] { return launch_copy(d_src, d_dst, bytes, ctx->stream); }));
+        return YGPU_OK;
+    });
+}
+
+int ygpu_get_tile(ygpu_ctx* ctx, int32_t* cpw, int32_t* twl) {
+    if (!cpw || !twl) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_ctx(ctx, [&]() -> int {
+        *cpw = ctx->tile.cpw;
+        *twl = ctx->tile.twl;
+        return YGPU_OK;
+    });
+}
+
+int ygpu_set_tile(ygpu_ctx* ctx, int32_t cpw, int32_t twl) {
+    return with_ctx(ctx, [&]() -> int {
+        BrTile c[8];
+        const int n = blind_rotate_tiles(ctx->tbits, ctx->logn, ctx->p.level, ctx->p.radix_bits, c, 8);
+        for (int i = 0; i < n; i++)
+            if (c[i].cpw == cpw && c[i].twl == twl) { ctx->tile = c[i]; return YGPU_OK; }
+        return fail(YGPU_ERR_UNSUPPORTED, "tile not instantiated for this parameter set");
+    });
+}
+
+int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, int32_t* twl, double* best_ms) {
+    if (count < 1) return fail(YGPU_ERR_INVALID_ARG, "count must be positive");
+    return with_ctx(ctx, [&]() -> int {
+        int rc = key_live(ctx, key_id);
+        if (rc) return rc;
+        const ygpu_params& p = ctx->p;
+        BrTile cand[8];
+        const int nc = blind_rotate_tiles(ctx->tbits, ctx->logn, p.level, p.radix_bits, cand, 8);
+        // synthetic batch: pseudo-random ciphertexts, a zero test polynomial
+        std::vector<int32_t> lwe(static_cast<size_t>(count) * (p.n + 1));
+        uint64_t st = 0x243F6A8885A308D3ULL;
+        for (auto& x : lwe) x = static_cast<int32_t>(splitmix64(st));
+        void *d_lwe = nullptr, *d_v = nullptr, *d_acc = nullptr;
+        YG_HIP(hipMalloc(&d_lwe, lwe.size() * sizeof(int32_t)));
+        YG_HIP(hipMalloc(&d_v, ctx->torus_bytes() * p.N));
+        YG_HIP(hipMalloc(&d_acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(count)));
+        YG_HIP(hipMemcpy(d_lwe, lwe.data(), lwe.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        YG_HIP(hipMemset(d_v, 0, ctx->torus_bytes() * p.N));
+        hipEvent_t e0, e1;
+        YG_HIP(hipEventCreate(&e0));
+        YG_HIP(hipEventCreate(&e1));
+        double best = 1e30;
+        BrTile bestT = ctx->tile;
+        for (int c = 0; c < nc; c++) {
+            BrArgs a = br_args(ctx, count, static_cast<int32_t*>(d_lwe), d_v, 0, d_acc, ctx->d_bsk_table + key_id, nullptr);
+            YG_HIP(launch_blind_rotate(ctx->tbits, ctx->logn, cand[c], p.radix_bits, a, ctx->stream));  // warm-up
+            double ms = 1e30;
+            for (int rep = 0; rep < 2; rep++) {
+                YG_HIP(hipEventRecord(e0, ctx->stream));
+                YG_HIP(launch_blind_rotate(ctx->tbits, ctx->logn, cand[c], p.radix_bits, a, ctx->stream));
+                YG_HIP(hipEventRecord(e1, ctx->stream));
+                YG_HIP(hipEventSynchronize(e1));
+                float t = 0;
+                YG_HIP(hipEventElapsedTime(&t, e0, e1));
+                ms = std::min(ms, static_cast<double>(t));
+            }
+            if (ms < best) { best = ms; bestT = cand[c]; }
+        }
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        hipFree(d_lwe);
+        hipFree(d_v);
+        hipFree(d_acc);
+        ctx->tile = bestT;
+        if (cpw) *cpw = bestT.cpw;
+        if (twl) *twl = bestT.twl;
+        if (best_ms) *best_ms = best;
+        return YGPU_OK;
+    });
+}
+
+namespace {
+int capture_graph(ygpu_ctx* ctx, GraphRec& g, const std::function<int()>& body, int32_t* graph_id) {
+    YG_HIP(hipStreamSynchronize(ctx->stream));
+    YG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    int rc = body();
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc) { if (graph) hipGraphDestroy(graph); return rc; }
+    if (e != hipSuccess) return fail(YGPU_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    g.graph = graph;
+    YG_HIP(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    g.live = true;
+    int id = -1;
+    for (size_t i = 0; i < ctx->graphs.size(); i++) if (!ctx->graphs[i].live) { id = static_cast<int>(i); break; }
+    if (id < 0) { id = static_cast<int>(ctx->graphs.size()); ctx->graphs.push_back(g); }
+    else ctx->graphs[id] = g;
+    *graph_id = id;
+    return YGPU_OK;
+}
+
+}  // namespace
+
+int ygpu_graph_create(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64_t total, const int32_t* d_lwe_in,
+                      const void* d_v, int32_t v_per_ct, int32_t* d_lwe_out, void* d_acc_out, int32_t* graph_id) {
+    if (!jobs || n_jobs < 1 || total < 1 || total > (1LL << 30) || !d_lwe_in || !d_v || !d_lwe_out || !graph_id)
+        return fail(YGPU_ERR_INVALID_ARG, "bad graph arguments");
+    return with_ctx(ctx, [&]() -> int {
+        std::vector<int32_t> slots;
+        int rc = job_slots(ctx, jobs, n_jobs, total, slots);
+        if (rc) return rc;
+        GraphRec g;
+        const ygpu_params& p = ctx->p;
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_slots), sizeof(int32_t) * slots.size()));
+        YG_HIP(hipMemcpy(g.d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice));
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_partial), sizeof(int32_t) * static_cast<size_t>(total) * (p.n + 1)));
+        void* acc = d_acc_out;
+        if (!acc) {
+            YG_HIP(hipMalloc(&g.d_acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(total)));
+            acc = g.d_acc;
+        }
+        std::vector<ygpu_job> js(jobs, jobs + n_jobs);
+        rc = capture_graph(ctx, g, [&]() {
+            return enqueue_jobs(ctx, js.data(), n_jobs, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc,
+                                g.d_slots, g.d_partial);
+        }, graph_id);
+        if (rc) free_graph(g);
+        return rc;
+    });
+}
+
+int ygpu_graph_launch(ygpu_ctx* ctx, int32_t graph_id) {
+    return with_ctx(ctx, [&]() -> int {
+        if (graph_id < 0 || graph_id >= static_cast<int32_t>(ctx->graphs.size()) || !ctx->graphs[graph_id].live)
+            return fail(YGPU_ERR_NOT_FOUND, "unknown graph id");
+        YG_HIP(timed(ctx, "graph_pbs", [&] { return hipGraphLaunch(ctx->graphs[graph_id].exec, ctx->stream); }));
+        return YGPU_OK;
+    });
+}
+
+int ygpu_graph_destroy(ygpu_ctx* ctx, int32_t graph_id) {
+    return with_ctx(ctx, [&]() -> int {
+        if (graph_id < 0 || graph_id >= static_cast<int32_t>(ctx->graphs.size()) || !ctx->graphs[graph_id].live)
+            return fail(YGPU_ERR_NOT_FOUND, "unknown graph id");
+        YG_HIP(hipStreamSynchronize(ctx->stream));
+        free_graph(ctx->graphs[graph_id]);
         return YGPU_OK;
     });
 }
