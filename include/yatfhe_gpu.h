@@ -109,6 +109,20 @@ int ygpu_pbs(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v
  * (blind_rotate.cpp:1538-1565).  acc_out: Torus[(k+1)N]. */
 int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v, void* acc_out);
 
+/* Blind rotation of an already rescaled input from a caller-supplied
+ * accumulator, host buffers: exactly blindRotateMP21Ntt / blindRotateJP22Ntt
+ * (blind_rotate.cpp:1448-1478, 1538-1565) with ScaledTlwe::a (abar, n values in
+ * [-2N, 2N)) and the in/out Trlwe accum (Torus[(k+1)N]). */
+int ygpu_blind_rotate_scaled(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar, void* acc);
+
+/* Sample extraction at index 0 alone, host buffers: extractTlweFromTrlwe
+ * (trlwe.cpp:241-250).  acc Torus[(k+1)N] -> extract_out int32[kN+1]. */
+int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out);
+
+/* LWE keyswitch of an arbitrary ciphertext of dimension kN, host buffers:
+ * switchKeyForTlwe (keyswitching.cpp:48-72).  lwe_in int32[kN+1] -> lwe_out int32[n+1]. */
+int ygpu_keyswitch(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, int32_t* lwe_out);
+
 /* Sample extract + keyswitch of an accumulator, host buffers:
  * extractTlweFromTrlwe (trlwe.cpp:241-250, index 0) then switchKeyForTlwe
  * (keyswitching.cpp:48-72).  acc: Torus[(k+1)N]; extract_out optional. */

@@ -118,6 +118,9 @@ def load_library(path: str | None = None):
         "ygpu_pbs": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
         "ygpu_blind_rotate": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp]),
         "ygpu_extract_keyswitch": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_blind_rotate_scaled": (c_int, [c_vp, c_i32, c_vp, c_vp]),
+        "ygpu_keyswitch": (c_int, [c_vp, c_i32, c_vp, c_vp]),
+        "ygpu_sample_extract": (c_int, [c_vp, c_vp, c_vp]),
         "ygpu_pbs_batch_device": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
         "ygpu_pbs_jobs_device": (c_int, [c_vp, P(_Job), c_i32, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp]),
         "ygpu_pbs_plain_device": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
@@ -157,7 +160,8 @@ def exported_symbols():
         "ygpu_malloc", "ygpu_free", "ygpu_memcpy_htod", "ygpu_memcpy_dtoh", "ygpu_sync",
         "ygpu_timing_enable", "ygpu_timing_get", "ygpu_timing_reset", "ygpu_keygen", "ygpu_lwe_encrypt",
         "ygpu_modmul_peak", "ygpu_copy_device", "ygpu_get_tile", "ygpu_set_tile", "ygpu_autotune",
-        "ygpu_graph_create", "ygpu_graph_launch", "ygpu_graph_destroy",
+        "ygpu_graph_create", "ygpu_graph_launch", "ygpu_graph_destroy", "ygpu_blind_rotate_scaled", "ygpu_keyswitch",
+        "ygpu_sample_extract",
     ]
 
 
@@ -351,6 +355,24 @@ class Context:
         acc = np.zeros((p.k + 1, p.N), p.torus_dtype)
         _check(self.lib.ygpu_blind_rotate(self.h, key_id, _ptr(lwe), _ptr(vv), _ptr(acc)))
         return acc
+
+    def blind_rotate_scaled(self, key_id: int, abar: np.ndarray, acc: np.ndarray):
+        p = self.params
+        ab = np.ascontiguousarray(abar, dtype=np.int32)
+        a = np.ascontiguousarray(acc, dtype=p.torus_dtype).copy()
+        _check(self.lib.ygpu_blind_rotate_scaled(self.h, key_id, _ptr(ab), _ptr(a)))
+        return a
+
+    def sample_extract(self, acc: np.ndarray):
+        p = self.params
+        out = np.zeros(p.k * p.N + 1, np.int32)
+        _check(self.lib.ygpu_sample_extract(self.h, _ptr(np.ascontiguousarray(acc, dtype=p.torus_dtype)), _ptr(out)))
+        return out
+
+    def keyswitch(self, key_id: int, lwe_in: np.ndarray):
+        out = np.zeros(self.params.n + 1, np.int32)
+        _check(self.lib.ygpu_keyswitch(self.h, key_id, _ptr(np.ascontiguousarray(lwe_in, dtype=np.int32)), _ptr(out)))
+        return out
 
     def extract_keyswitch(self, key_id: int, acc: np.ndarray):
         p = self.params

@@ -2,9 +2,9 @@
 
     python -m yatfhe_amd.build
 
-Produces yatfhe_amd/libyatfhe_gpu.so (HIP kernels + runtime + host keygen),
-yatfhe_amd/libyatfhe_dropin.so (the C++ shim with the reference's signatures)
-and yatfhe_amd/dropin_test (a C++ program exercising the shim).
+Produces yatfhe_amd/libyatfhe_gpu.so (HIP kernels + runtime + host keygen) and
+yatfhe_amd/dropin_test_t32, dropin_test_t56: the reference's bootstrapping tests
+written against the header-only C++ drop-in (csrc/dropin/yatfhe_dropin.h).
 """
 from __future__ import annotations
 
@@ -57,16 +57,14 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, lib)
     dropin = os.path.join(CSRC, "dropin")
-    if os.path.isdir(dropin):
-        shim = os.path.join(HERE, "libyatfhe_dropin.so")
-        _run([CXX, *CXX_FLAGS, f"-I{dropin}", "-shared", "-o", shim + ".tmp", os.path.join(dropin, "yatfhe_dropin.cpp"),
-              f"-L{HERE}", "-lyatfhe_gpu", f"-Wl,-rpath,{HERE}"], verbose)
-        os.replace(shim + ".tmp", shim)
-        test_src = os.path.join(dropin, "dropin_test.cpp")
-        if os.path.exists(test_src):
-            exe = os.path.join(HERE, "dropin_test")
-            _run([CXX, *CXX_FLAGS, f"-I{dropin}", "-o", exe + ".tmp", test_src, f"-L{HERE}", "-lyatfhe_dropin",
-                  "-lyatfhe_gpu", f"-Wl,-rpath,{HERE}"], verbose)
+    test_src = os.path.join(dropin, "dropin_test.cpp")
+    if os.path.exists(test_src):
+        # the header-only drop-in, compiled the way a reference user would: one
+        # program per torus width
+        for width in (32, 56):
+            exe = os.path.join(HERE, f"dropin_test_t{width}")
+            _run([CXX, *CXX_FLAGS, f"-DTORUS{width}", f"-I{dropin}", "-o", exe + ".tmp", test_src, f"-L{HERE}",
+                  "-lyatfhe_gpu", "-Wl,-rpath,$ORIGIN"], verbose)
             os.replace(exe + ".tmp", exe)
     return lib
 

@@ -47,6 +47,8 @@ struct CrtConst {
 
 struct BrArgs {
     int n, level, radix_bits, count;
+    int prescaled;                    // lwe_in already holds abar (mod 2N) values
+    const void* acc_in;               // optional initial accumulators Torus[count][2][N] (replaces v)
     const int32_t* lwe_in;            // [count][n+1]
     const void* v;                    // Torus[N] (v_stride 0) or Torus[count][N]
     int v_stride;

@@ -69,8 +69,13 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
     // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
     TorusT accr[16];
-    {
-        const int bbar = mod_switch_2n(lwe[n], LOGN + 1);
+    if (a.acc_in) {
+        // caller-supplied accumulator (blindRotateMP21Ntt's in/out argument)
+        const TorusT* ai = static_cast<const TorusT*>(a.acc_in) + static_cast<size_t>(ct) * K1 * N + tm * N;
+#pragma unroll
+        for (int r = 0; r < 16; r++) accr[r] = ai[tl + T * r];
+    } else {
+        const int bbar = a.prescaled ? lwe[n] : mod_switch_2n(lwe[n], LOGN + 1);
         int aT, wrap;
         validate_rotator(-bbar, N, aT, wrap);
 #pragma unroll
@@ -88,7 +93,7 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
     for (int i = 0; i < n; i++) {
         // (blind_rotate.cpp:1557-1559 skips a_i == 0; ExtProd(0) = 0 exactly,
         // so the lockstep workgroup simply computes it.)
-        const int abar = mod_switch_2n(lwe[i], LOGN + 1);
+        const int abar = a.prescaled ? lwe[i] : mod_switch_2n(lwe[i], LOGN + 1);
         int aT, wrap;
         validate_rotator(abar, N, aT, wrap);
 

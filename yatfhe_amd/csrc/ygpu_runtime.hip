@@ -233,6 +233,8 @@ BrArgs br_args(ygpu_ctx* ctx, int count, const int32_t* d_lwe, const void* d_v, 
     a.level = ctx->p.level;
     a.radix_bits = ctx->p.radix_bits;
     a.count = count;
+    a.prescaled = 0;
+    a.acc_in = nullptr;
     a.lwe_in = d_lwe;
     a.v = d_v;
     a.v_stride = v_stride;
@@ -667,6 +669,73 @@ int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, cons
         YG_HIP(hipStreamSynchronize(ctx->stream));
         return YGPU_OK;
     });
+}
+
+int ygpu_blind_rotate_scaled(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar, void* acc) {
+    if (!abar || !acc) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_ctx(ctx, [&]() -> int {
+        int rc = key_live(ctx, key_id);
+        if (rc) return rc;
+        rc = plain_buffers(ctx);
+        if (rc) return rc;
+        const ygpu_params& p = ctx->p;
+        const int twoN = 2 * p.N;
+        for (int i = 0; i < p.n; i++)
+            if (abar[i] < -twoN || abar[i] >= twoN) return fail(YGPU_ERR_INVALID_ARG, "abar out of [-2N, 2N)");
+        const size_t tb = ctx->torus_bytes();
+        YG_HIP(hipMemcpyAsync(ctx->d_lwe1, abar, sizeof(int32_t) * p.n, hipMemcpyHostToDevice, ctx->stream));
+        // the initial accumulator rides in the output buffer, read before written per lane
+        void* d_in = nullptr;
+        YG_HIP(hipMalloc(&d_in, tb * (p.k + 1) * p.N));
+        YG_HIP(hipMemcpyAsync(d_in, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
+        BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
+        a.prescaled = 1;
+        a.acc_in = d_in;
+        rc = run_br(ctx, plain_tile(ctx), a);
+        if (rc) { hipFree(d_in); return rc; }
+        YG_HIP(hipMemcpyAsync(acc, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
+        YG_HIP(hipStreamSynchronize(ctx->stream));
+        YG_HIP(hipFree(d_in));
+        return YGPU_OK;
+    });
+}
+
+int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out) {
+    if (!acc || !extract_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_ctx(ctx, [&]() -> int {
+        int rc = plain_buffers(ctx);
+        if (rc) return rc;
+        const ygpu_params& p = ctx->p;
+        const size_t tb = ctx->torus_bytes();
+        YG_HIP(hipMemcpyAsync(ctx->d_acc1, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
+        KsArgs a = ks_args(ctx, 1, ctx->d_acc1, nullptr, nullptr, nullptr, nullptr);
+        YG_HIP(timed(ctx, "k_extract", [&] { return launch_extract(ctx->tbits, a, ctx->d_ext1, ctx->stream); }));
+        YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
+        YG_HIP(hipStreamSynchronize(ctx->stream));
+        return YGPU_OK;
+    });
+}
+
+int ygpu_keyswitch(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, int32_t* lwe_out) {
+    if (!lwe_in || !lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    // Embed the LWE (dimension kN) as the accumulator whose index-0 extraction
+    // it is: acc_c[0] = a_{cN}, acc_c[N-j] = -a_{cN+j}, body[0] = b, scaled by
+    // 2^(w-32) (exact, and undone exactly by the extraction's modswitch).
+    const ygpu_params& p = ctx->p;
+    const int N = p.N, w = p.torus_bits;
+    std::vector<int64_t> acc(static_cast<size_t>(p.k + 1) * N, 0);
+    for (int c = 0; c < p.k; c++) {
+        acc[static_cast<size_t>(c) * N] = static_cast<int64_t>(lwe_in[c * N]) << (w - 32);
+        for (int j = 1; j < N; j++) acc[static_cast<size_t>(c) * N + N - j] = sext(-(static_cast<int64_t>(lwe_in[c * N + j]) << (w - 32)), w);
+    }
+    acc[static_cast<size_t>(p.k) * N] = static_cast<int64_t>(lwe_in[p.k * N]) << (w - 32);
+    if (w == 32) {
+        std::vector<int32_t> a32(acc.size());
+        for (size_t i = 0; i < acc.size(); i++) a32[i] = static_cast<int32_t>(acc[i]);
+        return ygpu_extract_keyswitch(ctx, key_id, a32.data(), nullptr, lwe_out);
+    }
+    return ygpu_extract_keyswitch(ctx, key_id, acc.data(), nullptr, lwe_out);
 }
 
 int ygpu_extract_keyswitch(ygpu_ctx* ctx, int32_t key_id, const void* acc, int32_t* extract_out, int32_t* lwe_out) {
