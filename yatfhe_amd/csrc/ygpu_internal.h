@@ -1,0 +1,444 @@
+// ygpu_internal.h -- runtime state and helpers shared by the C ABI
+// implementation (ygpu_runtime.hip) and the serving engine (ygpu_server.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <cstdio>
+#include <string>
+#include <vector>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <atomic>
+#include <memory>
+#include <functional>
+#include <cstdlib>
+#include "yatfhe_gpu.h"
+#include "ygpu_math.h"
+#include "ygpu_kernels.h"
+
+using namespace ygpu;
+
+
+namespace ygpu_rt {
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define YG_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return fail(YGPU_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct KeyRec {
+    bool live = false;
+    double* d_bsk = nullptr;
+    int32_t* d_ksk = nullptr;
+    size_t bytes = 0;
+};
+
+struct Timing {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+// A captured blind-rotate -> extract -> keyswitch pipeline with its own scratch.
+struct GraphRec {
+    bool live = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void* d_acc = nullptr;
+    int32_t* d_partial = nullptr;
+    int32_t* d_slots = nullptr;
+};
+
+inline void free_graph(GraphRec& g) {
+    if (g.exec) hipGraphExecDestroy(g.exec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    for (void* p : {g.d_acc, static_cast<void*>(g.d_partial), static_cast<void*>(g.d_slots)}) if (p) hipFree(p);
+    g = GraphRec{};
+}
+
+
+}  // namespace ygpu_rt
+
+using namespace ygpu_rt;
+
+struct ygpu_ctx {
+    ygpu_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int logn = 0, np = 1, tbits = 32;
+    BrTile tile{};          // batched-form tile (autotuned or default)
+    bool capturing = false;
+    std::vector<GraphRec> graphs;
+    uint64_t primes[2] = {0, 0};
+    std::vector<HostNtt> ntt;
+    double* d_tw = nullptr;
+    PrimeF64 prime[2]{};
+    CrtConst crt{};
+    std::vector<KeyRec> keys;
+    const double** d_bsk_table = nullptr;
+    const int32_t** d_ksk_table = nullptr;
+    int table_cap = 0;
+    void* d_acc = nullptr;
+    size_t acc_cap = 0;
+    int32_t* d_partial = nullptr;
+    size_t partial_cap = 0;
+    int32_t* d_slots = nullptr;
+    size_t slots_cap = 0;
+    int32_t* d_lwe1 = nullptr;
+    void* d_v1 = nullptr;
+    int32_t* d_out1 = nullptr;
+    int32_t* d_ext1 = nullptr;
+    void* d_acc1 = nullptr;
+    std::mutex mu;
+    bool timing = false;
+    std::vector<Timing> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<int64_t, double>> stats;
+
+    size_t torus_bytes() const { return tbits == 32 ? 4 : 8; }
+};
+
+namespace ygpu_rt {
+
+inline int check_params(const ygpu_params& p) {
+    if (p.torus_bits != 32 && p.torus_bits != 56) return fail(YGPU_ERR_UNSUPPORTED, "torus_bits must be 32 or 56");
+    if (p.k != 1) return fail(YGPU_ERR_UNSUPPORTED, "only GLWE dimension k = 1 is supported");
+    if (p.N != 1024 && p.N != 2048) return fail(YGPU_ERR_UNSUPPORTED, "ring degree N must be 1024 or 2048");
+    if (p.n < 1 || p.n > KS_MAX_N) return fail(YGPU_ERR_UNSUPPORTED, "LWE dimension n must be in [1, 1023]");
+    if (p.level < 1 || p.level > 4) return fail(YGPU_ERR_UNSUPPORTED, "gadget level must be in [1, 4]");
+    if (p.radix_bits < 1 || p.radix_bits > 16) return fail(YGPU_ERR_UNSUPPORTED, "radix_bits must be in [1, 16]");
+    // decomposeRow's own checks (gadget_decomposition.h:194-206)
+    if (p.level * p.radix_bits > p.torus_bits)
+        return fail(YGPU_ERR_INVALID_ARG, "decomposeRow: " + std::to_string(p.level) + " digits of " + std::to_string(p.radix_bits) +
+                                              " bits do not fit the " + std::to_string(p.torus_bits) + "-bit torus");
+    if (p.lwe_q_bits != 32) return fail(YGPU_ERR_UNSUPPORTED, "qLwe must be 2^32");
+    if (p.ks_width_bits < 1 || p.ks_width_bits > 32) return fail(YGPU_ERR_UNSUPPORTED, "ks_width_bits must be in [1, 32]");
+    if (p.ks_radix_bits < 1 || p.ks_radix_bits > 8) return fail(YGPU_ERR_UNSUPPORTED, "ks_radix_bits must be in [1, 8]");
+    if (p.ks_level < 1 || p.ks_level > KS_MAX_LEVEL) return fail(YGPU_ERR_UNSUPPORTED, "ks_level must be in [1, 16]");
+    if (p.ks_level * p.ks_radix_bits > p.ks_width_bits) return fail(YGPU_ERR_INVALID_ARG, "keyswitch digits exceed ks_width_bits");
+    if (!is_prime_u64(p.q_ntt)) return fail(YGPU_ERR_INVALID_ARG, "q_ntt is not prime");
+    if ((p.q_ntt - 1) % (2 * static_cast<uint64_t>(p.N)) != 0) return fail(YGPU_ERR_INVALID_ARG, "q_ntt is not 1 mod 2N");
+    if (p.q_ntt >= (1ULL << 63)) return fail(YGPU_ERR_UNSUPPORTED, "q_ntt must be below 2^63");
+    if (p.torus_bits == 56 && p.q_ntt <= (1ULL << 56)) return fail(YGPU_ERR_INVALID_ARG, "TORUS56 needs q_ntt > 2^56");
+    return YGPU_OK;
+}
+
+template <class F>
+inline int with_ctx(ygpu_ctx* ctx, F&& f) {
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail(YGPU_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    try {
+        return f();
+    } catch (const std::exception& ex) {
+        return fail(YGPU_ERR_INVALID_ARG, ex.what());
+    }
+}
+
+inline hipEvent_t take_event(ygpu_ctx* ctx) {
+    if (!ctx->event_pool.empty()) {
+        hipEvent_t e = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch `f` bracketed by events on the context stream when timing is on.
+template <class F>
+hipError_t timed(ygpu_ctx* ctx, const char* name, F&& f) {
+    if (!ctx->timing || ctx->capturing) return f();
+    Timing t{name, take_event(ctx), take_event(ctx)};
+    hipEventRecord(t.a, ctx->stream);
+    hipError_t e = f();
+    hipEventRecord(t.b, ctx->stream);
+    ctx->pending.push_back(t);
+    return e;
+}
+
+inline int resolve_timing(ygpu_ctx* ctx) {
+    for (auto& t : ctx->pending) {
+        YG_HIP(hipEventSynchronize(t.b));
+        float ms = 0;
+        YG_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+        auto& s = ctx->stats[t.name];
+        s.first += 1;
+        s.second += ms;
+        ctx->event_pool.push_back(t.a);
+        ctx->event_pool.push_back(t.b);
+    }
+    ctx->pending.clear();
+    return YGPU_OK;
+}
+
+inline int ensure(void** buf, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) return YGPU_OK;
+    if (*buf) hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(buf, bytes);
+    if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    *cap = bytes;
+    return YGPU_OK;
+}
+
+inline int upload_tables(ygpu_ctx* ctx) {
+    const int cap = static_cast<int>(ctx->keys.size());
+    if (cap > ctx->table_cap) {
+        if (ctx->d_bsk_table) hipFree(ctx->d_bsk_table);
+        if (ctx->d_ksk_table) hipFree(ctx->d_ksk_table);
+        const int newCap = std::max(16, cap * 2);
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_bsk_table), sizeof(double*) * newCap));
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ksk_table), sizeof(int32_t*) * newCap));
+        ctx->table_cap = newCap;
+    }
+    std::vector<const double*> b(ctx->table_cap, nullptr);
+    std::vector<const int32_t*> k(ctx->table_cap, nullptr);
+    for (size_t i = 0; i < ctx->keys.size(); i++) {
+        b[i] = ctx->keys[i].d_bsk;
+        k[i] = ctx->keys[i].d_ksk;
+    }
+    YG_HIP(hipMemcpy(ctx->d_bsk_table, b.data(), sizeof(double*) * ctx->table_cap, hipMemcpyHostToDevice));
+    YG_HIP(hipMemcpy(ctx->d_ksk_table, k.data(), sizeof(int32_t*) * ctx->table_cap, hipMemcpyHostToDevice));
+    return YGPU_OK;
+}
+
+inline int key_live(const ygpu_ctx* ctx, int32_t key_id) {
+    if (key_id < 0 || key_id >= static_cast<int32_t>(ctx->keys.size()) || !ctx->keys[key_id].live)
+        return fail(YGPU_ERR_NOT_FOUND, "unknown key id " + std::to_string(key_id));
+    return YGPU_OK;
+}
+
+// Device position of transform output index k for lane tl, register r.
+inline int last_pass_index(int logn, int tl, int r) {
+    return logn == 10 ? Pass<10, 2>::idx(tl, r) : Pass<11, 2>::idx(tl, r);
+}
+
+inline BrArgs br_args(ygpu_ctx* ctx, int count, const int32_t* d_lwe, const void* d_v, int v_stride, void* d_acc,
+               const double* const* table, const int32_t* slots) {
+    BrArgs a{};
+    a.n = ctx->p.n;
+    a.level = ctx->p.level;
+    a.radix_bits = ctx->p.radix_bits;
+    a.count = count;
+    a.prescaled = 0;
+    a.acc_in = nullptr;
+    a.lwe_in = d_lwe;
+    a.v = d_v;
+    a.v_stride = v_stride;
+    a.acc_out = d_acc;
+    a.bsk_table = table;
+    a.ct_slot = slots;
+    a.tw = ctx->d_tw;
+    a.prime[0] = ctx->prime[0];
+    a.prime[1] = ctx->prime[1];
+    a.crt = ctx->crt;
+    return a;
+}
+
+inline KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
+               int32_t* partial) {
+    KsArgs a{};
+    a.n = ctx->p.n;
+    a.N = ctx->p.N;
+    a.k = ctx->p.k;
+    a.count = count;
+    a.ks_level = ctx->p.ks_level;
+    a.ks_radix_bits = ctx->p.ks_radix_bits;
+    a.ks_width_bits = ctx->p.ks_width_bits;
+    a.acc = d_acc;
+    a.ksk_table = table;
+    a.ct_slot = slots;
+    a.lwe_out = d_out;
+    a.partial = partial;
+    return a;
+}
+
+// Keyswitch row slices for a batch of `count`: enough workgroups to cover the device.
+inline int ks_slices(const ygpu_ctx* ctx, int count) {
+    const int tiles = (count + KS_TC - 1) / KS_TC * ((ctx->p.n + 1 + KS_TK - 1) / KS_TK);
+    int slices = 1;
+    while (tiles * slices < 256 && slices < 32) slices *= 2;
+    return slices;
+}
+
+// Extract + keyswitch of `count` accumulators.  `partial` (count x (n+1)
+// words) is needed when the rows are sliced; null selects the context's buffer.
+inline int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
+           int32_t* d_ext, int32_t* partial = nullptr, hipStream_t stream = nullptr) {
+    hipStream_t st = stream ? stream : ctx->stream;
+    const int slices = ks_slices(ctx, count);
+    if (slices > 1 && !partial) {
+        int rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap,
+                        sizeof(int32_t) * static_cast<size_t>(count) * (ctx->p.n + 1));
+        if (rc) return rc;
+        partial = ctx->d_partial;
+    }
+    KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out, partial);
+    YG_HIP(timed(ctx, "k_extract_ks", [&] { return launch_extract_ks(ctx->tbits, a, slices, st); }));
+    if (d_ext) YG_HIP(timed(ctx, "k_extract", [&] { return launch_extract(ctx->tbits, a, d_ext, st); }));
+    return YGPU_OK;
+}
+
+inline int run_br(ygpu_ctx* ctx, const BrTile& tile, const BrArgs& a, hipStream_t stream = nullptr) {
+    hipStream_t st = stream ? stream : ctx->stream;
+    YG_HIP(timed(ctx, "k_blind_rotate", [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, tile, ctx->p.radix_bits, a, st); }));
+    return YGPU_OK;
+}
+
+inline BrTile plain_tile(const ygpu_ctx* ctx) { return BrTile{1, ctx->tile.twl}; }
+
+// Validated job list -> per-ciphertext key slots.
+inline int job_slots(const ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int64_t total, std::vector<int32_t>& slots) {
+    slots.assign(static_cast<size_t>(total), -1);
+    for (int j = 0; j < n_jobs; j++) {
+        int rc = key_live(ctx, jobs[j].key_id);
+        if (rc) return rc;
+        if (jobs[j].count < 0 || jobs[j].first < 0 || jobs[j].first + jobs[j].count > total)
+            return fail(YGPU_ERR_INVALID_ARG, "job " + std::to_string(j) + " out of range");
+        for (int64_t c = jobs[j].first; c < jobs[j].first + jobs[j].count; c++) {
+            if (slots[c] != -1) return fail(YGPU_ERR_INVALID_ARG, "ciphertext " + std::to_string(c) + " in two jobs");
+            slots[c] = jobs[j].key_id;
+        }
+    }
+    for (int64_t c = 0; c < total; c++)
+        if (slots[c] < 0) return fail(YGPU_ERR_INVALID_ARG, "ciphertext " + std::to_string(c) + " in no job");
+    return YGPU_OK;
+}
+
+// The key-grouped pipeline on device buffers (eager or under capture).
+inline int enqueue_jobs(ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int count, const int32_t* d_lwe_in, const void* d_v,
+                 int32_t v_per_ct, int32_t* d_lwe_out, void* acc, const int32_t* d_slots, int32_t* partial,
+                 hipStream_t stream = nullptr) {
+    BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table, d_slots);
+    int rc = run_br(ctx, ctx->tile, a, stream);
+    if (rc) return rc;
+    // keyswitch job by job: each job's rows are contiguous and share a KSK
+    const size_t accRow = ctx->torus_bytes() * (ctx->p.k + 1) * ctx->p.N;
+    for (int j = 0; j < n_jobs; j++) {
+        if (jobs[j].count == 0) continue;
+        rc = run_ks(ctx, jobs[j].count, static_cast<const char*>(acc) + accRow * jobs[j].first, ctx->d_ksk_table + jobs[j].key_id,
+                    nullptr, d_lwe_out + (ctx->p.n + 1) * jobs[j].first, nullptr,
+                    partial ? partial + (ctx->p.n + 1) * jobs[j].first : nullptr, stream);
+        if (rc) return rc;
+    }
+    return YGPU_OK;
+}
+
+inline int plain_buffers(ygpu_ctx* ctx) {
+    if (ctx->d_lwe1) return YGPU_OK;
+    const size_t tb = ctx->torus_bytes();
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_lwe1), sizeof(int32_t) * (ctx->p.n + 1)));
+    YG_HIP(hipMalloc(&ctx->d_v1, tb * ctx->p.N));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_out1), sizeof(int32_t) * (ctx->p.n + 1)));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ext1), sizeof(int32_t) * (ctx->p.k * ctx->p.N + 1)));
+    YG_HIP(hipMalloc(&ctx->d_acc1, tb * (ctx->p.k + 1) * ctx->p.N));
+    return YGPU_OK;
+}
+
+inline void parallel_for(int count, int threads, const std::function<void(int)>& fn) {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int i = next.fetch_add(1);
+            if (i >= count) break;
+            fn(i);
+        }
+    };
+    std::vector<std::thread> ts;
+    const int nt = std::max(1, std::min(threads, count));
+    for (int t = 1; t < nt; t++) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
+}
+
+inline int host_threads() {
+    const char* e = std::getenv("YGPU_HOST_THREADS");
+    if (e) return std::max(1, std::atoi(e));
+    const unsigned hc = std::thread::hardware_concurrency();
+    return static_cast<int>(std::max(1u, std::min(hc, 16u)));
+}
+
+// Device-ready key material: the BSK transformed into the engine's NTT order
+// and layout ([i][d][col][np][r][tl] doubles, 1/N folded in) and the KSK as
+// signed-byte planes.  Copying these to the device is all a key load needs.
+struct HostKeyForm {
+    std::vector<double> bsk;
+    std::vector<uint32_t> planes;
+    size_t bytes() const { return bsk.size() * sizeof(double) + planes.size() * sizeof(uint32_t); }
+};
+
+inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, HostKeyForm& out) {
+    const ygpu_params& p = ctx->p;
+    const int N = p.N, K1 = p.k + 1, L = p.level, NP = ctx->np, T = N / 16;
+    const uint64_t q = p.q_ntt;
+    const size_t polys = static_cast<size_t>(p.n) * L * K1 * K1;
+    for (size_t i = 0; i < polys * N; i++)
+        if (bsk_coeff[i] >= q) return fail(YGPU_ERR_INVALID_ARG, "BSK coefficient " + std::to_string(i) + " is not below q_ntt");
+    out.bsk.assign(polys * NP * N, 0.0);
+    parallel_for(static_cast<int>(polys), host_threads(), [&](int pi) {
+        std::vector<uint64_t> buf(N);
+        const uint64_t* src = bsk_coeff + static_cast<size_t>(pi) * N;
+        for (int k = 0; k < NP; k++) {
+            const HostNtt& t = ctx->ntt[k];
+            for (int j = 0; j < N; j++) {
+                // centred integer lift of the residue, reduced mod the transform prime
+                const int64_t c = src[j] > (q - 1) / 2 ? static_cast<int64_t>(src[j]) - static_cast<int64_t>(q) : static_cast<int64_t>(src[j]);
+                int64_t m = c % static_cast<int64_t>(t.p);
+                if (m < 0) m += static_cast<int64_t>(t.p);
+                buf[j] = static_cast<uint64_t>(m);
+            }
+            t.forward(buf.data());
+            double* dst = &out.bsk[(static_cast<size_t>(pi) * NP + k) * N];
+            for (int r = 0; r < 16; r++)
+                for (int tl = 0; tl < T; tl++)
+                    dst[r * T + tl] = centred_double(mulmod(buf[last_pass_index(ctx->logn, tl, r)], t.ninv, t.p), t.p);
+        }
+    });
+    // KSK as signed-byte planes: rows grouped by four, word = the four rows'
+    // byte b of the same column; x = sum_b s_b 2^(8b) (mod 2^32), s_b signed
+    const size_t cols = static_cast<size_t>(p.n) + 1;
+    const size_t kskRows = static_cast<size_t>(p.k) * N * p.ks_level;
+    const size_t groups = (kskRows + 3) / 4;
+    out.planes.assign(groups * 4 * cols, 0u);
+    for (size_t row = 0; row < kskRows; row++) {
+        const size_t g = row / 4, r = row % 4;
+        for (size_t c = 0; c < cols; c++) {
+            int64_t x = ksk[row * cols + c];
+            for (int b = 0; b < 4; b++) {
+                const int8_t sb = static_cast<int8_t>(static_cast<uint8_t>(x & 0xFF));
+                out.planes[(g * 4 + b) * cols + c] |= (static_cast<uint32_t>(static_cast<uint8_t>(sb))) << (8 * r);
+                x = (x - sb) >> 8;
+            }
+        }
+    }
+    return YGPU_OK;
+}
+
+// Bytes a key occupies on the device, without building it.
+inline size_t key_device_bytes(const ygpu_ctx* ctx) {
+    const ygpu_params& p = ctx->p;
+    const size_t bsk = static_cast<size_t>(p.n) * p.level * (p.k + 1) * (p.k + 1) * ctx->np * p.N * sizeof(double);
+    const size_t rows = static_cast<size_t>(p.k) * p.N * p.ks_level;
+    return bsk + (rows + 3) / 4 * 4 * (static_cast<size_t>(p.n) + 1) * sizeof(uint32_t);
+}
+
+// Grows the device key tables to at least `slots` entries (synchronous; no
+// kernel may be using the tables).
+inline int reserve_slots(ygpu_ctx* ctx, int slots) {
+    if (static_cast<int>(ctx->keys.size()) < slots) ctx->keys.resize(slots);
+    return upload_tables(ctx);
+}
+
+}  // namespace ygpu_rt
