@@ -1,16 +1,24 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched programmable bootstrap on one MI355X.
+"""Benchmark of the programmable bootstrap (PBS) on one MI355X.
 
 A step is one batched PBS (blind rotate -> sample extract -> keyswitch) of
 4096 ciphertexts under one key at TORUS32 n=630, k=1, N=1024, gadget 2^7 x 3,
-keyswitch 2^2 x 8, with inputs resident in device memory.  Prints one JSON line.
+keyswitch 2^2 x 8, with inputs resident in device memory; `value` is its
+throughput.  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs a,b,...]
 
-Besides the batched value the line carries: the plain one-ciphertext-per-call
-form, the roofline of the blind-rotation kernel against the measured modmul
-peak of this GPU, and a CPU baseline (the oracle restating the reference's
-algorithm, timed on a bounded sample on the host cores of one CCD).
+Besides the batched value the line carries
+  - the plain one-ciphertext-per-call form and its latency percentiles,
+  - the roofline of the blind-rotation kernel against the measured modmul peak
+    of this GPU (and, for the plain form, against HBM bandwidth),
+  - `configs`: TORUS56 n=742 N=2048 (8192 ciphertexts, random 16-entry LUT,
+    autotuned tile); 64 tenants with Zipf(1.1) ciphertext counts in one
+    key-grouped hipGraph launch; and the serving loop (Poisson arrivals swept
+    1k..50k PBS/s over 256 Zipf(1.1) tenants, 70 % TORUS32 / 30 % TORUS56,
+    keys under a 12 GB device budget with pinned-host spill) with p50/p99,
+  - a CPU baseline: the oracle restating the reference's algorithm, timed on a
+    bounded sample on the host cores of one CCD.
 """
 from __future__ import annotations
 
@@ -28,7 +36,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PBS/sec at N=1024 bit-exact, batched vs plain on GPU; p50/p99 serving latency"
 BATCH = 4096
+BATCH_T56 = 8192
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s (public spec)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_blind_rotate_t32_n1024_b4096_traffic.txt")
+SERVING_RATES = [1000, 2000, 5000, 10000, 20000, 30000, 40000, 50000]
+LEGS = ("config1", "config2", "config3", "serving")
 
 
 def modmul_per_pbs(p) -> int:
@@ -44,6 +56,10 @@ def bsk_bytes(p) -> int:
     return p.n * p.level * (p.k + 1) ** 2 * p.N * 8
 
 
+def ksk_bytes(p) -> int:
+    return p.k * p.N * p.ks_level * (p.n + 1) * 4
+
+
 def read_traffic():
     """HBM bytes per k_blind_rotate launch from the committed rocprof summary."""
     if not os.path.exists(PROFILE_TRAFFIC):
@@ -55,6 +71,13 @@ def read_traffic():
             except ValueError:
                 return None
     return None
+
+
+def pct(v, p):
+    """Nearest-rank percentile (ya_benchmark/blindrotate_serving.cpp:70-73)."""
+    v = sorted(v)
+    i = int(math.ceil(p / 100.0 * len(v))) - 1
+    return v[min(max(i, 0), len(v) - 1)]
 
 
 def ccd_cpus():
@@ -78,14 +101,24 @@ def cpu_baseline(params, keys, cts, v):
     from tests.oracle_py import Oracle
 
     cpus = ccd_cpus()[:16]
+    saved = os.sched_getaffinity(0)
     os.sched_setaffinity(0, set(cpus))
-    orc = Oracle(params, keys)
-    sample = 16 * len(cpus)
-    orc.pbs_batch(cts[: len(cpus)], v, threads=len(cpus))  # warm
-    t0 = time.perf_counter()
-    orc.pbs_batch(cts[:sample], v, threads=len(cpus))
-    dt = time.perf_counter() - t0
-    orc.close()
+    try:
+        orc = Oracle(params, keys)
+        sample = 16 * len(cpus)
+        orc.pbs_batch(cts[: len(cpus)], v, threads=len(cpus))  # warm
+        t0 = time.perf_counter()
+        orc.pbs_batch(cts[:sample], v, threads=len(cpus))
+        dt = time.perf_counter() - t0
+        # single-ciphertext latency, one thread
+        lat = []
+        for c in range(3):
+            t = time.perf_counter()
+            orc.pbs(cts[c], v)
+            lat.append(time.perf_counter() - t)
+        orc.close()
+    finally:
+        os.sched_setaffinity(0, saved)
     model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -102,33 +135,27 @@ def cpu_baseline(params, keys, cts, v):
         "sample": f"{sample} ciphertexts of the same workload, one per thread at a time, threads pinned to CPUs {cpus[0]}-{cpus[-1]} "
                   f"(one L3 domain) of {model or 'the host'}; the oracle's own scalar NTT, not Intel HEXL",
         "seconds": round(dt, 3),
+        "latency_ms_one_thread": round(min(lat) * 1e3, 1),
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--plain-calls", type=int, default=20)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "
-              f"(--gpus {args.gpus} requested). Tenant-affine sharding across GPUs is a design note only.",
-              file=sys.stderr)
-        sys.exit(2)
+def sign_inputs(yg, keys, count, seed):
+    bits = np.random.default_rng(seed).integers(0, 2, count)
+    return bits, yg.lwe_encrypt(keys.lwe_key, ((2 * bits - 1) * (1 << 29)).astype(np.int32), seed=seed)
 
-    import yatfhe_amd as yg
 
+def lut_inputs(yg, keys, count, seed, p=16):
+    msgs = np.random.default_rng(seed).integers(0, p, count)
+    return msgs, yg.lwe_encrypt(keys.lwe_key, (msgs * (1 << 32) // (2 * p)).astype(np.int64).astype(np.int32), seed=seed)
+
+
+# --------------------------------------------------------------------- config 1
+def leg_config1(yg, args, peaks):
     p = yg.PARAMS_T32_N1024
     keys = yg.keygen(p, seed=0)
-    rng = np.random.default_rng(1)
-    bits = rng.integers(0, 2, BATCH)
-    mu = ((2 * bits - 1) * (1 << 29)).astype(np.int32)
-    cts = yg.lwe_encrypt(keys.lwe_key, mu, seed=1)
+    bits, cts = sign_inputs(yg, keys, BATCH, 1)
     v = yg.sign_test_polynomial(p)
-
+    res = {}
     with yg.Context(p, 0) as ctx:
         kid = ctx.upload_key(keys)
         d_lwe = ctx.alloc(cts.nbytes)
@@ -136,9 +163,8 @@ def main():
         d_v = ctx.alloc(v.nbytes)
         d_v.upload(v)
         d_out = ctx.alloc(BATCH * (p.n + 1) * 4)
-
-        peak_f64 = ctx.modmul_peak(0)
-        peak_u64 = ctx.modmul_peak(1)
+        peaks["f64"] = ctx.modmul_peak(0)
+        peaks["u64"] = ctx.modmul_peak(1)
 
         for _ in range(args.warmup):
             ctx.pbs_batch_device(kid, BATCH, d_lwe, d_v, False, d_out)
@@ -154,12 +180,11 @@ def main():
         br_n, br_ms = ctx.timing_get("k_blind_rotate")
         ks_n, ks_ms = ctx.timing_get("k_extract_ks")
         ctx.timing(False)
-
         out = d_out.download(np.zeros((BATCH, p.n + 1), np.int32))
         ph = yg.lwe_decrypt_phase(out, keys.lwe_key)
-        decrypt_ok = bool(np.array_equal((ph > 0).astype(int), bits))
+        res["decrypt_ok"] = bool(np.array_equal((ph > 0).astype(int), bits))
 
-        # plain drop-in form: one ciphertext per call, the per-call persistent kernel
+        # plain drop-in form: one ciphertext per call, the per-call kernel
         d1 = ctx.alloc((p.n + 1) * 4)
         d1.upload(cts[0])
         o1 = ctx.alloc((p.n + 1) * 4)
@@ -172,9 +197,6 @@ def main():
             ctx.pbs_plain_device(kid, d1, d_v, o1)
             ctx.sync()
             lat.append(time.perf_counter() - t)
-        lat.sort()
-        plain_rate = len(lat) / sum(lat)
-
         # calibration copy (for the HBM counter runs of the same command)
         cbytes = 256 << 20
         ca = ctx.alloc(cbytes)
@@ -183,13 +205,247 @@ def main():
         ctx.sync()
         for b in (ca, cb, d1, o1, d_out, d_v, d_lwe):
             b.free()
+    res.update(value=args.steps * BATCH / dt, dt=dt, br_avg_s=br_ms / max(1, br_n) / 1e3,
+               ks_avg_ms=ks_ms / max(1, ks_n), plain_lat=lat)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(p, keys, cts, v)
+    return p, res
 
-    value = args.steps * BATCH / dt
-    br_avg_s = br_ms / max(1, br_n) / 1e3
+
+# --------------------------------------------------------------------- config 2
+def leg_config2(yg, args, peak):
+    p = yg.PARAMS_T56_N2048
+    keys = yg.keygen(p, seed=0)
+    lut = np.random.default_rng(2).integers(0, 16, 16)
+    v = yg.lut_test_polynomial(p, lut, 16)
+    msgs, cts = lut_inputs(yg, keys, BATCH_T56, 3)
+    with yg.Context(p, 0) as ctx:
+        kid = ctx.upload_key(keys)
+        cpw, twl, tune_ms = ctx.autotune(kid, BATCH_T56)
+        d_lwe = ctx.alloc(cts.nbytes)
+        d_lwe.upload(cts)
+        d_v = ctx.alloc(v.nbytes)
+        d_v.upload(v)
+        d_out = ctx.alloc(BATCH_T56 * (p.n + 1) * 4)
+        ctx.pbs_batch_device(kid, BATCH_T56, d_lwe, d_v, False, d_out)
+        ctx.sync()
+        ctx.timing(True)
+        ctx.timing_reset()
+        steps = max(1, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.pbs_batch_device(kid, BATCH_T56, d_lwe, d_v, False, d_out)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        br_n, br_ms = ctx.timing_get("k_blind_rotate")
+        ctx.timing(False)
+        out = d_out.download(np.zeros((BATCH_T56, p.n + 1), np.int32))
+        for b in (d_out, d_v, d_lwe):
+            b.free()
+    ph = yg.lwe_decrypt_phase(out, keys.lwe_key).astype(np.int64)
+    ok = bool(np.array_equal(np.round(ph / (1 << 32) * 16).astype(np.int64) % 16, lut[msgs] % 16))
+    value = steps * BATCH_T56 / dt
+    br_s = br_ms / max(1, br_n) / 1e3
     mm = modmul_per_pbs(p)
-    peak = max(peak_f64, peak_u64)
-    achieved = mm * BATCH / br_avg_s
-    traffic = read_traffic()
+    return {
+        "workload": "TORUS56 n=742 k=1 N=2048 gadget 2^10x3 KS 2^2x8, 8192 ciphertexts of uniform 4-bit messages "
+                    "(seed 3), random 16-entry LUT (seed 2), one key (seed 0)",
+        "value": round(value, 1), "unit": "PBS/s", "steps": steps, "ms_per_step": round(dt / steps * 1e3, 2),
+        "tile": {"cpw": cpw, "twl": twl, "autotune_best_ms": round(tune_ms, 2)},
+        "roofline": {"bound": "valu-modmul", "achieved": round(mm * BATCH_T56 / br_s / 1e9, 2), "peak": round(peak / 1e9, 2),
+                     "unit": "Gmodmul/s", "frac": round(mm * BATCH_T56 / br_s / peak, 4),
+                     "per_unit": f"{mm} modmul (mod q56) per PBS; each is two binary64 products, one per CRT prime",
+                     "avg_launch_ms": round(br_s * 1e3, 2)},
+        "lut_decrypt_check": ok,
+    }
+
+
+# --------------------------------------------------------------------- config 3
+def config3_counts():
+    """c_t = clamp(round(12288 r_t^-1.1 / H_64), 1, 512), ranks from a seed-4 permutation."""
+    r = np.random.default_rng(4).permutation(64) + 1
+    h = sum(k ** -1.1 for k in range(1, 65))
+    return np.clip(np.round(12288 * r.astype(np.float64) ** -1.1 / h), 1, 512).astype(int)
+
+
+def leg_config3(yg, args, peak):
+    p = yg.PARAMS_T32_N1024
+    counts = config3_counts()
+    total = int(counts.sum())
+    v = yg.sign_test_polynomial(p)
+    with yg.Context(p, 0) as ctx:
+        kids, cts, bits, lwe_keys = [], [], [], []
+        for t in range(64):
+            k = yg.keygen(p, seed=1000 + t)
+            kids.append(ctx.upload_key(k))
+            b, c = sign_inputs(yg, k, int(counts[t]), 2000 + t)
+            bits.append(b)
+            cts.append(c)
+            lwe_keys.append(k.lwe_key)
+            del k
+        key_bytes = sum(ctx.key_bytes(k) for k in kids)
+        all_cts = np.concatenate(cts)
+        jobs, first = [], 0
+        for t in range(64):
+            jobs.append((kids[t], int(counts[t]), first))
+            first += int(counts[t])
+        d_lwe = ctx.alloc(all_cts.nbytes)
+        d_lwe.upload(all_cts)
+        d_v = ctx.alloc(v.nbytes)
+        d_v.upload(v)
+        d_out = ctx.alloc(total * (p.n + 1) * 4)
+        d_eager = ctx.alloc(total * (p.n + 1) * 4)
+        gid = ctx.graph_create(jobs, total, d_lwe, d_v, False, d_out)
+        ctx.graph_launch(gid)
+        ctx.sync()
+        steps = max(1, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.graph_launch(gid)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        ctx.pbs_jobs_device(jobs, total, d_lwe, d_v, False, d_eager)
+        ctx.sync()
+        out = d_out.download(np.zeros((total, p.n + 1), np.int32))
+        eager = d_eager.download(np.zeros((total, p.n + 1), np.int32))
+        ctx.graph_destroy(gid)
+        for b in (d_lwe, d_v, d_out, d_eager):
+            b.free()
+    ok = True
+    for t in range(64):
+        s, e = jobs[t][2], jobs[t][2] + jobs[t][1]
+        ph = yg.lwe_decrypt_phase(out[s:e], lwe_keys[t])
+        ok &= bool(np.array_equal((ph > 0).astype(int), bits[t]))
+    value = steps * total / dt
+    return {
+        "workload": f"64 tenants (keys seed 1000+t) at TORUS32 n=630 N=1024, Zipf(1.1) counts in 1..512 "
+                    f"(total {total}, max {int(counts.max())}, min {int(counts.min())}), one key-grouped launch, "
+                    f"hipGraph-captured blind rotate -> extract -> keyswitch",
+        "value": round(value, 1), "unit": "PBS/s", "steps": steps, "ms_per_step": round(dt / steps * 1e3, 2),
+        "keys_resident_bytes": int(key_bytes),
+        "roofline": {"bound": "valu-modmul", "frac_whole_pipeline": round(modmul_per_pbs(p) * value / peak, 4)},
+        "graph_equals_eager": bool(np.array_equal(out, eager)),
+        "decrypt_check": ok,
+    }
+
+
+# ---------------------------------------------------------------------- serving
+def leg_serving(yg, args, peak):
+    p32, p56 = yg.PARAMS_T32_N1024, yg.PARAMS_T56_N2048
+    n_t = 256
+    perm = np.random.default_rng(5).permutation(n_t)
+    is56 = perm < int(round(0.3 * n_t))          # 30 % of tenants at TORUS56
+    mats32, mats56 = 8, 4                         # distinct key materials per width
+    budget = int(args.serving_budget_gb * (1 << 30))
+    lut = np.random.default_rng(6).integers(0, 16, 16)
+    v32 = yg.sign_test_polynomial(p32)
+    v56 = yg.lut_test_polynomial(p56, lut, 16)
+    keys32 = [yg.keygen(p32, seed=3000 + m) for m in range(mats32)]
+    keys56 = [yg.keygen(p56, seed=4000 + m) for m in range(mats56)]
+    in32 = [sign_inputs(yg, k, 1, 5000 + m)[1][0] for m, k in enumerate(keys32)]
+    in56 = [lut_inputs(yg, k, 1, 6000 + m)[1][0] for m, k in enumerate(keys56)]
+    # expected results from the single-context path
+    exp32, exp56 = [], []
+    with yg.Context(p32, 0) as ctx:
+        for k, c in zip(keys32, in32):
+            kid = ctx.upload_key(k)
+            exp32.append(ctx.pbs(kid, c, v32)[0])
+            ctx.release_key(kid)
+    with yg.Context(p56, 0) as ctx:
+        for k, c in zip(keys56, in56):
+            kid = ctx.upload_key(k)
+            exp56.append(ctx.pbs(kid, c, v56)[0])
+            ctx.release_key(kid)
+    sweep = []
+    with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16) as srv:
+        ps32 = srv.add_params(p32)
+        ps56 = srv.add_params(p56)
+        # tenant id == popularity rank (the load generator draws rank r as tenant r);
+        # materials are assigned round-robin within each width, registered at first use
+        first = {}
+        inputs, luts, expected = [], [], []
+        c32 = c56 = 0
+        for t in range(n_t):
+            if is56[t]:
+                w, m = "56", c56 % mats56
+                c56 += 1
+            else:
+                w, m = "32", c32 % mats32
+                c32 += 1
+            if (w, m) not in first:
+                tid = srv.add_tenant(ps56 if w == "56" else ps32, keys56[m] if w == "56" else keys32[m])
+                first[(w, m)] = tid
+            else:
+                tid = srv.add_tenant_shared(first[(w, m)])
+            assert tid == t
+            inputs.append(in56[m] if w == "56" else in32[m])
+            luts.append(v56 if w == "56" else v32)
+            expected.append(exp56[m] if w == "56" else exp32[m])
+        del keys32, keys56
+        share56 = sum((1.0 / (t + 1) ** 1.1) for t in range(n_t) if is56[t]) / sum(1.0 / (t + 1) ** 1.1 for t in range(n_t))
+        for rate in args.serving_rates:
+            n_req = int(max(500, rate * args.serving_seconds))
+            r = srv.loadgen(rate=rate, requests=n_req, inputs=inputs, luts=luts, expected=expected,
+                            warm=256 if not sweep else 0, zipf_s=1.1, seed=5, slo_ms=100.0)
+            sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()})
+            print(f"serving rate {rate}: {r['throughput_rps']:.0f} PBS/s p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms "
+                  f"shed {r['shed']} mismatches {r['mismatches']}", file=sys.stderr, flush=True)
+        st = srv.stats()
+    m32, m56 = modmul_per_pbs(p32), modmul_per_pbs(p56)
+    mm_avg = (1 - share56) * m32 + share56 * m56
+    ok = [s for s in sweep if s["shed"] == 0 and s["errors"] == 0]
+    best = max(ok, key=lambda s: s["throughput_rps"]) if ok else None
+    return {
+        "workload": f"Poisson arrivals over 256 tenants, Zipf(1.1) popularity (seed 5), 30 % of tenants TORUS56 "
+                    f"n=742 N=2048 with a 16-entry LUT, 70 % TORUS32 n=630 N=1024 sign gate (TORUS56 share of "
+                    f"traffic {share56:.3f}); {mats32}+{mats56} distinct key materials shared round-robin by the "
+                    f"tenants, each tenant cached on the device separately; device key budget "
+                    f"{args.serving_budget_gb} GiB against a key pool of "
+                    f"{(int((~is56).sum()) * 82.6e6 + int(is56.sum()) * 340.5e6) / 1e9:.1f} GB; full PBS per request",
+        "sweep": sweep,
+        "max_sustained": None if best is None else {
+            "offered_rps": best["offered_rps"], "throughput_rps": best["throughput_rps"],
+            "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+            "roofline_frac": round(best["throughput_rps"] * mm_avg / peak, 4)},
+        "all_results_bit_exact": all(s["mismatches"] == 0 for s in sweep),
+        "server_stats": st,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--plain-calls", type=int, default=50)
+    ap.add_argument("--legs", default=",".join(LEGS), help="comma-separated subset of " + ",".join(LEGS))
+    ap.add_argument("--serving-seconds", type=float, default=2.0)
+    ap.add_argument("--serving-budget-gb", type=float, default=12.0)
+    ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "
+              f"(--gpus {args.gpus} requested). Tenant-affine sharding across GPUs is a design note only.",
+              file=sys.stderr)
+        sys.exit(2)
+    legs = [l for l in args.legs.split(",") if l]
+    for l in legs:
+        if l not in LEGS:
+            sys.exit(f"unknown leg {l}")
+    args.serving_rates = [float(r) for r in args.serving_rates.split(",")]
+
+    import yatfhe_amd as yg
+
+    peaks = {}
+    p, r1 = leg_config1(yg, args, peaks)
+    peak = max(peaks["f64"], peaks["u64"])
+    value = r1["value"]
+    mm = modmul_per_pbs(p)
+    achieved = mm * BATCH / r1["br_avg_s"]
+    lat = r1["plain_lat"]
+    plain_rate = len(lat) / sum(lat)
+    plain_bytes = bsk_bytes(p) + ksk_bytes(p)
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -197,7 +453,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "ms_per_step": round(r1["dt"] / args.steps * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -215,22 +471,34 @@ def main():
             "peak": round(peak / 1e9, 2),
             "unit": "Gmodmul/s",
             "frac": round(achieved / peak, 4),
-            "traffic": traffic,
+            "traffic": read_traffic(),
             "kernel": "k_blind_rotate",
             "per_unit": f"{mm} modmul per PBS x {BATCH} PBS per launch",
-            "avg_launch_ms": round(br_avg_s * 1e3, 3),
-            "peak_source": f"measured on this GPU: binary64 exact modmul {peak_f64 / 1e9:.1f} G/s, "
-                           f"64-bit integer Shoup modmul {peak_u64 / 1e9:.1f} G/s (microbenchmark, same prime)",
+            "avg_launch_ms": round(r1["br_avg_s"] * 1e3, 3),
+            "peak_source": f"measured on this GPU: binary64 exact modmul {peaks['f64'] / 1e9:.1f} G/s, "
+                           f"64-bit integer Shoup modmul {peaks['u64'] / 1e9:.1f} G/s (microbenchmark, same prime)",
             "bsk_bytes_per_launch_if_streamed_once": bsk_bytes(p),
         },
-        "kernels_ms_per_step": {"k_blind_rotate": round(br_ms / max(1, br_n), 3), "k_extract_ks": round(ks_ms / max(1, ks_n), 3)},
-        "plain": {"value": round(plain_rate, 2), "unit": "PBS/s", "p50_ms": round(lat[len(lat) // 2] * 1e3, 3),
-                  "p99_ms": round(lat[min(len(lat) - 1, int(math.ceil(0.99 * len(lat))) - 1)] * 1e3, 3)},
+        "kernels_ms_per_step": {"k_blind_rotate": round(r1["br_avg_s"] * 1e3, 3), "k_extract_ks": round(r1["ks_avg_ms"], 3)},
+        "plain": {"value": round(plain_rate, 2), "unit": "PBS/s", "calls": len(lat),
+                  "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3),
+                  "roofline": {"bound": "hbm", "bytes_per_pbs": plain_bytes,
+                               "frac": round(plain_bytes * plain_rate / HBM_PEAK, 5),
+                               "modmul_frac": round(mm * plain_rate / peak, 5)}},
         "batched_over_plain": round(value / plain_rate, 1),
-        "decrypt_check": decrypt_ok,
+        "decrypt_check": r1["decrypt_ok"],
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p, keys, cts, v)
+    if "cpu_baseline" in r1:
+        line["cpu_baseline"] = r1["cpu_baseline"]
+    configs = {}
+    if "config2" in legs:
+        configs["config2_t56_n2048_b8192"] = leg_config2(yg, args, peak)
+    if "config3" in legs:
+        configs["config3_64_tenants_graph"] = leg_config3(yg, args, peak)
+    if "serving" in legs:
+        configs["config4_serving"] = leg_serving(yg, args, peak)
+    if configs:
+        line["configs"] = configs
     print(json.dumps(line), flush=True)
 
 

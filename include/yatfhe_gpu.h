@@ -47,6 +47,7 @@ extern "C" {
 #define YGPU_ERR_NO_DEVICE (-4)
 #define YGPU_ERR_OUT_OF_MEMORY (-5)
 #define YGPU_ERR_NOT_FOUND (-6)
+#define YGPU_ERR_BUSY (-7)      /* serving queue full: the request was shed */
 
 /* Parameter set.  Mirrors YatfheParameters (yatfhe_parameters.h:9-110) for the
  * fields the bootstrapping path reads; `level` is the gadget length used by the
@@ -206,6 +207,106 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
  * lwe_key, Gaussian noise 2^sigma_log2, seeded.  out: int32[count][n+1]. */
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count,
                      const int32_t* mu, int32_t* out);
+
+/* ---------------------------------------------------------------- serving
+ * Continuous-batching server on one GPU.  Replaces the reference's serving
+ * loop (ya_benchmark/blindrotate_serving.cpp:121-353): its shared key cache
+ * with single-flight loads (ConcurrentKeyCache,
+ * ya_benchmark/include/blindrotate_serving.h:25-111), its request scheduler
+ * (Scheduler, 132-273) and its per-request worker bootstrap.  Requests queue
+ * per parameter set; whenever one of a parameter set's lanes (streams with
+ * their own staging buffers) is free, everything pending is launched as one
+ * key-grouped batch, so batches overlap on the device.  Tenant keys live in a
+ * device arena of key_budget_bytes under LRU eviction, loaded on demand from
+ * their device-ready form in pinned host memory; a tenant whose key cannot be
+ * placed while other batches hold the memory waits in the queue (admission by
+ * device memory).  Results are bit-identical to the single-context entry
+ * points: each ciphertext is computed independently of its batch. */
+typedef struct ygpu_server ygpu_server;
+
+typedef struct ygpu_server_config {
+    int32_t device;
+    int32_t lanes;            /* in-flight batches per parameter set (0: 3) */
+    int32_t max_batch;        /* ciphertexts per launch (0: 4096) */
+    int32_t queue_max;        /* queued requests per parameter set before submit sheds (0: 2^20) */
+    int32_t max_tenants;      /* tenant table size (0: 4096) */
+    int32_t reserved;
+    int64_t key_budget_bytes; /* device bytes for resident keys (0: 3/4 of free memory) */
+} ygpu_server_config;
+
+/* One bootstrap request.  The caller owns the struct and the buffers, which
+ * must stay valid until the request completes (ygpu_server_wait). */
+typedef struct ygpu_request {
+    int32_t tenant;
+    int32_t batch_size;       /* out: ciphertexts in the launch that served it */
+    const int32_t* lwe_in;    /* int32[n+1] */
+    const void* v;            /* test polynomial, Torus[N] */
+    int32_t* lwe_out;         /* int32[n+1], written on completion */
+    int64_t arrival_ns;       /* in: steady-clock arrival (0: time of submit) */
+    int64_t launch_ns;        /* out: when its batch was launched */
+    int64_t done_ns;          /* out: when its result was copied out */
+    uint64_t seq;             /* internal */
+    int32_t status;           /* out: 1 pending, YGPU_OK done, < 0 error */
+    int32_t reserved;
+} ygpu_request;
+
+typedef struct ygpu_server_stats {
+    int64_t submitted, completed, failed, shed;
+    int64_t batches, batched_cts, request_hits;
+    int64_t key_loads, key_evictions, bytes_loaded, memory_waits;
+    int64_t resident_keys, resident_bytes, peak_resident_bytes, key_budget_bytes, host_key_bytes;
+} ygpu_server_stats;
+
+int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out);
+int ygpu_server_destroy(ygpu_server* s);
+/* Parameter sets (one engine context each); both torus widths may coexist. */
+int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* pset);
+int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl);
+/* Registers a tenant's keys (coefficient form, layouts as ygpu_key_upload):
+ * transformed once into device-ready form in pinned host memory, loaded into
+ * the device arena on first use. */
+int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* tenant);
+/* A new tenant sharing another tenant's host key material but cached on the
+ * device on its own (separate residency, loads and evictions). */
+int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* tenant);
+int ygpu_server_tenant_params(ygpu_server* s, int32_t tenant, int32_t* pset);
+/* Asynchronous submit (thread-safe).  YGPU_ERR_BUSY when the queue is full. */
+int ygpu_server_submit(ygpu_server* s, ygpu_request* req);
+/* Blocks until the request completes; returns its status. */
+int ygpu_server_wait(ygpu_server* s, ygpu_request* req);
+int ygpu_server_stats_get(ygpu_server* s, ygpu_server_stats* out);
+
+/* Open-loop load generator over the server (no counterpart in the reference
+ * library; it mirrors the reference's serving benchmark driver): Poisson
+ * arrivals at `rate` requests/s over tenants 0..n_tenants-1 with Zipf(zipf_s)
+ * popularity (rank r is tenant r), every request of tenant t bootstrapping
+ * lwe_in[t] with v[t]; when expected[t] is given each result is compared with
+ * it.  Latency runs from a request's scheduled arrival to its completion. */
+typedef struct ygpu_loadgen_config {
+    double rate;
+    int64_t requests;
+    int64_t warm;             /* burst of warm-up requests before the measured run */
+    double zipf_s;
+    uint64_t seed;
+    int32_t n_tenants;
+    int32_t reserved;
+    const int32_t* const* lwe_in;
+    const void* const* v;
+    const int32_t* const* expected;   /* may be NULL, or hold NULL entries */
+    double slo_ms;
+} ygpu_loadgen_config;
+
+typedef struct ygpu_loadgen_result {
+    double offered_rps, throughput_rps, wall_sec;
+    int64_t completed, shed, errors, mismatches;
+    double mean_ms, p50_ms, p90_ms, p99_ms, p99_9_ms, max_ms;
+    double queue_p50_ms, queue_p99_ms, slo_violation_rate;
+    int64_t batches;
+    double mean_batch, hit_rate;
+    int64_t key_loads, key_evictions, bytes_loaded, memory_waits;
+} ygpu_loadgen_result;
+
+int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* cfg, ygpu_loadgen_result* out);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,45 @@ class _Job(ctypes.Structure):
     _fields_ = [("key_id", ctypes.c_int32), ("count", ctypes.c_int32), ("first", ctypes.c_int64)]
 
 
+class _ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("queue_max", ctypes.c_int32), ("max_tenants", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("key_budget_bytes", ctypes.c_int64)]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("tenant", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("lwe_in", ctypes.c_void_p),
+                ("v", ctypes.c_void_p), ("lwe_out", ctypes.c_void_p), ("arrival_ns", ctypes.c_int64),
+                ("launch_ns", ctypes.c_int64), ("done_ns", ctypes.c_int64), ("seq", ctypes.c_uint64),
+                ("status", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+_STAT_FIELDS = ["submitted", "completed", "failed", "shed", "batches", "batched_cts", "request_hits", "key_loads",
+                "key_evictions", "bytes_loaded", "memory_waits", "resident_keys", "resident_bytes",
+                "peak_resident_bytes", "key_budget_bytes", "host_key_bytes"]
+
+
+class _ServerStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int64) for f in _STAT_FIELDS]
+
+
+class _LoadgenConfig(ctypes.Structure):
+    _fields_ = [("rate", ctypes.c_double), ("requests", ctypes.c_int64), ("warm", ctypes.c_int64),
+                ("zipf_s", ctypes.c_double), ("seed", ctypes.c_uint64), ("n_tenants", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("lwe_in", ctypes.POINTER(ctypes.c_void_p)),
+                ("v", ctypes.POINTER(ctypes.c_void_p)), ("expected", ctypes.POINTER(ctypes.c_void_p)),
+                ("slo_ms", ctypes.c_double)]
+
+
+class _LoadgenResult(ctypes.Structure):
+    _fields_ = ([(f, ctypes.c_double) for f in ("offered_rps", "throughput_rps", "wall_sec")]
+                + [(f, ctypes.c_int64) for f in ("completed", "shed", "errors", "mismatches")]
+                + [(f, ctypes.c_double) for f in ("mean_ms", "p50_ms", "p90_ms", "p99_ms", "p99_9_ms", "max_ms",
+                                                   "queue_p50_ms", "queue_p99_ms", "slo_violation_rate")]
+                + [("batches", ctypes.c_int64), ("mean_batch", ctypes.c_double), ("hit_rate", ctypes.c_double)]
+                + [(f, ctypes.c_int64) for f in ("key_loads", "key_evictions", "bytes_loaded", "memory_waits")])
+
+
 @dataclass(frozen=True)
 class Params:
     """Bootstrapping parameter set (the fields of YatfheParameters the path reads)."""
@@ -142,6 +181,17 @@ def load_library(path: str | None = None):
         "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
+        "ygpu_server_create": (c_int, [P(_ServerConfig), P(c_vp)]),
+        "ygpu_server_destroy": (c_int, [c_vp]),
+        "ygpu_server_add_params": (c_int, [c_vp, P(_Params), P(c_i32)]),
+        "ygpu_server_set_tile": (c_int, [c_vp, c_i32, c_i32, c_i32]),
+        "ygpu_server_add_tenant": (c_int, [c_vp, c_i32, c_vp, c_vp, P(c_i32)]),
+        "ygpu_server_add_tenant_shared": (c_int, [c_vp, c_i32, P(c_i32)]),
+        "ygpu_server_tenant_params": (c_int, [c_vp, c_i32, P(c_i32)]),
+        "ygpu_server_submit": (c_int, [c_vp, P(_Request)]),
+        "ygpu_server_wait": (c_int, [c_vp, P(_Request)]),
+        "ygpu_server_stats_get": (c_int, [c_vp, P(_ServerStats)]),
+        "ygpu_server_loadgen": (c_int, [c_vp, P(_LoadgenConfig), P(_LoadgenResult)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -161,7 +211,9 @@ def exported_symbols():
         "ygpu_timing_enable", "ygpu_timing_get", "ygpu_timing_reset", "ygpu_keygen", "ygpu_lwe_encrypt",
         "ygpu_modmul_peak", "ygpu_copy_device", "ygpu_get_tile", "ygpu_set_tile", "ygpu_autotune",
         "ygpu_graph_create", "ygpu_graph_launch", "ygpu_graph_destroy", "ygpu_blind_rotate_scaled", "ygpu_keyswitch",
-        "ygpu_sample_extract",
+        "ygpu_sample_extract", "ygpu_server_create", "ygpu_server_destroy", "ygpu_server_add_params",
+        "ygpu_server_set_tile", "ygpu_server_add_tenant", "ygpu_server_add_tenant_shared", "ygpu_server_tenant_params",
+        "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_stats_get", "ygpu_server_loadgen",
     ]
 
 
@@ -483,8 +535,115 @@ class Context:
         _check(self.lib.ygpu_timing_reset(self.h))
 
 
+class Request:
+    """One bootstrap request: owns copies of its input, test polynomial and output."""
+
+    def __init__(self, params: Params, tenant: int, lwe_in: np.ndarray, v: np.ndarray, arrival_ns: int = 0):
+        self.lwe_in = np.ascontiguousarray(lwe_in, dtype=np.int32)
+        self.v = np.ascontiguousarray(v, dtype=params.torus_dtype)
+        assert self.lwe_in.shape == (params.n + 1,) and self.v.shape == (params.N,)
+        self.out = np.zeros(params.n + 1, np.int32)
+        self.c = _Request(tenant, 0, self.lwe_in.ctypes.data, self.v.ctypes.data, self.out.ctypes.data, arrival_ns)
+
+    @property
+    def status(self):
+        return self.c.status
+
+    @property
+    def latency_ms(self):
+        return (self.c.done_ns - self.c.arrival_ns) * 1e-6
+
+    @property
+    def batch_size(self):
+        return self.c.batch_size
+
+
+class Server:
+    """Continuous-batching bootstrapping server (ygpu_server_*)."""
+
+    def __init__(self, device: int = 0, key_budget_bytes: int = 0, lanes: int = 0, max_batch: int = 0,
+                 queue_max: int = 0, max_tenants: int = 0):
+        self.lib = load_library()
+        self.h = ctypes.c_void_p()
+        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, 0, key_budget_bytes)
+        _check(self.lib.ygpu_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+        self.params = []
+        self.tenant_params = []
+
+    def close(self):
+        if self.h:
+            self.lib.ygpu_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_params(self, params: Params) -> int:
+        cp = params.c()
+        ps = ctypes.c_int32()
+        _check(self.lib.ygpu_server_add_params(self.h, ctypes.byref(cp), ctypes.byref(ps)))
+        self.params.append(params)
+        return ps.value
+
+    def set_tile(self, pset: int, cpw: int, twl: int):
+        _check(self.lib.ygpu_server_set_tile(self.h, pset, cpw, twl))
+
+    def add_tenant(self, pset: int, keys: KeySet) -> int:
+        t = ctypes.c_int32()
+        bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
+        ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        _check(self.lib.ygpu_server_add_tenant(self.h, pset, _ptr(bsk), _ptr(ksk), ctypes.byref(t)))
+        self.tenant_params.append(pset)
+        return t.value
+
+    def add_tenant_shared(self, src: int) -> int:
+        t = ctypes.c_int32()
+        _check(self.lib.ygpu_server_add_tenant_shared(self.h, src, ctypes.byref(t)))
+        self.tenant_params.append(self.tenant_params[src])
+        return t.value
+
+    def request(self, tenant: int, lwe_in: np.ndarray, v: np.ndarray) -> Request:
+        return Request(self.params[self.tenant_params[tenant]], tenant, lwe_in, v)
+
+    def submit(self, req: Request):
+        _check(self.lib.ygpu_server_submit(self.h, ctypes.byref(req.c)))
+
+    def wait(self, req: Request) -> np.ndarray:
+        _check(self.lib.ygpu_server_wait(self.h, ctypes.byref(req.c)))
+        return req.out
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(self.lib.ygpu_server_stats_get(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f in _STAT_FIELDS}
+
+    def loadgen(self, rate: float, requests: int, inputs, luts, expected=None, warm: int = 0, zipf_s: float = 1.1,
+                seed: int = 5, slo_ms: float = 100.0) -> dict:
+        """Open-loop Poisson/Zipf load over tenants 0..len(inputs)-1 (see ygpu_server_loadgen)."""
+        n = len(inputs)
+        keep = [np.ascontiguousarray(a, dtype=np.int32) for a in inputs]
+        keepv = [np.ascontiguousarray(v) for v in luts]
+        arr_in = (ctypes.c_void_p * n)(*[a.ctypes.data for a in keep])
+        arr_v = (ctypes.c_void_p * n)(*[v.ctypes.data for v in keepv])
+        arr_e = None
+        if expected is not None:
+            keepe = [np.ascontiguousarray(e, dtype=np.int32) for e in expected]
+            arr_e = (ctypes.c_void_p * n)(*[e.ctypes.data for e in keepe])
+        cfg = _LoadgenConfig(rate, requests, warm, zipf_s, seed, n, 0,
+                             ctypes.cast(arr_in, ctypes.POINTER(ctypes.c_void_p)),
+                             ctypes.cast(arr_v, ctypes.POINTER(ctypes.c_void_p)),
+                             ctypes.cast(arr_e, ctypes.POINTER(ctypes.c_void_p)) if arr_e is not None else None,
+                             slo_ms)
+        res = _LoadgenResult()
+        _check(self.lib.ygpu_server_loadgen(self.h, ctypes.byref(cfg), ctypes.byref(res)))
+        return {f: getattr(res, f) for f, _ in _LoadgenResult._fields_}
+
+
 __all__ = [
     "Params", "PARAMS_T32_N1024", "PARAMS_T56_N2048", "Q_49P_T32", "Q_61P_T56", "KeySet", "keygen",
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
-    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace",
+    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
 ]

@@ -155,14 +155,15 @@ inline hipEvent_t take_event(ygpu_ctx* ctx) {
     return e;
 }
 
-// Launch `f` bracketed by events on the context stream when timing is on.
+// Launch `f` bracketed by events on `stream` (the stream `f` launches on)
+// when timing is on.
 template <class F>
-hipError_t timed(ygpu_ctx* ctx, const char* name, F&& f) {
+hipError_t timed(ygpu_ctx* ctx, const char* name, hipStream_t stream, F&& f) {
     if (!ctx->timing || ctx->capturing) return f();
     Timing t{name, take_event(ctx), take_event(ctx)};
-    hipEventRecord(t.a, ctx->stream);
+    hipEventRecord(t.a, stream);
     hipError_t e = f();
-    hipEventRecord(t.b, ctx->stream);
+    hipEventRecord(t.b, stream);
     ctx->pending.push_back(t);
     return e;
 }
@@ -286,14 +287,14 @@ inline int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* co
         partial = ctx->d_partial;
     }
     KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out, partial);
-    YG_HIP(timed(ctx, "k_extract_ks", [&] { return launch_extract_ks(ctx->tbits, a, slices, st); }));
-    if (d_ext) YG_HIP(timed(ctx, "k_extract", [&] { return launch_extract(ctx->tbits, a, d_ext, st); }));
+    YG_HIP(timed(ctx, "k_extract_ks", st, [&] { return launch_extract_ks(ctx->tbits, a, slices, st); }));
+    if (d_ext) YG_HIP(timed(ctx, "k_extract", st, [&] { return launch_extract(ctx->tbits, a, d_ext, st); }));
     return YGPU_OK;
 }
 
 inline int run_br(ygpu_ctx* ctx, const BrTile& tile, const BrArgs& a, hipStream_t stream = nullptr) {
     hipStream_t st = stream ? stream : ctx->stream;
-    YG_HIP(timed(ctx, "k_blind_rotate", [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, tile, ctx->p.radix_bits, a, st); }));
+    YG_HIP(timed(ctx, "k_blind_rotate", st, [&] { return launch_blind_rotate(ctx->tbits, ctx->logn, tile, ctx->p.radix_bits, a, st); }));
     return YGPU_OK;
 }
 

@@ -303,7 +303,7 @@ int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out) {
         const size_t tb = ctx->torus_bytes();
         YG_HIP(hipMemcpyAsync(ctx->d_acc1, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
         KsArgs a = ks_args(ctx, 1, ctx->d_acc1, nullptr, nullptr, nullptr, nullptr);
-        YG_HIP(timed(ctx, "k_extract", [&] { return launch_extract(ctx->tbits, a, ctx->d_ext1, ctx->stream); }));
+        YG_HIP(timed(ctx, "k_extract", ctx->stream, [&] { return launch_extract(ctx->tbits, a, ctx->d_ext1, ctx->stream); }));
         YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
         YG_HIP(hipStreamSynchronize(ctx->stream));
         return YGPU_OK;
@@ -424,7 +424,7 @@ int ygpu_modmul_peak(ygpu_ctx* ctx, int32_t kind, double* modmul_per_sec) {
 int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes) {
     if (!d_dst || !d_src || bytes % 16) return fail(YGPU_ERR_INVALID_ARG, "copy needs device buffers and a multiple of 16 bytes");
     return with_ctx(ctx, [&]() -> int {
-        YG_HIP(timed(ctx, "k_copy_u32x4", [&] { return launch_copy(d_src, d_dst, bytes, ctx->stream); }));
+        YG_HIP(timed(ctx, "k_copy_u32x4", ctx->stream, [&] { return launch_copy(d_src, d_dst, bytes, ctx->stream); }));
         return YGPU_OK;
     });
 }
@@ -555,7 +555,7 @@ int ygpu_graph_launch(ygpu_ctx* ctx, int32_t graph_id) {
     return with_ctx(ctx, [&]() -> int {
         if (graph_id < 0 || graph_id >= static_cast<int32_t>(ctx->graphs.size()) || !ctx->graphs[graph_id].live)
             return fail(YGPU_ERR_NOT_FOUND, "unknown graph id");
-        YG_HIP(timed(ctx, "graph_pbs", [&] { return hipGraphLaunch(ctx->graphs[graph_id].exec, ctx->stream); }));
+        YG_HIP(timed(ctx, "graph_pbs", ctx->stream, [&] { return hipGraphLaunch(ctx->graphs[graph_id].exec, ctx->stream); }));
         return YGPU_OK;
     });
 }

@@ -1,0 +1,863 @@
+// Continuous-batching bootstrapping server on one GPU.
+//
+// The reference serves bootstraps from a CPU worker pool: a shared key cache
+// with single-flight loads (ya_benchmark/include/blindrotate_serving.h:25-111),
+// a request scheduler (132-273) and one blind rotation per request and worker
+// (ya_benchmark/blindrotate_serving.cpp:121-353).  On the GPU one bootstrap per
+// launch would leave the device idle, so this server batches instead:
+//
+//  * Requests queue per parameter set.  A dispatcher thread per parameter set
+//    takes everything pending (up to max_batch) whenever one of its lanes is
+//    free, groups it by tenant and launches one key-grouped blind rotation over
+//    the whole batch, then extract + keyswitch per tenant, on the lane's stream.
+//    Requests arriving while batches run queue up and form the next batch, so
+//    several batches of different sizes (and of both torus widths) overlap on
+//    the device.
+//  * Keys live in one device arena of a fixed byte budget (allocated once: no
+//    hipMalloc/hipFree while serving).  A tenant's key is loaded on first use
+//    from its device-ready form in pinned host memory (the spill tier) with an
+//    asynchronous copy on a dedicated stream; batches that need it wait on the
+//    load's event, so a key is loaded once however many batches ask for it.
+//    When the arena is full, the contiguous window of idle keys whose newest
+//    use is oldest is evicted (LRU with contiguity); keys used by batches in
+//    flight are never evicted.
+//  * Admission by device memory: a tenant whose key cannot be made resident
+//    (every candidate byte is held by batches in flight) stays queued, the rest
+//    of the batch launches, and the dispatcher retries when a batch completes.
+//  * A completion thread per parameter set retires lanes in launch order,
+//    copies results to the callers' buffers and wakes waiters.
+//
+// The PBS itself is a deterministic function of (key, ciphertext, test
+// polynomial), and the batched kernels compute every ciphertext independently,
+// so a request's result does not depend on what it was batched with.
+#include "ygpu_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <random>
+#include <thread>
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+inline int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
+}
+
+// Device-ready key in pinned host memory.
+struct PinnedKey {
+    double* bsk = nullptr;
+    uint32_t* planes = nullptr;
+    size_t bsk_bytes = 0, planes_bytes = 0;
+};
+
+struct Tenant {
+    int pset = 0;
+    int form = 0;
+    size_t bytes = 0;      // arena bytes (BSK + KSK planes, each 256-byte aligned)
+    bool resident = false;
+    size_t off = 0;        // arena offset when resident
+    hipEvent_t ready = nullptr;
+    int inuse = 0;         // batches in flight holding the key
+    uint64_t last_use = 0;
+};
+
+struct Region {
+    size_t off, size;
+    int tenant;  // -1: free
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int32_t* h_in = nullptr;
+    void* h_v = nullptr;
+    int32_t* h_out = nullptr;
+    int32_t* h_slots = nullptr;
+    int32_t* d_in = nullptr;
+    void* d_v = nullptr;
+    int32_t* d_out = nullptr;
+    void* d_acc = nullptr;
+    int32_t* d_slots = nullptr;
+    int32_t* d_partial = nullptr;
+    bool busy = false;
+    int rc = 0;
+    int64_t launch_ns = 0;
+    std::vector<ygpu_request*> reqs;
+    std::vector<int> tenants;
+    std::vector<int> hit;  // per request: key was resident when the batch formed
+};
+
+struct PSet {
+    ygpu_ctx* ctx = nullptr;
+    ygpu_params p{};
+    size_t tbytes = 4;
+    std::vector<Lane> lanes;
+    std::deque<ygpu_request*> queue;
+    std::deque<int> inflight;
+    std::condition_variable cv_work;  // queue grew, a lane freed or memory freed
+    std::condition_variable cv_done;  // a batch was launched
+    std::thread dispatcher, completer;
+    bool dispatcher_done = false;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~size_t{255}; }
+
+}  // namespace
+
+struct ygpu_server {
+    ygpu_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_req;
+    std::vector<std::unique_ptr<PSet>> psets;
+    std::vector<Tenant> tenants;
+    std::vector<PinnedKey> forms;
+    char* d_arena = nullptr;
+    size_t arena_bytes = 0;
+    std::vector<Region> regions;
+    hipStream_t copy_stream = nullptr;
+    const double** d_bsk_tab = nullptr;
+    const int32_t** d_ksk_tab = nullptr;
+    const double** h_bsk_tab = nullptr;   // pinned mirrors (copy sources)
+    const int32_t** h_ksk_tab = nullptr;
+    bool started = false;
+    bool stop = false;
+    uint64_t tick = 0;
+    ygpu_server_stats st{};
+};
+
+namespace {
+
+using namespace ygpu_rt;
+
+int server_fail_hip(hipError_t e, const char* what) {
+    return fail(YGPU_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---- arena ------------------------------------------------------------------
+
+void merge_free(ygpu_server* s) {
+    std::vector<Region> out;
+    for (const Region& r : s->regions) {
+        if (!out.empty() && out.back().tenant < 0 && r.tenant < 0) out.back().size += r.size;
+        else out.push_back(r);
+    }
+    s->regions.swap(out);
+}
+
+void evict(ygpu_server* s, int t) {
+    Tenant& tn = s->tenants[t];
+    tn.resident = false;
+    s->st.key_evictions++;
+    s->st.resident_bytes -= static_cast<int64_t>(tn.bytes);
+    s->st.resident_keys--;
+    for (Region& r : s->regions)
+        if (r.tenant == t) r.tenant = -1;
+}
+
+// Finds `bytes` of arena: best-fitting free region, else the contiguous window
+// of free regions and idle keys whose newest key use is oldest.  Returns false
+// when every window holds a key in use.
+bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
+    int best = -1;
+    for (size_t i = 0; i < s->regions.size(); i++) {
+        const Region& r = s->regions[i];
+        if (r.tenant < 0 && r.size >= bytes && (best < 0 || r.size < s->regions[best].size)) best = static_cast<int>(i);
+    }
+    if (best < 0) {
+        size_t bi = 0, bj = 0;
+        uint64_t bcost = UINT64_MAX;
+        size_t bevict = SIZE_MAX;
+        for (size_t i = 0; i < s->regions.size(); i++) {
+            size_t sum = 0, evicted = 0;
+            uint64_t cost = 0;
+            size_t j = i;
+            for (; j < s->regions.size() && sum < bytes; j++) {
+                const Region& r = s->regions[j];
+                if (r.tenant >= 0) {
+                    const Tenant& tn = s->tenants[r.tenant];
+                    if (tn.inuse > 0) break;
+                    cost = std::max(cost, tn.last_use);
+                    evicted += r.size;
+                }
+                sum += r.size;
+            }
+            if (sum >= bytes && (cost < bcost || (cost == bcost && evicted < bevict))) {
+                bcost = cost;
+                bevict = evicted;
+                bi = i;
+                bj = j;
+            }
+        }
+        if (bcost == UINT64_MAX) return false;
+        for (size_t k = bi; k < bj; k++)
+            if (s->regions[k].tenant >= 0) evict(s, s->regions[k].tenant);
+        merge_free(s);
+        for (size_t i = 0; i < s->regions.size(); i++) {
+            const Region& r = s->regions[i];
+            if (r.tenant < 0 && r.size >= bytes && (best < 0 || r.size < s->regions[best].size)) best = static_cast<int>(i);
+        }
+        if (best < 0) return false;  // not reached: the window merged into one free region
+    }
+    Region r = s->regions[best];
+    *off = r.off;
+    Region used{r.off, bytes, tenant};
+    if (r.size == bytes) {
+        s->regions[best] = used;
+    } else {
+        s->regions[best] = Region{r.off + bytes, r.size - bytes, -1};
+        s->regions.insert(s->regions.begin() + best, used);
+    }
+    return true;
+}
+
+// Makes `t` resident (loading it if needed) and takes a reference for a batch.
+// Returns 1 when memory is not available now, 0 on success, <0 on error.
+// Called with the server lock held.
+int acquire_key(ygpu_server* s, int t, bool* hit) {
+    Tenant& tn = s->tenants[t];
+    tn.last_use = ++s->tick;
+    if (tn.resident) {
+        tn.inuse++;
+        *hit = true;
+        return 0;
+    }
+    *hit = false;
+    size_t off = 0;
+    if (!arena_alloc(s, tn.bytes, t, &off)) return 1;
+    const PinnedKey& f = s->forms[tn.form];
+    char* base = s->d_arena + off;
+    s->h_bsk_tab[t] = reinterpret_cast<const double*>(base);
+    s->h_ksk_tab[t] = reinterpret_cast<const int32_t*>(base + align256(f.bsk_bytes));
+    hipError_t e = hipMemcpyAsync(base, f.bsk, f.bsk_bytes, hipMemcpyHostToDevice, s->copy_stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(base + align256(f.bsk_bytes), f.planes, f.planes_bytes, hipMemcpyHostToDevice, s->copy_stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_bsk_tab + t, s->h_bsk_tab + t, sizeof(void*), hipMemcpyHostToDevice, s->copy_stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_ksk_tab + t, s->h_ksk_tab + t, sizeof(void*), hipMemcpyHostToDevice, s->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(tn.ready, s->copy_stream);
+    if (e != hipSuccess) {
+        for (Region& r : s->regions)
+            if (r.tenant == t) r.tenant = -1;
+        merge_free(s);
+        return server_fail_hip(e, "key load");
+    }
+    tn.resident = true;
+    tn.off = off;
+    tn.inuse++;
+    s->st.key_loads++;
+    s->st.bytes_loaded += static_cast<int64_t>(f.bsk_bytes + f.planes_bytes);
+    s->st.resident_bytes += static_cast<int64_t>(tn.bytes);
+    s->st.resident_keys++;
+    s->st.peak_resident_bytes = std::max(s->st.peak_resident_bytes, s->st.resident_bytes);
+    return 0;
+}
+
+// ---- lanes --------------------------------------------------------------------
+
+int lane_alloc(const ygpu_server* s, PSet& ps, Lane& L) {
+    const ygpu_params& p = ps.p;
+    const size_t B = static_cast<size_t>(s->cfg.max_batch);
+    const size_t lwe = sizeof(int32_t) * (p.n + 1);
+    YG_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    YG_HIP(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_in), B * lwe, hipHostMallocDefault));
+    YG_HIP(hipHostMalloc(&L.h_v, B * ps.tbytes * p.N, hipHostMallocDefault));
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_out), B * lwe, hipHostMallocDefault));
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_slots), B * sizeof(int32_t), hipHostMallocDefault));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_in), B * lwe));
+    YG_HIP(hipMalloc(&L.d_v, B * ps.tbytes * p.N));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_out), B * lwe));
+    YG_HIP(hipMalloc(&L.d_acc, B * ps.tbytes * (p.k + 1) * p.N));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_slots), B * sizeof(int32_t)));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_partial), B * lwe));
+    return YGPU_OK;
+}
+
+void lane_free(Lane& L) {
+    if (L.stream) hipStreamDestroy(L.stream);
+    if (L.done) hipEventDestroy(L.done);
+    for (void* h : {static_cast<void*>(L.h_in), L.h_v, static_cast<void*>(L.h_out), static_cast<void*>(L.h_slots)})
+        if (h) hipHostFree(h);
+    for (void* d : {static_cast<void*>(L.d_in), L.d_v, static_cast<void*>(L.d_out), L.d_acc, static_cast<void*>(L.d_slots),
+                    static_cast<void*>(L.d_partial)})
+        if (d) hipFree(d);
+    L = Lane{};
+}
+
+// Stages the lane's requests (already grouped by tenant) and enqueues the
+// pipeline on the lane's stream.  Runs without the server lock.
+int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
+    const ygpu_params& p = ps.p;
+    const int count = static_cast<int>(L.reqs.size());
+    const size_t lwe = sizeof(int32_t) * (p.n + 1);
+    const size_t vbytes = ps.tbytes * p.N;
+    for (int c = 0; c < count; c++) {
+        const ygpu_request* r = L.reqs[c];
+        std::memcpy(L.h_in + static_cast<size_t>(c) * (p.n + 1), r->lwe_in, lwe);
+        std::memcpy(static_cast<char*>(L.h_v) + c * vbytes, r->v, vbytes);
+        L.h_slots[c] = r->tenant;
+    }
+    for (int t : L.tenants) YG_HIP(hipStreamWaitEvent(L.stream, s->tenants[t].ready, 0));
+    YG_HIP(hipMemcpyAsync(L.d_in, L.h_in, count * lwe, hipMemcpyHostToDevice, L.stream));
+    YG_HIP(hipMemcpyAsync(L.d_v, L.h_v, count * vbytes, hipMemcpyHostToDevice, L.stream));
+    YG_HIP(hipMemcpyAsync(L.d_slots, L.h_slots, count * sizeof(int32_t), hipMemcpyHostToDevice, L.stream));
+    BrArgs a = br_args(ps.ctx, count, L.d_in, L.d_v, p.N, L.d_acc, s->d_bsk_tab, L.d_slots);
+    int rc = run_br(ps.ctx, ps.ctx->tile, a, L.stream);
+    if (rc) return rc;
+    const size_t accRow = ps.tbytes * (p.k + 1) * p.N;
+    for (int c0 = 0; c0 < count;) {
+        int c1 = c0;
+        while (c1 < count && L.reqs[c1]->tenant == L.reqs[c0]->tenant) c1++;
+        rc = run_ks(ps.ctx, c1 - c0, static_cast<const char*>(L.d_acc) + accRow * c0, s->d_ksk_tab + L.reqs[c0]->tenant, nullptr,
+                    L.d_out + static_cast<size_t>(c0) * (p.n + 1), nullptr, L.d_partial + static_cast<size_t>(c0) * (p.n + 1),
+                    L.stream);
+        if (rc) return rc;
+        c0 = c1;
+    }
+    YG_HIP(hipMemcpyAsync(L.h_out, L.d_out, count * lwe, hipMemcpyDeviceToHost, L.stream));
+    YG_HIP(hipEventRecord(L.done, L.stream));
+    return YGPU_OK;
+}
+
+void finish_request(ygpu_request* r, int status, int64_t launch, int64_t done, int batch) {
+    r->launch_ns = launch;
+    r->done_ns = done;
+    r->batch_size = batch;
+    r->status = status;
+}
+
+void dispatcher_loop(ygpu_server* s, PSet* ps) {
+    hipSetDevice(s->cfg.device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        auto free_lane = [&]() -> int {
+            for (size_t i = 0; i < ps->lanes.size(); i++)
+                if (!ps->lanes[i].busy) return static_cast<int>(i);
+            return -1;
+        };
+        ps->cv_work.wait(lk, [&] { return s->stop || (!ps->queue.empty() && free_lane() >= 0); });
+        if (s->stop) break;
+        const int li = free_lane();
+        Lane& L = ps->lanes[li];
+        // everything pending, up to one batch, grouped by tenant (stable)
+        std::vector<ygpu_request*> take;
+        while (!ps->queue.empty() && static_cast<int>(take.size()) < s->cfg.max_batch) {
+            take.push_back(ps->queue.front());
+            ps->queue.pop_front();
+        }
+        std::stable_sort(take.begin(), take.end(), [](const ygpu_request* a, const ygpu_request* b) { return a->tenant < b->tenant; });
+        L.reqs.clear();
+        L.tenants.clear();
+        L.hit.clear();
+        std::vector<ygpu_request*> deferred;
+        int err = 0;
+        for (size_t c0 = 0; c0 < take.size();) {
+            size_t c1 = c0;
+            while (c1 < take.size() && take[c1]->tenant == take[c0]->tenant) c1++;
+            const int t = take[c0]->tenant;
+            bool hit = false;
+            const int rc = err ? 1 : acquire_key(s, t, &hit);
+            if (rc < 0) err = rc;
+            if (rc == 0) {
+                L.tenants.push_back(t);
+                for (size_t c = c0; c < c1; c++) {
+                    L.reqs.push_back(take[c]);
+                    L.hit.push_back(hit ? 1 : 0);
+                }
+            } else {
+                deferred.insert(deferred.end(), take.begin() + c0, take.begin() + c1);
+            }
+            c0 = c1;
+        }
+        // requests whose key could not be placed go back to the front, in order
+        std::stable_sort(deferred.begin(), deferred.end(), [](const ygpu_request* a, const ygpu_request* b) { return a->seq < b->seq; });
+        for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) ps->queue.push_front(*it);
+        if (err) {
+            // a key load failed: fail this batch's requests and the deferred ones
+            const int64_t t = now_ns();
+            for (ygpu_request* r : L.reqs) finish_request(r, err, t, t, 0);
+            for (int tn : L.tenants) s->tenants[tn].inuse--;
+            L.reqs.clear();
+            L.tenants.clear();
+            s->cv_req.notify_all();
+            continue;
+        }
+        if (L.reqs.empty()) {
+            s->st.memory_waits++;
+            bool any_inflight = false;
+            for (auto& q : s->psets) any_inflight |= !q->inflight.empty();
+            for (auto& q : s->psets)
+                for (auto& l : q->lanes) any_inflight |= l.busy;
+            if (!any_inflight) {
+                // nothing will free memory: cannot happen for keys that fit the budget
+                const int64_t t = now_ns();
+                while (!ps->queue.empty()) {
+                    finish_request(ps->queue.front(), fail(YGPU_ERR_OUT_OF_MEMORY, "key does not fit the key budget"), t, t, 0);
+                    ps->queue.pop_front();
+                }
+                s->cv_req.notify_all();
+                continue;
+            }
+            ps->cv_work.wait(lk);  // woken by a completion
+            continue;
+        }
+        L.busy = true;
+        L.launch_ns = now_ns();
+        s->st.batches++;
+        s->st.batched_cts += static_cast<int64_t>(L.reqs.size());
+        for (int h : L.hit) s->st.request_hits += h;
+        lk.unlock();
+        L.rc = launch_lane(s, *ps, L);
+        lk.lock();
+        ps->inflight.push_back(li);
+        ps->cv_done.notify_all();
+    }
+    ps->dispatcher_done = true;
+    ps->cv_done.notify_all();
+}
+
+void completer_loop(ygpu_server* s, PSet* ps) {
+    hipSetDevice(s->cfg.device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        ps->cv_done.wait(lk, [&] { return !ps->inflight.empty() || ps->dispatcher_done; });
+        if (ps->inflight.empty()) break;
+        const int li = ps->inflight.front();
+        Lane& L = ps->lanes[li];
+        lk.unlock();
+        int rc = L.rc;
+        if (rc == 0) {
+            const hipError_t e = hipEventSynchronize(L.done);
+            if (e != hipSuccess) rc = server_fail_hip(e, "batch");
+        } else {
+            hipStreamSynchronize(L.stream);
+        }
+        const int64_t t = now_ns();
+        const int count = static_cast<int>(L.reqs.size());
+        if (rc == 0) {
+            for (int c = 0; c < count; c++)
+                std::memcpy(L.reqs[c]->lwe_out, L.h_out + static_cast<size_t>(c) * (ps->p.n + 1), sizeof(int32_t) * (ps->p.n + 1));
+        }
+        lk.lock();
+        ps->inflight.pop_front();
+        for (int tn : L.tenants) s->tenants[tn].inuse--;
+        for (int c = 0; c < count; c++) finish_request(L.reqs[c], rc, L.launch_ns, t, count);
+        if (rc == 0) s->st.completed += count;
+        else s->st.failed += count;
+        L.reqs.clear();
+        L.tenants.clear();
+        L.busy = false;
+        s->cv_req.notify_all();
+        for (auto& q : s->psets) q->cv_work.notify_all();
+    }
+}
+
+int start_threads(ygpu_server* s) {
+    if (s->started) return YGPU_OK;
+    for (auto& ps : s->psets) {
+        ps->dispatcher = std::thread(dispatcher_loop, s, ps.get());
+        ps->completer = std::thread(completer_loop, s, ps.get());
+    }
+    s->started = true;
+    return YGPU_OK;
+}
+
+template <class F>
+int with_server(ygpu_server* s, F&& f) {
+    if (!s) return fail(YGPU_ERR_INVALID_ARG, "null server");
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        return fail(YGPU_ERR_INVALID_ARG, e.what());
+    }
+}
+
+}  // namespace
+
+// ======================================================================= C ABI
+
+extern "C" {
+
+int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
+    if (!cfg || !out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(YGPU_ERR_NO_DEVICE, "no HIP device");
+    if (cfg->device < 0 || cfg->device >= ndev) return fail(YGPU_ERR_INVALID_ARG, "device index out of range");
+    auto s = std::make_unique<ygpu_server>();
+    s->cfg = *cfg;
+    if (s->cfg.lanes <= 0) s->cfg.lanes = 3;
+    if (s->cfg.max_batch <= 0) s->cfg.max_batch = 4096;
+    if (s->cfg.queue_max <= 0) s->cfg.queue_max = 1 << 20;
+    if (s->cfg.max_tenants <= 0) s->cfg.max_tenants = 4096;
+    if (s->cfg.lanes > 16 || s->cfg.max_batch > (1 << 16) || s->cfg.max_tenants > (1 << 20))
+        return fail(YGPU_ERR_INVALID_ARG, "lanes <= 16, max_batch <= 65536 and max_tenants <= 2^20 required");
+    YG_HIP(hipSetDevice(cfg->device));
+    size_t freeb = 0, total = 0;
+    YG_HIP(hipMemGetInfo(&freeb, &total));
+    size_t budget = cfg->key_budget_bytes > 0 ? static_cast<size_t>(cfg->key_budget_bytes) : freeb / 4 * 3;
+    budget &= ~size_t{255};
+    if (budget == 0 || budget > freeb) return fail(YGPU_ERR_OUT_OF_MEMORY, "key budget exceeds free device memory");
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&s->d_arena), budget);
+    if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("key arena: ") + hipGetErrorString(e));
+    s->arena_bytes = budget;
+    s->regions.push_back(Region{0, budget, -1});
+    s->cfg.key_budget_bytes = static_cast<int64_t>(budget);
+    s->st.key_budget_bytes = static_cast<int64_t>(budget);
+    const size_t tab = sizeof(void*) * s->cfg.max_tenants;
+    YG_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&s->d_bsk_tab), tab));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&s->d_ksk_tab), tab));
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_bsk_tab), tab, hipHostMallocDefault));
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_ksk_tab), tab, hipHostMallocDefault));
+    std::memset(s->h_bsk_tab, 0, tab);
+    std::memset(s->h_ksk_tab, 0, tab);
+    YG_HIP(hipMemcpy(s->d_bsk_tab, s->h_bsk_tab, tab, hipMemcpyHostToDevice));
+    YG_HIP(hipMemcpy(s->d_ksk_tab, s->h_ksk_tab, tab, hipMemcpyHostToDevice));
+    *out = s.release();
+    return YGPU_OK;
+}
+
+int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* pset) {
+    return with_server(s, [&] {
+        if (!params || !pset) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->started) return fail(YGPU_ERR_INVALID_ARG, "parameter sets must be added before the first request");
+        auto ps = std::make_unique<PSet>();
+        int rc = ygpu_init(params, s->cfg.device, &ps->ctx);
+        if (rc) return rc;
+        ps->p = ps->ctx->p;
+        ps->tbytes = ps->ctx->torus_bytes();
+        ps->lanes.resize(s->cfg.lanes);
+        for (Lane& L : ps->lanes) {
+            rc = lane_alloc(s, *ps, L);
+            if (rc) {
+                for (Lane& l2 : ps->lanes) lane_free(l2);
+                ygpu_destroy(ps->ctx);
+                return rc;
+            }
+        }
+        *pset = static_cast<int32_t>(s->psets.size());
+        s->psets.push_back(std::move(ps));
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl) {
+    return with_server(s, [&] {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+        if (s->started) return fail(YGPU_ERR_INVALID_ARG, "tiles must be set before the first request");
+        return ygpu_set_tile(s->psets[pset]->ctx, cpw, twl);
+    });
+}
+
+int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* tenant) {
+    return with_server(s, [&] {
+        if (!bsk_coeff || !ksk || !tenant) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        ygpu_ctx* ctx = nullptr;
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+            if (static_cast<int>(s->tenants.size()) >= s->cfg.max_tenants) return fail(YGPU_ERR_INVALID_ARG, "max_tenants reached");
+            ctx = s->psets[pset]->ctx;
+        }
+        hipSetDevice(s->cfg.device);
+        HostKeyForm hk;
+        int rc = prepare_key_host(ctx, bsk_coeff, ksk, hk);
+        if (rc) return rc;
+        PinnedKey f;
+        f.bsk_bytes = hk.bsk.size() * sizeof(double);
+        f.planes_bytes = hk.planes.size() * sizeof(uint32_t);
+        if (align256(f.bsk_bytes) + align256(f.planes_bytes) > s->arena_bytes)
+            return fail(YGPU_ERR_OUT_OF_MEMORY, "key larger than the key budget");
+        YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&f.bsk), f.bsk_bytes, hipHostMallocDefault));
+        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&f.planes), f.planes_bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            hipHostFree(f.bsk);
+            return server_fail_hip(e, "pinned key");
+        }
+        std::memcpy(f.bsk, hk.bsk.data(), f.bsk_bytes);
+        std::memcpy(f.planes, hk.planes.data(), f.planes_bytes);
+        Tenant t;
+        t.pset = pset;
+        t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
+        e = hipEventCreateWithFlags(&t.ready, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            hipHostFree(f.bsk);
+            hipHostFree(f.planes);
+            return server_fail_hip(e, "event");
+        }
+        std::lock_guard<std::mutex> g(s->mu);
+        t.form = static_cast<int>(s->forms.size());
+        s->forms.push_back(f);
+        *tenant = static_cast<int32_t>(s->tenants.size());
+        s->tenants.push_back(t);
+        s->st.host_key_bytes += static_cast<int64_t>(f.bsk_bytes + f.planes_bytes);
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* tenant) {
+    return with_server(s, [&] {
+        if (!tenant) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        if (src_tenant < 0 || src_tenant >= static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown tenant");
+        if (static_cast<int>(s->tenants.size()) >= s->cfg.max_tenants) return fail(YGPU_ERR_INVALID_ARG, "max_tenants reached");
+        Tenant t;
+        t.pset = s->tenants[src_tenant].pset;
+        t.form = s->tenants[src_tenant].form;
+        t.bytes = s->tenants[src_tenant].bytes;
+        YG_HIP(hipSetDevice(s->cfg.device));
+        YG_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+        *tenant = static_cast<int32_t>(s->tenants.size());
+        s->tenants.push_back(t);
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_tenant_params(ygpu_server* s, int32_t tenant, int32_t* pset) {
+    return with_server(s, [&] {
+        if (!pset) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        if (tenant < 0 || tenant >= static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown tenant");
+        *pset = s->tenants[tenant].pset;
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_submit(ygpu_server* s, ygpu_request* req) {
+    return with_server(s, [&] {
+        if (!req || !req->lwe_in || !req->v || !req->lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null request buffer");
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (s->stop) return fail(YGPU_ERR_INVALID_ARG, "server is shutting down");
+        if (req->tenant < 0 || req->tenant >= static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown tenant");
+        PSet& ps = *s->psets[s->tenants[req->tenant].pset];
+        s->st.submitted++;
+        if (static_cast<int64_t>(ps.queue.size()) >= s->cfg.queue_max) {
+            s->st.shed++;
+            req->status = YGPU_ERR_BUSY;
+            return fail(YGPU_ERR_BUSY, "queue full: request shed");
+        }
+        if (!s->started) start_threads(s);
+        req->status = 1;
+        if (req->arrival_ns == 0) req->arrival_ns = now_ns();
+        req->seq = ++s->tick;
+        ps.queue.push_back(req);
+        ps.cv_work.notify_one();
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_wait(ygpu_server* s, ygpu_request* req) {
+    return with_server(s, [&] {
+        if (!req) return fail(YGPU_ERR_INVALID_ARG, "null request");
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_req.wait(lk, [&] { return req->status != 1; });
+        return static_cast<int>(req->status);
+    });
+}
+
+int ygpu_server_stats_get(ygpu_server* s, ygpu_server_stats* out) {
+    return with_server(s, [&] {
+        if (!out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        *out = s->st;
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_destroy(ygpu_server* s) {
+    if (!s) return YGPU_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+        for (auto& ps : s->psets) ps->cv_work.notify_all();
+    }
+    for (auto& ps : s->psets) {
+        if (ps->dispatcher.joinable()) ps->dispatcher.join();
+        if (ps->completer.joinable()) ps->completer.join();
+    }
+    hipSetDevice(s->cfg.device);
+    {
+        // requests still queued never ran
+        std::lock_guard<std::mutex> g(s->mu);
+        const int64_t t = now_ns();
+        for (auto& ps : s->psets) {
+            for (ygpu_request* r : ps->queue) finish_request(r, YGPU_ERR_INVALID_ARG, t, t, 0);
+            ps->queue.clear();
+        }
+        s->cv_req.notify_all();
+    }
+    if (s->copy_stream) hipStreamSynchronize(s->copy_stream);
+    for (auto& ps : s->psets) {
+        for (Lane& L : ps->lanes) lane_free(L);
+        ygpu_destroy(ps->ctx);
+    }
+    for (Tenant& t : s->tenants)
+        if (t.ready) hipEventDestroy(t.ready);
+    for (PinnedKey& f : s->forms) {
+        if (f.bsk) hipHostFree(f.bsk);
+        if (f.planes) hipHostFree(f.planes);
+    }
+    if (s->copy_stream) hipStreamDestroy(s->copy_stream);
+    if (s->d_arena) hipFree(s->d_arena);
+    if (s->d_bsk_tab) hipFree(s->d_bsk_tab);
+    if (s->d_ksk_tab) hipFree(s->d_ksk_tab);
+    if (s->h_bsk_tab) hipHostFree(s->h_bsk_tab);
+    if (s->h_ksk_tab) hipHostFree(s->h_ksk_tab);
+    delete s;
+    return YGPU_OK;
+}
+
+// Open-loop load: Poisson arrivals (exponential gaps, std::mt19937_64 seeded
+// with seed ^ 0x9e3779b97f4a7c15, as ArrivalProcess,
+// ya_benchmark/include/blindrotate_serving.h:300-350) over tenants drawn from a
+// Zipf(s) popularity (cumulative table + lower_bound, yatfhe/include/yautil/zipf.h:47-87;
+// popularity rank r is tenant r), latency from each request's scheduled
+// arrival to its completion, percentiles by the nearest-rank rule of
+// ya_benchmark/blindrotate_serving.cpp:70-73.
+int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadgen_result* out) {
+    return with_server(s, [&] {
+        if (!c || !out || !c->lwe_in || !c->v) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        if (c->rate <= 0 || c->requests < 0 || c->warm < 0 || c->n_tenants <= 0) return fail(YGPU_ERR_INVALID_ARG, "bad load parameters");
+        std::vector<int> cols(c->n_tenants);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            if (c->n_tenants > static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_INVALID_ARG, "more tenants than registered");
+            for (int t = 0; t < c->n_tenants; t++) cols[t] = s->psets[s->tenants[t].pset]->p.n + 1;
+        }
+        const int maxCols = *std::max_element(cols.begin(), cols.end());
+        // popularity
+        std::vector<double> cdf(c->n_tenants);
+        double h = 0;
+        for (int k = 1; k <= c->n_tenants; k++) h += 1.0 / std::pow(static_cast<double>(k), c->zipf_s);
+        double acc = 0;
+        for (int r = 0; r < c->n_tenants; r++) {
+            acc += 1.0 / (std::pow(static_cast<double>(r + 1), c->zipf_s) * h);
+            cdf[r] = acc;
+        }
+        cdf.back() = 1.0;
+        std::mt19937_64 zgen(c->seed);
+        std::uniform_real_distribution<double> uni(0.0, 1.0);
+        const int64_t total = c->warm + c->requests;
+        std::vector<int> pattern(total);
+        for (auto& t : pattern) {
+            const double u = uni(zgen);
+            t = std::min<int>(static_cast<int>(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin()), c->n_tenants - 1);
+        }
+        std::vector<ygpu_request> reqs(total);
+        std::vector<int32_t> outs(static_cast<size_t>(total) * maxCols);
+        auto make = [&](int64_t i) {
+            ygpu_request& r = reqs[i];
+            std::memset(&r, 0, sizeof(r));
+            r.tenant = pattern[i];
+            r.lwe_in = c->lwe_in[r.tenant];
+            r.v = c->v[r.tenant];
+            r.lwe_out = outs.data() + static_cast<size_t>(i) * maxCols;
+        };
+        // warm-up: one burst, not measured
+        for (int64_t i = 0; i < c->warm; i++) {
+            make(i);
+            int rc = ygpu_server_submit(s, &reqs[i]);
+            if (rc && rc != YGPU_ERR_BUSY) return rc;
+        }
+        for (int64_t i = 0; i < c->warm; i++)
+            if (reqs[i].status != YGPU_ERR_BUSY) ygpu_server_wait(s, &reqs[i]);
+        ygpu_server_stats s0{};
+        ygpu_server_stats_get(s, &s0);
+        // measured open loop
+        std::mt19937_64 agen(c->seed ^ 0x9e3779b97f4a7c15ULL);
+        std::exponential_distribution<double> gap(1.0);
+        const int64_t t0 = now_ns() + 1000000;  // start 1 ms from now
+        double cursor_ms = 0;
+        int64_t shed = 0;
+        std::vector<char> submitted(total, 0);
+        for (int64_t i = c->warm; i < total; i++) {
+            cursor_ms += gap(agen) * 1000.0 / c->rate;
+            const int64_t due = t0 + static_cast<int64_t>(cursor_ms * 1e6);
+            for (;;) {
+                const int64_t now = now_ns();
+                if (now >= due) break;
+                if (due - now > 200000) std::this_thread::sleep_for(std::chrono::nanoseconds(due - now - 100000));
+                else std::this_thread::yield();
+            }
+            make(i);
+            reqs[i].arrival_ns = due;
+            const int rc = ygpu_server_submit(s, &reqs[i]);
+            if (rc == YGPU_ERR_BUSY) shed++;
+            else if (rc) return rc;
+            else submitted[i] = 1;
+        }
+        int64_t errors = 0, mismatches = 0, done = 0;
+        int64_t t_end = t0;
+        std::vector<double> lat, queue;
+        lat.reserve(c->requests);
+        queue.reserve(c->requests);
+        for (int64_t i = c->warm; i < total; i++) {
+            if (!submitted[i]) continue;
+            const int rc = ygpu_server_wait(s, &reqs[i]);
+            if (rc) {
+                errors++;
+                continue;
+            }
+            done++;
+            t_end = std::max<int64_t>(t_end, reqs[i].done_ns);
+            lat.push_back((reqs[i].done_ns - reqs[i].arrival_ns) * 1e-6);
+            queue.push_back(std::max<int64_t>(0, reqs[i].launch_ns - reqs[i].arrival_ns) * 1e-6);
+            if (c->expected && c->expected[reqs[i].tenant] &&
+                std::memcmp(reqs[i].lwe_out, c->expected[reqs[i].tenant], sizeof(int32_t) * cols[reqs[i].tenant]) != 0)
+                mismatches++;
+        }
+        ygpu_server_stats s1{};
+        ygpu_server_stats_get(s, &s1);
+        auto pct = [](std::vector<double>& v, double p) {
+            if (v.empty()) return 0.0;
+            const long i = static_cast<long>(std::ceil(p / 100.0 * v.size())) - 1;
+            return v[std::min<long>(std::max<long>(i, 0), static_cast<long>(v.size()) - 1)];
+        };
+        std::sort(lat.begin(), lat.end());
+        std::sort(queue.begin(), queue.end());
+        std::memset(out, 0, sizeof(*out));
+        const double wall = (t_end - t0) * 1e-9;
+        out->offered_rps = c->rate;
+        out->wall_sec = wall;
+        out->throughput_rps = wall > 0 ? done / wall : 0;
+        out->completed = done;
+        out->shed = shed;
+        out->errors = errors;
+        out->mismatches = mismatches;
+        double sum = 0;
+        for (double x : lat) sum += x;
+        out->mean_ms = lat.empty() ? 0 : sum / lat.size();
+        out->p50_ms = pct(lat, 50);
+        out->p90_ms = pct(lat, 90);
+        out->p99_ms = pct(lat, 99);
+        out->p99_9_ms = pct(lat, 99.9);
+        out->max_ms = lat.empty() ? 0 : lat.back();
+        out->queue_p50_ms = pct(queue, 50);
+        out->queue_p99_ms = pct(queue, 99);
+        int64_t over = 0;
+        for (double x : lat) over += x > c->slo_ms;
+        out->slo_violation_rate = lat.empty() ? 0 : static_cast<double>(over) / lat.size();
+        const int64_t nb = s1.batches - s0.batches;
+        out->batches = nb;
+        out->mean_batch = nb ? static_cast<double>(s1.batched_cts - s0.batched_cts) / nb : 0;
+        const int64_t nc = s1.batched_cts - s0.batched_cts;
+        out->hit_rate = nc ? static_cast<double>(s1.request_hits - s0.request_hits) / nc : 0;
+        out->key_loads = s1.key_loads - s0.key_loads;
+        out->key_evictions = s1.key_evictions - s0.key_evictions;
+        out->bytes_loaded = s1.bytes_loaded - s0.bytes_loaded;
+        out->memory_waits = s1.memory_waits - s0.memory_waits;
+        return YGPU_OK;
+    });
+}
+
+}  // extern "C"
