@@ -437,7 +437,11 @@ def main():
 
     import yatfhe_amd as yg
 
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
     peaks = {}
+    log("config1: TORUS32 N=1024 batched 4096, plain form, CPU baseline")
     p, r1 = leg_config1(yg, args, peaks)
     peak = max(peaks["f64"], peaks["u64"])
     value = r1["value"]
@@ -492,10 +496,13 @@ def main():
         line["cpu_baseline"] = r1["cpu_baseline"]
     configs = {}
     if "config2" in legs:
+        log("config2: TORUS56 N=2048 batch 8192")
         configs["config2_t56_n2048_b8192"] = leg_config2(yg, args, peak)
     if "config3" in legs:
+        log("config3: 64 tenants, hipGraph")
         configs["config3_64_tenants_graph"] = leg_config3(yg, args, peak)
     if "serving" in legs:
+        log("serving sweep")
         configs["config4_serving"] = leg_serving(yg, args, peak)
     if configs:
         line["configs"] = configs

@@ -48,6 +48,8 @@ extern "C" {
 #define YGPU_ERR_OUT_OF_MEMORY (-5)
 #define YGPU_ERR_NOT_FOUND (-6)
 #define YGPU_ERR_BUSY (-7)      /* serving queue full: the request was shed */
+#define YGPU_ERR_IO (-8)        /* a file could not be opened, read or written */
+#define YGPU_ERR_CORRUPT (-9)   /* a key file failed its format or digest checks */
 
 /* Parameter set.  Mirrors YatfheParameters (yatfhe_parameters.h:9-110) for the
  * fields the bootstrapping path reads; `level` is the gadget length used by the
@@ -208,6 +210,24 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count,
                      const int32_t* mu, int32_t* out);
 
+/* ------------------------------------------------------ key interchange file
+ * The canonical coefficient form (the layouts of ygpu_key_upload) as one file:
+ * a 128-byte header (magic "YGPUKEY1", version, ygpu_params, word counts,
+ * digests of each section and of the header) followed by the BSK residues
+ * (uint64) and the KSK (int32).  Replaces the reference's NTT-form key stream
+ * (ya_serializer.cpp:21-30, 76-98, 176-218), whose words are in Intel HEXL's
+ * internal order; the reference has no KSK serializer.  Writes go to
+ * `path`.tmp and are renamed into place. */
+int ygpu_key_file_write(const char* path, const ygpu_params* params, const uint64_t* bsk_coeff, const int32_t* ksk);
+/* Reads and validates the header only. */
+int ygpu_key_file_info(const char* path, ygpu_params* params);
+/* Reads a key file into caller buffers sized for its parameters; `expect`
+ * (optional) must equal the file's parameters.  YGPU_ERR_CORRUPT on a bad
+ * magic, version, size, digest or out-of-range residue. */
+int ygpu_key_file_read(const char* path, const ygpu_params* expect, uint64_t* bsk_coeff, int32_t* ksk);
+/* ygpu_key_file_read + ygpu_key_upload. */
+int ygpu_key_upload_file(ygpu_ctx* ctx, const char* path, int32_t* key_id);
+
 /* ---------------------------------------------------------------- serving
  * Continuous-batching server on one GPU.  Replaces the reference's serving
  * loop (ya_benchmark/blindrotate_serving.cpp:121-353): its shared key cache
@@ -230,7 +250,7 @@ typedef struct ygpu_server_config {
     int32_t max_batch;        /* ciphertexts per launch (0: 4096) */
     int32_t queue_max;        /* queued requests per parameter set before submit sheds (0: 2^20) */
     int32_t max_tenants;      /* tenant table size (0: 4096) */
-    int32_t reserved;
+    int32_t max_loads;        /* key loads in flight per parameter set (0: 4) */
     int64_t key_budget_bytes; /* device bytes for resident keys (0: 3/4 of free memory) */
 } ygpu_server_config;
 
@@ -247,7 +267,7 @@ typedef struct ygpu_request {
     int64_t done_ns;          /* out: when its result was copied out */
     uint64_t seq;             /* internal */
     int32_t status;           /* out: 1 pending, YGPU_OK done, < 0 error */
-    int32_t reserved;
+    int32_t key_hit;          /* out: the tenant's key was resident when the request arrived */
 } ygpu_request;
 
 typedef struct ygpu_server_stats {
@@ -274,6 +294,8 @@ int ygpu_server_tenant_params(ygpu_server* s, int32_t tenant, int32_t* pset);
 int ygpu_server_submit(ygpu_server* s, ygpu_request* req);
 /* Blocks until the request completes; returns its status. */
 int ygpu_server_wait(ygpu_server* s, ygpu_request* req);
+/* Waits at most timeout_ms; returns the request's status (1 while pending). */
+int ygpu_server_wait_for(ygpu_server* s, ygpu_request* req, int32_t timeout_ms);
 int ygpu_server_stats_get(ygpu_server* s, ygpu_server_stats* out);
 
 /* Open-loop load generator over the server (no counterpart in the reference

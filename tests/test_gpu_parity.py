@@ -184,3 +184,18 @@ def test_full_batch_4096_t32(p32, keys32, ctx32, orc32):
     for c in idx[:3]:
         alone, _, _ = ctx.pbs(kid, cts[c], v)
         assert np.array_equal(alone, out1[c])
+
+
+def test_key_file_upload_matches_direct_upload(tmp_path, p32, keys32):
+    """A key read back from the interchange file bootstraps bit-identically."""
+    path = str(tmp_path / "keys32.ygkey")
+    yg.save_keys(path, keys32)
+    bits = np.random.default_rng(31).integers(0, 2, 5)
+    cts = yg.lwe_encrypt(keys32.lwe_key, ((2 * bits - 1) * (1 << 29)).astype(np.int32), seed=31)
+    v = yg.sign_test_polynomial(p32)
+    with yg.Context(p32, 0) as ctx:
+        a = ctx.upload_key(keys32)
+        b = ctx.upload_key_file(path)
+        out_a, _, _ = ctx.pbs_batch(a, cts, v)
+        out_b, _, _ = ctx.pbs_batch(b, cts, v)
+    assert np.array_equal(out_a, out_b)

@@ -68,7 +68,7 @@ def test_alone_equals_mix_across_evictions(p32, p56, tenants):
             for c in range(len(cts)):
                 r = srv.request(t, cts[c], v)
                 srv.submit(r)
-                alone[(t, c)] = srv.wait(r).copy()
+                alone[(t, c)] = srv.wait(r, timeout=60).copy()
                 assert r.batch_size == 1
         print("server: mix", flush=True)
         reqs = [(t, c, srv.request(t, cts[c], v)) for t, prm, k, cts, v in work for c in range(len(cts))]
@@ -86,7 +86,7 @@ def test_alone_equals_mix_across_evictions(p32, p56, tenants):
             th.join()
         sizes = []
         for t, c, r in reqs:
-            got = srv.wait(r)
+            got = srv.wait(r, timeout=60)
             sizes.append(r.batch_size)
             assert np.array_equal(got, alone[(t, c)]), f"tenant {t} ciphertext {c}: mix differs from alone"
         st = srv.stats()

@@ -53,7 +53,7 @@ class _Job(ctypes.Structure):
 
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("max_batch", ctypes.c_int32),
-                ("queue_max", ctypes.c_int32), ("max_tenants", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("queue_max", ctypes.c_int32), ("max_tenants", ctypes.c_int32), ("max_loads", ctypes.c_int32),
                 ("key_budget_bytes", ctypes.c_int64)]
 
 
@@ -61,7 +61,7 @@ class _Request(ctypes.Structure):
     _fields_ = [("tenant", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("lwe_in", ctypes.c_void_p),
                 ("v", ctypes.c_void_p), ("lwe_out", ctypes.c_void_p), ("arrival_ns", ctypes.c_int64),
                 ("launch_ns", ctypes.c_int64), ("done_ns", ctypes.c_int64), ("seq", ctypes.c_uint64),
-                ("status", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("status", ctypes.c_int32), ("key_hit", ctypes.c_int32)]
 
 
 _STAT_FIELDS = ["submitted", "completed", "failed", "shed", "batches", "batched_cts", "request_hits", "key_loads",
@@ -190,8 +190,13 @@ def load_library(path: str | None = None):
         "ygpu_server_tenant_params": (c_int, [c_vp, c_i32, P(c_i32)]),
         "ygpu_server_submit": (c_int, [c_vp, P(_Request)]),
         "ygpu_server_wait": (c_int, [c_vp, P(_Request)]),
+        "ygpu_server_wait_for": (c_int, [c_vp, P(_Request), c_i32]),
         "ygpu_server_stats_get": (c_int, [c_vp, P(_ServerStats)]),
         "ygpu_server_loadgen": (c_int, [c_vp, P(_LoadgenConfig), P(_LoadgenResult)]),
+        "ygpu_key_file_write": (c_int, [ctypes.c_char_p, P(_Params), c_vp, c_vp]),
+        "ygpu_key_file_info": (c_int, [ctypes.c_char_p, P(_Params)]),
+        "ygpu_key_file_read": (c_int, [ctypes.c_char_p, P(_Params), c_vp, c_vp]),
+        "ygpu_key_upload_file": (c_int, [c_vp, ctypes.c_char_p, P(c_i32)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -213,7 +218,8 @@ def exported_symbols():
         "ygpu_graph_create", "ygpu_graph_launch", "ygpu_graph_destroy", "ygpu_blind_rotate_scaled", "ygpu_keyswitch",
         "ygpu_sample_extract", "ygpu_server_create", "ygpu_server_destroy", "ygpu_server_add_params",
         "ygpu_server_set_tile", "ygpu_server_add_tenant", "ygpu_server_add_tenant_shared", "ygpu_server_tenant_params",
-        "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_stats_get", "ygpu_server_loadgen",
+        "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_wait_for", "ygpu_server_stats_get", "ygpu_server_loadgen",
+        "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
     ]
 
 
@@ -264,6 +270,32 @@ def keygen(params: Params, seed: int, sigma_log2_lwe: float | None = None,
     _check(lib.ygpu_keygen(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe,
                            _ptr(lk), _ptr(gk), _ptr(bsk), _ptr(ksk), threads))
     return KeySet(params, lk, gk, bsk, ksk)
+
+
+def save_keys(path: str, keys: KeySet):
+    """Write the keys' coefficient form to a key interchange file (ygpu_key_file_write)."""
+    cp = keys.params.c()
+    bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
+    ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+    _check(load_library().ygpu_key_file_write(os.fsencode(path), ctypes.byref(cp), _ptr(bsk), _ptr(ksk)))
+
+
+def key_file_params(path: str) -> Params:
+    cp = _Params()
+    _check(load_library().ygpu_key_file_info(os.fsencode(path), ctypes.byref(cp)))
+    return Params(cp.torus_bits, cp.n, cp.k, cp.N, cp.radix_bits, cp.level, cp.ks_radix_bits, cp.ks_level,
+                  cp.ks_width_bits, cp.lwe_q_bits, cp.q_ntt)
+
+
+def load_key_arrays(path: str, expect: Params | None = None):
+    """(params, bsk, ksk) from a key interchange file (ygpu_key_file_read)."""
+    p = key_file_params(path)
+    bsk = np.zeros(p.bsk_shape, np.uint64)
+    ksk = np.zeros(p.ksk_shape, np.int32)
+    ce = expect.c() if expect is not None else None
+    _check(load_library().ygpu_key_file_read(os.fsencode(path), ctypes.byref(ce) if ce is not None else None,
+                                             _ptr(bsk), _ptr(ksk)))
+    return p, bsk, ksk
 
 
 def lwe_encrypt(lwe_key: np.ndarray, mu: np.ndarray, seed: int, sigma_log2: float = SIGMA_LOG2_LWE) -> np.ndarray:
@@ -378,6 +410,11 @@ class Context:
         bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
         ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
         _check(self.lib.ygpu_key_upload(self.h, _ptr(bsk), _ptr(ksk), ctypes.byref(kid)))
+        return kid.value
+
+    def upload_key_file(self, path: str) -> int:
+        kid = ctypes.c_int32()
+        _check(self.lib.ygpu_key_upload_file(self.h, os.fsencode(path), ctypes.byref(kid)))
         return kid.value
 
     def release_key(self, key_id: int):
@@ -562,10 +599,10 @@ class Server:
     """Continuous-batching bootstrapping server (ygpu_server_*)."""
 
     def __init__(self, device: int = 0, key_budget_bytes: int = 0, lanes: int = 0, max_batch: int = 0,
-                 queue_max: int = 0, max_tenants: int = 0):
+                 queue_max: int = 0, max_tenants: int = 0, max_loads: int = 0):
         self.lib = load_library()
         self.h = ctypes.c_void_p()
-        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, 0, key_budget_bytes)
+        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, max_loads, key_budget_bytes)
         _check(self.lib.ygpu_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.params = []
         self.tenant_params = []
@@ -611,8 +648,16 @@ class Server:
     def submit(self, req: Request):
         _check(self.lib.ygpu_server_submit(self.h, ctypes.byref(req.c)))
 
-    def wait(self, req: Request) -> np.ndarray:
-        _check(self.lib.ygpu_server_wait(self.h, ctypes.byref(req.c)))
+    def wait(self, req: Request, timeout: float | None = None) -> np.ndarray:
+        """Result of a request; with `timeout` (seconds), raises TimeoutError
+        carrying the server's counters if it has not completed by then."""
+        if timeout is None:
+            _check(self.lib.ygpu_server_wait(self.h, ctypes.byref(req.c)))
+            return req.out
+        rc = self.lib.ygpu_server_wait_for(self.h, ctypes.byref(req.c), int(timeout * 1000))
+        if rc == 1:
+            raise TimeoutError(f"request of tenant {req.c.tenant} still pending after {timeout} s; server {self.stats()}")
+        _check(rc)
         return req.out
 
     def stats(self) -> dict:
@@ -646,4 +691,5 @@ __all__ = [
     "Params", "PARAMS_T32_N1024", "PARAMS_T56_N2048", "Q_49P_T32", "Q_61P_T56", "KeySet", "keygen",
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
     "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
+    "save_keys", "key_file_params", "load_key_arrays",
 ]

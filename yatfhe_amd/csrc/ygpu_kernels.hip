@@ -1,8 +1,9 @@
 // ygpu_kernels.hip -- CDNA4 kernels of the bootstrapping path.
 //
 //  k_blind_rotate   rescale to 2N, accumulator init, and all n CMUX steps of the
-//                   blind rotation in one persistent launch; one workgroup per
-//                   ciphertext, the accumulator resident in LDS for all steps.
+//                   blind rotation in one persistent launch; CPW ciphertexts per
+//                   workgroup, two one-wavefront teams per ciphertext, the
+//                   accumulator resident in registers for all n steps.
 //  k_extract_ks     sample extract (index 0) fused into the LWE keyswitch, a
 //                   GEMM-shaped integer reduction over the KSK, tiled over
 //                   ciphertexts so each KSK row is read once per tile.
@@ -39,6 +40,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
 
     __shared__ double tws[TWN];
     __shared__ double scratch[CPW][2 * SCR];
+    // NP == 2: the first prime's residues of each team's column wait here while
+    // the second prime is processed (registers are the binding resource)
+    constexpr int RESN = NP > 1 ? 2 * N : 1;
+    __shared__ double resbuf[CPW][RESN];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int first = grp * CPW;
@@ -138,7 +143,8 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
         }
 
         const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
-        double res[NP > 1 ? 16 : 1];  // prime-0 residues of the owned column (NP == 2)
+        // prime-0 residues of the owned column (NP == 2), at this lane's positions
+        double* res = resbuf[lc] + (NP > 1 ? tm * N : 0);
 
 #pragma unroll
         for (int np = 0; np < NP; np++) {
@@ -162,8 +168,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                 const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + np) * N + tl;
                 const double* __restrict__ kpo = kp + static_cast<size_t>(tm) * NP * N;
                 const double* __restrict__ kpt = kp + static_cast<size_t>(1 - tm) * NP * N;
+                if constexpr (NP == 1) {
 #pragma unroll
-                for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
+                    for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
+                }
                 double x[16];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
@@ -177,6 +185,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                     x[r] = static_cast<double>(dv);
                 }
                 ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+                if constexpr (NP > 1) {
+#pragma unroll
+                    for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
+                }
 #pragma unroll
                 for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
@@ -211,13 +223,14 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                     }
                 }
             } else {
+                // each lane reads back only what it wrote: no barrier needed
                 if (np == 0) {
 #pragma unroll
-                    for (int r = 0; r < 16; r++) res[r] = centre(x[r], pr);
+                    for (int r = 0; r < 16; r++) res[r * T + tl] = centre(x[r], pr);
                 } else {
 #pragma unroll
                     for (int r = 0; r < 16; r++) {
-                        const int64_t val = crt_mod_q(res[r], centre(x[r], pr), pr, a.crt);
+                        const int64_t val = crt_mod_q(res[r * T + tl], centre(x[r], pr), pr, a.crt);
                         accr[r] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accr[r]) + val));
                     }
                 }

@@ -6,24 +6,27 @@
 // (ya_benchmark/blindrotate_serving.cpp:121-353).  On the GPU one bootstrap per
 // launch would leave the device idle, so this server batches instead:
 //
-//  * Requests queue per parameter set.  A dispatcher thread per parameter set
-//    takes everything pending (up to max_batch) whenever one of its lanes is
-//    free, groups it by tenant and launches one key-grouped blind rotation over
-//    the whole batch, then extract + keyswitch per tenant, on the lane's stream.
-//    Requests arriving while batches run queue up and form the next batch, so
-//    several batches of different sizes (and of both torus widths) overlap on
-//    the device.
+//  * Requests queue per tenant within their parameter set.  A dispatcher
+//    thread per parameter set forms a batch whenever one of its lanes is free:
+//    it walks the tenants with pending requests, oldest first, takes the
+//    requests of every tenant whose key is resident and loaded (up to
+//    max_batch) and launches one key-grouped blind rotation over them, then
+//    extract + keyswitch per tenant, on the lane's stream.  Requests arriving
+//    while batches run form later batches, so batches of different sizes (and
+//    of both torus widths) overlap on the device.
 //  * Keys live in one device arena of a fixed byte budget (allocated once: no
-//    hipMalloc/hipFree while serving).  A tenant's key is loaded on first use
-//    from its device-ready form in pinned host memory (the spill tier) with an
-//    asynchronous copy on a dedicated stream; batches that need it wait on the
-//    load's event, so a key is loaded once however many batches ask for it.
+//    hipMalloc/hipFree while serving).  A tenant's key is loaded on demand from
+//    its device-ready form in pinned host memory (the spill tier) by an
+//    asynchronous copy on the parameter set's copy stream; a load watcher
+//    thread marks it ready when the copy completes.  Requests whose key is not
+//    ready stay queued, so batches never wait for other tenants' loads, and a
+//    key is loaded once however many requests ask for it (single flight).
 //    When the arena is full, the contiguous window of idle keys whose newest
 //    use is oldest is evicted (LRU with contiguity); keys used by batches in
-//    flight are never evicted.
-//  * Admission by device memory: a tenant whose key cannot be made resident
-//    (every candidate byte is held by batches in flight) stays queued, the rest
-//    of the batch launches, and the dispatcher retries when a batch completes.
+//    flight or being loaded are never evicted.
+//  * Admission by device memory: a tenant whose key cannot be placed (every
+//    window holds a key in use) stays queued and is retried when a batch or a
+//    load completes.
 //  * A completion thread per parameter set retires lanes in launch order,
 //    copies results to the callers' buffers and wakes waiters.
 //
@@ -40,6 +43,7 @@
 #include <memory>
 #include <random>
 #include <thread>
+#include <unordered_map>
 
 namespace {
 
@@ -60,11 +64,15 @@ struct Tenant {
     int pset = 0;
     int form = 0;
     size_t bytes = 0;      // arena bytes (BSK + KSK planes, each 256-byte aligned)
-    bool resident = false;
+    bool resident = false; // holds arena memory (loaded or loading)
+    bool ready = false;    // the load has completed
     size_t off = 0;        // arena offset when resident
-    hipEvent_t ready = nullptr;
+    hipEvent_t loaded = nullptr;
     int inuse = 0;         // batches in flight holding the key
+    bool fresh = false;    // loaded but not yet used by a batch
     uint64_t last_use = 0;
+    std::deque<ygpu_request*> pending;
+    bool listed = false;   // in its parameter set's `order`
 };
 
 struct Region {
@@ -90,7 +98,6 @@ struct Lane {
     int64_t launch_ns = 0;
     std::vector<ygpu_request*> reqs;
     std::vector<int> tenants;
-    std::vector<int> hit;  // per request: key was resident when the batch formed
 };
 
 struct PSet {
@@ -98,11 +105,15 @@ struct PSet {
     ygpu_params p{};
     size_t tbytes = 4;
     std::vector<Lane> lanes;
-    std::deque<ygpu_request*> queue;
-    std::deque<int> inflight;
-    std::condition_variable cv_work;  // queue grew, a lane freed or memory freed
+    std::deque<int> order;            // tenants with pending requests, oldest first
+    int64_t queued = 0;               // pending requests
+    hipStream_t copy_stream = nullptr;
+    std::deque<int> loads;            // tenants being loaded, in copy order
+    std::deque<int> inflight;         // lanes in launch order
+    std::condition_variable cv_work;  // requests, a free lane, a ready key or free memory
     std::condition_variable cv_done;  // a batch was launched
-    std::thread dispatcher, completer;
+    std::condition_variable cv_load;  // a load was started
+    std::thread dispatcher, completer, watcher;
     bool dispatcher_done = false;
 };
 
@@ -120,7 +131,6 @@ struct ygpu_server {
     char* d_arena = nullptr;
     size_t arena_bytes = 0;
     std::vector<Region> regions;
-    hipStream_t copy_stream = nullptr;
     const double** d_bsk_tab = nullptr;
     const int32_t** d_ksk_tab = nullptr;
     const double** h_bsk_tab = nullptr;   // pinned mirrors (copy sources)
@@ -139,6 +149,10 @@ int server_fail_hip(hipError_t e, const char* what) {
     return fail(YGPU_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void notify_all_psets(ygpu_server* s) {
+    for (auto& q : s->psets) q->cv_work.notify_all();
+}
+
 // ---- arena ------------------------------------------------------------------
 
 void merge_free(ygpu_server* s) {
@@ -153,6 +167,7 @@ void merge_free(ygpu_server* s) {
 void evict(ygpu_server* s, int t) {
     Tenant& tn = s->tenants[t];
     tn.resident = false;
+    tn.ready = false;
     s->st.key_evictions++;
     s->st.resident_bytes -= static_cast<int64_t>(tn.bytes);
     s->st.resident_keys--;
@@ -160,15 +175,29 @@ void evict(ygpu_server* s, int t) {
         if (r.tenant == t) r.tenant = -1;
 }
 
+// A key may be evicted once loaded and used: a freshly loaded key serves its
+// tenant's waiting requests at least once first, so two parameter sets short
+// of memory cannot keep evicting each other's new keys (every load makes
+// progress).
+bool evictable(const Tenant& tn) { return tn.inuse == 0 && tn.ready && !tn.fresh; }
+
+// Eviction preference: keys without waiting requests first, then least
+// recently used.
+uint64_t evict_cost(const Tenant& tn) { return tn.last_use + (tn.pending.empty() ? 0 : (uint64_t{1} << 48)); }
+
 // Finds `bytes` of arena: best-fitting free region, else the contiguous window
 // of free regions and idle keys whose newest key use is oldest.  Returns false
-// when every window holds a key in use.
+// when every window holds a key in use or being loaded.
 bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
-    int best = -1;
-    for (size_t i = 0; i < s->regions.size(); i++) {
-        const Region& r = s->regions[i];
-        if (r.tenant < 0 && r.size >= bytes && (best < 0 || r.size < s->regions[best].size)) best = static_cast<int>(i);
-    }
+    auto best_free = [&]() {
+        int best = -1;
+        for (size_t i = 0; i < s->regions.size(); i++) {
+            const Region& r = s->regions[i];
+            if (r.tenant < 0 && r.size >= bytes && (best < 0 || r.size < s->regions[best].size)) best = static_cast<int>(i);
+        }
+        return best;
+    };
+    int best = best_free();
     if (best < 0) {
         size_t bi = 0, bj = 0;
         uint64_t bcost = UINT64_MAX;
@@ -181,8 +210,8 @@ bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
                 const Region& r = s->regions[j];
                 if (r.tenant >= 0) {
                     const Tenant& tn = s->tenants[r.tenant];
-                    if (tn.inuse > 0) break;
-                    cost = std::max(cost, tn.last_use);
+                    if (!evictable(tn)) break;
+                    cost = std::max(cost, evict_cost(tn));
                     evicted += r.size;
                 }
                 sum += r.size;
@@ -198,10 +227,7 @@ bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
         for (size_t k = bi; k < bj; k++)
             if (s->regions[k].tenant >= 0) evict(s, s->regions[k].tenant);
         merge_free(s);
-        for (size_t i = 0; i < s->regions.size(); i++) {
-            const Region& r = s->regions[i];
-            if (r.tenant < 0 && r.size >= bytes && (best < 0 || r.size < s->regions[best].size)) best = static_cast<int>(i);
-        }
+        best = best_free();
         if (best < 0) return false;  // not reached: the window merged into one free region
     }
     Region r = s->regions[best];
@@ -216,32 +242,22 @@ bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
     return true;
 }
 
-// Makes `t` resident (loading it if needed) and takes a reference for a batch.
-// Returns 1 when memory is not available now, 0 on success, <0 on error.
-// Called with the server lock held.
-int acquire_key(ygpu_server* s, int t, bool* hit) {
+// Starts loading `t` into the arena.  Returns 1 when no memory can be freed
+// now, 0 when the load is enqueued, <0 on error.  Server lock held.
+int start_load(ygpu_server* s, PSet& ps, int t) {
     Tenant& tn = s->tenants[t];
-    tn.last_use = ++s->tick;
-    if (tn.resident) {
-        tn.inuse++;
-        *hit = true;
-        return 0;
-    }
-    *hit = false;
     size_t off = 0;
     if (!arena_alloc(s, tn.bytes, t, &off)) return 1;
     const PinnedKey& f = s->forms[tn.form];
     char* base = s->d_arena + off;
     s->h_bsk_tab[t] = reinterpret_cast<const double*>(base);
     s->h_ksk_tab[t] = reinterpret_cast<const int32_t*>(base + align256(f.bsk_bytes));
-    hipError_t e = hipMemcpyAsync(base, f.bsk, f.bsk_bytes, hipMemcpyHostToDevice, s->copy_stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(base + align256(f.bsk_bytes), f.planes, f.planes_bytes, hipMemcpyHostToDevice, s->copy_stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s->d_bsk_tab + t, s->h_bsk_tab + t, sizeof(void*), hipMemcpyHostToDevice, s->copy_stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s->d_ksk_tab + t, s->h_ksk_tab + t, sizeof(void*), hipMemcpyHostToDevice, s->copy_stream);
-    if (e == hipSuccess) e = hipEventRecord(tn.ready, s->copy_stream);
+    hipStream_t cs = ps.copy_stream;
+    hipError_t e = hipMemcpyAsync(base, f.bsk, f.bsk_bytes, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipMemcpyAsync(base + align256(f.bsk_bytes), f.planes, f.planes_bytes, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_bsk_tab + t, s->h_bsk_tab + t, sizeof(void*), hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->d_ksk_tab + t, s->h_ksk_tab + t, sizeof(void*), hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = hipEventRecord(tn.loaded, cs);
     if (e != hipSuccess) {
         for (Region& r : s->regions)
             if (r.tenant == t) r.tenant = -1;
@@ -249,8 +265,12 @@ int acquire_key(ygpu_server* s, int t, bool* hit) {
         return server_fail_hip(e, "key load");
     }
     tn.resident = true;
+    tn.ready = false;
+    tn.fresh = true;
     tn.off = off;
-    tn.inuse++;
+    tn.last_use = ++s->tick;
+    ps.loads.push_back(t);
+    ps.cv_load.notify_one();
     s->st.key_loads++;
     s->st.bytes_loaded += static_cast<int64_t>(f.bsk_bytes + f.planes_bytes);
     s->st.resident_bytes += static_cast<int64_t>(tn.bytes);
@@ -291,8 +311,8 @@ void lane_free(Lane& L) {
     L = Lane{};
 }
 
-// Stages the lane's requests (already grouped by tenant) and enqueues the
-// pipeline on the lane's stream.  Runs without the server lock.
+// Stages the lane's requests (grouped by tenant, keys loaded) and enqueues
+// the pipeline on the lane's stream.  Runs without the server lock.
 int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
     const ygpu_params& p = ps.p;
     const int count = static_cast<int>(L.reqs.size());
@@ -304,7 +324,6 @@ int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
         std::memcpy(static_cast<char*>(L.h_v) + c * vbytes, r->v, vbytes);
         L.h_slots[c] = r->tenant;
     }
-    for (int t : L.tenants) YG_HIP(hipStreamWaitEvent(L.stream, s->tenants[t].ready, 0));
     YG_HIP(hipMemcpyAsync(L.d_in, L.h_in, count * lwe, hipMemcpyHostToDevice, L.stream));
     YG_HIP(hipMemcpyAsync(L.d_v, L.h_v, count * vbytes, hipMemcpyHostToDevice, L.stream));
     YG_HIP(hipMemcpyAsync(L.d_slots, L.h_slots, count * sizeof(int32_t), hipMemcpyHostToDevice, L.stream));
@@ -333,94 +352,147 @@ void finish_request(ygpu_request* r, int status, int64_t launch, int64_t done, i
     r->status = status;
 }
 
+// Whether anything in progress will free arena memory: batches in flight,
+// loads in flight, or loaded keys waiting for their first batch.
+bool any_busy(const ygpu_server* s) {
+    for (const auto& q : s->psets) {
+        if (!q->inflight.empty() || !q->loads.empty()) return true;
+        for (const auto& l : q->lanes)
+            if (l.busy) return true;
+    }
+    for (const Tenant& t : s->tenants)
+        if (t.fresh) return true;
+    return false;
+}
+
+// Fails every pending request of `t` (server lock held).
+void fail_pending(ygpu_server* s, PSet& ps, int t, int rc) {
+    const int64_t now = now_ns();
+    Tenant& tn = s->tenants[t];
+    tn.fresh = false;
+    for (ygpu_request* r : tn.pending) finish_request(r, rc, now, now, 0);
+    ps.queued -= static_cast<int64_t>(tn.pending.size());
+    s->st.failed += static_cast<int64_t>(tn.pending.size());
+    tn.pending.clear();
+}
+
 void dispatcher_loop(ygpu_server* s, PSet* ps) {
     hipSetDevice(s->cfg.device);
     std::unique_lock<std::mutex> lk(s->mu);
-    for (;;) {
-        auto free_lane = [&]() -> int {
-            for (size_t i = 0; i < ps->lanes.size(); i++)
-                if (!ps->lanes[i].busy) return static_cast<int>(i);
-            return -1;
-        };
-        ps->cv_work.wait(lk, [&] { return s->stop || (!ps->queue.empty() && free_lane() >= 0); });
-        if (s->stop) break;
-        const int li = free_lane();
-        Lane& L = ps->lanes[li];
-        // everything pending, up to one batch, grouped by tenant (stable)
-        std::vector<ygpu_request*> take;
-        while (!ps->queue.empty() && static_cast<int>(take.size()) < s->cfg.max_batch) {
-            take.push_back(ps->queue.front());
-            ps->queue.pop_front();
-        }
-        std::stable_sort(take.begin(), take.end(), [](const ygpu_request* a, const ygpu_request* b) { return a->tenant < b->tenant; });
-        L.reqs.clear();
-        L.tenants.clear();
-        L.hit.clear();
-        std::vector<ygpu_request*> deferred;
-        int err = 0;
-        for (size_t c0 = 0; c0 < take.size();) {
-            size_t c1 = c0;
-            while (c1 < take.size() && take[c1]->tenant == take[c0]->tenant) c1++;
-            const int t = take[c0]->tenant;
-            bool hit = false;
-            const int rc = err ? 1 : acquire_key(s, t, &hit);
-            if (rc < 0) err = rc;
-            if (rc == 0) {
-                L.tenants.push_back(t);
-                for (size_t c = c0; c < c1; c++) {
-                    L.reqs.push_back(take[c]);
-                    L.hit.push_back(hit ? 1 : 0);
-                }
-            } else {
-                deferred.insert(deferred.end(), take.begin() + c0, take.begin() + c1);
-            }
-            c0 = c1;
-        }
-        // requests whose key could not be placed go back to the front, in order
-        std::stable_sort(deferred.begin(), deferred.end(), [](const ygpu_request* a, const ygpu_request* b) { return a->seq < b->seq; });
-        for (auto it = deferred.rbegin(); it != deferred.rend(); ++it) ps->queue.push_front(*it);
-        if (err) {
-            // a key load failed: fail this batch's requests and the deferred ones
-            const int64_t t = now_ns();
-            for (ygpu_request* r : L.reqs) finish_request(r, err, t, t, 0);
-            for (int tn : L.tenants) s->tenants[tn].inuse--;
-            L.reqs.clear();
-            L.tenants.clear();
-            s->cv_req.notify_all();
+    auto free_lane = [&]() -> int {
+        for (size_t i = 0; i < ps->lanes.size(); i++)
+            if (!ps->lanes[i].busy) return static_cast<int>(i);
+        return -1;
+    };
+    while (!s->stop) {
+        if (ps->order.empty()) {
+            ps->cv_work.wait(lk);
             continue;
         }
-        if (L.reqs.empty()) {
-            s->st.memory_waits++;
-            bool any_inflight = false;
-            for (auto& q : s->psets) any_inflight |= !q->inflight.empty();
-            for (auto& q : s->psets)
-                for (auto& l : q->lanes) any_inflight |= l.busy;
-            if (!any_inflight) {
-                // nothing will free memory: cannot happen for keys that fit the budget
-                const int64_t t = now_ns();
-                while (!ps->queue.empty()) {
-                    finish_request(ps->queue.front(), fail(YGPU_ERR_OUT_OF_MEMORY, "key does not fit the key budget"), t, t, 0);
-                    ps->queue.pop_front();
+        // 1. start loads for the oldest tenants whose keys are absent
+        bool mem_blocked = false;
+        for (size_t k = 0; k < ps->order.size() && static_cast<int>(ps->loads.size()) < s->cfg.max_loads; k++) {
+            const int t = ps->order[k];
+            if (s->tenants[t].resident) continue;
+            const int rc = start_load(s, *ps, t);
+            if (rc == 1) {
+                mem_blocked = true;
+                break;
+            }
+            if (rc < 0) fail_pending(s, *ps, t, rc);
+        }
+        // 2. a batch of every ready tenant's requests, oldest tenants first
+        const int li = free_lane();
+        Lane* L = li >= 0 ? &ps->lanes[li] : nullptr;
+        if (L) {
+            L->reqs.clear();
+            L->tenants.clear();
+            std::deque<int> keep;
+            for (int t : ps->order) {
+                Tenant& tn = s->tenants[t];
+                const int room = s->cfg.max_batch - static_cast<int>(L->reqs.size());
+                if (tn.pending.empty()) {
+                    tn.listed = false;
+                    continue;
                 }
+                if (!tn.ready || room == 0) {
+                    keep.push_back(t);
+                    continue;
+                }
+                const int take = std::min<int>(room, static_cast<int>(tn.pending.size()));
+                for (int c = 0; c < take; c++) {
+                    L->reqs.push_back(tn.pending.front());
+                    tn.pending.pop_front();
+                }
+                tn.inuse++;
+                tn.fresh = false;
+                tn.last_use = ++s->tick;
+                L->tenants.push_back(t);
+                if (tn.pending.empty()) tn.listed = false;
+                else keep.push_back(t);
+            }
+            ps->order.swap(keep);
+            if (!L->reqs.empty()) {
+                ps->queued -= static_cast<int64_t>(L->reqs.size());
+                L->busy = true;
+                L->launch_ns = now_ns();
+                s->st.batches++;
+                s->st.batched_cts += static_cast<int64_t>(L->reqs.size());
+                lk.unlock();
+                L->rc = launch_lane(s, *ps, *L);
+                lk.lock();
+                ps->inflight.push_back(li);
+                ps->cv_done.notify_all();
+                continue;  // look for more work at once
+            }
+        }
+        if (mem_blocked) {
+            s->st.memory_waits++;
+            if (!any_busy(s)) {
+                // nothing in flight will free memory: keys that fit the budget never get here
+                for (int t : ps->order) fail_pending(s, *ps, t, fail(YGPU_ERR_OUT_OF_MEMORY, "key does not fit the key budget"));
+                for (int t : ps->order) s->tenants[t].listed = false;
+                ps->order.clear();
                 s->cv_req.notify_all();
                 continue;
             }
-            ps->cv_work.wait(lk);  // woken by a completion
-            continue;
         }
-        L.busy = true;
-        L.launch_ns = now_ns();
-        s->st.batches++;
-        s->st.batched_cts += static_cast<int64_t>(L.reqs.size());
-        for (int h : L.hit) s->st.request_hits += h;
-        lk.unlock();
-        L.rc = launch_lane(s, *ps, L);
-        lk.lock();
-        ps->inflight.push_back(li);
-        ps->cv_done.notify_all();
+        // nothing to launch now: wait for a request, a lane, a key or memory
+        ps->cv_work.wait(lk);
     }
     ps->dispatcher_done = true;
     ps->cv_done.notify_all();
+    ps->cv_load.notify_all();
+}
+
+// Marks keys ready as their copies complete (copies run in order on the
+// parameter set's copy stream).
+void watcher_loop(ygpu_server* s, PSet* ps) {
+    hipSetDevice(s->cfg.device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        ps->cv_load.wait(lk, [&] { return !ps->loads.empty() || ps->dispatcher_done; });
+        if (ps->loads.empty()) break;
+        const int t = ps->loads.front();
+        hipEvent_t ev = s->tenants[t].loaded;
+        lk.unlock();
+        const hipError_t e = hipEventSynchronize(ev);
+        lk.lock();
+        ps->loads.pop_front();
+        Tenant& tn = s->tenants[t];
+        if (e != hipSuccess) {
+            // the key never arrived: release its memory and fail its requests
+            tn.ready = true;
+            tn.fresh = false;
+            evict(s, t);
+            merge_free(s);
+            fail_pending(s, *ps, t, server_fail_hip(e, "key load"));
+            s->cv_req.notify_all();
+        } else {
+            tn.ready = true;
+        }
+        notify_all_psets(s);
+    }
 }
 
 void completer_loop(ygpu_server* s, PSet* ps) {
@@ -455,7 +527,7 @@ void completer_loop(ygpu_server* s, PSet* ps) {
         L.tenants.clear();
         L.busy = false;
         s->cv_req.notify_all();
-        for (auto& q : s->psets) q->cv_work.notify_all();
+        notify_all_psets(s);
     }
 }
 
@@ -464,6 +536,7 @@ int start_threads(ygpu_server* s) {
     for (auto& ps : s->psets) {
         ps->dispatcher = std::thread(dispatcher_loop, s, ps.get());
         ps->completer = std::thread(completer_loop, s, ps.get());
+        ps->watcher = std::thread(watcher_loop, s, ps.get());
     }
     s->started = true;
     return YGPU_OK;
@@ -497,6 +570,7 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 4096;
     if (s->cfg.queue_max <= 0) s->cfg.queue_max = 1 << 20;
     if (s->cfg.max_tenants <= 0) s->cfg.max_tenants = 4096;
+    if (s->cfg.max_loads <= 0) s->cfg.max_loads = 4;
     if (s->cfg.lanes > 16 || s->cfg.max_batch > (1 << 16) || s->cfg.max_tenants > (1 << 20))
         return fail(YGPU_ERR_INVALID_ARG, "lanes <= 16, max_batch <= 65536 and max_tenants <= 2^20 required");
     YG_HIP(hipSetDevice(cfg->device));
@@ -512,7 +586,6 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
     s->cfg.key_budget_bytes = static_cast<int64_t>(budget);
     s->st.key_budget_bytes = static_cast<int64_t>(budget);
     const size_t tab = sizeof(void*) * s->cfg.max_tenants;
-    YG_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&s->d_bsk_tab), tab));
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&s->d_ksk_tab), tab));
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_bsk_tab), tab, hipHostMallocDefault));
@@ -535,6 +608,10 @@ int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* p
         if (rc) return rc;
         ps->p = ps->ctx->p;
         ps->tbytes = ps->ctx->torus_bytes();
+        if (hipStreamCreateWithFlags(&ps->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+            ygpu_destroy(ps->ctx);
+            return fail(YGPU_ERR_HIP, "copy stream");
+        }
         ps->lanes.resize(s->cfg.lanes);
         for (Lane& L : ps->lanes) {
             rc = lane_alloc(s, *ps, L);
@@ -589,7 +666,7 @@ int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coe
         Tenant t;
         t.pset = pset;
         t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
-        e = hipEventCreateWithFlags(&t.ready, hipEventDisableTiming);
+        e = hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming);
         if (e != hipSuccess) {
             hipHostFree(f.bsk);
             hipHostFree(f.planes);
@@ -616,7 +693,7 @@ int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* t
         t.form = s->tenants[src_tenant].form;
         t.bytes = s->tenants[src_tenant].bytes;
         YG_HIP(hipSetDevice(s->cfg.device));
-        YG_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+        YG_HIP(hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming));
         *tenant = static_cast<int32_t>(s->tenants.size());
         s->tenants.push_back(t);
         return YGPU_OK;
@@ -639,9 +716,10 @@ int ygpu_server_submit(ygpu_server* s, ygpu_request* req) {
         std::unique_lock<std::mutex> lk(s->mu);
         if (s->stop) return fail(YGPU_ERR_INVALID_ARG, "server is shutting down");
         if (req->tenant < 0 || req->tenant >= static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown tenant");
-        PSet& ps = *s->psets[s->tenants[req->tenant].pset];
+        Tenant& tn = s->tenants[req->tenant];
+        PSet& ps = *s->psets[tn.pset];
         s->st.submitted++;
-        if (static_cast<int64_t>(ps.queue.size()) >= s->cfg.queue_max) {
+        if (ps.queued >= s->cfg.queue_max) {
             s->st.shed++;
             req->status = YGPU_ERR_BUSY;
             return fail(YGPU_ERR_BUSY, "queue full: request shed");
@@ -650,7 +728,14 @@ int ygpu_server_submit(ygpu_server* s, ygpu_request* req) {
         req->status = 1;
         if (req->arrival_ns == 0) req->arrival_ns = now_ns();
         req->seq = ++s->tick;
-        ps.queue.push_back(req);
+        req->key_hit = tn.ready ? 1 : 0;
+        s->st.request_hits += req->key_hit;
+        tn.pending.push_back(req);
+        ps.queued++;
+        if (!tn.listed) {
+            tn.listed = true;
+            ps.order.push_back(req->tenant);
+        }
         ps.cv_work.notify_one();
         return YGPU_OK;
     });
@@ -661,6 +746,15 @@ int ygpu_server_wait(ygpu_server* s, ygpu_request* req) {
         if (!req) return fail(YGPU_ERR_INVALID_ARG, "null request");
         std::unique_lock<std::mutex> lk(s->mu);
         s->cv_req.wait(lk, [&] { return req->status != 1; });
+        return static_cast<int>(req->status);
+    });
+}
+
+int ygpu_server_wait_for(ygpu_server* s, ygpu_request* req, int32_t timeout_ms) {
+    return with_server(s, [&] {
+        if (!req) return fail(YGPU_ERR_INVALID_ARG, "null request");
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_req.wait_for(lk, std::chrono::milliseconds(std::max(0, timeout_ms)), [&] { return req->status != 1; });
         return static_cast<int>(req->status);
     });
 }
@@ -684,6 +778,7 @@ int ygpu_server_destroy(ygpu_server* s) {
     for (auto& ps : s->psets) {
         if (ps->dispatcher.joinable()) ps->dispatcher.join();
         if (ps->completer.joinable()) ps->completer.join();
+        if (ps->watcher.joinable()) ps->watcher.join();
     }
     hipSetDevice(s->cfg.device);
     {
@@ -691,23 +786,28 @@ int ygpu_server_destroy(ygpu_server* s) {
         std::lock_guard<std::mutex> g(s->mu);
         const int64_t t = now_ns();
         for (auto& ps : s->psets) {
-            for (ygpu_request* r : ps->queue) finish_request(r, YGPU_ERR_INVALID_ARG, t, t, 0);
-            ps->queue.clear();
+            for (int tn : ps->order) {
+                for (ygpu_request* r : s->tenants[tn].pending) finish_request(r, YGPU_ERR_INVALID_ARG, t, t, 0);
+                s->tenants[tn].pending.clear();
+            }
+            ps->order.clear();
         }
         s->cv_req.notify_all();
     }
-    if (s->copy_stream) hipStreamSynchronize(s->copy_stream);
     for (auto& ps : s->psets) {
+        if (ps->copy_stream) {
+            hipStreamSynchronize(ps->copy_stream);
+            hipStreamDestroy(ps->copy_stream);
+        }
         for (Lane& L : ps->lanes) lane_free(L);
         ygpu_destroy(ps->ctx);
     }
     for (Tenant& t : s->tenants)
-        if (t.ready) hipEventDestroy(t.ready);
+        if (t.loaded) hipEventDestroy(t.loaded);
     for (PinnedKey& f : s->forms) {
         if (f.bsk) hipHostFree(f.bsk);
         if (f.planes) hipHostFree(f.planes);
     }
-    if (s->copy_stream) hipStreamDestroy(s->copy_stream);
     if (s->d_arena) hipFree(s->d_arena);
     if (s->d_bsk_tab) hipFree(s->d_bsk_tab);
     if (s->d_ksk_tab) hipFree(s->d_ksk_tab);
@@ -753,8 +853,12 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             const double u = uni(zgen);
             t = std::min<int>(static_cast<int>(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin()), c->n_tenants - 1);
         }
-        std::vector<ygpu_request> reqs(total);
-        std::vector<int32_t> outs(static_cast<size_t>(total) * maxCols);
+        // request records and outputs are owned by shared state that outlives
+        // this call if requests are still pending at the deadline
+        auto reqsHold = std::make_shared<std::vector<ygpu_request>>(total);
+        auto outsHold = std::make_shared<std::vector<int32_t>>(static_cast<size_t>(total) * maxCols);
+        std::vector<ygpu_request>& reqs = *reqsHold;
+        std::vector<int32_t>& outs = *outsHold;
         auto make = [&](int64_t i) {
             ygpu_request& r = reqs[i];
             std::memset(&r, 0, sizeof(r));
@@ -770,7 +874,14 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             if (rc && rc != YGPU_ERR_BUSY) return rc;
         }
         for (int64_t i = 0; i < c->warm; i++)
-            if (reqs[i].status != YGPU_ERR_BUSY) ygpu_server_wait(s, &reqs[i]);
+            if (reqs[i].status != YGPU_ERR_BUSY && ygpu_server_wait_for(s, &reqs[i], 120000) == 1) {
+                static std::mutex graveMu;
+                static std::vector<std::shared_ptr<void>> grave;
+                std::lock_guard<std::mutex> g(graveMu);
+                grave.push_back(reqsHold);
+                grave.push_back(outsHold);
+                return fail(YGPU_ERR_BUSY, "load generator: warm-up requests still pending after 120 s");
+            }
         ygpu_server_stats s0{};
         ygpu_server_stats_get(s, &s0);
         // measured open loop
@@ -801,9 +912,25 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
         std::vector<double> lat, queue;
         lat.reserve(c->requests);
         queue.reserve(c->requests);
+        // a run that has not drained 120 s after its last arrival is stuck
+        const int64_t deadline = now_ns() + int64_t{120} * 1000000000;
         for (int64_t i = c->warm; i < total; i++) {
             if (!submitted[i]) continue;
-            const int rc = ygpu_server_wait(s, &reqs[i]);
+            const int64_t left_ms = (deadline - now_ns()) / 1000000;
+            const int rc = ygpu_server_wait_for(s, &reqs[i], static_cast<int32_t>(std::max<int64_t>(0, left_ms)));
+            if (rc == 1) {
+                int64_t pending = 0;
+                for (int64_t j = i; j < total; j++) pending += submitted[j] && reqs[j].status == 1;
+                // the server still references these buffers: keep them alive
+                static std::mutex graveMu;
+                static std::vector<std::shared_ptr<void>> grave;
+                {
+                    std::lock_guard<std::mutex> g(graveMu);
+                    grave.push_back(reqsHold);
+                    grave.push_back(outsHold);
+                }
+                return fail(YGPU_ERR_BUSY, "load generator: " + std::to_string(pending) + " requests still pending 120 s after the last arrival");
+            }
             if (rc) {
                 errors++;
                 continue;
@@ -850,8 +977,8 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
         const int64_t nb = s1.batches - s0.batches;
         out->batches = nb;
         out->mean_batch = nb ? static_cast<double>(s1.batched_cts - s0.batched_cts) / nb : 0;
-        const int64_t nc = s1.batched_cts - s0.batched_cts;
-        out->hit_rate = nc ? static_cast<double>(s1.request_hits - s0.request_hits) / nc : 0;
+        const int64_t ns = s1.submitted - s0.submitted;
+        out->hit_rate = ns ? static_cast<double>(s1.request_hits - s0.request_hits) / ns : 0;
         out->key_loads = s1.key_loads - s0.key_loads;
         out->key_evictions = s1.key_evictions - s0.key_evictions;
         out->bytes_loaded = s1.bytes_loaded - s0.bytes_loaded;
