@@ -33,6 +33,12 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The serving leg runs two parameter sets with three batch lanes and a key-copy
+# stream each; HIP multiplexes streams over GPU_MAX_HW_QUEUES hardware queues
+# (4 by default), and a short batch sharing a queue with a long one waits for
+# it.  The HIP runtime reads this when it initialises (first library call).
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "PBS/sec at N=1024 bit-exact, batched vs plain on GPU; p50/p99 serving latency"
 BATCH = 4096
@@ -240,7 +246,18 @@ def leg_config2(yg, args, peak):
         br_n, br_ms = ctx.timing_get("k_blind_rotate")
         ctx.timing(False)
         out = d_out.download(np.zeros((BATCH_T56, p.n + 1), np.int32))
-        for b in (d_out, d_v, d_lwe):
+        # plain form: one ciphertext per call
+        d1 = ctx.alloc((p.n + 1) * 4)
+        o1 = ctx.alloc((p.n + 1) * 4)
+        lat = []
+        for c in range(max(2, args.plain_calls // 10)):
+            d1.upload(cts[c])
+            t = time.perf_counter()
+            ctx.pbs_plain_device(kid, d1, d_v, o1)
+            ctx.sync()
+            lat.append(time.perf_counter() - t)
+        lat = lat[1:]
+        for b in (d_out, d_v, d_lwe, d1, o1):
             b.free()
     ph = yg.lwe_decrypt_phase(out, keys.lwe_key).astype(np.int64)
     ok = bool(np.array_equal(np.round(ph / (1 << 32) * 16).astype(np.int64) % 16, lut[msgs] % 16))
@@ -257,6 +274,8 @@ def leg_config2(yg, args, peak):
                      "per_unit": f"{mm} modmul (mod q56) per PBS; each is two binary64 products, one per CRT prime",
                      "avg_launch_ms": round(br_s * 1e3, 2)},
         "lut_decrypt_check": ok,
+        "plain": {"value": round(len(lat) / sum(lat), 2), "unit": "PBS/s", "calls": len(lat),
+                  "p50_ms": round(pct(lat, 50) * 1e3, 2)},
     }
 
 
@@ -336,7 +355,6 @@ def leg_serving(yg, args, peak):
     perm = np.random.default_rng(5).permutation(n_t)
     is56 = perm < int(round(0.3 * n_t))          # 30 % of tenants at TORUS56
     mats32, mats56 = 8, 4                         # distinct key materials per width
-    budget = int(args.serving_budget_gb * (1 << 30))
     lut = np.random.default_rng(6).integers(0, 16, 16)
     v32 = yg.sign_test_polynomial(p32)
     v56 = yg.lut_test_polynomial(p56, lut, 16)
@@ -356,59 +374,84 @@ def leg_serving(yg, args, peak):
             kid = ctx.upload_key(k)
             exp56.append(ctx.pbs(kid, c, v56)[0])
             ctx.release_key(kid)
-    sweep = []
-    with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16) as srv:
-        ps32 = srv.add_params(p32)
-        ps56 = srv.add_params(p56)
-        # tenant id == popularity rank (the load generator draws rank r as tenant r);
-        # materials are assigned round-robin within each width, registered at first use
-        first = {}
-        inputs, luts, expected = [], [], []
-        c32 = c56 = 0
-        for t in range(n_t):
-            if is56[t]:
-                w, m = "56", c56 % mats56
-                c56 += 1
-            else:
-                w, m = "32", c32 % mats32
-                c32 += 1
-            if (w, m) not in first:
-                tid = srv.add_tenant(ps56 if w == "56" else ps32, keys56[m] if w == "56" else keys32[m])
-                first[(w, m)] = tid
-            else:
-                tid = srv.add_tenant_shared(first[(w, m)])
-            assert tid == t
-            inputs.append(in56[m] if w == "56" else in32[m])
-            luts.append(v56 if w == "56" else v32)
-            expected.append(exp56[m] if w == "56" else exp32[m])
-        del keys32, keys56
-        share56 = sum((1.0 / (t + 1) ** 1.1) for t in range(n_t) if is56[t]) / sum(1.0 / (t + 1) ** 1.1 for t in range(n_t))
-        for rate in args.serving_rates:
-            n_req = int(max(500, rate * args.serving_seconds))
-            r = srv.loadgen(rate=rate, requests=n_req, inputs=inputs, luts=luts, expected=expected,
-                            warm=256 if not sweep else 0, zipf_s=1.1, seed=5, slo_ms=100.0)
-            sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()})
-            print(f"serving rate {rate}: {r['throughput_rps']:.0f} PBS/s p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms "
-                  f"shed {r['shed']} mismatches {r['mismatches']}", file=sys.stderr, flush=True)
-        st = srv.stats()
     m32, m56 = modmul_per_pbs(p32), modmul_per_pbs(p56)
+    key_dev = {"32": 82_608_128 + 0, "56": 340_475_904}  # engine-form BSK + KSK planes per key
+    pool = sum(key_dev["56" if is56[t] else "32"] for t in range(n_t))
+    share56 = sum((1.0 / (t + 1) ** 1.1) for t in range(n_t) if is56[t]) / sum(1.0 / (t + 1) ** 1.1 for t in range(n_t))
     mm_avg = (1 - share56) * m32 + share56 * m56
-    ok = [s for s in sweep if s["shed"] == 0 and s["errors"] == 0]
-    best = max(ok, key=lambda s: s["throughput_rps"]) if ok else None
+
+    def sweep_at(budget_gb):
+        budget = int(budget_gb * (1 << 30))
+        sweep = []
+        with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16) as srv:
+            ps32 = srv.add_params(p32)
+            ps56 = srv.add_params(p56)
+            # tenant id == popularity rank (the load generator draws rank r as tenant r);
+            # materials are assigned round-robin within each width, registered at first use
+            first = {}
+            inputs, luts, expected = [], [], []
+            c32 = c56 = 0
+            for t in range(n_t):
+                if is56[t]:
+                    w, m = "56", c56 % mats56
+                    c56 += 1
+                else:
+                    w, m = "32", c32 % mats32
+                    c32 += 1
+                if (w, m) not in first:
+                    tid = srv.add_tenant(ps56 if w == "56" else ps32, keys56[m] if w == "56" else keys32[m])
+                    first[(w, m)] = tid
+                else:
+                    tid = srv.add_tenant_shared(first[(w, m)])
+                assert tid == t
+                inputs.append(in56[m] if w == "56" else in32[m])
+                luts.append(v56 if w == "56" else v32)
+                expected.append(exp56[m] if w == "56" else exp32[m])
+            # warm start: the most popular tenants' keys, as many as the budget holds
+            used = 0
+            prefetched = 0
+            for t in range(n_t):
+                need = key_dev["56" if is56[t] else "32"] + 512
+                if used + need > budget:
+                    break
+                if not srv.prefetch(t):
+                    break
+                used += need
+                prefetched += 1
+            for rate in args.serving_rates:
+                n_req = int(max(500, rate * args.serving_seconds))
+                r = srv.loadgen(rate=rate, requests=n_req, inputs=inputs, luts=luts, expected=expected,
+                                warm=0, zipf_s=1.1, seed=5, slo_ms=100.0)
+                sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()})
+                print(f"serving budget {budget_gb} GiB rate {rate}: {r['throughput_rps']:.0f} PBS/s p50 {r['p50_ms']:.2f} "
+                      f"ms p99 {r['p99_ms']:.2f} ms (T32 p50 {r['pset_p50_ms'][0]:.1f}, T56 p50 {r['pset_p50_ms'][1]:.1f}) "
+                      f"hit {r['hit_rate']:.3f} batch {r['mean_batch']:.1f} shed {r['shed']} mismatches {r['mismatches']}",
+                      file=sys.stderr, flush=True)
+            st = srv.stats()
+        ok = [x for x in sweep if x["shed"] == 0 and x["errors"] == 0 and x["throughput_rps"] >= 0.9 * x["offered_rps"]]
+        best = max(ok, key=lambda x: x["throughput_rps"]) if ok else None
+        return {
+            "key_budget_gib": budget_gb,
+            "keys_prefetched": prefetched,
+            "sweep": sweep,
+            "max_sustained": None if best is None else {
+                "offered_rps": best["offered_rps"], "throughput_rps": best["throughput_rps"],
+                "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+                "roofline_frac": round(best["throughput_rps"] * mm_avg / peak, 4)},
+            "all_results_bit_exact": all(x["mismatches"] == 0 and x["errors"] == 0 for x in sweep),
+            "server_stats": st,
+        }
+
+    runs = [sweep_at(b) for b in args.serving_budgets_gb]
     return {
         "workload": f"Poisson arrivals over 256 tenants, Zipf(1.1) popularity (seed 5), 30 % of tenants TORUS56 "
                     f"n=742 N=2048 with a 16-entry LUT, 70 % TORUS32 n=630 N=1024 sign gate (TORUS56 share of "
                     f"traffic {share56:.3f}); {mats32}+{mats56} distinct key materials shared round-robin by the "
-                    f"tenants, each tenant cached on the device separately; device key budget "
-                    f"{args.serving_budget_gb} GiB against a key pool of "
-                    f"{(int((~is56).sum()) * 82.6e6 + int(is56.sum()) * 340.5e6) / 1e9:.1f} GB; full PBS per request",
-        "sweep": sweep,
-        "max_sustained": None if best is None else {
-            "offered_rps": best["offered_rps"], "throughput_rps": best["throughput_rps"],
-            "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
-            "roofline_frac": round(best["throughput_rps"] * mm_avg / peak, 4)},
-        "all_results_bit_exact": all(s["mismatches"] == 0 for s in sweep),
-        "server_stats": st,
+                    f"tenants, each tenant cached on the device separately; key pool {pool / 1e9:.1f} GB in device "
+                    f"form; device key budgets {args.serving_budgets_gb} GiB (the whole pool resident, and a budget "
+                    f"below the pool with loads from pinned host memory); {args.serving_seconds} s of arrivals per "
+                    f"rate; full PBS per request; latency from scheduled arrival to result in host memory",
+        "runs": runs,
     }
 
 
@@ -421,7 +464,8 @@ def main():
     ap.add_argument("--plain-calls", type=int, default=50)
     ap.add_argument("--legs", default=",".join(LEGS), help="comma-separated subset of " + ",".join(LEGS))
     ap.add_argument("--serving-seconds", type=float, default=2.0)
-    ap.add_argument("--serving-budget-gb", type=float, default=12.0)
+    ap.add_argument("--serving-budgets-gb", default="48,12",
+                    help="device key budgets (GiB) of the serving sweeps; 48 holds the whole key pool")
     ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -434,6 +478,7 @@ def main():
         if l not in LEGS:
             sys.exit(f"unknown leg {l}")
     args.serving_rates = [float(r) for r in args.serving_rates.split(",")]
+    args.serving_budgets_gb = [float(b) for b in args.serving_budgets_gb.split(",")]
 
     import yatfhe_amd as yg
 

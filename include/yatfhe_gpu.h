@@ -290,6 +290,9 @@ int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coe
  * device on its own (separate residency, loads and evictions). */
 int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* tenant);
 int ygpu_server_tenant_params(ygpu_server* s, int32_t tenant, int32_t* pset);
+/* Loads a tenant's key into the device cache now (evicting idle keys if
+ * needed) and waits for it; YGPU_ERR_BUSY when no memory can be freed. */
+int ygpu_server_prefetch(ygpu_server* s, int32_t tenant);
 /* Asynchronous submit (thread-safe).  YGPU_ERR_BUSY when the queue is full. */
 int ygpu_server_submit(ygpu_server* s, ygpu_request* req);
 /* Blocks until the request completes; returns its status. */
@@ -326,6 +329,9 @@ typedef struct ygpu_loadgen_result {
     int64_t batches;
     double mean_batch, hit_rate;
     int64_t key_loads, key_evictions, bytes_loaded, memory_waits;
+    double hit_p50_ms, hit_p99_ms;      /* requests whose key was resident on arrival */
+    double miss_p50_ms, miss_p99_ms;    /* requests that waited for a key load */
+    double pset_p50_ms[4], pset_p99_ms[4];  /* by parameter set (first four) */
 } ygpu_loadgen_result;
 
 int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* cfg, ygpu_loadgen_result* out);

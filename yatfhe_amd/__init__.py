@@ -87,7 +87,9 @@ class _LoadgenResult(ctypes.Structure):
                 + [(f, ctypes.c_double) for f in ("mean_ms", "p50_ms", "p90_ms", "p99_ms", "p99_9_ms", "max_ms",
                                                    "queue_p50_ms", "queue_p99_ms", "slo_violation_rate")]
                 + [("batches", ctypes.c_int64), ("mean_batch", ctypes.c_double), ("hit_rate", ctypes.c_double)]
-                + [(f, ctypes.c_int64) for f in ("key_loads", "key_evictions", "bytes_loaded", "memory_waits")])
+                + [(f, ctypes.c_int64) for f in ("key_loads", "key_evictions", "bytes_loaded", "memory_waits")]
+                + [(f, ctypes.c_double) for f in ("hit_p50_ms", "hit_p99_ms", "miss_p50_ms", "miss_p99_ms")]
+                + [("pset_p50_ms", ctypes.c_double * 4), ("pset_p99_ms", ctypes.c_double * 4)])
 
 
 @dataclass(frozen=True)
@@ -188,6 +190,7 @@ def load_library(path: str | None = None):
         "ygpu_server_add_tenant": (c_int, [c_vp, c_i32, c_vp, c_vp, P(c_i32)]),
         "ygpu_server_add_tenant_shared": (c_int, [c_vp, c_i32, P(c_i32)]),
         "ygpu_server_tenant_params": (c_int, [c_vp, c_i32, P(c_i32)]),
+        "ygpu_server_prefetch": (c_int, [c_vp, c_i32]),
         "ygpu_server_submit": (c_int, [c_vp, P(_Request)]),
         "ygpu_server_wait": (c_int, [c_vp, P(_Request)]),
         "ygpu_server_wait_for": (c_int, [c_vp, P(_Request), c_i32]),
@@ -218,6 +221,7 @@ def exported_symbols():
         "ygpu_graph_create", "ygpu_graph_launch", "ygpu_graph_destroy", "ygpu_blind_rotate_scaled", "ygpu_keyswitch",
         "ygpu_sample_extract", "ygpu_server_create", "ygpu_server_destroy", "ygpu_server_add_params",
         "ygpu_server_set_tile", "ygpu_server_add_tenant", "ygpu_server_add_tenant_shared", "ygpu_server_tenant_params",
+        "ygpu_server_prefetch",
         "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_wait_for", "ygpu_server_stats_get", "ygpu_server_loadgen",
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
     ]
@@ -642,6 +646,14 @@ class Server:
         self.tenant_params.append(self.tenant_params[src])
         return t.value
 
+    def prefetch(self, tenant: int) -> bool:
+        """Make a tenant's key resident now; False when no memory can be freed."""
+        rc = self.lib.ygpu_server_prefetch(self.h, tenant)
+        if rc == -7:
+            return False
+        _check(rc)
+        return True
+
     def request(self, tenant: int, lwe_in: np.ndarray, v: np.ndarray) -> Request:
         return Request(self.params[self.tenant_params[tenant]], tenant, lwe_in, v)
 
@@ -684,7 +696,11 @@ class Server:
                              slo_ms)
         res = _LoadgenResult()
         _check(self.lib.ygpu_server_loadgen(self.h, ctypes.byref(cfg), ctypes.byref(res)))
-        return {f: getattr(res, f) for f, _ in _LoadgenResult._fields_}
+        out = {}
+        for f, _ in _LoadgenResult._fields_:
+            v = getattr(res, f)
+            out[f] = list(v)[:len(self.params)] if f.startswith("pset_") else v
+        return out
 
 
 __all__ = [

@@ -97,6 +97,8 @@ struct ygpu_ctx {
     int32_t* d_out1 = nullptr;
     int32_t* d_ext1 = nullptr;
     void* d_acc1 = nullptr;
+    void* d_stage = nullptr;   // packed key staging for uploads
+    size_t stage_cap = 0;
     std::mutex mu;
     bool timing = false;
     std::vector<Timing> pending;
@@ -371,42 +373,54 @@ inline int host_threads() {
     return static_cast<int>(std::max(1u, std::min(hc, 16u)));
 }
 
-// Device-ready key material: the BSK transformed into the engine's NTT order
-// and layout ([i][d][col][np][r][tl] doubles, 1/N folded in) and the KSK as
-// signed-byte planes.  Copying these to the device is all a key load needs.
+// Key material as it crosses to the device: the BSK residues bit-packed
+// (bits = bit length of q - 1 per value, polynomials word-aligned because N is a
+// multiple of 64), expanded into the engine's form on the device by
+// k_bsk_expand, and the KSK as signed-byte planes.
 struct HostKeyForm {
-    std::vector<double> bsk;
+    std::vector<uint64_t> packed;
     std::vector<uint32_t> planes;
-    size_t bytes() const { return bsk.size() * sizeof(double) + planes.size() * sizeof(uint32_t); }
+    size_t bsk_bytes = 0;  // engine-form BSK bytes on the device
+    size_t bytes() const { return bsk_bytes + planes.size() * sizeof(uint32_t); }
 };
+
+inline int key_bits(uint64_t q) { return 64 - __builtin_clzll(q - 1); }
+
+inline size_t bsk_polys(const ygpu_params& p) { return static_cast<size_t>(p.n) * p.level * (p.k + 1) * (p.k + 1); }
+
+// Words of the packed BSK (+1 so that a two-word read of the last value stays in bounds).
+inline size_t packed_words(const ygpu_params& p) {
+    return bsk_polys(p) * p.N * static_cast<size_t>(key_bits(p.q_ntt)) / 64 + 1;
+}
 
 inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, HostKeyForm& out) {
     const ygpu_params& p = ctx->p;
-    const int N = p.N, K1 = p.k + 1, L = p.level, NP = ctx->np, T = N / 16;
+    const int N = p.N;
     const uint64_t q = p.q_ntt;
-    const size_t polys = static_cast<size_t>(p.n) * L * K1 * K1;
-    for (size_t i = 0; i < polys * N; i++)
-        if (bsk_coeff[i] >= q) return fail(YGPU_ERR_INVALID_ARG, "BSK coefficient " + std::to_string(i) + " is not below q_ntt");
-    out.bsk.assign(polys * NP * N, 0.0);
+    const int bits = key_bits(q);
+    const size_t polys = bsk_polys(p);
+    const size_t polyWords = static_cast<size_t>(N) * bits / 64;  // exact: N is a multiple of 64
+    out.bsk_bytes = polys * ctx->np * N * sizeof(double);
+    out.packed.assign(packed_words(p), 0);
+    std::atomic<int64_t> bad{-1};
     parallel_for(static_cast<int>(polys), host_threads(), [&](int pi) {
-        std::vector<uint64_t> buf(N);
         const uint64_t* src = bsk_coeff + static_cast<size_t>(pi) * N;
-        for (int k = 0; k < NP; k++) {
-            const HostNtt& t = ctx->ntt[k];
-            for (int j = 0; j < N; j++) {
-                // centred integer lift of the residue, reduced mod the transform prime
-                const int64_t c = src[j] > (q - 1) / 2 ? static_cast<int64_t>(src[j]) - static_cast<int64_t>(q) : static_cast<int64_t>(src[j]);
-                int64_t m = c % static_cast<int64_t>(t.p);
-                if (m < 0) m += static_cast<int64_t>(t.p);
-                buf[j] = static_cast<uint64_t>(m);
+        uint64_t* dst = out.packed.data() + static_cast<size_t>(pi) * polyWords;
+        size_t bit = 0;
+        for (int j = 0; j < N; j++, bit += bits) {
+            const uint64_t v = src[j];
+            if (v >= q) {
+                int64_t expect = -1;
+                bad.compare_exchange_strong(expect, static_cast<int64_t>(pi) * N + j);
+                return;
             }
-            t.forward(buf.data());
-            double* dst = &out.bsk[(static_cast<size_t>(pi) * NP + k) * N];
-            for (int r = 0; r < 16; r++)
-                for (int tl = 0; tl < T; tl++)
-                    dst[r * T + tl] = centred_double(mulmod(buf[last_pass_index(ctx->logn, tl, r)], t.ninv, t.p), t.p);
+            const size_t w = bit >> 6;
+            const int off = static_cast<int>(bit & 63);
+            dst[w] |= v << off;
+            if (off + bits > 64) dst[w + 1] |= v >> (64 - off);
         }
     });
+    if (bad.load() >= 0) return fail(YGPU_ERR_INVALID_ARG, "BSK coefficient " + std::to_string(bad.load()) + " is not below q_ntt");
     // KSK as signed-byte planes: rows grouped by four, word = the four rows'
     // byte b of the same column; x = sum_b s_b 2^(8b) (mod 2^32), s_b signed
     const size_t cols = static_cast<size_t>(p.n) + 1;
@@ -425,6 +439,24 @@ inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int3
         }
     }
     return YGPU_OK;
+}
+
+// Enqueues the expansion of a packed BSK (device memory) into the engine form.
+inline hipError_t expand_bsk(const ygpu_ctx* ctx, const uint64_t* d_packed, double* d_out, hipStream_t s) {
+    BskExpandArgs a{};
+    a.packed = d_packed;
+    a.bits = key_bits(ctx->p.q_ntt);
+    a.q = ctx->p.q_ntt;
+    a.polys = static_cast<int>(bsk_polys(ctx->p));
+    a.np = ctx->np;
+    a.out = d_out;
+    a.tw = ctx->d_tw;
+    for (int k = 0; k < ctx->np; k++) {
+        a.prime[k] = ctx->prime[k];
+        a.p[k] = ctx->primes[k];
+        a.ninv[k] = centred_double(ctx->ntt[k].ninv, ctx->ntt[k].p);
+    }
+    return launch_bsk_expand(ctx->logn, a, s);
 }
 
 // Bytes a key occupies on the device, without building it.

@@ -168,6 +168,25 @@ hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t
 hipError_t launch_extract(int tbits, const KsArgs& a, int32_t* out, hipStream_t s);
 hipError_t launch_modmul_peak(int kind, const PrimeF64& pr, uint64_t q, int blocks, int iters, void* sink, hipStream_t s);
 hipError_t launch_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+hipError_t launch_copy_words(const void* src, void* dst, size_t words, hipStream_t s);
 constexpr int MODMUL_CHAINS = 8;
+
+// Device-side key preparation: bootstrapping-key residues mod q, bit-packed
+// `bits` per value (polynomial after polynomial, natural coefficient order),
+// into the engine's BSK form: per transform prime, the forward transform of
+// the centred lift times N^-1, centred, as doubles in last-pass order.
+struct BskExpandArgs {
+    const uint64_t* packed;
+    int bits;
+    uint64_t q;
+    int polys;
+    int np;
+    double* out;
+    const double* tw;       // per prime: forward twiddles at np*2N
+    PrimeF64 prime[2];
+    uint64_t p[2];
+    double ninv[2];         // N^-1 mod p, centred
+};
+hipError_t launch_bsk_expand(int logn, const BskExpandArgs& a, hipStream_t s);
 
 }  // namespace ygpu

@@ -143,8 +143,6 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
         }
 
         const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
-        // prime-0 residues of the owned column (NP == 2), at this lane's positions
-        double* res = resbuf[lc] + (NP > 1 ? tm * N : 0);
 
 #pragma unroll
         for (int np = 0; np < NP; np++) {
@@ -223,7 +221,10 @@ __global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
                     }
                 }
             } else {
-                // each lane reads back only what it wrote: no barrier needed
+                // prime-0 residues of the owned column at this lane's
+                // positions; each lane reads back only what it wrote, so no
+                // barrier is needed
+                double* res = resbuf[lc] + tm * N;
                 if (np == 0) {
 #pragma unroll
                     for (int r = 0; r < 16; r++) res[r * T + tl] = centre(x[r], pr);
@@ -455,6 +456,20 @@ __global__ void __launch_bounds__(256) k_modmul_peak_u64(uint64_t q, int iters, 
 __global__ void __launch_bounds__(256) k_copy_u32x4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * blockDim.x)
         dst[i] = src[i];
+}
+
+// Word copy for small host<->device staging through mapped pinned memory: a
+// compute kernel, so it is not queued behind bulk DMA transfers.
+__global__ void __launch_bounds__(256) k_copy_u32(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
+hipError_t launch_copy_words(const void* src, void* dst, size_t words, hipStream_t s) {
+    if (words == 0) return hipSuccess;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((words + 255) / 256, 1024));
+    hipLaunchKernelGGL(k_copy_u32, dim3(blocks), dim3(256), 0, s, static_cast<const uint32_t*>(src), static_cast<uint32_t*>(dst), words);
+    return hipGetLastError();
 }
 
 hipError_t launch_modmul_peak(int kind, const PrimeF64& pr, uint64_t q, int blocks, int iters, void* sink, hipStream_t s) {

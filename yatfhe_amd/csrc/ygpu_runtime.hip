@@ -92,7 +92,7 @@ int ygpu_destroy(ygpu_ctx* ctx) {
         for (void* p : {static_cast<void*>(ctx->d_tw), static_cast<void*>(ctx->d_bsk_table), static_cast<void*>(ctx->d_ksk_table),
                         ctx->d_acc, static_cast<void*>(ctx->d_partial), static_cast<void*>(ctx->d_slots),
                         static_cast<void*>(ctx->d_lwe1), ctx->d_v1, static_cast<void*>(ctx->d_out1),
-                        static_cast<void*>(ctx->d_ext1), ctx->d_acc1})
+                        static_cast<void*>(ctx->d_ext1), ctx->d_acc1, ctx->d_stage})
             if (p) hipFree(p);
         for (auto& t : ctx->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
         for (auto e : ctx->event_pool) hipEventDestroy(e);
@@ -117,12 +117,17 @@ int ygpu_key_upload(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk
         KeyRec rec;
         rec.live = true;
         rec.bytes = hk.bytes();
-        hipError_t e = hipMalloc(reinterpret_cast<void**>(&rec.d_bsk), hk.bsk.size() * sizeof(double));
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&rec.d_bsk), hk.bsk_bytes);
         if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc BSK: ") + hipGetErrorString(e));
         e = hipMalloc(reinterpret_cast<void**>(&rec.d_ksk), hk.planes.size() * sizeof(uint32_t));
         if (e != hipSuccess) { hipFree(rec.d_bsk); return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc KSK: ") + hipGetErrorString(e)); }
-        YG_HIP(hipMemcpy(rec.d_bsk, hk.bsk.data(), hk.bsk.size() * sizeof(double), hipMemcpyHostToDevice));
-        YG_HIP(hipMemcpy(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        // packed residues -> staging -> engine form, on the context stream
+        rc = ensure(&ctx->d_stage, &ctx->stage_cap, hk.packed.size() * sizeof(uint64_t));
+        if (rc) { hipFree(rec.d_bsk); hipFree(rec.d_ksk); return rc; }
+        YG_HIP(hipMemcpyAsync(ctx->d_stage, hk.packed.data(), hk.packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+        YG_HIP(expand_bsk(ctx, static_cast<const uint64_t*>(ctx->d_stage), rec.d_bsk, ctx->stream));
+        YG_HIP(hipMemcpyAsync(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        YG_HIP(hipStreamSynchronize(ctx->stream));
         int id = -1;
         for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
         if (id < 0) { id = static_cast<int>(ctx->keys.size()); ctx->keys.push_back(rec); }

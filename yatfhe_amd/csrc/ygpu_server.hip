@@ -53,11 +53,12 @@ inline int64_t now_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
 }
 
-// Device-ready key in pinned host memory.
+// A key's transfer form in pinned host memory: bit-packed BSK residues and
+// KSK planes.  bsk_bytes is the engine-form BSK size on the device.
 struct PinnedKey {
-    double* bsk = nullptr;
+    uint64_t* packed = nullptr;
     uint32_t* planes = nullptr;
-    size_t bsk_bytes = 0, planes_bytes = 0;
+    size_t packed_bytes = 0, planes_bytes = 0, bsk_bytes = 0;
 };
 
 struct Tenant {
@@ -87,6 +88,11 @@ struct Lane {
     void* h_v = nullptr;
     int32_t* h_out = nullptr;
     int32_t* h_slots = nullptr;
+    // device addresses of the pinned staging buffers (mapped host memory)
+    const int32_t* hd_in = nullptr;
+    const void* hd_v = nullptr;
+    int32_t* hd_out = nullptr;
+    const int32_t* hd_slots = nullptr;
     int32_t* d_in = nullptr;
     void* d_v = nullptr;
     int32_t* d_out = nullptr;
@@ -108,6 +114,7 @@ struct PSet {
     std::deque<int> order;            // tenants with pending requests, oldest first
     int64_t queued = 0;               // pending requests
     hipStream_t copy_stream = nullptr;
+    uint64_t* d_stage = nullptr;      // packed BSK of the load being expanded
     std::deque<int> loads;            // tenants being loaded, in copy order
     std::deque<int> inflight;         // lanes in launch order
     std::condition_variable cv_work;  // requests, a free lane, a ready key or free memory
@@ -253,7 +260,10 @@ int start_load(ygpu_server* s, PSet& ps, int t) {
     s->h_bsk_tab[t] = reinterpret_cast<const double*>(base);
     s->h_ksk_tab[t] = reinterpret_cast<const int32_t*>(base + align256(f.bsk_bytes));
     hipStream_t cs = ps.copy_stream;
-    hipError_t e = hipMemcpyAsync(base, f.bsk, f.bsk_bytes, hipMemcpyHostToDevice, cs);
+    // packed residues -> staging -> engine form; loads of one parameter set
+    // run in order on its copy stream, so one staging buffer serves them all
+    hipError_t e = hipMemcpyAsync(ps.d_stage, f.packed, f.packed_bytes, hipMemcpyHostToDevice, cs);
+    if (e == hipSuccess) e = expand_bsk(ps.ctx, ps.d_stage, reinterpret_cast<double*>(base), cs);
     if (e == hipSuccess) e = hipMemcpyAsync(base + align256(f.bsk_bytes), f.planes, f.planes_bytes, hipMemcpyHostToDevice, cs);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_bsk_tab + t, s->h_bsk_tab + t, sizeof(void*), hipMemcpyHostToDevice, cs);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_ksk_tab + t, s->h_ksk_tab + t, sizeof(void*), hipMemcpyHostToDevice, cs);
@@ -272,7 +282,7 @@ int start_load(ygpu_server* s, PSet& ps, int t) {
     ps.loads.push_back(t);
     ps.cv_load.notify_one();
     s->st.key_loads++;
-    s->st.bytes_loaded += static_cast<int64_t>(f.bsk_bytes + f.planes_bytes);
+    s->st.bytes_loaded += static_cast<int64_t>(f.packed_bytes + f.planes_bytes);
     s->st.resident_bytes += static_cast<int64_t>(tn.bytes);
     s->st.resident_keys++;
     s->st.peak_resident_bytes = std::max(s->st.peak_resident_bytes, s->st.resident_bytes);
@@ -291,6 +301,15 @@ int lane_alloc(const ygpu_server* s, PSet& ps, Lane& L) {
     YG_HIP(hipHostMalloc(&L.h_v, B * ps.tbytes * p.N, hipHostMallocDefault));
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_out), B * lwe, hipHostMallocDefault));
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_slots), B * sizeof(int32_t), hipHostMallocDefault));
+    void* dp = nullptr;
+    YG_HIP(hipHostGetDevicePointer(&dp, L.h_in, 0));
+    L.hd_in = static_cast<const int32_t*>(dp);
+    YG_HIP(hipHostGetDevicePointer(&dp, L.h_v, 0));
+    L.hd_v = dp;
+    YG_HIP(hipHostGetDevicePointer(&dp, L.h_out, 0));
+    L.hd_out = static_cast<int32_t*>(dp);
+    YG_HIP(hipHostGetDevicePointer(&dp, L.h_slots, 0));
+    L.hd_slots = static_cast<const int32_t*>(dp);
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_in), B * lwe));
     YG_HIP(hipMalloc(&L.d_v, B * ps.tbytes * p.N));
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_out), B * lwe));
@@ -324,9 +343,10 @@ int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
         std::memcpy(static_cast<char*>(L.h_v) + c * vbytes, r->v, vbytes);
         L.h_slots[c] = r->tenant;
     }
-    YG_HIP(hipMemcpyAsync(L.d_in, L.h_in, count * lwe, hipMemcpyHostToDevice, L.stream));
-    YG_HIP(hipMemcpyAsync(L.d_v, L.h_v, count * vbytes, hipMemcpyHostToDevice, L.stream));
-    YG_HIP(hipMemcpyAsync(L.d_slots, L.h_slots, count * sizeof(int32_t), hipMemcpyHostToDevice, L.stream));
+    // staging copies by kernel over mapped pinned memory (not behind key loads)
+    YG_HIP(launch_copy_words(L.hd_in, L.d_in, count * lwe / 4, L.stream));
+    YG_HIP(launch_copy_words(L.hd_v, L.d_v, count * vbytes / 4, L.stream));
+    YG_HIP(launch_copy_words(L.hd_slots, L.d_slots, count, L.stream));
     BrArgs a = br_args(ps.ctx, count, L.d_in, L.d_v, p.N, L.d_acc, s->d_bsk_tab, L.d_slots);
     int rc = run_br(ps.ctx, ps.ctx->tile, a, L.stream);
     if (rc) return rc;
@@ -340,7 +360,7 @@ int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
         if (rc) return rc;
         c0 = c1;
     }
-    YG_HIP(hipMemcpyAsync(L.h_out, L.d_out, count * lwe, hipMemcpyDeviceToHost, L.stream));
+    YG_HIP(launch_copy_words(L.d_out, L.hd_out, count * lwe / 4, L.stream));
     YG_HIP(hipEventRecord(L.done, L.stream));
     return YGPU_OK;
 }
@@ -492,6 +512,7 @@ void watcher_loop(ygpu_server* s, PSet* ps) {
             tn.ready = true;
         }
         notify_all_psets(s);
+        s->cv_req.notify_all();
     }
 }
 
@@ -612,6 +633,11 @@ int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* p
             ygpu_destroy(ps->ctx);
             return fail(YGPU_ERR_HIP, "copy stream");
         }
+        if (hipMalloc(reinterpret_cast<void**>(&ps->d_stage), packed_words(ps->p) * sizeof(uint64_t)) != hipSuccess) {
+            hipStreamDestroy(ps->copy_stream);
+            ygpu_destroy(ps->ctx);
+            return fail(YGPU_ERR_OUT_OF_MEMORY, "key staging buffer");
+        }
         ps->lanes.resize(s->cfg.lanes);
         for (Lane& L : ps->lanes) {
             rc = lane_alloc(s, *ps, L);
@@ -651,24 +677,25 @@ int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coe
         int rc = prepare_key_host(ctx, bsk_coeff, ksk, hk);
         if (rc) return rc;
         PinnedKey f;
-        f.bsk_bytes = hk.bsk.size() * sizeof(double);
+        f.bsk_bytes = hk.bsk_bytes;
+        f.packed_bytes = hk.packed.size() * sizeof(uint64_t);
         f.planes_bytes = hk.planes.size() * sizeof(uint32_t);
         if (align256(f.bsk_bytes) + align256(f.planes_bytes) > s->arena_bytes)
             return fail(YGPU_ERR_OUT_OF_MEMORY, "key larger than the key budget");
-        YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&f.bsk), f.bsk_bytes, hipHostMallocDefault));
+        YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&f.packed), f.packed_bytes, hipHostMallocDefault));
         hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&f.planes), f.planes_bytes, hipHostMallocDefault);
         if (e != hipSuccess) {
-            hipHostFree(f.bsk);
+            hipHostFree(f.packed);
             return server_fail_hip(e, "pinned key");
         }
-        std::memcpy(f.bsk, hk.bsk.data(), f.bsk_bytes);
+        std::memcpy(f.packed, hk.packed.data(), f.packed_bytes);
         std::memcpy(f.planes, hk.planes.data(), f.planes_bytes);
         Tenant t;
         t.pset = pset;
         t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
         e = hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming);
         if (e != hipSuccess) {
-            hipHostFree(f.bsk);
+            hipHostFree(f.packed);
             hipHostFree(f.planes);
             return server_fail_hip(e, "event");
         }
@@ -677,7 +704,7 @@ int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coe
         s->forms.push_back(f);
         *tenant = static_cast<int32_t>(s->tenants.size());
         s->tenants.push_back(t);
-        s->st.host_key_bytes += static_cast<int64_t>(f.bsk_bytes + f.planes_bytes);
+        s->st.host_key_bytes += static_cast<int64_t>(f.packed_bytes + f.planes_bytes);
         return YGPU_OK;
     });
 }
@@ -697,6 +724,23 @@ int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* t
         *tenant = static_cast<int32_t>(s->tenants.size());
         s->tenants.push_back(t);
         return YGPU_OK;
+    });
+}
+
+int ygpu_server_prefetch(ygpu_server* s, int32_t tenant) {
+    return with_server(s, [&] {
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (tenant < 0 || tenant >= static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown tenant");
+        if (!s->started) start_threads(s);
+        Tenant& tn = s->tenants[tenant];
+        if (!tn.resident) {
+            const int rc = start_load(s, *s->psets[tn.pset], tenant);
+            if (rc == 1) return fail(YGPU_ERR_BUSY, "no evictable key memory for a prefetch");
+            if (rc) return rc;
+            tn.fresh = false;  // no request is waiting for it: an ordinary LRU entry
+        }
+        s->cv_req.wait(lk, [&] { return tn.ready || !tn.resident; });
+        return tn.resident ? YGPU_OK : fail(YGPU_ERR_HIP, "prefetch load failed");
     });
 }
 
@@ -799,13 +843,14 @@ int ygpu_server_destroy(ygpu_server* s) {
             hipStreamSynchronize(ps->copy_stream);
             hipStreamDestroy(ps->copy_stream);
         }
+        if (ps->d_stage) hipFree(ps->d_stage);
         for (Lane& L : ps->lanes) lane_free(L);
         ygpu_destroy(ps->ctx);
     }
     for (Tenant& t : s->tenants)
         if (t.loaded) hipEventDestroy(t.loaded);
     for (PinnedKey& f : s->forms) {
-        if (f.bsk) hipHostFree(f.bsk);
+        if (f.packed) hipHostFree(f.packed);
         if (f.planes) hipHostFree(f.planes);
     }
     if (s->d_arena) hipFree(s->d_arena);
@@ -828,11 +873,14 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
     return with_server(s, [&] {
         if (!c || !out || !c->lwe_in || !c->v) return fail(YGPU_ERR_INVALID_ARG, "null argument");
         if (c->rate <= 0 || c->requests < 0 || c->warm < 0 || c->n_tenants <= 0) return fail(YGPU_ERR_INVALID_ARG, "bad load parameters");
-        std::vector<int> cols(c->n_tenants);
+        std::vector<int> cols(c->n_tenants), psetOf(c->n_tenants);
         {
             std::lock_guard<std::mutex> g(s->mu);
             if (c->n_tenants > static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_INVALID_ARG, "more tenants than registered");
-            for (int t = 0; t < c->n_tenants; t++) cols[t] = s->psets[s->tenants[t].pset]->p.n + 1;
+            for (int t = 0; t < c->n_tenants; t++) {
+                psetOf[t] = s->tenants[t].pset;
+                cols[t] = s->psets[psetOf[t]]->p.n + 1;
+            }
         }
         const int maxCols = *std::max_element(cols.begin(), cols.end());
         // popularity
@@ -909,7 +957,7 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
         }
         int64_t errors = 0, mismatches = 0, done = 0;
         int64_t t_end = t0;
-        std::vector<double> lat, queue;
+        std::vector<double> lat, queue, latHit, latMiss, latPset[4];
         lat.reserve(c->requests);
         queue.reserve(c->requests);
         // a run that has not drained 120 s after its last arrival is stuck
@@ -938,6 +986,8 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             done++;
             t_end = std::max<int64_t>(t_end, reqs[i].done_ns);
             lat.push_back((reqs[i].done_ns - reqs[i].arrival_ns) * 1e-6);
+            (reqs[i].key_hit ? latHit : latMiss).push_back(lat.back());
+            if (psetOf[reqs[i].tenant] < 4) latPset[psetOf[reqs[i].tenant]].push_back(lat.back());
             queue.push_back(std::max<int64_t>(0, reqs[i].launch_ns - reqs[i].arrival_ns) * 1e-6);
             if (c->expected && c->expected[reqs[i].tenant] &&
                 std::memcmp(reqs[i].lwe_out, c->expected[reqs[i].tenant], sizeof(int32_t) * cols[reqs[i].tenant]) != 0)
@@ -952,6 +1002,8 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
         };
         std::sort(lat.begin(), lat.end());
         std::sort(queue.begin(), queue.end());
+        std::sort(latHit.begin(), latHit.end());
+        std::sort(latMiss.begin(), latMiss.end());
         std::memset(out, 0, sizeof(*out));
         const double wall = (t_end - t0) * 1e-9;
         out->offered_rps = c->rate;
@@ -983,6 +1035,15 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
         out->key_evictions = s1.key_evictions - s0.key_evictions;
         out->bytes_loaded = s1.bytes_loaded - s0.bytes_loaded;
         out->memory_waits = s1.memory_waits - s0.memory_waits;
+        out->hit_p50_ms = pct(latHit, 50);
+        out->hit_p99_ms = pct(latHit, 99);
+        out->miss_p50_ms = pct(latMiss, 50);
+        out->miss_p99_ms = pct(latMiss, 99);
+        for (int k = 0; k < 4; k++) {
+            std::sort(latPset[k].begin(), latPset[k].end());
+            out->pset_p50_ms[k] = pct(latPset[k], 50);
+            out->pset_p99_ms[k] = pct(latPset[k], 99);
+        }
         return YGPU_OK;
     });
 }
