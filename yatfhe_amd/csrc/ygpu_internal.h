@@ -320,23 +320,17 @@ inline int job_slots(const ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int6
     return YGPU_OK;
 }
 
-// The key-grouped pipeline on device buffers (eager or under capture).
-inline int enqueue_jobs(ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int count, const int32_t* d_lwe_in, const void* d_v,
+// The key-grouped pipeline on device buffers (eager or under capture);
+// d_slots holds each ciphertext's key slot.
+inline int enqueue_jobs(ygpu_ctx* ctx, int count, const int32_t* d_lwe_in, const void* d_v,
                  int32_t v_per_ct, int32_t* d_lwe_out, void* acc, const int32_t* d_slots, int32_t* partial,
                  hipStream_t stream = nullptr) {
     BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table, d_slots);
     int rc = run_br(ctx, ctx->tile, a, stream);
     if (rc) return rc;
-    // keyswitch job by job: each job's rows are contiguous and share a KSK
-    const size_t accRow = ctx->torus_bytes() * (ctx->p.k + 1) * ctx->p.N;
-    for (int j = 0; j < n_jobs; j++) {
-        if (jobs[j].count == 0) continue;
-        rc = run_ks(ctx, jobs[j].count, static_cast<const char*>(acc) + accRow * jobs[j].first, ctx->d_ksk_table + jobs[j].key_id,
-                    nullptr, d_lwe_out + (ctx->p.n + 1) * jobs[j].first, nullptr,
-                    partial ? partial + (ctx->p.n + 1) * jobs[j].first : nullptr, stream);
-        if (rc) return rc;
-    }
-    return YGPU_OK;
+    // one keyswitch launch for all jobs: the kernel walks each tile's runs of
+    // one key through the per-ciphertext slots
+    return run_ks(ctx, count, acc, ctx->d_ksk_table, d_slots, d_lwe_out, nullptr, partial, stream);
 }
 
 inline int plain_buffers(ygpu_ctx* ctx) {

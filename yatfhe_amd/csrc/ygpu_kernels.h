@@ -30,6 +30,10 @@ struct BrCfg {
     static constexpr int CPW = CPW_;
     static constexpr bool PACK8 = PACK8_;
     static constexpr int THREADS = 2 * T * CPW;
+    // Waves per SIMD the register allocation must allow.  The one-prime kernel
+    // fits two (256 VGPRs), so small workgroups keep two waves per SIMD; the
+    // two-prime kernel needs more registers than two waves leave.
+    static constexpr int MIN_WAVES = NP == 1 ? 2 : 1;
     // twiddles in LDS when they fit beside the per-ciphertext state
     static constexpr bool TW_LDS = TWL_ < 0 ? (NP * 2 * N * 8 <= 16384) : (TWL_ == 1);
 };

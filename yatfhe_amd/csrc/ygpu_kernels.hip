@@ -31,7 +31,7 @@ namespace ygpu {
 // loads of one step hit in L1 after the first ciphertext; the transform
 // twiddles sit in LDS, shared by the workgroup.
 template <class C>
-__global__ void __launch_bounds__(C::THREADS) k_blind_rotate(BrArgs a) {
+__global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArgs a) {
     using TorusT = typename C::TorusT;
     constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = C::NP, TW = C::TEAM_WAVES, CPW = C::CPW;
     constexpr int SCR = Scratch<LOGN>::DOUBLES;
@@ -276,6 +276,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
     const int rows = a.k * N * t;
     const int groups = (rows + 3) / 4;
     const int ct0 = static_cast<int>(blockIdx.x) * KS_TC;
+    const int ctEnd = min(ct0 + KS_TC, a.count);
     const int col0 = static_cast<int>(blockIdx.y) * KS_TK;
     const int slices = static_cast<int>(gridDim.z);
     const int per = ((groups + slices - 1) / slices + KS_RG - 1) / KS_RG * KS_RG;
@@ -284,90 +285,103 @@ __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
     const int tid = static_cast<int>(threadIdx.x);
     const int tx = tid % 16, ty = tid / 16;   // 4 columns x 4 ciphertexts per thread
 
-    // one key per launch (the host launches a key-grouped batch job by job)
-    const uint32_t* __restrict__ planes = reinterpret_cast<const uint32_t*>(a.ksk_table[0]);
-
-    int32_t acc[4][4][4];  // [plane][ct][col]
-#pragma unroll
-    for (int b = 0; b < 4; b++)
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) acc[b][c][k] = 0;
-
-    for (int gb = g0; gb < g1; gb += KS_RG) {
-        __syncthreads();
-        // KSK planes of this chunk (coalesced along columns)
-        for (int e = tid; e < KS_RG * 4 * KS_TK; e += KS_THREADS) {
-            const int k = e % KS_TK, b = (e / KS_TK) % 4, g = e / (4 * KS_TK);
-            const int gg = gb + g, col = col0 + k;
-            sk[g][b][k] = (gg < g1 && col < cols) ? planes[(static_cast<size_t>(gg) * 4 + b) * cols + col] : 0u;
+    // The tile's ciphertexts are processed in runs of one key (a key-grouped
+    // batch has contiguous jobs, so most tiles hold one run); without a slot
+    // array the whole batch uses key slot 0.
+    for (int segStart = ct0; segStart < ctEnd;) {
+        const int slot = a.ct_slot ? a.ct_slot[segStart] : 0;
+        int segEnd = segStart + 1;
+        if (a.ct_slot) {
+            while (segEnd < ctEnd && a.ct_slot[segEnd] == slot) segEnd++;
+        } else {
+            segEnd = ctEnd;
         }
-        // digits of this chunk: one (ct, group) word per entry; the digits of
-        // one extracted coefficient are computed once for the rows it spans
-        for (int e = tid; e < KS_RG * KS_TC; e += KS_THREADS) {
-            const int c = e % KS_TC, g = e / KS_TC;
-            const int gg = gb + g, ct = ct0 + c;
-            uint32_t w = 0;
-            if (gg < g1 && ct < a.count) {
-                const size_t accBase = static_cast<size_t>(ct) * (a.k + 1) * N;
-                int8_t dig[KS_MAX_LEVEL];
-                int lastI = -1;
-                for (int r = 0; r < 4; r++) {
-                    const int row = gg * 4 + r;
-                    if (row >= rows) break;
-                    const int i = row / t, j = row % t;
-                    if (i != lastI) {
-                        const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N);
-                        ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dig);
-                        lastI = i;
+        const uint32_t* __restrict__ planes = reinterpret_cast<const uint32_t*>(a.ksk_table[slot]);
+
+        int32_t acc[4][4][4];  // [plane][ct][col]
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc[b][c][k] = 0;
+
+        for (int gb = g0; gb < g1; gb += KS_RG) {
+            __syncthreads();
+            // KSK planes of this chunk (coalesced along columns)
+            for (int e = tid; e < KS_RG * 4 * KS_TK; e += KS_THREADS) {
+                const int k = e % KS_TK, b = (e / KS_TK) % 4, g = e / (4 * KS_TK);
+                const int gg = gb + g, col = col0 + k;
+                sk[g][b][k] = (gg < g1 && col < cols) ? planes[(static_cast<size_t>(gg) * 4 + b) * cols + col] : 0u;
+            }
+            // digits of this chunk: one (ct, group) word per entry; the digits
+            // of one extracted coefficient are computed once for the rows it
+            // spans; ciphertexts outside the run get zero digits
+            for (int e = tid; e < KS_RG * KS_TC; e += KS_THREADS) {
+                const int c = e % KS_TC, g = e / KS_TC;
+                const int gg = gb + g, ct = ct0 + c;
+                uint32_t w = 0;
+                if (gg < g1 && ct >= segStart && ct < segEnd) {
+                    const size_t accBase = static_cast<size_t>(ct) * (a.k + 1) * N;
+                    int8_t dig[KS_MAX_LEVEL];
+                    int lastI = -1;
+                    for (int r = 0; r < 4; r++) {
+                        const int row = gg * 4 + r;
+                        if (row >= rows) break;
+                        const int i = row / t, j = row % t;
+                        if (i != lastI) {
+                            const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N);
+                            ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dig);
+                            lastI = i;
+                        }
+                        w |= (static_cast<uint32_t>(dig[j]) & 0xFFu) << (8 * r);
                     }
-                    w |= (static_cast<uint32_t>(dig[j]) & 0xFFu) << (8 * r);
+                }
+                sd[g][c] = w;
+            }
+            __syncthreads();
+            const int gEnd = min(KS_RG, g1 - gb);
+            for (int g = 0; g < gEnd; g++) {
+                const uint4 dw = *reinterpret_cast<const uint4*>(&sd[g][ty * 4]);
+                const uint32_t d[4] = {dw.x, dw.y, dw.z, dw.w};
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    const uint4 kw = *reinterpret_cast<const uint4*>(&sk[g][b][tx * 4]);
+                    const uint32_t kv[4] = {kw.x, kw.y, kw.z, kw.w};
+#pragma unroll
+                    for (int c = 0; c < 4; c++)
+#pragma unroll
+                        for (int k = 0; k < 4; k++)
+                            acc[b][c][k] = __builtin_amdgcn_sdot4(static_cast<int>(d[c]), static_cast<int>(kv[k]), acc[b][c][k], false);
                 }
             }
-            sd[g][c] = w;
         }
-        __syncthreads();
-        const int gEnd = min(KS_RG, g1 - gb);
-        for (int g = 0; g < gEnd; g++) {
-            const uint4 dw = *reinterpret_cast<const uint4*>(&sd[g][ty * 4]);
-            const uint32_t d[4] = {dw.x, dw.y, dw.z, dw.w};
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const uint4 kw = *reinterpret_cast<const uint4*>(&sk[g][b][tx * 4]);
-                const uint32_t kv[4] = {kw.x, kw.y, kw.z, kw.w};
-#pragma unroll
-                for (int c = 0; c < 4; c++)
-#pragma unroll
-                    for (int k = 0; k < 4; k++)
-                        acc[b][c][k] = __builtin_amdgcn_sdot4(static_cast<int>(d[c]), static_cast<int>(kv[k]), acc[b][c][k], false);
-            }
-        }
-    }
 
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int ct = ct0 + ty * 4 + c;
-        if (ct >= a.count) continue;
+        for (int c = 0; c < 4; c++) {
+            const int ct = ct0 + ty * 4 + c;
+            if (ct < segStart || ct >= segEnd) continue;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int col = col0 + tx * 4 + k;
-            if (col >= cols) continue;
-            const uint32_t sum = static_cast<uint32_t>(acc[0][c][k]) + (static_cast<uint32_t>(acc[1][c][k]) << 8) +
-                                 (static_cast<uint32_t>(acc[2][c][k]) << 16) + (static_cast<uint32_t>(acc[3][c][k]) << 24);
-            if (slices == 1) {
-                uint32_t o;
-                if (col < n) {
-                    o = 0u - sum;
+            for (int k = 0; k < 4; k++) {
+                const int col = col0 + tx * 4 + k;
+                if (col >= cols) continue;
+                const uint32_t sum = static_cast<uint32_t>(acc[0][c][k]) + (static_cast<uint32_t>(acc[1][c][k]) << 8) +
+                                     (static_cast<uint32_t>(acc[2][c][k]) << 16) + (static_cast<uint32_t>(acc[3][c][k]) << 24);
+                if (slices == 1) {
+                    uint32_t o;
+                    if (col < n) {
+                        o = 0u - sum;
+                    } else {
+                        const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0);
+                        o = static_cast<uint32_t>(b) - sum;
+                    }
+                    a.lwe_out[static_cast<size_t>(ct) * cols + col] = static_cast<int32_t>(o);
                 } else {
-                    const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0);
-                    o = static_cast<uint32_t>(b) - sum;
+                    atomicAdd(reinterpret_cast<unsigned int*>(a.partial) + static_cast<size_t>(ct) * cols + col, sum);
                 }
-                a.lwe_out[static_cast<size_t>(ct) * cols + col] = static_cast<int32_t>(o);
-            } else {
-                atomicAdd(reinterpret_cast<unsigned int*>(a.partial) + static_cast<size_t>(ct) * cols + col, sum);
             }
         }
+        segStart = segEnd;
     }
 }
 

@@ -45,7 +45,9 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         ctx->primes[0] = P_AUX_0;
         ctx->primes[1] = P_AUX_1;
     }
-    ctx->tile = BrTile{params->torus_bits == 32 && params->N == 1024 ? 4 : 1, params->torus_bits == 32 && params->N == 1024 ? 1 : 0};
+    // default tile: one ciphertext per workgroup (barriers couple only its two
+    // teams), twiddles in LDS where they fit (TORUS32, N = 1024)
+    ctx->tile = BrTile{1, params->torus_bits == 32 && params->N == 1024 ? 1 : 0};
     if (const char* e = std::getenv("YGPU_CPW")) ctx->tile.cpw = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -201,7 +203,7 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
         }
         rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap, sizeof(int32_t) * static_cast<size_t>(total) * (ctx->p.n + 1));
         if (rc) return rc;
-        rc = enqueue_jobs(ctx, jobs, n_jobs, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc, ctx->d_slots, ctx->d_partial);
+        rc = enqueue_jobs(ctx, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc, ctx->d_slots, ctx->d_partial);
         if (rc) return rc;
         // the host slot vector dies with this call: finish the copy first
         YG_HIP(hipStreamSynchronize(ctx->stream));
@@ -546,9 +548,8 @@ int ygpu_graph_create(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64
             YG_HIP(hipMalloc(&g.d_acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(total)));
             acc = g.d_acc;
         }
-        std::vector<ygpu_job> js(jobs, jobs + n_jobs);
         rc = capture_graph(ctx, g, [&]() {
-            return enqueue_jobs(ctx, js.data(), n_jobs, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc,
+            return enqueue_jobs(ctx, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc,
                                 g.d_slots, g.d_partial);
         }, graph_id);
         if (rc) free_graph(g);

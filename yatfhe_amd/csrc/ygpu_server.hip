@@ -350,16 +350,9 @@ int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
     BrArgs a = br_args(ps.ctx, count, L.d_in, L.d_v, p.N, L.d_acc, s->d_bsk_tab, L.d_slots);
     int rc = run_br(ps.ctx, ps.ctx->tile, a, L.stream);
     if (rc) return rc;
-    const size_t accRow = ps.tbytes * (p.k + 1) * p.N;
-    for (int c0 = 0; c0 < count;) {
-        int c1 = c0;
-        while (c1 < count && L.reqs[c1]->tenant == L.reqs[c0]->tenant) c1++;
-        rc = run_ks(ps.ctx, c1 - c0, static_cast<const char*>(L.d_acc) + accRow * c0, s->d_ksk_tab + L.reqs[c0]->tenant, nullptr,
-                    L.d_out + static_cast<size_t>(c0) * (p.n + 1), nullptr, L.d_partial + static_cast<size_t>(c0) * (p.n + 1),
-                    L.stream);
-        if (rc) return rc;
-        c0 = c1;
-    }
+    // one keyswitch launch; the kernel follows the per-ciphertext tenant slots
+    rc = run_ks(ps.ctx, count, L.d_acc, s->d_ksk_tab, L.d_slots, L.d_out, nullptr, L.d_partial, L.stream);
+    if (rc) return rc;
     YG_HIP(launch_copy_words(L.d_out, L.hd_out, count * lwe / 4, L.stream));
     YG_HIP(hipEventRecord(L.done, L.stream));
     return YGPU_OK;
