@@ -77,8 +77,11 @@ def test_t56_tiles_bit_identical(p56, keys56):
         kid = ctx.upload_key(keys56)
         ctx.set_tile(1, 0)
         a, _, _ = ctx.pbs_batch(kid, cts, v)
-        ctx.set_tile(2, 0)
+        cpw, twl, ms = ctx.autotune(kid, 64)
+        assert (cpw, twl) == (1, 0) and ms > 0
         b, _, _ = ctx.pbs_batch(kid, cts, v)
-    assert np.array_equal(a, b)
+        # a plain call per ciphertext runs the same kernel with a one-ciphertext grid
+        c = np.stack([ctx.pbs(kid, cts[i], v)[0] for i in range(len(cts))])
+    assert np.array_equal(a, b) and np.array_equal(a, c)
     ph = yg.lwe_decrypt_phase(a, keys56.lwe_key).astype(np.int64)
     assert np.array_equal(np.round(ph / (1 << 32) * 16).astype(np.int64) % 16, lut[msgs] % 16)

@@ -29,11 +29,11 @@ struct BrCfg {
     static constexpr int NP = NP_;
     static constexpr int CPW = CPW_;
     static constexpr bool PACK8 = PACK8_;
-    static constexpr int THREADS = 2 * T * CPW;
-    // Waves per SIMD the register allocation must allow.  The one-prime kernel
-    // fits two (256 VGPRs), so small workgroups keep two waves per SIMD; the
-    // two-prime kernel needs more registers than two waves leave.
-    static constexpr int MIN_WAVES = NP == 1 ? 2 : 1;
+    // teams per ciphertext: one per accumulator row, times one per transform prime
+    static constexpr int TEAMS = 2 * NP;
+    static constexpr int THREADS = TEAMS * T * CPW;
+    // Waves per SIMD the register allocation must allow: two (256 VGPRs).
+    static constexpr int MIN_WAVES = 2;
     // twiddles in LDS when they fit beside the per-ciphertext state
     static constexpr bool TW_LDS = TWL_ < 0 ? (NP * 2 * N * 8 <= 16384) : (TWL_ == 1);
 };
@@ -166,6 +166,8 @@ struct BrTile {
     int twl = 1;
 };
 hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& tile, int radix_bits, const BrArgs& a, hipStream_t s);
+// The two-prime (CRT) blind rotation, in its own translation unit.
+hipError_t launch_blind_rotate_crt(int logn, const BrTile& tile, int radix_bits, int level, const BrArgs& a, hipStream_t s);
 // Tiles instantiated for a parameter set (the autotuner's candidates).
 int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* out, int max);
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s);

@@ -32,6 +32,7 @@ namespace ygpu {
 // twiddles sit in LDS, shared by the workgroup.
 template <class C>
 __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArgs a) {
+    static_assert(C::NP == 1, "two-prime moduli use k_blind_rotate_crt");
     using TorusT = typename C::TorusT;
     constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = C::NP, TW = C::TEAM_WAVES, CPW = C::CPW;
     constexpr int SCR = Scratch<LOGN>::DOUBLES;
@@ -517,17 +518,12 @@ hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& t, int radix_b
         if (t.cpw == 2) return t.twl ? launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 0>>(a, s);
         return t.twl ? launch_br<BrCfg<32, 10, 1, 1, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, true, 3, 7, 0>>(a, s);
     }
-    if (tbits == 56 && logn == 11 && a.level == 3 && radix_bits == 10) {
-        if (t.cpw >= 2) return launch_br<BrCfg<56, 11, 2, 2, false, 3, 10, 0>>(a, s);
-        return launch_br<BrCfg<56, 11, 2, 1, false, 3, 10, 0>>(a, s);
-    }
+    if (tbits == 56) return launch_blind_rotate_crt(logn, t, radix_bits, a.level, a, s);
     if (tbits == 32 && logn == 10) {
         if (t.cpw >= 4) return p8 ? launch_br<BrCfg<32, 10, 1, 4, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, false>>(a, s);
         return p8 ? launch_br<BrCfg<32, 10, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, false>>(a, s);
     }
     if (tbits == 32 && logn == 11) return p8 ? launch_br<BrCfg<32, 11, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 11, 1, 1, false>>(a, s);
-    if (tbits == 56 && logn == 10) return p8 ? launch_br<BrCfg<56, 10, 2, 1, true>>(a, s) : launch_br<BrCfg<56, 10, 2, 1, false>>(a, s);
-    if (tbits == 56 && logn == 11) return p8 ? launch_br<BrCfg<56, 11, 2, 1, true>>(a, s) : launch_br<BrCfg<56, 11, 2, 1, false>>(a, s);
     return hipErrorInvalidValue;
 }
 
@@ -537,9 +533,8 @@ int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* o
     if (tbits == 32 && logn == 10 && level == 3 && radix_bits == 7) {
         for (int cpw : {4, 2, 1})
             for (int twl : {1, 0}) c[n++] = BrTile{cpw, twl};
-    } else if (tbits == 56 && logn == 11 && level == 3 && radix_bits == 10) {
+    } else if (tbits == 56) {
         c[n++] = BrTile{1, 0};
-        c[n++] = BrTile{2, 0};
     } else if (tbits == 32 && logn == 10) {
         c[n++] = BrTile{4, 1};
         c[n++] = BrTile{1, 1};

@@ -1,0 +1,210 @@
+// ygpu_kernels_crt.hip -- blind rotation for NTT moduli above binary64 range
+// (TORUS56: q56 = 27 * 2^56 + 1, bootstrapping.cpp:71-80 at 56 bits).
+//
+// The external product runs modulo two ~49-bit primes whose product bounds
+// the exact result, and the value mod q56 is recovered by Garner CRT
+// (crt_mod_q).  Each prime gets its own teams: a ciphertext is four teams of
+// T = N/16 lanes, team (row r, prime p) holding a copy of accumulator row r,
+// transforming row r's digit polynomials mod p, owning output column r mod p.
+// Per step and per team: digits of row r; l forward transforms and the
+// multiply-accumulate against both BSK columns; the partner-column swap with
+// team (1-r, p); the inverse transform of column r; then the prime-1 team hands
+// its residues to the prime-0 team, which reconstructs the value mod q56 and
+// updates row r.  The accumulator rows live in LDS, read by both prime teams.
+// Splitting the primes over teams keeps each team's registers within two waves
+// per SIMD (no residue of one prime is held across the other), and the two
+// primes proceed in parallel.
+//
+// Kept in its own translation unit so that the one-prime kernel's register
+// allocation is not perturbed.
+#include "ygpu_kernels.h"
+
+namespace ygpu {
+
+template <class C>
+__global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(BrArgs a) {
+    static_assert(C::NP == 2, "two transform primes");
+    using TorusT = int64_t;
+    constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = 2, TW = C::TEAM_WAVES, CPW = C::CPW;
+    constexpr int SCR = Scratch<LOGN>::DOUBLES;
+    constexpr int CT_THREADS = 4 * T;
+
+    __shared__ double scratch[CPW][4 * SCR];
+    // the accumulator rows live in LDS, shared by the two prime teams of a row
+    // (registers are the binding resource); the prime-0 team writes them
+    __shared__ int64_t accbuf[CPW][K1][N];
+    // forward twiddles of both primes (the forward transform runs l times per
+    // step); the inverse twiddles are read through L1
+    __shared__ double ftw[NP][N];
+
+    const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    const int first = grp * CPW;
+    if (first >= a.count) return;  // whole workgroup
+    const int lc = static_cast<int>(threadIdx.x) / CT_THREADS;
+    const int ctid = static_cast<int>(threadIdx.x) % CT_THREADS;
+    const int tt = ctid / T;       // team: row tm = tt & 1, prime tp = tt >> 1
+    const int tm = tt & 1;
+    const int tp = tt >> 1;
+    const int tl = ctid % T;
+    const bool live = first + lc < a.count;
+    const int ct = live ? first + lc : a.count - 1;
+    double* scr = scratch[lc] + tt * SCR;
+    int64_t* acc = accbuf[lc][tm];
+
+    const int n = a.n;
+    const int L = a.level;
+    const int rb = a.radix_bits;
+    const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
+    const double* __restrict__ bsk = a.bsk_table[slot];
+    const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
+    const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
+    const PrimeF64 pr = a.prime[tp];
+    for (int e = static_cast<int>(threadIdx.x); e < NP * N; e += C::THREADS)
+        ftw[e / N][e % N] = a.tw[static_cast<size_t>(e / N) * 2 * N + e % N];
+    const double* tw = ftw[tp];
+    const double* __restrict__ itw = a.tw + static_cast<size_t>(tp) * 2 * N + N;
+
+    // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
+    // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
+    if (tp == 0) {
+        if (a.acc_in) {
+            const TorusT* ai = static_cast<const TorusT*>(a.acc_in) + static_cast<size_t>(ct) * K1 * N + tm * N;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[tl + T * r] = ai[tl + T * r];
+        } else {
+            const int bbar = a.prescaled ? lwe[n] : mod_switch_2n(lwe[n], LOGN + 1);
+            int aT, wrap;
+            validate_rotator(-bbar, N, aT, wrap);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int j = tl + T * r;
+                const int src = j - aT;
+                const int64_t val = src < 0 ? -static_cast<int64_t>(v[src + N]) * wrap : static_cast<int64_t>(v[src]) * wrap;
+                acc[j] = tm == 0 ? TorusT(0) : sext_t<C::TBITS>(val);
+            }
+        }
+    }
+    __syncthreads();
+
+    const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
+    for (int i = 0; i < n; i++) {
+        const int abar = a.prescaled ? lwe[i] : mod_switch_2n(lwe[i], LOGN + 1);
+        int aT, wrap;
+        validate_rotator(abar, N, aT, wrap);
+
+        // (X^a - 1) acc_r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238)
+        // at this lane's pass-0 positions, and its digits
+        uint32_t dg0[16], dg1[16];
+        const bool flip = wrap < 0;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int j = tl + T * r;
+            const int src = (j - aT) & (N - 1);
+            const bool negs = (j < aT) != flip;
+            const int64_t av = acc[src];
+            const int64_t rot = sext_t<C::TBITS>((negs ? -av : av) - acc[j]);
+            if constexpr (C::L > 0) {
+                digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
+            } else {
+                int d4[4] = {0, 0, 0, 0};
+                digits_of<C::TBITS>(rot, L, rb, d4);
+                dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
+                dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+            }
+        }
+
+        const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
+        double own[16], oth[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
+        for (int lvl = 0; lvl < L; lvl++) {
+            const int d = lvl * K1 + tm;
+            const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + tp) * N + tl;
+            const double* __restrict__ kpo = kp + static_cast<size_t>(tm) * NP * N;
+            const double* __restrict__ kpt = kp + static_cast<size_t>(1 - tm) * NP * N;
+            double x[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int dv;
+                if constexpr (C::L > 0 && C::RB <= 8) {
+                    dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
+                } else {
+                    const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
+                    dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
+                }
+                x[r] = static_cast<double>(dv);
+            }
+            ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+            double kbo[16], kbt[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
+#pragma unroll
+            for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
+                oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
+            }
+        }
+
+        // partner sums (same prime): hand column 1-tm to team (1-tm, tp)
+        team_sync<TW>();
+#pragma unroll
+        for (int r = 0; r < 16; r++) scr[r * T + tl] = oth[r];
+        __syncthreads();
+        double x[16];
+        {
+            const double* other = scratch[lc] + (tt ^ 1) * SCR;
+#pragma unroll
+            for (int r = 0; r < 16; r++) x[r] = __dadd_rn(own[r], other[r * T + tl]);
+        }
+        __syncthreads();
+
+        ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+
+        // residues of column tm mod both primes: swap with team (tm, 1-tp)
+#pragma unroll
+        for (int r = 0; r < 16; r++) x[r] = centre(x[r], pr);
+        team_sync<TW>();
+#pragma unroll
+        for (int r = 0; r < 16; r++) scr[r * T + tl] = x[r];
+        __syncthreads();
+        {
+            const double* other = scratch[lc] + (tt ^ 2) * SCR;
+            if (tp == 0) {
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int j = tl + T * r;
+                    const int64_t val = crt_mod_q(x[r], other[r * T + tl], a.prime[1], a.crt);
+                    acc[j] = sext_t<C::TBITS>(acc[j] + val);
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    if (live && tp == 0) {
+        TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + tm * N;
+#pragma unroll
+        for (int r = 0; r < 16; r++) outAcc[tl + T * r] = acc[tl + T * r];
+    }
+}
+
+template <class C>
+hipError_t launch_br_crt(const BrArgs& a, hipStream_t s) {
+    const int groups = (a.count + C::CPW - 1) / C::CPW;
+    const unsigned grid = static_cast<unsigned>((groups + 7) / 8 * 8);  // map_workgroup needs a multiple of 8
+    hipLaunchKernelGGL(k_blind_rotate_crt<C>, dim3(grid), dim3(C::THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_blind_rotate_crt(int logn, const BrTile& t, int radix_bits, int level, const BrArgs& a, hipStream_t s) {
+    if (logn == 11 && level == 3 && radix_bits == 10) return launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 0>>(a, s);
+    const bool p8 = radix_bits <= 8;
+    if (logn == 10) return p8 ? launch_br_crt<BrCfg<56, 10, 2, 1, true>>(a, s) : launch_br_crt<BrCfg<56, 10, 2, 1, false>>(a, s);
+    if (logn == 11) return p8 ? launch_br_crt<BrCfg<56, 11, 2, 1, true>>(a, s) : launch_br_crt<BrCfg<56, 11, 2, 1, false>>(a, s);
+    (void)t;
+    return hipErrorInvalidValue;
+}
+
+}  // namespace ygpu
