@@ -430,9 +430,13 @@ def leg_serving(yg, args, peak):
             st = srv.stats()
         ok = [x for x in sweep if x["shed"] == 0 and x["errors"] == 0 and x["throughput_rps"] >= 0.9 * x["offered_rps"]]
         best = max(ok, key=lambda x: x["throughput_rps"]) if ok else None
+        top = max(sweep, key=lambda x: x["throughput_rps"]) if sweep else None
         return {
             "key_budget_gib": budget_gb,
             "keys_prefetched": prefetched,
+            "saturation": None if top is None else {
+                "offered_rps": top["offered_rps"], "throughput_rps": top["throughput_rps"],
+                "roofline_frac": round(top["throughput_rps"] * mm_avg / peak, 4)},
             "sweep": sweep,
             "max_sustained": None if best is None else {
                 "offered_rps": best["offered_rps"], "throughput_rps": best["throughput_rps"],
