@@ -630,24 +630,34 @@ void or_pbs(void* key, const int32_t* ksk, const int32_t* lwe, const int64_t* v,
 
 // Many independent ciphertexts, one per thread at a time (the CPU baseline's
 // throughput form).  lwe/out are [count][n+1]; v is one test polynomial.
-void or_pbs_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,
-                  int32_t* out, int32_t threads) {
+// Many ciphertexts, one per thread at a time; acc_out ([count][(k+1)N]) and
+// ext_out ([count][kN+1]) are optional.
+void or_pbs_batch_all(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,
+                      int32_t* out, int64_t* acc_out, int32_t* ext_out, int32_t threads) {
     const Key& k = *static_cast<Key*>(key);
     const or_params& p = k.p;
+    const size_t accLen = static_cast<size_t>(p.k + 1) * p.N, extLen = static_cast<size_t>(p.k) * p.N + 1;
     std::atomic<int> next{0};
     auto work = [&]() {
-        std::vector<int64_t> a(static_cast<size_t>(p.k + 1) * p.N);
-        std::vector<int32_t> e(static_cast<size_t>(p.k) * p.N + 1);
+        std::vector<int64_t> a(accLen);
+        std::vector<int32_t> e(extLen);
         for (;;) {
             const int c = next.fetch_add(1);
             if (c >= count) break;
             pbs_one(k, ksk, lwe + static_cast<size_t>(c) * (p.n + 1), v, a.data(), e.data(),
                     out + static_cast<size_t>(c) * (p.n + 1), false);
+            if (acc_out) std::memcpy(acc_out + c * accLen, a.data(), accLen * sizeof(int64_t));
+            if (ext_out) std::memcpy(ext_out + c * extLen, e.data(), extLen * sizeof(int32_t));
         }
     };
     std::vector<std::thread> ts;
     for (int t = 0; t < std::max(1, threads); t++) ts.emplace_back(work);
     for (auto& t : ts) t.join();
+}
+
+void or_pbs_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,
+                  int32_t* out, int32_t threads) {
+    or_pbs_batch_all(key, ksk, count, lwe, v, out, nullptr, nullptr, threads);
 }
 
 // Decryption helpers for semantic gates.

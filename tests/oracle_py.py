@@ -62,6 +62,7 @@ def lib():
             "or_keyswitch_u32": (None, [P(_OrParams), vp, vp, vp]),
             "or_pbs": (None, [vp, vp, vp, vp, vp, vp, vp, i32]),
             "or_pbs_batch": (None, [vp, vp, i32, vp, vp, vp, i32]),
+            "or_pbs_batch_all": (None, [vp, vp, i32, vp, vp, vp, vp, vp, i32]),
             "or_lwe_phase": (i32, [vp, vp, i32]),
             "or_glwe_phase": (None, [i32, i32, i32, vp, vp, vp]),
             "or_glwe_phase_modq": (None, [u64, i32, i32, vp, vp, vp]),
@@ -143,6 +144,18 @@ class Oracle:
         t = threads or min(16, os.cpu_count() or 1)
         self.L.or_pbs_batch(self.key, P(self.ksk), lwes.shape[0], P(lwes), P(self._v64(v)), P(out), t)
         return out
+
+    def pbs_batch_all(self, lwes, v, threads=None):
+        """(accumulators, extracted LWEs, keyswitched LWEs) of many ciphertexts."""
+        p = self.p
+        lwes = np.ascontiguousarray(lwes, dtype=np.int32)
+        c = lwes.shape[0]
+        out = np.zeros((c, p.n + 1), np.int32)
+        acc = np.zeros((c, p.k + 1, p.N), np.int64)
+        ext = np.zeros((c, p.k * p.N + 1), np.int32)
+        t = threads or min(16, os.cpu_count() or 1)
+        self.L.or_pbs_batch_all(self.key, P(self.ksk), c, P(lwes), P(self._v64(v)), P(out), P(acc), P(ext), t)
+        return acc.astype(p.torus_dtype), ext, out
 
     def rescale(self, lwe):
         out = np.zeros(self.p.n + 1, np.int32)

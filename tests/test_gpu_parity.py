@@ -1,4 +1,6 @@
 """GPU parity: the HIP engine against the CPU oracle, bit for bit, through the C ABI."""
+import os
+
 import numpy as np
 import pytest
 
@@ -199,3 +201,17 @@ def test_key_file_upload_matches_direct_upload(tmp_path, p32, keys32):
         out_a, _, _ = ctx.pbs_batch(a, cts, v)
         out_b, _, _ = ctx.pbs_batch(b, cts, v)
     assert np.array_equal(out_a, out_b)
+
+
+def test_all_4096_outputs_match_oracle(p32, keys32, ctx32, orc32):
+    """Config 1's full batch: all 4096 x 3 outputs (accumulator, extracted LWE,
+    keyswitched LWE) against the oracle, the oracle on all host threads."""
+    ctx, kid = ctx32
+    bits, cts = sign_cts(keys32, 4096, seed=1)
+    v = yg.sign_test_polynomial(p32)
+    out, acc, ext = ctx.pbs_batch(kid, cts, v, want_acc=True, want_extract=True)
+    print("oracle: 4096 ciphertexts", flush=True)
+    o_acc, o_ext, o_out = orc32.pbs_batch_all(cts, v, threads=min(16, os.cpu_count() or 1))
+    assert np.array_equal(acc, o_acc), "accumulators differ"
+    assert np.array_equal(ext, o_ext), "extracted LWEs differ"
+    assert np.array_equal(out, o_out), "keyswitched LWEs differ"

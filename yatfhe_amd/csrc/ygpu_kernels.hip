@@ -188,13 +188,15 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 #pragma unroll
                     for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
                 }
+                // one column at a time: the scheduling barrier keeps the other
+                // column's loads from being hoisted over these products
+#pragma unroll
+                for (int r = 0; r < 16; r++) own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
-                    oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
-                }
+                for (int r = 0; r < 16; r++) oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
             }
 
             // partner sums: hand column 1-tm to the other team, gather the full

@@ -135,16 +135,19 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                 x[r] = static_cast<double>(dv);
             }
             ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
-            double kbo[16], kbt[16];
+            // one column at a time: the scheduling barrier keeps the second
+            // column's loads from being hoisted over the first column's
+            // products, which would hold 32 more registers live
+            double kb[16];
 #pragma unroll
-            for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
+            for (int r = 0; r < 16; r++) kb[r] = kpo[r * T];
 #pragma unroll
-            for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
+            for (int r = 0; r < 16; r++) own[r] = __dadd_rn(own[r], mulmod(x[r], kb[r], pr));
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
-                oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
-            }
+            for (int r = 0; r < 16; r++) kb[r] = kpt[r * T];
+#pragma unroll
+            for (int r = 0; r < 16; r++) oth[r] = __dadd_rn(oth[r], mulmod(x[r], kb[r], pr));
         }
 
         // partner sums (same prime): hand column 1-tm to team (1-tm, tp)

@@ -45,9 +45,10 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         ctx->primes[0] = P_AUX_0;
         ctx->primes[1] = P_AUX_1;
     }
-    // default tile: one ciphertext per workgroup (barriers couple only its two
-    // teams), twiddles in LDS where they fit (TORUS32, N = 1024)
-    ctx->tile = BrTile{1, params->torus_bits == 32 && params->N == 1024 ? 1 : 0};
+    // default tile: small workgroups (a barrier couples few waves), twiddles in
+    // LDS where they fit; TORUS32 N = 1024: two ciphertexts per workgroup
+    const bool t32 = params->torus_bits == 32 && params->N == 1024;
+    ctx->tile = BrTile{t32 ? 2 : 1, t32 ? 1 : 0};
     if (const char* e = std::getenv("YGPU_CPW")) ctx->tile.cpw = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
