@@ -383,7 +383,8 @@ def leg_serving(yg, args, peak):
     def sweep_at(budget_gb):
         budget = int(budget_gb * (1 << 30))
         sweep = []
-        with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16) as srv:
+        with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16,
+                       eviction=args.serving_eviction) as srv:
             ps32 = srv.add_params(p32)
             ps56 = srv.add_params(p56)
             # tenant id == popularity rank (the load generator draws rank r as tenant r);
@@ -433,6 +434,7 @@ def leg_serving(yg, args, peak):
         top = max(sweep, key=lambda x: x["throughput_rps"]) if sweep else None
         return {
             "key_budget_gib": budget_gb,
+            "eviction": args.serving_eviction,
             "keys_prefetched": prefetched,
             "saturation": None if top is None else {
                 "offered_rps": top["offered_rps"], "throughput_rps": top["throughput_rps"],
@@ -471,6 +473,7 @@ def main():
     ap.add_argument("--serving-budgets-gb", default="48,12",
                     help="device key budgets (GiB) of the serving sweeps; 48 holds the whole key pool")
     ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
+    ap.add_argument("--serving-eviction", default="freq_bytes", choices=["freq_bytes", "lru"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "

@@ -251,8 +251,17 @@ typedef struct ygpu_server_config {
     int32_t queue_max;        /* queued requests per parameter set before submit sheds (0: 2^20) */
     int32_t max_tenants;      /* tenant table size (0: 4096) */
     int32_t max_loads;        /* key loads in flight per parameter set (0: 4) */
+    int32_t eviction;         /* YGPU_EVICT_FREQ_BYTES (0) or YGPU_EVICT_LRU (1) */
+    int32_t reserved;
     int64_t key_budget_bytes; /* device bytes for resident keys (0: 3/4 of free memory) */
 } ygpu_server_config;
+
+/* Key eviction policies of the server's device cache.  FREQ_BYTES evicts the
+ * keys whose loss costs the least host-to-device traffic: decayed request rate
+ * (time constant 1 s) times the key's transfer bytes; LRU evicts the least
+ * recently used.  Both evict only idle keys and keep keys contiguous. */
+#define YGPU_EVICT_FREQ_BYTES 0
+#define YGPU_EVICT_LRU 1
 
 /* One bootstrap request.  The caller owns the struct and the buffers, which
  * must stay valid until the request completes (ygpu_server_wait). */

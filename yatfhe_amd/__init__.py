@@ -54,7 +54,7 @@ class _Job(ctypes.Structure):
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("max_batch", ctypes.c_int32),
                 ("queue_max", ctypes.c_int32), ("max_tenants", ctypes.c_int32), ("max_loads", ctypes.c_int32),
-                ("key_budget_bytes", ctypes.c_int64)]
+                ("eviction", ctypes.c_int32), ("reserved", ctypes.c_int32), ("key_budget_bytes", ctypes.c_int64)]
 
 
 class _Request(ctypes.Structure):
@@ -603,10 +603,11 @@ class Server:
     """Continuous-batching bootstrapping server (ygpu_server_*)."""
 
     def __init__(self, device: int = 0, key_budget_bytes: int = 0, lanes: int = 0, max_batch: int = 0,
-                 queue_max: int = 0, max_tenants: int = 0, max_loads: int = 0):
+                 queue_max: int = 0, max_tenants: int = 0, max_loads: int = 0, eviction: str = "freq_bytes"):
         self.lib = load_library()
         self.h = ctypes.c_void_p()
-        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, max_loads, key_budget_bytes)
+        policy = {"freq_bytes": 0, "lru": 1}[eviction]
+        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, max_loads, policy, 0, key_budget_bytes)
         _check(self.lib.ygpu_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.params = []
         self.tenant_params = []

@@ -21,9 +21,10 @@
 //    thread marks it ready when the copy completes.  Requests whose key is not
 //    ready stay queued, so batches never wait for other tenants' loads, and a
 //    key is loaded once however many requests ask for it (single flight).
-//    When the arena is full, the contiguous window of idle keys whose newest
-//    use is oldest is evicted (LRU with contiguity); keys used by batches in
-//    flight or being loaded are never evicted.
+//    When the arena is full, a contiguous window of idle keys is evicted:
+//    the one whose keys imply the least reload traffic (decayed request rate
+//    times transfer bytes), or, under the LRU policy, whose newest use is
+//    oldest; keys used by batches in flight or being loaded are never evicted.
 //  * Admission by device memory: a tenant whose key cannot be placed (every
 //    window holds a key in use) stays queued and is retried when a batch or a
 //    load completes.
@@ -72,6 +73,9 @@ struct Tenant {
     int inuse = 0;         // batches in flight holding the key
     bool fresh = false;    // loaded but not yet used by a batch
     uint64_t last_use = 0;
+    double freq = 0;       // request rate, decayed with time constant FREQ_TAU_NS
+    int64_t freq_ns = 0;   // when freq was last brought up to date
+    size_t load_bytes = 0; // host-to-device bytes of one load
     std::deque<ygpu_request*> pending;
     bool listed = false;   // in its parameter set's `order`
 };
@@ -188,13 +192,25 @@ void evict(ygpu_server* s, int t) {
 // progress).
 bool evictable(const Tenant& tn) { return tn.inuse == 0 && tn.ready && !tn.fresh; }
 
-// Eviction preference: keys without waiting requests first, then least
-// recently used.
-uint64_t evict_cost(const Tenant& tn) { return tn.last_use + (tn.pending.empty() ? 0 : (uint64_t{1} << 48)); }
+constexpr double FREQ_TAU_NS = 1e9;
+
+double decayed_freq(const Tenant& tn, int64_t now) {
+    return tn.freq * std::exp(-static_cast<double>(now - tn.freq_ns) / FREQ_TAU_NS);
+}
+
+// Cost of evicting a key.  Keys with waiting requests come last under both
+// policies.  LRU: the newest use (a window's cost is its newest key);
+// FREQ_BYTES: the reload traffic the key's request rate implies (a window's
+// cost is the sum over its keys).
+double evict_cost(const ygpu_server* s, const Tenant& tn, int64_t now) {
+    const double waiting = tn.pending.empty() ? 0.0 : 1e30;
+    if (s->cfg.eviction == YGPU_EVICT_LRU) return static_cast<double>(tn.last_use) + waiting;
+    return decayed_freq(tn, now) * static_cast<double>(tn.load_bytes) + waiting;
+}
 
 // Finds `bytes` of arena: best-fitting free region, else the contiguous window
-// of free regions and idle keys whose newest key use is oldest.  Returns false
-// when every window holds a key in use or being loaded.
+// of free regions and idle keys of least eviction cost.  Returns false when
+// every window holds a key in use or being loaded.
 bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
     auto best_free = [&]() {
         int best = -1;
@@ -206,31 +222,36 @@ bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
     };
     int best = best_free();
     if (best < 0) {
+        const int64_t now = now_ns();
+        const bool lru = s->cfg.eviction == YGPU_EVICT_LRU;
         size_t bi = 0, bj = 0;
-        uint64_t bcost = UINT64_MAX;
+        bool found = false;
+        double bcost = 0;
         size_t bevict = SIZE_MAX;
         for (size_t i = 0; i < s->regions.size(); i++) {
             size_t sum = 0, evicted = 0;
-            uint64_t cost = 0;
+            double cost = 0;
             size_t j = i;
             for (; j < s->regions.size() && sum < bytes; j++) {
                 const Region& r = s->regions[j];
                 if (r.tenant >= 0) {
                     const Tenant& tn = s->tenants[r.tenant];
                     if (!evictable(tn)) break;
-                    cost = std::max(cost, evict_cost(tn));
+                    const double c = evict_cost(s, tn, now);
+                    cost = lru ? std::max(cost, c) : cost + c;
                     evicted += r.size;
                 }
                 sum += r.size;
             }
-            if (sum >= bytes && (cost < bcost || (cost == bcost && evicted < bevict))) {
+            if (sum >= bytes && (!found || cost < bcost || (cost == bcost && evicted < bevict))) {
+                found = true;
                 bcost = cost;
                 bevict = evicted;
                 bi = i;
                 bj = j;
             }
         }
-        if (bcost == UINT64_MAX) return false;
+        if (!found) return false;
         for (size_t k = bi; k < bj; k++)
             if (s->regions[k].tenant >= 0) evict(s, s->regions[k].tenant);
         merge_free(s);
@@ -585,6 +606,8 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
     if (s->cfg.queue_max <= 0) s->cfg.queue_max = 1 << 20;
     if (s->cfg.max_tenants <= 0) s->cfg.max_tenants = 4096;
     if (s->cfg.max_loads <= 0) s->cfg.max_loads = 4;
+    if (s->cfg.eviction != YGPU_EVICT_FREQ_BYTES && s->cfg.eviction != YGPU_EVICT_LRU)
+        return fail(YGPU_ERR_INVALID_ARG, "unknown eviction policy");
     if (s->cfg.lanes > 16 || s->cfg.max_batch > (1 << 16) || s->cfg.max_tenants > (1 << 20))
         return fail(YGPU_ERR_INVALID_ARG, "lanes <= 16, max_batch <= 65536 and max_tenants <= 2^20 required");
     YG_HIP(hipSetDevice(cfg->device));
@@ -686,6 +709,7 @@ int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coe
         Tenant t;
         t.pset = pset;
         t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
+        t.load_bytes = f.packed_bytes + f.planes_bytes;
         e = hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming);
         if (e != hipSuccess) {
             hipHostFree(f.packed);
@@ -712,6 +736,7 @@ int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* t
         t.pset = s->tenants[src_tenant].pset;
         t.form = s->tenants[src_tenant].form;
         t.bytes = s->tenants[src_tenant].bytes;
+        t.load_bytes = s->tenants[src_tenant].load_bytes;
         YG_HIP(hipSetDevice(s->cfg.device));
         YG_HIP(hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming));
         *tenant = static_cast<int32_t>(s->tenants.size());
@@ -766,6 +791,11 @@ int ygpu_server_submit(ygpu_server* s, ygpu_request* req) {
         if (req->arrival_ns == 0) req->arrival_ns = now_ns();
         req->seq = ++s->tick;
         req->key_hit = tn.ready ? 1 : 0;
+        {
+            const int64_t now = now_ns();
+            tn.freq = decayed_freq(tn, now) + 1.0;
+            tn.freq_ns = now;
+        }
         s->st.request_hits += req->key_hit;
         tn.pending.push_back(req);
         ps.queued++;
