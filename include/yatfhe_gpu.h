@@ -93,6 +93,19 @@ int ygpu_get_params(const ygpu_ctx* ctx, ygpu_params* out);
  * bootstrapping.h:16-114) and its keyswitching key (keyswitching.h:14-31)
  * resident on the device, the BSK transformed into the engine's NTT form. */
 int ygpu_key_upload(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* key_id);
+/* WWL+24 succinct bootstrapping key (BootstrappingKeyWWL24,
+ * bootstrapping.h:116-136): per step only the body-digit rows cPrime[lvl] of
+ * the RGSW, plus one TRLEV of s^2 shared by all steps (BootstrappingKeyWWL24::
+ * s2Dft; symEncTrlevWithKeyNtt, trlev.cpp:75-94), both in coefficient form:
+ * bsk_half[i][lvl][col][N] (i < n, lvl < level, col < k+1) and s2[l2][col][N]
+ * (l2 < s2_level, the reference's param.l; s2_level * radix_bits must fit the
+ * torus).  The device derives the mask-digit rows c[lvl][k] once, exactly as
+ * switchTrlweToSecretEmbeddingNtt (trgsw.cpp:861-909) forms them at every step
+ * of blindRotateWWL24Ntt (blind_rotate.cpp:1383-1412), so the key id bootstraps
+ * with the results of blindRotateWWL24Ntt through every entry point below.  The
+ * transfer is about half the full key's. */
+int ygpu_key_upload_succinct(ygpu_ctx* ctx, const uint64_t* bsk_half, const uint64_t* s2, int32_t s2_level, const int32_t* ksk,
+                             int32_t* key_id);
 int ygpu_key_release(ygpu_ctx* ctx, int32_t key_id);
 /* Device bytes held by a key (BSK in NTT form + KSK). */
 int ygpu_key_bytes(const ygpu_ctx* ctx, int32_t key_id, int64_t* bytes);
@@ -205,6 +218,16 @@ int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes
 int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe,
                 int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads);
 
+/* Seeded WWL+24 succinct key generation (genBootstrappingKeyWWL24,
+ * bootstrapping.cpp:106-126): cPrime[lvl] of step i encrypts s_i * 2^(w -
+ * (lvl+1) b) at coefficient 0 (symEncTrlweMultiSampleNtt, trlwe.cpp:103-106:
+ * mask uniform over the torus, body completed mod qNtt); s2[l2] is the TRLEV
+ * (a - s * 2^(w - (l2+1) b), a s + e) of symEncTrlevWithKeyNtt (trlev.cpp:75-94),
+ * formed in torus arithmetic.  Secret keys and KSK equal ygpu_keygen's for the
+ * same seed.  Outputs as ygpu_key_upload_succinct's inputs. */
+int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t s2_level,
+                         int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_half, uint64_t* s2, int32_t* ksk, int32_t threads);
+
 /* LWE encryption of torus messages mu[count] (symEncTlwe, tlwe.cpp:20-31) under
  * lwe_key, Gaussian noise 2^sigma_log2, seeded.  out: int32[count][n+1]. */
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count,
@@ -295,6 +318,11 @@ int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl)
  * transformed once into device-ready form in pinned host memory, loaded into
  * the device arena on first use. */
 int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* tenant);
+/* A tenant with a WWL+24 succinct key (inputs as ygpu_key_upload_succinct):
+ * the host holds and transfers the succinct form, about half the full key's
+ * bytes; each load derives the full rows on the device. */
+int ygpu_server_add_tenant_succinct(ygpu_server* s, int32_t pset, const uint64_t* bsk_half, const uint64_t* s2, int32_t s2_level,
+                                    const int32_t* ksk, int32_t* tenant);
 /* A new tenant sharing another tenant's host key material but cached on the
  * device on its own (separate residency, loads and evictions). */
 int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* tenant);

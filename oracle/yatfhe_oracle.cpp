@@ -23,6 +23,10 @@
 //   blind rotate / extract  decryption identities (phase = X^{-b+sum a s} v)
 //   keyswitch               int64 and uint32-wrap accumulations cross-checked,
 //                           decryption of every message slot
+//   succinct-key derivation schoolbook route vs NTT route, bit for bit; every
+//                           derived row decrypts to -s_i 2^(w-(lvl+1)b) z (the RGSW
+//                           mask-digit row) within the noise bound; blind rotation
+//                           with the derived key decrypts to X^{sum a s} v
 #include <cstdint>
 #include <cstring>
 #include <cmath>
@@ -396,6 +400,64 @@ void blind_rotate(const Key& key, const int32_t* abar, int64_t* acc, bool school
     }
 }
 
+// switchTrlweToSecretEmbeddingNtt (trgsw.cpp:861-909) for one (step, level) of
+// a WWL+24 succinct key (BootstrappingKeyWWL24, bootstrapping.h:116-136), in
+// coefficient form.  cp: the body-digit row cPrime[lvl] [k+1][N]; s2: the TRLEV
+// of s^2 [s2_level][k+1][N]; out: the full RGSW rows of this level [k+1 rows][k+1
+// cols][N].  Row k1 < k is derived: the torus values of cPrime.a[k1] (applyIntt:
+// centre, low w bits), decomposed into s2_level digits (decomposeRow with
+// param.l levels), and
+//   c[k1].a[k2] = sum_l2 D_l2 * s2[l2].a[k2] + cPrime.b,   c[k1].b = sum_l2 D_l2 * s2[l2].b   (mod q);
+// row k is cPrime itself.  The products are negacyclic mod q, by this oracle's
+// NTT (the reference's route, one inverse per output column) or by schoolbook.
+void derive_succinct_row(const or_params& p, const Ntt& ntt, const std::vector<uint64_t>& s2ntt, int s2_level,
+                         const uint64_t* cp, const uint64_t* s2, uint64_t* out, bool schoolbook) {
+    const int N = p.N, K = p.k, K1 = p.k + 1, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    std::vector<int64_t> ta(N), dig(static_cast<size_t>(s2_level) * N);
+    std::vector<uint64_t> acc(static_cast<size_t>(K1) * N), tmp(N), bntt(cp + static_cast<size_t>(K) * N, cp + static_cast<size_t>(K1) * N);
+    if (!schoolbook) ntt.forward(bntt.data());
+    for (int k1 = 0; k1 < K; k1++) {
+        for (int j = 0; j < N; j++) ta[j] = intt_exit(cp[static_cast<size_t>(k1) * N + j], q, w);
+        digits_row(ta.data(), N, s2_level, p.radix_bits, w, dig.data());
+        std::fill(acc.begin(), acc.end(), 0);
+        for (int l2 = 0; l2 < s2_level; l2++) {
+            const int64_t* d = &dig[static_cast<size_t>(l2) * N];
+            if (schoolbook) {
+                for (int c = 0; c < K1; c++) {
+                    schoolbook_modq(q, N, d, s2 + (static_cast<size_t>(l2) * K1 + c) * N, tmp.data());
+                    for (int j = 0; j < N; j++) acc[static_cast<size_t>(c) * N + j] = (acc[static_cast<size_t>(c) * N + j] + tmp[j]) % q;
+                }
+            } else {
+                for (int j = 0; j < N; j++) tmp[j] = lift(d[j], q);
+                ntt.forward(tmp.data());
+                for (int c = 0; c < K1; c++) {
+                    const uint64_t* sk = &s2ntt[(static_cast<size_t>(l2) * K1 + c) * N];
+                    for (int j = 0; j < N; j++) {
+                        const uint64_t prod = static_cast<uint64_t>(static_cast<u128>(tmp[j]) * sk[j] % q);
+                        acc[static_cast<size_t>(c) * N + j] = (acc[static_cast<size_t>(c) * N + j] + prod) % q;
+                    }
+                }
+            }
+        }
+        for (int c = 0; c < K; c++) {
+            const uint64_t* b = schoolbook ? cp + static_cast<size_t>(K) * N : bntt.data();
+            for (int j = 0; j < N; j++) acc[static_cast<size_t>(c) * N + j] = (acc[static_cast<size_t>(c) * N + j] + b[j]) % q;
+        }
+        for (int c = 0; c < K1; c++) {
+            if (!schoolbook) ntt.inverse(&acc[static_cast<size_t>(c) * N]);
+            std::memcpy(out + (static_cast<size_t>(k1) * K1 + c) * N, &acc[static_cast<size_t>(c) * N], sizeof(uint64_t) * N);
+        }
+    }
+    std::memcpy(out + static_cast<size_t>(K) * K1 * N, cp, sizeof(uint64_t) * K1 * N);
+}
+
+std::vector<uint64_t> s2_transformed(const Ntt& ntt, int s2_level, int K1, int N, const uint64_t* s2) {
+    std::vector<uint64_t> t(s2, s2 + static_cast<size_t>(s2_level) * K1 * N);
+    for (size_t o = 0; o < t.size(); o += N) ntt.forward(&t[o]);
+    return t;
+}
+
 int64_t torus_q(const or_params& p) { return p.torus_bits >= 63 ? 0 : (int64_t{1} << p.torus_bits); }
 int64_t lwe_q(const or_params& p) { return int64_t{1} << p.lwe_q_bits; }
 
@@ -658,6 +720,38 @@ void or_pbs_batch_all(void* key, const int32_t* ksk, int32_t count, const int32_
 void or_pbs_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,
                   int32_t* out, int32_t threads) {
     or_pbs_batch_all(key, ksk, count, lwe, v, out, nullptr, nullptr, threads);
+}
+
+// The full key of a WWL+24 succinct key: every (step, level) through
+// derive_succinct_row by the NTT route.  half [n][level][k+1][N], s2
+// [s2_level][k+1][N] -> full [n][level][k+1][k+1][N], all residues mod q.
+// blindRotateWWL24Ntt (blind_rotate.cpp:1383-1412) derives the same rows at every
+// step and then runs the MP21 external product on them, so blind rotation with
+// the full key reproduces it.
+void or_derive_succinct(const or_params* p, int32_t s2_level, const uint64_t* half, const uint64_t* s2, uint64_t* full, int32_t threads) {
+    const Ntt ntt(p->q_ntt, p->N);
+    const int K1 = p->k + 1, N = p->N;
+    const std::vector<uint64_t> s2ntt = s2_transformed(ntt, s2_level, K1, N, s2);
+    const int rows = p->n * p->level;
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int r = next.fetch_add(1);
+            if (r >= rows) break;
+            derive_succinct_row(*p, ntt, s2ntt, s2_level, half + static_cast<size_t>(r) * K1 * N, s2,
+                                full + static_cast<size_t>(r) * K1 * K1 * N, false);
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 0; t < std::max(1, threads); t++) ts.emplace_back(work);
+    for (auto& t : ts) t.join();
+}
+
+// One (step, level) of the derivation, by the NTT route (0) or schoolbook (1).
+void or_derive_succinct_row(const or_params* p, int32_t s2_level, const uint64_t* cp, const uint64_t* s2, uint64_t* out, int32_t schoolbook) {
+    const Ntt ntt(p->q_ntt, p->N);
+    const std::vector<uint64_t> s2ntt = schoolbook ? std::vector<uint64_t>{} : s2_transformed(ntt, s2_level, p->k + 1, p->N, s2);
+    derive_succinct_row(*p, ntt, s2ntt, s2_level, cp, s2, out, schoolbook != 0);
 }
 
 // Decryption helpers for semantic gates.

@@ -66,6 +66,8 @@ def lib():
             "or_lwe_phase": (i32, [vp, vp, i32]),
             "or_glwe_phase": (None, [i32, i32, i32, vp, vp, vp]),
             "or_glwe_phase_modq": (None, [u64, i32, i32, vp, vp, vp]),
+            "or_derive_succinct": (None, [P(_OrParams), i32, vp, vp, vp, i32]),
+            "or_derive_succinct_row": (None, [P(_OrParams), i32, vp, vp, vp, i32]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -197,4 +199,36 @@ def glwe_phase(params, acc, glwe_key):
     out = np.zeros(params.N, np.int64)
     a = np.ascontiguousarray(np.asarray(acc).astype(np.int64).reshape(-1))
     lib().or_glwe_phase(params.torus_bits, params.k, params.N, P(a), P(np.ascontiguousarray(glwe_key, dtype=np.int32)), P(out))
+    return out
+
+
+def derive_succinct(params, keys, threads=8):
+    """Full coefficient-form BSK [n][l][k+1][k+1][N] of a succinct key (oracle NTT route)."""
+    k1 = params.k + 1
+    full = np.zeros((params.n, params.level, k1, k1, params.N), np.uint64)
+    half = np.ascontiguousarray(keys.bsk_half, dtype=np.uint64)
+    s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+    cp = or_params(params)
+    lib().or_derive_succinct(ctypes.byref(cp), s2.shape[0], P(half), P(s2), P(full), threads)
+    return full
+
+
+def derive_succinct_row(params, keys, i, lvl, schoolbook=False):
+    """Rows [k+1][k+1][N] of step i, level lvl, by the NTT route or schoolbook."""
+    k1 = params.k + 1
+    out = np.zeros((k1, k1, params.N), np.uint64)
+    cp_row = np.ascontiguousarray(keys.bsk_half[i, lvl], dtype=np.uint64)
+    s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+    cp = or_params(params)
+    lib().or_derive_succinct_row(ctypes.byref(cp), s2.shape[0], P(cp_row), P(s2), P(out), 1 if schoolbook else 0)
+    return out
+
+
+def glwe_phase_modq(q, row, glwe_key):
+    """(b - sum a_i z_i) mod q, centred, of a coefficient-form row [k+1][N]."""
+    row = np.ascontiguousarray(row, dtype=np.uint64)
+    k = row.shape[0] - 1
+    N = row.shape[1]
+    out = np.zeros(N, np.int64)
+    lib().or_glwe_phase_modq(q, k, N, P(row), P(np.ascontiguousarray(glwe_key, dtype=np.int32)), P(out))
     return out

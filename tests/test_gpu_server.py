@@ -130,3 +130,49 @@ def test_loadgen_results_match_and_account(p32, tenants):
         assert res["completed"] + res["shed"] == 600
         assert 0 < res["p50_ms"] <= res["p99_ms"] <= res["max_ms"]
         assert res["key_evictions"] > 0
+
+
+def test_succinct_tenants_alone_equal_mix(p32):
+    """Tenants holding WWL+24 succinct keys: each load moves the succinct form and
+    derives the full rows on the device; results equal the single-context
+    succinct upload's, alone and in a mix with full-key tenants, across evictions."""
+    v32 = yg.sign_test_polynomial(p32)
+    full_keys = [yg.keygen(p32, seed=700 + t) for t in range(2)]
+    succ_keys = [yg.keygen_succinct(p32, seed=710 + t, s2_level=4) for t in range(3)]
+    budget = 3 * 83 << 20  # three of the five keys
+    with yg.Server(0, key_budget_bytes=budget, lanes=2, max_batch=128) as srv:
+        ps = srv.add_params(p32)
+        tids = [srv.add_tenant(ps, k) for k in full_keys] + [srv.add_tenant_succinct(ps, k) for k in succ_keys]
+        keys = full_keys + succ_keys
+        work = [(t, k, t32_inputs(k, 5, 80 + i)) for i, (t, k) in enumerate(zip(tids, keys))]
+        print("server: succinct alone", flush=True)
+        alone = {}
+        for t, k, cts in work:
+            for c in range(len(cts)):
+                r = srv.request(t, cts[c], v32)
+                srv.submit(r)
+                alone[(t, c)] = srv.wait(r, timeout=60).copy()
+        loads_alone = srv.stats()
+        print("server: succinct mix", flush=True)
+        reqs = [(t, c, srv.request(t, cts[c], v32)) for t, k, cts in work for c in range(len(cts))]
+        for i in np.random.default_rng(4).permutation(len(reqs)):
+            srv.submit(reqs[i][2])
+        for t, c, r in reqs:
+            assert np.array_equal(srv.wait(r, timeout=60), alone[(t, c)]), f"tenant {t} ciphertext {c}"
+        st = srv.stats()
+        print("server stats", st, flush=True)
+        assert st["key_evictions"] > 0 and st["failed"] == 0
+        assert st["peak_resident_bytes"] <= budget
+        assert loads_alone["key_loads"] >= len(tids)
+        # transfer forms: 49-bit packed residues (+1 word) and the KSK byte planes
+        poly_words = p32.N * 49 // 64
+        planes = p32.k * p32.N * p32.ks_level * (p32.n + 1) * 4
+        full_b = (p32.n * p32.level * 4 * poly_words + 1) * 8 + planes
+        succ_b = ((p32.n * p32.level * 2 + 4 * 2) * poly_words + 1) * 8 + planes
+        assert st["host_key_bytes"] == 2 * full_b + 3 * succ_b
+        assert succ_b < 0.66 * full_b
+    print("server: succinct context", flush=True)
+    for t, k, cts in work:
+        ref = context_results(p32, k, cts, v32)
+        for c in range(len(cts)):
+            assert np.array_equal(ref[c], alone[(t, c)]), f"tenant {t} ciphertext {c}: server differs from context"

@@ -154,6 +154,7 @@ def load_library(path: str | None = None):
         "ygpu_destroy": (c_int, [c_vp]),
         "ygpu_get_params": (c_int, [c_vp, P(_Params)]),
         "ygpu_key_upload": (c_int, [c_vp, c_vp, c_vp, P(c_i32)]),
+        "ygpu_key_upload_succinct": (c_int, [c_vp, c_vp, c_vp, c_i32, c_vp, P(c_i32)]),
         "ygpu_key_release": (c_int, [c_vp, c_i32]),
         "ygpu_key_bytes": (c_int, [c_vp, c_i32, P(c_i64)]),
         "ygpu_pbs": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
@@ -182,6 +183,7 @@ def load_library(path: str | None = None):
         "ygpu_graph_destroy": (c_int, [c_vp, c_i32]),
         "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
+        "ygpu_keygen_succinct": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
         "ygpu_server_create": (c_int, [P(_ServerConfig), P(c_vp)]),
         "ygpu_server_destroy": (c_int, [c_vp]),
@@ -189,6 +191,7 @@ def load_library(path: str | None = None):
         "ygpu_server_set_tile": (c_int, [c_vp, c_i32, c_i32, c_i32]),
         "ygpu_server_add_tenant": (c_int, [c_vp, c_i32, c_vp, c_vp, P(c_i32)]),
         "ygpu_server_add_tenant_shared": (c_int, [c_vp, c_i32, P(c_i32)]),
+        "ygpu_server_add_tenant_succinct": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, P(c_i32)]),
         "ygpu_server_tenant_params": (c_int, [c_vp, c_i32, P(c_i32)]),
         "ygpu_server_prefetch": (c_int, [c_vp, c_i32]),
         "ygpu_server_submit": (c_int, [c_vp, P(_Request)]),
@@ -224,6 +227,7 @@ def exported_symbols():
         "ygpu_server_prefetch",
         "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_wait_for", "ygpu_server_stats_get", "ygpu_server_loadgen",
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
+        "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
     ]
 
 
@@ -274,6 +278,46 @@ def keygen(params: Params, seed: int, sigma_log2_lwe: float | None = None,
     _check(lib.ygpu_keygen(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe,
                            _ptr(lk), _ptr(gk), _ptr(bsk), _ptr(ksk), threads))
     return KeySet(params, lk, gk, bsk, ksk)
+
+
+@dataclass
+class SuccinctKeySet:
+    """WWL+24 succinct key (BootstrappingKeyWWL24): body-digit rows and the TRLEV of s^2."""
+    params: Params
+    lwe_key: np.ndarray   # int32[n]
+    glwe_key: np.ndarray  # int32[kN]
+    bsk_half: np.ndarray  # uint64[n][l][k+1][N], coefficient residues mod q (rows cPrime[lvl])
+    s2: np.ndarray        # uint64[s2_level][k+1][N], coefficient residues mod q
+    ksk: np.ndarray       # int32[kN][t][n+1]
+
+    @property
+    def s2_level(self) -> int:
+        return int(self.s2.shape[0])
+
+
+def max_s2_level(params: Params) -> int:
+    """Most s^2 digits that fit the torus (decomposeRow's limit), at most 8."""
+    return min(8, params.torus_bits // params.radix_bits)
+
+
+def keygen_succinct(params: Params, seed: int, s2_level: int | None = None, sigma_log2_lwe: float | None = None,
+                    sigma_log2_glwe: float | None = None, threads: int = 0) -> SuccinctKeySet:
+    """Seeded WWL+24 succinct keys (ygpu_keygen_succinct); secret keys and KSK equal keygen's for the seed."""
+    lib = load_library()
+    dl, dg = default_sigmas(params)
+    sigma_log2_lwe = dl if sigma_log2_lwe is None else sigma_log2_lwe
+    sigma_log2_glwe = dg if sigma_log2_glwe is None else sigma_log2_glwe
+    s2_level = max_s2_level(params) if s2_level is None else s2_level
+    k1 = params.k + 1
+    lk = np.zeros(params.n, np.int32)
+    gk = np.zeros(params.k * params.N, np.int32)
+    half = np.zeros((params.n, params.level, k1, params.N), np.uint64)
+    s2 = np.zeros((s2_level, k1, params.N), np.uint64)
+    ksk = np.zeros(params.ksk_shape, np.int32)
+    cp = params.c()
+    _check(lib.ygpu_keygen_succinct(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe, s2_level,
+                                    _ptr(lk), _ptr(gk), _ptr(half), _ptr(s2), _ptr(ksk), threads))
+    return SuccinctKeySet(params, lk, gk, half, s2, ksk)
 
 
 def save_keys(path: str, keys: KeySet):
@@ -409,11 +453,21 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def upload_key(self, keys: KeySet) -> int:
+    def upload_key(self, keys) -> int:
+        if isinstance(keys, SuccinctKeySet):
+            return self.upload_key_succinct(keys)
         kid = ctypes.c_int32(-1)
         bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
         ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
         _check(self.lib.ygpu_key_upload(self.h, _ptr(bsk), _ptr(ksk), ctypes.byref(kid)))
+        return kid.value
+
+    def upload_key_succinct(self, keys: SuccinctKeySet) -> int:
+        kid = ctypes.c_int32(-1)
+        half = np.ascontiguousarray(keys.bsk_half, dtype=np.uint64)
+        s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+        ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        _check(self.lib.ygpu_key_upload_succinct(self.h, _ptr(half), _ptr(s2), keys.s2_level, _ptr(ksk), ctypes.byref(kid)))
         return kid.value
 
     def upload_key_file(self, path: str) -> int:
@@ -633,11 +687,23 @@ class Server:
     def set_tile(self, pset: int, cpw: int, twl: int):
         _check(self.lib.ygpu_server_set_tile(self.h, pset, cpw, twl))
 
-    def add_tenant(self, pset: int, keys: KeySet) -> int:
+    def add_tenant(self, pset: int, keys) -> int:
+        if isinstance(keys, SuccinctKeySet):
+            return self.add_tenant_succinct(pset, keys)
         t = ctypes.c_int32()
         bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
         ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
         _check(self.lib.ygpu_server_add_tenant(self.h, pset, _ptr(bsk), _ptr(ksk), ctypes.byref(t)))
+        self.tenant_params.append(pset)
+        return t.value
+
+    def add_tenant_succinct(self, pset: int, keys: SuccinctKeySet) -> int:
+        t = ctypes.c_int32()
+        half = np.ascontiguousarray(keys.bsk_half, dtype=np.uint64)
+        s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+        ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        _check(self.lib.ygpu_server_add_tenant_succinct(self.h, pset, _ptr(half), _ptr(s2), keys.s2_level, _ptr(ksk),
+                                                        ctypes.byref(t)))
         self.tenant_params.append(pset)
         return t.value
 
@@ -708,5 +774,5 @@ __all__ = [
     "Params", "PARAMS_T32_N1024", "PARAMS_T56_N2048", "Q_49P_T32", "Q_61P_T56", "KeySet", "keygen",
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
     "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
-    "save_keys", "key_file_params", "load_key_arrays",
+    "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
 ]

@@ -13,6 +13,7 @@
 #include <memory>
 #include <functional>
 #include <cstdlib>
+#include <algorithm>
 #include "yatfhe_gpu.h"
 #include "ygpu_math.h"
 #include "ygpu_kernels.h"
@@ -369,40 +370,74 @@ inline int host_threads() {
 
 // Key material as it crosses to the device: the BSK residues bit-packed
 // (bits = bit length of q - 1 per value, polynomials word-aligned because N is a
-// multiple of 64), expanded into the engine's form on the device by
-// k_bsk_expand, and the KSK as signed-byte planes.
+// multiple of 64), expanded into the engine's form on the device, and the KSK as
+// signed-byte planes.  Two BSK forms: the full key (every RGSW row; expanded by
+// k_bsk_expand) and the WWL+24 succinct key (only the body-digit rows cPrime
+// plus a TRLEV of s^2 with s2_level levels, appended after them; the mask-digit
+// rows are derived on the device by k_bsk_derive).  Either way the device holds
+// the same engine-form BSK.
 struct HostKeyForm {
     std::vector<uint64_t> packed;
     std::vector<uint32_t> planes;
     size_t bsk_bytes = 0;  // engine-form BSK bytes on the device
+    int s2_level = 0;      // 0: full key; > 0: succinct key
     size_t bytes() const { return bsk_bytes + planes.size() * sizeof(uint32_t); }
+    size_t transfer_bytes() const { return packed.size() * sizeof(uint64_t) + planes.size() * sizeof(uint32_t); }
 };
 
 inline int key_bits(uint64_t q) { return 64 - __builtin_clzll(q - 1); }
 
 inline size_t bsk_polys(const ygpu_params& p) { return static_cast<size_t>(p.n) * p.level * (p.k + 1) * (p.k + 1); }
+// Body-digit rows of a succinct key, and the polynomials of its s^2 TRLEV.
+inline size_t half_polys(const ygpu_params& p) { return static_cast<size_t>(p.n) * p.level * (p.k + 1); }
+inline size_t s2_polys(const ygpu_params& p, int s2_level) { return static_cast<size_t>(s2_level) * (p.k + 1); }
+inline size_t poly_words(const ygpu_params& p) { return static_cast<size_t>(p.N) * key_bits(p.q_ntt) / 64; }
 
 // Words of the packed BSK (+1 so that a two-word read of the last value stays in bounds).
-inline size_t packed_words(const ygpu_params& p) {
-    return bsk_polys(p) * p.N * static_cast<size_t>(key_bits(p.q_ntt)) / 64 + 1;
+inline size_t packed_words(const ygpu_params& p) { return bsk_polys(p) * poly_words(p) + 1; }
+inline size_t packed_words_succinct(const ygpu_params& p, int s2_level) {
+    return (half_polys(p) + s2_polys(p, s2_level)) * poly_words(p) + 1;
+}
+constexpr int S2_MAX_LEVEL = 8;
+// Device staging for one key load: the packed words, then (succinct keys) the
+// engine-form s^2 TRLEV, 256-byte aligned.
+inline size_t s2_stage_offset(const ygpu_params& p, int s2_level) {
+    return (packed_words_succinct(p, s2_level) * sizeof(uint64_t) + 255) / 256 * 256;
+}
+inline size_t stage_bytes(const ygpu_params& p, int np, int s2_level) {
+    if (s2_level <= 0) return packed_words(p) * sizeof(uint64_t);
+    return s2_stage_offset(p, s2_level) + s2_polys(p, s2_level) * np * p.N * sizeof(double);
+}
+// Enough staging for a load of either form.
+inline size_t stage_bytes_any(const ygpu_params& p, int np) {
+    size_t b = stage_bytes(p, np, 0);
+    for (int l = 1; l <= S2_MAX_LEVEL; l++) b = std::max(b, stage_bytes(p, np, l));
+    return b;
 }
 
-inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, HostKeyForm& out) {
-    const ygpu_params& p = ctx->p;
+inline int check_s2_level(const ygpu_params& p, int s2_level) {
+    if (s2_level < 1 || s2_level > S2_MAX_LEVEL) return fail(YGPU_ERR_UNSUPPORTED, "s2_level must be in [1, 8]");
+    // decomposeRow's own check (gadget_decomposition.h:194-206) on the s^2 digits
+    if (s2_level * p.radix_bits > p.torus_bits)
+        return fail(YGPU_ERR_INVALID_ARG, "decomposeRow: " + std::to_string(s2_level) + " digits of " + std::to_string(p.radix_bits) +
+                                              " bits do not fit the " + std::to_string(p.torus_bits) + "-bit torus");
+    return YGPU_OK;
+}
+
+// Packs `polys` polynomials of N residues (each < q) at `bits` per value.
+// Returns the index of the first residue not below q, or -1.
+inline int64_t pack_residues(const uint64_t* src, size_t polys, const ygpu_params& p, uint64_t* dst) {
     const int N = p.N;
     const uint64_t q = p.q_ntt;
     const int bits = key_bits(q);
-    const size_t polys = bsk_polys(p);
-    const size_t polyWords = static_cast<size_t>(N) * bits / 64;  // exact: N is a multiple of 64
-    out.bsk_bytes = polys * ctx->np * N * sizeof(double);
-    out.packed.assign(packed_words(p), 0);
+    const size_t polyWords = poly_words(p);  // exact: N is a multiple of 64
     std::atomic<int64_t> bad{-1};
     parallel_for(static_cast<int>(polys), host_threads(), [&](int pi) {
-        const uint64_t* src = bsk_coeff + static_cast<size_t>(pi) * N;
-        uint64_t* dst = out.packed.data() + static_cast<size_t>(pi) * polyWords;
+        const uint64_t* in = src + static_cast<size_t>(pi) * N;
+        uint64_t* out = dst + static_cast<size_t>(pi) * polyWords;
         size_t bit = 0;
         for (int j = 0; j < N; j++, bit += bits) {
-            const uint64_t v = src[j];
+            const uint64_t v = in[j];
             if (v >= q) {
                 int64_t expect = -1;
                 bad.compare_exchange_strong(expect, static_cast<int64_t>(pi) * N + j);
@@ -410,38 +445,68 @@ inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int3
             }
             const size_t w = bit >> 6;
             const int off = static_cast<int>(bit & 63);
-            dst[w] |= v << off;
-            if (off + bits > 64) dst[w + 1] |= v >> (64 - off);
+            out[w] |= v << off;
+            if (off + bits > 64) out[w + 1] |= v >> (64 - off);
         }
     });
-    if (bad.load() >= 0) return fail(YGPU_ERR_INVALID_ARG, "BSK coefficient " + std::to_string(bad.load()) + " is not below q_ntt");
-    // KSK as signed-byte planes: rows grouped by four, word = the four rows'
-    // byte b of the same column; x = sum_b s_b 2^(8b) (mod 2^32), s_b signed
+    return bad.load();
+}
+
+// KSK as signed-byte planes: rows grouped by four, word = the four rows' byte b
+// of the same column; x = sum_b s_b 2^(8b) (mod 2^32), s_b signed.
+inline void ksk_planes(const ygpu_params& p, const int32_t* ksk, std::vector<uint32_t>& planes) {
     const size_t cols = static_cast<size_t>(p.n) + 1;
-    const size_t kskRows = static_cast<size_t>(p.k) * N * p.ks_level;
+    const size_t kskRows = static_cast<size_t>(p.k) * p.N * p.ks_level;
     const size_t groups = (kskRows + 3) / 4;
-    out.planes.assign(groups * 4 * cols, 0u);
+    planes.assign(groups * 4 * cols, 0u);
     for (size_t row = 0; row < kskRows; row++) {
         const size_t g = row / 4, r = row % 4;
         for (size_t c = 0; c < cols; c++) {
             int64_t x = ksk[row * cols + c];
             for (int b = 0; b < 4; b++) {
                 const int8_t sb = static_cast<int8_t>(static_cast<uint8_t>(x & 0xFF));
-                out.planes[(g * 4 + b) * cols + c] |= (static_cast<uint32_t>(static_cast<uint8_t>(sb))) << (8 * r);
+                planes[(g * 4 + b) * cols + c] |= (static_cast<uint32_t>(static_cast<uint8_t>(sb))) << (8 * r);
                 x = (x - sb) >> 8;
             }
         }
     }
+}
+
+inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, HostKeyForm& out) {
+    const ygpu_params& p = ctx->p;
+    out.s2_level = 0;
+    out.bsk_bytes = bsk_polys(p) * ctx->np * p.N * sizeof(double);
+    out.packed.assign(packed_words(p), 0);
+    const int64_t bad = pack_residues(bsk_coeff, bsk_polys(p), p, out.packed.data());
+    if (bad >= 0) return fail(YGPU_ERR_INVALID_ARG, "BSK coefficient " + std::to_string(bad) + " is not below q_ntt");
+    ksk_planes(p, ksk, out.planes);
     return YGPU_OK;
 }
 
-// Enqueues the expansion of a packed BSK (device memory) into the engine form.
-inline hipError_t expand_bsk(const ygpu_ctx* ctx, const uint64_t* d_packed, double* d_out, hipStream_t s) {
+// Succinct (WWL+24) key: bsk_half [i][lvl][col][N] (the body-digit rows
+// cPrime[lvl] of every step) and s2 [l2][col][N] (TRLEV of s^2), residues mod q.
+inline int prepare_key_host_succinct(ygpu_ctx* ctx, const uint64_t* bsk_half, const uint64_t* s2, int s2_level, const int32_t* ksk,
+                                     HostKeyForm& out) {
+    const ygpu_params& p = ctx->p;
+    int rc = check_s2_level(p, s2_level);
+    if (rc) return rc;
+    out.s2_level = s2_level;
+    out.bsk_bytes = bsk_polys(p) * ctx->np * p.N * sizeof(double);
+    out.packed.assign(packed_words_succinct(p, s2_level), 0);
+    int64_t bad = pack_residues(bsk_half, half_polys(p), p, out.packed.data());
+    if (bad >= 0) return fail(YGPU_ERR_INVALID_ARG, "succinct BSK coefficient " + std::to_string(bad) + " is not below q_ntt");
+    bad = pack_residues(s2, s2_polys(p, s2_level), p, out.packed.data() + half_polys(p) * poly_words(p));
+    if (bad >= 0) return fail(YGPU_ERR_INVALID_ARG, "s2 coefficient " + std::to_string(bad) + " is not below q_ntt");
+    ksk_planes(p, ksk, out.planes);
+    return YGPU_OK;
+}
+
+inline BskExpandArgs expand_args(const ygpu_ctx* ctx, const uint64_t* d_packed, int polys, double* d_out) {
     BskExpandArgs a{};
     a.packed = d_packed;
     a.bits = key_bits(ctx->p.q_ntt);
     a.q = ctx->p.q_ntt;
-    a.polys = static_cast<int>(bsk_polys(ctx->p));
+    a.polys = polys;
     a.np = ctx->np;
     a.out = d_out;
     a.tw = ctx->d_tw;
@@ -450,7 +515,41 @@ inline hipError_t expand_bsk(const ygpu_ctx* ctx, const uint64_t* d_packed, doub
         a.p[k] = ctx->primes[k];
         a.ninv[k] = centred_double(ctx->ntt[k].ninv, ctx->ntt[k].p);
     }
-    return launch_bsk_expand(ctx->logn, a, s);
+    return a;
+}
+
+// Enqueues the expansion of a packed BSK (device memory) into the engine form.
+inline hipError_t expand_bsk(const ygpu_ctx* ctx, const uint64_t* d_packed, double* d_out, hipStream_t s) {
+    return launch_bsk_expand(ctx->logn, expand_args(ctx, d_packed, static_cast<int>(bsk_polys(ctx->p)), d_out), s);
+}
+
+// Enqueues the expansion of a key staged at d_stage (stage_bytes of its form):
+// full keys as above; succinct keys by expanding the s^2 TRLEV into the staging
+// tail and deriving every row with k_bsk_derive.
+inline hipError_t expand_key(const ygpu_ctx* ctx, int s2_level, void* d_stage, double* d_out, hipStream_t s) {
+    const uint64_t* packed = static_cast<const uint64_t*>(d_stage);
+    if (s2_level <= 0) return expand_bsk(ctx, packed, d_out, s);
+    const ygpu_params& p = ctx->p;
+    double* s2x = reinterpret_cast<double*>(static_cast<char*>(d_stage) + s2_stage_offset(p, s2_level));
+    hipError_t e = launch_bsk_expand(ctx->logn, expand_args(ctx, packed + half_polys(p) * poly_words(p),
+                                                            static_cast<int>(s2_polys(p, s2_level)), s2x), s);
+    if (e != hipSuccess) return e;
+    const BskExpandArgs x = expand_args(ctx, packed, 0, d_out);
+    BskDeriveArgs a{};
+    a.half = packed;
+    a.s2 = s2x;
+    a.bits = x.bits;
+    a.q = x.q;
+    a.n = p.n;
+    a.level = p.level;
+    a.s2_level = s2_level;
+    a.radix_bits = p.radix_bits;
+    a.np = ctx->np;
+    a.out = d_out;
+    a.tw = ctx->d_tw;
+    for (int k = 0; k < 2; k++) { a.prime[k] = x.prime[k]; a.p[k] = x.p[k]; a.ninv[k] = x.ninv[k]; }
+    a.crt = ctx->crt;
+    return launch_bsk_derive(ctx->logn, ctx->tbits, a, s);
 }
 
 // Bytes a key occupies on the device, without building it.

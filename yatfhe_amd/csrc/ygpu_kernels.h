@@ -195,4 +195,23 @@ struct BskExpandArgs {
 };
 hipError_t launch_bsk_expand(int logn, const BskExpandArgs& a, hipStream_t s);
 
+// Derivation of the full key from a WWL+24 succinct key: the packed body-digit
+// rows ([i][lvl][col][N] residues, `bits` per value) and the TRLEV of s^2 in
+// engine form ([l2][col][prime][N]) -> the engine-form BSK [i][lvl][row][col][prime][N].
+struct BskDeriveArgs {
+    const uint64_t* half;
+    const double* s2;
+    int bits;
+    uint64_t q;
+    int n, level, s2_level, radix_bits;
+    int np;
+    double* out;
+    const double* tw;       // per prime: forward then inverse twiddles at np*2N
+    PrimeF64 prime[2];
+    uint64_t p[2];
+    double ninv[2];
+    CrtConst crt;
+};
+hipError_t launch_bsk_derive(int logn, int tbits, const BskDeriveArgs& a, hipStream_t s);
+
 }  // namespace ygpu

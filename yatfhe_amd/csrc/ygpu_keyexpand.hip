@@ -55,4 +55,154 @@ hipError_t launch_bsk_expand(int logn, const BskExpandArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// ===================================================== succinct-key derivation
+// Residue number g of a packed array, as its centred representative mod q.
+__device__ __forceinline__ int64_t unpack_centred(const uint64_t* packed, uint64_t g, int bits, uint64_t q) {
+    const uint64_t mask = (bits >= 64) ? ~0ULL : ((1ULL << bits) - 1);
+    const uint64_t bit = g * static_cast<uint64_t>(bits);
+    const uint64_t w = bit >> 6;
+    const int off = static_cast<int>(bit & 63);
+    uint64_t v = packed[w] >> off;
+    if (off + bits > 64) v |= packed[w + 1] << (64 - off);
+    v &= mask;
+    return v > (q - 1) / 2 ? static_cast<int64_t>(v) - static_cast<int64_t>(q) : static_cast<int64_t>(v);
+}
+
+// Digit of level `lvl` among the l zero-mean balanced digits of a TBITS-bit
+// torus value (gadget_decomposition.h:75-95): the carry chain runs from the
+// least significant level (l - 1) up to lvl.
+template <int TBITS>
+__device__ __forceinline__ int digit_at(int64_t x, int l, int rb, int lvl) {
+    const uint64_t mask = (TBITS >= 64) ? ~0ULL : ((1ULL << TBITS) - 1);
+    const uint64_t u = static_cast<uint64_t>(x) & mask;
+    const bool neg = (u >> (TBITS - 1)) & 1;
+    const int dropped = TBITS - l * rb;
+    const uint64_t bias = dropped > 0 ? (1ULL << (dropped - 1)) : 0;
+    const uint64_t m = (neg ? ((0ULL - u) & mask) : u) + bias;
+    const int B = 1 << rb, halfB = B >> 1;
+    int carry = 0, bal = 0;
+    for (int v = l - 1; v >= lvl; v--) {
+        const int shift = TBITS - (v + 1) * rb;
+        const int digit = static_cast<int>((m >> shift) & static_cast<uint64_t>(B - 1)) + carry;
+        const int tie = v > 0 ? static_cast<int>((m >> (shift + 2 * rb - 1)) & 1) : 0;
+        carry = (digit > halfB) | ((digit == halfB) & tie);
+        bal = carry ? digit - B : digit;
+    }
+    return neg ? -bal : bal;
+}
+
+// Engine form of one polynomial of centred residues mod q: reduce to the
+// centred range mod p, transform forward, scale by N^-1, centre, store.
+template <int LOGN, int TW>
+__device__ __forceinline__ void store_engine_form(const int64_t (&c)[16], double* scr, int tl, const double* tw, const PrimeF64& pr,
+                                                  int64_t p, double ninv, double* out) {
+    constexpr int T = (1 << LOGN) / 16;
+    double x[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        int64_t v = c[r] % p;
+        if (v > p / 2) v -= p;
+        if (v < -(p / 2)) v += p;
+        x[r] = static_cast<double>(v);
+    }
+    ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+#pragma unroll
+    for (int r = 0; r < 16; r++) out[r * T + tl] = centre(mulmod(x[r], ninv, pr), pr);
+}
+
+// One team of T = N/16 lanes per (step i, level lvl, column col) of a WWL+24
+// succinct key (BootstrappingKeyWWL24, bootstrapping.h:116-136): writes the
+// body-digit row cPrime[lvl] (copied) and the mask-digit row c[lvl][0]
+// derived by switchTrlweToSecretEmbeddingNtt (trgsw.cpp:861-909):
+//   c.a = sum_l2 D_l2(cPrime.a) * s2[l2].a + cPrime.b,   c.b = sum_l2 D_l2(cPrime.a) * s2[l2].b
+// where D_l2 is digit level l2 of the torus value of cPrime.a (applyIntt's
+// centring and sign extension, then decomposeRow with s2_level digits).  The
+// products are exact negacyclic products mod q: per transform prime p the sum
+// Y = sum D * centred(s2) is formed exactly mod p (|Y| < s2_level N B/2 q/2,
+// below p0 p1 / 2 for two primes; Y mod q needs only one prime when p0 = q),
+// reduced mod q by Garner CRT, and the result is brought to engine form like
+// any uploaded residue.  s2 arrives in engine form ([l2][col][prime][N]).
+template <int LOGN, int TBITS>
+__global__ void __launch_bounds__((1 << LOGN) / 16) k_bsk_derive(BskDeriveArgs a) {
+    constexpr int N = 1 << LOGN, T = N / 16, TW = T / 64, K1 = 2;
+    __shared__ double scr[Scratch<LOGN>::DOUBLES];
+    const int col = static_cast<int>(blockIdx.x) % K1;
+    const int il = static_cast<int>(blockIdx.x) / K1;  // i * level + lvl
+    const int tl = static_cast<int>(threadIdx.x);
+    const int np = a.np;
+    const int64_t q = static_cast<int64_t>(a.q);
+    const int64_t qhalf = (q - 1) / 2;
+    // half-key polynomials of this (i, lvl): cPrime.a at 2 il, cPrime.b at 2 il + 1
+    const uint64_t polyA = static_cast<uint64_t>(il) * K1;
+    double* rowBase = a.out + static_cast<size_t>(il) * K1 * K1 * np * N;  // [row][col][prime][N]
+
+    // body-digit row (row 1): cPrime's column col, unchanged
+    int64_t c[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) c[r] = unpack_centred(a.half, (polyA + col) * N + tl + T * r, a.bits, a.q);
+    for (int j = 0; j < np; j++) {
+        store_engine_form<LOGN, TW>(c, scr, tl, a.tw + static_cast<size_t>(j) * 2 * N, a.prime[j], static_cast<int64_t>(a.p[j]),
+                                    a.ninv[j], rowBase + ((static_cast<size_t>(1) * K1 + col) * np + j) * N);
+    }
+
+    // mask-digit row (row 0): torus values of cPrime.a, and cPrime.b for column 0
+    int64_t t[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) t[r] = sext_t<TBITS>(unpack_centred(a.half, polyA * N + tl + T * r, a.bits, a.q));
+    double y0[16];
+    for (int j = 0; j < np; j++) {
+        const PrimeF64 pr = a.prime[j];
+        const double* tw = a.tw + static_cast<size_t>(j) * 2 * N;
+        double acc[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = 0.0;
+        for (int l2 = 0; l2 < a.s2_level; l2++) {
+            double x[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) x[r] = static_cast<double>(digit_at<TBITS>(t[r], a.s2_level, a.radix_bits, l2));
+            ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+            const double* s2 = a.s2 + ((static_cast<size_t>(l2) * K1 + col) * np + j) * N + tl;
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[r] = centre(__dadd_rn(acc[r], mulmod(x[r], s2[r * T], pr)), pr);
+        }
+        ntt_inverse<LOGN, TW>(acc, scr, tl, tw + N, pr);
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[r] = centre(acc[r], pr);
+        if (j == 0) {
+#pragma unroll
+            for (int r = 0; r < 16; r++) y0[r] = acc[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; r++) c[r] = crt_mod_q(y0[r], acc[r], a.prime[1], a.crt);
+        }
+    }
+    if (np == 1) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) c[r] = f64_to_i64(y0[r]);  // p0 = q: already centred mod q
+    }
+    if (col == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            int64_t v = c[r] + unpack_centred(a.half, (polyA + 1) * N + tl + T * r, a.bits, a.q);
+            if (v > qhalf) v -= q;
+            if (v < -qhalf) v += q;
+            c[r] = v;
+        }
+    }
+    for (int j = 0; j < np; j++) {
+        store_engine_form<LOGN, TW>(c, scr, tl, a.tw + static_cast<size_t>(j) * 2 * N, a.prime[j], static_cast<int64_t>(a.p[j]),
+                                    a.ninv[j], rowBase + (static_cast<size_t>(col) * np + j) * N);
+    }
+}
+
+hipError_t launch_bsk_derive(int logn, int tbits, const BskDeriveArgs& a, hipStream_t s) {
+    const unsigned blocks = static_cast<unsigned>(a.n) * static_cast<unsigned>(a.level) * 2u;
+    if (logn == 10 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<10, 32>), dim3(blocks), dim3(64), 0, s, a);
+    else if (logn == 11 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<11, 32>), dim3(blocks), dim3(128), 0, s, a);
+    else if (logn == 10 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<10, 56>), dim3(blocks), dim3(64), 0, s, a);
+    else if (logn == 11 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<11, 56>), dim3(blocks), dim3(128), 0, s, a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 }  // namespace ygpu

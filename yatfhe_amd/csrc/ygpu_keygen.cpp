@@ -11,6 +11,9 @@
 //                                   (trlwe.cpp:25-48 samples them in the NTT domain,
 //                                   which is the same distribution)
 //   genTlweKeySwitchingKey          (keyswitching.cpp:19-29): KSK[i][j] = LWE_s(z_i * 2^(w - (j+1) b))
+//   genBootstrappingKeyWWL24        (bootstrapping.cpp:106-126): the succinct key, body-digit
+//                                   rows only (torus-uniform masks, symEncTrlweMultiSampleNtt)
+//                                   plus the TRLEV of s^2 (symEncTrlevWithKeyNtt, trlev.cpp:75-94)
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -51,6 +54,55 @@ inline uint64_t lift(int64_t v, uint64_t q) {
 // stream identifiers
 constexpr uint64_t S_LWE_KEY = 1, S_GLWE_KEY = 2;
 constexpr uint64_t S_BSK = 1ULL << 40, S_KSK = 2ULL << 40, S_ENC = 3ULL << 40;
+constexpr uint64_t S_HALF = 4ULL << 40, S_S2 = 5ULL << 40;
+
+int worker_threads(int threads) {
+    return threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+}
+
+// genTlweKey / genTrlweKey: binary secret keys.
+void gen_secret_keys(const ygpu_params& p, uint64_t seed, int32_t* lwe_key, int32_t* glwe_key) {
+    Rng rl(seed, S_LWE_KEY);
+    for (int i = 0; i < p.n; i++) lwe_key[i] = static_cast<int32_t>(rl.next() & 1);
+    Rng rg(seed, S_GLWE_KEY);
+    for (int i = 0; i < p.k * p.N; i++) glwe_key[i] = static_cast<int32_t>(rg.next() & 1);
+}
+
+// genTlweKeySwitchingKey (keyswitching.cpp:19-29): rows [i][j][n+1], i < kN.
+void gen_ksk(const ygpu_params& p, uint64_t seed, double sigma_log2_lwe, const int32_t* lwe_key, const int32_t* glwe_key,
+             int32_t* ksk, int nt) {
+    const int n = p.n, t = p.ks_level;
+    const double sl = std::pow(2.0, sigma_log2_lwe + 32);
+    par_for(p.k * p.N * t, nt, [&](int idx) {
+        const int i = idx / t, j = idx % t;
+        Rng r(seed, S_KSK + static_cast<uint64_t>(idx));
+        int32_t* out = ksk + static_cast<size_t>(idx) * (n + 1);
+        uint32_t dot = 0;
+        for (int u = 0; u < n; u++) {
+            out[u] = static_cast<int32_t>(static_cast<uint32_t>(r.next()));
+            if (lwe_key[u]) dot += static_cast<uint32_t>(out[u]);
+        }
+        const uint32_t m = static_cast<uint32_t>(glwe_key[i]) << (p.ks_width_bits - (j + 1) * p.ks_radix_bits);
+        const uint32_t e = static_cast<uint32_t>(static_cast<int64_t>(r.gauss(sl)));
+        out[n] = static_cast<int32_t>(dot + e + m);
+    });
+}
+
+// Negacyclic product in the torus, a * z mod (X^N + 1, 2^w) for a binary z,
+// accumulated into acc (multTorusPolynomialAcc, polynomial.h): wrapping 64-bit
+// sums, whose low w bits are the torus result.
+void torus_mul_binary_acc(int N, int w, const int64_t* a, const int32_t* z, int64_t* acc) {
+    std::vector<uint64_t> sum(N, 0);
+    for (int m = 0; m < N; m++) {
+        if (!z[m]) continue;
+        for (int j = 0; j < N; j++) {
+            const int d = j + m;
+            const uint64_t v = static_cast<uint64_t>(a[j]);
+            if (d < N) sum[d] += v; else sum[d - N] -= v;
+        }
+    }
+    for (int j = 0; j < N; j++) acc[j] = sext(static_cast<int64_t>(static_cast<uint64_t>(acc[j]) + sum[j]), w);
+}
 
 }  // namespace
 
@@ -64,14 +116,7 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
     const int N = p.N, K = p.k, K1 = p.k + 1, L = p.level, n = p.n, w = p.torus_bits;
     const uint64_t q = p.q_ntt;
     try {
-        {
-            Rng r(seed, S_LWE_KEY);
-            for (int i = 0; i < n; i++) lwe_key[i] = static_cast<int32_t>(r.next() & 1);
-        }
-        {
-            Rng r(seed, S_GLWE_KEY);
-            for (int i = 0; i < K * N; i++) glwe_key[i] = static_cast<int32_t>(r.next() & 1);
-        }
+        gen_secret_keys(p, seed, lwe_key, glwe_key);
         HostNtt ntt(q, N);
         // transformed GLWE key
         std::vector<std::vector<uint64_t>> zt(K, std::vector<uint64_t>(N));
@@ -80,7 +125,7 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
             ntt.forward(zt[c].data());
         }
         const double sg = std::pow(2.0, sigma_log2_glwe + w);
-        const int nt = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+        const int nt = worker_threads(threads);
         // BSK rows: [i][lvl][row][col][N]
         par_for(n * L * K1, nt, [&](int idx) {
             const int i = idx / (L * K1);
@@ -109,22 +154,76 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
             }
             std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
         });
-        // KSK rows: [i][j][n+1], i < kN
-        const double sl = std::pow(2.0, sigma_log2_lwe + 32);
-        const int t = p.ks_level;
-        par_for(K * N * t, nt, [&](int idx) {
-            const int i = idx / t, j = idx % t;
-            Rng r(seed, S_KSK + static_cast<uint64_t>(idx));
-            int32_t* out = ksk + static_cast<size_t>(idx) * (n + 1);
-            uint32_t dot = 0;
-            for (int u = 0; u < n; u++) {
-                out[u] = static_cast<int32_t>(static_cast<uint32_t>(r.next()));
-                if (lwe_key[u]) dot += static_cast<uint32_t>(out[u]);
+        gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
+    } catch (...) {
+        return YGPU_ERR_INVALID_ARG;
+    }
+    return YGPU_OK;
+}
+
+int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t s2_level,
+                         int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_half, uint64_t* s2, int32_t* ksk, int32_t threads) {
+    if (!params || !lwe_key || !glwe_key || !bsk_half || !s2 || !ksk) return YGPU_ERR_INVALID_ARG;
+    const ygpu_params p = *params;
+    if (p.k < 1 || p.N < 2 || p.n < 1 || p.level < 1 || p.ks_level < 1) return YGPU_ERR_INVALID_ARG;
+    if (s2_level < 1 || s2_level * p.radix_bits > p.torus_bits) return YGPU_ERR_INVALID_ARG;
+    const int N = p.N, K = p.k, K1 = p.k + 1, L = p.level, n = p.n, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    try {
+        gen_secret_keys(p, seed, lwe_key, glwe_key);
+        HostNtt ntt(q, N);
+        std::vector<std::vector<uint64_t>> zt(K, std::vector<uint64_t>(N));
+        for (int c = 0; c < K; c++) {
+            for (int j = 0; j < N; j++) zt[c][j] = static_cast<uint64_t>(glwe_key[c * N + j]);
+            ntt.forward(zt[c].data());
+        }
+        const double sg = std::pow(2.0, sigma_log2_glwe + w);
+        const int nt = worker_threads(threads);
+        // cPrime rows [i][lvl][col][N] (genBootstrappingKeyWWL24, bootstrapping.cpp:106-126):
+        // symEncTrlweMultiSampleNtt of mu = s_i 2^(w - (lvl+1) b) at coefficient 0
+        par_for(n * L, nt, [&](int idx) {
+            const int i = idx / L, lvl = idx % L;
+            Rng r(seed, S_HALF + static_cast<uint64_t>(idx));
+            uint64_t* out = bsk_half + static_cast<size_t>(idx) * K1 * N;
+            std::vector<uint64_t> body(N, 0), tmp(N);
+            for (int c = 0; c < K; c++) {
+                uint64_t* a = out + static_cast<size_t>(c) * N;
+                for (int j = 0; j < N; j++) a[j] = lift(sext(static_cast<int64_t>(r.next()), w), q);
+                std::memcpy(tmp.data(), a, sizeof(uint64_t) * N);
+                ntt.forward(tmp.data());
+                for (int j = 0; j < N; j++) tmp[j] = mulmod(tmp[j], zt[c][j], q);
+                ntt.inverse(tmp.data());
+                for (int j = 0; j < N; j++) body[j] = (body[j] + tmp[j]) % q;
             }
-            const uint32_t m = static_cast<uint32_t>(glwe_key[i]) << (p.ks_width_bits - (j + 1) * p.ks_radix_bits);
-            const uint32_t e = static_cast<uint32_t>(static_cast<int64_t>(r.gauss(sl)));
-            out[n] = static_cast<int32_t>(dot + e + m);
+            const int64_t mu = sext(static_cast<int64_t>(lwe_key[i]) << (w - (lvl + 1) * p.radix_bits), w);
+            for (int j = 0; j < N; j++) {
+                const int64_t bt = sext(r.gauss(sg) + (j == 0 ? mu : 0), w);
+                body[j] = (body[j] + lift(bt, q)) % q;
+            }
+            std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
         });
+        // TRLEV of s^2 [l2][col][N] (symEncTrlevWithKeyNtt, trlev.cpp:75-94): a
+        // TRLWE of 0 in the torus (encTrlevSingleSample), then a_c -= s_c 2^(w - (l2+1) b)
+        par_for(s2_level, nt, [&](int l2) {
+            Rng r(seed, S_S2 + static_cast<uint64_t>(l2));
+            std::vector<int64_t> a(static_cast<size_t>(K) * N), b(N);
+            for (int j = 0; j < N; j++) b[j] = sext(r.gauss(sg), w);
+            for (int c = 0; c < K; c++) {
+                int64_t* ac = &a[static_cast<size_t>(c) * N];
+                for (int j = 0; j < N; j++) ac[j] = sext(static_cast<int64_t>(r.next()), w);
+                torus_mul_binary_acc(N, w, ac, glwe_key + static_cast<size_t>(c) * N, b.data());
+            }
+            const int shift = w - (l2 + 1) * p.radix_bits;
+            uint64_t* out = s2 + static_cast<size_t>(l2) * K1 * N;
+            for (int c = 0; c < K; c++) {
+                for (int j = 0; j < N; j++) {
+                    const int64_t v = sext(a[static_cast<size_t>(c) * N + j] - (static_cast<int64_t>(glwe_key[c * N + j]) << shift), w);
+                    out[static_cast<size_t>(c) * N + j] = lift(v, q);
+                }
+            }
+            for (int j = 0; j < N; j++) out[static_cast<size_t>(K) * N + j] = lift(b[j], q);
+        });
+        gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
     } catch (...) {
         return YGPU_ERR_INVALID_ARG;
     }

@@ -111,34 +111,55 @@ int ygpu_get_params(const ygpu_ctx* ctx, ygpu_params* out) {
     return YGPU_OK;
 }
 
+namespace {
+
+// Makes a host key form resident: device BSK + KSK, the packed residues staged
+// and expanded on the context stream.  Called with the context lock held.
+int upload_host_form(ygpu_ctx* ctx, const HostKeyForm& hk, int32_t* key_id) {
+    KeyRec rec;
+    rec.live = true;
+    rec.bytes = hk.bytes();
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&rec.d_bsk), hk.bsk_bytes);
+    if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc BSK: ") + hipGetErrorString(e));
+    e = hipMalloc(reinterpret_cast<void**>(&rec.d_ksk), hk.planes.size() * sizeof(uint32_t));
+    if (e != hipSuccess) { hipFree(rec.d_bsk); return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc KSK: ") + hipGetErrorString(e)); }
+    // packed residues -> staging -> engine form, on the context stream
+    int rc = ensure(&ctx->d_stage, &ctx->stage_cap, stage_bytes(ctx->p, ctx->np, hk.s2_level));
+    if (rc) { hipFree(rec.d_bsk); hipFree(rec.d_ksk); return rc; }
+    YG_HIP(hipMemcpyAsync(ctx->d_stage, hk.packed.data(), hk.packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    YG_HIP(expand_key(ctx, hk.s2_level, ctx->d_stage, rec.d_bsk, ctx->stream));
+    YG_HIP(hipMemcpyAsync(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    YG_HIP(hipStreamSynchronize(ctx->stream));
+    int id = -1;
+    for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
+    if (id < 0) { id = static_cast<int>(ctx->keys.size()); ctx->keys.push_back(rec); }
+    else ctx->keys[id] = rec;
+    rc = upload_tables(ctx);
+    if (rc) return rc;
+    *key_id = id;
+    return YGPU_OK;
+}
+
+}  // namespace
+
 int ygpu_key_upload(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* key_id) {
     if (!bsk_coeff || !ksk || !key_id) return fail(YGPU_ERR_INVALID_ARG, "null argument");
     return with_ctx(ctx, [&]() -> int {
         HostKeyForm hk;
         int rc = prepare_key_host(ctx, bsk_coeff, ksk, hk);
         if (rc) return rc;
-        KeyRec rec;
-        rec.live = true;
-        rec.bytes = hk.bytes();
-        hipError_t e = hipMalloc(reinterpret_cast<void**>(&rec.d_bsk), hk.bsk_bytes);
-        if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc BSK: ") + hipGetErrorString(e));
-        e = hipMalloc(reinterpret_cast<void**>(&rec.d_ksk), hk.planes.size() * sizeof(uint32_t));
-        if (e != hipSuccess) { hipFree(rec.d_bsk); return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc KSK: ") + hipGetErrorString(e)); }
-        // packed residues -> staging -> engine form, on the context stream
-        rc = ensure(&ctx->d_stage, &ctx->stage_cap, hk.packed.size() * sizeof(uint64_t));
-        if (rc) { hipFree(rec.d_bsk); hipFree(rec.d_ksk); return rc; }
-        YG_HIP(hipMemcpyAsync(ctx->d_stage, hk.packed.data(), hk.packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
-        YG_HIP(expand_bsk(ctx, static_cast<const uint64_t*>(ctx->d_stage), rec.d_bsk, ctx->stream));
-        YG_HIP(hipMemcpyAsync(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        int id = -1;
-        for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
-        if (id < 0) { id = static_cast<int>(ctx->keys.size()); ctx->keys.push_back(rec); }
-        else ctx->keys[id] = rec;
-        rc = upload_tables(ctx);
+        return upload_host_form(ctx, hk, key_id);
+    });
+}
+
+int ygpu_key_upload_succinct(ygpu_ctx* ctx, const uint64_t* bsk_half, const uint64_t* s2, int32_t s2_level, const int32_t* ksk,
+                             int32_t* key_id) {
+    if (!bsk_half || !s2 || !ksk || !key_id) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_ctx(ctx, [&]() -> int {
+        HostKeyForm hk;
+        int rc = prepare_key_host_succinct(ctx, bsk_half, s2, s2_level, ksk, hk);
         if (rc) return rc;
-        *key_id = id;
-        return YGPU_OK;
+        return upload_host_form(ctx, hk, key_id);
     });
 }
 

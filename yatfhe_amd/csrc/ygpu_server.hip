@@ -54,12 +54,14 @@ inline int64_t now_ns() {
     return std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
 }
 
-// A key's transfer form in pinned host memory: bit-packed BSK residues and
-// KSK planes.  bsk_bytes is the engine-form BSK size on the device.
+// A key's transfer form in pinned host memory: bit-packed BSK residues (full
+// key, or succinct key with s2_level > 0) and KSK planes.  bsk_bytes is the
+// engine-form BSK size on the device.
 struct PinnedKey {
     uint64_t* packed = nullptr;
     uint32_t* planes = nullptr;
     size_t packed_bytes = 0, planes_bytes = 0, bsk_bytes = 0;
+    int s2_level = 0;
 };
 
 struct Tenant {
@@ -118,7 +120,7 @@ struct PSet {
     std::deque<int> order;            // tenants with pending requests, oldest first
     int64_t queued = 0;               // pending requests
     hipStream_t copy_stream = nullptr;
-    uint64_t* d_stage = nullptr;      // packed BSK of the load being expanded
+    void* d_stage = nullptr;          // packed BSK of the load being expanded (either form)
     std::deque<int> loads;            // tenants being loaded, in copy order
     std::deque<int> inflight;         // lanes in launch order
     std::condition_variable cv_work;  // requests, a free lane, a ready key or free memory
@@ -284,7 +286,7 @@ int start_load(ygpu_server* s, PSet& ps, int t) {
     // packed residues -> staging -> engine form; loads of one parameter set
     // run in order on its copy stream, so one staging buffer serves them all
     hipError_t e = hipMemcpyAsync(ps.d_stage, f.packed, f.packed_bytes, hipMemcpyHostToDevice, cs);
-    if (e == hipSuccess) e = expand_bsk(ps.ctx, ps.d_stage, reinterpret_cast<double*>(base), cs);
+    if (e == hipSuccess) e = expand_key(ps.ctx, f.s2_level, ps.d_stage, reinterpret_cast<double*>(base), cs);
     if (e == hipSuccess) e = hipMemcpyAsync(base + align256(f.bsk_bytes), f.planes, f.planes_bytes, hipMemcpyHostToDevice, cs);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_bsk_tab + t, s->h_bsk_tab + t, sizeof(void*), hipMemcpyHostToDevice, cs);
     if (e == hipSuccess) e = hipMemcpyAsync(s->d_ksk_tab + t, s->h_ksk_tab + t, sizeof(void*), hipMemcpyHostToDevice, cs);
@@ -649,7 +651,7 @@ int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* p
             ygpu_destroy(ps->ctx);
             return fail(YGPU_ERR_HIP, "copy stream");
         }
-        if (hipMalloc(reinterpret_cast<void**>(&ps->d_stage), packed_words(ps->p) * sizeof(uint64_t)) != hipSuccess) {
+        if (hipMalloc(&ps->d_stage, stage_bytes_any(ps->p, ps->ctx->np)) != hipSuccess) {
             hipStreamDestroy(ps->copy_stream);
             ygpu_destroy(ps->ctx);
             return fail(YGPU_ERR_OUT_OF_MEMORY, "key staging buffer");
@@ -678,51 +680,77 @@ int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl)
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Registers a tenant whose host form `make` builds with the parameter set's
+// context: pinned copy of the transfer form, arena size, load bytes.
+template <class MakeForm>
+int add_tenant_form(ygpu_server* s, int32_t pset, int32_t* tenant, MakeForm&& make) {
+    ygpu_ctx* ctx = nullptr;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+        if (static_cast<int>(s->tenants.size()) >= s->cfg.max_tenants) return fail(YGPU_ERR_INVALID_ARG, "max_tenants reached");
+        ctx = s->psets[pset]->ctx;
+    }
+    hipSetDevice(s->cfg.device);
+    HostKeyForm hk;
+    int rc = make(ctx, hk);
+    if (rc) return rc;
+    PinnedKey f;
+    f.bsk_bytes = hk.bsk_bytes;
+    f.s2_level = hk.s2_level;
+    f.packed_bytes = hk.packed.size() * sizeof(uint64_t);
+    f.planes_bytes = hk.planes.size() * sizeof(uint32_t);
+    if (align256(f.bsk_bytes) + align256(f.planes_bytes) > s->arena_bytes)
+        return fail(YGPU_ERR_OUT_OF_MEMORY, "key larger than the key budget");
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&f.packed), f.packed_bytes, hipHostMallocDefault));
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&f.planes), f.planes_bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        hipHostFree(f.packed);
+        return server_fail_hip(e, "pinned key");
+    }
+    std::memcpy(f.packed, hk.packed.data(), f.packed_bytes);
+    std::memcpy(f.planes, hk.planes.data(), f.planes_bytes);
+    Tenant t;
+    t.pset = pset;
+    t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
+    t.load_bytes = f.packed_bytes + f.planes_bytes;
+    e = hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        hipHostFree(f.packed);
+        hipHostFree(f.planes);
+        return server_fail_hip(e, "event");
+    }
+    std::lock_guard<std::mutex> g(s->mu);
+    t.form = static_cast<int>(s->forms.size());
+    s->forms.push_back(f);
+    *tenant = static_cast<int32_t>(s->tenants.size());
+    s->tenants.push_back(t);
+    s->st.host_key_bytes += static_cast<int64_t>(f.packed_bytes + f.planes_bytes);
+    return YGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int ygpu_server_add_tenant(ygpu_server* s, int32_t pset, const uint64_t* bsk_coeff, const int32_t* ksk, int32_t* tenant) {
     return with_server(s, [&] {
         if (!bsk_coeff || !ksk || !tenant) return fail(YGPU_ERR_INVALID_ARG, "null argument");
-        ygpu_ctx* ctx = nullptr;
-        {
-            std::lock_guard<std::mutex> g(s->mu);
-            if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
-            if (static_cast<int>(s->tenants.size()) >= s->cfg.max_tenants) return fail(YGPU_ERR_INVALID_ARG, "max_tenants reached");
-            ctx = s->psets[pset]->ctx;
-        }
-        hipSetDevice(s->cfg.device);
-        HostKeyForm hk;
-        int rc = prepare_key_host(ctx, bsk_coeff, ksk, hk);
-        if (rc) return rc;
-        PinnedKey f;
-        f.bsk_bytes = hk.bsk_bytes;
-        f.packed_bytes = hk.packed.size() * sizeof(uint64_t);
-        f.planes_bytes = hk.planes.size() * sizeof(uint32_t);
-        if (align256(f.bsk_bytes) + align256(f.planes_bytes) > s->arena_bytes)
-            return fail(YGPU_ERR_OUT_OF_MEMORY, "key larger than the key budget");
-        YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&f.packed), f.packed_bytes, hipHostMallocDefault));
-        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&f.planes), f.planes_bytes, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            hipHostFree(f.packed);
-            return server_fail_hip(e, "pinned key");
-        }
-        std::memcpy(f.packed, hk.packed.data(), f.packed_bytes);
-        std::memcpy(f.planes, hk.planes.data(), f.planes_bytes);
-        Tenant t;
-        t.pset = pset;
-        t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
-        t.load_bytes = f.packed_bytes + f.planes_bytes;
-        e = hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming);
-        if (e != hipSuccess) {
-            hipHostFree(f.packed);
-            hipHostFree(f.planes);
-            return server_fail_hip(e, "event");
-        }
-        std::lock_guard<std::mutex> g(s->mu);
-        t.form = static_cast<int>(s->forms.size());
-        s->forms.push_back(f);
-        *tenant = static_cast<int32_t>(s->tenants.size());
-        s->tenants.push_back(t);
-        s->st.host_key_bytes += static_cast<int64_t>(f.packed_bytes + f.planes_bytes);
-        return YGPU_OK;
+        return add_tenant_form(s, pset, tenant, [&](ygpu_ctx* ctx, HostKeyForm& hk) { return prepare_key_host(ctx, bsk_coeff, ksk, hk); });
+    });
+}
+
+int ygpu_server_add_tenant_succinct(ygpu_server* s, int32_t pset, const uint64_t* bsk_half, const uint64_t* s2, int32_t s2_level,
+                                    const int32_t* ksk, int32_t* tenant) {
+    return with_server(s, [&] {
+        if (!bsk_half || !s2 || !ksk || !tenant) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        return add_tenant_form(s, pset, tenant, [&](ygpu_ctx* ctx, HostKeyForm& hk) {
+            return prepare_key_host_succinct(ctx, bsk_half, s2, s2_level, ksk, hk);
+        });
     });
 }
 
