@@ -358,29 +358,36 @@ def leg_serving(yg, args, peak):
     lut = np.random.default_rng(6).integers(0, 16, 16)
     v32 = yg.sign_test_polynomial(p32)
     v56 = yg.lut_test_polynomial(p56, lut, 16)
-    keys32 = [yg.keygen(p32, seed=3000 + m) for m in range(mats32)]
-    keys56 = [yg.keygen(p56, seed=4000 + m) for m in range(mats56)]
-    in32 = [sign_inputs(yg, k, 1, 5000 + m)[1][0] for m, k in enumerate(keys32)]
-    in56 = [lut_inputs(yg, k, 1, 6000 + m)[1][0] for m, k in enumerate(keys56)]
-    # expected results from the single-context path
-    exp32, exp56 = [], []
-    with yg.Context(p32, 0) as ctx:
-        for k, c in zip(keys32, in32):
-            kid = ctx.upload_key(k)
-            exp32.append(ctx.pbs(kid, c, v32)[0])
-            ctx.release_key(kid)
-    with yg.Context(p56, 0) as ctx:
-        for k, c in zip(keys56, in56):
-            kid = ctx.upload_key(k)
-            exp56.append(ctx.pbs(kid, c, v56)[0])
-            ctx.release_key(kid)
+    forms = {"full": (yg.keygen, "full")}
+    if args.serving_succinct_budgets_gb:
+        forms["succinct"] = (yg.keygen_succinct, "succinct")
+    material = {}
+    for form, (gen, _) in forms.items():
+        keys32 = [gen(p32, seed=3000 + m) for m in range(mats32)]
+        keys56 = [gen(p56, seed=4000 + m) for m in range(mats56)]
+        in32 = [sign_inputs(yg, k, 1, 5000 + m)[1][0] for m, k in enumerate(keys32)]
+        in56 = [lut_inputs(yg, k, 1, 6000 + m)[1][0] for m, k in enumerate(keys56)]
+        # expected results from the single-context path
+        exp32, exp56 = [], []
+        with yg.Context(p32, 0) as ctx:
+            for k, c in zip(keys32, in32):
+                kid = ctx.upload_key(k)
+                exp32.append(ctx.pbs(kid, c, v32)[0])
+                ctx.release_key(kid)
+        with yg.Context(p56, 0) as ctx:
+            for k, c in zip(keys56, in56):
+                kid = ctx.upload_key(k)
+                exp56.append(ctx.pbs(kid, c, v56)[0])
+                ctx.release_key(kid)
+        material[form] = (keys32, keys56, in32, in56, exp32, exp56)
     m32, m56 = modmul_per_pbs(p32), modmul_per_pbs(p56)
     key_dev = {"32": 82_608_128 + 0, "56": 340_475_904}  # engine-form BSK + KSK planes per key
     pool = sum(key_dev["56" if is56[t] else "32"] for t in range(n_t))
     share56 = sum((1.0 / (t + 1) ** 1.1) for t in range(n_t) if is56[t]) / sum(1.0 / (t + 1) ** 1.1 for t in range(n_t))
     mm_avg = (1 - share56) * m32 + share56 * m56
 
-    def sweep_at(budget_gb):
+    def sweep_at(budget_gb, form="full"):
+        keys32, keys56, in32, in56, exp32, exp56 = material[form]
         budget = int(budget_gb * (1 << 30))
         sweep = []
         with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16,
@@ -424,7 +431,7 @@ def leg_serving(yg, args, peak):
                 r = srv.loadgen(rate=rate, requests=n_req, inputs=inputs, luts=luts, expected=expected,
                                 warm=0, zipf_s=1.1, seed=5, slo_ms=100.0)
                 sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items()})
-                print(f"serving budget {budget_gb} GiB rate {rate}: {r['throughput_rps']:.0f} PBS/s p50 {r['p50_ms']:.2f} "
+                print(f"serving {form} keys, budget {budget_gb} GiB rate {rate}: {r['throughput_rps']:.0f} PBS/s p50 {r['p50_ms']:.2f} "
                       f"ms p99 {r['p99_ms']:.2f} ms (T32 p50 {r['pset_p50_ms'][0]:.1f}, T56 p50 {r['pset_p50_ms'][1]:.1f}) "
                       f"hit {r['hit_rate']:.3f} batch {r['mean_batch']:.1f} shed {r['shed']} mismatches {r['mismatches']}",
                       file=sys.stderr, flush=True)
@@ -433,6 +440,7 @@ def leg_serving(yg, args, peak):
         best = max(ok, key=lambda x: x["throughput_rps"]) if ok else None
         top = max(sweep, key=lambda x: x["throughput_rps"]) if sweep else None
         return {
+            "key_form": form,
             "key_budget_gib": budget_gb,
             "eviction": args.serving_eviction,
             "keys_prefetched": prefetched,
@@ -449,6 +457,7 @@ def leg_serving(yg, args, peak):
         }
 
     runs = [sweep_at(b) for b in args.serving_budgets_gb]
+    runs += [sweep_at(b, "succinct") for b in args.serving_succinct_budgets_gb]
     return {
         "workload": f"Poisson arrivals over 256 tenants, Zipf(1.1) popularity (seed 5), 30 % of tenants TORUS56 "
                     f"n=742 N=2048 with a 16-entry LUT, 70 % TORUS32 n=630 N=1024 sign gate (TORUS56 share of "
@@ -456,7 +465,9 @@ def leg_serving(yg, args, peak):
                     f"tenants, each tenant cached on the device separately; key pool {pool / 1e9:.1f} GB in device "
                     f"form; device key budgets {args.serving_budgets_gb} GiB (the whole pool resident, and a budget "
                     f"below the pool with loads from pinned host memory); {args.serving_seconds} s of arrivals per "
-                    f"rate; full PBS per request; latency from scheduled arrival to result in host memory",
+                    f"rate; full PBS per request; latency from scheduled arrival to result in host memory; runs with "
+                    f"key_form 'succinct' register the same tenants with WWL+24 succinct keys (body-digit rows and "
+                    f"the s^2 TRLEV in pinned host memory, full rows derived on the device at each load)",
         "runs": runs,
     }
 
@@ -472,6 +483,8 @@ def main():
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     ap.add_argument("--serving-budgets-gb", default="48,12",
                     help="device key budgets (GiB) of the serving sweeps; 48 holds the whole key pool")
+    ap.add_argument("--serving-succinct-budgets-gb", default="12",
+                    help="key budgets (GiB) of extra sweeps whose tenants hold WWL+24 succinct keys ('' for none)")
     ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
     ap.add_argument("--serving-eviction", default="freq_bytes", choices=["freq_bytes", "lru"])
     args = ap.parse_args()
@@ -486,6 +499,7 @@ def main():
             sys.exit(f"unknown leg {l}")
     args.serving_rates = [float(r) for r in args.serving_rates.split(",")]
     args.serving_budgets_gb = [float(b) for b in args.serving_budgets_gb.split(",")]
+    args.serving_succinct_budgets_gb = [float(b) for b in args.serving_succinct_budgets_gb.split(",") if b]
 
     import yatfhe_amd as yg
 

@@ -2,8 +2,9 @@
 // reference's own function names, run on the MI355X engine through the drop-in
 // header.  Build with -DTORUS32 or -DTORUS56.  Exit status 0 when every check
 // passes.  Mirrors BOOTSTRAPPING.MP21_FR (bootstrappping_test.cpp:133-180),
-// COMPARE.MP21_NTT's fused-vs-assembled agreement (compare_test.cpp:18-69) and
-// KSK.KSK (keyswitch_test.cpp:12-40).
+// COMPARE.MP21_NTT's fused-vs-assembled agreement (compare_test.cpp:18-69),
+// KSK.KSK (keyswitch_test.cpp:12-40) and BLIND_ROT.BLIND_ROT_WWL24_NTT
+// (blindrot_test.cpp:279-333, TORUS32 defaults).
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -85,13 +86,55 @@ static void mp21_fr(const std::string& which) {
     }
 }
 
+// BLIND_ROT.BLIND_ROT_WWL24_NTT (blindrot_test.cpp:279-333) carried through the
+// bootstrap: blindRotateWWL24Ntt with a succinct key decrypts every slot.
+static void wwl24_fr(const std::string& which) {
+    YatfheParameters param = params(which);
+    param.torusBase = 8;
+    initYatfhe(param);
+    const int p = param.torusBase;
+    TlweKey tlweKey{param};
+    TrgswKey trgswKey{param};
+    TrlweKey& trlweKey = trgswKey.trlweKey;
+    BootstrappingKeyWWL24 bsk{param, param.lApprox};
+    TlweKeySwitchingKey ksKey{param};
+    genTlweKey(tlweKey);
+    genTrlweKey(trlweKey);
+    genBootstrappingKeyWWL24(bsk, trgswKey, tlweKey, param);
+    genTlweKeySwitchingKey(ksKey, trlweKey, tlweKey, param);
+    std::printf("%s WWL+24: n:%d, N:%d, b:%d, lApprox:%d, s2 levels:%d\n", which.c_str(), param.n, param.N, param.radixBits,
+                param.lApprox, bsk.s2Level);
+    TorusPolynomial v{param.N};
+    generateTestPolynomialFR(v, p, 2 * param.N);
+    for (int plain = -p / 2; plain < p / 2; plain++) {
+        const int slot = ((plain % p) + p) % p;
+        Tlwe input{param.n};
+        Tlwe output{param.n};
+        symEncTlwe(input, modSwitchToTorusGeneral(slot, 2 * p, param.qLwe), tlweKey);
+        ScaledTlwe inputModN2{2 * param.N, param.n};
+        Trlwe accum{param};
+        Tlwe tmp{ksKey.nCurrKey};
+        rescaleTlweToNewMod(inputModN2, input);
+        genNoiselessTrlweSample(accum, v, inputModN2);
+        blindRotateWWL24Ntt(accum, bsk, inputModN2, param);
+        extractTlweFromTrlwe(tmp, accum, 0);
+        switchKeyForTlwe(output, ksKey, tmp, param);
+        const int decAft = symDecTlweToInt(output, tlweKey, p);
+        std::printf("m: %d, slot: %d, decAft: %d\n", plain, slot, decAft);
+        EXPECT_EQ(plain, decAft, "WWL24 decrypted slot");
+    }
+}
+
 int main(int argc, char** argv) {
     setYatfheSeed(2024);
     try {
         mp21_fr("small");
 #if defined(TORUS32)
         // the reference's TORUS32 defaults: n=680, N=2048, B=2^4, lApprox=4, KS 2^2 x 16
-        if (argc < 2 || std::string(argv[1]) != "--small-only") mp21_fr("default");
+        if (argc < 2 || std::string(argv[1]) != "--small-only") {
+            mp21_fr("default");
+            wwl24_fr("default");
+        }
 #endif
     } catch (const std::exception& e) {
         std::printf("FAIL exception: %s\n", e.what());

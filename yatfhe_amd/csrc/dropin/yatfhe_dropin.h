@@ -173,6 +173,29 @@ struct BootstrappingKeyMP {
 };
 using BootstrappingKey = BootstrappingKeyMP;  // same arithmetic when l == lApprox (classic row order)
 
+namespace yatfhe_dropin {
+inline int s2_levels(const YatfheParameters& p);
+}
+
+// WWL+24 succinct key (bootstrapping.h:116-136) in coefficient form: the
+// body-digit rows cPrime of every step and the TRLEV of s^2 with param.l levels
+// (the l the caller passed to initYatfhe).
+struct BootstrappingKeyWWL24 {
+    std::vector<uint64_t> halfCoeff;  // [n][level][k+1][N]
+    std::vector<uint64_t> s2Coeff;    // [s2Level][k+1][N]
+    int n{};
+    int level{};
+    int s2Level{};
+    int group{1};
+    uint64_t generation{0};
+    BootstrappingKeyWWL24() = default;
+    BootstrappingKeyWWL24(const YatfheParameters& p, int lv)
+        : halfCoeff(static_cast<size_t>(p.n) * lv * (p.k + 1) * p.N), n(p.n), level(lv), s2Level(yatfhe_dropin::s2_levels(p)),
+          group(p.group) {
+        s2Coeff.assign(static_cast<size_t>(s2Level) * (p.k + 1) * p.N, 0);
+    }
+};
+
 struct TlweKeySwitchingKey {
     std::vector<std::vector<Tlwe>> decomposedKsk;
     int nCurrKey{};
@@ -203,6 +226,7 @@ struct State {
     YatfheParameters param{};
     uint64_t seed = 0x5EED;
     uint64_t draws = 0;
+    int requestedL = 0;  // param.l as passed to initYatfhe (the s^2 levels of WWL+24 keys)
     uint64_t nextGeneration = 1;
     // device key cache: (bsk generation, ksk generation) -> key id
     struct Entry { const void* bsk; uint64_t bg; const void* ksk; uint64_t kg; int32_t id; };
@@ -258,6 +282,26 @@ inline int32_t device_key(const BootstrappingKeyMP& bsk, const TlweKeySwitchingK
     return id;
 }
 
+inline int s2_levels(const YatfheParameters& p) {
+    const State& s = state();
+    return s.ctx && s.requestedL > 0 ? s.requestedL : p.l;
+}
+
+// Device copy of a WWL+24 key (blind rotation only; an all-zero KSK rides along).
+inline int32_t device_key_wwl24(const BootstrappingKeyWWL24& bsk) {
+    State& s = state();
+    if (!s.ctx) throw std::logic_error("initYatfhe must be called first");
+    for (const auto& e : s.keys)
+        if (e.bsk == &bsk && e.bg == bsk.generation && e.ksk == nullptr) return e.id;
+    const YatfheParameters& p = s.param;
+    if (bsk.n != p.n || bsk.level != p.lApprox) throw std::invalid_argument("WWL+24 key dimensions do not match the initialised parameters");
+    std::vector<int32_t> zero(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1), 0);
+    int32_t id = -1;
+    check(ygpu_key_upload_succinct(s.ctx, bsk.halfCoeff.data(), bsk.s2Coeff.data(), bsk.s2Level, zero.data(), &id));
+    s.keys.push_back({&bsk, bsk.generation, nullptr, 0, id});
+    return id;
+}
+
 // Any resident key whose BSK matches (blind rotation needs no KSK).
 inline int32_t device_key_bsk(const BootstrappingKeyMP& bsk) {
     State& s = state();
@@ -279,6 +323,7 @@ inline int32_t device_key_bsk(const BootstrappingKeyMP& bsk) {
 inline void initYatfhe(YatfheParameters& param) {
     auto& s = yatfhe_dropin::state();
     std::lock_guard<std::mutex> lk(s.mu);
+    s.requestedL = param.l;
     if (param.l != param.lApprox)
         param.l = param.lApprox;  // the engine decomposes into exactly the levels it multiplies
     if (s.ctx) {
@@ -439,6 +484,22 @@ inline void genTlweKeySwitchingKey(TlweKeySwitchingKey& ksk, const TrlweKey&, co
     ksk.generation = yatfhe_dropin::state().nextGeneration++;
 }
 
+// genBootstrappingKeyWWL24 (bootstrapping.cpp:106-126), from the same draw as
+// the other keys (same secret keys), with the GLWE noise of param.rlweNoiseB
+// (a TUniform bound 2^B in torus units: log2 sigma = B - 0.79).
+inline void genBootstrappingKeyWWL24(BootstrappingKeyWWL24& bsk, TrgswKey&, const TlweKey&, const YatfheParameters& p) {
+    yatfhe_dropin::ensure_draw(false);
+    auto& s = yatfhe_dropin::state();
+    const auto& d = yatfhe_dropin::draw();
+    if (bsk.level != p.lApprox) throw std::invalid_argument("genBootstrappingKeyWWL24: level must equal param.lApprox");
+    const ygpu_params q = yatfhe_dropin::to_params(p, p.lApprox);
+    std::vector<int32_t> lwe(p.n), glwe(static_cast<size_t>(p.k) * p.N), ksk(d.ksk.size());
+    const double sg = p.rlweNoiseB - 0.79 - p.torusBits;
+    yatfhe_dropin::check(ygpu_keygen_succinct(&q, d.seed, -15.0, sg, bsk.s2Level, lwe.data(), glwe.data(), bsk.halfCoeff.data(),
+                                              bsk.s2Coeff.data(), ksk.data(), 0));
+    bsk.generation = s.nextGeneration++;
+}
+
 // symEncTlwe (tlwe.cpp:20-31), seeded
 inline void symEncTlwe(Tlwe& tlweSample, const Torus message, const TlweKey& key) {
     std::vector<int32_t> s(key.s.begin(), key.s.end());
@@ -480,6 +541,22 @@ inline void blindRotateJP22Ntt(Trlwe& accum, const BootstrappingKeyMP& bsk, cons
 inline void blindRotateNtt(Trlwe& accum, const BootstrappingKey& bsk, const ScaledTlwe& input, const YatfheParameters& p) {
     if (p.group != 1) throw std::invalid_argument("blindRotateNtt: only group == 1 keys are supported");
     blindRotateMP21Ntt(accum, bsk, input, p);
+}
+
+// blindRotateWWL24Ntt (blind_rotate.cpp:1383-1412): the mask-digit rows the
+// reference derives at every step (switchTrlweToSecretEmbeddingNtt) are derived
+// once, on the device, when the key is first used.
+inline void blindRotateWWL24Ntt(Trlwe& accum, const BootstrappingKeyWWL24& bsk, const ScaledTlwe& input, const YatfheParameters&) {
+    auto& s = yatfhe_dropin::state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    const int32_t id = yatfhe_dropin::device_key_wwl24(bsk);
+    const int N = accum.N;
+    std::vector<Torus> acc(static_cast<size_t>(accum.k + 1) * N);
+    for (int c = 0; c < accum.k; c++) std::copy(accum.a[c].coeffs.begin(), accum.a[c].coeffs.end(), acc.begin() + static_cast<size_t>(c) * N);
+    std::copy(accum.b.coeffs.begin(), accum.b.coeffs.end(), acc.begin() + static_cast<size_t>(accum.k) * N);
+    yatfhe_dropin::check(ygpu_blind_rotate_scaled(s.ctx, id, input.a.data(), acc.data()));
+    for (int c = 0; c < accum.k; c++) std::copy(acc.begin() + static_cast<size_t>(c) * N, acc.begin() + static_cast<size_t>(c + 1) * N, accum.a[c].coeffs.begin());
+    std::copy(acc.begin() + static_cast<size_t>(accum.k) * N, acc.end(), accum.b.coeffs.begin());
 }
 
 // extractTlweFromTrlwe (trlwe.cpp:241-250), index 0, on the GPU

@@ -92,12 +92,10 @@ __device__ __forceinline__ int digit_at(int64_t x, int l, int rb, int lvl) {
 }
 
 // Engine form of one polynomial of centred residues mod q: reduce to the
-// centred range mod p, transform forward, scale by N^-1, centre, store.
+// centred range mod p, transform forward, scale by N^-1, centre (x on return).
 template <int LOGN, int TW>
-__device__ __forceinline__ void store_engine_form(const int64_t (&c)[16], double* scr, int tl, const double* tw, const PrimeF64& pr,
-                                                  int64_t p, double ninv, double* out) {
-    constexpr int T = (1 << LOGN) / 16;
-    double x[16];
+__device__ __forceinline__ void engine_form(const int64_t (&c)[16], double (&x)[16], double* scr, int tl, const double* tw,
+                                            const PrimeF64& pr, int64_t p, double ninv) {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         int64_t v = c[r] % p;
@@ -107,101 +105,135 @@ __device__ __forceinline__ void store_engine_form(const int64_t (&c)[16], double
     }
     ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
 #pragma unroll
-    for (int r = 0; r < 16; r++) out[r * T + tl] = centre(mulmod(x[r], ninv, pr), pr);
+    for (int r = 0; r < 16; r++) x[r] = centre(mulmod(x[r], ninv, pr), pr);
 }
 
-// One team of T = N/16 lanes per (step i, level lvl, column col) of a WWL+24
-// succinct key (BootstrappingKeyWWL24, bootstrapping.h:116-136): writes the
-// body-digit row cPrime[lvl] (copied) and the mask-digit row c[lvl][0]
-// derived by switchTrlweToSecretEmbeddingNtt (trgsw.cpp:861-909):
+template <int LOGN>
+__device__ __forceinline__ void store_poly(const double (&x)[16], double* out, int tl) {
+    constexpr int T = (1 << LOGN) / 16;
+#pragma unroll
+    for (int r = 0; r < 16; r++) out[r * T + tl] = x[r];
+}
+
+// One workgroup per (step i, level lvl) of a WWL+24 succinct key
+// (BootstrappingKeyWWL24, bootstrapping.h:116-136), one team of T = N/16 lanes
+// per transform prime.  Writes the body-digit row cPrime[lvl] (copied) and the
+// mask-digit row c[lvl][0] derived by switchTrlweToSecretEmbeddingNtt
+// (trgsw.cpp:861-909):
 //   c.a = sum_l2 D_l2(cPrime.a) * s2[l2].a + cPrime.b,   c.b = sum_l2 D_l2(cPrime.a) * s2[l2].b
 // where D_l2 is digit level l2 of the torus value of cPrime.a (applyIntt's
-// centring and sign extension, then decomposeRow with s2_level digits).  The
-// products are exact negacyclic products mod q: per transform prime p the sum
-// Y = sum D * centred(s2) is formed exactly mod p (|Y| < s2_level N B/2 q/2,
-// below p0 p1 / 2 for two primes; Y mod q needs only one prime when p0 = q),
-// reduced mod q by Garner CRT, and the result is brought to engine form like
-// any uploaded residue.  s2 arrives in engine form ([l2][col][prime][N]).
-template <int LOGN, int TBITS>
-__global__ void __launch_bounds__((1 << LOGN) / 16) k_bsk_derive(BskDeriveArgs a) {
+// centring and sign extension, then decomposeRow with s2_level digits), each
+// digit polynomial transformed once for both columns.  The products are exact
+// negacyclic products mod q.  With one transform prime (p = q) the derived row
+// is formed directly in the transform domain: the s^2 rows and cPrime.b are
+// already scaled by N^-1, so sum NTT(D) s2 + cPrime.b is the engine form.  With
+// two primes each team forms Y = sum D * centred(s2) exactly mod its prime
+// (|Y| < s2_level N B/2 q/2 < p0 p1 / 2), inverts the transform, and the teams
+// swap residues through LDS one column at a time; Garner CRT gives Y mod q, and
+// the result is brought to engine form like an uploaded residue.  s2 arrives
+// in engine form ([l2][col][prime][N]).
+template <int LOGN, int TBITS, int NP>
+__global__ void __launch_bounds__(NP * (1 << LOGN) / 16) k_bsk_derive(BskDeriveArgs a) {
     constexpr int N = 1 << LOGN, T = N / 16, TW = T / 64, K1 = 2;
-    __shared__ double scr[Scratch<LOGN>::DOUBLES];
-    const int col = static_cast<int>(blockIdx.x) % K1;
-    const int il = static_cast<int>(blockIdx.x) / K1;  // i * level + lvl
-    const int tl = static_cast<int>(threadIdx.x);
-    const int np = a.np;
-    const int64_t q = static_cast<int64_t>(a.q);
-    const int64_t qhalf = (q - 1) / 2;
-    // half-key polynomials of this (i, lvl): cPrime.a at 2 il, cPrime.b at 2 il + 1
-    const uint64_t polyA = static_cast<uint64_t>(il) * K1;
-    double* rowBase = a.out + static_cast<size_t>(il) * K1 * K1 * np * N;  // [row][col][prime][N]
+    __shared__ double scratch[NP][Scratch<LOGN>::DOUBLES];
+    __shared__ double ybuf[NP][NP == 2 ? N : 1];
+    const int il = static_cast<int>(blockIdx.x);  // i * level + lvl
+    const int j = static_cast<int>(threadIdx.x) / T;
+    const int tl = static_cast<int>(threadIdx.x) % T;
+    double* scr = scratch[j];
+    const PrimeF64 pr = a.prime[j];
+    const int64_t p = static_cast<int64_t>(a.p[j]);
+    const double ninv = a.ninv[j];
+    const double* tw = a.tw + static_cast<size_t>(j) * 2 * N;
+    const uint64_t polyA = static_cast<uint64_t>(il) * K1;  // cPrime.a; cPrime.b follows
+    double* rowBase = a.out + static_cast<size_t>(il) * K1 * K1 * NP * N;  // [row][col][prime][N]
+    auto slot = [&](int row, int col) { return rowBase + ((static_cast<size_t>(row) * K1 + col) * NP + j) * N; };
 
-    // body-digit row (row 1): cPrime's column col, unchanged
-    int64_t c[16];
+    // body-digit row (row 1): cPrime unchanged; keep the torus values of cPrime.a
+    int64_t c[16], t[16];
+    double x[16], bx[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) c[r] = unpack_centred(a.half, (polyA + col) * N + tl + T * r, a.bits, a.q);
-    for (int j = 0; j < np; j++) {
-        store_engine_form<LOGN, TW>(c, scr, tl, a.tw + static_cast<size_t>(j) * 2 * N, a.prime[j], static_cast<int64_t>(a.p[j]),
-                                    a.ninv[j], rowBase + ((static_cast<size_t>(1) * K1 + col) * np + j) * N);
+    for (int r = 0; r < 16; r++) {
+        c[r] = unpack_centred(a.half, polyA * N + tl + T * r, a.bits, a.q);
+        t[r] = sext_t<TBITS>(c[r]);
     }
+    engine_form<LOGN, TW>(c, x, scr, tl, tw, pr, p, ninv);
+    store_poly<LOGN>(x, slot(1, 0), tl);
+#pragma unroll
+    for (int r = 0; r < 16; r++) c[r] = unpack_centred(a.half, (polyA + 1) * N + tl + T * r, a.bits, a.q);
+    engine_form<LOGN, TW>(c, bx, scr, tl, tw, pr, p, ninv);
+    store_poly<LOGN>(bx, slot(1, 1), tl);
 
-    // mask-digit row (row 0): torus values of cPrime.a, and cPrime.b for column 0
-    int64_t t[16];
+    // mask-digit row (row 0): both columns from one transform per digit level
+    double acc0[16], acc1[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) t[r] = sext_t<TBITS>(unpack_centred(a.half, polyA * N + tl + T * r, a.bits, a.q));
-    double y0[16];
-    for (int j = 0; j < np; j++) {
-        const PrimeF64 pr = a.prime[j];
-        const double* tw = a.tw + static_cast<size_t>(j) * 2 * N;
-        double acc[16];
+    for (int r = 0; r < 16; r++) { acc0[r] = 0.0; acc1[r] = 0.0; }
+    for (int l2 = 0; l2 < a.s2_level; l2++) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = 0.0;
-        for (int l2 = 0; l2 < a.s2_level; l2++) {
-            double x[16];
-#pragma unroll
-            for (int r = 0; r < 16; r++) x[r] = static_cast<double>(digit_at<TBITS>(t[r], a.s2_level, a.radix_bits, l2));
-            ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
-            const double* s2 = a.s2 + ((static_cast<size_t>(l2) * K1 + col) * np + j) * N + tl;
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[r] = centre(__dadd_rn(acc[r], mulmod(x[r], s2[r * T], pr)), pr);
-        }
-        ntt_inverse<LOGN, TW>(acc, scr, tl, tw + N, pr);
-#pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = centre(acc[r], pr);
-        if (j == 0) {
-#pragma unroll
-            for (int r = 0; r < 16; r++) y0[r] = acc[r];
-        } else {
-#pragma unroll
-            for (int r = 0; r < 16; r++) c[r] = crt_mod_q(y0[r], acc[r], a.prime[1], a.crt);
-        }
-    }
-    if (np == 1) {
-#pragma unroll
-        for (int r = 0; r < 16; r++) c[r] = f64_to_i64(y0[r]);  // p0 = q: already centred mod q
-    }
-    if (col == 0) {
+        for (int r = 0; r < 16; r++) x[r] = static_cast<double>(digit_at<TBITS>(t[r], a.s2_level, a.radix_bits, l2));
+        ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+        const double* s2a = a.s2 + ((static_cast<size_t>(l2) * K1 + 0) * NP + j) * N + tl;
+        const double* s2b = a.s2 + ((static_cast<size_t>(l2) * K1 + 1) * NP + j) * N + tl;
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            int64_t v = c[r] + unpack_centred(a.half, (polyA + 1) * N + tl + T * r, a.bits, a.q);
-            if (v > qhalf) v -= q;
-            if (v < -qhalf) v += q;
-            c[r] = v;
+            acc0[r] = centre(__dadd_rn(acc0[r], mulmod(x[r], s2a[r * T], pr)), pr);
+            acc1[r] = centre(__dadd_rn(acc1[r], mulmod(x[r], s2b[r * T], pr)), pr);
         }
     }
-    for (int j = 0; j < np; j++) {
-        store_engine_form<LOGN, TW>(c, scr, tl, a.tw + static_cast<size_t>(j) * 2 * N, a.prime[j], static_cast<int64_t>(a.p[j]),
-                                    a.ninv[j], rowBase + (static_cast<size_t>(col) * np + j) * N);
+    if constexpr (NP == 1) {
+        // p = q: the engine form of the derived residues, directly
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            acc0[r] = centre(__dadd_rn(acc0[r], bx[r]), pr);
+            acc1[r] = centre(acc1[r], pr);
+        }
+        store_poly<LOGN>(acc0, slot(0, 0), tl);
+        store_poly<LOGN>(acc1, slot(0, 1), tl);
+    } else {
+        const int64_t q = static_cast<int64_t>(a.q);
+        const int64_t qhalf = (q - 1) / 2;
+        ntt_inverse<LOGN, TW>(acc0, scr, tl, tw + N, pr);
+        ntt_inverse<LOGN, TW>(acc1, scr, tl, tw + N, pr);
+        for (int col = 0; col < K1; col++) {
+            // this team's residues of column col -> LDS; Y mod q from both primes
+#pragma unroll
+            for (int r = 0; r < 16; r++) ybuf[j][tl + T * r] = centre(col == 0 ? acc0[r] : acc1[r], pr);
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                int64_t v = crt_mod_q(ybuf[0][tl + T * r], ybuf[1][tl + T * r], a.prime[1], a.crt);
+                if (col == 0) {
+                    v += unpack_centred(a.half, (polyA + 1) * N + tl + T * r, a.bits, a.q);
+                    if (v > qhalf) v -= q;
+                    if (v < -qhalf) v += q;
+                }
+                c[r] = v;
+            }
+            __syncthreads();
+            engine_form<LOGN, TW>(c, x, scr, tl, tw, pr, p, ninv);
+            store_poly<LOGN>(x, slot(0, col), tl);
+        }
     }
 }
 
 hipError_t launch_bsk_derive(int logn, int tbits, const BskDeriveArgs& a, hipStream_t s) {
-    const unsigned blocks = static_cast<unsigned>(a.n) * static_cast<unsigned>(a.level) * 2u;
-    if (logn == 10 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<10, 32>), dim3(blocks), dim3(64), 0, s, a);
-    else if (logn == 11 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<11, 32>), dim3(blocks), dim3(128), 0, s, a);
-    else if (logn == 10 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<10, 56>), dim3(blocks), dim3(64), 0, s, a);
-    else if (logn == 11 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<11, 56>), dim3(blocks), dim3(128), 0, s, a);
-    else return hipErrorInvalidValue;
+    const dim3 grid(static_cast<unsigned>(a.n) * static_cast<unsigned>(a.level));
+    const unsigned t = (1u << logn) / 16 * static_cast<unsigned>(a.np);
+    if (a.np == 1) {
+        if (logn == 10 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<10, 32, 1>), grid, dim3(t), 0, s, a);
+        else if (logn == 11 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<11, 32, 1>), grid, dim3(t), 0, s, a);
+        else if (logn == 10 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<10, 56, 1>), grid, dim3(t), 0, s, a);
+        else if (logn == 11 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<11, 56, 1>), grid, dim3(t), 0, s, a);
+        else return hipErrorInvalidValue;
+    } else if (a.np == 2) {
+        if (logn == 10 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<10, 32, 2>), grid, dim3(t), 0, s, a);
+        else if (logn == 11 && tbits == 32) hipLaunchKernelGGL((k_bsk_derive<11, 32, 2>), grid, dim3(t), 0, s, a);
+        else if (logn == 10 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<10, 56, 2>), grid, dim3(t), 0, s, a);
+        else if (logn == 11 && tbits == 56) hipLaunchKernelGGL((k_bsk_derive<11, 56, 2>), grid, dim3(t), 0, s, a);
+        else return hipErrorInvalidValue;
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
