@@ -46,7 +46,7 @@ BATCH_T56 = 8192
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s (public spec)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_blind_rotate_t32_n1024_b4096_traffic.txt")
 SERVING_RATES = [1000, 2000, 5000, 10000, 20000, 30000, 40000, 50000]
-LEGS = ("config1", "config2", "config3", "serving")
+LEGS = ("config1", "config2", "config3", "serving", "keyload")
 
 
 def modmul_per_pbs(p) -> int:
@@ -472,6 +472,47 @@ def leg_serving(yg, args, peak):
     }
 
 
+def transfer_bytes(p, s2_level=0):
+    """Host-to-device bytes of one key: packed residues (+1 word) and KSK byte planes."""
+    bits = (p.q_ntt - 1).bit_length()
+    poly_words = p.N * bits // 64
+    polys = p.n * p.level * (p.k + 1) * (p.k + 1) if s2_level == 0 else p.n * p.level * (p.k + 1) + s2_level * (p.k + 1)
+    rows = p.k * p.N * p.ks_level
+    return (polys * poly_words + 1) * 8 + (rows + 3) // 4 * 4 * (p.n + 1) * 4
+
+
+def leg_keyload(yg, args):
+    """Key loads of the full and the WWL+24 succinct form: bytes moved, device
+    expansion time (HIP events around the expansion kernels), and the whole
+    upload call (host packing, copy, expansion)."""
+    out = {}
+    for name, p in (("t32_n1024", yg.PARAMS_T32_N1024), ("t56_n2048", yg.PARAMS_T56_N2048)):
+        fk = yg.keygen(p, seed=1)
+        sk = yg.keygen_succinct(p, seed=1)
+        row = {}
+        with yg.Context(p, 0) as ctx:
+            ctx.timing(True)
+            for form, keys, kernel in (("full", fk, "k_bsk_expand"), ("succinct", sk, "succinct_expand")):
+                kid = ctx.upload_key(keys)  # warm-up
+                ctx.release_key(kid)
+                ctx.timing_reset()
+                calls = []
+                for _ in range(args.keyload_reps):
+                    t0 = time.perf_counter()
+                    kid = ctx.upload_key(keys)
+                    calls.append((time.perf_counter() - t0) * 1e3)
+                    ctx.release_key(kid)
+                n, ms = ctx.timing_get(kernel)
+                row[form] = {"transfer_bytes": transfer_bytes(p, sk.s2_level if form == "succinct" else 0),
+                             "device_expand_ms": round(ms / max(n, 1), 4), "upload_call_ms_p50": round(float(np.median(calls)), 2),
+                             "s2_level": sk.s2_level if form == "succinct" else None}
+            ctx.timing(False)
+        row["transfer_ratio"] = round(row["succinct"]["transfer_bytes"] / row["full"]["transfer_bytes"], 4)
+        out[name] = row
+        print(f"keyload {name}: full {row['full']}, succinct {row['succinct']}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -487,6 +528,7 @@ def main():
                     help="key budgets (GiB) of extra sweeps whose tenants hold WWL+24 succinct keys ('' for none)")
     ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
     ap.add_argument("--serving-eviction", default="freq_bytes", choices=["freq_bytes", "lru"])
+    ap.add_argument("--keyload-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "
@@ -567,6 +609,9 @@ def main():
     if "config3" in legs:
         log("config3: 64 tenants, hipGraph")
         configs["config3_64_tenants_graph"] = leg_config3(yg, args, peak)
+    if "keyload" in legs:
+        log("key loads: full vs WWL+24 succinct form")
+        configs["key_load_forms"] = leg_keyload(yg, args)
     if "serving" in legs:
         log("serving sweep")
         configs["config4_serving"] = leg_serving(yg, args, peak)

@@ -248,7 +248,17 @@ int ygpu_key_file_info(const char* path, ygpu_params* params);
  * (optional) must equal the file's parameters.  YGPU_ERR_CORRUPT on a bad
  * magic, version, size, digest or out-of-range residue. */
 int ygpu_key_file_read(const char* path, const ygpu_params* expect, uint64_t* bsk_coeff, int32_t* ksk);
-/* ygpu_key_file_read + ygpu_key_upload. */
+/* The WWL+24 succinct form (inputs as ygpu_key_upload_succinct) as a version-2
+ * key file: the BSK section holds the body-digit rows, then the s^2 TRLEV; the
+ * header records s2_level.  Replaces the reference's serializeBskWWL24 /
+ * deserializeBskWWL24 streams (ya_serializer.cpp), which hold HEXL-order words. */
+int ygpu_key_file_write_succinct(const char* path, const ygpu_params* params, const uint64_t* bsk_half, const uint64_t* s2,
+                                 int32_t s2_level, const int32_t* ksk);
+/* The key form of a file: *s2_level = 0 for a full key, > 0 for a succinct key. */
+int ygpu_key_file_form(const char* path, int32_t* s2_level);
+int ygpu_key_file_read_succinct(const char* path, const ygpu_params* expect, uint64_t* bsk_half, uint64_t* s2, int32_t* ksk);
+/* Reads a key file of either form and uploads it (ygpu_key_upload or
+ * ygpu_key_upload_succinct). */
 int ygpu_key_upload_file(ygpu_ctx* ctx, const char* path, int32_t* key_id);
 
 /* ---------------------------------------------------------------- serving

@@ -66,3 +66,18 @@ def test_succinct_reference_defaults_decrypt():
     out = check_against_full(p, sk, cts, yg.sign_test_polynomial(p), 2)
     ph = yg.lwe_decrypt_phase(out, sk.lwe_key).astype(np.int64)
     assert np.array_equal((ph > 0).astype(np.int64), bits), "succinct-key bootstrap does not decrypt to the sign"
+
+
+def test_succinct_key_file_upload(tmp_path, p32):
+    sk = yg.keygen_succinct(p32, seed=24, s2_level=4)
+    path = str(tmp_path / "succinct.ygkey")
+    yg.save_keys(path, sk)
+    _, cts = sign_inputs(sk, 16, 9)
+    v = yg.sign_test_polynomial(p32)
+    print("succinct: key file upload", flush=True)
+    with yg.Context(p32, 0) as ctx:
+        kf = ctx.upload_key_file(path)
+        ks = ctx.upload_key_succinct(sk)
+        a, acc_a, _ = ctx.pbs_batch(kf, cts, v, want_acc=True)
+        b, acc_b, _ = ctx.pbs_batch(ks, cts, v, want_acc=True)
+    assert np.array_equal(a, b) and np.array_equal(acc_a, acc_b)

@@ -118,3 +118,50 @@ def test_digest_is_order_sensitive(tmp_path, p32, keys32):
     with pytest.raises(yg.YgpuError) as e:
         yg.load_key_arrays(path)
     assert e.value.code == -9
+
+
+@pytest.fixture(scope="module")
+def sk_small(p32):
+    return yg.keygen_succinct(yg.replace(p32, n=24), seed=31, s2_level=4)
+
+
+def test_succinct_round_trip_and_layout(tmp_path, sk_small):
+    p = sk_small.params
+    path = write(tmp_path, sk_small, "s.ygkey")
+    assert yg.key_file_params(path) == p
+    assert yg.key_file_s2_level(path) == 4
+    q, half, s2, ksk = yg.load_succinct_arrays(path, expect=p)
+    assert q == p
+    assert np.array_equal(half, sk_small.bsk_half) and np.array_equal(s2, sk_small.s2) and np.array_equal(ksk, sk_small.ksk)
+    bsk_words = sk_small.bsk_half.size + sk_small.s2.size
+    assert os.path.getsize(path) == HEADER + bsk_words * 8 + sk_small.ksk.size * 4
+    with open(path, "rb") as f:
+        h = f.read(HEADER)
+    assert int.from_bytes(h[8:12], "little") == 2
+    assert int.from_bytes(h[64:72], "little") == bsk_words
+    assert int.from_bytes(h[104:108], "little") == 4 and h[108:112] == bytes(4)
+    assert int.from_bytes(h[112:120], "little") != 0 and h[120:128] == bytes(8)
+    # s2 starts right after the body-digit rows
+    with open(path, "rb") as f:
+        f.seek(HEADER + sk_small.bsk_half.size * 8)
+        assert np.array_equal(np.frombuffer(f.read(8 * p.N), "<u8"), sk_small.s2[0, 0])
+
+
+def test_succinct_form_checks(tmp_path, p32, keys32, sk_small):
+    spath = write(tmp_path, sk_small, "s.ygkey")
+    fpath = write(tmp_path, keys32, "f.ygkey")
+    assert yg.key_file_s2_level(fpath) == 0
+    with pytest.raises(yg.YgpuError):
+        yg.load_key_arrays(spath)          # a succinct file is not a full key
+    with pytest.raises(yg.YgpuError):
+        yg.load_succinct_arrays(fpath)     # nor the other way round
+    for offset in (104, 108, 112, HEADER + 5 * 8 + 1, HEADER + sk_small.bsk_half.size * 8 + 2):
+        bad = str(tmp_path / f"bad{offset}.ygkey")
+        with open(spath, "rb") as f, open(bad, "wb") as g:
+            g.write(f.read())
+        corrupt(bad, offset)
+        with pytest.raises(yg.YgpuError) as e:
+            yg.load_succinct_arrays(bad)
+        assert e.value.code == -9, offset  # YGPU_ERR_CORRUPT
+    with pytest.raises(yg.YgpuError):
+        yg.save_keys(str(tmp_path / "x.ygkey"), yg.replace(sk_small, s2=sk_small.s2[:0]))  # s2_level 0

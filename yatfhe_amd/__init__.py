@@ -203,6 +203,9 @@ def load_library(path: str | None = None):
         "ygpu_key_file_info": (c_int, [ctypes.c_char_p, P(_Params)]),
         "ygpu_key_file_read": (c_int, [ctypes.c_char_p, P(_Params), c_vp, c_vp]),
         "ygpu_key_upload_file": (c_int, [c_vp, ctypes.c_char_p, P(c_i32)]),
+        "ygpu_key_file_write_succinct": (c_int, [ctypes.c_char_p, P(_Params), c_vp, c_vp, c_i32, c_vp]),
+        "ygpu_key_file_form": (c_int, [ctypes.c_char_p, P(c_i32)]),
+        "ygpu_key_file_read_succinct": (c_int, [ctypes.c_char_p, P(_Params), c_vp, c_vp, c_vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -228,6 +231,7 @@ def exported_symbols():
         "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_wait_for", "ygpu_server_stats_get", "ygpu_server_loadgen",
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
+        "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct",
     ]
 
 
@@ -320,9 +324,17 @@ def keygen_succinct(params: Params, seed: int, s2_level: int | None = None, sigm
     return SuccinctKeySet(params, lk, gk, half, s2, ksk)
 
 
-def save_keys(path: str, keys: KeySet):
-    """Write the keys' coefficient form to a key interchange file (ygpu_key_file_write)."""
+def save_keys(path: str, keys):
+    """Write the keys' coefficient form to a key interchange file (ygpu_key_file_write,
+    or ygpu_key_file_write_succinct for a SuccinctKeySet)."""
     cp = keys.params.c()
+    if isinstance(keys, SuccinctKeySet):
+        half = np.ascontiguousarray(keys.bsk_half, dtype=np.uint64)
+        s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+        ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        _check(load_library().ygpu_key_file_write_succinct(os.fsencode(path), ctypes.byref(cp), _ptr(half), _ptr(s2),
+                                                           keys.s2_level, _ptr(ksk)))
+        return
     bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
     ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
     _check(load_library().ygpu_key_file_write(os.fsencode(path), ctypes.byref(cp), _ptr(bsk), _ptr(ksk)))
@@ -344,6 +356,27 @@ def load_key_arrays(path: str, expect: Params | None = None):
     _check(load_library().ygpu_key_file_read(os.fsencode(path), ctypes.byref(ce) if ce is not None else None,
                                              _ptr(bsk), _ptr(ksk)))
     return p, bsk, ksk
+
+
+def key_file_s2_level(path: str) -> int:
+    """0 for a full-key file, the s^2 TRLEV levels for a succinct-key file."""
+    lv = ctypes.c_int32()
+    _check(load_library().ygpu_key_file_form(os.fsencode(path), ctypes.byref(lv)))
+    return lv.value
+
+
+def load_succinct_arrays(path: str, expect: Params | None = None):
+    """(params, bsk_half, s2, ksk) from a succinct-key file (ygpu_key_file_read_succinct)."""
+    p = key_file_params(path)
+    lv = key_file_s2_level(path)
+    k1 = p.k + 1
+    half = np.zeros((p.n, p.level, k1, p.N), np.uint64)
+    s2 = np.zeros((max(lv, 1), k1, p.N), np.uint64)
+    ksk = np.zeros(p.ksk_shape, np.int32)
+    ce = expect.c() if expect is not None else None
+    _check(load_library().ygpu_key_file_read_succinct(os.fsencode(path), ctypes.byref(ce) if ce is not None else None,
+                                                      _ptr(half), _ptr(s2), _ptr(ksk)))
+    return p, half, s2, ksk
 
 
 def lwe_encrypt(lwe_key: np.ndarray, mu: np.ndarray, seed: int, sigma_log2: float = SIGMA_LOG2_LWE) -> np.ndarray:
@@ -775,4 +808,5 @@ __all__ = [
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
     "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
     "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
+    "key_file_s2_level", "load_succinct_arrays",
 ]

@@ -127,7 +127,10 @@ int upload_host_form(ygpu_ctx* ctx, const HostKeyForm& hk, int32_t* key_id) {
     int rc = ensure(&ctx->d_stage, &ctx->stage_cap, stage_bytes(ctx->p, ctx->np, hk.s2_level));
     if (rc) { hipFree(rec.d_bsk); hipFree(rec.d_ksk); return rc; }
     YG_HIP(hipMemcpyAsync(ctx->d_stage, hk.packed.data(), hk.packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
-    YG_HIP(expand_key(ctx, hk.s2_level, ctx->d_stage, rec.d_bsk, ctx->stream));
+    // timed as "k_bsk_expand" (full keys) or "succinct_expand" (the s^2 TRLEV's
+    // k_bsk_expand + k_bsk_derive)
+    YG_HIP(timed(ctx, hk.s2_level ? "succinct_expand" : "k_bsk_expand", ctx->stream,
+                 [&] { return expand_key(ctx, hk.s2_level, ctx->d_stage, rec.d_bsk, ctx->stream); }));
     YG_HIP(hipMemcpyAsync(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
     YG_HIP(hipStreamSynchronize(ctx->stream));
     int id = -1;
