@@ -103,7 +103,11 @@ def ccd_cpus():
         return sorted(os.sched_getaffinity(0))
 
 
-def cpu_baseline(params, keys, cts, v):
+def cpu_baseline(params, keys, cts, v, per_thread=16, lat_runs=100):
+    """The oracle on one L3 domain of the host: throughput with one ciphertext
+    per thread (per_thread ciphertexts each), and single-ciphertext latency with
+    the external product's gadget levels on `level` threads (the reference's
+    threading, trgsw.cpp:725-760), p50/p99 over lat_runs runs."""
     from tests.oracle_py import Oracle
 
     cpus = ccd_cpus()[:16]
@@ -111,17 +115,18 @@ def cpu_baseline(params, keys, cts, v):
     os.sched_setaffinity(0, set(cpus))
     try:
         orc = Oracle(params, keys)
-        sample = 16 * len(cpus)
+        sample = per_thread * len(cpus)
         orc.pbs_batch(cts[: len(cpus)], v, threads=len(cpus))  # warm
         t0 = time.perf_counter()
         orc.pbs_batch(cts[:sample], v, threads=len(cpus))
         dt = time.perf_counter() - t0
-        # single-ciphertext latency, one thread
-        lat = []
+        # single-ciphertext latency: one thread, and levels on a pool
+        one = []
         for c in range(3):
             t = time.perf_counter()
             orc.pbs(cts[c], v)
-            lat.append(time.perf_counter() - t)
+            one.append(time.perf_counter() - t)
+        _, secs = orc.pbs_level_parallel(cts[0], v, reps=lat_runs)
         orc.close()
     finally:
         os.sched_setaffinity(0, saved)
@@ -133,6 +138,7 @@ def cpu_baseline(params, keys, cts, v):
                 break
     except OSError:
         pass
+    secs = list(secs)
     return {
         "value": round(sample / dt, 2),
         "unit": "PBS/s",
@@ -141,7 +147,9 @@ def cpu_baseline(params, keys, cts, v):
         "sample": f"{sample} ciphertexts of the same workload, one per thread at a time, threads pinned to CPUs {cpus[0]}-{cpus[-1]} "
                   f"(one L3 domain) of {model or 'the host'}; the oracle's own scalar NTT, not Intel HEXL",
         "seconds": round(dt, 3),
-        "latency_ms_one_thread": round(min(lat) * 1e3, 1),
+        "latency_ms_one_thread": round(min(one) * 1e3, 1),
+        "latency_level_parallel": {"threads": params.level, "runs": lat_runs, "p50_ms": round(pct(secs, 50) * 1e3, 1),
+                                   "p99_ms": round(pct(secs, 99) * 1e3, 1)},
     }
 
 
@@ -276,6 +284,7 @@ def leg_config2(yg, args, peak):
         "lut_decrypt_check": ok,
         "plain": {"value": round(len(lat) / sum(lat), 2), "unit": "PBS/s", "calls": len(lat),
                   "p50_ms": round(pct(lat, 50) * 1e3, 2)},
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(p, keys, cts, v, per_thread=2, lat_runs=20),
     }
 
 

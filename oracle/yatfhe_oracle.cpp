@@ -35,6 +35,8 @@
 #include <atomic>
 #include <algorithm>
 #include <stdexcept>
+#include <functional>
+#include <chrono>
 
 typedef unsigned __int128 u128;
 typedef __int128 i128;
@@ -510,6 +512,106 @@ void keyswitch(const or_params& p, const int32_t* ksk, const int32_t* in, int32_
     out[nOut] = static_cast<int32_t>(long_mod_p(static_cast<int64_t>(in[nIn]) - delta_b, lq));
 }
 
+// blindRotateMP21Ntt with the reference's threading of the external product
+// (externalProductTrgswMPNttImpl, trgsw.cpp:725-760): the calling thread rotates
+// and decomposes, then each gadget level's forward transforms and MAC into its
+// own partial sum run on a worker of a fixed pool (ThreadPool::run + join per
+// step), and the calling thread sums the partials and inverse-transforms.
+// Bit-identical to blind_rotate (the MAC order is irrelevant mod q); used to
+// time the reference's single-ciphertext latency form.
+class LevelPool {
+  public:
+    explicit LevelPool(int workers) : n_(workers) {
+        for (int w = 0; w < n_; w++) ts_.emplace_back([this, w] { loop(w); });
+    }
+    ~LevelPool() {
+        stop_.store(true);
+        gen_.fetch_add(1);
+        for (auto& t : ts_) t.join();
+    }
+    // Runs fn(w) for w < n on the workers and waits for all of them.
+    void run(const std::function<void(int)>& fn) {
+        fn_ = &fn;
+        done_.store(0);
+        gen_.fetch_add(1, std::memory_order_acq_rel);
+        while (done_.load(std::memory_order_acquire) < n_) __builtin_ia32_pause();
+    }
+
+  private:
+    void loop(int w) {
+        uint64_t seen = 0;
+        for (;;) {
+            uint64_t g;
+            while ((g = gen_.load(std::memory_order_acquire)) == seen) __builtin_ia32_pause();
+            seen = g;
+            if (stop_.load()) return;
+            (*fn_)(w);
+            done_.fetch_add(1, std::memory_order_acq_rel);
+        }
+    }
+    int n_;
+    std::vector<std::thread> ts_;
+    std::atomic<uint64_t> gen_{0};
+    std::atomic<int> done_{0};
+    std::atomic<bool> stop_{false};
+    const std::function<void(int)>* fn_ = nullptr;
+};
+
+void blind_rotate_level_parallel(const Key& key, const int32_t* abar, int64_t* acc, LevelPool& pool) {
+    const or_params& p = key.p;
+    const int N = p.N, K1 = p.k + 1, L = p.level, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    std::vector<int64_t> tmp(static_cast<size_t>(K1) * N), dig(static_cast<size_t>(K1) * L * N);
+    std::vector<std::vector<uint64_t>> partial(L, std::vector<uint64_t>(static_cast<size_t>(K1) * N));
+    size_t step = 0;
+    const std::function<void(int)> level = [&](int lvl) {
+        std::vector<uint64_t>& part = partial[lvl];
+        std::fill(part.begin(), part.end(), 0);
+        std::vector<uint64_t> t(N);
+        const size_t rowBase = step * static_cast<size_t>(L) * K1 * K1 * N;
+        for (int r = 0; r < K1; r++) {
+            const int64_t* d = &dig[(static_cast<size_t>(r) * L + lvl) * N];
+            for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
+            key.ntt.forward(t.data());
+            const size_t keyOff = rowBase + (static_cast<size_t>(lvl) * K1 + r) * K1 * N;
+            for (int c = 0; c < K1; c++) {
+                const uint64_t* kb = &key.bsk[keyOff + static_cast<size_t>(c) * N];
+                const uint64_t* kp = &key.bskp[keyOff + static_cast<size_t>(c) * N];
+                uint64_t* a = &part[static_cast<size_t>(c) * N];
+                for (int j = 0; j < N; j++) {
+                    uint64_t s = a[j] + shoup(t[j], kb[j], kp[j], q);
+                    a[j] = s >= q ? s - q : s;
+                }
+            }
+        }
+    };
+    std::vector<uint64_t> sum(N);
+    for (int i = 0; i < p.n; i++) {
+        if (abar[i] == 0) continue;
+        for (int r = 0; r < K1; r++) {
+            rotate_minus_one(w, N, abar[i], acc + static_cast<size_t>(r) * N, &tmp[static_cast<size_t>(r) * N]);
+            digits_row(&tmp[static_cast<size_t>(r) * N], N, L, p.radix_bits, w, &dig[static_cast<size_t>(r) * L * N]);
+        }
+        step = static_cast<size_t>(i);
+        pool.run(level);
+        for (int c = 0; c < K1; c++) {
+            for (int j = 0; j < N; j++) {
+                uint64_t s = 0;
+                for (int lvl = 0; lvl < L; lvl++) {
+                    s += partial[lvl][static_cast<size_t>(c) * N + j];
+                    if (s >= q) s -= q;
+                }
+                sum[j] = s;
+            }
+            key.ntt.inverse(sum.data());
+            for (int j = 0; j < N; j++) {
+                int64_t& a = acc[static_cast<size_t>(c) * N + j];
+                a = sext(a + intt_exit(sum[j], q, w), w);
+            }
+        }
+    }
+}
+
 void pbs_one(const Key& key, const int32_t* ksk, const int32_t* lwe, const int64_t* v,
              int64_t* acc, int32_t* ext, int32_t* out, bool schoolbook) {
     const or_params& p = key.p;
@@ -715,6 +817,26 @@ void or_pbs_batch_all(void* key, const int32_t* ksk, int32_t count, const int32_
     std::vector<std::thread> ts;
     for (int t = 0; t < std::max(1, threads); t++) ts.emplace_back(work);
     for (auto& t : ts) t.join();
+}
+
+// functionalBootstrappingMPNtt for one ciphertext with the external product's
+// gadget levels on a pool of param.level workers (the reference's latency form).
+void or_pbs_level_parallel(void* key, const int32_t* ksk, const int32_t* lwe, const int64_t* v, int32_t* out, int32_t reps,
+                           double* seconds) {
+    const Key& k = *static_cast<Key*>(key);
+    const or_params& p = k.p;
+    LevelPool pool(p.level);
+    std::vector<int64_t> acc(static_cast<size_t>(p.k + 1) * p.N);
+    std::vector<int32_t> scaled(p.n + 1), ext(static_cast<size_t>(p.k) * p.N + 1);
+    for (int r = 0; r < reps; r++) {
+        const auto t0 = std::chrono::steady_clock::now();
+        rescale(p, lwe, scaled.data());
+        init_acc(p, v, scaled[p.n], acc.data());
+        blind_rotate_level_parallel(k, scaled.data(), acc.data(), pool);
+        extract(p, acc.data(), ext.data());
+        keyswitch(p, ksk, ext.data(), out);
+        if (seconds) seconds[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
 }
 
 void or_pbs_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,

@@ -67,6 +67,7 @@ def lib():
             "or_glwe_phase": (None, [i32, i32, i32, vp, vp, vp]),
             "or_glwe_phase_modq": (None, [u64, i32, i32, vp, vp, vp]),
             "or_derive_succinct": (None, [P(_OrParams), i32, vp, vp, vp, i32]),
+            "or_pbs_level_parallel": (None, [vp, vp, vp, vp, vp, i32, vp]),
             "or_derive_succinct_row": (None, [P(_OrParams), i32, vp, vp, vp, i32]),
         }
         for name, (res, args) in sigs.items():
@@ -146,6 +147,15 @@ class Oracle:
         t = threads or min(16, os.cpu_count() or 1)
         self.L.or_pbs_batch(self.key, P(self.ksk), lwes.shape[0], P(lwes), P(self._v64(v)), P(out), t)
         return out
+
+    def pbs_level_parallel(self, lwe, v, reps=1):
+        """(keyswitched LWE, seconds per run) with the external product's levels
+        on a pool of `level` workers, as the reference threads it."""
+        out = np.zeros(self.p.n + 1, np.int32)
+        secs = np.zeros(reps, np.float64)
+        self.L.or_pbs_level_parallel(self.key, P(self.ksk), P(np.ascontiguousarray(lwe, dtype=np.int32)), P(self._v64(v)),
+                                     P(out), reps, P(secs))
+        return out, secs
 
     def pbs_batch_all(self, lwes, v, threads=None):
         """(accumulators, extracted LWEs, keyswitched LWEs) of many ciphertexts."""

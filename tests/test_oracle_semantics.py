@@ -123,3 +123,16 @@ def test_pbs_lut_t56(p56, keys56):
     dec = np.round(ph / (1 << 32) * p).astype(np.int64) % p
     assert np.array_equal(dec, lut[msgs] % p)
     orc.close()
+
+
+def test_level_parallel_blind_rotation_is_identical(p32, keys32):
+    """The reference's threading of the external product (levels on a pool,
+    trgsw.cpp:725-760) changes nothing in the result."""
+    orc = Oracle(p32, keys32)
+    v = yg.sign_test_polynomial(p32)
+    cts = yg.lwe_encrypt(keys32.lwe_key, np.array([1 << 29, -(1 << 29), 7], np.int32), seed=12)
+    for ct in cts:
+        _, _, want = orc.pbs_batch_all(ct[None, :], v)
+        got, secs = orc.pbs_level_parallel(ct, v, reps=2)
+        assert np.array_equal(got, want[0]) and (secs > 0).all()
+    orc.close()

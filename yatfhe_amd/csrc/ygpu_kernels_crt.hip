@@ -161,7 +161,10 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
 #pragma unroll
             for (int r = 0; r < 16; r++) x[r] = __dadd_rn(own[r], other[r * T + tl]);
         }
-        __syncthreads();
+        // the partner's reads must finish before this team's scratch is
+        // rewritten; with two-wave teams the inverse transform's first exchange
+        // opens with a workgroup barrier, which already orders them
+        if constexpr (TW == 1) __syncthreads();
 
         ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
 

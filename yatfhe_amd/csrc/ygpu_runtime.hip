@@ -126,13 +126,21 @@ int upload_host_form(ygpu_ctx* ctx, const HostKeyForm& hk, int32_t* key_id) {
     // packed residues -> staging -> engine form, on the context stream
     int rc = ensure(&ctx->d_stage, &ctx->stage_cap, stage_bytes(ctx->p, ctx->np, hk.s2_level));
     if (rc) { hipFree(rec.d_bsk); hipFree(rec.d_ksk); return rc; }
-    YG_HIP(hipMemcpyAsync(ctx->d_stage, hk.packed.data(), hk.packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+    e = hipMemcpyAsync(ctx->d_stage, hk.packed.data(), hk.packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream);
     // timed as "k_bsk_expand" (full keys) or "succinct_expand" (the s^2 TRLEV's
     // k_bsk_expand + k_bsk_derive)
-    YG_HIP(timed(ctx, hk.s2_level ? "succinct_expand" : "k_bsk_expand", ctx->stream,
-                 [&] { return expand_key(ctx, hk.s2_level, ctx->d_stage, rec.d_bsk, ctx->stream); }));
-    YG_HIP(hipMemcpyAsync(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-    YG_HIP(hipStreamSynchronize(ctx->stream));
+    if (e == hipSuccess)
+        e = timed(ctx, hk.s2_level ? "succinct_expand" : "k_bsk_expand", ctx->stream,
+                  [&] { return expand_key(ctx, hk.s2_level, ctx->d_stage, rec.d_bsk, ctx->stream); });
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(rec.d_ksk, hk.planes.data(), hk.planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        hipStreamSynchronize(ctx->stream);
+        hipFree(rec.d_bsk);
+        hipFree(rec.d_ksk);
+        return fail(YGPU_ERR_HIP, std::string("key upload: ") + hipGetErrorString(e));
+    }
     int id = -1;
     for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
     if (id < 0) { id = static_cast<int>(ctx->keys.size()); ctx->keys.push_back(rec); }
