@@ -27,6 +27,7 @@
 //                           derived row decrypts to -s_i 2^(w-(lvl+1)b) z (the RGSW
 //                           mask-digit row) within the noise bound; blind rotation
 //                           with the derived key decrypts to X^{sum a s} v
+//   level-parallel PBS      equal to the single-threaded PBS, bit for bit
 #include <cstdint>
 #include <cstring>
 #include <cmath>
