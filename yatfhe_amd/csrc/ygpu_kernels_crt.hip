@@ -8,9 +8,10 @@
 // transforming row r's digit polynomials mod p, owning output column r mod p.
 // Per step and per team: digits of row r; l forward transforms and the
 // multiply-accumulate against both BSK columns; the partner-column swap with
-// team (1-r, p); the inverse transform of column r; then the prime-1 team hands
-// its residues to the prime-0 team, which reconstructs the value mod q56 and
-// updates row r.  The accumulator rows live in LDS, read by both prime teams.
+// team (1-r, p); the inverse transform of column r; then the two prime teams of
+// row r swap half their residues and each reconstructs half the values mod q56
+// and updates them in row r.  The accumulator rows live in LDS, read by both
+// prime teams.
 // Splitting the primes over teams keeps each team's registers within two waves
 // per SIMD (no residue of one prime is held across the other), and the two
 // primes proceed in parallel.
@@ -31,7 +32,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
 
     __shared__ double scratch[CPW][4 * SCR];
     // the accumulator rows live in LDS, shared by the two prime teams of a row
-    // (registers are the binding resource); the prime-0 team writes them
+    // (registers are the binding resource); each updates half the positions
     __shared__ int64_t accbuf[CPW][K1][N];
     // forward twiddles of both primes (the forward transform runs l times per
     // step); the inverse twiddles are read through L1
@@ -168,22 +169,28 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
 
         ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
 
-        // residues of column tm mod both primes: swap with team (tm, 1-tp)
+        // residues of column tm mod both primes: the two prime teams of row tm
+        // split the reconstruction, team tp taking registers [8 tp, 8 tp + 8);
+        // each hands the other team the half it needs
 #pragma unroll
         for (int r = 0; r < 16; r++) x[r] = centre(x[r], pr);
         team_sync<TW>();
 #pragma unroll
-        for (int r = 0; r < 16; r++) scr[r * T + tl] = x[r];
+        for (int r = 0; r < 8; r++) {
+            const int rr = r + 8 * (1 - tp);
+            scr[rr * T + tl] = tp ? x[r] : x[r + 8];
+        }
         __syncthreads();
         {
             const double* other = scratch[lc] + (tt ^ 2) * SCR;
-            if (tp == 0) {
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const int j = tl + T * r;
-                    const int64_t val = crt_mod_q(x[r], other[r * T + tl], a.prime[1], a.crt);
-                    acc[j] = sext_t<C::TBITS>(acc[j] + val);
-                }
+            for (int r = 0; r < 8; r++) {
+                const int rr = r + 8 * tp;
+                const double mine = tp ? x[r + 8] : x[r];
+                const double theirs = other[rr * T + tl];
+                const int j = tl + T * rr;
+                const int64_t val = crt_mod_q(tp ? theirs : mine, tp ? mine : theirs, a.prime[1], a.crt);
+                acc[j] = sext_t<C::TBITS>(acc[j] + val);
             }
         }
         __syncthreads();
