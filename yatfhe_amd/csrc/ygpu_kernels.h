@@ -146,17 +146,19 @@ __device__ __forceinline__ void ks_digits(int32_t x, int t, int b, int width, in
 // Exact value mod q, centred in [-(q-1)/2, (q-1)/2], of the integer C with
 // C = c0 mod p0, C = c1 mod p1 and |C| < p0 p1 / 2 (Garner), from the centred
 // residues c0, c1.  C = c0 + p0 * y with y = (c1 - c0) p0^{-1} mod p1 centred;
-// the quotient C / q is estimated in binary64 (off by at most one) and the
-// remainder formed exactly in 128-bit integers.
+// the quotient t = C / q is estimated in binary64 (off by at most one), so the
+// remainder R = C - t q lies in (-3q/2, 3q/2), inside int64 for q < 2^61: it is
+// formed exactly by wrapping 64-bit arithmetic (C itself may exceed 64 bits;
+// its low 64 bits suffice) and corrected into the centred range.
 __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF64& P1, const CrtConst& k) {
     const double y = centre(mulmod(__dadd_rn(c1, -c0), k.inv01, P1), P1);
-    const __int128 C = static_cast<__int128>(k.p0i) * f64_to_i64(y) + f64_to_i64(c0);
     const double Cd = __fma_rn(k.p0, y, c0);
     const int64_t qt = static_cast<int64_t>(__builtin_rint(__dmul_rn(Cd, k.qinv)));
-    __int128 R = C - static_cast<__int128>(qt) * static_cast<__int128>(k.q);
-    if (R > k.qhalf) R -= static_cast<__int128>(k.q);
-    if (R < -k.qhalf) R += static_cast<__int128>(k.q);
-    return static_cast<int64_t>(R);
+    const uint64_t lowC = static_cast<uint64_t>(k.p0i) * static_cast<uint64_t>(f64_to_i64(y)) + static_cast<uint64_t>(f64_to_i64(c0));
+    int64_t R = static_cast<int64_t>(lowC - static_cast<uint64_t>(qt) * k.q);
+    if (R > k.qhalf) R -= static_cast<int64_t>(k.q);
+    if (R < -k.qhalf) R += static_cast<int64_t>(k.q);
+    return R;
 }
 
 // Tile choice of the blind-rotation launch: ciphertexts per workgroup and
