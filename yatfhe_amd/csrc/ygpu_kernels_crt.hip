@@ -121,8 +121,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         for (int lvl = 0; lvl < L; lvl++) {
             const int d = lvl * K1 + tm;
             const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + tp) * N + tl;
-            const double* __restrict__ kpo = kp + static_cast<size_t>(tm) * NP * N;
-            const double* __restrict__ kpt = kp + static_cast<size_t>(1 - tm) * NP * N;
+            // global address space: the compiler cannot infer it through the
+            // key table and would issue flat loads (which also count against
+            // the LDS wait counter)
+            using GD = const __attribute__((address_space(1))) double;
+            GD* __restrict__ kpo = (GD*)(kp + static_cast<size_t>(tm) * NP * N);
+            GD* __restrict__ kpt = (GD*)(kp + static_cast<size_t>(1 - tm) * NP * N);
             double x[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) {
