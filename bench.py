@@ -47,7 +47,7 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s (public spec)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_blind_rotate_t32_n1024_b4096_traffic.txt")
 SERVING_RATES = [1000, 2000, 5000, 10000, 20000, 30000, 40000, 50000]
 FP64_PEAK_TFLOPS = 78.6  # MI355X published dense FP64 (vector and matrix)
-LEGS = ("config1", "config2", "config3", "serving", "keyload")
+LEGS = ("t32_batch", "t56_batch", "multitenant", "serving", "keyload")
 
 
 def modmul_per_pbs(p) -> int:
@@ -164,8 +164,8 @@ def lut_inputs(yg, keys, count, seed, p=16):
     return msgs, yg.lwe_encrypt(keys.lwe_key, (msgs * (1 << 32) // (2 * p)).astype(np.int64).astype(np.int32), seed=seed)
 
 
-# --------------------------------------------------------------------- config 1
-def leg_config1(yg, args, peaks):
+# ------------------------------------------------------ TORUS32 N=1024 batch 4096
+def leg_t32_batch(yg, args, peaks):
     p = yg.PARAMS_T32_N1024
     keys = yg.keygen(p, seed=0)
     bits, cts = sign_inputs(yg, keys, BATCH, 1)
@@ -227,8 +227,8 @@ def leg_config1(yg, args, peaks):
     return p, res
 
 
-# --------------------------------------------------------------------- config 2
-def leg_config2(yg, args, peak):
+# ------------------------------------------------------ TORUS56 N=2048 batch 8192
+def leg_t56_batch(yg, args, peak):
     p = yg.PARAMS_T56_N2048
     keys = yg.keygen(p, seed=0)
     lut = np.random.default_rng(2).integers(0, 16, 16)
@@ -289,17 +289,17 @@ def leg_config2(yg, args, peak):
     }
 
 
-# --------------------------------------------------------------------- config 3
-def config3_counts():
+# ------------------------------------------------------ 64 tenants, one graph
+def multitenant_counts():
     """c_t = clamp(round(12288 r_t^-1.1 / H_64), 1, 512), ranks from a seed-4 permutation."""
     r = np.random.default_rng(4).permutation(64) + 1
     h = sum(k ** -1.1 for k in range(1, 65))
     return np.clip(np.round(12288 * r.astype(np.float64) ** -1.1 / h), 1, 512).astype(int)
 
 
-def leg_config3(yg, args, peak):
+def leg_multitenant(yg, args, peak):
     p = yg.PARAMS_T32_N1024
-    counts = config3_counts()
+    counts = multitenant_counts()
     total = int(counts.sum())
     v = yg.sign_test_polynomial(p)
     with yg.Context(p, 0) as ctx:
@@ -559,8 +559,8 @@ def main():
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     peaks = {}
-    log("config1: TORUS32 N=1024 batched 4096, plain form, CPU baseline")
-    p, r1 = leg_config1(yg, args, peaks)
+    log("TORUS32 N=1024: batched 4096, plain form, CPU baseline")
+    p, r1 = leg_t32_batch(yg, args, peaks)
     peak = max(peaks["f64"], peaks["u64"])
     value = r1["value"]
     mm = modmul_per_pbs(p)
@@ -620,18 +620,18 @@ def main():
     if "cpu_baseline" in r1:
         line["cpu_baseline"] = r1["cpu_baseline"]
     configs = {}
-    if "config2" in legs:
-        log("config2: TORUS56 N=2048 batch 8192")
-        configs["config2_t56_n2048_b8192"] = leg_config2(yg, args, peak)
-    if "config3" in legs:
-        log("config3: 64 tenants, hipGraph")
-        configs["config3_64_tenants_graph"] = leg_config3(yg, args, peak)
+    if "t56_batch" in legs:
+        log("TORUS56 N=2048 batch 8192")
+        configs["t56_n2048_b8192"] = leg_t56_batch(yg, args, peak)
+    if "multitenant" in legs:
+        log("64 tenants, hipGraph")
+        configs["multitenant_64_graph"] = leg_multitenant(yg, args, peak)
     if "keyload" in legs:
         log("key loads: full vs WWL+24 succinct form")
         configs["key_load_forms"] = leg_keyload(yg, args)
     if "serving" in legs:
         log("serving sweep")
-        configs["config4_serving"] = leg_serving(yg, args, peak)
+        configs["serving_256_tenants"] = leg_serving(yg, args, peak)
     if configs:
         line["configs"] = configs
     print(json.dumps(line), flush=True)
