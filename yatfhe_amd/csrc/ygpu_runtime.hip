@@ -36,6 +36,7 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
     ctx->device = device;
     ctx->tbits = params->torus_bits;
     ctx->logn = params->N == 1024 ? 10 : 11;
+    if (hipDeviceGetAttribute(&ctx->cu_count, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) ctx->cu_count = 0;
     const uint64_t q = params->q_ntt;
     if (q < (1ULL << 49)) {
         ctx->np = 1;
@@ -210,7 +211,7 @@ int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const in
             acc = ctx->d_acc;
         }
         BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, ctx->tile, a);
+        rc = run_br(ctx, batch_tile(ctx, count), a);
         if (rc) return rc;
         return run_ks(ctx, count, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, d_extract_out);
     });
@@ -483,7 +484,11 @@ int ygpu_set_tile(ygpu_ctx* ctx, int32_t cpw, int32_t twl) {
         BrTile c[8];
         const int n = blind_rotate_tiles(ctx->tbits, ctx->logn, ctx->p.level, ctx->p.radix_bits, c, 8);
         for (int i = 0; i < n; i++)
-            if (c[i].cpw == cpw && c[i].twl == twl) { ctx->tile = c[i]; return YGPU_OK; }
+            if (c[i].cpw == cpw && c[i].twl == twl) {
+                ctx->tile = c[i];
+                ctx->tile_fixed = true;
+                return YGPU_OK;
+            }
         return fail(YGPU_ERR_UNSUPPORTED, "tile not instantiated for this parameter set");
     });
 }
@@ -500,15 +505,25 @@ int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, in
         std::vector<int32_t> lwe(static_cast<size_t>(count) * (p.n + 1));
         uint64_t st = 0x243F6A8885A308D3ULL;
         for (auto& x : lwe) x = static_cast<int32_t>(splitmix64(st));
-        void *d_lwe = nullptr, *d_v = nullptr, *d_acc = nullptr;
-        YG_HIP(hipMalloc(&d_lwe, lwe.size() * sizeof(int32_t)));
-        YG_HIP(hipMalloc(&d_v, ctx->torus_bytes() * p.N));
-        YG_HIP(hipMalloc(&d_acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(count)));
-        YG_HIP(hipMemcpy(d_lwe, lwe.data(), lwe.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        YG_HIP(hipMemset(d_v, 0, ctx->torus_bytes() * p.N));
-        hipEvent_t e0, e1;
-        YG_HIP(hipEventCreate(&e0));
-        YG_HIP(hipEventCreate(&e1));
+        // released on every return path
+        struct AutotuneBuffers {
+            void *lwe = nullptr, *v = nullptr, *acc = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            ~AutotuneBuffers() {
+                for (void* b : {lwe, v, acc}) if (b) hipFree(b);
+                if (e0) hipEventDestroy(e0);
+                if (e1) hipEventDestroy(e1);
+            }
+        } sc;
+        YG_HIP(hipMalloc(&sc.lwe, lwe.size() * sizeof(int32_t)));
+        YG_HIP(hipMalloc(&sc.v, ctx->torus_bytes() * p.N));
+        YG_HIP(hipMalloc(&sc.acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(count)));
+        YG_HIP(hipMemcpy(sc.lwe, lwe.data(), lwe.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        YG_HIP(hipMemset(sc.v, 0, ctx->torus_bytes() * p.N));
+        YG_HIP(hipEventCreate(&sc.e0));
+        YG_HIP(hipEventCreate(&sc.e1));
+        void *d_lwe = sc.lwe, *d_v = sc.v, *d_acc = sc.acc;
+        hipEvent_t e0 = sc.e0, e1 = sc.e1;
         double best = 1e30;
         BrTile bestT = ctx->tile;
         for (int c = 0; c < nc; c++) {
@@ -526,11 +541,6 @@ int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, in
             }
             if (ms < best) { best = ms; bestT = cand[c]; }
         }
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        hipFree(d_lwe);
-        hipFree(d_v);
-        hipFree(d_acc);
         ctx->tile = bestT;
         if (cpw) *cpw = bestT.cpw;
         if (twl) *twl = bestT.twl;

@@ -371,7 +371,7 @@ int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
     YG_HIP(launch_copy_words(L.hd_v, L.d_v, count * vbytes / 4, L.stream));
     YG_HIP(launch_copy_words(L.hd_slots, L.d_slots, count, L.stream));
     BrArgs a = br_args(ps.ctx, count, L.d_in, L.d_v, p.N, L.d_acc, s->d_bsk_tab, L.d_slots);
-    int rc = run_br(ps.ctx, ps.ctx->tile, a, L.stream);
+    int rc = run_br(ps.ctx, batch_tile(ps.ctx, count), a, L.stream);
     if (rc) return rc;
     // one keyswitch launch; the kernel follows the per-ciphertext tenant slots
     rc = run_ks(ps.ctx, count, L.d_acc, s->d_ksk_tab, L.d_slots, L.d_out, nullptr, L.d_partial, L.stream);
