@@ -400,7 +400,7 @@ def leg_serving(yg, args, peak):
         keys32, keys56, in32, in56, exp32, exp56 = material[form]
         budget = int(budget_gb * (1 << 30))
         sweep = []
-        with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16,
+        with yg.Server(0, key_budget_bytes=budget, lanes=args.serving_lanes, max_batch=4096, queue_max=1 << 16,
                        eviction=args.serving_eviction) as srv:
             ps32 = srv.add_params(p32)
             ps56 = srv.add_params(p56)
@@ -539,6 +539,7 @@ def main():
     ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
     ap.add_argument("--serving-eviction", default="freq_bytes", choices=["freq_bytes", "lru"])
     ap.add_argument("--keyload-reps", type=int, default=5)
+    ap.add_argument("--serving-lanes", type=int, default=3, help="in-flight batches per parameter set")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "
