@@ -218,6 +218,18 @@ int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes
 int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe,
                 int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads);
 
+/* Seeded classic bootstrapping key (BootstrappingKey, bootstrapping.h:16-45;
+ * genBootstrappingKey -> genBootstrappingKeyNormal -> encryptTrgswNtt,
+ * bootstrapping.cpp:40-44, 98-104, trgsw.cpp:185-191, 227-246, 281-284): masks
+ * uniform over the torus, mu_lvl = s_i 2^(w - (lvl+1) b) added in the torus to
+ * a_row[0] of row r < k (the body keeps the mask's product from before) and to
+ * the torus value of the body of row k (centred and reduced to the torus, so
+ * that row's body residues are lifted torus values).  Layout, secret keys and
+ * KSK as ygpu_keygen; bootstraps through ygpu_pbs like any coefficient-form
+ * key (functionalBootstrappingNtt with l = lApprox). */
+int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe,
+                        int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads);
+
 /* Seeded WWL+24 succinct key generation (genBootstrappingKeyWWL24,
  * bootstrapping.cpp:106-126): cPrime[lvl] of step i encrypts s_i * 2^(w -
  * (lvl+1) b) at coefficient 0 (symEncTrlweMultiSampleNtt, trlwe.cpp:103-106:

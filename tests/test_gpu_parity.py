@@ -215,3 +215,24 @@ def test_all_4096_outputs_match_oracle(p32, keys32, ctx32, orc32):
     assert np.array_equal(acc, o_acc), "accumulators differ"
     assert np.array_equal(ext, o_ext), "extracted LWEs differ"
     assert np.array_equal(out, o_out), "keyswitched LWEs differ"
+
+
+def test_classic_key_bit_exact(p32):
+    """A classic key (genBootstrappingKey) through the batched and plain forms:
+    bit-exact against the oracle, decrypting every sign."""
+    keys = yg.keygen_classic(p32, seed=43)
+    bits = np.random.default_rng(8).integers(0, 2, 40)
+    cts = yg.lwe_encrypt(keys.lwe_key, ((2 * bits - 1) * (1 << 29)).astype(np.int32), seed=8)
+    v = yg.sign_test_polynomial(p32)
+    print("classic key: batch 40", flush=True)
+    with yg.Context(p32, 0) as ctx:
+        kid = ctx.upload_key(keys)
+        out, acc, _ = ctx.pbs_batch(kid, cts, v, want_acc=True)
+        one, _, _ = ctx.pbs(kid, cts[3], v)
+    orc = Oracle(p32, keys)
+    ref_acc, _, ref_out = orc.pbs_batch_all(cts[:4], v)
+    orc.close()
+    assert np.array_equal(ref_out, out[:4]) and np.array_equal(one, out[3])
+    assert np.array_equal(ref_acc.reshape(4, -1).astype(np.int64), acc[:4].reshape(4, -1).astype(np.int64))
+    ph = yg.lwe_decrypt_phase(out, keys.lwe_key).astype(np.int64)
+    assert np.array_equal((ph > 0).astype(np.int64), bits)

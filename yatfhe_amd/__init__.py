@@ -183,6 +183,7 @@ def load_library(path: str | None = None):
         "ygpu_graph_destroy": (c_int, [c_vp, c_i32]),
         "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
+        "ygpu_keygen_classic": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_succinct": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
         "ygpu_server_create": (c_int, [P(_ServerConfig), P(c_vp)]),
@@ -231,7 +232,7 @@ def exported_symbols():
         "ygpu_server_submit", "ygpu_server_wait", "ygpu_server_wait_for", "ygpu_server_stats_get", "ygpu_server_loadgen",
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
-        "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct",
+        "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
     ]
 
 
@@ -322,6 +323,23 @@ def keygen_succinct(params: Params, seed: int, s2_level: int | None = None, sigm
     _check(lib.ygpu_keygen_succinct(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe, s2_level,
                                     _ptr(lk), _ptr(gk), _ptr(half), _ptr(s2), _ptr(ksk), threads))
     return SuccinctKeySet(params, lk, gk, half, s2, ksk)
+
+
+def keygen_classic(params: Params, seed: int, sigma_log2_lwe: float | None = None,
+                   sigma_log2_glwe: float | None = None, threads: int = 0) -> KeySet:
+    """Seeded classic bootstrapping key (ygpu_keygen_classic): same layout as keygen."""
+    lib = load_library()
+    dl, dg = default_sigmas(params)
+    sigma_log2_lwe = dl if sigma_log2_lwe is None else sigma_log2_lwe
+    sigma_log2_glwe = dg if sigma_log2_glwe is None else sigma_log2_glwe
+    lk = np.zeros(params.n, np.int32)
+    gk = np.zeros(params.k * params.N, np.int32)
+    bsk = np.zeros(params.bsk_shape, np.uint64)
+    ksk = np.zeros(params.ksk_shape, np.int32)
+    cp = params.c()
+    _check(lib.ygpu_keygen_classic(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe,
+                                   _ptr(lk), _ptr(gk), _ptr(bsk), _ptr(ksk), threads))
+    return KeySet(params, lk, gk, bsk, ksk)
 
 
 def save_keys(path: str, keys):
@@ -808,5 +826,5 @@ __all__ = [
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
     "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
     "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
-    "key_file_s2_level", "load_succinct_arrays",
+    "key_file_s2_level", "load_succinct_arrays", "keygen_classic",
 ]

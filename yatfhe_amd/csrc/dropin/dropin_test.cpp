@@ -74,6 +74,17 @@ static void mp21_fr(const std::string& which) {
         for (int i = 0; i < param.n; i++) EXPECT_EQ(fused.a[i], output.a[i], "fused vs assembled mask");
         EXPECT_EQ(fused.b, output.b, "fused vs assembled body");
     }
+    // the classic key (genBootstrappingKey) through functionalBootstrappingNtt
+    BootstrappingKey bskClassic{param, param.lApprox};
+    genBootstrappingKey(bskClassic, trgswKey, tlweKey, param);
+    for (int plain = -p / 2; plain < p / 2; plain++) {
+        const int slot = ((plain % p) + p) % p;
+        Tlwe input{param.n};
+        Tlwe output{param.n};
+        symEncTlwe(input, modSwitchToTorusGeneral(slot, encMod, param.qLwe), tlweKey);
+        functionalBootstrappingNtt(output, input, bskClassic, ksKey, v, param);
+        EXPECT_EQ(plain, symDecTlweToInt(output, tlweKey, p), "classic key decrypted slot");
+    }
     // KSK.KSK: a fresh encryption under the flattened GLWE key switches to the LWE key
     TlweKey inKey(param.k * param.N, param.rlweNoiseB);
     for (int j = 0; j < param.N; j++) inKey.s[j] = trlweKey.s[0].coeffs[j];

@@ -484,6 +484,21 @@ inline void genTlweKeySwitchingKey(TlweKeySwitchingKey& ksk, const TrlweKey&, co
     ksk.generation = yatfhe_dropin::state().nextGeneration++;
 }
 
+// genBootstrappingKey (bootstrapping.cpp:98-104), group 1: the classic key
+// (encryptTrgswNtt per step), from the same draw as the other keys.
+inline void genBootstrappingKey(BootstrappingKey& bsk, TrgswKey&, const TlweKey&, const YatfheParameters& p) {
+    if (p.group != 1) throw std::invalid_argument("genBootstrappingKey: only group == 1 keys are supported");
+    yatfhe_dropin::ensure_draw(false);
+    auto& s = yatfhe_dropin::state();
+    const auto& d = yatfhe_dropin::draw();
+    if (bsk.bskCoeff.size() != d.bsk.size()) throw std::invalid_argument("genBootstrappingKey: level must equal param.lApprox");
+    const ygpu_params q = yatfhe_dropin::to_params(p, p.lApprox);
+    std::vector<int32_t> lwe(p.n), glwe(static_cast<size_t>(p.k) * p.N), ksk(d.ksk.size());
+    const double sg = p.torusBits == 32 ? -25.0 : -48.0;
+    yatfhe_dropin::check(ygpu_keygen_classic(&q, d.seed, -15.0, sg, lwe.data(), glwe.data(), bsk.bskCoeff.data(), ksk.data(), 0));
+    bsk.generation = s.nextGeneration++;
+}
+
 // genBootstrappingKeyWWL24 (bootstrapping.cpp:106-126), from the same draw as
 // the other keys (same secret keys), with the GLWE noise of param.rlweNoiseB
 // (a TUniform bound 2^B in torus units: log2 sigma = B - 0.79).

@@ -11,6 +11,9 @@
 //                                   (trlwe.cpp:25-48 samples them in the NTT domain,
 //                                   which is the same distribution)
 //   genTlweKeySwitchingKey          (keyswitching.cpp:19-29): KSK[i][j] = LWE_s(z_i * 2^(w - (j+1) b))
+//   genBootstrappingKey (classic)   (bootstrapping.cpp:40-44, 98-104; encryptTrgswNtt,
+//                                   trgsw.cpp:281-284): torus-uniform masks, mu added in the
+//                                   torus, the cPrime body reduced to the torus before it
 //   genBootstrappingKeyWWL24        (bootstrapping.cpp:106-126): the succinct key, body-digit
 //                                   rows only (torus-uniform masks, symEncTrlweMultiSampleNtt)
 //                                   plus the TRLEV of s^2 (symEncTrlevWithKeyNtt, trlev.cpp:75-94)
@@ -54,7 +57,7 @@ inline uint64_t lift(int64_t v, uint64_t q) {
 // stream identifiers
 constexpr uint64_t S_LWE_KEY = 1, S_GLWE_KEY = 2;
 constexpr uint64_t S_BSK = 1ULL << 40, S_KSK = 2ULL << 40, S_ENC = 3ULL << 40;
-constexpr uint64_t S_HALF = 4ULL << 40, S_S2 = 5ULL << 40;
+constexpr uint64_t S_HALF = 4ULL << 40, S_S2 = 5ULL << 40, S_CLASSIC = 6ULL << 40;
 
 int worker_threads(int threads) {
     return threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
@@ -222,6 +225,71 @@ int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_
                 }
             }
             for (int j = 0; j < N; j++) out[static_cast<size_t>(K) * N + j] = lift(b[j], q);
+        });
+        gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
+    } catch (...) {
+        return YGPU_ERR_INVALID_ARG;
+    }
+    return YGPU_OK;
+}
+
+int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t* lwe_key,
+                        int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
+    if (!params || !lwe_key || !glwe_key || !bsk_coeff || !ksk) return YGPU_ERR_INVALID_ARG;
+    const ygpu_params p = *params;
+    if (p.k < 1 || p.N < 2 || p.n < 1 || p.level < 1 || p.ks_level < 1) return YGPU_ERR_INVALID_ARG;
+    const int N = p.N, K = p.k, K1 = p.k + 1, L = p.level, n = p.n, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    try {
+        gen_secret_keys(p, seed, lwe_key, glwe_key);
+        HostNtt ntt(q, N);
+        std::vector<std::vector<uint64_t>> zt(K, std::vector<uint64_t>(N));
+        for (int c = 0; c < K; c++) {
+            for (int j = 0; j < N; j++) zt[c][j] = static_cast<uint64_t>(glwe_key[c * N + j]);
+            ntt.forward(zt[c].data());
+        }
+        const double sg = std::pow(2.0, sigma_log2_glwe + w);
+        const int nt = worker_threads(threads);
+        const uint64_t qhalf = (q - 1) / 2;
+        // rows [i][lvl][row][col][N]: encryptTrgswNtt (trgsw.cpp:281-284) =
+        // encZeroTrgswNtt (185-191: symEncTrlweSingleSampleNtt, torus-uniform
+        // mask, body completed mod q) + addIntegerToTrgswNtt (227-246: mu_lvl
+        // added in the torus to a_row[0] for row < k, re-transformed; for row k
+        // to the torus value of the body, i.e. after applyIntt's centring)
+        par_for(n * L * K1, nt, [&](int idx) {
+            const int i = idx / (L * K1);
+            const int lvl = (idx / K1) % L;
+            const int row = idx % K1;
+            Rng r(seed, S_CLASSIC + static_cast<uint64_t>(idx));
+            uint64_t* out = bsk_coeff + static_cast<size_t>(idx) * K1 * N;
+            std::vector<int64_t> at(static_cast<size_t>(K) * N);
+            std::vector<uint64_t> body(N, 0), tmp(N);
+            for (int j = 0; j < N; j++) body[j] = lift(sext(r.gauss(sg), w), q);
+            for (int c = 0; c < K; c++) {
+                uint64_t* a = out + static_cast<size_t>(c) * N;
+                for (int j = 0; j < N; j++) {
+                    at[static_cast<size_t>(c) * N + j] = sext(static_cast<int64_t>(r.next()), w);
+                    a[j] = lift(at[static_cast<size_t>(c) * N + j], q);
+                }
+                std::memcpy(tmp.data(), a, sizeof(uint64_t) * N);
+                ntt.forward(tmp.data());
+                for (int j = 0; j < N; j++) tmp[j] = mulmod(tmp[j], zt[c][j], q);
+                ntt.inverse(tmp.data());
+                for (int j = 0; j < N; j++) body[j] = (body[j] + tmp[j]) % q;
+            }
+            const int64_t mu = sext(static_cast<int64_t>(lwe_key[i]) << (w - (lvl + 1) * p.radix_bits), w);
+            if (row < K) {
+                uint64_t* a = out + static_cast<size_t>(row) * N;
+                a[0] = lift(sext(at[static_cast<size_t>(row) * N] + mu, w), q);
+            } else {
+                for (int j = 0; j < N; j++) {
+                    const int64_t centred = body[j] > qhalf ? static_cast<int64_t>(body[j]) - static_cast<int64_t>(q)
+                                                            : static_cast<int64_t>(body[j]);
+                    const int64_t bt = sext(centred + (j == 0 ? mu : 0), w);
+                    body[j] = lift(bt, q);
+                }
+            }
+            std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
         });
         gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
     } catch (...) {
