@@ -303,12 +303,15 @@ inline int run_br(ygpu_ctx* ctx, const BrTile& tile, const BrArgs& a, hipStream_
     return YGPU_OK;
 }
 
-inline BrTile plain_tile(const ygpu_ctx* ctx) { return BrTile{1, ctx->tile.twl}; }
+// One ciphertext per workgroup, on the latency form where it is instantiated.
+inline BrTile plain_tile(const ygpu_ctx* ctx) {
+    return BrTile{1, ctx->tile.twl, blind_rotate_lat_supported(ctx->tbits, ctx->logn, ctx->p.level, ctx->p.radix_bits) ? 1 : 0};
+}
 // Tile of a batch of `count` ciphertexts: a batch no larger than the number of
-// compute units gives every ciphertext a workgroup of its own on the one-
-// ciphertext tile, which finishes sooner than pairs in lockstep (7.4 against
-// 8.6 ms at TORUS32 N = 1024); larger batches take the throughput tile.  A tile
-// set explicitly is used for every size.
+// compute units gives every ciphertext a workgroup of its own on the plain
+// tile (the latency form where instantiated), which finishes sooner than pairs
+// in lockstep; larger batches take the throughput tile.  A tile set explicitly
+// is used for every size.
 inline BrTile batch_tile(const ygpu_ctx* ctx, int count) {
     if (!ctx->tile_fixed && count <= ctx->cu_count) return plain_tile(ctx);
     return ctx->tile;

@@ -166,8 +166,14 @@ __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF6
 struct BrTile {
     int cpw = 4;
     int twl = 1;
+    int lat = 0;  // the latency form: one wavefront per (row, gadget level)
 };
 hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& tile, int radix_bits, const BrArgs& a, hipStream_t s);
+// The latency form of the TORUS32 N = 1024 gadget-2^7x3 blind rotation (one
+// ciphertext per workgroup, a wavefront per row and level), in its own
+// translation unit.
+bool blind_rotate_lat_supported(int tbits, int logn, int level, int radix_bits);
+hipError_t launch_blind_rotate_lat(const BrArgs& a, hipStream_t s);
 // The two-prime (CRT) blind rotation, in its own translation unit.
 hipError_t launch_blind_rotate_crt(int logn, const BrTile& tile, int radix_bits, int level, const BrArgs& a, hipStream_t s);
 // Tiles instantiated for a parameter set (the autotuner's candidates).

@@ -514,6 +514,7 @@ hipError_t launch_br(const BrArgs& a, hipStream_t s) {
 // Specialised gadget shapes (the headline parameter sets) come in several
 // tiles; every other parameter set runs a runtime-gadget instantiation.
 hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& t, int radix_bits, const BrArgs& a, hipStream_t s) {
+    if (t.lat && blind_rotate_lat_supported(tbits, logn, a.level, radix_bits)) return launch_blind_rotate_lat(a, s);
     const bool p8 = radix_bits <= 8;
     if (tbits == 32 && logn == 10 && a.level == 3 && radix_bits == 7) {
         if (t.cpw >= 4) return t.twl ? launch_br<BrCfg<32, 10, 1, 4, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, true, 3, 7, 0>>(a, s);

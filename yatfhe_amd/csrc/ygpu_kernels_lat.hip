@@ -1,0 +1,169 @@
+// ygpu_kernels_lat.hip -- latency form of the TORUS32 blind rotation.
+//
+// One ciphertext per workgroup, one wavefront per (accumulator row r, gadget
+// level l): 2 x L waves.  The throughput kernel gives each row one wavefront
+// that runs all L forward transforms and multiply-accumulates in turn.  Here
+// the levels run side by side, and the per-step chain gets shorter:
+//   every wave: (X^a - 1) row r and its digits (all levels; the carry chain
+//               needs them), the forward transform of digit level l, and the
+//               products with the BSK entries of (level l, row r) for both
+//               output columns;
+//   barrier;
+//   wave (c, 0): sums the 2 L partial products of column c, inverse
+//               transforms it, centres, reduces to the torus and accumulates
+//               row c (held in LDS);
+//   barrier.
+// This costs the rotation and digit work L times over and leaves most waves
+// idle during the inverse transform, so it is the form for batches too small
+// to fill the GPU (the plain drop-in call, short serving batches), not for
+// throughput.  The arithmetic is the same exact mod-q arithmetic as
+// k_blind_rotate: partial sums differ only in representative, and the inverse
+// transform reduces them, so the results are bit-identical.
+//
+// Its own translation unit: the throughput kernels' register allocation is not
+// perturbed.
+#include "ygpu_kernels.h"
+
+namespace ygpu {
+
+namespace {
+
+constexpr int LAT_LOGN = 10, LAT_N = 1 << LAT_LOGN, LAT_T = LAT_N / 16, LAT_L = 3, LAT_RB = 7, LAT_K1 = 2;
+constexpr int LAT_WAVES = LAT_K1 * LAT_L;
+constexpr int LAT_THREADS = 64 * LAT_WAVES;
+constexpr int LAT_SCR = Scratch<LAT_LOGN>::DOUBLES;
+
+}  // namespace
+
+__global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
+    constexpr int N = LAT_N, T = LAT_T, L = LAT_L, K1 = LAT_K1, LOGN = LAT_LOGN;
+    // per wave: transform scratch (also its partial sum of column r), and its
+    // partial sum of column 1 - r
+    __shared__ double scratch[LAT_WAVES][LAT_SCR];
+    __shared__ double other[LAT_WAVES][N];
+    __shared__ double tws[2 * N];
+    __shared__ int32_t acc[K1][N];
+
+    const int ct = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
+    if (ct >= a.count) return;  // whole workgroup
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / 64);
+    const int r = w / L;      // accumulator row whose digits this wave transforms
+    const int lvl = w % L;    // gadget level
+    const int tl = static_cast<int>(threadIdx.x) % 64;
+    double* scr = scratch[w];
+
+    const int n = a.n;
+    const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
+    const double* __restrict__ bsk = a.bsk_table[slot];
+    const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
+    const int32_t* __restrict__ v = static_cast<const int32_t*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
+    const PrimeF64 pr = a.prime[0];
+
+    for (int j = static_cast<int>(threadIdx.x); j < 2 * N; j += LAT_THREADS) tws[j] = a.tw[j];
+    // accumulator: caller's, or genNoiselessTrlweSample (trlwe.cpp:175-182)
+    // after rescaleTlweToNewMod (tlwe.cpp:75-81): (0, X^{-bbar} v)
+    if (lvl == 0) {
+        if (a.acc_in) {
+            const int32_t* ai = static_cast<const int32_t*>(a.acc_in) + static_cast<size_t>(ct) * K1 * N + r * N;
+            for (int q = 0; q < 16; q++) acc[r][tl + T * q] = ai[tl + T * q];
+        } else {
+            const int bbar = a.prescaled ? lwe[n] : mod_switch_2n(lwe[n], LOGN + 1);
+            int aT, wrap;
+            validate_rotator(-bbar, N, aT, wrap);
+            for (int q = 0; q < 16; q++) {
+                const int j = tl + T * q;
+                const int src = j - aT;
+                const int64_t val = src < 0 ? -static_cast<int64_t>(v[src + N]) * wrap : static_cast<int64_t>(v[src]) * wrap;
+                acc[r][j] = r == 0 ? 0 : static_cast<int32_t>(sext_t<32>(val));
+            }
+        }
+    }
+    __syncthreads();
+
+    const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * N;  // doubles per BSK row
+    using GD = const __attribute__((address_space(1))) double;
+    for (int i = 0; i < n; i++) {
+        const int abar = a.prescaled ? lwe[i] : mod_switch_2n(lwe[i], LOGN + 1);
+        int aT, wrap;
+        validate_rotator(abar, N, aT, wrap);
+        // (X^a - 1) row r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238),
+        // digit level lvl of it (gadget_decomposition.h:75-95)
+        const bool flip = wrap < 0;
+        double x[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int j = tl + T * q;
+            const int src = (j - aT) & (N - 1);
+            const bool negs = (j < aT) != flip;
+            const uint32_t av = static_cast<uint32_t>(acc[r][src]);
+            const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(acc[r][j]);
+            uint32_t d0, d1;
+            digits_packed<32, L, LAT_RB>(static_cast<int32_t>(rot), d0, d1);
+            // the signed byte of level lvl.  Not __builtin_amdgcn_sbfe: with
+            // this wave-uniform (non-constant) offset, hipcc converts its
+            // result to double as an unsigned value (v_cvt_f64_u32).
+            x[q] = static_cast<double>(static_cast<int8_t>(d0 >> (8 * lvl)));
+        }
+        ntt_forward<LOGN, 1>(x, scr, tl, tws, pr);
+        // products with BSK row (level lvl, input row r): column r and 1 - r
+        const int d = lvl * K1 + r;
+        const double* kp = bsk + static_cast<size_t>(i) * stepStride + static_cast<size_t>(d) * K1 * N + tl;
+        GD* __restrict__ kown = (GD*)(kp + static_cast<size_t>(r) * N);
+        GD* __restrict__ koth = (GD*)(kp + static_cast<size_t>(1 - r) * N);
+        double po[16], pt[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) po[q] = mulmod(x[q], kown[q * T], pr);
+#pragma unroll
+        for (int q = 0; q < 16; q++) pt[q] = mulmod(x[q], koth[q * T], pr);
+        // the forward transform's last exchange reads are this wave's own
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            scr[q * T + tl] = po[q];
+            other[w][q * T + tl] = pt[q];
+        }
+        __syncthreads();
+        if (lvl == 0) {
+            // column r: the own products of the waves of row r and the other
+            // products of the waves of row 1 - r
+            double s[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                double t = 0.0;
+                for (int l2 = 0; l2 < L; l2++) {
+                    t = __dadd_rn(t, scratch[r * L + l2][q * T + tl]);
+                    t = __dadd_rn(t, other[(1 - r) * L + l2][q * T + tl]);
+                }
+                s[q] = t;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            ntt_inverse<LOGN, 1>(s, scr, tl, tws + N, pr);
+            // centre mod q, reduce to the torus, accumulate (mod 2^32)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int j = tl + T * q;
+                acc[r][j] = static_cast<int32_t>(static_cast<uint32_t>(acc[r][j]) + f64_low32(centre(s[q], pr)));
+            }
+        }
+        __syncthreads();
+    }
+
+    if (lvl == 0) {
+        int32_t* __restrict__ outAcc = static_cast<int32_t*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + r * N;
+        for (int q = 0; q < 16; q++) outAcc[tl + T * q] = acc[r][tl + T * q];
+    }
+}
+
+bool blind_rotate_lat_supported(int tbits, int logn, int level, int radix_bits) {
+    return tbits == 32 && logn == LAT_LOGN && level == LAT_L && radix_bits == LAT_RB;
+}
+
+hipError_t launch_blind_rotate_lat(const BrArgs& a, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>((a.count + 7) / 8 * 8);  // map_workgroup needs a multiple of 8
+    hipLaunchKernelGGL(k_blind_rotate_lat, dim3(grid), dim3(LAT_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace ygpu
