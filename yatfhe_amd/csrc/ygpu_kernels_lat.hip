@@ -43,6 +43,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     __shared__ double other[LAT_WAVES][N];
     __shared__ double tws[2 * N];
     __shared__ int32_t acc[K1][N];
+    __shared__ int32_t abars[KS_MAX_N];  // the ciphertext's rotation amounts, mod-switched once
 
     const int ct = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     if (ct >= a.count) return;  // whole workgroup
@@ -60,6 +61,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     const PrimeF64 pr = a.prime[0];
 
     for (int j = static_cast<int>(threadIdx.x); j < 2 * N; j += LAT_THREADS) tws[j] = a.tw[j];
+    for (int j = static_cast<int>(threadIdx.x); j < n; j += LAT_THREADS) abars[j] = a.prescaled ? lwe[j] : mod_switch_2n(lwe[j], LOGN + 1);
     // accumulator: caller's, or genNoiselessTrlweSample (trlwe.cpp:175-182)
     // after rescaleTlweToNewMod (tlwe.cpp:75-81): (0, X^{-bbar} v)
     if (lvl == 0) {
@@ -82,10 +84,20 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * N;  // doubles per BSK row
     using GD = const __attribute__((address_space(1))) double;
+    // this wave's BSK entries (level lvl, input row r) of step i, for output
+    // column r (own) and 1 - r (oth): kp + i * stepStride.  Step i + 1's are
+    // loaded while step i's barriers and inverse transform run.
+    const double* kp = bsk + static_cast<size_t>(lvl * K1 + r) * K1 * N + tl;
+    double kown[16], koth[16];
+    {
+        GD* __restrict__ ko = (GD*)(kp + static_cast<size_t>(r) * N);
+        GD* __restrict__ kt = (GD*)(kp + static_cast<size_t>(1 - r) * N);
+#pragma unroll
+        for (int q = 0; q < 16; q++) { kown[q] = ko[q * T]; koth[q] = kt[q * T]; }
+    }
     for (int i = 0; i < n; i++) {
-        const int abar = a.prescaled ? lwe[i] : mod_switch_2n(lwe[i], LOGN + 1);
         int aT, wrap;
-        validate_rotator(abar, N, aT, wrap);
+        validate_rotator(abars[i], N, aT, wrap);
         // (X^a - 1) row r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238),
         // digit level lvl of it (gadget_decomposition.h:75-95)
         const bool flip = wrap < 0;
@@ -106,15 +118,18 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
         }
         ntt_forward<LOGN, 1>(x, scr, tl, tws, pr);
         // products with BSK row (level lvl, input row r): column r and 1 - r
-        const int d = lvl * K1 + r;
-        const double* kp = bsk + static_cast<size_t>(i) * stepStride + static_cast<size_t>(d) * K1 * N + tl;
-        GD* __restrict__ kown = (GD*)(kp + static_cast<size_t>(r) * N);
-        GD* __restrict__ koth = (GD*)(kp + static_cast<size_t>(1 - r) * N);
         double po[16], pt[16];
 #pragma unroll
-        for (int q = 0; q < 16; q++) po[q] = mulmod(x[q], kown[q * T], pr);
+        for (int q = 0; q < 16; q++) po[q] = mulmod(x[q], kown[q], pr);
 #pragma unroll
-        for (int q = 0; q < 16; q++) pt[q] = mulmod(x[q], koth[q * T], pr);
+        for (int q = 0; q < 16; q++) pt[q] = mulmod(x[q], koth[q], pr);
+        if (i + 1 < n) {
+            const double* kn = kp + static_cast<size_t>(i + 1) * stepStride;
+            GD* __restrict__ ko = (GD*)(kn + static_cast<size_t>(r) * N);
+            GD* __restrict__ kt = (GD*)(kn + static_cast<size_t>(1 - r) * N);
+#pragma unroll
+            for (int q = 0; q < 16; q++) { kown[q] = ko[q * T]; koth[q] = kt[q * T]; }
+        }
         // the forward transform's last exchange reads are this wave's own
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -161,6 +176,7 @@ bool blind_rotate_lat_supported(int tbits, int logn, int level, int radix_bits) 
 }
 
 hipError_t launch_blind_rotate_lat(const BrArgs& a, hipStream_t s) {
+    if (a.n < 1 || a.n > KS_MAX_N) return hipErrorInvalidValue;  // abars[] holds at most KS_MAX_N
     const unsigned grid = static_cast<unsigned>((a.count + 7) / 8 * 8);  // map_workgroup needs a multiple of 8
     hipLaunchKernelGGL(k_blind_rotate_lat, dim3(grid), dim3(LAT_THREADS), 0, s, a);
     return hipGetLastError();
