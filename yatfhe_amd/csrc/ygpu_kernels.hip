@@ -96,10 +96,15 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     const double* twBase = C::TW_LDS ? static_cast<const double*>(tws) : a.tw;
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
+    // the next step's rotation amount is loaded a step ahead, so no step
+    // starts with a memory round trip
+    int32_t rawNext = lwe[0];
     for (int i = 0; i < n; i++) {
         // (blind_rotate.cpp:1557-1559 skips a_i == 0; ExtProd(0) = 0 exactly,
         // so the lockstep workgroup simply computes it.)
-        const int abar = a.prescaled ? lwe[i] : mod_switch_2n(lwe[i], LOGN + 1);
+        const int32_t raw = rawNext;
+        if (i + 1 < n) rawNext = lwe[i + 1];
+        const int abar = a.prescaled ? raw : mod_switch_2n(raw, LOGN + 1);
         int aT, wrap;
         validate_rotator(abar, N, aT, wrap);
 

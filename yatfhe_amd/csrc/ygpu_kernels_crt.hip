@@ -88,8 +88,13 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     __syncthreads();
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
+    // the next step's rotation amount is loaded a step ahead, so no step
+    // starts with a memory round trip
+    int32_t rawNext = lwe[0];
     for (int i = 0; i < n; i++) {
-        const int abar = a.prescaled ? lwe[i] : mod_switch_2n(lwe[i], LOGN + 1);
+        const int32_t raw = rawNext;
+        if (i + 1 < n) rawNext = lwe[i + 1];
+        const int abar = a.prescaled ? raw : mod_switch_2n(raw, LOGN + 1);
         int aT, wrap;
         validate_rotator(abar, N, aT, wrap);
 
