@@ -80,6 +80,12 @@ typedef struct ygpu_job {
 
 const char* ygpu_last_error(void);
 int ygpu_device_count(int32_t* count);
+/* Sizes in bytes of the ABI structs, in the order ygpu_params, ygpu_job,
+ * ygpu_server_config, ygpu_request, ygpu_server_stats, ygpu_loadgen_config,
+ * ygpu_loadgen_result; writes min(max, 7) of them and returns how many exist.
+ * No counterpart in the reference (its boundary is C++): it lets a binding
+ * check its struct mirrors against the library it loaded. */
+int ygpu_abi_struct_sizes(int64_t* out, int32_t max);
 
 /* Replaces initYatfhe (initializer.cpp:50-85): validates the parameter set,
  * builds the transform tables and binds the context to `device`.  Unlike the
@@ -299,7 +305,24 @@ typedef struct ygpu_server_config {
     int32_t eviction;         /* YGPU_EVICT_FREQ_BYTES (0) or YGPU_EVICT_LRU (1) */
     int32_t reserved;
     int64_t key_budget_bytes; /* device bytes for resident keys (0: 3/4 of free memory) */
+    int32_t sched;            /* YGPU_SCHED_SHARED (0), YGPU_SCHED_AFFINITY (1) or YGPU_SCHED_SOFT (2) */
+    int32_t spill_threshold;  /* soft: owner-lane depth (requests) at which a request spills (0: 2) */
+    int32_t steal_threshold;  /* soft: depth (requests) a lane needs before an idle lane steals (0: 2) */
+    int32_t reserved2;
 } ygpu_server_config;
+
+/* Request-to-lane scheduling of the server, the modes of the reference's
+ * serving Scheduler (ya_benchmark/include/blindrotate_serving.h:132-273) with
+ * the server's lanes (in-flight batches of a parameter set) as its workers.
+ * SHARED: any free lane batches any tenant's requests.  AFFINITY: tenant t's
+ * requests are batched only by lane t % lanes of its parameter set.  SOFT:
+ * affinity, but a request whose owner lane already has spill_threshold
+ * requests waiting goes to the shortest lane when that is shorter by more
+ * than one, and a lane with nothing of its own takes (steals) the requests of
+ * the longest lane once that holds steal_threshold or more. */
+#define YGPU_SCHED_SHARED 0
+#define YGPU_SCHED_AFFINITY 1
+#define YGPU_SCHED_SOFT 2
 
 /* Key eviction policies of the server's device cache.  FREQ_BYTES evicts the
  * keys whose loss costs the least host-to-device traffic: decayed request rate
@@ -322,6 +345,9 @@ typedef struct ygpu_request {
     uint64_t seq;             /* internal */
     int32_t status;           /* out: 1 pending, YGPU_OK done, < 0 error */
     int32_t key_hit;          /* out: the tenant's key was resident when the request arrived */
+    int32_t lane;             /* out: lane of its parameter set that served it */
+    int32_t coalesced;        /* out: its key was absent and already awaited by an earlier request
+                                 (it shared that load: the reference's waitedForPeer) */
 } ygpu_request;
 
 typedef struct ygpu_server_stats {
@@ -329,6 +355,7 @@ typedef struct ygpu_server_stats {
     int64_t batches, batched_cts, request_hits;
     int64_t key_loads, key_evictions, bytes_loaded, memory_waits;
     int64_t resident_keys, resident_bytes, peak_resident_bytes, key_budget_bytes, host_key_bytes;
+    int64_t coalesced, spilled, stolen, peak_queue_depth;
 } ygpu_server_stats;
 
 int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out);
@@ -391,6 +418,15 @@ typedef struct ygpu_loadgen_result {
     double hit_p50_ms, hit_p99_ms;      /* requests whose key was resident on arrival */
     double miss_p50_ms, miss_p99_ms;    /* requests that waited for a key load */
     double pset_p50_ms[4], pset_p99_ms[4];  /* by parameter set (first four) */
+    /* the rest of the reference's serving record (blindrotate_serving.cpp:289-353) */
+    double sd_ms;                       /* latency (sojourn) standard deviation */
+    double queue_mean_ms, queue_p90_ms, queue_max_ms;
+    double service_mean_ms, service_p50_ms, service_p90_ms, service_p99_ms, service_max_ms;  /* launch to result */
+    double slo_violation_rate_100, slo_violation_rate_300, slo_violation_rate_500;
+    int64_t coalesced_loads, peak_queue_depth, spilled, stolen;
+    double tenant_fairness_jain;        /* Jain's index of the per-tenant mean latencies */
+    double locality;                    /* share served by the tenant's owner lane (0 under SHARED) */
+    int64_t lane_requests[16];          /* requests served, by parameter set * lanes + lane (first 16) */
 } ygpu_loadgen_result;
 
 int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* cfg, ygpu_loadgen_result* out);

@@ -27,6 +27,11 @@ def test_header_symbols_exported():
     assert set(names) == set(yg.exported_symbols())
 
 
+def test_struct_mirrors_match_the_library():
+    # every ctypes mirror has the size the library was compiled with
+    assert yg.abi_struct_mismatches() == []
+
+
 def test_no_device_is_an_error_not_a_crash():
     if yg.device_count() > 0:
         pytest.skip("a GPU is present")

@@ -132,6 +132,46 @@ def test_loadgen_results_match_and_account(p32, tenants):
         assert res["key_evictions"] > 0
 
 
+@pytest.mark.parametrize("sched", ["shared", "affinity", "soft"])
+def test_scheduling_modes(p32, tenants, sched):
+    """The reference Scheduler's modes with lanes as its workers
+    (blindrotate_serving.h:132-273): every result is the same, affinity serves
+    each tenant on lane t % lanes only, and the reference's record fields
+    account for every request."""
+    k32, _ = tenants
+    v32 = yg.sign_test_polynomial(p32)
+    base_cts = [t32_inputs(k, 1, 95 + i)[0] for i, k in enumerate(k32[:2])]
+    expected_base = [context_results(p32, k, ct[None, :], v32)[0] for k, ct in zip(k32[:2], base_cts)]
+    lanes = 3
+    with yg.Server(0, lanes=lanes, max_batch=256, sched=sched, spill_threshold=2, steal_threshold=2) as srv:
+        ps = srv.add_params(p32)
+        base = [srv.add_tenant(ps, k) for k in k32[:2]]
+        n_t = 12
+        for t in range(2, n_t):
+            srv.add_tenant_shared(base[t % 2])
+        inputs = [base_cts[t % 2] for t in range(n_t)]
+        expected = [expected_base[t % 2] for t in range(n_t)]
+        print(f"server: loadgen sched={sched}", flush=True)
+        res = srv.loadgen(rate=4000, requests=800, inputs=inputs, luts=[v32] * n_t, expected=expected, warm=24, seed=9)
+        rep = srv.report(res)
+        print("report", {k: rep[k] for k in ("throughput_rps", "locality", "spilled", "stolen",
+                                             "per_worker_requests", "peak_queue_depth", "coalesced_loads")}, flush=True)
+    assert res["errors"] == 0 and res["mismatches"] == 0
+    assert res["completed"] + res["shed"] == 800
+    assert sum(res["lane_requests"]) == res["completed"]
+    assert rep["sched"] == sched and rep["shed_plus_completed"] == 800
+    assert rep["sojourn"]["p50_ms"] <= rep["sojourn"]["p99_ms"] <= rep["sojourn"]["max_ms"]
+    assert 0 < rep["tenant_fairness_jain"] <= 1.0
+    if sched == "shared":
+        assert res["locality"] == 0 and res["spilled"] == 0 and res["stolen"] == 0
+    elif sched == "affinity":
+        # tenant t only on lane t % lanes: with 12 tenants every lane has owners
+        assert res["locality"] == 1.0 and res["spilled"] == 0 and res["stolen"] == 0
+        assert all(c > 0 for c in res["lane_requests"])
+    else:
+        assert 0 < res["locality"] <= 1.0
+
+
 def test_succinct_tenants_alone_equal_mix(p32):
     """Tenants holding WWL+24 succinct keys: each load moves the succinct form and
     derives the full rows on the device; results equal the single-context

@@ -54,19 +54,26 @@ class _Job(ctypes.Structure):
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("lanes", ctypes.c_int32), ("max_batch", ctypes.c_int32),
                 ("queue_max", ctypes.c_int32), ("max_tenants", ctypes.c_int32), ("max_loads", ctypes.c_int32),
-                ("eviction", ctypes.c_int32), ("reserved", ctypes.c_int32), ("key_budget_bytes", ctypes.c_int64)]
+                ("eviction", ctypes.c_int32), ("reserved", ctypes.c_int32), ("key_budget_bytes", ctypes.c_int64),
+                ("sched", ctypes.c_int32), ("spill_threshold", ctypes.c_int32), ("steal_threshold", ctypes.c_int32),
+                ("reserved2", ctypes.c_int32)]
+
+# Scheduling modes of the server (the reference serving Scheduler's names).
+SCHED_MODES = {"shared": 0, "affinity": 1, "soft": 2}
 
 
 class _Request(ctypes.Structure):
     _fields_ = [("tenant", ctypes.c_int32), ("batch_size", ctypes.c_int32), ("lwe_in", ctypes.c_void_p),
                 ("v", ctypes.c_void_p), ("lwe_out", ctypes.c_void_p), ("arrival_ns", ctypes.c_int64),
                 ("launch_ns", ctypes.c_int64), ("done_ns", ctypes.c_int64), ("seq", ctypes.c_uint64),
-                ("status", ctypes.c_int32), ("key_hit", ctypes.c_int32)]
+                ("status", ctypes.c_int32), ("key_hit", ctypes.c_int32), ("lane", ctypes.c_int32),
+                ("coalesced", ctypes.c_int32)]
 
 
 _STAT_FIELDS = ["submitted", "completed", "failed", "shed", "batches", "batched_cts", "request_hits", "key_loads",
                 "key_evictions", "bytes_loaded", "memory_waits", "resident_keys", "resident_bytes",
-                "peak_resident_bytes", "key_budget_bytes", "host_key_bytes"]
+                "peak_resident_bytes", "key_budget_bytes", "host_key_bytes", "coalesced", "spilled", "stolen",
+                "peak_queue_depth"]
 
 
 class _ServerStats(ctypes.Structure):
@@ -89,7 +96,14 @@ class _LoadgenResult(ctypes.Structure):
                 + [("batches", ctypes.c_int64), ("mean_batch", ctypes.c_double), ("hit_rate", ctypes.c_double)]
                 + [(f, ctypes.c_int64) for f in ("key_loads", "key_evictions", "bytes_loaded", "memory_waits")]
                 + [(f, ctypes.c_double) for f in ("hit_p50_ms", "hit_p99_ms", "miss_p50_ms", "miss_p99_ms")]
-                + [("pset_p50_ms", ctypes.c_double * 4), ("pset_p99_ms", ctypes.c_double * 4)])
+                + [("pset_p50_ms", ctypes.c_double * 4), ("pset_p99_ms", ctypes.c_double * 4)]
+                + [(f, ctypes.c_double) for f in ("sd_ms", "queue_mean_ms", "queue_p90_ms", "queue_max_ms",
+                                                   "service_mean_ms", "service_p50_ms", "service_p90_ms",
+                                                   "service_p99_ms", "service_max_ms", "slo_violation_rate_100",
+                                                   "slo_violation_rate_300", "slo_violation_rate_500")]
+                + [(f, ctypes.c_int64) for f in ("coalesced_loads", "peak_queue_depth", "spilled", "stolen")]
+                + [("tenant_fairness_jain", ctypes.c_double), ("locality", ctypes.c_double),
+                   ("lane_requests", ctypes.c_int64 * 16)])
 
 
 @dataclass(frozen=True)
@@ -150,6 +164,7 @@ def load_library(path: str | None = None):
     sigs = {
         "ygpu_last_error": (ctypes.c_char_p, []),
         "ygpu_device_count": (c_int, [P(c_i32)]),
+        "ygpu_abi_struct_sizes": (c_int, [P(c_i64), c_i32]),
         "ygpu_init": (c_int, [P(_Params), c_i32, P(c_vp)]),
         "ygpu_destroy": (c_int, [c_vp]),
         "ygpu_get_params": (c_int, [c_vp, P(_Params)]),
@@ -233,7 +248,22 @@ def exported_symbols():
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
+        "ygpu_abi_struct_sizes",
     ]
+
+
+def abi_struct_mismatches():
+    """(name, library size, binding size) of every ABI struct whose ctypes
+    mirror disagrees with the loaded library (empty when they all agree)."""
+    mirrors = [("ygpu_params", _Params), ("ygpu_job", _Job), ("ygpu_server_config", _ServerConfig),
+               ("ygpu_request", _Request), ("ygpu_server_stats", _ServerStats),
+               ("ygpu_loadgen_config", _LoadgenConfig), ("ygpu_loadgen_result", _LoadgenResult)]
+    sizes = (ctypes.c_int64 * len(mirrors))()
+    n = load_library().ygpu_abi_struct_sizes(sizes, len(mirrors))
+    if n != len(mirrors):
+        return [("struct count", n, len(mirrors))]
+    return [(name, sizes[i], ctypes.sizeof(cls)) for i, (name, cls) in enumerate(mirrors)
+            if sizes[i] != ctypes.sizeof(cls)]
 
 
 def _check(rc):
@@ -708,14 +738,19 @@ class Server:
     """Continuous-batching bootstrapping server (ygpu_server_*)."""
 
     def __init__(self, device: int = 0, key_budget_bytes: int = 0, lanes: int = 0, max_batch: int = 0,
-                 queue_max: int = 0, max_tenants: int = 0, max_loads: int = 0, eviction: str = "freq_bytes"):
+                 queue_max: int = 0, max_tenants: int = 0, max_loads: int = 0, eviction: str = "freq_bytes",
+                 sched: str = "shared", spill_threshold: int = 0, steal_threshold: int = 0):
         self.lib = load_library()
         self.h = ctypes.c_void_p()
         policy = {"freq_bytes": 0, "lru": 1}[eviction]
-        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, max_loads, policy, 0, key_budget_bytes)
+        cfg = _ServerConfig(device, lanes, max_batch, queue_max, max_tenants, max_loads, policy, 0, key_budget_bytes,
+                            SCHED_MODES[sched], spill_threshold, steal_threshold, 0)
         _check(self.lib.ygpu_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.params = []
         self.tenant_params = []
+        self.config = {"lanes": lanes or 3, "max_batch": max_batch or 4096, "queue_max": queue_max or (1 << 20),
+                       "eviction": eviction, "sched": sched, "spill_threshold": spill_threshold or 2,
+                       "steal_threshold": steal_threshold or 2, "key_budget_bytes": key_budget_bytes}
 
     def close(self):
         if self.h:
@@ -817,8 +852,60 @@ class Server:
         out = {}
         for f, _ in _LoadgenResult._fields_:
             v = getattr(res, f)
-            out[f] = list(v)[:len(self.params)] if f.startswith("pset_") else v
+            if f.startswith("pset_"):
+                v = list(v)[:len(self.params)]
+            elif f == "lane_requests":
+                v = list(v)[:min(16, len(self.params) * self.config["lanes"])]
+            out[f] = v
+        out["load"] = {"rate": rate, "requests": requests, "warm": warm, "zipf_s": zipf_s, "seed": seed,
+                       "slo_ms": slo_ms, "users": n}
         return out
+
+    def report(self, res: dict) -> dict:
+        """One loadgen result as the reference serving benchmark's JSON record
+        (ya_benchmark/blindrotate_serving.cpp:289-353): the same keys, the
+        server's lanes standing for its workers.  Keys with no counterpart on
+        a GPU (CPU affinity, device read concurrency) are None."""
+        ld, cfg = res["load"], self.config
+
+        def summary(prefix):
+            if prefix == "sojourn":
+                keys = {"mean_ms": "mean_ms", "sd_ms": "sd_ms", "p50_ms": "p50_ms", "p90_ms": "p90_ms",
+                        "p99_ms": "p99_ms", "p99_9_ms": "p99_9_ms", "max_ms": "max_ms"}
+            elif prefix == "queue_delay":
+                keys = {"mean_ms": "queue_mean_ms", "p50_ms": "queue_p50_ms", "p90_ms": "queue_p90_ms",
+                        "p99_ms": "queue_p99_ms", "max_ms": "queue_max_ms"}
+            else:
+                keys = {"mean_ms": "service_mean_ms", "p50_ms": "service_p50_ms", "p90_ms": "service_p90_ms",
+                        "p99_ms": "service_p99_ms", "max_ms": "service_max_ms"}
+            return {"count": res["completed"], **{k: res[v] for k, v in keys.items()}}
+
+        return {
+            "method": "GPU-MP21", "schema_version": 1,
+            "config": {"arrival": "poisson", "offered_rate_rps": ld["rate"], "workers": cfg["lanes"],
+                       "requests": ld["requests"], "warm": ld["warm"], "users": ld["users"],
+                       "cache_capacity_keys": None, "key_budget_bytes": cfg["key_budget_bytes"],
+                       "zipf_s": ld["zipf_s"], "seed": ld["seed"], "slo_ms": ld["slo_ms"],
+                       "queue_max": cfg["queue_max"], "batch": cfg["max_batch"], "sched": cfg["sched"],
+                       "spill_threshold": cfg["spill_threshold"], "steal_threshold": cfg["steal_threshold"]},
+            "environment": {"affinity_cpus": None, "cpus_available": os.cpu_count()},
+            "throughput_rps": res["throughput_rps"], "offered_rps": res["offered_rps"], "wall_sec": res["wall_sec"],
+            "shed": res["shed"], "shed_rate": res["shed"] / ld["requests"] if ld["requests"] else 0.0,
+            "hit_rate": res["hit_rate"], "coalesced_loads": res["coalesced_loads"], "slo_ms": ld["slo_ms"],
+            "slo_violation_rate": res["slo_violation_rate"],
+            "slo_violation_rate_100": res["slo_violation_rate_100"],
+            "slo_violation_rate_300": res["slo_violation_rate_300"],
+            "slo_violation_rate_500": res["slo_violation_rate_500"],
+            "shed_plus_completed": res["shed"] + res["completed"], "peak_queue_depth": res["peak_queue_depth"],
+            "load_concurrency_peak": None, "load_concurrency_mean": None, "device": "hbm",
+            "device_inflight_reads_peak": None, "device_inflight_reads_mean": None, "device_inflight_samples": 0,
+            "batch": cfg["max_batch"], "sched": cfg["sched"], "per_worker_requests": res["lane_requests"],
+            "tenant_fairness_jain": res["tenant_fairness_jain"], "locality": res["locality"],
+            "spilled": res["spilled"], "stolen": res["stolen"], "spill_threshold": cfg["spill_threshold"],
+            "sojourn": summary("sojourn"), "queue_delay": summary("queue_delay"), "service": summary("service"),
+            "mean_batch": res["mean_batch"], "key_loads": res["key_loads"], "bytes_loaded": res["bytes_loaded"],
+            "mismatches": res["mismatches"],
+        }
 
 
 __all__ = [
@@ -826,5 +913,5 @@ __all__ = [
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
     "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
     "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
-    "key_file_s2_level", "load_succinct_arrays", "keygen_classic",
+    "key_file_s2_level", "load_succinct_arrays", "keygen_classic", "SCHED_MODES", "abi_struct_mismatches",
 ]

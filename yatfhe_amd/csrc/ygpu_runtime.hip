@@ -14,6 +14,15 @@ extern "C" {
 
 const char* ygpu_last_error(void) { return g_err.c_str(); }
 
+int ygpu_abi_struct_sizes(int64_t* out, int32_t max) {
+    const int64_t sizes[] = {sizeof(ygpu_params), sizeof(ygpu_job), sizeof(ygpu_server_config), sizeof(ygpu_request),
+                             sizeof(ygpu_server_stats), sizeof(ygpu_loadgen_config), sizeof(ygpu_loadgen_result)};
+    const int n = static_cast<int>(sizeof(sizes) / sizeof(sizes[0]));
+    if (out)
+        for (int i = 0; i < n && i < max; i++) out[i] = sizes[i];
+    return n;
+}
+
 int ygpu_device_count(int32_t* count) {
     if (!count) return fail(YGPU_ERR_INVALID_ARG, "null count");
     int c = 0;

@@ -30,6 +30,10 @@
 //    load completes.
 //  * A completion thread per parameter set retires lanes in launch order,
 //    copies results to the callers' buffers and wakes waiters.
+//  * Which lane batches which requests follows the reference Scheduler's
+//    modes (blindrotate_serving.h:132-273), lanes standing for its workers:
+//    shared (any lane, any tenant), affinity (tenant t on lane t % lanes) and
+//    soft (affinity with spill on admission and stealing by idle lanes).
 //
 // The PBS itself is a deterministic function of (key, ciphertext, test
 // polynomial), and the batched kernels compute every ciphertext independently,
@@ -38,6 +42,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -79,6 +84,7 @@ struct Tenant {
     int64_t freq_ns = 0;   // when freq was last brought up to date
     size_t load_bytes = 0; // host-to-device bytes of one load
     std::deque<ygpu_request*> pending;
+    std::deque<int> pending_lane;  // per pending request: the lane it is queued on (-1: any)
     bool listed = false;   // in its parameter set's `order`
 };
 
@@ -119,6 +125,7 @@ struct PSet {
     std::vector<Lane> lanes;
     std::deque<int> order;            // tenants with pending requests, oldest first
     int64_t queued = 0;               // pending requests
+    std::vector<int64_t> lane_depth;  // pending requests queued on each lane (affinity, soft)
     hipStream_t copy_stream = nullptr;
     void* d_stage = nullptr;          // packed BSK of the load being expanded (either form)
     std::deque<int> loads;            // tenants being loaded, in copy order
@@ -407,19 +414,96 @@ void fail_pending(ygpu_server* s, PSet& ps, int t, int rc) {
     Tenant& tn = s->tenants[t];
     tn.fresh = false;
     for (ygpu_request* r : tn.pending) finish_request(r, rc, now, now, 0);
+    for (int l : tn.pending_lane)
+        if (l >= 0) ps.lane_depth[l]--;
     ps.queued -= static_cast<int64_t>(tn.pending.size());
     s->st.failed += static_cast<int64_t>(tn.pending.size());
     tn.pending.clear();
+    tn.pending_lane.clear();
+}
+
+// Queue lane of a new request of tenant t (server lock held): -1 under SHARED,
+// else the owner lane t % lanes, or under SOFT the shortest lane when the
+// owner already holds spill_threshold requests and the shortest is shorter
+// by more than one (Scheduler::push, blindrotate_serving.h:154-175).
+int queue_lane(ygpu_server* s, PSet& ps, int t) {
+    if (s->cfg.sched == YGPU_SCHED_SHARED) return -1;
+    const int nl = static_cast<int>(ps.lanes.size());
+    int lane = t % nl;
+    if (s->cfg.sched == YGPU_SCHED_SOFT && ps.lane_depth[lane] >= s->cfg.spill_threshold) {
+        int alt = 0;
+        for (int l = 1; l < nl; l++)
+            if (ps.lane_depth[l] < ps.lane_depth[alt]) alt = l;
+        if (ps.lane_depth[alt] + 1 < ps.lane_depth[lane]) {
+            lane = alt;
+            s->st.spilled++;
+        }
+    }
+    return lane;
+}
+
+// Fills lane L's batch from the ready tenants, oldest first (server lock
+// held): every pending request when `want` < 0, else the requests queued on
+// lane `want`; at most `cap`.  Tenants left with requests stay listed.
+void form_batch(ygpu_server* s, PSet& ps, Lane& L, int want, int cap) {
+    L.reqs.clear();
+    L.tenants.clear();
+    std::deque<int> keep;
+    for (int t : ps.order) {
+        Tenant& tn = s->tenants[t];
+        if (tn.pending.empty()) {
+            tn.listed = false;
+            continue;
+        }
+        const int room = cap - static_cast<int>(L.reqs.size());
+        if (!tn.ready || room == 0) {
+            keep.push_back(t);
+            continue;
+        }
+        int took = 0;
+        if (want < 0) {
+            took = std::min<int>(room, static_cast<int>(tn.pending.size()));
+            for (int c = 0; c < took; c++) {
+                L.reqs.push_back(tn.pending.front());
+                if (tn.pending_lane.front() >= 0) ps.lane_depth[tn.pending_lane.front()]--;
+                tn.pending.pop_front();
+                tn.pending_lane.pop_front();
+            }
+        } else {
+            std::deque<ygpu_request*> restR;
+            std::deque<int> restL;
+            while (!tn.pending.empty()) {
+                ygpu_request* r = tn.pending.front();
+                const int l = tn.pending_lane.front();
+                tn.pending.pop_front();
+                tn.pending_lane.pop_front();
+                if (l == want && took < room) {
+                    L.reqs.push_back(r);
+                    ps.lane_depth[l]--;
+                    took++;
+                } else {
+                    restR.push_back(r);
+                    restL.push_back(l);
+                }
+            }
+            tn.pending.swap(restR);
+            tn.pending_lane.swap(restL);
+        }
+        if (took) {
+            tn.inuse++;
+            tn.fresh = false;
+            tn.last_use = ++s->tick;
+            L.tenants.push_back(t);
+        }
+        if (tn.pending.empty()) tn.listed = false;
+        else keep.push_back(t);
+    }
+    ps.order.swap(keep);
 }
 
 void dispatcher_loop(ygpu_server* s, PSet* ps) {
     hipSetDevice(s->cfg.device);
     std::unique_lock<std::mutex> lk(s->mu);
-    auto free_lane = [&]() -> int {
-        for (size_t i = 0; i < ps->lanes.size(); i++)
-            if (!ps->lanes[i].busy) return static_cast<int>(i);
-        return -1;
-    };
     while (!s->stop) {
         if (ps->order.empty()) {
             ps->cv_work.wait(lk);
@@ -437,51 +521,48 @@ void dispatcher_loop(ygpu_server* s, PSet* ps) {
             }
             if (rc < 0) fail_pending(s, *ps, t, rc);
         }
-        // 2. a batch of every ready tenant's requests, oldest tenants first
-        const int li = free_lane();
-        Lane* L = li >= 0 ? &ps->lanes[li] : nullptr;
-        if (L) {
-            L->reqs.clear();
-            L->tenants.clear();
-            std::deque<int> keep;
-            for (int t : ps->order) {
-                Tenant& tn = s->tenants[t];
-                const int room = s->cfg.max_batch - static_cast<int>(L->reqs.size());
-                if (tn.pending.empty()) {
-                    tn.listed = false;
-                    continue;
+        // 2. a batch for a free lane: under SHARED the requests of every
+        // ready tenant, oldest tenants first; under AFFINITY and SOFT those
+        // queued on that lane, and under SOFT a lane with none of its own
+        // takes the longest lane's once it holds steal_threshold requests
+        // (Scheduler::pop, blindrotate_serving.h:177-215)
+        // Under SHARED one lane takes at most a lane's share of the queue (and
+        // never less than a ciphertext per compute unit), so a backlog is
+        // spread over the lanes instead of one batch holding it all while the
+        // other lanes wait for new arrivals.
+        bool launched = false;
+        const int nl = static_cast<int>(ps->lanes.size());
+        const int share = static_cast<int>(std::min<int64_t>(
+            s->cfg.max_batch, std::max<int64_t>(ps->ctx->cu_count, (ps->queued + nl - 1) / nl)));
+        for (int li = 0; li < nl && !launched; li++) {
+            Lane* L = &ps->lanes[li];
+            if (L->busy) continue;
+            const int own = s->cfg.sched == YGPU_SCHED_SHARED ? -1 : li;
+            form_batch(s, *ps, *L, own, own < 0 ? share : s->cfg.max_batch);
+            if (L->reqs.empty() && s->cfg.sched == YGPU_SCHED_SOFT) {
+                int lm = -1;
+                for (int l = 0; l < nl; l++)
+                    if (ps->lane_depth[l] > 0 && (lm < 0 || ps->lane_depth[l] > ps->lane_depth[lm])) lm = l;
+                if (lm >= 0 && lm != li && ps->lane_depth[lm] >= s->cfg.steal_threshold) {
+                    form_batch(s, *ps, *L, lm, s->cfg.max_batch);
+                    s->st.stolen += static_cast<int64_t>(L->reqs.size());
                 }
-                if (!tn.ready || room == 0) {
-                    keep.push_back(t);
-                    continue;
-                }
-                const int take = std::min<int>(room, static_cast<int>(tn.pending.size()));
-                for (int c = 0; c < take; c++) {
-                    L->reqs.push_back(tn.pending.front());
-                    tn.pending.pop_front();
-                }
-                tn.inuse++;
-                tn.fresh = false;
-                tn.last_use = ++s->tick;
-                L->tenants.push_back(t);
-                if (tn.pending.empty()) tn.listed = false;
-                else keep.push_back(t);
             }
-            ps->order.swap(keep);
-            if (!L->reqs.empty()) {
-                ps->queued -= static_cast<int64_t>(L->reqs.size());
-                L->busy = true;
-                L->launch_ns = now_ns();
-                s->st.batches++;
-                s->st.batched_cts += static_cast<int64_t>(L->reqs.size());
-                lk.unlock();
-                L->rc = launch_lane(s, *ps, *L);
-                lk.lock();
-                ps->inflight.push_back(li);
-                ps->cv_done.notify_all();
-                continue;  // look for more work at once
-            }
+            if (L->reqs.empty()) continue;
+            for (ygpu_request* r : L->reqs) r->lane = li;
+            ps->queued -= static_cast<int64_t>(L->reqs.size());
+            L->busy = true;
+            L->launch_ns = now_ns();
+            s->st.batches++;
+            s->st.batched_cts += static_cast<int64_t>(L->reqs.size());
+            lk.unlock();
+            L->rc = launch_lane(s, *ps, *L);
+            lk.lock();
+            ps->inflight.push_back(li);
+            ps->cv_done.notify_all();
+            launched = true;
         }
+        if (launched) continue;  // look for more work at once
         if (mem_blocked) {
             s->st.memory_waits++;
             if (!any_busy(s)) {
@@ -610,6 +691,10 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
     if (s->cfg.max_loads <= 0) s->cfg.max_loads = 4;
     if (s->cfg.eviction != YGPU_EVICT_FREQ_BYTES && s->cfg.eviction != YGPU_EVICT_LRU)
         return fail(YGPU_ERR_INVALID_ARG, "unknown eviction policy");
+    if (s->cfg.sched != YGPU_SCHED_SHARED && s->cfg.sched != YGPU_SCHED_AFFINITY && s->cfg.sched != YGPU_SCHED_SOFT)
+        return fail(YGPU_ERR_INVALID_ARG, "unknown scheduling mode");
+    if (s->cfg.spill_threshold <= 0) s->cfg.spill_threshold = 2;
+    if (s->cfg.steal_threshold <= 0) s->cfg.steal_threshold = 2;
     if (s->cfg.lanes > 16 || s->cfg.max_batch > (1 << 16) || s->cfg.max_tenants > (1 << 20))
         return fail(YGPU_ERR_INVALID_ARG, "lanes <= 16, max_batch <= 65536 and max_tenants <= 2^20 required");
     YG_HIP(hipSetDevice(cfg->device));
@@ -657,6 +742,7 @@ int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* p
             return fail(YGPU_ERR_OUT_OF_MEMORY, "key staging buffer");
         }
         ps->lanes.resize(s->cfg.lanes);
+        ps->lane_depth.assign(s->cfg.lanes, 0);
         for (Lane& L : ps->lanes) {
             rc = lane_alloc(s, *ps, L);
             if (rc) {
@@ -819,14 +905,22 @@ int ygpu_server_submit(ygpu_server* s, ygpu_request* req) {
         if (req->arrival_ns == 0) req->arrival_ns = now_ns();
         req->seq = ++s->tick;
         req->key_hit = tn.ready ? 1 : 0;
+        // absent key already awaited (loading, or an earlier request queued for it)
+        req->coalesced = (!tn.ready && (tn.resident || !tn.pending.empty())) ? 1 : 0;
+        req->lane = -1;
+        s->st.coalesced += req->coalesced;
         {
             const int64_t now = now_ns();
             tn.freq = decayed_freq(tn, now) + 1.0;
             tn.freq_ns = now;
         }
         s->st.request_hits += req->key_hit;
+        const int ql = queue_lane(s, ps, req->tenant);
         tn.pending.push_back(req);
+        tn.pending_lane.push_back(ql);
         ps.queued++;
+        if (ql >= 0) ps.lane_depth[ql]++;
+        s->st.peak_queue_depth = std::max<int64_t>(s->st.peak_queue_depth, ql >= 0 ? ps.lane_depth[ql] : ps.queued);
         if (!tn.listed) {
             tn.listed = true;
             ps.order.push_back(req->tenant);
@@ -884,6 +978,7 @@ int ygpu_server_destroy(ygpu_server* s) {
             for (int tn : ps->order) {
                 for (ygpu_request* r : s->tenants[tn].pending) finish_request(r, YGPU_ERR_INVALID_ARG, t, t, 0);
                 s->tenants[tn].pending.clear();
+                s->tenants[tn].pending_lane.clear();
             }
             ps->order.clear();
         }
@@ -981,6 +1076,11 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
                 grave.push_back(outsHold);
                 return fail(YGPU_ERR_BUSY, "load generator: warm-up requests still pending after 120 s");
             }
+        {
+            // the queue-depth peak is per measured run
+            std::lock_guard<std::mutex> g(s->mu);
+            s->st.peak_queue_depth = 0;
+        }
         ygpu_server_stats s0{};
         ygpu_server_stats_get(s, &s0);
         // measured open loop
@@ -1006,11 +1106,16 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             else if (rc) return rc;
             else submitted[i] = 1;
         }
-        int64_t errors = 0, mismatches = 0, done = 0;
+        int64_t errors = 0, mismatches = 0, done = 0, coalesced = 0, owned = 0;
         int64_t t_end = t0;
-        std::vector<double> lat, queue, latHit, latMiss, latPset[4];
+        std::vector<double> lat, queue, service, latHit, latMiss, latPset[4];
         lat.reserve(c->requests);
         queue.reserve(c->requests);
+        service.reserve(c->requests);
+        std::unordered_map<int, std::pair<double, int64_t>> perTenant;
+        int64_t laneCount[16] = {};
+        const int lanes = s->cfg.lanes;
+        const bool affine = s->cfg.sched != YGPU_SCHED_SHARED;
         // a run that has not drained 120 s after its last arrival is stuck
         const int64_t deadline = now_ns() + int64_t{120} * 1000000000;
         for (int64_t i = c->warm; i < total; i++) {
@@ -1040,6 +1145,14 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             (reqs[i].key_hit ? latHit : latMiss).push_back(lat.back());
             if (psetOf[reqs[i].tenant] < 4) latPset[psetOf[reqs[i].tenant]].push_back(lat.back());
             queue.push_back(std::max<int64_t>(0, reqs[i].launch_ns - reqs[i].arrival_ns) * 1e-6);
+            service.push_back(std::max<int64_t>(0, reqs[i].done_ns - reqs[i].launch_ns) * 1e-6);
+            auto& pt = perTenant[reqs[i].tenant];
+            pt.first += lat.back();
+            pt.second++;
+            coalesced += reqs[i].coalesced;
+            const int gl = psetOf[reqs[i].tenant] * lanes + reqs[i].lane;
+            if (reqs[i].lane >= 0 && gl < 16) laneCount[gl]++;
+            owned += affine && reqs[i].lane == reqs[i].tenant % lanes;
             if (c->expected && c->expected[reqs[i].tenant] &&
                 std::memcmp(reqs[i].lwe_out, c->expected[reqs[i].tenant], sizeof(int32_t) * cols[reqs[i].tenant]) != 0)
                 mismatches++;
@@ -1095,6 +1208,49 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             out->pset_p50_ms[k] = pct(latPset[k], 50);
             out->pset_p99_ms[k] = pct(latPset[k], 99);
         }
+        // the rest of the reference's record (blindrotate_serving.cpp:60-80, 253-353)
+        double sq = 0;
+        for (double x : lat) sq += (x - out->mean_ms) * (x - out->mean_ms);
+        out->sd_ms = lat.empty() ? 0 : std::sqrt(sq / lat.size());
+        double qsum = 0;
+        for (double x : queue) qsum += x;
+        out->queue_mean_ms = queue.empty() ? 0 : qsum / queue.size();
+        out->queue_p90_ms = pct(queue, 90);
+        out->queue_max_ms = queue.empty() ? 0 : queue.back();
+        std::sort(service.begin(), service.end());
+        double ssum = 0;
+        for (double x : service) ssum += x;
+        out->service_mean_ms = service.empty() ? 0 : ssum / service.size();
+        out->service_p50_ms = pct(service, 50);
+        out->service_p90_ms = pct(service, 90);
+        out->service_p99_ms = pct(service, 99);
+        out->service_max_ms = service.empty() ? 0 : service.back();
+        int64_t o100 = 0, o300 = 0, o500 = 0;
+        for (double x : lat) {
+            o100 += x > 100.0;
+            o300 += x > 300.0;
+            o500 += x > 500.0;
+        }
+        out->slo_violation_rate_100 = lat.empty() ? 0 : static_cast<double>(o100) / lat.size();
+        out->slo_violation_rate_300 = lat.empty() ? 0 : static_cast<double>(o300) / lat.size();
+        out->slo_violation_rate_500 = lat.empty() ? 0 : static_cast<double>(o500) / lat.size();
+        out->coalesced_loads = coalesced;
+        out->peak_queue_depth = s1.peak_queue_depth;
+        out->spilled = s1.spilled - s0.spilled;
+        out->stolen = s1.stolen - s0.stolen;
+        double fair = 1.0;
+        if (perTenant.size() > 1) {
+            double sm = 0, sm2 = 0;
+            for (const auto& kv : perTenant) {
+                const double m = kv.second.first / kv.second.second;
+                sm += m;
+                sm2 += m * m;
+            }
+            fair = sm * sm / (perTenant.size() * sm2);
+        }
+        out->tenant_fairness_jain = fair;
+        out->locality = done ? static_cast<double>(owned) / done : 0;
+        for (int k = 0; k < 16; k++) out->lane_requests[k] = laneCount[k];
         return YGPU_OK;
     });
 }
