@@ -253,8 +253,15 @@ inline BrArgs br_args(ygpu_ctx* ctx, int count, const int32_t* d_lwe, const void
     return a;
 }
 
+// Bytes of the keyswitch workspace (sliced partial sums + packed digits) of
+// `count` ciphertexts.
+inline size_t ks_workspace_bytes(const ygpu_ctx* ctx, int count) {
+    return sizeof(uint32_t) * ks_workspace_words(ctx->p.n, ctx->p.k, ctx->p.N, ctx->p.ks_level, count);
+}
+
+// `workspace` holds ks_workspace_bytes(count) (null: extraction only).
 inline KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
-               int32_t* partial) {
+               int32_t* workspace) {
     KsArgs a{};
     a.n = ctx->p.n;
     a.N = ctx->p.N;
@@ -267,7 +274,9 @@ inline KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t
     a.ksk_table = table;
     a.ct_slot = slots;
     a.lwe_out = d_out;
-    a.partial = partial;
+    a.partial = workspace;
+    a.digits = workspace ? reinterpret_cast<uint32_t*>(workspace) + (static_cast<size_t>(count) * (ctx->p.n + 1) + 63) / 64 * 64
+                         : nullptr;
     return a;
 }
 
@@ -279,19 +288,18 @@ inline int ks_slices(const ygpu_ctx* ctx, int count) {
     return slices;
 }
 
-// Extract + keyswitch of `count` accumulators.  `partial` (count x (n+1)
-// words) is needed when the rows are sliced; null selects the context's buffer.
+// Extract + keyswitch of `count` accumulators.  `workspace` holds
+// ks_workspace_bytes(count); null selects the context's buffer.
 inline int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
-           int32_t* d_ext, int32_t* partial = nullptr, hipStream_t stream = nullptr) {
+           int32_t* d_ext, int32_t* workspace = nullptr, hipStream_t stream = nullptr) {
     hipStream_t st = stream ? stream : ctx->stream;
     const int slices = ks_slices(ctx, count);
-    if (slices > 1 && !partial) {
-        int rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap,
-                        sizeof(int32_t) * static_cast<size_t>(count) * (ctx->p.n + 1));
+    if (!workspace) {
+        int rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap, ks_workspace_bytes(ctx, count));
         if (rc) return rc;
-        partial = ctx->d_partial;
+        workspace = ctx->d_partial;
     }
-    KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out, partial);
+    KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out, workspace);
     YG_HIP(timed(ctx, "k_extract_ks", st, [&] { return launch_extract_ks(ctx->tbits, a, slices, st); }));
     if (d_ext) YG_HIP(timed(ctx, "k_extract", st, [&] { return launch_extract(ctx->tbits, a, d_ext, st); }));
     return YGPU_OK;

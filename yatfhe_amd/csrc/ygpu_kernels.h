@@ -78,7 +78,16 @@ struct KsArgs {
     const int32_t* ct_slot;
     int32_t* lwe_out;                  // [count][n+1]
     int32_t* partial;                  // [count][n+1] (slices > 1)
+    uint32_t* digits;                  // [groups][count]: the keyswitch digits of four rows per word
 };
+
+// Words of the keyswitch workspace of `count` ciphertexts: the sliced partial
+// sums ([count][n+1]) and the packed digits ([(k N t + 3) / 4][count]).
+inline size_t ks_workspace_words(int n, int k, int N, int ks_level, int count) {
+    const size_t partial = static_cast<size_t>(count) * (n + 1);
+    const size_t groups = (static_cast<size_t>(k) * N * ks_level + 3) / 4;
+    return (partial + 63) / 64 * 64 + groups * static_cast<size_t>(count);
+}
 
 // ---------------------------------------------------------------- helpers
 // Workgroup b runs on XCD b % 8 (round-robin dispatch); map logical work so each

@@ -276,6 +276,38 @@ __device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base
     return torus_to_lwe32<TBITS>(x);
 }
 
+// Keyswitch digits of the extracted samples (ks_digits of every coefficient
+// a'_i, b' excluded): word [g][ct] packs the signed 8-bit digits of rows
+// 4g .. 4g + 3 (row = i t + j: level j of coefficient i).  One thread per
+// word; a coefficient spanning two words is decomposed for each.
+template <int TBITS>
+__global__ void __launch_bounds__(256) k_ks_digits(KsArgs a) {
+    const int N = a.N, t = a.ks_level;
+    const int rows = a.k * N * t;
+    const int groups = (rows + 3) / 4;
+    const size_t total = static_cast<size_t>(groups) * a.count;
+    for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
+         idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+        const int g = static_cast<int>(idx / a.count), ct = static_cast<int>(idx % a.count);
+        const size_t accBase = static_cast<size_t>(ct) * (a.k + 1) * N;
+        int8_t dig[KS_MAX_LEVEL];
+        int lastI = -1;
+        uint32_t w = 0;
+        for (int r = 0; r < 4; r++) {
+            const int row = g * 4 + r;
+            if (row >= rows) break;
+            const int i = row / t, j = row % t;
+            if (i != lastI) {
+                const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N);
+                ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dig);
+                lastI = i;
+            }
+            w |= (static_cast<uint32_t>(dig[j]) & 0xFFu) << (8 * r);
+        }
+        a.digits[idx] = w;
+    }
+}
+
 template <int TBITS>
 __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
     __shared__ uint32_t sk[KS_RG][4][KS_TK];   // KSK planes: [group][plane][col]
@@ -322,30 +354,13 @@ __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
                 const int gg = gb + g, col = col0 + k;
                 sk[g][b][k] = (gg < g1 && col < cols) ? planes[(static_cast<size_t>(gg) * 4 + b) * cols + col] : 0u;
             }
-            // digits of this chunk: one (ct, group) word per entry; the digits
-            // of one extracted coefficient are computed once for the rows it
-            // spans; ciphertexts outside the run get zero digits
+            // digits of this chunk (computed once per batch by k_ks_digits,
+            // group-major so these loads are coalesced along ciphertexts);
+            // ciphertexts outside the run get zero digits
             for (int e = tid; e < KS_RG * KS_TC; e += KS_THREADS) {
                 const int c = e % KS_TC, g = e / KS_TC;
                 const int gg = gb + g, ct = ct0 + c;
-                uint32_t w = 0;
-                if (gg < g1 && ct >= segStart && ct < segEnd) {
-                    const size_t accBase = static_cast<size_t>(ct) * (a.k + 1) * N;
-                    int8_t dig[KS_MAX_LEVEL];
-                    int lastI = -1;
-                    for (int r = 0; r < 4; r++) {
-                        const int row = gg * 4 + r;
-                        if (row >= rows) break;
-                        const int i = row / t, j = row % t;
-                        if (i != lastI) {
-                            const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N);
-                            ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dig);
-                            lastI = i;
-                        }
-                        w |= (static_cast<uint32_t>(dig[j]) & 0xFFu) << (8 * r);
-                    }
-                }
-                sd[g][c] = w;
+                sd[g][c] = (gg < g1 && ct >= segStart && ct < segEnd) ? a.digits[static_cast<size_t>(gg) * a.count + ct] : 0u;
             }
             __syncthreads();
             const int gEnd = min(KS_RG, g1 - gb);
@@ -554,7 +569,14 @@ int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* o
 }
 
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s) {
+    if (!a.digits) return hipErrorInvalidValue;
     const dim3 grid((a.count + KS_TC - 1) / KS_TC, (a.n + 1 + KS_TK - 1) / KS_TK, slices);
+    {
+        const size_t words = (static_cast<size_t>(a.k) * a.N * a.ks_level + 3) / 4 * a.count;
+        const unsigned blocks = static_cast<unsigned>(std::min<size_t>((words + 255) / 256, 65536));
+        if (tbits == 32) hipLaunchKernelGGL(k_ks_digits<32>, dim3(blocks), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(k_ks_digits<56>, dim3(blocks), dim3(256), 0, s, a);
+    }
     if (slices > 1) {
         const size_t total = static_cast<size_t>(a.count) * (a.n + 1);
         hipLaunchKernelGGL(k_zero_u32, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s,

@@ -244,7 +244,7 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
             if (rc) return rc;
             acc = ctx->d_acc;
         }
-        rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap, sizeof(int32_t) * static_cast<size_t>(total) * (ctx->p.n + 1));
+        rc = ensure(reinterpret_cast<void**>(&ctx->d_partial), &ctx->partial_cap, ks_workspace_bytes(ctx, static_cast<int>(total)));
         if (rc) return rc;
         rc = enqueue_jobs(ctx, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc, ctx->d_slots, ctx->d_partial);
         if (rc) return rc;
@@ -594,7 +594,7 @@ int ygpu_graph_create(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64
         const ygpu_params& p = ctx->p;
         YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_slots), sizeof(int32_t) * slots.size()));
         YG_HIP(hipMemcpy(g.d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice));
-        YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_partial), sizeof(int32_t) * static_cast<size_t>(total) * (p.n + 1)));
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_partial), ks_workspace_bytes(ctx, static_cast<int>(total))));
         void* acc = d_acc_out;
         if (!acc) {
             YG_HIP(hipMalloc(&g.d_acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(total)));

@@ -345,7 +345,7 @@ int lane_alloc(const ygpu_server* s, PSet& ps, Lane& L) {
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_out), B * lwe));
     YG_HIP(hipMalloc(&L.d_acc, B * ps.tbytes * (p.k + 1) * p.N));
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_slots), B * sizeof(int32_t)));
-    YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_partial), B * lwe));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_partial), ks_workspace_bytes(ps.ctx, static_cast<int>(B))));
     return YGPU_OK;
 }
 
