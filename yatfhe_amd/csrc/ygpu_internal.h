@@ -476,22 +476,45 @@ inline int64_t pack_residues(const uint64_t* src, size_t polys, const ygpu_param
 
 // KSK as signed-byte planes: rows grouped by four, word = the four rows' byte b
 // of the same column; x = sum_b s_b 2^(8b) (mod 2^32), s_b signed.
+// KSK [row][col] (int32) -> four signed-byte planes, x = sum_b s_b 2^(8b)
+// mod 2^32 (s_b = byte b of x plus the carry out of byte b - 1, taken in
+// [-128, 127]), laid out [plane][col][group] with four consecutive rows of a
+// group packed one byte each (the keyswitch's MFMA B fragments are 16
+// contiguous bytes).  The transpose runs on a few host threads, each taking
+// a range of columns; blocks of 64 groups keep reads and writes in cache.
 inline void ksk_planes(const ygpu_params& p, const int32_t* ksk, std::vector<uint32_t>& planes) {
     const size_t cols = static_cast<size_t>(p.n) + 1;
     const size_t kskRows = static_cast<size_t>(p.k) * p.N * p.ks_level;
     const size_t groups = (kskRows + 3) / 4;
     planes.assign(groups * 4 * cols, 0u);
-    for (size_t row = 0; row < kskRows; row++) {
-        const size_t g = row / 4, r = row % 4;
-        for (size_t c = 0; c < cols; c++) {
-            int64_t x = ksk[row * cols + c];
-            for (int b = 0; b < 4; b++) {
-                const int8_t sb = static_cast<int8_t>(static_cast<uint8_t>(x & 0xFF));
-                planes[(g * 4 + b) * cols + c] |= (static_cast<uint32_t>(static_cast<uint8_t>(sb))) << (8 * r);
-                x = (x - sb) >> 8;
+    uint32_t* out = planes.data();
+    auto work = [&](size_t cLo, size_t cHi) {
+        constexpr size_t GB = 64;
+        for (size_t g0 = 0; g0 < groups; g0 += GB) {
+            const size_t g1 = std::min(groups, g0 + GB);
+            for (size_t c = cLo; c < cHi; c++) {
+                uint32_t* o[4];
+                for (int b = 0; b < 4; b++) o[b] = out + (b * cols + c) * groups;
+                for (size_t g = g0; g < g1; g++) {
+                    uint32_t w[4] = {0, 0, 0, 0};
+                    for (size_t r = 0; r < 4 && g * 4 + r < kskRows; r++) {
+                        uint32_t x = static_cast<uint32_t>(ksk[(g * 4 + r) * cols + c]);
+                        for (int b = 0; b < 4; b++) {
+                            const uint32_t byte = x & 0xFFu;
+                            w[b] |= byte << (8 * r);
+                            x = (x >> 8) + (byte >> 7);  // (x - s_b) / 256 with s_b = byte - 256 [byte >= 128]
+                        }
+                    }
+                    for (int b = 0; b < 4; b++) o[b][g] = w[b];
+                }
             }
         }
-    }
+    };
+    const size_t nt = std::min<size_t>(8, std::max<size_t>(1, std::min<size_t>(std::thread::hardware_concurrency(), cols / 64)));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(work, cols * t / nt, cols * (t + 1) / nt);
+    work(0, cols / nt);
+    for (auto& th : pool) th.join();
 }
 
 inline int prepare_key_host(ygpu_ctx* ctx, const uint64_t* bsk_coeff, const int32_t* ksk, HostKeyForm& out) {

@@ -67,7 +67,6 @@ struct BrArgs {
 constexpr int KS_THREADS = 256;
 constexpr int KS_TC = 64;    // ciphertexts per keyswitch tile
 constexpr int KS_TK = 64;    // output columns per keyswitch tile
-constexpr int KS_RG = 16;    // row groups (4 KSK rows each) per LDS chunk
 constexpr int KS_MAX_LEVEL = 16;
 constexpr int KS_MAX_N = 1023;
 

@@ -261,13 +261,14 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 //   out.a[k] = -sum_{i,j} d_{i,j} KSK[i][j].a[k],  out.b = b' - sum d KSK[i][j].b
 // (mod 2^32; the reference's int64 sum + longModP is the same residue).
 //
-// Shaped as an integer GEMM: digits D[ct][row] (row = i*t + j) times KSK[row][col].
-// The KSK is stored as signed-byte planes: x = sum_b s_b 2^(8b) mod 2^32 with
-// s_b in [-128, 127], four consecutive rows of one plane packed per word, so
-// one v_dot4_i32_i8 performs four exact MACs of one plane.  Tile: KS_TC
-// ciphertexts x KS_TK columns per workgroup, 4 x 4 outputs per thread; row
-// groups stream through LDS in chunks of KS_RG; grid.z slices the rows for
-// small batches (partial sums then meet in an atomic add, exact mod 2^32).
+// An integer GEMM on the matrix cores: digits D[ct][row] (row = i*t + j,
+// signed 8-bit) times KSK[row][col].  The KSK is stored as four signed-byte
+// planes, x = sum_b s_b 2^(8b) mod 2^32 with s_b in [-128, 127], so each plane
+// is an int8 GEMM with exact int32 sums (|sum| <= rows * 2^(b_ks-1) * 2^7,
+// 2^22 at most here) and the planes recombine mod 2^32.  Both operands are
+// laid out so that one MFMA fragment (16 consecutive rows of one ciphertext
+// or one column) is 16 contiguous bytes: digits [ct][group], planes
+// [plane][col][group], a group being four rows packed one byte each.
 template <int TBITS>
 __device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base, int N, int j) {
     using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
@@ -277,7 +278,7 @@ __device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base
 }
 
 // Keyswitch digits of the extracted samples (ks_digits of every coefficient
-// a'_i, b' excluded): word [g][ct] packs the signed 8-bit digits of rows
+// a'_i, b' excluded): word [ct][g] packs the signed 8-bit digits of rows
 // 4g .. 4g + 3 (row = i t + j: level j of coefficient i).  One thread per
 // word; a coefficient spanning two words is decomposed for each.
 template <int TBITS>
@@ -288,7 +289,7 @@ __global__ void __launch_bounds__(256) k_ks_digits(KsArgs a) {
     const size_t total = static_cast<size_t>(groups) * a.count;
     for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
          idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
-        const int g = static_cast<int>(idx / a.count), ct = static_cast<int>(idx % a.count);
+        const int ct = static_cast<int>(idx / groups), g = static_cast<int>(idx % groups);
         const size_t accBase = static_cast<size_t>(ct) * (a.k + 1) * N;
         int8_t dig[KS_MAX_LEVEL];
         int lastI = -1;
@@ -308,26 +309,40 @@ __global__ void __launch_bounds__(256) k_ks_digits(KsArgs a) {
     }
 }
 
+typedef int ks_v4i __attribute__((ext_vector_type(4)));
+
+// 16 consecutive rows (four groups) of one ciphertext or one column, or
+// zeros outside the matrix.
+__device__ __forceinline__ ks_v4i ks_fragment(const uint32_t* __restrict__ base, bool valid) {
+    if (!valid) return ks_v4i{0, 0, 0, 0};
+    using GV = const __attribute__((address_space(1))) ks_v4i;
+    return *(GV*)(base);
+}
+
+// Tile: KS_TC ciphertexts x KS_TK columns per workgroup of four waves, each
+// wave 32 x 32 as 2 x 2 tiles of v_mfma_i32_16x16x64_i8 per plane.  Lane l
+// supplies rows 16 (l / 16) .. + 16 of the 64-row step for ciphertext (A) or
+// column (B) l % 16, and holds outputs rows 4 (l / 16) + v, column l % 16.
+// grid.z slices the rows for small batches (partial sums meet in an atomic
+// add, exact mod 2^32).  A tile's ciphertexts are processed in runs of one
+// key (a key-grouped batch has contiguous jobs, so most tiles hold one run).
 template <int TBITS>
 __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
-    __shared__ uint32_t sk[KS_RG][4][KS_TK];   // KSK planes: [group][plane][col]
-    __shared__ uint32_t sd[KS_RG][KS_TC];      // packed digits: [group][ct]
-    const int n = a.n, N = a.N, t = a.ks_level, cols = n + 1;
-    const int rows = a.k * N * t;
-    const int groups = (rows + 3) / 4;
+    const int n = a.n, N = a.N, cols = n + 1;
+    const int groups = (a.k * N * a.ks_level + 3) / 4;
+    const int steps = (groups + 15) / 16;  // 64-row MFMA steps
     const int ct0 = static_cast<int>(blockIdx.x) * KS_TC;
     const int ctEnd = min(ct0 + KS_TC, a.count);
     const int col0 = static_cast<int>(blockIdx.y) * KS_TK;
     const int slices = static_cast<int>(gridDim.z);
-    const int per = ((groups + slices - 1) / slices + KS_RG - 1) / KS_RG * KS_RG;
-    const int g0 = static_cast<int>(blockIdx.z) * per;
-    const int g1 = min(groups, g0 + per);
-    const int tid = static_cast<int>(threadIdx.x);
-    const int tx = tid % 16, ty = tid / 16;   // 4 columns x 4 ciphertexts per thread
+    const int per = (steps + slices - 1) / slices;
+    const int s0 = static_cast<int>(blockIdx.z) * per;
+    const int s1 = min(steps, s0 + per);
+    const int lane = static_cast<int>(threadIdx.x) % 64;
+    const int wave = static_cast<int>(threadIdx.x) / 64;
+    const int li = lane % 16, lq = lane / 16;
+    const int ctW = ct0 + (wave / 2) * 32, colW = col0 + (wave % 2) * 32;
 
-    // The tile's ciphertexts are processed in runs of one key (a key-grouped
-    // batch has contiguous jobs, so most tiles hold one run); without a slot
-    // array the whole batch uses key slot 0.
     for (int segStart = ct0; segStart < ctEnd;) {
         const int slot = a.ct_slot ? a.ct_slot[segStart] : 0;
         int segEnd = segStart + 1;
@@ -338,72 +353,71 @@ __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
         }
         const uint32_t* __restrict__ planes = reinterpret_cast<const uint32_t*>(a.ksk_table[slot]);
 
-        int32_t acc[4][4][4];  // [plane][ct][col]
+        ks_v4i acc[4][2][2];
 #pragma unroll
         for (int b = 0; b < 4; b++)
 #pragma unroll
-            for (int c = 0; c < 4; c++)
+            for (int x = 0; x < 2; x++)
 #pragma unroll
-                for (int k = 0; k < 4; k++) acc[b][c][k] = 0;
+                for (int y = 0; y < 2; y++) acc[b][x][y] = ks_v4i{0, 0, 0, 0};
 
-        for (int gb = g0; gb < g1; gb += KS_RG) {
-            __syncthreads();
-            // KSK planes of this chunk (coalesced along columns)
-            for (int e = tid; e < KS_RG * 4 * KS_TK; e += KS_THREADS) {
-                const int k = e % KS_TK, b = (e / KS_TK) % 4, g = e / (4 * KS_TK);
-                const int gg = gb + g, col = col0 + k;
-                sk[g][b][k] = (gg < g1 && col < cols) ? planes[(static_cast<size_t>(gg) * 4 + b) * cols + col] : 0u;
-            }
-            // digits of this chunk (computed once per batch by k_ks_digits,
-            // group-major so these loads are coalesced along ciphertexts);
-            // ciphertexts outside the run get zero digits
-            for (int e = tid; e < KS_RG * KS_TC; e += KS_THREADS) {
-                const int c = e % KS_TC, g = e / KS_TC;
-                const int gg = gb + g, ct = ct0 + c;
-                sd[g][c] = (gg < g1 && ct >= segStart && ct < segEnd) ? a.digits[static_cast<size_t>(gg) * a.count + ct] : 0u;
-            }
-            __syncthreads();
-            const int gEnd = min(KS_RG, g1 - gb);
-            for (int g = 0; g < gEnd; g++) {
-                const uint4 dw = *reinterpret_cast<const uint4*>(&sd[g][ty * 4]);
-                const uint32_t d[4] = {dw.x, dw.y, dw.z, dw.w};
+        // this lane's fragment sources: ciphertexts of the run, columns of the matrix
+        bool aok[2], bok[2];
+        const uint32_t* ap[2];
+        const uint32_t* bp[2];
 #pragma unroll
-                for (int b = 0; b < 4; b++) {
-                    const uint4 kw = *reinterpret_cast<const uint4*>(&sk[g][b][tx * 4]);
-                    const uint32_t kv[4] = {kw.x, kw.y, kw.z, kw.w};
+        for (int x = 0; x < 2; x++) {
+            const int ct = ctW + 16 * x + li;
+            aok[x] = ct >= segStart && ct < segEnd;
+            ap[x] = a.digits + static_cast<size_t>(aok[x] ? ct : 0) * groups + 4 * lq;
+            const int col = colW + 16 * x + li;
+            bok[x] = col < cols;
+            bp[x] = planes + static_cast<size_t>(bok[x] ? col : 0) * groups + 4 * lq;
+        }
+        const size_t planeStride = static_cast<size_t>(cols) * groups;
+        for (int st = s0; st < s1; st++) {
+            const int g = st * 16;
+            const bool inK = g + 4 * lq < groups;  // groups is a multiple of 4
+            ks_v4i fa[2], fb[4][2];
 #pragma unroll
-                    for (int c = 0; c < 4; c++)
+            for (int x = 0; x < 2; x++) fa[x] = ks_fragment(ap[x] + g, aok[x] && inK);
 #pragma unroll
-                        for (int k = 0; k < 4; k++)
-                            acc[b][c][k] = __builtin_amdgcn_sdot4(static_cast<int>(d[c]), static_cast<int>(kv[k]), acc[b][c][k], false);
-                }
-            }
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int y = 0; y < 2; y++) fb[b][y] = ks_fragment(bp[y] + b * planeStride + g, bok[y] && inK);
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+                        acc[b][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa[x], fb[b][y], acc[b][x][y], 0, 0, 0);
         }
 
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int ct = ct0 + ty * 4 + c;
-            if (ct < segStart || ct >= segEnd) continue;
+        for (int x = 0; x < 2; x++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int col = col0 + tx * 4 + k;
-                if (col >= cols) continue;
-                const uint32_t sum = static_cast<uint32_t>(acc[0][c][k]) + (static_cast<uint32_t>(acc[1][c][k]) << 8) +
-                                     (static_cast<uint32_t>(acc[2][c][k]) << 16) + (static_cast<uint32_t>(acc[3][c][k]) << 24);
-                if (slices == 1) {
-                    uint32_t o;
-                    if (col < n) {
-                        o = 0u - sum;
+            for (int y = 0; y < 2; y++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int ct = ctW + 16 * x + 4 * lq + v;
+                    const int col = colW + 16 * y + li;
+                    if (ct < segStart || ct >= segEnd || col >= cols) continue;
+                    const uint32_t sum = static_cast<uint32_t>(acc[0][x][y][v]) + (static_cast<uint32_t>(acc[1][x][y][v]) << 8) +
+                                         (static_cast<uint32_t>(acc[2][x][y][v]) << 16) + (static_cast<uint32_t>(acc[3][x][y][v]) << 24);
+                    if (slices == 1) {
+                        uint32_t o;
+                        if (col < n) {
+                            o = 0u - sum;
+                        } else {
+                            const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0);
+                            o = static_cast<uint32_t>(b) - sum;
+                        }
+                        a.lwe_out[static_cast<size_t>(ct) * cols + col] = static_cast<int32_t>(o);
                     } else {
-                        const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0);
-                        o = static_cast<uint32_t>(b) - sum;
+                        atomicAdd(reinterpret_cast<unsigned int*>(a.partial) + static_cast<size_t>(ct) * cols + col, sum);
                     }
-                    a.lwe_out[static_cast<size_t>(ct) * cols + col] = static_cast<int32_t>(o);
-                } else {
-                    atomicAdd(reinterpret_cast<unsigned int*>(a.partial) + static_cast<size_t>(ct) * cols + col, sum);
                 }
-            }
-        }
         segStart = segEnd;
     }
 }
@@ -569,7 +583,8 @@ int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* o
 }
 
 hipError_t launch_extract_ks(int tbits, const KsArgs& a, int slices, hipStream_t s) {
-    if (!a.digits) return hipErrorInvalidValue;
+    // the MFMA fragments are 16-byte loads of four groups
+    if (!a.digits || (static_cast<size_t>(a.k) * a.N * a.ks_level) % 16 != 0) return hipErrorInvalidValue;
     const dim3 grid((a.count + KS_TC - 1) / KS_TC, (a.n + 1 + KS_TK - 1) / KS_TK, slices);
     {
         const size_t words = (static_cast<size_t>(a.k) * a.N * a.ks_level + 3) / 4 * a.count;
