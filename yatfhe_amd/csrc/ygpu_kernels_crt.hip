@@ -6,8 +6,9 @@
 // (crt_mod_q).  Each prime gets its own teams: a ciphertext is four teams of
 // T = N/16 lanes, team (row r, prime p) holding a copy of accumulator row r,
 // transforming row r's digit polynomials mod p, owning output column r mod p.
-// Per step and per team: digits of row r; l forward transforms and the
-// multiply-accumulate against both BSK columns; the partner-column swap with
+// Per step and per team: digits of half of row r's positions, swapped with
+// the other prime team of row r through LDS (both need all of them); l
+// forward transforms and the multiply-accumulate against both BSK columns; the partner-column swap with
 // team (1-r, p); the inverse transform of column r; then the two prime teams of
 // row r swap half their residues and each reconstructs half the values mod q56
 // and updates them in row r.  The accumulator rows live in LDS, read by both
@@ -99,11 +100,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         validate_rotator(abar, N, aT, wrap);
 
         // (X^a - 1) acc_r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238)
-        // at this lane's pass-0 positions, and its digits
+        // at this lane's pass-0 positions, and its digits: the two prime teams
+        // of row tm each decompose half the positions (registers [8 tp, 8 tp + 8))
+        // and swap the packed digits through their scratch
         uint32_t dg0[16], dg1[16];
         const bool flip = wrap < 0;
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
+        auto decompose = [&](int r) {
             const int j = tl + T * r;
             const int src = (j - aT) & (N - 1);
             const bool negs = (j < aT) != flip;
@@ -117,6 +119,41 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                 dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
                 dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
             }
+        };
+        {
+            uint32_t* mine = reinterpret_cast<uint32_t*>(scr);
+            const uint32_t* theirs = reinterpret_cast<const uint32_t*>(scratch[lc] + (tt ^ 2) * SCR);
+            if (tp == 0) {
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    decompose(r);
+                    mine[r * T + tl] = dg0[r];
+                    mine[N + r * T + tl] = dg1[r];
+                }
+            } else {
+#pragma unroll
+                for (int r = 8; r < 16; r++) {
+                    decompose(r);
+                    mine[r * T + tl] = dg0[r];
+                    mine[N + r * T + tl] = dg1[r];
+                }
+            }
+            __syncthreads();
+            if (tp == 0) {
+#pragma unroll
+                for (int r = 8; r < 16; r++) {
+                    dg0[r] = theirs[r * T + tl];
+                    dg1[r] = theirs[N + r * T + tl];
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    dg0[r] = theirs[r * T + tl];
+                    dg1[r] = theirs[N + r * T + tl];
+                }
+            }
+            // the partner has read this team's half before its transforms reuse the scratch
+            __syncthreads();
         }
 
         const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
