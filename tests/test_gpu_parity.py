@@ -118,6 +118,51 @@ def test_edge_inputs_t32(p32, keys32, ctx32, orc32):
             assert np.array_equal(out[c], o_out), f"keyswitch differs, edge ct {c}"
 
 
+@pytest.fixture(scope="module")
+def orc56(p56, keys56):
+    o = Oracle(p56, keys56)
+    yield o
+    o.close()
+
+
+def test_edge_inputs_t56(p56, keys56, ctx56, orc56):
+    ctx, kid = ctx56
+    cts = edge_ciphertexts(p56, keys56)
+    rng = np.random.default_rng(15)
+    half = 1 << 55
+    v = rng.integers(-half, half, p56.N).astype(np.int64)
+    v[:4] = [-half, half - 1, 0, -1]
+    out, acc, ext = ctx.pbs_batch(kid, cts, v, want_acc=True, want_extract=True)
+    for c in range(len(cts)):
+        o_acc, o_ext, o_out = orc56.pbs(cts[c], v)
+        assert np.array_equal(acc[c], o_acc), f"accumulator differs, edge ct {c}"
+        assert np.array_equal(ext[c], o_ext), f"extract differs, edge ct {c}"
+        assert np.array_equal(out[c], o_out), f"keyswitch differs, edge ct {c}"
+
+
+def test_extract_keyswitch_adversarial_t56(p56, keys56, ctx56, orc56):
+    # the 2^56 -> 2^32 modswitch of the extracted sample rounds half away from
+    # zero: accumulators at exact ties (k 2^24 + 2^23), at the torus extremes
+    # and random
+    ctx, kid = ctx56
+    rng = np.random.default_rng(16)
+    half = 1 << 55
+    for trial in range(4):
+        acc = rng.integers(-half, half, (2, p56.N)).astype(np.int64)
+        if trial == 1:
+            acc = (rng.integers(-(1 << 31), 1 << 31, (2, p56.N)) * (1 << 24) + (1 << 23)).astype(np.int64)
+            acc = ((acc + half) % (1 << 56)) - half
+        if trial == 2:
+            acc[:] = -half
+            acc[:, 1::2] = half - 1
+        if trial == 3:
+            acc[:] = 0
+        ext, out = ctx.extract_keyswitch(kid, acc)
+        o_ext = orc56.extract(acc)
+        assert np.array_equal(ext, o_ext), f"extract differs, trial {trial}"
+        assert np.array_equal(out, orc56.keyswitch(o_ext)), f"keyswitch differs, trial {trial}"
+
+
 def test_extract_keyswitch_adversarial_t32(p32, keys32, ctx32, orc32):
     ctx, kid = ctx32
     rng = np.random.default_rng(6)
