@@ -4,8 +4,10 @@
 // level l): 2 x L waves.  The throughput kernel gives each row one wavefront
 // that runs all L forward transforms and multiply-accumulates in turn.  Here
 // the levels run side by side, and the per-step chain gets shorter:
-//   every wave: (X^a - 1) row r and its digits (all levels; the carry chain
-//               needs them), the forward transform of digit level l, and the
+//   every wave: (X^a - 1) row r and its digits (all levels: the carry chain
+//               needs them) at a third of the row's positions, into LDS;
+//   barrier;
+//   every wave: the forward transform of digit level l of row r, and the
 //               products with the BSK entries of (level l, row r) for both
 //               output columns;
 //   barrier;
@@ -13,10 +15,9 @@
 //               transforms it, centres, reduces to the torus and accumulates
 //               row c (held in LDS);
 //   barrier.
-// This costs the rotation and digit work L times over and leaves most waves
-// idle during the inverse transform, so it is the form for batches too small
-// to fill the GPU (the plain drop-in call, short serving batches), not for
-// throughput.  The arithmetic is the same exact mod-q arithmetic as
+// This leaves most waves idle during the inverse transform, so it is the form
+// for batches too small to fill the GPU (the plain drop-in call, short serving
+// batches), not for throughput.  The arithmetic is the same exact mod-q arithmetic as
 // k_blind_rotate: partial sums differ only in representative, and the inverse
 // transform reduces them, so the results are bit-identical.
 //
@@ -44,6 +45,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     __shared__ double tws[2 * N];
     __shared__ int32_t acc[K1][N];
     __shared__ int32_t abars[KS_MAX_N];  // the ciphertext's rotation amounts, mod-switched once
+    __shared__ uint32_t digs[K1][N];      // packed digits of (X^a - 1) row r, all levels
 
     const int ct = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     if (ct >= a.count) return;  // whole workgroup
@@ -101,20 +103,30 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
         // (X^a - 1) row r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238),
         // digit level lvl of it (gadget_decomposition.h:75-95)
         const bool flip = wrap < 0;
+        // the three waves of row r each decompose a share of its positions
+        // (registers [QB(lvl), QB(lvl + 1)) of the lane layout) into LDS
+        {
+            const int q0 = lvl == 0 ? 0 : (lvl == 1 ? 6 : 11);
+            const int q1 = lvl == 0 ? 6 : (lvl == 1 ? 11 : 16);
+            for (int q = q0; q < q1; q++) {
+                const int j = tl + T * q;
+                const int src = (j - aT) & (N - 1);
+                const bool negs = (j < aT) != flip;
+                const uint32_t av = static_cast<uint32_t>(acc[r][src]);
+                const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(acc[r][j]);
+                uint32_t d0, d1;
+                digits_packed<32, L, LAT_RB>(static_cast<int32_t>(rot), d0, d1);
+                digs[r][j] = d0;
+            }
+        }
+        __syncthreads();
         double x[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            const int j = tl + T * q;
-            const int src = (j - aT) & (N - 1);
-            const bool negs = (j < aT) != flip;
-            const uint32_t av = static_cast<uint32_t>(acc[r][src]);
-            const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(acc[r][j]);
-            uint32_t d0, d1;
-            digits_packed<32, L, LAT_RB>(static_cast<int32_t>(rot), d0, d1);
             // the signed byte of level lvl.  Not __builtin_amdgcn_sbfe: with
             // this wave-uniform (non-constant) offset, hipcc converts its
             // result to double as an unsigned value (v_cvt_f64_u32).
-            x[q] = static_cast<double>(static_cast<int8_t>(d0 >> (8 * lvl)));
+            x[q] = static_cast<double>(static_cast<int8_t>(digs[r][tl + T * q] >> (8 * lvl)));
         }
         ntt_forward<LOGN, 1>(x, scr, tl, tws, pr);
         // products with BSK row (level lvl, input row r): column r and 1 - r
