@@ -100,11 +100,12 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     for (int i = 0; i < n; i++) {
         int aT, wrap;
         validate_rotator(abars[i], N, aT, wrap);
-        // (X^a - 1) row r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238),
-        // digit level lvl of it (gadget_decomposition.h:75-95)
+        // (X^a - 1) row r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238)
+        // and its digits (gadget_decomposition.h:75-95): the three waves of
+        // row r each decompose a share of the positions into LDS, registers
+        // q in [0, 6), [6, 11) and [11, 16) of the lane layout for levels 0,
+        // 1 and 2
         const bool flip = wrap < 0;
-        // the three waves of row r each decompose a share of its positions
-        // (registers [QB(lvl), QB(lvl + 1)) of the lane layout) into LDS
         {
             const int q0 = lvl == 0 ? 0 : (lvl == 1 ? 6 : 11);
             const int q1 = lvl == 0 ? 6 : (lvl == 1 ? 11 : 16);
