@@ -356,6 +356,8 @@ typedef struct ygpu_server_stats {
     int64_t key_loads, key_evictions, bytes_loaded, memory_waits;
     int64_t resident_keys, resident_bytes, peak_resident_bytes, key_budget_bytes, host_key_bytes;
     int64_t coalesced, spilled, stolen, peak_queue_depth;
+    int64_t loads_in_flight, peak_loads_in_flight;  /* key loads (copy + expansion) under way */
+    int64_t load_inflight_ns;                       /* integral over time of loads in flight (ns) */
 } ygpu_server_stats;
 
 int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out);
@@ -424,6 +426,8 @@ typedef struct ygpu_loadgen_result {
     double service_mean_ms, service_p50_ms, service_p90_ms, service_p99_ms, service_max_ms;  /* launch to result */
     double slo_violation_rate_100, slo_violation_rate_300, slo_violation_rate_500;
     int64_t coalesced_loads, peak_queue_depth, spilled, stolen;
+    double load_concurrency_mean;       /* key loads under way, time-averaged over the run */
+    int64_t load_concurrency_peak;
     double tenant_fairness_jain;        /* Jain's index of the per-tenant mean latencies */
     double locality;                    /* share served by the tenant's owner lane (0 under SHARED) */
     int64_t lane_requests[16];          /* requests served, by parameter set * lanes + lane (first 16) */

@@ -130,6 +130,9 @@ def test_loadgen_results_match_and_account(p32, tenants):
         assert res["completed"] + res["shed"] == 600
         assert 0 < res["p50_ms"] <= res["p99_ms"] <= res["max_ms"]
         assert res["key_evictions"] > 0
+        # loads happened during the run: the reference record's load concurrency
+        assert res["load_concurrency_peak"] >= 1 and res["load_concurrency_mean"] > 0
+        assert srv.report(res)["load_concurrency_peak"] == res["load_concurrency_peak"]
 
 
 @pytest.mark.parametrize("sched", ["shared", "affinity", "soft"])

@@ -73,7 +73,7 @@ class _Request(ctypes.Structure):
 _STAT_FIELDS = ["submitted", "completed", "failed", "shed", "batches", "batched_cts", "request_hits", "key_loads",
                 "key_evictions", "bytes_loaded", "memory_waits", "resident_keys", "resident_bytes",
                 "peak_resident_bytes", "key_budget_bytes", "host_key_bytes", "coalesced", "spilled", "stolen",
-                "peak_queue_depth"]
+                "peak_queue_depth", "loads_in_flight", "peak_loads_in_flight", "load_inflight_ns"]
 
 
 class _ServerStats(ctypes.Structure):
@@ -102,6 +102,7 @@ class _LoadgenResult(ctypes.Structure):
                                                    "service_p99_ms", "service_max_ms", "slo_violation_rate_100",
                                                    "slo_violation_rate_300", "slo_violation_rate_500")]
                 + [(f, ctypes.c_int64) for f in ("coalesced_loads", "peak_queue_depth", "spilled", "stolen")]
+                + [("load_concurrency_mean", ctypes.c_double), ("load_concurrency_peak", ctypes.c_int64)]
                 + [("tenant_fairness_jain", ctypes.c_double), ("locality", ctypes.c_double),
                    ("lane_requests", ctypes.c_int64 * 16)])
 
@@ -897,7 +898,8 @@ class Server:
             "slo_violation_rate_300": res["slo_violation_rate_300"],
             "slo_violation_rate_500": res["slo_violation_rate_500"],
             "shed_plus_completed": res["shed"] + res["completed"], "peak_queue_depth": res["peak_queue_depth"],
-            "load_concurrency_peak": None, "load_concurrency_mean": None, "device": "hbm",
+            "load_concurrency_peak": res["load_concurrency_peak"], "load_concurrency_mean": res["load_concurrency_mean"],
+            "device": "hbm",
             "device_inflight_reads_peak": None, "device_inflight_reads_mean": None, "device_inflight_samples": 0,
             "batch": cfg["max_batch"], "sched": cfg["sched"], "per_worker_requests": res["lane_requests"],
             "tenant_fairness_jain": res["tenant_fairness_jain"], "locality": res["locality"],

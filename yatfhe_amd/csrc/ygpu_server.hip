@@ -158,6 +158,7 @@ struct ygpu_server {
     bool started = false;
     bool stop = false;
     uint64_t tick = 0;
+    int64_t load_change_ns = 0;  // when the count of loads under way last changed
     ygpu_server_stats st{};
 };
 
@@ -281,6 +282,16 @@ bool arena_alloc(ygpu_server* s, size_t bytes, int tenant, size_t* off) {
 
 // Starts loading `t` into the arena.  Returns 1 when no memory can be freed
 // now, 0 when the load is enqueued, <0 on error.  Server lock held.
+// Keeps the count of key loads under way, its peak and its integral over
+// time (server lock held).
+void load_count(ygpu_server* s, int delta) {
+    const int64_t now = now_ns();
+    if (s->load_change_ns) s->st.load_inflight_ns += s->st.loads_in_flight * (now - s->load_change_ns);
+    s->load_change_ns = now;
+    s->st.loads_in_flight += delta;
+    s->st.peak_loads_in_flight = std::max(s->st.peak_loads_in_flight, s->st.loads_in_flight);
+}
+
 int start_load(ygpu_server* s, PSet& ps, int t) {
     Tenant& tn = s->tenants[t];
     size_t off = 0;
@@ -310,6 +321,7 @@ int start_load(ygpu_server* s, PSet& ps, int t) {
     tn.off = off;
     tn.last_use = ++s->tick;
     ps.loads.push_back(t);
+    load_count(s, +1);
     ps.cv_load.notify_one();
     s->st.key_loads++;
     s->st.bytes_loaded += static_cast<int64_t>(f.packed_bytes + f.planes_bytes);
@@ -596,6 +608,7 @@ void watcher_loop(ygpu_server* s, PSet* ps) {
         const hipError_t e = hipEventSynchronize(ev);
         lk.lock();
         ps->loads.pop_front();
+        load_count(s, -1);
         Tenant& tn = s->tenants[t];
         if (e != hipSuccess) {
             // the key never arrived: release its memory and fail its requests
@@ -1076,13 +1089,17 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
                 grave.push_back(outsHold);
                 return fail(YGPU_ERR_BUSY, "load generator: warm-up requests still pending after 120 s");
             }
+        int64_t loadsFrom = 0, loadsTo = 0;  // the window of the loads-in-flight integral
+        ygpu_server_stats s0{};
         {
-            // the queue-depth peak is per measured run
+            // the queue-depth and load peaks are per measured run
             std::lock_guard<std::mutex> g(s->mu);
             s->st.peak_queue_depth = 0;
+            s->st.peak_loads_in_flight = s->st.loads_in_flight;
+            load_count(s, 0);
+            loadsFrom = s->load_change_ns;
+            s0 = s->st;
         }
-        ygpu_server_stats s0{};
-        ygpu_server_stats_get(s, &s0);
         // measured open loop
         std::mt19937_64 agen(c->seed ^ 0x9e3779b97f4a7c15ULL);
         std::exponential_distribution<double> gap(1.0);
@@ -1158,7 +1175,12 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
                 mismatches++;
         }
         ygpu_server_stats s1{};
-        ygpu_server_stats_get(s, &s1);
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            load_count(s, 0);
+            loadsTo = s->load_change_ns;
+            s1 = s->st;
+        }
         auto pct = [](std::vector<double>& v, double p) {
             if (v.empty()) return 0.0;
             const long i = static_cast<long>(std::ceil(p / 100.0 * v.size())) - 1;
@@ -1249,6 +1271,9 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             fair = sm * sm / (perTenant.size() * sm2);
         }
         out->tenant_fairness_jain = fair;
+        const int64_t span = loadsTo - loadsFrom;
+        out->load_concurrency_mean = span > 0 ? static_cast<double>(s1.load_inflight_ns - s0.load_inflight_ns) / span : 0;
+        out->load_concurrency_peak = s1.peak_loads_in_flight;
         out->locality = done ? static_cast<double>(owned) / done : 0;
         for (int k = 0; k < 16; k++) out->lane_requests[k] = laneCount[k];
         return YGPU_OK;
