@@ -236,10 +236,70 @@ __device__ __forceinline__ uint32_t f64_low32(double v) {  // |v| < 2^51: v mod 
     return static_cast<uint32_t>(__double_as_longlong(__dadd_rn(v, 0x1.8p52)));
 }
 
+// Packing of the balanced digits below: 8-bit fields (RB <= 7, where every
+// digit in [-B/2, B/2] fits a signed byte) into w0, or 16-bit fields into w0
+// (levels 0, 1) and w1 (levels 2, 3).
+template <int RB>
+__device__ __forceinline__ void pack_digit(int lvl, int sd, uint32_t& w0, uint32_t& w1) {
+    if constexpr (RB <= 7) {
+        w0 |= (static_cast<uint32_t>(sd) & 0xFFu) << (8 * lvl);
+    } else {
+        if (lvl < 2) w0 |= (static_cast<uint32_t>(sd) & 0xFFFFu) << (16 * lvl);
+        else w1 |= (static_cast<uint32_t>(sd) & 0xFFFFu) << (16 * (lvl - 2));
+    }
+}
+
+// The balanced digits of a 32-bit torus value with the carry chain folded into
+// one addition (same digits as digits_packed<32, L, RB>).  With the windows
+// w_l of m (level 0 most significant), the tie bit t_l (top bit of window
+// l - 1, zero for level 0) and the carry c_l into window l, the rule "carry
+// out when w_l + c_l + t_l > B/2" is the binary carry out of
+// w_l + c_l + (B/2 - 1) + t_l.  So m + K + T, with K = (B/2 - 1) in every
+// window and T = the tie bits moved down to their window's base, carries
+// exactly like the reference's loop, and digit l = window l of the sum
+// - (B/2 - 1) - t_l (checked against digits_of over all 2^32 inputs for
+// 2^7 x 3, 2^10 x 3, 2^8 x 2 and 2^8 x 4).
+template <int L, int RB>
+struct DigitsAdd32 {
+    static constexpr uint32_t window_constant() {
+        uint32_t k = 0;
+        for (int l = 0; l < L; l++) k += static_cast<uint32_t>((1 << (RB - 1)) - 1) << (32 - (l + 1) * RB);
+        return k;
+    }
+    static constexpr uint32_t tie_mask() {
+        uint32_t t = 0;
+        for (int l = 1; l < L; l++) t |= 1u << (32 - (l + 1) * RB);
+        return t;
+    }
+};
+
+template <int L, int RB>
+__device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, uint32_t& w1) {
+    constexpr int DROP = 32 - L * RB;
+    static_assert(DROP >= 0, "gadget wider than the torus");
+    constexpr uint32_t BIAS = DROP > 0 ? (1u << (DROP - 1)) : 0u;
+    constexpr int B = 1 << RB, HB = B >> 1;
+    constexpr uint32_t K = DigitsAdd32<L, RB>::window_constant();
+    constexpr uint32_t TM = DigitsAdd32<L, RB>::tie_mask();
+    const uint32_t s = static_cast<uint32_t>(static_cast<int32_t>(u) >> 31);  // all ones when negative
+    const uint32_t m = ((u ^ s) - s) + BIAS;
+    const uint32_t t = (m >> (2 * RB - 1)) & TM;
+    const uint32_t sum = m + K + t;
+    const int sneg = static_cast<int>(s);
+    w0 = 0;
+    w1 = 0;
+#pragma unroll
+    for (int lvl = 0; lvl < L; lvl++) {
+        const int shift = 32 - (lvl + 1) * RB;
+        const int tl = lvl > 0 ? static_cast<int>((t >> shift) & 1u) : 0;
+        const int bal = static_cast<int>((sum >> shift) & static_cast<uint32_t>(B - 1)) - (HB - 1) - tl;
+        pack_digit<RB>(lvl, (bal ^ sneg) - sneg, w0, w1);
+    }
+}
+
 // All L balanced digits (gadget_decomposition.h:75-95) of a value of a
-// TBITS-bit torus, with the gadget shape known at compile time.  Packed in
-// 8-bit fields (RB <= 8) into w0, or 16-bit fields into w0 (levels 0, 1) and
-// w1 (levels 2, 3).
+// TBITS-bit torus, with the gadget shape known at compile time, packed as
+// pack_digit describes.
 template <int TBITS, int L, int RB>
 __device__ __forceinline__ void digits_packed(int64_t x, uint32_t& w0, uint32_t& w1) {
     using U = typename std::conditional<TBITS == 32, uint32_t, uint64_t>::type;
@@ -263,13 +323,7 @@ __device__ __forceinline__ void digits_packed(int64_t x, uint32_t& w0, uint32_t&
         // digit > B/2, or digit == B/2 with the tie bit set
         carry = (digit + tie) > HB;
         const int bal = carry ? digit - B : digit;
-        const int sd = (bal ^ sneg) - sneg;
-        if constexpr (RB <= 8) {
-            w0 |= (static_cast<uint32_t>(sd) & 0xFFu) << (8 * lvl);
-        } else {
-            if (lvl < 2) w0 |= (static_cast<uint32_t>(sd) & 0xFFFFu) << (16 * lvl);
-            else w1 |= (static_cast<uint32_t>(sd) & 0xFFFFu) << (16 * (lvl - 2));
-        }
+        pack_digit<RB>(lvl, (bal ^ sneg) - sneg, w0, w1);
     }
 }
 

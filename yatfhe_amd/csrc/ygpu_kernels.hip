@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     int dv;
-                    if constexpr (C::L > 0 && C::RB <= 8) {
+                    if constexpr (C::L > 0 && C::RB <= 7) {
                         dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
                     } else {
                         const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];

@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
                 const uint32_t av = static_cast<uint32_t>(acc[r][src]);
                 const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(acc[r][j]);
                 uint32_t d0, d1;
-                digits_packed<32, L, LAT_RB>(static_cast<int32_t>(rot), d0, d1);
+                digits_packed_add32<L, LAT_RB>(rot, d0, d1);
                 digs[r][j] = d0;
             }
         }
