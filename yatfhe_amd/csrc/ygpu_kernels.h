@@ -20,6 +20,7 @@ template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int
 struct BrCfg {
     static constexpr int L = L_;
     static constexpr int RB = RB_;
+    static constexpr int TWL = TWL_;
     static constexpr int TBITS = TBITS_;
     using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
     static constexpr int LOGN = LOGN_;

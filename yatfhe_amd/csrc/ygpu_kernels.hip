@@ -570,8 +570,11 @@ int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* o
     if (tbits == 32 && logn == 10 && level == 3 && radix_bits == 7) {
         for (int cpw : {4, 2, 1})
             for (int twl : {1, 0}) c[n++] = BrTile{cpw, twl};
-    } else if (tbits == 56) {
+    } else if (tbits == 56 && logn == 11 && level == 3 && radix_bits == 10) {
+        c[n++] = BrTile{1, 1};
         c[n++] = BrTile{1, 0};
+    } else if (tbits == 56) {
+        c[n++] = BrTile{1, 1};
     } else if (tbits == 32 && logn == 10) {
         c[n++] = BrTile{4, 1};
         c[n++] = BrTile{1, 1};

@@ -36,8 +36,10 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // (registers are the binding resource); each updates half the positions
     __shared__ int64_t accbuf[CPW][K1][N];
     // forward twiddles of both primes (the forward transform runs l times per
-    // step); the inverse twiddles are read through L1
-    __shared__ double ftw[NP][N];
+    // step) in LDS unless the tile reads them through L1 (TWL == 0); the
+    // inverse twiddles are always read through L1
+    constexpr bool FTW_LDS = C::TWL != 0;
+    __shared__ double ftw[FTW_LDS ? NP : 1][FTW_LDS ? N : 1];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int first = grp * CPW;
@@ -61,9 +63,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
     const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
     const PrimeF64 pr = a.prime[tp];
-    for (int e = static_cast<int>(threadIdx.x); e < NP * N; e += C::THREADS)
-        ftw[e / N][e % N] = a.tw[static_cast<size_t>(e / N) * 2 * N + e % N];
-    const double* tw = ftw[tp];
+    if constexpr (FTW_LDS) {
+        for (int e = static_cast<int>(threadIdx.x); e < NP * N; e += C::THREADS)
+            ftw[e / N][e % N] = a.tw[static_cast<size_t>(e / N) * 2 * N + e % N];
+    }
+    const double* tw = FTW_LDS ? static_cast<const double*>(ftw[tp]) : a.tw + static_cast<size_t>(tp) * 2 * N;
     const double* __restrict__ itw = a.tw + static_cast<size_t>(tp) * 2 * N + N;
 
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
@@ -258,11 +262,11 @@ hipError_t launch_br_crt(const BrArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_blind_rotate_crt(int logn, const BrTile& t, int radix_bits, int level, const BrArgs& a, hipStream_t s) {
-    if (logn == 11 && level == 3 && radix_bits == 10) return launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 0>>(a, s);
+    if (logn == 11 && level == 3 && radix_bits == 10)
+        return t.twl ? launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 1>>(a, s) : launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 0>>(a, s);
     const bool p8 = radix_bits <= 8;
     if (logn == 10) return p8 ? launch_br_crt<BrCfg<56, 10, 2, 1, true>>(a, s) : launch_br_crt<BrCfg<56, 10, 2, 1, false>>(a, s);
     if (logn == 11) return p8 ? launch_br_crt<BrCfg<56, 11, 2, 1, true>>(a, s) : launch_br_crt<BrCfg<56, 11, 2, 1, false>>(a, s);
-    (void)t;
     return hipErrorInvalidValue;
 }
 

@@ -58,7 +58,7 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
     // default tile: small workgroups (a barrier couples few waves), twiddles in
     // LDS where they fit; TORUS32 N = 1024: two ciphertexts per workgroup
     const bool t32 = params->torus_bits == 32 && params->N == 1024;
-    ctx->tile = BrTile{t32 ? 2 : 1, t32 ? 1 : 0};
+    ctx->tile = BrTile{t32 ? 2 : 1, (t32 || params->torus_bits == 56) ? 1 : 0};
     if (const char* e = std::getenv("YGPU_CPW")) ctx->tile.cpw = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
     YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
