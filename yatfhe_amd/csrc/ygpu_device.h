@@ -110,6 +110,45 @@ __device__ __forceinline__ void team_sync() {
     }
 }
 
+template <int TEAM_WAVES>
+struct WorkgroupSync {
+    __device__ __forceinline__ void sync() { team_sync<TEAM_WAVES>(); }
+};
+
+// Barrier of the W wavefronts of one team through a counter in LDS, for teams
+// smaller than the workgroup: other teams do not wait.  Each wave's first lane
+// adds one after the wave's LDS writes have completed; the wave then waits
+// until the counter reaches W times the number of barriers it has passed.
+// Every team member passes the same sequence of barriers, so the counts agree.
+// The wait is bounded: a wave that polls POLLS times without the count
+// arriving sets *fail, and from then on no barrier waits, so a miscount ends
+// in wrong results and a set flag, never in a hang.
+#ifndef YGPU_TEAM_BARRIER_POLLS
+#define YGPU_TEAM_BARRIER_POLLS (1 << 16)
+#endif
+template <int W, int POLLS = YGPU_TEAM_BARRIER_POLLS>
+struct TeamBarrier {
+    uint32_t* ctr;
+    uint32_t* fail;
+    uint32_t passed;
+    __device__ __forceinline__ void sync() {
+        passed += W;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (int poll = 0;; poll++) {
+            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= passed) break;
+            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
+            if (poll >= POLLS) {
+                __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
+
 // Cooley-Tukey pass (forward): U, V -> U + wV, U - wV.
 template <int LOGN, int P>
 __device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* __restrict__ tw, const PrimeF64& pr) {
@@ -152,12 +191,12 @@ __device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* 
     }
 }
 
-template <int LOGN, int TEAM_WAVES, int PW, int PR, int X>
-__device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl) {
-    team_sync<TEAM_WAVES>();
+template <int LOGN, int PW, int PR, int X, class Sync>
+__device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl, Sync& sy) {
+    sy.sync();
 #pragma unroll
     for (int r = 0; r < 16; r++) scr[Xch<LOGN, X>::addr(Pass<LOGN, PW>::idx(tl, r))] = x[r];
-    team_sync<TEAM_WAVES>();
+    sy.sync();
 #pragma unroll
     for (int r = 0; r < 16; r++) x[r] = scr[Xch<LOGN, X>::addr(Pass<LOGN, PR>::idx(tl, r))];
 }
@@ -166,31 +205,41 @@ __device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl) {
 // in pass-2 layout.  Inputs are digits (|x| <= 2^15); no reduction is needed:
 // a stage adds at most p(0.5 + 1.5|x| 2^-53) to the bound (mulmod above), so
 // after 10 (11) stages every value is below 7.75p (8.98p) < 16p <= 2^53.
+template <int LOGN, class Sync>
+__device__ __forceinline__ void ntt_forward_sync(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr, Sync& sy) {
+    fwd_pass<LOGN, 0>(x, tl, tw, pr);
+    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
+    fwd_pass<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
+    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+}
 template <int LOGN, int TEAM_WAVES>
 __device__ __forceinline__ void ntt_forward(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr) {
-    fwd_pass<LOGN, 0>(x, tl, tw, pr);
-    exchange<LOGN, TEAM_WAVES, 0, 1, 0>(x, scr, tl);
-    fwd_pass<LOGN, 1>(x, tl, tw, pr);
-    exchange<LOGN, TEAM_WAVES, 1, 2, 1>(x, scr, tl);
-    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+    WorkgroupSync<TEAM_WAVES> sy;
+    ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, sy);
 }
 
 // Inverse negacyclic NTT without the 1/N factor (folded into the key):
 // pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Values are reduced
 // at the start of every pass, which bounds them by 8.8p throughout.
-template <int LOGN, int TEAM_WAVES>
-__device__ __forceinline__ void ntt_inverse(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr) {
+template <int LOGN, class Sync>
+__device__ __forceinline__ void ntt_inverse_sync(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr, Sync& sy) {
 #pragma unroll
     for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
     inv_pass<LOGN, 2>(x, tl, itw, pr);
-    exchange<LOGN, TEAM_WAVES, 2, 1, 1>(x, scr, tl);
+    exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
 #pragma unroll
     for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
     inv_pass<LOGN, 1>(x, tl, itw, pr);
-    exchange<LOGN, TEAM_WAVES, 1, 0, 0>(x, scr, tl);
+    exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
 #pragma unroll
     for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
     inv_pass<LOGN, 0>(x, tl, itw, pr);
+}
+template <int LOGN, int TEAM_WAVES>
+__device__ __forceinline__ void ntt_inverse(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr) {
+    WorkgroupSync<TEAM_WAVES> sy;
+    ntt_inverse_sync<LOGN>(x, scr, tl, itw, pr, sy);
 }
 
 // ---------------------------------------------------------------------------

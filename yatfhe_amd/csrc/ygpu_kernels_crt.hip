@@ -40,6 +40,9 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // inverse twiddles are always read through L1
     constexpr bool FTW_LDS = C::TWL != 0;
     __shared__ double ftw[FTW_LDS ? NP : 1][FTW_LDS ? N : 1];
+    // barrier counters of the four two-wave teams of each ciphertext, and the
+    // flag a timed-out team barrier sets
+    __shared__ uint32_t teamCount[CPW * 4 + 1];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int first = grp * CPW;
@@ -63,6 +66,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
     const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
     const PrimeF64 pr = a.prime[tp];
+    if (threadIdx.x < CPW * 4 + 1) teamCount[threadIdx.x] = 0u;
     if constexpr (FTW_LDS) {
         for (int e = static_cast<int>(threadIdx.x); e < NP * N; e += C::THREADS)
             ftw[e / N][e % N] = a.tw[static_cast<size_t>(e / N) * 2 * N + e % N];
@@ -91,6 +95,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         }
     }
     __syncthreads();
+    // the transforms and the team's own scratch reuse synchronise only the
+    // team's two waves; hand-overs between teams keep workgroup barriers
+    using TeamSync = typename std::conditional<TW == 2, TeamBarrier<2>, WorkgroupSync<TW>>::type;
+    TeamSync ts{};
+    if constexpr (TW == 2) ts = TeamSync{&teamCount[lc * 4 + tt], &teamCount[CPW * 4], 0u};
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
     // the next step's rotation amount is loaded a step ahead, so no step
@@ -185,7 +194,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                 }
                 x[r] = static_cast<double>(dv);
             }
-            ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
+            ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, ts);
             // one column at a time: the scheduling barrier keeps the second
             // column's loads from being hoisted over the first column's
             // products, which would hold 32 more registers live
@@ -202,7 +211,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         }
 
         // partner sums (same prime): hand column 1-tm to team (1-tm, tp)
-        team_sync<TW>();
+        ts.sync();
 #pragma unroll
         for (int r = 0; r < 16; r++) scr[r * T + tl] = oth[r];
         __syncthreads();
@@ -213,18 +222,17 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             for (int r = 0; r < 16; r++) x[r] = __dadd_rn(own[r], other[r * T + tl]);
         }
         // the partner's reads must finish before this team's scratch is
-        // rewritten; with two-wave teams the inverse transform's first exchange
-        // opens with a workgroup barrier, which already orders them
-        if constexpr (TW == 1) __syncthreads();
+        // rewritten (the transform's own barriers order only this team)
+        __syncthreads();
 
-        ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+        ntt_inverse_sync<LOGN>(x, scr, tl, itw, pr, ts);
 
         // residues of column tm mod both primes: the two prime teams of row tm
         // split the reconstruction, team tp taking registers [8 tp, 8 tp + 8);
         // each hands the other team the half it needs
 #pragma unroll
         for (int r = 0; r < 16; r++) x[r] = centre(x[r], pr);
-        team_sync<TW>();
+        ts.sync();
 #pragma unroll
         for (int r = 0; r < 8; r++) {
             const int rr = r + 8 * (1 - tp);
@@ -248,8 +256,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
 
     if (live && tp == 0) {
         TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + tm * N;
+        // a team barrier that timed out (TeamBarrier) leaves the step order
+        // unproven: every accumulator of the workgroup reads INT64_MIN instead
+        const bool failed = teamCount[CPW * 4] != 0u;
 #pragma unroll
-        for (int r = 0; r < 16; r++) outAcc[tl + T * r] = acc[tl + T * r];
+        for (int r = 0; r < 16; r++) outAcc[tl + T * r] = failed ? INT64_MIN : acc[tl + T * r];
     }
 }
 
