@@ -51,6 +51,10 @@ def test_parameter_validation_messages():
     bad = yg.replace(yg.PARAMS_T32_N1024, level=4, radix_bits=9).c()   # 36 bits > 32-bit torus
     assert lib.ygpu_init(ctypes.byref(bad), 0, ctypes.byref(h)) != 0
     assert b"do not fit" in lib.ygpu_last_error() or yg.device_count() == 0
+    # a 2^16 gadget has the digit +2^15, which a 16-bit digit field cannot hold
+    bad = yg.replace(yg.PARAMS_T56_N2048, level=3, radix_bits=16).c()
+    assert lib.ygpu_init(ctypes.byref(bad), 0, ctypes.byref(h)) != 0
+    assert b"radix_bits" in lib.ygpu_last_error() or yg.device_count() == 0
 
 
 def test_keygen_deterministic_and_seed_sensitive(p32):

@@ -119,7 +119,8 @@ inline int check_params(const ygpu_params& p) {
     if (p.N != 1024 && p.N != 2048) return fail(YGPU_ERR_UNSUPPORTED, "ring degree N must be 1024 or 2048");
     if (p.n < 1 || p.n > KS_MAX_N) return fail(YGPU_ERR_UNSUPPORTED, "LWE dimension n must be in [1, 1023]");
     if (p.level < 1 || p.level > 4) return fail(YGPU_ERR_UNSUPPORTED, "gadget level must be in [1, 4]");
-    if (p.radix_bits < 1 || p.radix_bits > 16) return fail(YGPU_ERR_UNSUPPORTED, "radix_bits must be in [1, 16]");
+    // digits lie in [-B/2, B/2] and travel in 16-bit fields: B/2 <= 2^14
+    if (p.radix_bits < 1 || p.radix_bits > 15) return fail(YGPU_ERR_UNSUPPORTED, "radix_bits must be in [1, 15]");
     // decomposeRow's own checks (gadget_decomposition.h:194-206)
     if (p.level * p.radix_bits > p.torus_bits)
         return fail(YGPU_ERR_INVALID_ARG, "decomposeRow: " + std::to_string(p.level) + " digits of " + std::to_string(p.radix_bits) +
