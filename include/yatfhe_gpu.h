@@ -59,8 +59,8 @@ typedef struct ygpu_params {
     int32_t n;              /* LWE dimension */
     int32_t k;              /* GLWE dimension (1) */
     int32_t N;              /* ring degree (1024 or 2048) */
-    int32_t radix_bits;     /* gadget base log2 */
-    int32_t level;          /* gadget levels */
+    int32_t radix_bits;     /* gadget base log2, 1..15 */
+    int32_t level;          /* gadget levels, 1..4; level * radix_bits <= torus_bits */
     int32_t ks_radix_bits;  /* keyswitch base log2 */
     int32_t ks_level;       /* keyswitch levels */
     int32_t ks_width_bits;  /* keyswitch precision (32) */
