@@ -29,6 +29,9 @@
  *  - KSK layout: ksk[i][j][n+1] int32, i < k*N, j < ks_level, entries
  *    a[0..n-1] then b (TlweKeySwitchingKey::decomposedKsk, keyswitching.h:14-31).
  *  - One context per GPU; calls on one context are serialised by the library.
+ *  - Device-side failures of asynchronous calls (YGPU_ERR_DEVICE) are reported
+ *    by the next ygpu_sync, ygpu_memcpy_dtoh or synchronous bootstrapping call
+ *    on the same context.
  */
 #ifndef YATFHE_GPU_H
 #define YATFHE_GPU_H
@@ -50,6 +53,9 @@ extern "C" {
 #define YGPU_ERR_BUSY (-7)      /* serving queue full: the request was shed */
 #define YGPU_ERR_IO (-8)        /* a file could not be opened, read or written */
 #define YGPU_ERR_CORRUPT (-9)   /* a key file failed its format or digest checks */
+#define YGPU_ERR_DEVICE (-10)   /* a kernel reported a failed device-side check (a blind-rotation team
+                                   barrier timed out): the results of the calls since the last
+                                   synchronisation of that context are invalid */
 
 /* Parameter set.  Mirrors YatfheParameters (yatfhe_parameters.h:9-110) for the
  * fields the bootstrapping path reads; `level` is the gadget length used by the
@@ -61,7 +67,7 @@ typedef struct ygpu_params {
     int32_t N;              /* ring degree (1024 or 2048) */
     int32_t radix_bits;     /* gadget base log2, 1..15 */
     int32_t level;          /* gadget levels, 1..4; level * radix_bits <= torus_bits */
-    int32_t ks_radix_bits;  /* keyswitch base log2 */
+    int32_t ks_radix_bits;  /* keyswitch base log2, 1..7 (digits travel as signed bytes) */
     int32_t ks_level;       /* keyswitch levels */
     int32_t ks_width_bits;  /* keyswitch precision (32) */
     int32_t lwe_q_bits;     /* log2 qLwe (32) */
@@ -212,6 +218,14 @@ int ygpu_timing_reset(ygpu_ctx* ctx);
  * kernel for HBM byte counters. */
 int ygpu_modmul_peak(ygpu_ctx* ctx, int32_t kind, double* modmul_per_sec);
 int ygpu_copy_device(ygpu_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+/* Bound on the polls of each team-barrier wait inside the blind-rotation
+ * kernels (default 65536, milliseconds against microseconds of real waiting).
+ * A wait that runs out marks the workgroup failed: its accumulators read the
+ * most negative torus value and the call reports YGPU_ERR_DEVICE.  polls = 0
+ * fails every team barrier at once without waiting, which exercises that
+ * report path with every kernel still finishing normally (a test hook; no
+ * counterpart in the reference). */
+int ygpu_set_barrier_polls(ygpu_ctx* ctx, int32_t polls);
 
 /* Seeded key generation on the host (genTlweKey, genTrlweKey,
  * genBootstrappingKeyMP, genTlweKeySwitchingKey: tlwe.cpp, trlwe.cpp:70-81,

@@ -15,6 +15,7 @@ def sign_batch(keys, count, seed):
 
 def test_graph_replays_match_eager(p32):
     keysets = [yg.keygen(p32, seed=300 + t) for t in range(2)]
+    orcs = [Oracle(p32, k) for k in keysets]
     counts = [70, 130]
     v = yg.sign_test_polynomial(p32)
     with yg.Context(p32, 0) as ctx:
@@ -37,11 +38,50 @@ def test_graph_replays_match_eager(p32):
             ctx.sync()
             want = d_out2.download(np.zeros((total, p32.n + 1), np.int32))
             assert np.array_equal(got, want), f"replay {rep} differs from the eager pipeline"
+            # the oracle on the first and last ciphertext of each job
+            for (kid, cnt, first), orc in zip(jobs, orcs):
+                for c in (first, first + cnt - 1):
+                    assert np.array_equal(got[c], orc.pbs(cts[c], v)[2]), f"replay {rep}, ct {c} differs from the oracle"
             ph = yg.lwe_decrypt_phase(got[:counts[0]], keysets[0].lwe_key)
             assert (np.abs(ph.astype(np.int64)) > (1 << 27)).all()
         ctx.graph_destroy(gid)
         for b in (d_lwe, d_v, d_out, d_out2):
             b.free()
+    for o in orcs:
+        o.close()
+
+
+def test_graph_survives_later_key_uploads(p32, keys32):
+    """Keys uploaded after a graph was captured leave its key tables in place
+    (the tables never move), and a key a live graph reads cannot be released."""
+    v = yg.sign_test_polynomial(p32)
+    bits, cts = sign_batch(keys32, 24, seed=77)
+    extra = yg.keygen(p32, seed=401)
+    with yg.Context(p32, 0) as ctx:
+        kid = ctx.upload_key(keys32)
+        d_lwe = ctx.alloc(cts.nbytes); d_lwe.upload(cts)
+        d_v = ctx.alloc(v.nbytes); d_v.upload(v)
+        d_out = ctx.alloc(len(cts) * (p32.n + 1) * 4)
+        gid = ctx.graph_create([(kid, len(cts), 0)], len(cts), d_lwe, d_v, False, d_out)
+        # more keys than any earlier table size held
+        later = [ctx.upload_key(extra) for _ in range(20)]
+        ctx.graph_launch(gid)
+        ctx.sync()
+        got = d_out.download(np.zeros((len(cts), p32.n + 1), np.int32))
+        with pytest.raises(yg.YgpuError) as e:
+            ctx.release_key(kid)
+        assert e.value.code == yg.YGPU_ERR_BUSY
+        ctx.graph_destroy(gid)
+        ctx.release_key(kid)
+        for k in later:
+            ctx.release_key(k)
+        for b in (d_lwe, d_v, d_out):
+            b.free()
+    orc = Oracle(p32, keys32)
+    want = orc.pbs_batch(cts, v)
+    orc.close()
+    assert np.array_equal(got, want)
+    assert np.array_equal((yg.lwe_decrypt_phase(got, keys32.lwe_key) > 0).astype(int), bits)
 
 
 def test_every_tile_is_bit_identical(p32, keys32):

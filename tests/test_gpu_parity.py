@@ -244,8 +244,15 @@ def test_key_file_upload_matches_direct_upload(tmp_path, p32, keys32):
         a = ctx.upload_key(keys32)
         b = ctx.upload_key_file(path)
         out_a, _, _ = ctx.pbs_batch(a, cts, v)
-        out_b, _, _ = ctx.pbs_batch(b, cts, v)
+        out_b, acc_b, _ = ctx.pbs_batch(b, cts, v, want_acc=True)
     assert np.array_equal(out_a, out_b)
+    # and the file-loaded key against the oracle, accumulators too
+    orc = Oracle(p32, keys32)
+    ref_acc, _, ref_out = orc.pbs_batch_all(cts, v)
+    orc.close()
+    assert np.array_equal(ref_out, out_b)
+    assert np.array_equal(ref_acc, acc_b)
+    assert np.array_equal((yg.lwe_decrypt_phase(out_b, keys32.lwe_key) > 0).astype(int), bits)
 
 
 def test_all_4096_outputs_match_oracle(p32, keys32, ctx32, orc32):

@@ -1,18 +1,21 @@
-"""The serving loop: every result is the same alone and in the mix.
+"""The serving loop against the oracle: every served result equals the CPU
+oracle's bootstrap of the same ciphertext under the tenant's key, alone and in
+the mix.
 
 Mirrors the reference's serving benchmark setting (tenants with their own
 keys, a key cache smaller than the key pool, both torus widths in one process)
-on the GPU server: results of requests served one at a time, of the same
-requests submitted together from several threads (batched with other tenants'
-requests, across evictions and reloads), and of the single-context batched
-entry point (itself pinned to the oracle by the parity tests) must be
-bit-identical."""
+on the GPU server: requests served one at a time, the same requests submitted
+together from several threads (batched with other tenants' requests, across
+evictions and reloads), and open-loop load whose every result the server
+compares with the oracle's."""
 import threading
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import pytest
 
 import yatfhe_amd as yg
+from tests.oracle_py import Oracle, derive_succinct
 
 pytestmark = pytest.mark.gpu
 
@@ -36,11 +39,20 @@ def tenants(p32, p56):
     return k32, k56
 
 
-def context_results(params, keys, cts, v):
-    with yg.Context(params, 0) as ctx:
-        kid = ctx.upload_key(keys)
-        out, _, _ = ctx.pbs_batch(kid, cts, v)
+def oracle_results(params, keys, cts, v):
+    """The oracle's keyswitched outputs (the reference's CPU path restated)."""
+    orc = Oracle(params, keys)
+    out = orc.pbs_batch(np.atleast_2d(cts), v)
+    orc.close()
     return out
+
+
+class _Derived:
+    """A succinct key's full coefficient-form key, as the oracle derives it."""
+
+    def __init__(self, params, keys):
+        self.bsk = derive_succinct(params, keys)
+        self.ksk = keys.ksk
 
 
 def test_alone_equals_mix_across_evictions(p32, p56, tenants):
@@ -95,12 +107,13 @@ def test_alone_equals_mix_across_evictions(p32, p56, tenants):
         assert st["peak_resident_bytes"] <= budget
         assert st["completed"] == 2 * len(reqs) and st["failed"] == 0
         assert max(sizes) > 1, "no request was batched with another"
-    # the single-context batched path gives the same bits
-    print("server: context", flush=True)
-    for t, prm, k, cts, v in work:
-        ref = context_results(prm, k, cts, v)
+    # the oracle gives the same bits
+    print("server: oracle", flush=True)
+    with ThreadPoolExecutor(4) as pool:
+        refs = list(pool.map(lambda w: oracle_results(w[1], w[2], w[3], w[4]), work))
+    for (t, prm, k, cts, v), ref in zip(work, refs):
         for c in range(len(cts)):
-            assert np.array_equal(ref[c], alone[(t, c)]), f"tenant {t} ciphertext {c}: server differs from context"
+            assert np.array_equal(ref[c], alone[(t, c)]), f"tenant {t} ciphertext {c}: server differs from the oracle"
     # and they decrypt to the intended values
     for i, (t, prm, k, cts, v) in enumerate(work[:len(k32)]):
         ph = yg.lwe_decrypt_phase(np.stack([alone[(t, c)] for c in range(len(cts))]), k.lwe_key)
@@ -114,7 +127,7 @@ def test_loadgen_results_match_and_account(p32, tenants):
     k32, _ = tenants
     v32 = yg.sign_test_polynomial(p32)
     base_cts = [t32_inputs(k, 1, 90 + i)[0] for i, k in enumerate(k32[:2])]
-    expected_base = [context_results(p32, k, ct[None, :], v32)[0] for k, ct in zip(k32[:2], base_cts)]
+    expected_base = [oracle_results(p32, k, ct, v32)[0] for k, ct in zip(k32[:2], base_cts)]
     with yg.Server(0, key_budget_bytes=6 * 83 << 20, lanes=2, max_batch=512) as srv:
         ps = srv.add_params(p32)
         base = [srv.add_tenant(ps, k) for k in k32[:2]]
@@ -144,7 +157,7 @@ def test_scheduling_modes(p32, tenants, sched):
     k32, _ = tenants
     v32 = yg.sign_test_polynomial(p32)
     base_cts = [t32_inputs(k, 1, 95 + i)[0] for i, k in enumerate(k32[:2])]
-    expected_base = [context_results(p32, k, ct[None, :], v32)[0] for k, ct in zip(k32[:2], base_cts)]
+    expected_base = [oracle_results(p32, k, ct, v32)[0] for k, ct in zip(k32[:2], base_cts)]
     lanes = 3
     with yg.Server(0, lanes=lanes, max_batch=256, sched=sched, spill_threshold=2, steal_threshold=2) as srv:
         ps = srv.add_params(p32)
@@ -177,8 +190,9 @@ def test_scheduling_modes(p32, tenants, sched):
 
 def test_succinct_tenants_alone_equal_mix(p32):
     """Tenants holding WWL+24 succinct keys: each load moves the succinct form and
-    derives the full rows on the device; results equal the single-context
-    succinct upload's, alone and in a mix with full-key tenants, across evictions."""
+    derives the full rows on the device; results equal the oracle's bootstrap
+    under the oracle-derived full key (blindRotateWWL24Ntt's rows), alone and in
+    a mix with full-key tenants, across evictions."""
     v32 = yg.sign_test_polynomial(p32)
     full_keys = [yg.keygen(p32, seed=700 + t) for t in range(2)]
     succ_keys = [yg.keygen_succinct(p32, seed=710 + t, s2_level=4) for t in range(3)]
@@ -214,8 +228,72 @@ def test_succinct_tenants_alone_equal_mix(p32):
         succ_b = ((p32.n * p32.level * 2 + 4 * 2) * poly_words + 1) * 8 + planes
         assert st["host_key_bytes"] == 2 * full_b + 3 * succ_b
         assert succ_b < 0.66 * full_b
-    print("server: succinct context", flush=True)
+    print("server: succinct oracle", flush=True)
     for t, k, cts in work:
-        ref = context_results(p32, k, cts, v32)
+        okeys = _Derived(p32, k) if isinstance(k, yg.SuccinctKeySet) else k
+        ref = oracle_results(p32, okeys, cts, v32)
         for c in range(len(cts)):
-            assert np.array_equal(ref[c], alone[(t, c)]), f"tenant {t} ciphertext {c}: server differs from context"
+            assert np.array_equal(ref[c], alone[(t, c)]), f"tenant {t} ciphertext {c}: server differs from the oracle"
+
+
+def test_serving_256_tenants_mixed_widths_vs_oracle(p32, p56):
+    """The serving workload's shape: 256 tenants with Zipf(1.1) popularity, 30 %
+    of them TORUS56 N=2048 (16-entry LUT) and 70 % TORUS32 N=1024 (sign gate),
+    key materials shared round-robin within each width but every tenant cached
+    on the device on its own, under a key budget below the key pool (loads from
+    pinned host memory, evictions); every result of open-loop Poisson load is
+    compared by the server with the oracle's bootstrap of that tenant's input."""
+    n_t, mats32, mats56 = 256, 8, 4
+    perm = np.random.default_rng(5).permutation(n_t)
+    is56 = perm < int(round(0.3 * n_t))
+    lut = np.random.default_rng(6).integers(0, P16, P16)
+    v32 = yg.sign_test_polynomial(p32)
+    v56 = yg.lut_test_polynomial(p56, lut, P16)
+    keys32 = [yg.keygen(p32, seed=3000 + m) for m in range(mats32)]
+    keys56 = [yg.keygen(p56, seed=4000 + m) for m in range(mats56)]
+    in32 = [t32_inputs(k, 1, 5000 + m)[0] for m, k in enumerate(keys32)]
+    msg56, in56 = [], []
+    for m, k in enumerate(keys56):
+        msgs, cts = t56_inputs(k, 1, 6000 + m)
+        msg56.append(int(msgs[0]))
+        in56.append(cts[0])
+    print("serving 256: oracle expectations", flush=True)
+    with ThreadPoolExecutor(8) as pool:
+        exp32 = list(pool.map(lambda a: oracle_results(p32, a[0], a[1], v32)[0], zip(keys32, in32)))
+        exp56 = list(pool.map(lambda a: oracle_results(p56, a[0], a[1], v56)[0], zip(keys56, in56)))
+    for m, k in enumerate(keys56):
+        ph = int(yg.lwe_decrypt_phase(exp56[m], k.lwe_key)[0])
+        assert round(ph / (1 << 32) * P16) % P16 == lut[msg56[m]] % P16
+    budget = 12 << 30  # the pool is ~41 GB in device form: loads and evictions
+    with yg.Server(0, key_budget_bytes=budget, lanes=3, max_batch=4096, queue_max=1 << 16) as srv:
+        ps32 = srv.add_params(p32)
+        ps56 = srv.add_params(p56)
+        first, inputs, luts, expected = {}, [], [], []
+        c32 = c56 = 0
+        for t in range(n_t):
+            if is56[t]:
+                w, m = 56, c56 % mats56
+                c56 += 1
+            else:
+                w, m = 32, c32 % mats32
+                c32 += 1
+            if (w, m) not in first:
+                tid = srv.add_tenant(ps56 if w == 56 else ps32, keys56[m] if w == 56 else keys32[m])
+                first[(w, m)] = tid
+            else:
+                tid = srv.add_tenant_shared(first[(w, m)])
+            assert tid == t
+            inputs.append(in56[m] if w == 56 else in32[m])
+            luts.append(v56 if w == 56 else v32)
+            expected.append(exp56[m] if w == 56 else exp32[m])
+        for rate in (2000, 10000):
+            print(f"serving 256: loadgen {rate}/s", flush=True)
+            res = srv.loadgen(rate=rate, requests=rate, inputs=inputs, luts=luts, expected=expected, warm=0,
+                              zipf_s=1.1, seed=5, slo_ms=100.0)
+            print({k: res[k] for k in ("throughput_rps", "p50_ms", "p99_ms", "hit_rate", "key_loads", "key_evictions",
+                                      "mismatches", "errors", "shed")}, flush=True)
+            assert res["errors"] == 0 and res["mismatches"] == 0 and res["shed"] == 0
+            assert res["completed"] == rate
+            assert 0 < res["p50_ms"] <= res["p99_ms"]
+        st = srv.stats()
+        assert st["key_evictions"] > 0 and st["peak_resident_bytes"] <= budget and st["failed"] == 0

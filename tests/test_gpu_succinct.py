@@ -81,3 +81,10 @@ def test_succinct_key_file_upload(tmp_path, p32):
         a, acc_a, _ = ctx.pbs_batch(kf, cts, v, want_acc=True)
         b, acc_b, _ = ctx.pbs_batch(ks, cts, v, want_acc=True)
     assert np.array_equal(a, b) and np.array_equal(acc_a, acc_b)
+    # the file-loaded key against the oracle with the oracle-derived full key
+    keys = yg.KeySet(p32, sk.lwe_key, sk.glwe_key, derive_succinct(p32, sk), sk.ksk)
+    orc = Oracle(p32, keys)
+    ref_acc, _, ref_out = orc.pbs_batch_all(cts[:4], v)
+    orc.close()
+    assert np.array_equal(ref_out, a[:4])
+    assert np.array_equal(ref_acc.reshape(4, -1).astype(np.int64), acc_a[:4].reshape(4, -1).astype(np.int64))

@@ -12,6 +12,7 @@ import ctypes
 import numpy as np
 import pytest
 
+import yatfhe_amd as yg
 from tests.oracle_py import P, lib, ref_digits_lib
 
 THREADS = 8
@@ -93,3 +94,30 @@ def test_zero_mean_invariants(l, b):
     mask = xs != -(1 << 31)
     lib().or_digits_row(P(np.ascontiguousarray(-xs)), xs.size, l, b, w, P(neg))
     assert np.array_equal(neg[:, mask], -out[:, mask])
+
+
+def test_ks_digits_base8_match_reference_and_reach_plus_128():
+    """At keyswitch base 2^8 (setKsRadixBits(8), legal in the reference) a
+    balanced digit can be +128, which no signed byte holds: the oracle must
+    agree with the reference's own decomposition there, and the engine, whose
+    keyswitch GEMM takes signed-byte digits, must refuse the base."""
+    ref = ref_digits_lib(32)
+    xs = [(1 << 31) - 1, (1 << 31) - 2, 0x7F800000, 0x7FFFFFFF - (1 << 20), 0x00800000, -(1 << 31), -1, 12345]
+    rng = np.random.default_rng(6)
+    xs += list(rng.integers(-(1 << 31), 1 << 31, 2000))
+    a = np.zeros(4, np.int64)
+    bb = np.zeros(4, np.int64)
+    seen = False
+    for x in xs:
+        lib().or_digits_scalar(int(x), 4, 8, 32, P(a))
+        seen |= bool((a == 128).any())
+        assert np.abs(a).max() <= 128
+        if ref is not None:
+            ref.ref_decompose_ks(int(x), 4, 8, 32, P(bb))
+            assert np.array_equal(a, bb), x
+    assert seen, "no input produced the digit +128"
+    h = ctypes.c_void_p()
+    cp = yg.replace(yg.PARAMS_T32_N1024, ks_radix_bits=8, ks_level=4).c()
+    L = yg.load_library()
+    assert L.ygpu_init(ctypes.byref(cp), 0, ctypes.byref(h)) == -2
+    assert b"ks_radix_bits" in L.ygpu_last_error()

@@ -23,6 +23,8 @@ Q_49P_T32 = 562941363486721  # torus.h:151
 Q_61P_T56 = 1945555039024054273  # torus.h:157
 
 YGPU_OK = 0
+YGPU_ERR_BUSY = -7
+YGPU_ERR_DEVICE = -10
 
 
 class YgpuError(RuntimeError):
@@ -198,6 +200,7 @@ def load_library(path: str | None = None):
         "ygpu_graph_launch": (c_int, [c_vp, c_i32]),
         "ygpu_graph_destroy": (c_int, [c_vp, c_i32]),
         "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
+        "ygpu_set_barrier_polls": (c_int, [c_vp, c_i32]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_classic": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_succinct": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
@@ -249,7 +252,7 @@ def exported_symbols():
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
-        "ygpu_abi_struct_sizes",
+        "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls",
     ]
 
 
@@ -695,6 +698,11 @@ class Context:
 
     def graph_destroy(self, graph_id: int):
         _check(self.lib.ygpu_graph_destroy(self.h, graph_id))
+
+    def set_barrier_polls(self, polls: int):
+        """Bound of the blind-rotation kernels' team-barrier waits (0: every
+        team barrier fails at once, to exercise the YGPU_ERR_DEVICE report)."""
+        _check(self.lib.ygpu_set_barrier_polls(self.h, polls))
 
     def copy_device(self, d_dst, d_src, nbytes: int):
         _check(self.lib.ygpu_copy_device(self.h, d_dst.value, d_src.value, nbytes))

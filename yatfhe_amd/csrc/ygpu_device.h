@@ -120,34 +120,41 @@ struct WorkgroupSync {
 // adds one after the wave's LDS writes have completed; the wave then waits
 // until the counter reaches W times the number of barriers it has passed.
 // Every team member passes the same sequence of barriers, so the counts agree.
-// The wait is bounded: a wave that polls POLLS times without the count
+// The wait is bounded: a wave that has polled `polls` times without the count
 // arriving sets *fail, and from then on no barrier waits, so a miscount ends
-// in wrong results and a set flag, never in a hang.
-#ifndef YGPU_TEAM_BARRIER_POLLS
-#define YGPU_TEAM_BARRIER_POLLS (1 << 16)
-#endif
-template <int W, int POLLS = YGPU_TEAM_BARRIER_POLLS>
+// in wrong results and a set flag, never in a hang (polls = 0 fails every
+// barrier at once).  The kernel reports the flag (report_barrier_failure).
+constexpr int TEAM_BARRIER_POLLS = 1 << 16;
+template <int W>
 struct TeamBarrier {
     uint32_t* ctr;
     uint32_t* fail;
     uint32_t passed;
+    int polls;
     __device__ __forceinline__ void sync() {
         passed += W;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         for (int poll = 0;; poll++) {
-            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= passed) break;
-            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
-            if (poll >= POLLS) {
+            if (poll >= polls) {
                 __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 break;
             }
+            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= passed) break;
+            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
             __builtin_amdgcn_s_sleep(1);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
 };
+
+// A workgroup whose team barrier timed out reports it to the host: one lane
+// stores 1 into the launch's status word (host-visible memory; every failing
+// workgroup stores the same value, so no atomic is needed).
+__device__ __forceinline__ void report_barrier_failure(bool failed, uint32_t* status) {
+    if (failed && threadIdx.x == 0 && status) *reinterpret_cast<volatile uint32_t*>(status) = 1u;
+}
 
 // Cooley-Tukey pass (forward): U, V -> U + wV, U - wV.
 template <int LOGN, int P>

@@ -48,7 +48,9 @@ struct Timing {
     hipEvent_t a, b;
 };
 
-// A captured blind-rotate -> extract -> keyswitch pipeline with its own scratch.
+// A captured blind-rotate -> extract -> keyswitch pipeline with its own
+// scratch; `keys` are the key slots its kernels read (they cannot be released
+// while the graph exists).
 struct GraphRec {
     bool live = false;
     hipGraph_t graph = nullptr;
@@ -56,6 +58,7 @@ struct GraphRec {
     void* d_acc = nullptr;
     int32_t* d_partial = nullptr;
     int32_t* d_slots = nullptr;
+    std::vector<int32_t> keys;
 };
 
 inline void free_graph(GraphRec& g) {
@@ -86,9 +89,16 @@ struct ygpu_ctx {
     PrimeF64 prime[2]{};
     CrtConst crt{};
     std::vector<KeyRec> keys;
+    // device key tables (BSK / KSK base per key slot), allocated once at init
+    // with KEY_SLOTS entries: captured graphs hold their addresses
     const double** d_bsk_table = nullptr;
     const int32_t** d_ksk_table = nullptr;
-    int table_cap = 0;
+    // failure report of the team barriers (pinned host word and its device
+    // address): set by a kernel whose bounded barrier wait ran out, read and
+    // cleared after each synchronisation
+    uint32_t* h_status = nullptr;
+    uint32_t* hd_status = nullptr;
+    int barrier_polls = TEAM_BARRIER_POLLS;
     void* d_acc = nullptr;
     size_t acc_cap = 0;
     int32_t* d_partial = nullptr;
@@ -100,6 +110,7 @@ struct ygpu_ctx {
     int32_t* d_out1 = nullptr;
     int32_t* d_ext1 = nullptr;
     void* d_acc1 = nullptr;
+    void* d_accin1 = nullptr;  // caller's initial accumulator (ygpu_blind_rotate_scaled)
     void* d_stage = nullptr;   // packed key staging for uploads
     size_t stage_cap = 0;
     std::mutex mu;
@@ -127,7 +138,11 @@ inline int check_params(const ygpu_params& p) {
                                               " bits do not fit the " + std::to_string(p.torus_bits) + "-bit torus");
     if (p.lwe_q_bits != 32) return fail(YGPU_ERR_UNSUPPORTED, "qLwe must be 2^32");
     if (p.ks_width_bits < 1 || p.ks_width_bits > 32) return fail(YGPU_ERR_UNSUPPORTED, "ks_width_bits must be in [1, 32]");
-    if (p.ks_radix_bits < 1 || p.ks_radix_bits > 8) return fail(YGPU_ERR_UNSUPPORTED, "ks_radix_bits must be in [1, 8]");
+    // the keyswitch GEMM takes its digits as signed bytes: a balanced digit of
+    // base 2^b lies in [-2^(b-1), 2^(b-1)], and +128 (b = 8) has no int8 form
+    if (p.ks_radix_bits < 1 || p.ks_radix_bits > 7)
+        return fail(YGPU_ERR_UNSUPPORTED, "ks_radix_bits must be in [1, 7]: the keyswitch digits travel as signed bytes, "
+                                          "and base 2^8 yields the digit +128");
     if (p.ks_level < 1 || p.ks_level > KS_MAX_LEVEL) return fail(YGPU_ERR_UNSUPPORTED, "ks_level must be in [1, 16]");
     if (p.ks_level * p.ks_radix_bits > p.ks_width_bits) return fail(YGPU_ERR_INVALID_ARG, "keyswitch digits exceed ks_width_bits");
     if (!is_prime_u64(p.q_ntt)) return fail(YGPU_ERR_INVALID_ARG, "q_ntt is not prime");
@@ -200,25 +215,39 @@ inline int ensure(void** buf, size_t* cap, size_t bytes) {
     return YGPU_OK;
 }
 
-inline int upload_tables(ygpu_ctx* ctx) {
-    const int cap = static_cast<int>(ctx->keys.size());
-    if (cap > ctx->table_cap) {
-        if (ctx->d_bsk_table) hipFree(ctx->d_bsk_table);
-        if (ctx->d_ksk_table) hipFree(ctx->d_ksk_table);
-        const int newCap = std::max(16, cap * 2);
-        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_bsk_table), sizeof(double*) * newCap));
-        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ksk_table), sizeof(int32_t*) * newCap));
-        ctx->table_cap = newCap;
-    }
-    std::vector<const double*> b(ctx->table_cap, nullptr);
-    std::vector<const int32_t*> k(ctx->table_cap, nullptr);
-    for (size_t i = 0; i < ctx->keys.size(); i++) {
-        b[i] = ctx->keys[i].d_bsk;
-        k[i] = ctx->keys[i].d_ksk;
-    }
-    YG_HIP(hipMemcpy(ctx->d_bsk_table, b.data(), sizeof(double*) * ctx->table_cap, hipMemcpyHostToDevice));
-    YG_HIP(hipMemcpy(ctx->d_ksk_table, k.data(), sizeof(int32_t*) * ctx->table_cap, hipMemcpyHostToDevice));
+// Key slots per context: the device key tables never move, so graphs that
+// captured their addresses stay valid whatever keys are uploaded later.
+constexpr int KEY_SLOTS = 4096;
+
+// Writes key slot `id`'s table entries (the key's device bases, or null for a
+// released slot) on the context stream.  A kernel in flight reads only the
+// slots of its own keys, which are live.
+inline int set_table_slot(ygpu_ctx* ctx, int id) {
+    if (id < 0 || id >= KEY_SLOTS) return fail(YGPU_ERR_INVALID_ARG, "key slot out of range");
+    const KeyRec& k = ctx->keys[id];
+    const double* b = k.live ? k.d_bsk : nullptr;
+    const int32_t* q = k.live ? k.d_ksk : nullptr;
+    YG_HIP(hipMemcpyAsync(ctx->d_bsk_table + id, &b, sizeof(b), hipMemcpyHostToDevice, ctx->stream));
+    YG_HIP(hipMemcpyAsync(ctx->d_ksk_table + id, &q, sizeof(q), hipMemcpyHostToDevice, ctx->stream));
+    YG_HIP(hipStreamSynchronize(ctx->stream));
     return YGPU_OK;
+}
+
+// Reads and clears the context's barrier-failure report (after the stream it
+// covers has been synchronised).
+inline int check_status(ygpu_ctx* ctx) {
+    if (!ctx->h_status) return YGPU_OK;
+    volatile uint32_t* st = ctx->h_status;
+    if (*st == 0u) return YGPU_OK;
+    *st = 0u;
+    return fail(YGPU_ERR_DEVICE, "a blind-rotation workgroup's team barrier timed out: the results of the calls since the "
+                                 "last synchronisation are invalid (their accumulators read the most negative torus value)");
+}
+
+// Synchronises the context stream and reports a device-side failure.
+inline int sync_and_check(ygpu_ctx* ctx) {
+    YG_HIP(hipStreamSynchronize(ctx->stream));
+    return check_status(ctx);
 }
 
 inline int key_live(const ygpu_ctx* ctx, int32_t key_id) {
@@ -251,6 +280,8 @@ inline BrArgs br_args(ygpu_ctx* ctx, int count, const int32_t* d_lwe, const void
     a.prime[0] = ctx->prime[0];
     a.prime[1] = ctx->prime[1];
     a.crt = ctx->crt;
+    a.status = ctx->hd_status;
+    a.barrier_polls = ctx->barrier_polls;
     return a;
 }
 
@@ -365,6 +396,7 @@ inline int plain_buffers(ygpu_ctx* ctx) {
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_out1), sizeof(int32_t) * (ctx->p.n + 1)));
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ext1), sizeof(int32_t) * (ctx->p.k * ctx->p.N + 1)));
     YG_HIP(hipMalloc(&ctx->d_acc1, tb * (ctx->p.k + 1) * ctx->p.N));
+    YG_HIP(hipMalloc(&ctx->d_accin1, tb * (ctx->p.k + 1) * ctx->p.N));
     return YGPU_OK;
 }
 
@@ -606,11 +638,5 @@ inline size_t key_device_bytes(const ygpu_ctx* ctx) {
     return bsk + (rows + 3) / 4 * 4 * (static_cast<size_t>(p.n) + 1) * sizeof(uint32_t);
 }
 
-// Grows the device key tables to at least `slots` entries (synchronous; no
-// kernel may be using the tables).
-inline int reserve_slots(ygpu_ctx* ctx, int slots) {
-    if (static_cast<int>(ctx->keys.size()) < slots) ctx->keys.resize(slots);
-    return upload_tables(ctx);
-}
 
 }  // namespace ygpu_rt

@@ -63,6 +63,12 @@ struct BrArgs {
     const double* tw;                 // [NP][2][N]: forward then inverse powers
     PrimeF64 prime[2];
     CrtConst crt;
+    // Failure report of the bounded team barriers: a workgroup whose barrier
+    // wait ran out writes 1 here (host-visible memory) and poisons its
+    // accumulators; barrier_polls bounds every wait (0: every team barrier
+    // fails at once without waiting, which exercises the report path).
+    uint32_t* status;
+    int barrier_polls;
 };
 
 constexpr int KS_THREADS = 256;

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // team's two waves; hand-overs between teams keep workgroup barriers
     using TeamSync = typename std::conditional<TW == 2, TeamBarrier<2>, WorkgroupSync<TW>>::type;
     TeamSync ts{};
-    if constexpr (TW == 2) ts = TeamSync{&teamCount[lc * 4 + tt], &teamCount[CPW * 4], 0u};
+    if constexpr (TW == 2) ts = TeamSync{&teamCount[lc * 4 + tt], &teamCount[CPW * 4], 0u, a.barrier_polls};
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
     // the next step's rotation amount is loaded a step ahead, so no step
@@ -254,11 +254,14 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         __syncthreads();
     }
 
+    // a team barrier that timed out (TeamBarrier) leaves the step order
+    // unproven: every accumulator of the workgroup reads INT64_MIN instead and
+    // the launch's status word reports it (the step loop ends with a workgroup
+    // barrier, so the flag is final here)
+    const bool failed = teamCount[CPW * 4] != 0u;
+    report_barrier_failure(failed, a.status);
     if (live && tp == 0) {
         TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + tm * N;
-        // a team barrier that timed out (TeamBarrier) leaves the step order
-        // unproven: every accumulator of the workgroup reads INT64_MIN instead
-        const bool failed = teamCount[CPW * 4] != 0u;
 #pragma unroll
         for (int r = 0; r < 16; r++) outAcc[tl + T * r] = failed ? INT64_MIN : acc[tl + T * r];
     }

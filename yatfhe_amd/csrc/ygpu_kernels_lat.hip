@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     const int tl = static_cast<int>(threadIdx.x) % 64;
     double* scr = scratch[w];
     if (threadIdx.x < K1 + 1) rowCount[threadIdx.x] = 0u;
-    TeamBarrier<L> rowSync{&rowCount[r], &rowCount[K1], 0u};
+    TeamBarrier<L> rowSync{&rowCount[r], &rowCount[K1], 0u, a.barrier_polls};
 
     const int n = a.n;
     const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
@@ -182,12 +182,14 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
         __syncthreads();
     }
 
+    // a row barrier that timed out (TeamBarrier) leaves the step order
+    // unproven: the accumulators read INT32_MIN instead and the launch's
+    // status word reports it (the step loop ends with a workgroup barrier, so
+    // the flag is final here)
+    const bool failed = rowCount[K1] != 0u;
+    report_barrier_failure(failed, a.status);
     if (lvl == 0) {
         int32_t* __restrict__ outAcc = static_cast<int32_t*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + r * N;
-        // a row barrier that timed out (TeamBarrier) leaves the step order
-        // unproven: the accumulators read INT32_MIN instead (the step loop
-        // ends with a workgroup barrier, so the flag is final here)
-        const bool failed = rowCount[K1] != 0u;
         for (int q = 0; q < 16; q++) outAcc[tl + T * q] = failed ? INT32_MIN : acc[r][tl + T * q];
     }
 }

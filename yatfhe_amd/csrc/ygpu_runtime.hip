@@ -87,6 +87,21 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
     }
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_tw), sizeof(double) * tw.size()));
     YG_HIP(hipMemcpy(ctx->d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    // key tables of fixed capacity, all slots empty
+    {
+        std::vector<const void*> empty(KEY_SLOTS, nullptr);
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_bsk_table), sizeof(void*) * KEY_SLOTS));
+        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ksk_table), sizeof(void*) * KEY_SLOTS));
+        YG_HIP(hipMemcpy(ctx->d_bsk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice));
+        YG_HIP(hipMemcpy(ctx->d_ksk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice));
+    }
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_status), 64, hipHostMallocDefault));
+    *ctx->h_status = 0u;
+    {
+        void* dp = nullptr;
+        YG_HIP(hipHostGetDevicePointer(&dp, ctx->h_status, 0));
+        ctx->hd_status = static_cast<uint32_t*>(dp);
+    }
     *out = ctx.release();
     return YGPU_OK;
 }
@@ -105,8 +120,9 @@ int ygpu_destroy(ygpu_ctx* ctx) {
         for (void* p : {static_cast<void*>(ctx->d_tw), static_cast<void*>(ctx->d_bsk_table), static_cast<void*>(ctx->d_ksk_table),
                         ctx->d_acc, static_cast<void*>(ctx->d_partial), static_cast<void*>(ctx->d_slots),
                         static_cast<void*>(ctx->d_lwe1), ctx->d_v1, static_cast<void*>(ctx->d_out1),
-                        static_cast<void*>(ctx->d_ext1), ctx->d_acc1, ctx->d_stage})
+                        static_cast<void*>(ctx->d_ext1), ctx->d_acc1, ctx->d_accin1, ctx->d_stage})
             if (p) hipFree(p);
+        if (ctx->h_status) hipHostFree(ctx->h_status);
         for (auto& t : ctx->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
         for (auto e : ctx->event_pool) hipEventDestroy(e);
         hipStreamDestroy(ctx->stream);
@@ -153,10 +169,24 @@ int upload_host_form(ygpu_ctx* ctx, const HostKeyForm& hk, int32_t* key_id) {
     }
     int id = -1;
     for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
-    if (id < 0) { id = static_cast<int>(ctx->keys.size()); ctx->keys.push_back(rec); }
-    else ctx->keys[id] = rec;
-    rc = upload_tables(ctx);
-    if (rc) return rc;
+    if (id < 0) {
+        if (static_cast<int>(ctx->keys.size()) >= KEY_SLOTS) {
+            hipFree(rec.d_bsk);
+            hipFree(rec.d_ksk);
+            return fail(YGPU_ERR_UNSUPPORTED, "a context holds at most " + std::to_string(KEY_SLOTS) + " keys");
+        }
+        id = static_cast<int>(ctx->keys.size());
+        ctx->keys.push_back(rec);
+    } else {
+        ctx->keys[id] = rec;
+    }
+    rc = set_table_slot(ctx, id);
+    if (rc) {
+        hipFree(ctx->keys[id].d_bsk);
+        hipFree(ctx->keys[id].d_ksk);
+        ctx->keys[id] = KeyRec{};
+        return rc;
+    }
     *key_id = id;
     return YGPU_OK;
 }
@@ -188,12 +218,15 @@ int ygpu_key_release(ygpu_ctx* ctx, int32_t key_id) {
     return with_ctx(ctx, [&]() -> int {
         int rc = key_live(ctx, key_id);
         if (rc) return rc;
+        for (size_t g = 0; g < ctx->graphs.size(); g++)
+            if (ctx->graphs[g].live && std::find(ctx->graphs[g].keys.begin(), ctx->graphs[g].keys.end(), key_id) != ctx->graphs[g].keys.end())
+                return fail(YGPU_ERR_BUSY, "key " + std::to_string(key_id) + " is used by graph " + std::to_string(g) + ": destroy it first");
         YG_HIP(hipStreamSynchronize(ctx->stream));
         KeyRec& k = ctx->keys[key_id];
         hipFree(k.d_bsk);
         hipFree(k.d_ksk);
         k = KeyRec{};
-        return upload_tables(ctx);
+        return set_table_slot(ctx, key_id);
     });
 }
 
@@ -249,8 +282,7 @@ int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, in
         rc = enqueue_jobs(ctx, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc, ctx->d_slots, ctx->d_partial);
         if (rc) return rc;
         // the host slot vector dies with this call: finish the copy first
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
@@ -290,8 +322,7 @@ int ygpu_pbs(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v
         YG_HIP(hipMemcpyAsync(lwe_out, ctx->d_out1, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, ctx->stream));
         if (acc_out) YG_HIP(hipMemcpyAsync(acc_out, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
         if (extract_out) YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
@@ -310,8 +341,7 @@ int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, cons
         rc = run_br(ctx, plain_tile(ctx), a);
         if (rc) return rc;
         YG_HIP(hipMemcpyAsync(acc_out, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
@@ -328,19 +358,15 @@ int ygpu_blind_rotate_scaled(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar,
             if (abar[i] < -twoN || abar[i] >= twoN) return fail(YGPU_ERR_INVALID_ARG, "abar out of [-2N, 2N)");
         const size_t tb = ctx->torus_bytes();
         YG_HIP(hipMemcpyAsync(ctx->d_lwe1, abar, sizeof(int32_t) * p.n, hipMemcpyHostToDevice, ctx->stream));
-        // the initial accumulator rides in the output buffer, read before written per lane
-        void* d_in = nullptr;
-        YG_HIP(hipMalloc(&d_in, tb * (p.k + 1) * p.N));
-        YG_HIP(hipMemcpyAsync(d_in, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
+        // the caller's initial accumulator, in a buffer of its own
+        YG_HIP(hipMemcpyAsync(ctx->d_accin1, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
         BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
         a.prescaled = 1;
-        a.acc_in = d_in;
+        a.acc_in = ctx->d_accin1;
         rc = run_br(ctx, plain_tile(ctx), a);
-        if (rc) { hipFree(d_in); return rc; }
+        if (rc) return rc;
         YG_HIP(hipMemcpyAsync(acc, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        YG_HIP(hipFree(d_in));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
@@ -355,8 +381,7 @@ int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out) {
         KsArgs a = ks_args(ctx, 1, ctx->d_acc1, nullptr, nullptr, nullptr, nullptr);
         YG_HIP(timed(ctx, "k_extract", ctx->stream, [&] { return launch_extract(ctx->tbits, a, ctx->d_ext1, ctx->stream); }));
         YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
@@ -396,8 +421,7 @@ int ygpu_extract_keyswitch(ygpu_ctx* ctx, int32_t key_id, const void* acc, int32
         if (rc) return rc;
         YG_HIP(hipMemcpyAsync(lwe_out, ctx->d_out1, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, ctx->stream));
         if (extract_out) YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
@@ -429,15 +453,24 @@ int ygpu_memcpy_htod(ygpu_ctx* ctx, void* d_dst, const void* src, size_t bytes) 
 int ygpu_memcpy_dtoh(ygpu_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
     return with_ctx(ctx, [&]() -> int {
         YG_HIP(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        return YGPU_OK;
+        return sync_and_check(ctx);
     });
 }
 
 int ygpu_sync(ygpu_ctx* ctx) {
     return with_ctx(ctx, [&]() -> int {
         YG_HIP(hipStreamSynchronize(ctx->stream));
-        return ctx->timing ? resolve_timing(ctx) : YGPU_OK;
+        const int rc = ctx->timing ? resolve_timing(ctx) : YGPU_OK;
+        const int st = check_status(ctx);
+        return st ? st : rc;
+    });
+}
+
+int ygpu_set_barrier_polls(ygpu_ctx* ctx, int32_t polls) {
+    if (polls < 0) return fail(YGPU_ERR_INVALID_ARG, "polls must be >= 0");
+    return with_ctx(ctx, [&]() -> int {
+        ctx->barrier_polls = polls > 0 ? polls : 0;
+        return YGPU_OK;
     });
 }
 
@@ -600,6 +633,8 @@ int ygpu_graph_create(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64
             YG_HIP(hipMalloc(&g.d_acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(total)));
             acc = g.d_acc;
         }
+        for (int j = 0; j < n_jobs; j++)
+            if (std::find(g.keys.begin(), g.keys.end(), jobs[j].key_id) == g.keys.end()) g.keys.push_back(jobs[j].key_id);
         rc = capture_graph(ctx, g, [&]() {
             return enqueue_jobs(ctx, static_cast<int>(total), d_lwe_in, d_v, v_per_ct, d_lwe_out, acc,
                                 g.d_slots, g.d_partial);

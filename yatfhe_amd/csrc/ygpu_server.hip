@@ -100,11 +100,13 @@ struct Lane {
     void* h_v = nullptr;
     int32_t* h_out = nullptr;
     int32_t* h_slots = nullptr;
+    uint32_t* h_status = nullptr;  // the lane's barrier-failure report (pinned)
     // device addresses of the pinned staging buffers (mapped host memory)
     const int32_t* hd_in = nullptr;
     const void* hd_v = nullptr;
     int32_t* hd_out = nullptr;
     const int32_t* hd_slots = nullptr;
+    uint32_t* hd_status = nullptr;
     int32_t* d_in = nullptr;
     void* d_v = nullptr;
     int32_t* d_out = nullptr;
@@ -343,6 +345,8 @@ int lane_alloc(const ygpu_server* s, PSet& ps, Lane& L) {
     YG_HIP(hipHostMalloc(&L.h_v, B * ps.tbytes * p.N, hipHostMallocDefault));
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_out), B * lwe, hipHostMallocDefault));
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_slots), B * sizeof(int32_t), hipHostMallocDefault));
+    YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&L.h_status), 64, hipHostMallocDefault));
+    *L.h_status = 0u;
     void* dp = nullptr;
     YG_HIP(hipHostGetDevicePointer(&dp, L.h_in, 0));
     L.hd_in = static_cast<const int32_t*>(dp);
@@ -352,6 +356,8 @@ int lane_alloc(const ygpu_server* s, PSet& ps, Lane& L) {
     L.hd_out = static_cast<int32_t*>(dp);
     YG_HIP(hipHostGetDevicePointer(&dp, L.h_slots, 0));
     L.hd_slots = static_cast<const int32_t*>(dp);
+    YG_HIP(hipHostGetDevicePointer(&dp, L.h_status, 0));
+    L.hd_status = static_cast<uint32_t*>(dp);
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_in), B * lwe));
     YG_HIP(hipMalloc(&L.d_v, B * ps.tbytes * p.N));
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&L.d_out), B * lwe));
@@ -364,7 +370,8 @@ int lane_alloc(const ygpu_server* s, PSet& ps, Lane& L) {
 void lane_free(Lane& L) {
     if (L.stream) hipStreamDestroy(L.stream);
     if (L.done) hipEventDestroy(L.done);
-    for (void* h : {static_cast<void*>(L.h_in), L.h_v, static_cast<void*>(L.h_out), static_cast<void*>(L.h_slots)})
+    for (void* h : {static_cast<void*>(L.h_in), L.h_v, static_cast<void*>(L.h_out), static_cast<void*>(L.h_slots),
+                    static_cast<void*>(L.h_status)})
         if (h) hipHostFree(h);
     for (void* d : {static_cast<void*>(L.d_in), L.d_v, static_cast<void*>(L.d_out), L.d_acc, static_cast<void*>(L.d_slots),
                     static_cast<void*>(L.d_partial)})
@@ -390,6 +397,7 @@ int launch_lane(ygpu_server* s, PSet& ps, Lane& L) {
     YG_HIP(launch_copy_words(L.hd_v, L.d_v, count * vbytes / 4, L.stream));
     YG_HIP(launch_copy_words(L.hd_slots, L.d_slots, count, L.stream));
     BrArgs a = br_args(ps.ctx, count, L.d_in, L.d_v, p.N, L.d_acc, s->d_bsk_tab, L.d_slots);
+    a.status = L.hd_status;  // this batch's own failure report
     int rc = run_br(ps.ctx, batch_tile(ps.ctx, count), a, L.stream);
     if (rc) return rc;
     // one keyswitch launch; the kernel follows the per-ciphertext tenant slots
@@ -638,7 +646,12 @@ void completer_loop(ygpu_server* s, PSet* ps) {
         int rc = L.rc;
         if (rc == 0) {
             const hipError_t e = hipEventSynchronize(L.done);
-            if (e != hipSuccess) rc = server_fail_hip(e, "batch");
+            if (e != hipSuccess) {
+                rc = server_fail_hip(e, "batch");
+            } else if (*reinterpret_cast<volatile uint32_t*>(L.h_status) != 0u) {
+                *reinterpret_cast<volatile uint32_t*>(L.h_status) = 0u;
+                rc = fail(YGPU_ERR_DEVICE, "a team barrier of the batch's blind rotation timed out: its results are invalid");
+            }
         } else {
             hipStreamSynchronize(L.stream);
         }
@@ -824,6 +837,13 @@ int add_tenant_form(ygpu_server* s, int32_t pset, int32_t* tenant, MakeForm&& ma
         return server_fail_hip(e, "event");
     }
     std::lock_guard<std::mutex> g(s->mu);
+    // another registration may have taken the last slot meanwhile
+    if (static_cast<int>(s->tenants.size()) >= s->cfg.max_tenants) {
+        hipHostFree(f.packed);
+        hipHostFree(f.planes);
+        hipEventDestroy(t.loaded);
+        return fail(YGPU_ERR_INVALID_ARG, "max_tenants reached");
+    }
     t.form = static_cast<int>(s->forms.size());
     s->forms.push_back(f);
     *tenant = static_cast<int32_t>(s->tenants.size());
