@@ -4,19 +4,25 @@
 A step is one batched PBS (blind rotate -> sample extract -> keyswitch) of
 4096 ciphertexts under one key at TORUS32 n=630, k=1, N=1024, gadget 2^7 x 3,
 keyswitch 2^2 x 8, with inputs resident in device memory; `value` is its
-throughput.  Prints one JSON line.
+throughput.  The last line of stdout is one compact JSON object (a few KB); the full
+record of every leg goes to gpurun_out/bench_detail.json (--detail).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs a,b,...]
 
 Besides the batched value the line carries
-  - the plain one-ciphertext-per-call form and its latency percentiles,
-  - the roofline of the blind-rotation kernel against the measured modmul peak
-    of this GPU (and, for the plain form, against HBM bandwidth),
-  - `configs`: TORUS56 n=742 N=2048 (8192 ciphertexts, random 16-entry LUT,
-    autotuned tile); 64 tenants with Zipf(1.1) ciphertext counts in one
-    key-grouped hipGraph launch; and the serving loop (Poisson arrivals swept
-    1k..50k PBS/s over 256 Zipf(1.1) tenants, 70 % TORUS32 / 30 % TORUS56,
-    keys under a 12 GB device budget with pinned-host spill) with p50/p99,
+  - the roofline of the blind-rotation kernel: its algorithmic modular
+    products per launch over the launch time (HIP events on its stream),
+    against the ceiling MI355X's published FP64 rate implies for an exact
+    modular product (6 binary64 instructions), and beside it against the
+    modmul peak a microbenchmark measures on the same GPU,
+  - the plain one-ciphertext-per-call form on device buffers and through the
+    host-buffer drop-in call, with latency percentiles,
+  - one summary per further workload: TORUS56 n=742 N=2048 (8192
+    ciphertexts, random 16-entry LUT, autotuned tile); 64 tenants with
+    Zipf(1.1) ciphertext counts in one key-grouped hipGraph launch; the
+    serving loop (Poisson arrivals swept 1k..50k PBS/s over 256 Zipf(1.1)
+    tenants, 70 % TORUS32 / 30 % TORUS56, keys under a device budget with
+    pinned-host spill) with p50/p99,
   - a CPU baseline: the oracle restating the reference's algorithm, timed on a
     bounded sample on the host cores of one CCD.
 """
@@ -47,6 +53,11 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s (public spec)
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_blind_rotate_t32_n1024_b4096_traffic.txt")
 SERVING_RATES = [1000, 2000, 5000, 10000, 20000, 30000, 40000, 50000]
 FP64_PEAK_TFLOPS = 78.6  # MI355X published dense FP64 (vector and matrix)
+# An exact modular product is 6 binary64 VALU instructions (mul, fma, mul,
+# rndne, fma, add), each issued at the FP64 FMA rate (2 flop per issue).
+INSTR_PER_MODMUL = 6
+MODMUL_CEILING = FP64_PEAK_TFLOPS * 1e12 / 2 / INSTR_PER_MODMUL  # 6.55e12 modmul/s
+DETAIL_PATH = os.path.join(ROOT, "gpurun_out", "bench_detail.json")
 LEGS = ("t32_batch", "t56_batch", "multitenant", "serving", "keyload")
 
 
@@ -212,6 +223,13 @@ def leg_t32_batch(yg, args, peaks):
             ctx.pbs_plain_device(kid, d1, d_v, o1)
             ctx.sync()
             lat.append(time.perf_counter() - t)
+        # the drop-in call itself (ygpu_pbs): host buffers in and out
+        lat_host = []
+        ctx.pbs(kid, cts[0], v)
+        for c in range(args.plain_calls):
+            t = time.perf_counter()
+            ctx.pbs(kid, cts[c], v)
+            lat_host.append(time.perf_counter() - t)
         # calibration copy (for the HBM counter runs of the same command)
         cbytes = 256 << 20
         ca = ctx.alloc(cbytes)
@@ -221,7 +239,7 @@ def leg_t32_batch(yg, args, peaks):
         for b in (ca, cb, d1, o1, d_out, d_v, d_lwe):
             b.free()
     res.update(value=args.steps * BATCH / dt, dt=dt, br_avg_s=br_ms / max(1, br_n) / 1e3,
-               ks_avg_ms=ks_ms / max(1, ks_n), plain_lat=lat)
+               ks_avg_ms=ks_ms / max(1, ks_n), plain_lat=lat, plain_host_lat=lat_host)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(p, keys, cts, v)
     return p, res
@@ -266,6 +284,12 @@ def leg_t56_batch(yg, args, peak):
             ctx.sync()
             lat.append(time.perf_counter() - t)
         lat = lat[1:]
+        lat_host = []
+        for c in range(max(2, args.plain_calls // 10)):
+            t = time.perf_counter()
+            ctx.pbs(kid, cts[c], v)
+            lat_host.append(time.perf_counter() - t)
+        lat_host = lat_host[1:]
         for b in (d_out, d_v, d_lwe, d1, o1):
             b.free()
     ph = yg.lwe_decrypt_phase(out, keys.lwe_key).astype(np.int64)
@@ -278,13 +302,17 @@ def leg_t56_batch(yg, args, peak):
                     "(seed 3), random 16-entry LUT (seed 2), one key (seed 0)",
         "value": round(value, 1), "unit": "PBS/s", "steps": steps, "ms_per_step": round(dt / steps * 1e3, 2),
         "tile": {"cpw": cpw, "twl": twl, "autotune_best_ms": round(tune_ms, 2)},
-        "roofline": {"bound": "valu-modmul", "achieved": round(mm * BATCH_T56 / br_s / 1e9, 2), "peak": round(peak / 1e9, 2),
-                     "unit": "Gmodmul/s", "frac": round(mm * BATCH_T56 / br_s / peak, 4),
+        "roofline": {"bound": "fp64-valu", "achieved": round(mm * BATCH_T56 / br_s / 1e9, 2),
+                     "peak": round(MODMUL_CEILING / 1e9, 2), "unit": "Gmodmul/s",
+                     "frac": round(mm * BATCH_T56 / br_s / MODMUL_CEILING, 4),
+                     "frac_binary64_products": round(2 * mm * BATCH_T56 / br_s / MODMUL_CEILING, 4),
+                     "frac_measured_peak": round(mm * BATCH_T56 / br_s / peak, 4),
                      "per_unit": f"{mm} modmul (mod q56) per PBS; each is two binary64 products, one per CRT prime",
                      "avg_launch_ms": round(br_s * 1e3, 2)},
         "lut_decrypt_check": ok,
         "plain": {"value": round(len(lat) / sum(lat), 2), "unit": "PBS/s", "calls": len(lat),
-                  "p50_ms": round(pct(lat, 50) * 1e3, 2)},
+                  "p50_ms": round(pct(lat, 50) * 1e3, 2), "p99_ms": round(pct(lat, 99) * 1e3, 2),
+                  "host_p50_ms": round(pct(lat_host, 50) * 1e3, 2)},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(p, keys, cts, v, per_thread=2, lat_runs=20),
     }
 
@@ -352,7 +380,8 @@ def leg_multitenant(yg, args, peak):
                     f"hipGraph-captured blind rotate -> extract -> keyswitch",
         "value": round(value, 1), "unit": "PBS/s", "steps": steps, "ms_per_step": round(dt / steps * 1e3, 2),
         "keys_resident_bytes": int(key_bytes),
-        "roofline": {"bound": "valu-modmul", "frac_whole_pipeline": round(modmul_per_pbs(p) * value / peak, 4)},
+        "roofline": {"bound": "fp64-valu", "frac_whole_pipeline": round(modmul_per_pbs(p) * value / MODMUL_CEILING, 4),
+                     "frac_whole_pipeline_measured_peak": round(modmul_per_pbs(p) * value / peak, 4)},
         "graph_equals_eager": bool(np.array_equal(out, eager)),
         "decrypt_check": ok,
     }
@@ -442,11 +471,12 @@ def leg_serving(yg, args, peak):
                                 warm=0, zipf_s=1.1, seed=5, slo_ms=100.0)
                 raw.append(srv.report(r))
                 sweep.append({k: (round(v, 3) if isinstance(v, float) else v) for k, v in r.items() if k != "load"})
-                print(f"serving {form} keys, {sched}, budget {budget_gb} GiB rate {rate}: {r['throughput_rps']:.0f} PBS/s "
-                      f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms (T32 p50 {r['pset_p50_ms'][0]:.1f}, T56 p50 "
-                      f"{r['pset_p50_ms'][1]:.1f}) hit {r['hit_rate']:.3f} batch {r['mean_batch']:.1f} locality "
-                      f"{r['locality']:.2f} spilled {r['spilled']} stolen {r['stolen']} shed {r['shed']} mismatches "
-                      f"{r['mismatches']}", file=sys.stderr, flush=True)
+                if args.verbose:
+                    print(f"serving {form} keys, {sched}, budget {budget_gb} GiB rate {rate}: {r['throughput_rps']:.0f} PBS/s "
+                          f"p50 {r['p50_ms']:.2f} ms p99 {r['p99_ms']:.2f} ms (T32 p50 {r['pset_p50_ms'][0]:.1f}, T56 p50 "
+                          f"{r['pset_p50_ms'][1]:.1f}) hit {r['hit_rate']:.3f} batch {r['mean_batch']:.1f} locality "
+                          f"{r['locality']:.2f} spilled {r['spilled']} stolen {r['stolen']} shed {r['shed']} mismatches "
+                          f"{r['mismatches']}", file=sys.stderr, flush=True)
             st = srv.stats()
         ok = [i for i, x in enumerate(sweep)
               if x["shed"] == 0 and x["errors"] == 0 and x["throughput_rps"] >= 0.9 * x["offered_rps"]]
@@ -460,12 +490,12 @@ def leg_serving(yg, args, peak):
             "keys_prefetched": prefetched,
             "saturation": None if top is None else {
                 "offered_rps": top["offered_rps"], "throughput_rps": top["throughput_rps"],
-                "roofline_frac": round(top["throughput_rps"] * mm_avg / peak, 4)},
+                "roofline_frac": round(top["throughput_rps"] * mm_avg / MODMUL_CEILING, 4)},
             "sweep": sweep,
             "max_sustained": None if best is None else {
                 "offered_rps": best["offered_rps"], "throughput_rps": best["throughput_rps"],
                 "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
-                "roofline_frac": round(best["throughput_rps"] * mm_avg / peak, 4)},
+                "roofline_frac": round(best["throughput_rps"] * mm_avg / MODMUL_CEILING, 4)},
             "all_results_bit_exact": all(x["mismatches"] == 0 and x["errors"] == 0 for x in sweep),
             "server_stats": st,
             # the highest sustained point as the reference serving benchmark's own record
@@ -530,8 +560,28 @@ def leg_keyload(yg, args):
             ctx.timing(False)
         row["transfer_ratio"] = round(row["succinct"]["transfer_bytes"] / row["full"]["transfer_bytes"], 4)
         out[name] = row
-        print(f"keyload {name}: full {row['full']}, succinct {row['succinct']}", file=sys.stderr, flush=True)
+        if args.verbose:
+            print(f"keyload {name}: full {row['full']}, succinct {row['succinct']}", file=sys.stderr, flush=True)
     return out
+
+
+def serving_summary(run):
+    """The figures of one serving run that fit the summary line."""
+    if not run:
+        return None
+    sweep = {int(x["offered_rps"]): x for x in run["sweep"]}
+    lo = sweep.get(1000) or (run["sweep"][0] if run["sweep"] else None)
+    ms = run["max_sustained"]
+    return {
+        "key_form": run["key_form"], "sched": run["sched"], "key_budget_gib": run["key_budget_gib"],
+        "max_sustained_rps": None if ms is None else ms["throughput_rps"],
+        "max_sustained_p50_ms": None if ms is None else ms["p50_ms"],
+        "max_sustained_p99_ms": None if ms is None else ms["p99_ms"],
+        "p50_ms_1k": None if lo is None else round(lo["p50_ms"], 2),
+        "p99_ms_1k": None if lo is None else round(lo["p99_ms"], 2),
+        "saturation_rps": None if run["saturation"] is None else run["saturation"]["throughput_rps"],
+        "bit_exact": run["all_results_bit_exact"],
+    }
 
 
 def main():
@@ -554,6 +604,8 @@ def main():
     ap.add_argument("--serving-sched-modes", default="affinity,soft",
                     help="extra serving runs (all keys resident) with these scheduling modes ('' for none)")
     ap.add_argument("--serving-sched-rates", default="5000,20000")
+    ap.add_argument("--detail", default=DETAIL_PATH, help="where the full record of every leg is written")
+    ap.add_argument("--verbose", action="store_true", help="progress of every serving rate on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "
@@ -572,8 +624,10 @@ def main():
 
     import yatfhe_amd as yg
 
+    t_start = time.perf_counter()
+
     def log(msg):
-        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+        print(f"[bench {time.perf_counter() - t_start:6.1f} s] {msg}", file=sys.stderr, flush=True)
 
     peaks = {}
     log("TORUS32 N=1024: batched 4096, plain form, CPU baseline")
@@ -583,6 +637,7 @@ def main():
     mm = modmul_per_pbs(p)
     achieved = mm * BATCH / r1["br_avg_s"]
     lat = r1["plain_lat"]
+    lat_host = r1["plain_host_lat"]
     plain_rate = len(lat) / sum(lat)
     plain_bytes = bsk_bytes(p) + ksk_bytes(p)
     line = {
@@ -596,62 +651,89 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "exact integers mod q (binary64-held residues), int32 torus",
-        "data": "synthetic: seeded keys (seed 0) and 4096 LWE encryptions of uniform bits (seed 1)",
+        "dtype": "int (exact residues mod q held in binary64; int32 torus)",
+        "data": "synthetic: seeded keys (seed 0), 4096 LWE encryptions of uniform bits (seed 1)",
         "config": {
-            "workload": "batched programmable bootstrap (blind rotate, sample extract, keyswitch), TORUS32 n=630 k=1 "
-                        "N=1024 gadget 2^7x3 KS 2^2x8, 4096 ciphertexts under one key, sign test polynomial",
+            "workload": "batched PBS (blind rotate, sample extract, keyswitch), TORUS32 n=630 k=1 N=1024 gadget 2^7x3 "
+                        "KS 2^2x8, 4096 ciphertexts under one key, sign test polynomial, inputs resident in HBM",
             "batch": BATCH,
-            "form": "batched, one persistent blind-rotation launch + one keyswitch launch",
         },
         "roofline": {
-            "bound": "valu-modmul",
-            "achieved": round(achieved / 1e9, 2),
-            "peak": round(peak / 1e9, 2),
-            "unit": "Gmodmul/s",
-            "frac": round(achieved / peak, 4),
-            "traffic": read_traffic(),
+            "bound": "fp64-valu",
             "kernel": "k_blind_rotate",
-            "per_unit": f"{mm} modmul per PBS x {BATCH} PBS per launch",
+            "achieved": round(achieved / 1e9, 1),
+            "peak": round(MODMUL_CEILING / 1e9, 1),
+            "unit": "Gmodmul/s",
+            "frac": round(achieved / MODMUL_CEILING, 4),
+            "traffic": read_traffic(),
             "avg_launch_ms": round(r1["br_avg_s"] * 1e3, 3),
-            "peak_source": f"measured on this GPU: binary64 exact modmul {peaks['f64'] / 1e9:.1f} G/s, "
-                           f"64-bit integer Shoup modmul {peaks['u64'] / 1e9:.1f} G/s (microbenchmark, same prime)",
-            "bsk_bytes_per_launch_if_streamed_once": bsk_bytes(p),
-            # the same work against MI355X's published FP64 vector peak (78.6
-            # TFLOP/s = 39.3 T FMA issues/s): one exact modmul is 6 binary64
-            # VALU instructions (mul, fma, mul, rndne, fma, add)
-            "published_fp64": {"peak_tflops": FP64_PEAK_TFLOPS, "instr_per_modmul": 6,
-                               "achieved_tinstr_per_s": round(achieved * 6 / 1e12, 2),
-                               "peak_tinstr_per_s": FP64_PEAK_TFLOPS / 2,
-                               "frac": round(achieved * 6 / (FP64_PEAK_TFLOPS / 2 * 1e12), 4)},
+            "per_unit": f"{mm} modmul per PBS x {BATCH} PBS per launch",
+            "peak_derivation": f"MI355X published FP64 {FP64_PEAK_TFLOPS} TFLOP/s = {FP64_PEAK_TFLOPS / 2} T instr/s, "
+                               f"{INSTR_PER_MODMUL} binary64 instructions per exact modmul",
+            "instr_per_modmul": INSTR_PER_MODMUL,
+            "measured_peak": round(peak / 1e9, 1),
+            "frac_measured_peak": round(achieved / peak, 4),
         },
         "kernels_ms_per_step": {"k_blind_rotate": round(r1["br_avg_s"] * 1e3, 3), "k_extract_ks": round(r1["ks_avg_ms"], 3)},
         "plain": {"value": round(plain_rate, 2), "unit": "PBS/s", "calls": len(lat),
                   "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3),
-                  "roofline": {"bound": "hbm", "bytes_per_pbs": plain_bytes,
-                               "frac": round(plain_bytes * plain_rate / HBM_PEAK, 5),
-                               "modmul_frac": round(mm * plain_rate / peak, 5)}},
+                  "host_call_p50_ms": round(pct(lat_host, 50) * 1e3, 3), "host_call_p99_ms": round(pct(lat_host, 99) * 1e3, 3),
+                  "hbm_frac": round(plain_bytes * plain_rate / HBM_PEAK, 5)},
         "batched_over_plain": round(value / plain_rate, 1),
         "decrypt_check": r1["decrypt_ok"],
     }
     if "cpu_baseline" in r1:
-        line["cpu_baseline"] = r1["cpu_baseline"]
-    configs = {}
+        cb = r1["cpu_baseline"]
+        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "latency_ms_one_thread")}
+        line["cpu_baseline"]["latency_level_parallel_p50_ms"] = cb["latency_level_parallel"]["p50_ms"]
+    detail = {"summary": None, "t32_n1024_b4096": {k: v for k, v in r1.items() if k not in ("plain_lat", "plain_host_lat")},
+              "peaks": peaks}
+    detail["t32_n1024_b4096"]["plain_lat_ms"] = [round(x * 1e3, 3) for x in lat]
+    detail["t32_n1024_b4096"]["plain_host_lat_ms"] = [round(x * 1e3, 3) for x in lat_host]
+    summaries = {}
     if "t56_batch" in legs:
         log("TORUS56 N=2048 batch 8192")
-        configs["t56_n2048_b8192"] = leg_t56_batch(yg, args, peak)
+        r = leg_t56_batch(yg, args, peak)
+        detail["t56_n2048_b8192"] = r
+        summaries["t56_n2048_b8192"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                        "frac": r["roofline"]["frac"], "avg_launch_ms": r["roofline"]["avg_launch_ms"],
+                                        "tile": [r["tile"]["cpw"], r["tile"]["twl"]], "lut_decrypt_check": r["lut_decrypt_check"],
+                                        "plain_p50_ms": r["plain"]["p50_ms"], "plain_host_p50_ms": r["plain"]["host_p50_ms"],
+                                        "cpu_value": None if r["cpu_baseline"] is None else r["cpu_baseline"]["value"]}
     if "multitenant" in legs:
         log("64 tenants, hipGraph")
-        configs["multitenant_64_graph"] = leg_multitenant(yg, args, peak)
+        r = leg_multitenant(yg, args, peak)
+        detail["multitenant_64_graph"] = r
+        summaries["multitenant_64_graph"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                                             "frac": r["roofline"]["frac_whole_pipeline"],
+                                             "graph_equals_eager": r["graph_equals_eager"], "decrypt_check": r["decrypt_check"]}
     if "keyload" in legs:
         log("key loads: full vs WWL+24 succinct form")
-        configs["key_load_forms"] = leg_keyload(yg, args)
+        r = leg_keyload(yg, args)
+        detail["key_load_forms"] = r
+        summaries["key_load_ms"] = {k: [v["full"]["device_expand_ms"], v["succinct"]["device_expand_ms"]] for k, v in r.items()}
     if "serving" in legs:
         log("serving sweep")
-        configs["serving_256_tenants"] = leg_serving(yg, args, peak)
-    if configs:
-        line["configs"] = configs
-    print(json.dumps(line), flush=True)
+        r = leg_serving(yg, args, peak)
+        detail["serving_256_tenants"] = r
+        summaries["serving_256_tenants"] = [serving_summary(x) for x in r["runs"]]
+    if summaries:
+        line["configs"] = summaries
+    line["detail"] = os.path.relpath(args.detail, ROOT) if args.detail.startswith(ROOT) else args.detail
+    detail["summary"] = line
+    try:
+        os.makedirs(os.path.dirname(args.detail), exist_ok=True)
+        with open(args.detail, "w") as f:
+            json.dump(detail, f, indent=1)
+    except OSError as e:
+        line["detail"] = f"not written: {e}"
+    log("done")
+    out = json.dumps(line, separators=(",", ":"))
+    if len(out) > 4096:
+        # keep the line parseable by its reader: drop the per-run serving list first
+        line["configs"].pop("serving_256_tenants", None)
+        out = json.dumps(line, separators=(",", ":"))
+    print(out, flush=True)
 
 
 if __name__ == "__main__":
