@@ -165,6 +165,22 @@ int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const in
                           const void* d_v, int32_t v_per_ct, int32_t* d_lwe_out,
                           void* d_acc_out, int32_t* d_extract_out);
 
+/* ygpu_pbs_batch_device without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * accumulators (when d_acc_out is NULL) and the keyswitch workspace live in a
+ * static device array of 96 MiB that the library holds on every device it
+ * runs on until the process ends, so the call makes no hipMalloc, hipFree or
+ * other allocating call.  A batch whose scratch exceeds the array (about
+ * 5,300 TORUS32 N=1024 or 1,900 TORUS56 N=2048 ciphertexts) is rejected with
+ * YGPU_ERR_INVALID_ARG naming the limit.  Every call on one device shares that
+ * device's array: calls that overlap on one GPU (other contexts, other
+ * streams) race and must be ordered by the caller.  Arguments, results and
+ * asynchrony as ygpu_pbs_batch_device; no kernel timing is recorded. */
+int ygpu_pbs_batch_device_noalloc(ygpu_ctx* ctx, int32_t key_id, int32_t count, const int32_t* d_lwe_in,
+                                  const void* d_v, int32_t v_per_ct, int32_t* d_lwe_out,
+                                  void* d_acc_out, int32_t* d_extract_out);
+
 /* Multi-key batched form: jobs grouped by key in one persistent launch;
  * buffers as in ygpu_pbs_batch_device with `total` rows. */
 int ygpu_pbs_jobs_device(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64_t total,

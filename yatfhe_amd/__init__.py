@@ -182,6 +182,7 @@ def load_library(path: str | None = None):
         "ygpu_keyswitch": (c_int, [c_vp, c_i32, c_vp, c_vp]),
         "ygpu_sample_extract": (c_int, [c_vp, c_vp, c_vp]),
         "ygpu_pbs_batch_device": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_pbs_batch_device_noalloc": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
         "ygpu_pbs_jobs_device": (c_int, [c_vp, P(_Job), c_i32, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp]),
         "ygpu_pbs_plain_device": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp]),
         "ygpu_malloc": (c_int, [c_vp, ctypes.c_size_t, P(c_vp)]),
@@ -252,7 +253,7 @@ def exported_symbols():
         "ygpu_key_file_write", "ygpu_key_file_info", "ygpu_key_file_read", "ygpu_key_upload_file",
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
-        "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls",
+        "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls", "ygpu_pbs_batch_device_noalloc",
     ]
 
 
@@ -622,6 +623,12 @@ class Context:
         _check(self.lib.ygpu_pbs_batch_device(self.h, key_id, count, d_lwe.value, d_v.value, 1 if v_per_ct else 0,
                                               d_out.value, d_acc.value if d_acc else None,
                                               d_ext.value if d_ext else None))
+
+    def pbs_batch_device_noalloc(self, key_id: int, count: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None, d_ext=None):
+        """ygpu_pbs_batch_device with its scratch in the library's static device array."""
+        _check(self.lib.ygpu_pbs_batch_device_noalloc(self.h, key_id, count, d_lwe.value, d_v.value, 1 if v_per_ct else 0,
+                                                      d_out.value, d_acc.value if d_acc else None,
+                                                      d_ext.value if d_ext else None))
 
     def pbs_jobs_device(self, jobs, total: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None):
         arr = (_Job * max(1, len(jobs)))()

@@ -99,6 +99,9 @@ struct ygpu_ctx {
     uint32_t* h_status = nullptr;
     uint32_t* hd_status = nullptr;
     int barrier_polls = TEAM_BARRIER_POLLS;
+    // this device's copy of the library's static scratch array (the device
+    // address hipGetSymbolAddress returned at init; null if that failed)
+    void* d_static_scratch = nullptr;
     void* d_acc = nullptr;
     size_t acc_cap = 0;
     int32_t* d_partial = nullptr;

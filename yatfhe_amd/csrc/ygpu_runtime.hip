@@ -9,6 +9,43 @@
 
 using namespace ygpu_rt;
 
+// Scratch of ygpu_pbs_batch_device_noalloc: the accumulators and the keyswitch
+// workspace of one call, in a static device array (each device holds its own
+// copy from module load until the process ends).  In host code `scratch` only
+// names the symbol: kernels get the device address hipGetSymbolAddress returns
+// (ygpu_ctx::d_static_scratch), never the array's name.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t{96} << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// The static scratch's device address on `device` (current), looked up once
+// per device and kept.
+void* static_scratch_address(int device) {
+    static std::mutex mu;
+    static std::map<int, void*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it != addr.end()) return it->second;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+        (void)hipGetLastError();
+        ptr = nullptr;
+    }
+    addr[device] = ptr;
+    return ptr;
+}
+
+// Scratch bytes of a no-allocation batch: the accumulators (unless the caller
+// supplies them), then the keyswitch workspace, each 256-byte aligned.
+size_t noalloc_bytes(const ygpu_ctx* ctx, int count, bool acc_given, size_t* acc_bytes) {
+    const size_t acc = acc_given ? 0 : (ctx->torus_bytes() * (ctx->p.k + 1) * ctx->p.N * static_cast<size_t>(count) + 255) / 256 * 256;
+    *acc_bytes = acc;
+    return acc + ks_workspace_bytes(ctx, count);
+}
+
+}  // namespace
+
 // ======================================================================= C ABI
 extern "C" {
 
@@ -95,6 +132,7 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         YG_HIP(hipMemcpy(ctx->d_bsk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice));
         YG_HIP(hipMemcpy(ctx->d_ksk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice));
     }
+    ctx->d_static_scratch = static_scratch_address(device);
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_status), 64, hipHostMallocDefault));
     *ctx->h_status = 0u;
     {
@@ -256,6 +294,43 @@ int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const in
         rc = run_br(ctx, batch_tile(ctx, count), a);
         if (rc) return rc;
         return run_ks(ctx, count, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, d_extract_out);
+    });
+}
+
+int ygpu_pbs_batch_device_noalloc(ygpu_ctx* ctx, int32_t key_id, int32_t count, const int32_t* d_lwe_in, const void* d_v,
+                                  int32_t v_per_ct, int32_t* d_lwe_out, void* d_acc_out, int32_t* d_extract_out) {
+    if (count < 0) return fail(YGPU_ERR_INVALID_ARG, "negative count");
+    if (count == 0) return YGPU_OK;
+    if (!d_lwe_in || !d_v || !d_lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null buffer");
+    return with_ctx(ctx, [&]() -> int {
+        int rc = key_live(ctx, key_id);
+        if (rc) return rc;
+        if (!ctx->d_static_scratch) return fail(YGPU_ERR_HIP, "the static scratch array has no device address on this device");
+        size_t accBytes = 0;
+        const size_t need = noalloc_bytes(ctx, count, d_acc_out != nullptr, &accBytes);
+        if (need > NOALLOC_SCRATCH_BYTES) {
+            size_t fit = 0;
+            for (size_t lo = 1, hi = static_cast<size_t>(count); lo <= hi;) {
+                const size_t mid = (lo + hi) / 2;
+                size_t a = 0;
+                if (noalloc_bytes(ctx, static_cast<int>(mid), d_acc_out != nullptr, &a) <= NOALLOC_SCRATCH_BYTES) { fit = mid; lo = mid + 1; }
+                else hi = mid - 1;
+            }
+            return fail(YGPU_ERR_INVALID_ARG, "ygpu_pbs_batch_device_noalloc: " + std::to_string(count) + " ciphertexts need " +
+                                                  std::to_string(need) + " bytes of scratch; the static scratch holds " +
+                                                  std::to_string(NOALLOC_SCRATCH_BYTES) + " (at most " + std::to_string(fit) +
+                                                  " ciphertexts of this parameter set per call)");
+        }
+        char* base = static_cast<char*>(ctx->d_static_scratch);
+        void* acc = d_acc_out ? d_acc_out : base;
+        int32_t* workspace = reinterpret_cast<int32_t*>(base + accBytes);
+        BrArgs a = br_args(ctx, count, d_lwe_in, d_v, v_per_ct ? ctx->p.N : 0, acc, ctx->d_bsk_table + key_id, nullptr);
+        // no timing events here: taking one may create it
+        YG_HIP(launch_blind_rotate(ctx->tbits, ctx->logn, batch_tile(ctx, count), ctx->p.radix_bits, a, ctx->stream));
+        KsArgs k = ks_args(ctx, count, acc, ctx->d_ksk_table + key_id, nullptr, d_lwe_out, workspace);
+        YG_HIP(launch_extract_ks(ctx->tbits, k, ks_slices(ctx, count), ctx->stream));
+        if (d_extract_out) YG_HIP(launch_extract(ctx->tbits, k, d_extract_out, ctx->stream));
+        return YGPU_OK;
     });
 }
 
