@@ -28,7 +28,13 @@
  *    col < k+1 (col c < k is the mask polynomial a_c, col k the body b).
  *  - KSK layout: ksk[i][j][n+1] int32, i < k*N, j < ks_level, entries
  *    a[0..n-1] then b (TlweKeySwitchingKey::decomposedKsk, keyswitching.h:14-31).
- *  - One context per GPU; calls on one context are serialised by the library.
+ *  - One context per GPU.  The synchronous host-buffer calls (ygpu_pbs,
+ *    ygpu_blind_rotate[_scaled], ygpu_sample_extract[_index], ygpu_keyswitch,
+ *    ygpu_extract_keyswitch) may be made from several threads at once: each
+ *    call runs on a stream and buffers of its own, so bootstraps of different
+ *    threads run side by side (the reference runs blindRotateJP22Ntt on many
+ *    worker threads, blindrotate_serving.cpp:197-228).  Every other call on a
+ *    context is serialised by the library and runs on the context's stream.
  *  - Device-side failures of asynchronous calls (YGPU_ERR_DEVICE) are reported
  *    by the next ygpu_sync, ygpu_memcpy_dtoh or synchronous bootstrapping call
  *    on the same context.
@@ -100,6 +106,13 @@ int ygpu_abi_struct_sizes(int64_t* out, int32_t max);
 int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out);
 int ygpu_destroy(ygpu_ctx* ctx);
 int ygpu_get_params(const ygpu_ctx* ctx, ygpu_params* out);
+/* The coefficient the bootstrap's sample extraction takes (YatfheParameters::
+ * driftPhase, yatfhe_parameters.h:65, passed by functionalBootstrappingMPNtt
+ * to extractTlweFromTrlwe, bootstrapping.cpp:78): 0 by default, in [0, N).
+ * Applies to every later bootstrap of the context, batched and plain, and to
+ * graphs captured afterwards. */
+int ygpu_set_extract_index(ygpu_ctx* ctx, int32_t index);
+int ygpu_get_extract_index(ygpu_ctx* ctx, int32_t* index);
 
 /* Makes a bootstrapping key (BootstrappingKeyMP / BootstrappingKey,
  * bootstrapping.h:16-114) and its keyswitching key (keyswitching.h:14-31)
@@ -143,8 +156,12 @@ int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, cons
  * [-2N, 2N)) and the in/out Trlwe accum (Torus[(k+1)N]). */
 int ygpu_blind_rotate_scaled(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar, void* acc);
 
-/* Sample extraction at index 0 alone, host buffers: extractTlweFromTrlwe
- * (trlwe.cpp:241-250).  acc Torus[(k+1)N] -> extract_out int32[kN+1]. */
+/* Sample extraction alone, host buffers: extractTlweFromTrlwe(out, in, index)
+ * (trlwe.cpp:241-250): a'_{iN+j} = a_i[index - j] for j <= index, else
+ * -a_i[N + index - j]; b' = b[index]; each modswitched to 2^32.
+ * acc Torus[(k+1)N] -> extract_out int32[kN+1].  ygpu_sample_extract takes the
+ * context's extraction index. */
+int ygpu_sample_extract_index(ygpu_ctx* ctx, const void* acc, int32_t index, int32_t* extract_out);
 int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out);
 
 /* LWE keyswitch of an arbitrary ciphertext of dimension kN, host buffers:
@@ -152,7 +169,7 @@ int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out);
 int ygpu_keyswitch(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, int32_t* lwe_out);
 
 /* Sample extract + keyswitch of an accumulator, host buffers:
- * extractTlweFromTrlwe (trlwe.cpp:241-250, index 0) then switchKeyForTlwe
+ * extractTlweFromTrlwe (trlwe.cpp:241-250, the context's index) then switchKeyForTlwe
  * (keyswitching.cpp:48-72).  acc: Torus[(k+1)N]; extract_out optional. */
 int ygpu_extract_keyswitch(ygpu_ctx* ctx, int32_t key_id, const void* acc, int32_t* extract_out, int32_t* lwe_out);
 
@@ -281,6 +298,25 @@ int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count,
                      const int32_t* mu, int32_t* out);
 
+/* Key generation and encryption with the reference's own noise law,
+ * TUniform (genTUniformNoise / addTUniformNoise, numeric.cpp:47-60): integers
+ * in [-2^b, 2^b] in torus units, the end points with probability 2^-(b+2) and
+ * every other point with probability 2^-(b+1); b is YatfheParameters::
+ * lweNoiseB for LWE and KSK rows and rlweNoiseB for the GLWE rows of the
+ * bootstrapping key (yatfhe_parameters.h:31-51).  `form` selects the key:
+ * YGPU_KEY_MP (outputs as ygpu_keygen), YGPU_KEY_CLASSIC (as
+ * ygpu_keygen_classic) or YGPU_KEY_SUCCINCT (as ygpu_keygen_succinct: bsk_out
+ * is bsk_half, s2_out and s2_level are used).  Same seeded streams as the
+ * Gaussian forms: the secret keys equal theirs for the same seed. */
+#define YGPU_KEY_MP 0
+#define YGPU_KEY_CLASSIC 1
+#define YGPU_KEY_SUCCINCT 2
+int ygpu_keygen_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits, int32_t form,
+                         int32_t s2_level, int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_out, uint64_t* s2_out,
+                         int32_t* ksk, int32_t threads);
+int ygpu_lwe_encrypt_tuniform(int32_t n, const int32_t* lwe_key, uint64_t seed, int32_t bound_bits, int32_t count,
+                              const int32_t* mu, int32_t* out);
+
 /* ------------------------------------------------------ key interchange file
  * The canonical coefficient form (the layouts of ygpu_key_upload) as one file:
  * a 128-byte header (magic "YGPUKEY1", version, ygpu_params, word counts,
@@ -395,6 +431,9 @@ int ygpu_server_destroy(ygpu_server* s);
 /* Parameter sets (one engine context each); both torus widths may coexist. */
 int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* pset);
 int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl);
+/* Extraction coefficient (param.driftPhase) of a parameter set's bootstraps;
+ * before the first request.  See ygpu_set_extract_index. */
+int ygpu_server_set_extract_index(ygpu_server* s, int32_t pset, int32_t index);
 /* Registers a tenant's keys (coefficient form, layouts as ygpu_key_upload):
  * transformed once into device-ready form in pinned host memory, loaded into
  * the device arena on first use. */

@@ -55,6 +55,8 @@ typedef struct or_params {
     int32_t ks_width_bits;  // param.ksWidthBits
     int32_t lwe_q_bits;     // log2 qLwe (32)
     uint64_t q_ntt;         // param.qNtt
+    int32_t extract_index;  // param.driftPhase: the coefficient extractTlweFromTrlwe takes
+    int32_t reserved;
 } or_params;
 }
 
@@ -477,18 +479,21 @@ void init_acc(const or_params& p, const int64_t* v, int32_t bbar, int64_t* acc) 
     rotate(p.torus_bits, N, -bbar, v, acc + static_cast<size_t>(p.k) * N);
 }
 
-// extractTlweFromTrlwe (trlwe.cpp:241-250), index 0, with the torus -> LWE_Q modswitch.
+// extractTlweFromTrlwe(out, in, index) (trlwe.cpp:241-250) at index =
+// p.extract_index, with the torus -> LWE_Q modswitch: a'_{iN+j} = a_i[index - j]
+// for j <= index, else -a_i[N + index - j] (negated in the Torus type, so
+// INT32_MIN wraps at 32 bits); b' = b[index].
 void extract(const or_params& p, const int64_t* acc, int32_t* out) {
-    const int N = p.N, w = p.torus_bits;
+    const int N = p.N, w = p.torus_bits, index = p.extract_index;
     const int64_t tq = torus_q(p), lq = lwe_q(p);
     for (int i = 0; i < p.k; i++) {
         const int64_t* a = acc + static_cast<size_t>(i) * N;
         for (int j = 0; j < N; j++) {
-            const int64_t x = (j == 0) ? a[0] : sext(-a[N - j], w == 32 ? 32 : 64);
+            const int64_t x = (j <= index) ? a[index - j] : sext(-a[N + index - j], w == 32 ? 32 : 64);
             out[i * N + j] = static_cast<int32_t>(mod_switch_from_torus(x, lq, tq));
         }
     }
-    out[p.k * N] = static_cast<int32_t>(mod_switch_from_torus(acc[static_cast<size_t>(p.k) * N], lq, tq));
+    out[p.k * N] = static_cast<int32_t>(mod_switch_from_torus(acc[static_cast<size_t>(p.k) * N + index], lq, tq));
 }
 
 // switchKeyForTlwe (keyswitching.cpp:48-72) with signedGadgetDecompositionKs

@@ -24,7 +24,7 @@ class _OrParams(ctypes.Structure):
         ("torus_bits", ctypes.c_int32), ("n", ctypes.c_int32), ("k", ctypes.c_int32), ("N", ctypes.c_int32),
         ("radix_bits", ctypes.c_int32), ("level", ctypes.c_int32), ("ks_radix_bits", ctypes.c_int32),
         ("ks_level", ctypes.c_int32), ("ks_width_bits", ctypes.c_int32), ("lwe_q_bits", ctypes.c_int32),
-        ("q_ntt", ctypes.c_uint64),
+        ("q_ntt", ctypes.c_uint64), ("extract_index", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -83,9 +83,9 @@ def P(a: np.ndarray):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
-def or_params(p) -> _OrParams:
+def or_params(p, extract_index: int = 0) -> _OrParams:
     return _OrParams(p.torus_bits, p.n, p.k, p.N, p.radix_bits, p.level, p.ks_radix_bits, p.ks_level,
-                     p.ks_width_bits, p.lwe_q_bits, p.q_ntt)
+                     p.ks_width_bits, p.lwe_q_bits, p.q_ntt, extract_index, 0)
 
 
 def ref_digits_lib(width: int):
@@ -107,10 +107,11 @@ def ref_digits_lib(width: int):
 class Oracle:
     """Oracle bound to one parameter set and one key set."""
 
-    def __init__(self, params, keys):
+    def __init__(self, params, keys, extract_index: int = 0):
+        """extract_index: the coefficient the bootstrap extracts (param.driftPhase)."""
         self.L = lib()
         self.p = params
-        self.cp = or_params(params)
+        self.cp = or_params(params, extract_index)
         self.bsk = np.ascontiguousarray(keys.bsk, dtype=np.uint64)
         self.ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
         self.key = self.L.or_key_create(ctypes.byref(self.cp), P(self.bsk))

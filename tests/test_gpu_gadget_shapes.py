@@ -3,7 +3,8 @@
 The headline shapes (TORUS32 N=1024 2^7 x 3, TORUS56 N=2048 2^10 x 3) run
 specialised kernels.  Every other shape the engine accepts runs a kernel that
 reads the gadget at run time: TORUS32 at N=2048 (two-wave teams), TORUS56 at
-N=1024 (one-wave teams) and other gadget bases and depths at either width.
+N=1024 (one-wave teams) and other gadget bases and depths at either width, up
+to 8 levels for bases up to 2^7.
 Each shape here is checked bit for bit against the oracle, batched and plain,
 with a short LWE dimension so the oracle stays fast.
 """
@@ -25,6 +26,12 @@ SHAPES = [
     ("t56_n1024_b10x3", yg.PARAMS_T56_N2048, dict(n=64, N=1024)),
     ("t56_n2048_b8x4", yg.PARAMS_T56_N2048, dict(n=64, radix_bits=8, level=4)),
     ("t56_n2048_b15x3", yg.PARAMS_T56_N2048, dict(n=48, radix_bits=15, level=3)),
+    # more than four levels (8-bit digit fields, partial sums reduced after four):
+    # the reference's TORUS32 default gadget 2^4 x 8 at N=2048, a rounded
+    # 2^3 x 5 (17 dropped bits), and a full 56-bit 2^7 x 8
+    ("t32_n2048_b4x8", yg.PARAMS_T32_N1024, dict(n=40, N=2048, radix_bits=4, level=8)),
+    ("t32_n1024_b3x5", yg.PARAMS_T32_N1024, dict(n=48, radix_bits=3, level=5)),
+    ("t56_n2048_b7x8", yg.PARAMS_T56_N2048, dict(n=32, radix_bits=7, level=8)),
 ]
 
 

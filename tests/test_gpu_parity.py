@@ -288,3 +288,83 @@ def test_classic_key_bit_exact(p32):
     assert np.array_equal(ref_acc.reshape(4, -1).astype(np.int64), acc[:4].reshape(4, -1).astype(np.int64))
     ph = yg.lwe_decrypt_phase(out, keys.lwe_key).astype(np.int64)
     assert np.array_equal((ph > 0).astype(np.int64), bits)
+
+
+@pytest.mark.parametrize("index", [1, 64, 1023])
+def test_extract_index_t32(p32, keys32, orc32, index):
+    """The bootstrap extracting at param.driftPhase != 0 (as compare_test.cpp:22
+    sets it): batched, plain and the standalone extraction against the oracle."""
+    bits, cts = sign_cts(keys32, 6, seed=60 + index)
+    v = yg.sign_test_polynomial(p32)
+    orc = Oracle(p32, keys32, extract_index=index)
+    with yg.Context(p32, 0) as ctx:
+        kid = ctx.upload_key(keys32)
+        ctx.set_extract_index(index)
+        assert ctx.extract_index() == index
+        out, acc, ext = ctx.pbs_batch(kid, cts, v, want_acc=True, want_extract=True)
+        one, _, one_ext = ctx.pbs(kid, cts[2], v, want_extract=True)
+        alone = ctx.sample_extract(acc[1], index)
+        at0 = ctx.sample_extract(acc[1], 0)
+    o_acc, o_ext, o_out = orc.pbs_batch_all(cts, v)
+    orc.close()
+    assert np.array_equal(acc, o_acc)
+    assert np.array_equal(ext, o_ext), "extracted LWEs differ"
+    assert np.array_equal(out, o_out), "keyswitched LWEs differ"
+    assert np.array_equal(one, o_out[2]) and np.array_equal(one_ext, o_ext[2])
+    assert np.array_equal(alone, o_ext[1])
+    assert np.array_equal(at0, orc32.extract(acc[1]))
+
+
+def test_extract_index_t56_and_server(p56, keys56):
+    index = 300
+    msgs = np.random.default_rng(61).integers(0, 16, 4)
+    cts = yg.lwe_encrypt(keys56.lwe_key, (msgs * (1 << 32) // 32).astype(np.int64).astype(np.int32), seed=61)
+    v = yg.lut_test_polynomial(p56, np.random.default_rng(62).integers(0, 16, 16), 16)
+    orc = Oracle(p56, keys56, extract_index=index)
+    o_acc, o_ext, o_out = orc.pbs_batch_all(cts, v)
+    orc.close()
+    with yg.Context(p56, 0) as ctx:
+        kid = ctx.upload_key(keys56)
+        ctx.set_extract_index(index)
+        out, _, ext = ctx.pbs_batch(kid, cts, v, want_extract=True)
+    assert np.array_equal(ext, o_ext) and np.array_equal(out, o_out)
+    with yg.Server(0, lanes=1, max_batch=64) as srv:
+        ps = srv.add_params(p56)
+        srv.set_extract_index(ps, index)
+        t = srv.add_tenant(ps, keys56)
+        reqs = [srv.request(t, c, v) for c in cts]
+        for r in reqs:
+            srv.submit(r)
+        for r, want in zip(reqs, o_out):
+            assert np.array_equal(srv.wait(r, timeout=60), want)
+
+
+def test_concurrent_plain_calls_one_context(p32, keys32, orc32):
+    """Synchronous calls from several threads on one context run on lanes of
+    their own (the reference's serving workers call the bootstrap at once):
+    every result equals the oracle's, and a key release waits for them."""
+    import threading
+    bits, cts = sign_cts(keys32, 24, seed=63)
+    v = yg.sign_test_polynomial(p32)
+    want = orc32.pbs_batch(cts, v)
+    got = [None] * len(cts)
+    errors = []
+    with yg.Context(p32, 0) as ctx:
+        kid = ctx.upload_key(keys32)
+
+        def work(idx):
+            try:
+                for c in idx:
+                    got[c] = ctx.pbs(kid, cts[c], v)[0]
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        ths = [threading.Thread(target=work, args=(range(t, len(cts), 6),)) for t in range(6)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        ctx.release_key(kid)
+    assert not errors, errors
+    for c in range(len(cts)):
+        assert np.array_equal(got[c], want[c]), f"ciphertext {c} differs from the oracle"

@@ -59,6 +59,47 @@ def test_extract_is_coefficient_zero_phase(p32, keys32):
     orc.close()
 
 
+@pytest.mark.parametrize("index", [0, 1, 37, 511, 1023])
+def test_extract_at_index_is_coefficient_phase(p32, keys32, index):
+    """extractTlweFromTrlwe(out, in, index) (trlwe.cpp:241-250): the extracted
+    LWE decrypts under the flattened GLWE key to coefficient `index` of the
+    GLWE phase, exactly at 32 bits (an index or sign slip changes it by a
+    torus-uniform amount)."""
+    from tests.oracle_py import or_params
+    import ctypes
+    rng = np.random.default_rng(40 + index)
+    acc = rng.integers(-(1 << 31), 1 << 31, (2, p32.N)).astype(np.int64)
+    acc[0, (index + 3) % p32.N] = -(1 << 31)   # the wrapping negation
+    cp = or_params(p32, index)
+    ext = np.zeros(p32.N + 1, np.int32)
+    lib().or_extract(ctypes.byref(cp), P(np.ascontiguousarray(acc.reshape(-1))), P(ext))
+    gp = glwe_phase(p32, acc, keys32.glwe_key)
+    assert yg.lwe_decrypt_phase(ext, keys32.glwe_key)[0] == np.int32(gp[index])
+    if index == 0:
+        orc = Oracle(p32, keys32)
+        assert np.array_equal(ext, orc.extract(acc))
+        orc.close()
+
+
+def test_extract_at_index_t56(p56, keys56):
+    """At 56 bits each extracted coefficient is modswitched to 2^32 (round half
+    away from zero), so the phase matches coefficient `index` of the GLWE phase
+    up to the rounding of kN + 1 terms."""
+    from tests.oracle_py import or_params
+    import ctypes
+    rng = np.random.default_rng(9)
+    N = p56.N
+    acc = rng.integers(-(1 << 55), 1 << 55, (2, N)).astype(np.int64)
+    for index in (0, 5, N - 1):
+        cp = or_params(p56, index)
+        ext = np.zeros(N + 1, np.int32)
+        lib().or_extract(ctypes.byref(cp), P(np.ascontiguousarray(acc.reshape(-1))), P(ext))
+        gp = int(glwe_phase(p56, acc, keys56.glwe_key)[index])
+        want = wrap(round(gp / (1 << 24)), 32)
+        got = int(yg.lwe_decrypt_phase(ext, keys56.glwe_key)[0])
+        assert abs(wrap(got - want, 32)) <= (N + 1) // 2 + 1
+
+
 def test_keyswitch_every_message(p32, keys32):
     orc = Oracle(p32, keys32)
     for m in range(-4, 4):

@@ -202,10 +202,16 @@ def load_library(path: str | None = None):
         "ygpu_graph_destroy": (c_int, [c_vp, c_i32]),
         "ygpu_copy_device": (c_int, [c_vp, c_vp, c_vp, ctypes.c_size_t]),
         "ygpu_set_barrier_polls": (c_int, [c_vp, c_i32]),
+        "ygpu_set_extract_index": (c_int, [c_vp, c_i32]),
+        "ygpu_get_extract_index": (c_int, [c_vp, P(c_i32)]),
+        "ygpu_sample_extract_index": (c_int, [c_vp, c_vp, c_i32, c_vp]),
+        "ygpu_server_set_extract_index": (c_int, [c_vp, c_i32, c_i32]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_classic": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_succinct": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
+        "ygpu_lwe_encrypt_tuniform": (c_int, [c_i32, c_vp, c_u64, c_i32, c_i32, c_vp, c_vp]),
+        "ygpu_keygen_tuniform": (c_int, [P(_Params), c_u64, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_server_create": (c_int, [P(_ServerConfig), P(c_vp)]),
         "ygpu_server_destroy": (c_int, [c_vp]),
         "ygpu_server_add_params": (c_int, [c_vp, P(_Params), P(c_i32)]),
@@ -254,6 +260,8 @@ def exported_symbols():
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
         "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls", "ygpu_pbs_batch_device_noalloc",
+        "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index",
+        "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform",
     ]
 
 
@@ -377,6 +385,33 @@ def keygen_classic(params: Params, seed: int, sigma_log2_lwe: float | None = Non
     return KeySet(params, lk, gk, bsk, ksk)
 
 
+KEY_FORMS = {"mp": 0, "classic": 1, "succinct": 2}
+
+
+def keygen_tuniform(params: Params, seed: int, lwe_bound_bits: int, glwe_bound_bits: int, form: str = "mp",
+                    s2_level: int | None = None, threads: int = 0):
+    """Seeded keys with the reference's TUniform noise (ygpu_keygen_tuniform):
+    noise in [-2^b, 2^b] torus units, b = lweNoiseB for LWE/KSK rows and
+    rlweNoiseB for the bootstrapping key's GLWE rows."""
+    lib = load_library()
+    k1 = params.k + 1
+    lk = np.zeros(params.n, np.int32)
+    gk = np.zeros(params.k * params.N, np.int32)
+    ksk = np.zeros(params.ksk_shape, np.int32)
+    cp = params.c()
+    if form == "succinct":
+        s2_level = max_s2_level(params) if s2_level is None else s2_level
+        half = np.zeros((params.n, params.level, k1, params.N), np.uint64)
+        s2 = np.zeros((s2_level, k1, params.N), np.uint64)
+        _check(lib.ygpu_keygen_tuniform(ctypes.byref(cp), seed, lwe_bound_bits, glwe_bound_bits, KEY_FORMS[form], s2_level,
+                                        _ptr(lk), _ptr(gk), _ptr(half), _ptr(s2), _ptr(ksk), threads))
+        return SuccinctKeySet(params, lk, gk, half, s2, ksk)
+    bsk = np.zeros(params.bsk_shape, np.uint64)
+    _check(lib.ygpu_keygen_tuniform(ctypes.byref(cp), seed, lwe_bound_bits, glwe_bound_bits, KEY_FORMS[form], 0,
+                                    _ptr(lk), _ptr(gk), _ptr(bsk), None, _ptr(ksk), threads))
+    return KeySet(params, lk, gk, bsk, ksk)
+
+
 def save_keys(path: str, keys):
     """Write the keys' coefficient form to a key interchange file (ygpu_key_file_write,
     or ygpu_key_file_write_succinct for a SuccinctKeySet)."""
@@ -439,6 +474,17 @@ def lwe_encrypt(lwe_key: np.ndarray, mu: np.ndarray, seed: int, sigma_log2: floa
     out = np.zeros((mu.shape[0], n + 1), np.int32)
     key = np.ascontiguousarray(lwe_key, dtype=np.int32)
     _check(lib.ygpu_lwe_encrypt(n, _ptr(key), seed, sigma_log2, mu.shape[0], _ptr(mu), _ptr(out)))
+    return out
+
+
+def lwe_encrypt_tuniform(lwe_key: np.ndarray, mu: np.ndarray, seed: int, bound_bits: int) -> np.ndarray:
+    """symEncTlwe with TUniform noise in [-2^bound_bits, 2^bound_bits] (the reference's lweNoiseB)."""
+    lib = load_library()
+    mu = np.ascontiguousarray(mu, dtype=np.int32)
+    n = lwe_key.shape[0]
+    out = np.zeros((mu.shape[0], n + 1), np.int32)
+    key = np.ascontiguousarray(lwe_key, dtype=np.int32)
+    _check(lib.ygpu_lwe_encrypt_tuniform(n, _ptr(key), seed, bound_bits, mu.shape[0], _ptr(mu), _ptr(out)))
     return out
 
 
@@ -596,11 +642,25 @@ class Context:
         _check(self.lib.ygpu_blind_rotate_scaled(self.h, key_id, _ptr(ab), _ptr(a)))
         return a
 
-    def sample_extract(self, acc: np.ndarray):
+    def sample_extract(self, acc: np.ndarray, index: int | None = None):
+        """extractTlweFromTrlwe at `index` (default: the context's extraction index)."""
         p = self.params
         out = np.zeros(p.k * p.N + 1, np.int32)
-        _check(self.lib.ygpu_sample_extract(self.h, _ptr(np.ascontiguousarray(acc, dtype=p.torus_dtype)), _ptr(out)))
+        a = np.ascontiguousarray(acc, dtype=p.torus_dtype)
+        if index is None:
+            _check(self.lib.ygpu_sample_extract(self.h, _ptr(a), _ptr(out)))
+        else:
+            _check(self.lib.ygpu_sample_extract_index(self.h, _ptr(a), index, _ptr(out)))
         return out
+
+    def set_extract_index(self, index: int):
+        """The bootstraps' extraction coefficient (the reference's param.driftPhase)."""
+        _check(self.lib.ygpu_set_extract_index(self.h, index))
+
+    def extract_index(self) -> int:
+        i = ctypes.c_int32(0)
+        _check(self.lib.ygpu_get_extract_index(self.h, ctypes.byref(i)))
+        return i.value
 
     def keyswitch(self, key_id: int, lwe_in: np.ndarray):
         out = np.zeros(self.params.n + 1, np.int32)
@@ -789,6 +849,9 @@ class Server:
     def set_tile(self, pset: int, cpw: int, twl: int):
         _check(self.lib.ygpu_server_set_tile(self.h, pset, cpw, twl))
 
+    def set_extract_index(self, pset: int, index: int):
+        _check(self.lib.ygpu_server_set_extract_index(self.h, pset, index))
+
     def add_tenant(self, pset: int, keys) -> int:
         if isinstance(keys, SuccinctKeySet):
             return self.add_tenant_succinct(pset, keys)
@@ -931,4 +994,5 @@ __all__ = [
     "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
     "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
     "key_file_s2_level", "load_succinct_arrays", "keygen_classic", "SCHED_MODES", "abi_struct_mismatches",
+    "keygen_tuniform", "lwe_encrypt_tuniform", "KEY_FORMS",
 ]

@@ -3,28 +3,52 @@
 //
 // Header-only.  Like the reference, the torus width is chosen at compile time
 // with -DTORUS32 or -DTORUS56 (torus.h:11-21).  Types and free functions keep
-// the reference's names, member names, argument order and meaning; the
-// bootstrapping path (blind rotation, sample extract, keyswitch) runs on the GPU.
+// the reference's names, member names, argument order and meaning, so the
+// reference's call sites compile unchanged, e.g.
+//   blindRotateMP21Ntt(accum, bsKey.bskDft, inputModN2, param);   (compare_test.cpp:58)
+//   blindRotateNtt(in2, bsk.bskDft, sTlwe, param);                 (blindrot_test.cpp:191)
+//   extractTlweFromTrlwe(tmp, accum, param.driftPhase);
+// The bootstrapping path (blind rotation, sample extract, keyswitch) runs on
+// the GPU.
 //
-// Differences a caller must know:
-//  * BootstrappingKeyMP / BootstrappingKey hold the key in coefficient form
-//    (residues mod qNtt, the reference's NTT-form words after an inverse
-//    transform without centring) in `bskCoeff`; the reference's HEXL-order
-//    `bskDft` words are internal to HEXL and are not exchanged.
+// What a caller must know:
+//  * The key containers are the reference's (BootstrappingKeyMP::bskDft of
+//    TrgswMPDft, BootstrappingKey::bskDft of TrgswDft, their TrlweDft rows of
+//    NttPolynomial), but every NttPolynomial holds the COEFFICIENT form of the
+//    reference's NTT-form polynomial: its residues mod qNtt, i.e. the HEXL
+//    words after an inverse transform without centring.  HEXL's word order is
+//    internal to HEXL and never crosses this boundary.  The key generation
+//    functions below fill that form; a reference user converts HEXL-form keys
+//    with HEXL's inverse transform.
+//  * A key reaches the device on its first use and is found again by its
+//    address and a fingerprint of sampled words; after changing a key's words
+//    in place call yatfhe_dropin::invalidateDeviceKeys().
 //  * Key generation is seeded (setYatfheSeed); the reference's ChaCha20 stream
-//    is OS-seeded and cannot be reproduced.
-//  * extractTlweFromTrlwe supports index 0 (the index the bootstrap uses) and
-//    throws std::invalid_argument otherwise.
-//  * Only k = 1, N in {1024, 2048}, l = lApprox <= 4, qLwe = 2^32 are supported
-//    (ygpu_init validates and the shim throws std::invalid_argument).
+//    is OS-seeded and cannot be reproduced.  Noise follows the reference's
+//    TUniform law with param.lweNoiseB / param.rlweNoiseB (numeric.cpp:47-60).
+//  * The MP path runs at the key's own level (bskDft[0][0].l, as
+//    blindRotateMP21Ntt reads it, blind_rotate.cpp:1539): up to 8 levels for
+//    radixBits <= 7, 4 above.  The classic path (blindRotateNtt,
+//    functionalBootstrappingNtt) decomposes into param.l digits and multiplies
+//    param.lApprox of them (trgsw.cpp:550-576, cmux.cpp:16-21); the engine
+//    computes it only for l == lApprox and throws std::invalid_argument
+//    otherwise rather than return a different result.
+//  * Only k = 1, N in {1024, 2048}, qLwe = 2^32 and group = 1 are supported
+//    (std::invalid_argument otherwise).
+//  * The path functions may be called from several threads at once: each
+//    call runs on a stream of its own (the reference's serving workers call
+//    blindRotateJP22Ntt concurrently, blindrotate_serving.cpp:197-228).
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -46,6 +70,7 @@ using LweTorus = int32_t;
 #else
 using LweTorus = Torus;
 #endif
+using NttType = uint64_t;
 using Integer = Torus;
 using Binary = Integer;
 
@@ -104,12 +129,24 @@ struct YatfheParameters {
     }
 };
 
+// ------------------------------------------------------------------ containers
+// polynomial.h:84-110, tlwe.h:13-33, trlwe.h:29-117, trgsw.h:14-135, trlev.h:24-36
 struct TorusPolynomial {
     std::vector<Torus> coeffs;
     int N{};
     TorusPolynomial() = default;
     explicit TorusPolynomial(int n) : coeffs(n, 0), N(n) {}
     TorusPolynomial(int n, Torus val) : coeffs(n, val), N(n) {}
+};
+using IntPolynomial = TorusPolynomial;
+
+// Coefficient-form residues mod qNtt here (see the header comment).
+struct NttPolynomial {
+    std::vector<NttType> coeffs{};
+    int N{};
+    NttPolynomial() = default;
+    explicit NttPolynomial(int n) : coeffs(n, 0), N(n) {}
+    NttPolynomial(int n, NttType value) : coeffs(n, value), N(n) {}
 };
 
 struct Tlwe {
@@ -137,6 +174,73 @@ struct Trlwe {
     Trlwe() = default;
     explicit Trlwe(const YatfheParameters& p) : a(p.k, TorusPolynomial(p.N)), b(p.N), N(p.N), k(p.k) {}
     Trlwe(int k_, int N_) : a(k_, TorusPolynomial(N_)), b(N_), N(N_), k(k_) {}
+    Trlwe(int k_, int N_, Torus val) : a(k_, TorusPolynomial(N_, val)), b(N_, val), N(N_), k(k_) {}
+};
+
+struct TrlweDft {
+    std::vector<NttPolynomial> a;
+    NttPolynomial b;
+    int k{};
+    bool a_initialized{};
+    TrlweDft() = default;
+    explicit TrlweDft(const YatfheParameters& p) : a(p.k, NttPolynomial(p.N)), b(p.N), k(p.k), a_initialized(true) {}
+    TrlweDft(int k_, int N) : a(k_, NttPolynomial(N)), b(N), k(k_), a_initialized(true) {}
+    TrlweDft(int k_, int N, bool onlyB) : b(N), k(k_), a_initialized(!onlyB) {
+        if (a_initialized) a.resize(k_, NttPolynomial(N));
+    }
+};
+
+// c[lvl][r]: the RGSW row of mask digit a_r; cPrime[lvl]: the row of the body
+// digit (externalProductTrgswMPNttImpl, trgsw.cpp:725-748).
+struct TrgswMPDft {
+    std::vector<std::vector<TrlweDft>> c;
+    std::vector<TrlweDft> cPrime;
+    int l{};
+    int k{};
+    bool isHalf{false};
+    TrgswMPDft() = default;
+    explicit TrgswMPDft(const YatfheParameters& p) : TrgswMPDft(p, p.l) {}
+    TrgswMPDft(const YatfheParameters& p, const int level)
+        : c(level, std::vector<TrlweDft>(p.k, TrlweDft(p.k, p.N))), cPrime(level, TrlweDft(p.k, p.N)), l(level), k(p.k) {}
+    TrgswMPDft(const YatfheParameters& p, const int level, bool half) : cPrime(level, TrlweDft(p.k, p.N)), l(level), k(p.k), isHalf(half) {
+        if (!isHalf) c.resize(level, std::vector<TrlweDft>(p.k, TrlweDft(p.k, p.N)));
+    }
+};
+
+// Coefficient-domain RGSW types: the reference's BootstrappingKey[MP] also
+// carry them (member `bsk`), unused by the NTT path; here they are declared
+// for source compatibility and left empty.
+struct TrgswMP {
+    std::vector<std::vector<Trlwe>> c;
+    std::vector<Trlwe> cPrime;
+    int l{};
+    int k{};
+    bool isHalf{false};
+};
+struct Trgsw {
+    std::vector<std::vector<Trlwe>> trlweSamples{};
+    int l{};
+    int k{};
+};
+
+// trlweDftSamples[lvl][row]: row r < k multiplies mask digit a_r, row k the
+// body digit (externalProductTrgswNtt, trgsw.cpp:550-576).
+struct TrgswDft {
+    std::vector<std::vector<TrlweDft>> trlweDftSamples;
+    int l{};
+    int k{};
+    TrgswDft() = default;
+    explicit TrgswDft(const YatfheParameters& p) : TrgswDft(p, p.l) {}
+    TrgswDft(const YatfheParameters& p, const int level)
+        : trlweDftSamples(level, std::vector<TrlweDft>(p.k + 1, TrlweDft(p.k, p.N))), l(level), k(p.k) {}
+};
+
+struct TrlevDft {
+    std::vector<TrlweDft> trlweDfts;
+    int l{};
+    TrlevDft() = default;
+    explicit TrlevDft(const YatfheParameters& p) : TrlevDft(p, p.l) {}
+    TrlevDft(const YatfheParameters& p, const int level) : trlweDfts(level, TrlweDft(p.k, p.N)), l(level) {}
 };
 
 struct TlweKey {
@@ -159,55 +263,66 @@ struct TrgswKey {
     explicit TrgswKey(const YatfheParameters& p) : trlweKey(p) {}
 };
 
-// Keys in coefficient form; see the header comment.
-struct BootstrappingKeyMP {
-    std::vector<uint64_t> bskCoeff;  // [n][level][k+1][k+1][N]
+// BootstrappingKey (bootstrapping.h:16-45): the classic key, group 1.
+struct BootstrappingKey {
+    std::vector<TrgswDft> bskDft{};
+    std::vector<Trgsw> bsk{};
     int n{};
-    int level{};
-    int group{1};
-    uint64_t generation{0};          // bumped by key generation, invalidates device copies
-    BootstrappingKeyMP() = default;
-    explicit BootstrappingKeyMP(const YatfheParameters& p) : BootstrappingKeyMP(p, p.l) {}
-    BootstrappingKeyMP(const YatfheParameters& p, int lv)
-        : bskCoeff(static_cast<size_t>(p.n) * lv * (p.k + 1) * (p.k + 1) * p.N), n(p.n), level(lv), group(p.group) {}
-};
-using BootstrappingKey = BootstrappingKeyMP;  // same arithmetic when l == lApprox (classic row order)
-
-namespace yatfhe_dropin {
-inline int s2_levels(const YatfheParameters& p);
-}
-
-// WWL+24 succinct key (bootstrapping.h:116-136) in coefficient form: the
-// body-digit rows cPrime of every step and the TRLEV of s^2 with param.l levels
-// (the l the caller passed to initYatfhe).
-struct BootstrappingKeyWWL24 {
-    std::vector<uint64_t> halfCoeff;  // [n][level][k+1][N]
-    std::vector<uint64_t> s2Coeff;    // [s2Level][k+1][N]
-    int n{};
-    int level{};
-    int s2Level{};
-    int group{1};
-    uint64_t generation{0};
-    BootstrappingKeyWWL24() = default;
-    BootstrappingKeyWWL24(const YatfheParameters& p, int lv)
-        : halfCoeff(static_cast<size_t>(p.n) * lv * (p.k + 1) * p.N), n(p.n), level(lv), s2Level(yatfhe_dropin::s2_levels(p)),
-          group(p.group) {
-        s2Coeff.assign(static_cast<size_t>(s2Level) * (p.k + 1) * p.N, 0);
+    int group{};
+    explicit BootstrappingKey(const YatfheParameters& p) : BootstrappingKey(p, p.l) {}
+    BootstrappingKey(const YatfheParameters& p, const int level) : group(p.group) {
+        if (group != 1) throw std::invalid_argument("BootstrappingKey: only group == 1 keys are supported");
+        n = p.n;
+        bskDft.assign(n, TrgswDft(p, level));
     }
 };
 
+// BootstrappingKeyMP (bootstrapping.h:47-114), binary secrets: n x 1 TrgswMPDft.
+struct BootstrappingKeyMP {
+    std::vector<std::vector<TrgswMPDft>> bskDft{};
+    std::vector<std::vector<TrgswMP>> bsk{};
+    int n{};
+    int group{};
+    bool isHalf{false};
+    BootstrappingKeyMP() = default;
+    explicit BootstrappingKeyMP(const YatfheParameters& p) : BootstrappingKeyMP(p, p.l) {}
+    BootstrappingKeyMP(const YatfheParameters& p, const int level) : group(p.group) {
+        if (group != 1) throw std::invalid_argument("BootstrappingKeyMP: only group == 1 keys are supported");
+        n = p.n;
+        bskDft.assign(n, std::vector<TrgswMPDft>(1, TrgswMPDft(p, level)));
+    }
+};
+
+// BootstrappingKeyWWL24 (bootstrapping.h:116-136): per step only the rows of
+// the body digit (TrgswMPDft with isHalf), and one TRLEV of s^2 with param.l
+// levels.
+struct BootstrappingKeyWWL24 {
+    std::vector<TrgswMPDft> bskDft{};
+    TrlevDft s2Dft{};
+    int n{};
+    int group{};
+    BootstrappingKeyWWL24() = default;
+    BootstrappingKeyWWL24(const YatfheParameters& p, const int level) : group(p.group) {
+        if (group != 1) throw std::invalid_argument("BootstrappingKeyWWL24: only group == 1 keys are supported");
+        n = p.n;
+        bskDft.assign(n, TrgswMPDft(p, level, true));
+        s2Dft = TrlevDft(p, p.l);
+    }
+};
+
+// TlweKeySwitchingKey (keyswitching.h:14-31)
 struct TlweKeySwitchingKey {
     std::vector<std::vector<Tlwe>> decomposedKsk;
     int nCurrKey{};
     int nTargetKey{};
     int level{};
-    uint64_t generation{0};
     explicit TlweKeySwitchingKey(const YatfheParameters& p)
         : decomposedKsk(p.N * p.k, std::vector<Tlwe>(p.ksLevel, Tlwe(p.n))), nCurrKey(p.N * p.k), nTargetKey(p.n), level(p.ksLevel) {}
     TlweKeySwitchingKey(int nCurr, int nTarget, int l)
         : decomposedKsk(nCurr, std::vector<Tlwe>(l, Tlwe(nTarget))), nCurrKey(nCurr), nTargetKey(nTarget), level(l) {}
 };
 
+// ------------------------------------------------------------------ engine state
 namespace yatfhe_dropin {
 
 inline void check(int rc) {
@@ -219,21 +334,24 @@ inline void check(int rc) {
 }
 
 // Process-global engine state, the counterpart of the reference's globals set
-// by initYatfhe (initializer.cpp:17-29): one context for the current parameters.
+// by initYatfhe (initializer.cpp:17-29): the parameters, and one engine
+// context per (gadget level, extraction index) in use, each holding the device
+// copies of the keys used with it.
 struct State {
     std::mutex mu;
-    ygpu_ctx* ctx = nullptr;
+    bool ready = false;
     YatfheParameters param{};
     uint64_t seed = 0x5EED;
     uint64_t draws = 0;
-    int requestedL = 0;  // param.l as passed to initYatfhe (the s^2 levels of WWL+24 keys)
-    uint64_t nextGeneration = 1;
-    // device key cache: (bsk generation, ksk generation) -> key id
-    struct Entry { const void* bsk; uint64_t bg; const void* ksk; uint64_t kg; int32_t id; };
-    std::vector<Entry> keys;
-    ~State() {
-        if (ctx) ygpu_destroy(ctx);
+    std::map<std::pair<int, int>, ygpu_ctx*> ctxs;
+    // device key cache: (context, BSK address, BSK fingerprint, KSK address, KSK fingerprint) -> key id
+    std::map<std::tuple<ygpu_ctx*, const void*, uint64_t, const void*, uint64_t>, int32_t> keys;
+    void clear() {
+        for (auto& c : ctxs) ygpu_destroy(c.second);
+        ctxs.clear();
+        keys.clear();
     }
+    ~State() { clear(); }
 };
 inline State& state() {
     static State s;
@@ -256,84 +374,272 @@ inline ygpu_params to_params(const YatfheParameters& p, int level) {
     return q;
 }
 
-inline uint64_t next_seed() {
-    State& s = state();
-    return s.seed * 0x9E3779B97F4A7C15ULL + (++s.draws);
+inline void require_init() {
+    if (!state().ready) throw std::logic_error("initYatfhe must be called first");
 }
 
-inline int32_t device_key(const BootstrappingKeyMP& bsk, const TlweKeySwitchingKey& ksk) {
+// The context of (level, extraction index), created on first use.  s.mu held.
+inline ygpu_ctx* context_locked(int level, int extract_index) {
     State& s = state();
-    if (!s.ctx) throw std::logic_error("initYatfhe must be called first");
-    for (const auto& e : s.keys)
-        if (e.bsk == &bsk && e.bg == bsk.generation && e.ksk == &ksk && e.kg == ksk.generation) return e.id;
-    const YatfheParameters& p = s.param;
-    if (bsk.n != p.n || static_cast<int>(ksk.decomposedKsk.size()) != p.k * p.N || ksk.level != p.ksLevel)
-        throw std::invalid_argument("key dimensions do not match the initialised parameters");
-    std::vector<int32_t> flat(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1));
+    auto it = s.ctxs.find({level, extract_index});
+    if (it != s.ctxs.end()) return it->second;
+    const ygpu_params q = to_params(s.param, level);
+    ygpu_ctx* ctx = nullptr;
+    check(ygpu_init(&q, 0, &ctx));
+    const int rc = ygpu_set_extract_index(ctx, extract_index);
+    if (rc) {
+        ygpu_destroy(ctx);
+        check(rc);
+    }
+    s.ctxs[{level, extract_index}] = ctx;
+    return ctx;
+}
+
+inline uint64_t mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+    h ^= h >> 31;
+    return h * 0xBF58476D1CE4E5B9ULL;
+}
+
+// Coefficient-form BSK [n][level][k+1][k+1][N] of an MP key (bskDft[i][0]).
+inline void flatten_mp(const std::vector<std::vector<TrgswMPDft>>& bskDft, int level, int k, int N, std::vector<uint64_t>& out) {
+    const size_t n = bskDft.size();
+    out.assign(n * level * (k + 1) * (k + 1) * N, 0);
+    size_t o = 0;
+    for (size_t i = 0; i < n; i++) {
+        const TrgswMPDft& g = bskDft[i].at(0);
+        if (g.l != level || g.isHalf || static_cast<int>(g.c.size()) != level)
+            throw std::invalid_argument("BootstrappingKeyMP: every step must hold a full TrgswMPDft of the same level");
+        for (int lvl = 0; lvl < level; lvl++)
+            for (int row = 0; row <= k; row++) {
+                const TrlweDft& t = row < k ? g.c[lvl].at(row) : g.cPrime.at(lvl);
+                for (int col = 0; col <= k; col++) {
+                    const NttPolynomial& poly = col < k ? t.a.at(col) : t.b;
+                    if (static_cast<int>(poly.coeffs.size()) != N) throw std::invalid_argument("key polynomial of the wrong degree");
+                    std::memcpy(out.data() + o, poly.coeffs.data(), sizeof(uint64_t) * N);
+                    o += N;
+                }
+            }
+    }
+}
+
+// Coefficient-form BSK of a classic key (bskDft[i].trlweDftSamples[lvl][row]).
+inline void flatten_classic(const std::vector<TrgswDft>& bskDft, int level, int k, int N, std::vector<uint64_t>& out) {
+    const size_t n = bskDft.size();
+    out.assign(n * level * (k + 1) * (k + 1) * N, 0);
+    size_t o = 0;
+    for (size_t i = 0; i < n; i++) {
+        const TrgswDft& g = bskDft[i];
+        if (static_cast<int>(g.trlweDftSamples.size()) < level) throw std::invalid_argument("BootstrappingKey: too few levels");
+        for (int lvl = 0; lvl < level; lvl++)
+            for (int row = 0; row <= k; row++) {
+                const TrlweDft& t = g.trlweDftSamples[lvl].at(row);
+                for (int col = 0; col <= k; col++) {
+                    const NttPolynomial& poly = col < k ? t.a.at(col) : t.b;
+                    if (static_cast<int>(poly.coeffs.size()) != N) throw std::invalid_argument("key polynomial of the wrong degree");
+                    std::memcpy(out.data() + o, poly.coeffs.data(), sizeof(uint64_t) * N);
+                    o += N;
+                }
+            }
+    }
+}
+
+inline void unflatten_mp(const std::vector<uint64_t>& in, int level, int k, int N, std::vector<std::vector<TrgswMPDft>>& bskDft) {
+    size_t o = 0;
+    for (auto& step : bskDft) {
+        TrgswMPDft& g = step.at(0);
+        for (int lvl = 0; lvl < level; lvl++)
+            for (int row = 0; row <= k; row++) {
+                TrlweDft& t = row < k ? g.c[lvl][row] : g.cPrime[lvl];
+                for (int col = 0; col <= k; col++) {
+                    NttPolynomial& poly = col < k ? t.a[col] : t.b;
+                    std::memcpy(poly.coeffs.data(), in.data() + o, sizeof(uint64_t) * N);
+                    o += N;
+                }
+            }
+    }
+}
+
+inline void unflatten_classic(const std::vector<uint64_t>& in, int level, int k, int N, std::vector<TrgswDft>& bskDft) {
+    size_t o = 0;
+    for (auto& g : bskDft)
+        for (int lvl = 0; lvl < level; lvl++)
+            for (int row = 0; row <= k; row++) {
+                TrlweDft& t = g.trlweDftSamples[lvl][row];
+                for (int col = 0; col <= k; col++) {
+                    NttPolynomial& poly = col < k ? t.a[col] : t.b;
+                    std::memcpy(poly.coeffs.data(), in.data() + o, sizeof(uint64_t) * N);
+                    o += N;
+                }
+            }
+}
+
+// A fingerprint of 64 words spread over a flat-indexed key (get(idx) returns
+// word idx of `words`), plus its size.
+template <class Get>
+inline uint64_t fingerprint(size_t words, Get&& get) {
+    uint64_t h = mix(0x243F6A8885A308D3ULL, words);
+    if (words == 0) return h;
+    for (size_t s = 0; s < 64; s++) {
+        const size_t idx = (s * 0x9E3779B97F4A7C15ULL) % words;
+        h = mix(h, get(idx) ^ idx);
+    }
+    return h;
+}
+
+inline uint64_t fingerprint_mp(const std::vector<std::vector<TrgswMPDft>>& bskDft, int level, int k, int N) {
+    const size_t per = static_cast<size_t>(level) * (k + 1) * (k + 1) * N;
+    return fingerprint(bskDft.size() * per, [&](size_t idx) -> uint64_t {
+        const size_t i = idx / per;
+        size_t r = idx % per;
+        const int lvl = static_cast<int>(r / ((k + 1) * (k + 1) * N));
+        r %= (k + 1) * (k + 1) * N;
+        const int row = static_cast<int>(r / ((k + 1) * N));
+        r %= (k + 1) * N;
+        const int col = static_cast<int>(r / N);
+        const int j = static_cast<int>(r % N);
+        const TrgswMPDft& g = bskDft[i].at(0);
+        const TrlweDft& t = row < k ? g.c.at(lvl).at(row) : g.cPrime.at(lvl);
+        const NttPolynomial& poly = col < k ? t.a.at(col) : t.b;
+        return poly.coeffs.at(j);
+    });
+}
+
+inline uint64_t fingerprint_classic(const std::vector<TrgswDft>& bskDft, int level, int k, int N) {
+    const size_t per = static_cast<size_t>(level) * (k + 1) * (k + 1) * N;
+    return fingerprint(bskDft.size() * per, [&](size_t idx) -> uint64_t {
+        const size_t i = idx / per;
+        size_t r = idx % per;
+        const int lvl = static_cast<int>(r / ((k + 1) * (k + 1) * N));
+        r %= (k + 1) * (k + 1) * N;
+        const int row = static_cast<int>(r / ((k + 1) * N));
+        r %= (k + 1) * N;
+        const int col = static_cast<int>(r / N);
+        const int j = static_cast<int>(r % N);
+        const TrlweDft& t = bskDft[i].trlweDftSamples.at(lvl).at(row);
+        const NttPolynomial& poly = col < k ? t.a.at(col) : t.b;
+        return poly.coeffs.at(j);
+    });
+}
+
+inline uint64_t fingerprint_ksk(const TlweKeySwitchingKey& ksk) {
+    const size_t t = static_cast<size_t>(ksk.level), cols = static_cast<size_t>(ksk.nTargetKey) + 1;
+    return fingerprint(ksk.decomposedKsk.size() * t * cols, [&](size_t idx) -> uint64_t {
+        const size_t i = idx / (t * cols), j = (idx / cols) % t, c = idx % cols;
+        const Tlwe& row = ksk.decomposedKsk.at(i).at(j);
+        return static_cast<uint64_t>(static_cast<int64_t>(c < cols - 1 ? row.a.at(c) : row.b));
+    });
+}
+
+inline void flatten_ksk(const TlweKeySwitchingKey& ksk, const YatfheParameters& p, std::vector<int32_t>& flat) {
+    if (static_cast<int>(ksk.decomposedKsk.size()) != p.k * p.N || ksk.level != p.ksLevel || ksk.nTargetKey != p.n)
+        throw std::invalid_argument("keyswitching key dimensions do not match the initialised parameters");
+    flat.assign(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1), 0);
     size_t o = 0;
     for (const auto& row : ksk.decomposedKsk)
         for (const auto& t : row) {
-            for (int i = 0; i < p.n; i++) flat[o++] = static_cast<int32_t>(t.a[i]);
+            for (int i = 0; i < p.n; i++) flat[o++] = static_cast<int32_t>(t.a.at(i));
             flat[o++] = static_cast<int32_t>(t.b);
         }
-    int32_t id = -1;
-    check(ygpu_key_upload(s.ctx, bsk.bskCoeff.data(), flat.data(), &id));
-    s.keys.push_back({&bsk, bsk.generation, &ksk, ksk.generation, id});
-    return id;
 }
 
-inline int s2_levels(const YatfheParameters& p) {
-    const State& s = state();
-    return s.ctx && s.requestedL > 0 ? s.requestedL : p.l;
-}
-
-// Device copy of a WWL+24 key (blind rotation only; an all-zero KSK rides along).
-inline int32_t device_key_wwl24(const BootstrappingKeyWWL24& bsk) {
+// The device key (context + key id) for a coefficient-form BSK supplied by
+// `flatten` (called only on a cache miss) and an optional KSK (null: an
+// all-zero KSK rides along; blind rotation does not read it).  s.mu held.
+template <class Flatten>
+inline std::pair<ygpu_ctx*, int32_t> device_key_locked(int level, int extract_index, const void* bskAddr, uint64_t bskFp,
+                                                        Flatten&& flatten, const TlweKeySwitchingKey* ksk) {
     State& s = state();
-    if (!s.ctx) throw std::logic_error("initYatfhe must be called first");
-    for (const auto& e : s.keys)
-        if (e.bsk == &bsk && e.bg == bsk.generation && e.ksk == nullptr) return e.id;
-    const YatfheParameters& p = s.param;
-    if (bsk.n != p.n || bsk.level != p.lApprox) throw std::invalid_argument("WWL+24 key dimensions do not match the initialised parameters");
-    std::vector<int32_t> zero(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1), 0);
+    ygpu_ctx* ctx = context_locked(level, extract_index);
+    const uint64_t kfp = ksk ? fingerprint_ksk(*ksk) : 0;
+    const auto key = std::make_tuple(ctx, bskAddr, bskFp, static_cast<const void*>(ksk), kfp);
+    auto it = s.keys.find(key);
+    if (it != s.keys.end()) return {ctx, it->second};
+    std::vector<uint64_t> bsk;
+    flatten(bsk);
+    std::vector<int32_t> flat;
+    if (ksk) flatten_ksk(*ksk, s.param, flat);
+    else flat.assign(static_cast<size_t>(s.param.k) * s.param.N * s.param.ksLevel * (s.param.n + 1), 0);
     int32_t id = -1;
-    check(ygpu_key_upload_succinct(s.ctx, bsk.halfCoeff.data(), bsk.s2Coeff.data(), bsk.s2Level, zero.data(), &id));
-    s.keys.push_back({&bsk, bsk.generation, nullptr, 0, id});
-    return id;
+    check(ygpu_key_upload(ctx, bsk.data(), flat.data(), &id));
+    s.keys[key] = id;
+    return {ctx, id};
 }
 
-// Any resident key whose BSK matches (blind rotation needs no KSK).
-inline int32_t device_key_bsk(const BootstrappingKeyMP& bsk) {
+inline std::pair<ygpu_ctx*, int32_t> mp_key(const std::vector<std::vector<TrgswMPDft>>& bskDft, const TlweKeySwitchingKey* ksk,
+                                            const YatfheParameters& p) {
     State& s = state();
-    for (const auto& e : s.keys)
-        if (e.bsk == &bsk && e.bg == bsk.generation) return e.id;
-    // upload with an all-zero KSK (unused by blind rotation)
-    static TlweKeySwitchingKey* zero = nullptr;
-    if (!zero || zero->nTargetKey != s.param.n || static_cast<int>(zero->decomposedKsk.size()) != s.param.k * s.param.N) {
-        delete zero;
-        zero = new TlweKeySwitchingKey(s.param);
-    }
-    return device_key(bsk, *zero);
+    std::lock_guard<std::mutex> lk(s.mu);
+    require_init();
+    if (bskDft.empty() || bskDft[0].empty()) throw std::invalid_argument("empty bootstrapping key");
+    if (static_cast<int>(bskDft.size()) != s.param.n) throw std::invalid_argument("bootstrapping key has the wrong number of steps");
+    const int level = bskDft[0][0].l;
+    const int k = s.param.k, N = s.param.N;
+    const uint64_t fp = fingerprint_mp(bskDft, level, k, N);
+    return device_key_locked(level, p.driftPhase, &bskDft, fp, [&](std::vector<uint64_t>& out) { flatten_mp(bskDft, level, k, N, out); }, ksk);
+}
+
+// The classic path computes what the MP path computes only when the key is
+// decomposed into exactly the levels it multiplies.
+inline std::pair<ygpu_ctx*, int32_t> classic_key(const std::vector<TrgswDft>& bskDft, const TlweKeySwitchingKey* ksk,
+                                                 const YatfheParameters& p) {
+    if (p.group != 1) throw std::invalid_argument("blindRotateNtt: only group == 1 keys are supported");
+    if (p.l != p.lApprox)
+        throw std::invalid_argument("blindRotateNtt: the classic path decomposes into param.l digits and multiplies param.lApprox of "
+                                    "them; this engine computes it only for l == lApprox (l = " + std::to_string(p.l) +
+                                    ", lApprox = " + std::to_string(p.lApprox) + ")");
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    require_init();
+    if (static_cast<int>(bskDft.size()) != s.param.n) throw std::invalid_argument("bootstrapping key has the wrong number of steps");
+    const int level = p.lApprox;
+    const int k = s.param.k, N = s.param.N;
+    const uint64_t fp = fingerprint_classic(bskDft, level, k, N);
+    return device_key_locked(level, p.driftPhase, &bskDft, fp, [&](std::vector<uint64_t>& out) { flatten_classic(bskDft, level, k, N, out); },
+                             ksk);
+}
+
+inline uint64_t next_seed() {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    return s.seed * 0x9E3779B97F4A7C15ULL + (++s.draws);
+}
+
+// Accumulator <-> flat Torus[(k+1)N].
+inline void to_flat(const Trlwe& t, std::vector<Torus>& acc) {
+    const int N = t.N;
+    acc.resize(static_cast<size_t>(t.k + 1) * N);
+    for (int c = 0; c < t.k; c++) std::copy(t.a[c].coeffs.begin(), t.a[c].coeffs.end(), acc.begin() + static_cast<size_t>(c) * N);
+    std::copy(t.b.coeffs.begin(), t.b.coeffs.end(), acc.begin() + static_cast<size_t>(t.k) * N);
+}
+inline void from_flat(const std::vector<Torus>& acc, Trlwe& t) {
+    const int N = t.N;
+    for (int c = 0; c < t.k; c++) std::copy(acc.begin() + static_cast<size_t>(c) * N, acc.begin() + static_cast<size_t>(c + 1) * N, t.a[c].coeffs.begin());
+    std::copy(acc.begin() + static_cast<size_t>(t.k) * N, acc.end(), t.b.coeffs.begin());
+}
+
+// Forgets every device copy of a key (call after changing a key's words in place).
+inline void invalidateDeviceKeys() {
+    State& s = state();
+    std::lock_guard<std::mutex> lk(s.mu);
+    for (auto& e : s.keys) ygpu_key_release(std::get<0>(e.first), e.second);
+    s.keys.clear();
 }
 
 }  // namespace yatfhe_dropin
 
 // ------------------------------------------------------------------ init
 // initYatfhe (initializer.cpp:50-85): binds the process to one parameter set.
+// The parameters are validated (std::invalid_argument) and left unchanged.
 inline void initYatfhe(YatfheParameters& param) {
     auto& s = yatfhe_dropin::state();
     std::lock_guard<std::mutex> lk(s.mu);
-    s.requestedL = param.l;
-    if (param.l != param.lApprox)
-        param.l = param.lApprox;  // the engine decomposes into exactly the levels it multiplies
-    if (s.ctx) {
-        ygpu_destroy(s.ctx);
-        s.ctx = nullptr;
-        s.keys.clear();
-    }
-    const ygpu_params q = yatfhe_dropin::to_params(param, param.lApprox);
-    yatfhe_dropin::check(ygpu_init(&q, 0, &s.ctx));
+    s.clear();
+    s.ready = false;
     s.param = param;
+    // validate now: the contexts of the levels keys will use are made on first use
+    yatfhe_dropin::context_locked(param.lApprox, param.driftPhase);
+    s.ready = true;
 }
 
 inline void setYatfheSeed(uint64_t seed) {
@@ -368,6 +674,10 @@ inline Torus modSwitchToTorusGeneral(const int32_t mu, const uint32_t mSize, con
     const int64_t scale = torusQ / mSize;
     return static_cast<Torus>(intModP(mu, static_cast<int32_t>(mSize)) * scale);
 }
+// modSwitchToTorus32 (numeric.cpp:210-214): the same on the torus modulus
+inline Torus modSwitchToTorus32(const int32_t mu, const uint32_t mSize) {
+    return modSwitchToTorusGeneral(mu, mSize, yatfhe_dropin::state().param.q);
+}
 
 // modSwitchFromTorusGeneral (numeric.cpp:189-208), power-of-two ratios
 inline int64_t modSwitchFromTorusGeneral(const Torus in, const int64_t newMod, const int64_t torusQ) {
@@ -389,16 +699,49 @@ inline void rescaleTlweToNewMod(ScaledTlwe& output, const Tlwe& input) {
     for (int i = 0; i < input.n; i++) output.a[i] = static_cast<int32_t>(modSwitchFromTorusGeneral(static_cast<Torus>(input.a[i]), output.mod, lweQ));
 }
 
-// generateTestPolynomialFR (polynomial.cpp:92-107)
+namespace yatfhe_dropin {
+// The box layout shared by the test-polynomial generators (polynomial.cpp:75-127):
+// coefficient i takes the value of message slot i / box, the first half box is
+// negated and the polynomial is rotated left by half a box.
+inline void box_polynomial(TorusPolynomial& v, const int box, const std::vector<Torus>& slotValue) {
+    const int N = v.N, half = box / 2;
+    for (int i = 0; i < N; i++) v.coeffs[i] = slotValue[i / box];
+    for (int i = 0; i < half; i++) v.coeffs[i] = static_cast<Torus>(-v.coeffs[i]);
+    std::rotate(v.coeffs.begin(), v.coeffs.begin() + half, v.coeffs.end());
+}
+}  // namespace yatfhe_dropin
+
+// generateTestPolynomial (polynomial.cpp:75-89): slot value intModP(slot, modP)
+inline void generateTestPolynomial(TorusPolynomial& v, const int modP, const int modQ) {
+    const int box = modQ / modP;
+    std::vector<Torus> vals(v.N / box + 1);
+    for (size_t s = 0; s < vals.size(); s++) vals[s] = modSwitchToTorus32(intModP(static_cast<int>(s), modP), modP);
+    yatfhe_dropin::box_polynomial(v, box, vals);
+}
+
+// generateTestPolynomialFR (polynomial.cpp:92-107): the full plaintext range,
+// boxes of modQ / modP / 2 coefficients
 inline void generateTestPolynomialFR(TorusPolynomial& v, const int modP, const int modQ) {
-    const int N = v.N;
     const int box = modQ / modP, half = box / 2, hh = half / 2;
+    const int N = v.N;
     const int64_t torusQ = yatfhe_dropin::state().param.q;
     for (int i = 0; i < N; i++) v.coeffs[i] = modSwitchToTorusGeneral(intModP(i / half, modP), modP, torusQ);
     for (int i = 0; i < hh; i++) v.coeffs[i] = static_cast<Torus>(-v.coeffs[i]);
-    std::vector<Torus> r(N);
-    for (int i = 0; i < N; i++) r[i] = v.coeffs[(i + hh) % N];
-    v.coeffs = r;
+    std::rotate(v.coeffs.begin(), v.coeffs.begin() + hh, v.coeffs.end());
+}
+
+// generateTestPolynomialLt1 (polynomial.cpp:110-127): slot value 1 when its
+// message is below t (intModP over param.torusBase), else 0, scaled to the torus
+inline void generateTestPolynomialLt1(TorusPolynomial& v, const int t) {
+    const int modP = yatfhe_dropin::state().param.torusBase;
+    const int box = 2 * v.N / modP;
+    const int threshold = intModP(t, modP);
+    std::vector<Torus> vals(v.N / box + 1);
+    for (size_t s = 0; s < vals.size(); s++) {
+        const int msg = intModP(static_cast<int>(s), modP);
+        vals[s] = modSwitchToTorus32(msg < threshold ? 1 : 0, modP);
+    }
+    yatfhe_dropin::box_polynomial(v, box, vals);
 }
 
 // genNoiselessTrlweSample (trlwe.cpp:175-182): accum = (0, X^{-b} v)
@@ -418,34 +761,50 @@ inline void genNoiselessTrlweSample(Trlwe& accum, const TorusPolynomial& v, cons
 }
 
 // ------------------------------------------------------------------ keys
-// genTlweKey / genTrlweKey / genBootstrappingKeyMP / genTlweKeySwitchingKey
-// (tlwe.cpp, trlwe.cpp:70-81, bootstrapping.cpp:150-174, keyswitching.cpp:19-29):
-// one seeded draw generates a consistent key set; the TLWE and TRLWE keys must
-// be generated before the keys that depend on them.
+// genTlweKey / genTrlweKey / genBootstrappingKey[MP|WWL24] / genTlweKeySwitchingKey
+// (tlwe.cpp, trlwe.cpp:70-81, bootstrapping.cpp:98-174, keyswitching.cpp:19-29):
+// one seeded draw makes a consistent key set; genTlweKey starts a new draw, the
+// other generators reuse its secret keys.  Noise: TUniform with param.lweNoiseB
+// (LWE and KSK rows) and param.rlweNoiseB (bootstrapping-key rows).
 namespace yatfhe_dropin {
 struct KeyDraw {
     uint64_t seed = 0;
+    bool valid = false;
     std::vector<int32_t> lwe, glwe;
-    std::vector<uint64_t> bsk;
-    std::vector<int32_t> ksk;
 };
 inline KeyDraw& draw() {
     static KeyDraw d;
     return d;
 }
 inline void ensure_draw(bool fresh) {
-    State& s = state();
+    require_init();
     KeyDraw& d = draw();
-    if (!fresh && !d.lwe.empty()) return;
-    const YatfheParameters& p = s.param;
+    if (!fresh && d.valid) return;
     d.seed = next_seed();
-    const ygpu_params q = to_params(p, p.lApprox);
+    d.valid = true;
+    const YatfheParameters& p = state().param;
     d.lwe.assign(p.n, 0);
     d.glwe.assign(static_cast<size_t>(p.k) * p.N, 0);
-    d.bsk.assign(static_cast<size_t>(p.n) * p.lApprox * (p.k + 1) * (p.k + 1) * p.N, 0);
-    d.ksk.assign(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1), 0);
-    const double sl = -15.0, sg = p.torusBits == 32 ? -25.0 : -48.0;
-    check(ygpu_keygen(&q, d.seed, sl, sg, d.lwe.data(), d.glwe.data(), d.bsk.data(), d.ksk.data(), 0));
+    // the secret keys depend only on the seed: a one-level draw gives them
+    const ygpu_params q = to_params(p, 1);
+    std::vector<uint64_t> bsk(static_cast<size_t>(p.n) * (p.k + 1) * (p.k + 1) * p.N);
+    std::vector<int32_t> ksk(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1));
+    check(ygpu_keygen_tuniform(&q, d.seed, p.lweNoiseB, p.rlweNoiseB, YGPU_KEY_MP, 0, d.lwe.data(), d.glwe.data(), bsk.data(), nullptr,
+                               ksk.data(), 0));
+}
+// Keys of one form at `level` from the current draw.
+inline void draw_keys(int form, int level, int s2Level, std::vector<uint64_t>& bsk, std::vector<uint64_t>* s2, std::vector<int32_t>& ksk) {
+    ensure_draw(false);
+    const YatfheParameters& p = state().param;
+    const ygpu_params q = to_params(p, level);
+    std::vector<int32_t> lwe(p.n), glwe(static_cast<size_t>(p.k) * p.N);
+    const size_t rows = form == YGPU_KEY_SUCCINCT ? static_cast<size_t>(p.n) * level * (p.k + 1)
+                                                  : static_cast<size_t>(p.n) * level * (p.k + 1) * (p.k + 1);
+    bsk.assign(rows * p.N, 0);
+    if (s2) s2->assign(static_cast<size_t>(s2Level) * (p.k + 1) * p.N, 0);
+    ksk.assign(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1), 0);
+    check(ygpu_keygen_tuniform(&q, draw().seed, p.lweNoiseB, p.rlweNoiseB, form, s2Level, lwe.data(), glwe.data(), bsk.data(),
+                               s2 ? s2->data() : nullptr, ksk.data(), 0));
 }
 }  // namespace yatfhe_dropin
 
@@ -463,64 +822,72 @@ inline void genTrlweKey(TrlweKey& key) {
         for (int j = 0; j < key.N; j++) key.s[i].coeffs[j] = d.glwe[static_cast<size_t>(i) * key.N + j];
 }
 
-inline void genBootstrappingKeyMP(BootstrappingKeyMP& bsk, TrgswKey&, const TlweKey&, const YatfheParameters&) {
-    yatfhe_dropin::ensure_draw(false);
-    const auto& d = yatfhe_dropin::draw();
-    if (bsk.bskCoeff.size() != d.bsk.size()) throw std::invalid_argument("genBootstrappingKeyMP: level must equal param.lApprox");
-    bsk.bskCoeff = d.bsk;
-    bsk.generation = yatfhe_dropin::state().nextGeneration++;
+// genBootstrappingKeyMP (bootstrapping.cpp:150-174), at the key's level.
+inline void genBootstrappingKeyMP(BootstrappingKeyMP& bsk, TrgswKey&, const TlweKey&, const YatfheParameters& p) {
+    if (bsk.bskDft.empty() || bsk.bskDft[0].empty()) throw std::invalid_argument("genBootstrappingKeyMP: empty key");
+    const int level = bsk.bskDft[0][0].l;
+    std::vector<uint64_t> flat;
+    std::vector<int32_t> ksk;
+    yatfhe_dropin::draw_keys(YGPU_KEY_MP, level, 0, flat, nullptr, ksk);
+    yatfhe_dropin::unflatten_mp(flat, level, p.k, p.N, bsk.bskDft);
 }
 
+// genBootstrappingKey (bootstrapping.cpp:98-104), group 1: encryptTrgswNtt per step.
+inline void genBootstrappingKey(BootstrappingKey& bsk, TrgswKey&, const TlweKey&, const YatfheParameters& p) {
+    if (p.group != 1) throw std::invalid_argument("genBootstrappingKey: only group == 1 keys are supported");
+    if (bsk.bskDft.empty()) throw std::invalid_argument("genBootstrappingKey: empty key");
+    const int level = bsk.bskDft[0].l;
+    std::vector<uint64_t> flat;
+    std::vector<int32_t> ksk;
+    yatfhe_dropin::draw_keys(YGPU_KEY_CLASSIC, level, 0, flat, nullptr, ksk);
+    yatfhe_dropin::unflatten_classic(flat, level, p.k, p.N, bsk.bskDft);
+}
+
+// genTlweKeySwitchingKey (keyswitching.cpp:19-29)
 inline void genTlweKeySwitchingKey(TlweKeySwitchingKey& ksk, const TrlweKey&, const TlweKey&, const YatfheParameters& p) {
-    yatfhe_dropin::ensure_draw(false);
-    const auto& d = yatfhe_dropin::draw();
+    std::vector<uint64_t> flat;
+    std::vector<int32_t> k;
+    yatfhe_dropin::draw_keys(YGPU_KEY_MP, 1, 0, flat, nullptr, k);
     size_t o = 0;
     for (int i = 0; i < p.k * p.N; i++)
         for (int j = 0; j < p.ksLevel; j++) {
             Tlwe& t = ksk.decomposedKsk[i][j];
-            for (int u = 0; u < p.n; u++) t.a[u] = static_cast<LweTorus>(d.ksk[o++]);
-            t.b = static_cast<LweTorus>(d.ksk[o++]);
+            for (int u = 0; u < p.n; u++) t.a[u] = static_cast<LweTorus>(k[o++]);
+            t.b = static_cast<LweTorus>(k[o++]);
         }
-    ksk.generation = yatfhe_dropin::state().nextGeneration++;
 }
 
-// genBootstrappingKey (bootstrapping.cpp:98-104), group 1: the classic key
-// (encryptTrgswNtt per step), from the same draw as the other keys.
-inline void genBootstrappingKey(BootstrappingKey& bsk, TrgswKey&, const TlweKey&, const YatfheParameters& p) {
-    if (p.group != 1) throw std::invalid_argument("genBootstrappingKey: only group == 1 keys are supported");
-    yatfhe_dropin::ensure_draw(false);
-    auto& s = yatfhe_dropin::state();
-    const auto& d = yatfhe_dropin::draw();
-    if (bsk.bskCoeff.size() != d.bsk.size()) throw std::invalid_argument("genBootstrappingKey: level must equal param.lApprox");
-    const ygpu_params q = yatfhe_dropin::to_params(p, p.lApprox);
-    std::vector<int32_t> lwe(p.n), glwe(static_cast<size_t>(p.k) * p.N), ksk(d.ksk.size());
-    const double sg = p.torusBits == 32 ? -25.0 : -48.0;
-    yatfhe_dropin::check(ygpu_keygen_classic(&q, d.seed, -15.0, sg, lwe.data(), glwe.data(), bsk.bskCoeff.data(), ksk.data(), 0));
-    bsk.generation = s.nextGeneration++;
-}
-
-// genBootstrappingKeyWWL24 (bootstrapping.cpp:106-126), from the same draw as
-// the other keys (same secret keys), with the GLWE noise of param.rlweNoiseB
-// (a TUniform bound 2^B in torus units: log2 sigma = B - 0.79).
+// genBootstrappingKeyWWL24 (bootstrapping.cpp:106-126): the body-digit rows of
+// every step and the TRLEV of s^2 with bsk.s2Dft.l levels.
 inline void genBootstrappingKeyWWL24(BootstrappingKeyWWL24& bsk, TrgswKey&, const TlweKey&, const YatfheParameters& p) {
-    yatfhe_dropin::ensure_draw(false);
-    auto& s = yatfhe_dropin::state();
-    const auto& d = yatfhe_dropin::draw();
-    if (bsk.level != p.lApprox) throw std::invalid_argument("genBootstrappingKeyWWL24: level must equal param.lApprox");
-    const ygpu_params q = yatfhe_dropin::to_params(p, p.lApprox);
-    std::vector<int32_t> lwe(p.n), glwe(static_cast<size_t>(p.k) * p.N), ksk(d.ksk.size());
-    const double sg = p.rlweNoiseB - 0.79 - p.torusBits;
-    yatfhe_dropin::check(ygpu_keygen_succinct(&q, d.seed, -15.0, sg, bsk.s2Level, lwe.data(), glwe.data(), bsk.halfCoeff.data(),
-                                              bsk.s2Coeff.data(), ksk.data(), 0));
-    bsk.generation = s.nextGeneration++;
+    if (bsk.bskDft.empty()) throw std::invalid_argument("genBootstrappingKeyWWL24: empty key");
+    const int level = bsk.bskDft[0].l, s2Level = bsk.s2Dft.l;
+    std::vector<uint64_t> half, s2;
+    std::vector<int32_t> ksk;
+    yatfhe_dropin::draw_keys(YGPU_KEY_SUCCINCT, level, s2Level, half, &s2, ksk);
+    size_t o = 0;
+    for (auto& g : bsk.bskDft)
+        for (int lvl = 0; lvl < level; lvl++)
+            for (int col = 0; col <= p.k; col++) {
+                NttPolynomial& poly = col < p.k ? g.cPrime[lvl].a[col] : g.cPrime[lvl].b;
+                std::memcpy(poly.coeffs.data(), half.data() + o, sizeof(uint64_t) * p.N);
+                o += p.N;
+            }
+    o = 0;
+    for (int l2 = 0; l2 < s2Level; l2++)
+        for (int col = 0; col <= p.k; col++) {
+            NttPolynomial& poly = col < p.k ? bsk.s2Dft.trlweDfts[l2].a[col] : bsk.s2Dft.trlweDfts[l2].b;
+            std::memcpy(poly.coeffs.data(), s2.data() + o, sizeof(uint64_t) * p.N);
+            o += p.N;
+        }
 }
 
-// symEncTlwe (tlwe.cpp:20-31), seeded
+// symEncTlwe (tlwe.cpp:20-31): TUniform noise with the key's errorB, seeded
 inline void symEncTlwe(Tlwe& tlweSample, const Torus message, const TlweKey& key) {
     std::vector<int32_t> s(key.s.begin(), key.s.end());
     const int32_t mu = static_cast<int32_t>(static_cast<uint32_t>(static_cast<uint64_t>(message)));
     std::vector<int32_t> out(key.n + 1);
-    yatfhe_dropin::check(ygpu_lwe_encrypt(key.n, s.data(), yatfhe_dropin::next_seed(), -15.0, 1, &mu, out.data()));
+    yatfhe_dropin::check(ygpu_lwe_encrypt_tuniform(key.n, s.data(), yatfhe_dropin::next_seed(), key.errorB, 1, &mu, out.data()));
     for (int i = 0; i < key.n; i++) tlweSample.a[i] = out[i];
     tlweSample.b = out[key.n];
 }
@@ -536,95 +903,146 @@ inline Integer symDecTlweToInt(Tlwe& in, const TlweKey& key, const int torusBase
 }
 
 // ------------------------------------------------------------------ the path
-// blindRotateMP21Ntt (blind_rotate.cpp:1538-1565) / blindRotateJP22Ntt (1448-1478)
-inline void blindRotateMP21Ntt(Trlwe& accum, const BootstrappingKeyMP& bsk, const ScaledTlwe& input, const YatfheParameters&) {
-    auto& s = yatfhe_dropin::state();
-    std::lock_guard<std::mutex> lk(s.mu);
-    const int32_t id = yatfhe_dropin::device_key_bsk(bsk);
-    const int N = accum.N;
-    std::vector<Torus> acc(static_cast<size_t>(accum.k + 1) * N);
-    for (int c = 0; c < accum.k; c++) std::copy(accum.a[c].coeffs.begin(), accum.a[c].coeffs.end(), acc.begin() + static_cast<size_t>(c) * N);
-    std::copy(accum.b.coeffs.begin(), accum.b.coeffs.end(), acc.begin() + static_cast<size_t>(accum.k) * N);
-    yatfhe_dropin::check(ygpu_blind_rotate_scaled(s.ctx, id, input.a.data(), acc.data()));
-    for (int c = 0; c < accum.k; c++) std::copy(acc.begin() + static_cast<size_t>(c) * N, acc.begin() + static_cast<size_t>(c + 1) * N, accum.a[c].coeffs.begin());
-    std::copy(acc.begin() + static_cast<size_t>(accum.k) * N, acc.end(), accum.b.coeffs.begin());
+namespace yatfhe_dropin {
+inline void blind_rotate_on(std::pair<ygpu_ctx*, int32_t> key, Trlwe& accum, const ScaledTlwe& input) {
+    std::vector<Torus> acc;
+    to_flat(accum, acc);
+    check(ygpu_blind_rotate_scaled(key.first, key.second, input.a.data(), acc.data()));
+    from_flat(acc, accum);
 }
-inline void blindRotateJP22Ntt(Trlwe& accum, const BootstrappingKeyMP& bsk, const ScaledTlwe& input, const YatfheParameters& p) {
-    blindRotateMP21Ntt(accum, bsk, input, p);
+inline void bootstrap_on(std::pair<ygpu_ctx*, int32_t> key, Tlwe& out, const Tlwe& input, const TorusPolynomial& v) {
+    std::vector<int32_t> in(input.n + 1), o(out.n + 1);
+    for (int i = 0; i < input.n; i++) in[i] = static_cast<int32_t>(input.a[i]);
+    in[input.n] = static_cast<int32_t>(input.b);
+    check(ygpu_pbs(key.first, key.second, in.data(), v.coeffs.data(), o.data(), nullptr, nullptr));
+    for (int i = 0; i < out.n; i++) out.a[i] = o[i];
+    out.b = o[out.n];
 }
-// blindRotateNtt (blind_rotate.cpp:550-561), classic keys with group 1
-inline void blindRotateNtt(Trlwe& accum, const BootstrappingKey& bsk, const ScaledTlwe& input, const YatfheParameters& p) {
-    if (p.group != 1) throw std::invalid_argument("blindRotateNtt: only group == 1 keys are supported");
-    blindRotateMP21Ntt(accum, bsk, input, p);
+}  // namespace yatfhe_dropin
+
+// blindRotateMP21Ntt (blind_rotate.cpp:1538-1565), at the key's level bskDft[0][0].l
+inline void blindRotateMP21Ntt(Trlwe& accum, const std::vector<std::vector<TrgswMPDft>>& bskDft, const ScaledTlwe& input,
+                               const YatfheParameters& param) {
+    yatfhe_dropin::blind_rotate_on(yatfhe_dropin::mp_key(bskDft, nullptr, param), accum, input);
+}
+// blindRotateJP22Ntt (blind_rotate.cpp:1448-1478): the same arithmetic
+inline void blindRotateJP22Ntt(Trlwe& accum, const BootstrappingKeyMP& bsk, const ScaledTlwe& input, const YatfheParameters& param) {
+    blindRotateMP21Ntt(accum, bsk.bskDft, input, param);
+}
+// blindRotateNtt (blind_rotate.cpp:550-561 -> blindRotateNormalNtt, 202-212), group 1
+inline void blindRotateNtt(Trlwe& accum, const std::vector<TrgswDft>& bskDft, const ScaledTlwe& input, const YatfheParameters& param) {
+    yatfhe_dropin::blind_rotate_on(yatfhe_dropin::classic_key(bskDft, nullptr, param), accum, input);
 }
 
 // blindRotateWWL24Ntt (blind_rotate.cpp:1383-1412): the mask-digit rows the
 // reference derives at every step (switchTrlweToSecretEmbeddingNtt) are derived
 // once, on the device, when the key is first used.
-inline void blindRotateWWL24Ntt(Trlwe& accum, const BootstrappingKeyWWL24& bsk, const ScaledTlwe& input, const YatfheParameters&) {
-    auto& s = yatfhe_dropin::state();
-    std::lock_guard<std::mutex> lk(s.mu);
-    const int32_t id = yatfhe_dropin::device_key_wwl24(bsk);
-    const int N = accum.N;
-    std::vector<Torus> acc(static_cast<size_t>(accum.k + 1) * N);
-    for (int c = 0; c < accum.k; c++) std::copy(accum.a[c].coeffs.begin(), accum.a[c].coeffs.end(), acc.begin() + static_cast<size_t>(c) * N);
-    std::copy(accum.b.coeffs.begin(), accum.b.coeffs.end(), acc.begin() + static_cast<size_t>(accum.k) * N);
-    yatfhe_dropin::check(ygpu_blind_rotate_scaled(s.ctx, id, input.a.data(), acc.data()));
-    for (int c = 0; c < accum.k; c++) std::copy(acc.begin() + static_cast<size_t>(c) * N, acc.begin() + static_cast<size_t>(c + 1) * N, accum.a[c].coeffs.begin());
-    std::copy(acc.begin() + static_cast<size_t>(accum.k) * N, acc.end(), accum.b.coeffs.begin());
+inline void blindRotateWWL24Ntt(Trlwe& accum, const BootstrappingKeyWWL24& bsk, const ScaledTlwe& input, const YatfheParameters& param) {
+    using namespace yatfhe_dropin;
+    State& s = state();
+    std::pair<ygpu_ctx*, int32_t> key;
+    {
+        std::lock_guard<std::mutex> lk(s.mu);
+        require_init();
+        if (bsk.bskDft.empty() || static_cast<int>(bsk.bskDft.size()) != s.param.n)
+            throw std::invalid_argument("WWL+24 key has the wrong number of steps");
+        const int level = bsk.bskDft[0].l, s2Level = bsk.s2Dft.l;
+        const YatfheParameters& p = s.param;
+        const size_t per = static_cast<size_t>(level) * (p.k + 1) * p.N;
+        const uint64_t fp = fingerprint(bsk.bskDft.size() * per + 1, [&](size_t idx) -> uint64_t {
+            if (idx == bsk.bskDft.size() * per) return static_cast<uint64_t>(s2Level);
+            const size_t i = idx / per;
+            size_t r = idx % per;
+            const int lvl = static_cast<int>(r / ((p.k + 1) * p.N));
+            r %= (p.k + 1) * p.N;
+            const int col = static_cast<int>(r / p.N), j = static_cast<int>(r % p.N);
+            const TrlweDft& t = bsk.bskDft[i].cPrime.at(lvl);
+            return (col < p.k ? t.a.at(col) : t.b).coeffs.at(j);
+        });
+        ygpu_ctx* ctx = context_locked(level, param.driftPhase);
+        const auto ck = std::make_tuple(ctx, static_cast<const void*>(&bsk), fp, static_cast<const void*>(nullptr), uint64_t{1});
+        auto it = s.keys.find(ck);
+        if (it != s.keys.end()) {
+            key = {ctx, it->second};
+        } else {
+            std::vector<uint64_t> half(static_cast<size_t>(p.n) * per), s2(static_cast<size_t>(s2Level) * (p.k + 1) * p.N);
+            size_t o = 0;
+            for (const auto& g : bsk.bskDft)
+                for (int lvl = 0; lvl < level; lvl++)
+                    for (int col = 0; col <= p.k; col++) {
+                        const NttPolynomial& poly = col < p.k ? g.cPrime.at(lvl).a.at(col) : g.cPrime.at(lvl).b;
+                        std::memcpy(half.data() + o, poly.coeffs.data(), sizeof(uint64_t) * p.N);
+                        o += p.N;
+                    }
+            o = 0;
+            for (int l2 = 0; l2 < s2Level; l2++)
+                for (int col = 0; col <= p.k; col++) {
+                    const TrlweDft& t = bsk.s2Dft.trlweDfts.at(l2);
+                    const NttPolynomial& poly = col < p.k ? t.a.at(col) : t.b;
+                    std::memcpy(s2.data() + o, poly.coeffs.data(), sizeof(uint64_t) * p.N);
+                    o += p.N;
+                }
+            std::vector<int32_t> zero(static_cast<size_t>(p.k) * p.N * p.ksLevel * (p.n + 1), 0);
+            int32_t id = -1;
+            check(ygpu_key_upload_succinct(ctx, half.data(), s2.data(), s2Level, zero.data(), &id));
+            s.keys[ck] = id;
+            key = {ctx, id};
+        }
+    }
+    blind_rotate_on(key, accum, input);
 }
 
-// extractTlweFromTrlwe (trlwe.cpp:241-250), index 0, on the GPU
+// extractTlweFromTrlwe (trlwe.cpp:241-250) at `index`, on the GPU
 inline void extractTlweFromTrlwe(Tlwe& out, const Trlwe& in, const int index) {
-    if (index != 0) throw std::invalid_argument("extractTlweFromTrlwe: this engine extracts index 0 only");
-    auto& s = yatfhe_dropin::state();
-    std::lock_guard<std::mutex> lk(s.mu);
-    const int N = in.N;
-    std::vector<Torus> acc(static_cast<size_t>(in.k + 1) * N);
-    for (int c = 0; c < in.k; c++) std::copy(in.a[c].coeffs.begin(), in.a[c].coeffs.end(), acc.begin() + static_cast<size_t>(c) * N);
-    std::copy(in.b.coeffs.begin(), in.b.coeffs.end(), acc.begin() + static_cast<size_t>(in.k) * N);
-    std::vector<int32_t> o(static_cast<size_t>(in.k) * N + 1);
-    yatfhe_dropin::check(ygpu_sample_extract(s.ctx, acc.data(), o.data()));
-    for (int i = 0; i < in.k * N; i++) out.a[i] = o[i];
-    out.b = o[static_cast<size_t>(in.k) * N];
+    using namespace yatfhe_dropin;
+    ygpu_ctx* ctx;
+    {
+        State& s = state();
+        std::lock_guard<std::mutex> lk(s.mu);
+        require_init();
+        ctx = context_locked(s.param.lApprox, s.param.driftPhase);
+    }
+    std::vector<Torus> acc;
+    to_flat(in, acc);
+    std::vector<int32_t> o(static_cast<size_t>(in.k) * in.N + 1);
+    check(ygpu_sample_extract_index(ctx, acc.data(), index, o.data()));
+    for (int i = 0; i < in.k * in.N; i++) out.a[i] = o[i];
+    out.b = o[static_cast<size_t>(in.k) * in.N];
 }
 
 // switchKeyForTlwe (keyswitching.cpp:48-72)
-inline void switchKeyForTlwe(Tlwe& output, const TlweKeySwitchingKey& ksk, const Tlwe& input, const YatfheParameters&) {
-    auto& s = yatfhe_dropin::state();
-    std::lock_guard<std::mutex> lk(s.mu);
-    static BootstrappingKeyMP* zeroBsk = nullptr;
-    // a KSK travels with a BSK on the device; pair it with a zero BSK when used alone
-    if (!zeroBsk || zeroBsk->n != s.param.n) {
-        delete zeroBsk;
-        zeroBsk = new BootstrappingKeyMP(s.param, s.param.lApprox);
-        zeroBsk->generation = s.nextGeneration++;
+inline void switchKeyForTlwe(Tlwe& output, const TlweKeySwitchingKey& ksk, const Tlwe& input, const YatfheParameters& param) {
+    using namespace yatfhe_dropin;
+    std::pair<ygpu_ctx*, int32_t> key;
+    {
+        State& s = state();
+        std::lock_guard<std::mutex> lk(s.mu);
+        require_init();
+        // a KSK travels with a BSK on the device: pair it with an all-zero BSK
+        static const int zeroTag = 0;
+        const YatfheParameters& p = s.param;
+        const int level = p.lApprox;
+        key = device_key_locked(level, param.driftPhase, &zeroTag, 0,
+                                [&](std::vector<uint64_t>& out) { out.assign(static_cast<size_t>(p.n) * level * (p.k + 1) * (p.k + 1) * p.N, 0); },
+                                &ksk);
     }
-    const int32_t id = yatfhe_dropin::device_key(*zeroBsk, ksk);
     std::vector<int32_t> in(input.n + 1), out(output.n + 1);
     for (int i = 0; i < input.n; i++) in[i] = static_cast<int32_t>(input.a[i]);
     in[input.n] = static_cast<int32_t>(input.b);
-    yatfhe_dropin::check(ygpu_keyswitch(s.ctx, id, in.data(), out.data()));
+    check(ygpu_keyswitch(key.first, key.second, in.data(), out.data()));
     for (int i = 0; i < output.n; i++) output.a[i] = out[i];
     output.b = out[output.n];
 }
 
-// functionalBootstrappingMPNtt (bootstrapping.cpp:71-80): the whole path in one
-// GPU call (rescale, accumulator init, blind rotation, extract, keyswitch).
-inline void functionalBootstrappingMPNtt(Tlwe& out, const Tlwe& input, const BootstrappingKeyMP& bsk,
-                                         const TlweKeySwitchingKey& ksk, const TorusPolynomial& v, const YatfheParameters&) {
-    auto& s = yatfhe_dropin::state();
-    std::lock_guard<std::mutex> lk(s.mu);
-    const int32_t id = yatfhe_dropin::device_key(bsk, ksk);
-    std::vector<int32_t> in(input.n + 1), o(out.n + 1);
-    for (int i = 0; i < input.n; i++) in[i] = static_cast<int32_t>(input.a[i]);
-    in[input.n] = static_cast<int32_t>(input.b);
-    yatfhe_dropin::check(ygpu_pbs(s.ctx, id, in.data(), v.coeffs.data(), o.data(), nullptr, nullptr));
-    for (int i = 0; i < out.n; i++) out.a[i] = o[i];
-    out.b = o[out.n];
+// functionalBootstrappingMPNtt (bootstrapping.cpp:71-80): rescale, accumulator
+// init, blind rotation at the key's level, extraction at param.driftPhase and
+// keyswitch, in one GPU call.
+inline void functionalBootstrappingMPNtt(Tlwe& out, const Tlwe& input, const BootstrappingKeyMP& bsk, const TlweKeySwitchingKey& ksk,
+                                         const TorusPolynomial& v, const YatfheParameters& param) {
+    yatfhe_dropin::bootstrap_on(yatfhe_dropin::mp_key(bsk.bskDft, &ksk, param), out, input, v);
 }
 // functionalBootstrappingNtt (bootstrapping.cpp:60-69), classic keys, l == lApprox
-inline void functionalBootstrappingNtt(Tlwe& out, const Tlwe& input, const BootstrappingKey& bsk,
-                                       const TlweKeySwitchingKey& ksk, const TorusPolynomial& v, const YatfheParameters& p) {
-    functionalBootstrappingMPNtt(out, input, bsk, ksk, v, p);
+inline void functionalBootstrappingNtt(Tlwe& out, const Tlwe& input, const BootstrappingKey& bsk, const TlweKeySwitchingKey& ksk,
+                                       const TorusPolynomial& v, const YatfheParameters& param) {
+    yatfhe_dropin::bootstrap_on(yatfhe_dropin::classic_key(bsk.bskDft, &ksk, param), out, input, v);
 }

@@ -283,6 +283,53 @@ __device__ __forceinline__ void digits_of(int64_t x, int l, int rb, int (&out)[4
     }
 }
 
+// All l balanced digits of a TBITS-bit torus value for a gadget shape known
+// only at run time, packed for the level loop: 8-bit fields when rb <= 7
+// (levels 0-3 in w0, 4-7 in w1, so up to 8 levels), else 16-bit fields
+// (levels 0-1 in w0, 2-3 in w1, up to 4 levels).  Same digits as digits_of.
+template <int TBITS>
+__device__ __forceinline__ void digits_runtime_packed(int64_t x, int l, int rb, uint32_t& w0, uint32_t& w1) {
+    using U = typename std::conditional<TBITS == 32, uint32_t, uint64_t>::type;
+    const U mask = (TBITS >= static_cast<int>(sizeof(U) * 8)) ? ~U{0} : static_cast<U>((uint64_t{1} << (TBITS % 64)) - 1);
+    const U u = static_cast<U>(x) & mask;
+    const bool neg = (u >> (TBITS - 1)) & 1;
+    const int dropped = TBITS - l * rb;
+    const U bias = dropped > 0 ? (U{1} << (dropped - 1)) : U{0};
+    const U m = (neg ? ((U{0} - u) & mask) : u) + bias;
+    const int B = 1 << rb, halfB = B >> 1;
+    const bool p8 = rb <= 7;
+    int carry = 0;
+    w0 = 0;
+    w1 = 0;
+#pragma unroll
+    for (int lvl = 7; lvl >= 0; lvl--) {
+        if (lvl < l) {
+            const int shift = TBITS - (lvl + 1) * rb;
+            const int window = static_cast<int>((m >> shift) & static_cast<U>(B - 1));
+            const int digit = window + carry;
+            const int tie = lvl > 0 ? static_cast<int>((m >> (shift + 2 * rb - 1)) & 1) : 0;
+            carry = (digit + tie) > halfB;
+            const int bal = carry ? digit - B : digit;
+            const uint32_t sd = static_cast<uint32_t>(neg ? -bal : bal);
+            if (p8) {
+                if (lvl < 4) w0 |= (sd & 0xFFu) << (8 * lvl);
+                else w1 |= (sd & 0xFFu) << (8 * (lvl - 4));
+            } else {
+                if (lvl < 2) w0 |= (sd & 0xFFFFu) << (16 * lvl);
+                else w1 |= (sd & 0xFFFFu) << (16 * (lvl - 2));
+            }
+        }
+    }
+}
+
+// Digit `lvl` of digits_runtime_packed's words, sign-extended by a shift and a
+// narrowing cast (not __builtin_amdgcn_sbfe: with a wave-uniform but
+// non-constant offset hipcc has converted its result to double as unsigned).
+__device__ __forceinline__ int digit_runtime(uint32_t w0, uint32_t w1, int lvl, int rb) {
+    if (rb <= 7) return static_cast<int>(static_cast<int8_t>((lvl < 4 ? w0 : w1) >> (8 * (lvl & 3))));
+    return static_cast<int>(static_cast<int16_t>((lvl < 2 ? w0 : w1) >> (16 * (lvl & 1))));
+}
+
 // Exact conversions of an integer-valued double: adding 1.5 * 2^52 keeps the
 // sum inside one binade, where the mantissa field is the integer plus 2^51.
 __device__ __forceinline__ int64_t f64_to_i64(double v) {  // |v| < 2^51

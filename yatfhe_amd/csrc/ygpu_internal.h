@@ -12,6 +12,7 @@
 #include <atomic>
 #include <memory>
 #include <functional>
+#include <condition_variable>
 #include <cstdlib>
 #include <algorithm>
 #include "yatfhe_gpu.h"
@@ -41,7 +42,40 @@ struct KeyRec {
     double* d_bsk = nullptr;
     int32_t* d_ksk = nullptr;
     size_t bytes = 0;
+    int users = 0;  // synchronous calls running on a plain lane with this key
 };
+
+// One calling thread's resources for the synchronous host-buffer entry
+// points (ygpu_pbs and its parts): a stream, one ciphertext's device buffers
+// and a failure-report word of its own.  Threads that call at the same time
+// each take a lane, so their bootstraps run side by side on the device
+// instead of queueing behind one lock.
+struct PlainLane {
+    bool busy = false;
+    // the context's settings as the current call found them
+    BrTile tile{};
+    int extract_index = 0;
+    int barrier_polls = TEAM_BARRIER_POLLS;
+    hipStream_t stream = nullptr;
+    int32_t* d_lwe = nullptr;
+    void* d_v = nullptr;
+    int32_t* d_out = nullptr;
+    int32_t* d_ext = nullptr;
+    void* d_acc = nullptr;
+    void* d_accin = nullptr;
+    int32_t* d_partial = nullptr;   // keyswitch workspace of one ciphertext
+    uint32_t* h_status = nullptr;   // pinned; the lane's barrier-failure report
+    uint32_t* hd_status = nullptr;
+};
+
+inline void free_lane(PlainLane& L) {
+    if (L.stream) hipStreamDestroy(L.stream);
+    for (void* p : {static_cast<void*>(L.d_lwe), L.d_v, static_cast<void*>(L.d_out), static_cast<void*>(L.d_ext), L.d_acc,
+                    L.d_accin, static_cast<void*>(L.d_partial)})
+        if (p) hipFree(p);
+    if (L.h_status) hipHostFree(L.h_status);
+    L = PlainLane{};
+}
 
 struct Timing {
     std::string name;
@@ -102,6 +136,7 @@ struct ygpu_ctx {
     // this device's copy of the library's static scratch array (the device
     // address hipGetSymbolAddress returned at init; null if that failed)
     void* d_static_scratch = nullptr;
+    int extract_index = 0;  // extraction coefficient of the bootstraps (param.driftPhase)
     void* d_acc = nullptr;
     size_t acc_cap = 0;
     int32_t* d_partial = nullptr;
@@ -117,7 +152,10 @@ struct ygpu_ctx {
     void* d_stage = nullptr;   // packed key staging for uploads
     size_t stage_cap = 0;
     std::mutex mu;
-    bool timing = false;
+    std::condition_variable cv;   // a plain lane or a key's last user finished
+    std::vector<std::unique_ptr<PlainLane>> lanes;
+    std::atomic<bool> timing{false};
+    std::mutex timing_mu;         // pending, event_pool, stats
     std::vector<Timing> pending;
     std::vector<hipEvent_t> event_pool;
     std::map<std::string, std::pair<int64_t, double>> stats;
@@ -132,9 +170,12 @@ inline int check_params(const ygpu_params& p) {
     if (p.k != 1) return fail(YGPU_ERR_UNSUPPORTED, "only GLWE dimension k = 1 is supported");
     if (p.N != 1024 && p.N != 2048) return fail(YGPU_ERR_UNSUPPORTED, "ring degree N must be 1024 or 2048");
     if (p.n < 1 || p.n > KS_MAX_N) return fail(YGPU_ERR_UNSUPPORTED, "LWE dimension n must be in [1, 1023]");
-    if (p.level < 1 || p.level > 4) return fail(YGPU_ERR_UNSUPPORTED, "gadget level must be in [1, 4]");
     // digits lie in [-B/2, B/2] and travel in 16-bit fields: B/2 <= 2^14
     if (p.radix_bits < 1 || p.radix_bits > 15) return fail(YGPU_ERR_UNSUPPORTED, "radix_bits must be in [1, 15]");
+    // two 32-bit words of digits per position: 8-bit fields (B <= 2^7, up to 8
+    // levels) or 16-bit fields (up to 4 levels)
+    if (p.level < 1 || p.level > (p.radix_bits <= 7 ? 8 : 4))
+        return fail(YGPU_ERR_UNSUPPORTED, "gadget level must be in [1, 8] for radix_bits <= 7 and in [1, 4] above");
     // decomposeRow's own checks (gadget_decomposition.h:194-206)
     if (p.level * p.radix_bits > p.torus_bits)
         return fail(YGPU_ERR_INVALID_ARG, "decomposeRow: " + std::to_string(p.level) + " digits of " + std::to_string(p.radix_bits) +
@@ -168,6 +209,7 @@ inline int with_ctx(ygpu_ctx* ctx, F&& f) {
     }
 }
 
+// timing_mu held.
 inline hipEvent_t take_event(ygpu_ctx* ctx) {
     if (!ctx->event_pool.empty()) {
         hipEvent_t e = ctx->event_pool.back();
@@ -184,15 +226,21 @@ inline hipEvent_t take_event(ygpu_ctx* ctx) {
 template <class F>
 hipError_t timed(ygpu_ctx* ctx, const char* name, hipStream_t stream, F&& f) {
     if (!ctx->timing || ctx->capturing) return f();
-    Timing t{name, take_event(ctx), take_event(ctx)};
+    Timing t;
+    {
+        std::lock_guard<std::mutex> lk(ctx->timing_mu);
+        t = Timing{name, take_event(ctx), take_event(ctx)};
+    }
     hipEventRecord(t.a, stream);
     hipError_t e = f();
     hipEventRecord(t.b, stream);
+    std::lock_guard<std::mutex> lk(ctx->timing_mu);
     ctx->pending.push_back(t);
     return e;
 }
 
 inline int resolve_timing(ygpu_ctx* ctx) {
+    std::lock_guard<std::mutex> lk(ctx->timing_mu);
     for (auto& t : ctx->pending) {
         YG_HIP(hipEventSynchronize(t.b));
         float ms = 0;
@@ -312,6 +360,7 @@ inline KsArgs ks_args(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t
     a.partial = workspace;
     a.digits = workspace ? reinterpret_cast<uint32_t*>(workspace) + (static_cast<size_t>(count) * (ctx->p.n + 1) + 63) / 64 * 64
                          : nullptr;
+    a.extract_index = ctx->extract_index;
     return a;
 }
 
@@ -325,8 +374,9 @@ inline int ks_slices(const ygpu_ctx* ctx, int count) {
 
 // Extract + keyswitch of `count` accumulators.  `workspace` holds
 // ks_workspace_bytes(count); null selects the context's buffer.
+// extract_index < 0 takes the context's.
 inline int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* const* table, const int32_t* slots, int32_t* d_out,
-           int32_t* d_ext, int32_t* workspace = nullptr, hipStream_t stream = nullptr) {
+           int32_t* d_ext, int32_t* workspace = nullptr, hipStream_t stream = nullptr, int extract_index = -1) {
     hipStream_t st = stream ? stream : ctx->stream;
     const int slices = ks_slices(ctx, count);
     if (!workspace) {
@@ -335,6 +385,7 @@ inline int run_ks(ygpu_ctx* ctx, int count, const void* d_acc, const int32_t* co
         workspace = ctx->d_partial;
     }
     KsArgs a = ks_args(ctx, count, d_acc, table, slots, d_out, workspace);
+    if (extract_index >= 0) a.extract_index = extract_index;
     YG_HIP(timed(ctx, "k_extract_ks", st, [&] { return launch_extract_ks(ctx->tbits, a, slices, st); }));
     if (d_ext) YG_HIP(timed(ctx, "k_extract", st, [&] { return launch_extract(ctx->tbits, a, d_ext, st); }));
     return YGPU_OK;

@@ -85,6 +85,7 @@ struct KsArgs {
     int32_t* lwe_out;                  // [count][n+1]
     int32_t* partial;                  // [count][n+1] (slices > 1)
     uint32_t* digits;                  // [groups][count]: the keyswitch digits of four rows per word
+    int extract_index;                 // coefficient the sample is extracted at (param.driftPhase), in [0, N)
 };
 
 // Words of the keyswitch workspace of `count` ciphertexts: the sliced partial

@@ -129,10 +129,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 if constexpr (C::L > 0) {
                     digits_packed<32, C::L, C::RB>(static_cast<int32_t>(rot), dg0[r], dg1[r]);
                 } else {
-                    int d4[4] = {0, 0, 0, 0};
-                    digits_of<32>(static_cast<int32_t>(rot), L, rb, d4);
-                    dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
-                    dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+                    digits_runtime_packed<32>(static_cast<int32_t>(rot), L, rb, dg0[r], dg1[r]);
                 }
             } else {
                 const int64_t av = static_cast<int64_t>(stage[src]);
@@ -140,10 +137,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 if constexpr (C::L > 0) {
                     digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
                 } else {
-                    int d4[4] = {0, 0, 0, 0};
-                    digits_of<C::TBITS>(rot, L, rb, d4);
-                    dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
-                    dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+                    digits_runtime_packed<C::TBITS>(rot, L, rb, dg0[r], dg1[r]);
                 }
             }
         }
@@ -182,9 +176,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     int dv;
                     if constexpr (C::L > 0 && C::RB <= 7) {
                         dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
-                    } else {
+                    } else if constexpr (C::L > 0) {
                         const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
                         dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
+                    } else {
+                        dv = digit_runtime(dg0[r], dg1[r], lvl, rb);
                     }
                     x[r] = static_cast<double>(dv);
                 }
@@ -202,6 +198,15 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
                 for (int r = 0; r < 16; r++) oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
+                if constexpr (C::L == 0) {
+                    // more than four levels: reduce the partial sums once after
+                    // four, so that with the partner's they stay below 16p
+                    // (each product is at most 1.34p)
+                    if (L > 4 && lvl == 3) {
+#pragma unroll
+                        for (int r = 0; r < 16; r++) { own[r] = reduce(own[r], pr); oth[r] = reduce(oth[r], pr); }
+                    }
+                }
             }
 
             // partner sums: hand column 1-tm to the other team, gather the full
@@ -255,9 +260,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 }
 
 // ============================================================ keyswitch
-// switchKeyForTlwe (keyswitching.cpp:48-72) of the sample extracted at index 0
-// (extractTlweFromTrlwe, trlwe.cpp:241-250):
-//   a'_0 = a_0, a'_j = -a_{N-j}, b' = b_0, each modswitched 2^w -> 2^32;
+// switchKeyForTlwe (keyswitching.cpp:48-72) of the sample extracted at index
+// idx (extractTlweFromTrlwe, trlwe.cpp:241-250; idx = param.driftPhase, 0 by
+// default):
+//   a'_j = a_{idx-j} (j <= idx), a'_j = -a_{N+idx-j} (j > idx), b' = b_idx,
+//   each modswitched 2^w -> 2^32;
 //   out.a[k] = -sum_{i,j} d_{i,j} KSK[i][j].a[k],  out.b = b' - sum d KSK[i][j].b
 // (mod 2^32; the reference's int64 sum + longModP is the same residue).
 //
@@ -269,11 +276,16 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 // laid out so that one MFMA fragment (16 consecutive rows of one ciphertext
 // or one column) is 16 contiguous bytes: digits [ct][group], planes
 // [plane][col][group], a group being four rows packed one byte each.
+// Coefficient j of the sample extracted at index `idx` from the polynomial at
+// `base` (extractTlweFromTrlwe, trlwe.cpp:241-250): acc[idx - j] for j <= idx,
+// else -acc[N + idx - j] (negated in the Torus type: INT32_MIN wraps at 32
+// bits), then modswitched to 2^32.  The body is coefficient 0 of the body
+// polynomial's form: acc_b[idx].
 template <int TBITS>
-__device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base, int N, int j) {
+__device__ __forceinline__ int32_t extracted_coeff(const void* accv, size_t base, int N, int j, int idx) {
     using TorusT = typename std::conditional<TBITS == 32, int32_t, int64_t>::type;
     const TorusT* acc = static_cast<const TorusT*>(accv) + base;
-    const int64_t x = (j == 0) ? static_cast<int64_t>(acc[0]) : sext_t<TBITS == 32 ? 32 : 64>(-static_cast<int64_t>(acc[N - j]));
+    const int64_t x = (j <= idx) ? static_cast<int64_t>(acc[idx - j]) : sext_t<TBITS == 32 ? 32 : 64>(-static_cast<int64_t>(acc[N + idx - j]));
     return torus_to_lwe32<TBITS>(x);
 }
 
@@ -299,7 +311,7 @@ __global__ void __launch_bounds__(256) k_ks_digits(KsArgs a) {
             if (row >= rows) break;
             const int i = row / t, j = row % t;
             if (i != lastI) {
-                const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N);
+                const int32_t x = extracted_coeff<TBITS>(a.acc, accBase + static_cast<size_t>(i / N) * N, N, i % N, a.extract_index);
                 ks_digits(x, t, a.ks_radix_bits, a.ks_width_bits, dig);
                 lastI = i;
             }
@@ -410,7 +422,7 @@ __global__ void __launch_bounds__(KS_THREADS) k_extract_ks(KsArgs a) {
                         if (col < n) {
                             o = 0u - sum;
                         } else {
-                            const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0);
+                            const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * N + static_cast<size_t>(a.k) * N, N, 0, a.extract_index);
                             o = static_cast<uint32_t>(b) - sum;
                         }
                         a.lwe_out[static_cast<size_t>(ct) * cols + col] = static_cast<int32_t>(o);
@@ -433,7 +445,7 @@ __global__ void k_ks_finalize(KsArgs a) {
         if (col < a.n) {
             o = 0u - s;
         } else {
-            const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * a.N + static_cast<size_t>(a.k) * a.N, a.N, 0);
+            const int32_t b = extracted_coeff<TBITS>(a.acc, static_cast<size_t>(ct) * (a.k + 1) * a.N + static_cast<size_t>(a.k) * a.N, a.N, 0, a.extract_index);
             o = static_cast<uint32_t>(b) - s;
         }
         a.lwe_out[idx] = static_cast<int32_t>(o);
@@ -452,8 +464,8 @@ __global__ void k_extract(KsArgs a, int32_t* out) {
         const int ct = static_cast<int>(idx / len), i = static_cast<int>(idx % len);
         const size_t base = static_cast<size_t>(ct) * (a.k + 1) * a.N;
         int32_t v;
-        if (i < a.k * a.N) v = extracted_coeff<TBITS>(a.acc, base + static_cast<size_t>(i / a.N) * a.N, a.N, i % a.N);
-        else v = extracted_coeff<TBITS>(a.acc, base + static_cast<size_t>(a.k) * a.N, a.N, 0);
+        if (i < a.k * a.N) v = extracted_coeff<TBITS>(a.acc, base + static_cast<size_t>(i / a.N) * a.N, a.N, i % a.N, a.extract_index);
+        else v = extracted_coeff<TBITS>(a.acc, base + static_cast<size_t>(a.k) * a.N, a.N, 0, a.extract_index);
         out[idx] = v;
     }
 }

@@ -127,10 +127,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             if constexpr (C::L > 0) {
                 digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
             } else {
-                int d4[4] = {0, 0, 0, 0};
-                digits_of<C::TBITS>(rot, L, rb, d4);
-                dg0[r] = (static_cast<uint32_t>(d4[0]) & 0xFFFFu) | (static_cast<uint32_t>(d4[1]) << 16);
-                dg1[r] = (static_cast<uint32_t>(d4[2]) & 0xFFFFu) | (static_cast<uint32_t>(d4[3]) << 16);
+                digits_runtime_packed<C::TBITS>(rot, L, rb, dg0[r], dg1[r]);
             }
         };
         {
@@ -188,9 +185,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                 int dv;
                 if constexpr (C::L > 0 && C::RB <= 7) {
                     dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
-                } else {
+                } else if constexpr (C::L > 0) {
                     const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
                     dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
+                } else {
+                    dv = digit_runtime(dg0[r], dg1[r], lvl, rb);
                 }
                 x[r] = static_cast<double>(dv);
             }
@@ -208,6 +207,14 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             for (int r = 0; r < 16; r++) kb[r] = kpt[r * T];
 #pragma unroll
             for (int r = 0; r < 16; r++) oth[r] = __dadd_rn(oth[r], mulmod(x[r], kb[r], pr));
+            if constexpr (C::L == 0) {
+                // more than four levels: reduce the partial sums once after
+                // four, so that with the partner's they stay below 16p
+                if (L > 4 && lvl == 3) {
+#pragma unroll
+                    for (int r = 0; r < 16; r++) { own[r] = reduce(own[r], pr); oth[r] = reduce(oth[r], pr); }
+                }
+            }
         }
 
         // partner sums (same prime): hand column 1-tm to team (1-tm, tp)

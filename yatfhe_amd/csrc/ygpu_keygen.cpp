@@ -3,7 +3,10 @@
 // The reference draws keys from an OS-seeded ChaCha20 (csprng.cpp:63-72) with
 // no seed API, so its key bits cannot be reproduced; this generator follows the
 // same encryption equations with a seeded, thread-count-independent stream per
-// ciphertext row, and Gaussian noise rounded to integers.
+// ciphertext row.  Noise is either Gaussian, rounded to integers, or the
+// reference's own TUniform law (genTUniformNoise, numeric.cpp:47-55): integers
+// in [-2^b, 2^b], the end points with probability 2^-(b+2) and every other
+// point with probability 2^-(b+1).
 //   genTlweKey / genTrlweKey        binary keys (tlwe.cpp, trlwe.cpp:70-81)
 //   encryptTrgswMPNtt               (trgsw.cpp:36-51): row r < k encrypts 0 and adds
 //                                   mu_lvl to its mask polynomial a_r; row k (cPrime)
@@ -54,6 +57,34 @@ inline uint64_t lift(int64_t v, uint64_t q) {
     return static_cast<uint64_t>(m);
 }
 
+// A noise law in the units of the torus it is added to.
+struct Noise {
+    bool tuniform = false;
+    double sigma = 0;  // Gaussian standard deviation
+    int b = 0;         // TUniform bound exponent
+    static Noise gaussian_log2(double sigma_log2_fraction, int torus_bits) {
+        Noise z;
+        z.sigma = std::pow(2.0, sigma_log2_fraction + torus_bits);
+        return z;
+    }
+    static Noise tuniform_bound(int b) {
+        Noise z;
+        z.tuniform = true;
+        z.b = b;
+        return z;
+    }
+    // genTUniformNoise (numeric.cpp:47-55): a uniform (b+2)-bit candidate c,
+    // (c >> 1) + (c & 1) - 2^b.
+    int64_t draw(Rng& r) const {
+        if (!tuniform) return r.gauss(sigma);
+        const int bits = b + 2;
+        const uint64_t mask = bits >= 64 ? ~0ULL : ((uint64_t{1} << bits) - 1);
+        uint64_t c = r.next() & mask;
+        c = (c >> 1) + (c & 1);
+        return static_cast<int64_t>(c) - (int64_t{1} << b);
+    }
+};
+
 // stream identifiers
 constexpr uint64_t S_LWE_KEY = 1, S_GLWE_KEY = 2;
 constexpr uint64_t S_BSK = 1ULL << 40, S_KSK = 2ULL << 40, S_ENC = 3ULL << 40;
@@ -72,10 +103,9 @@ void gen_secret_keys(const ygpu_params& p, uint64_t seed, int32_t* lwe_key, int3
 }
 
 // genTlweKeySwitchingKey (keyswitching.cpp:19-29): rows [i][j][n+1], i < kN.
-void gen_ksk(const ygpu_params& p, uint64_t seed, double sigma_log2_lwe, const int32_t* lwe_key, const int32_t* glwe_key,
+void gen_ksk(const ygpu_params& p, uint64_t seed, const Noise& nl, const int32_t* lwe_key, const int32_t* glwe_key,
              int32_t* ksk, int nt) {
     const int n = p.n, t = p.ks_level;
-    const double sl = std::pow(2.0, sigma_log2_lwe + 32);
     par_for(p.k * p.N * t, nt, [&](int idx) {
         const int i = idx / t, j = idx % t;
         Rng r(seed, S_KSK + static_cast<uint64_t>(idx));
@@ -86,7 +116,7 @@ void gen_ksk(const ygpu_params& p, uint64_t seed, double sigma_log2_lwe, const i
             if (lwe_key[u]) dot += static_cast<uint32_t>(out[u]);
         }
         const uint32_t m = static_cast<uint32_t>(glwe_key[i]) << (p.ks_width_bits - (j + 1) * p.ks_radix_bits);
-        const uint32_t e = static_cast<uint32_t>(static_cast<int64_t>(r.gauss(sl)));
+        const uint32_t e = static_cast<uint32_t>(nl.draw(r));
         out[n] = static_cast<int32_t>(dot + e + m);
     });
 }
@@ -107,12 +137,8 @@ void torus_mul_binary_acc(int N, int w, const int64_t* a, const int32_t* z, int6
     for (int j = 0; j < N; j++) acc[j] = sext(static_cast<int64_t>(static_cast<uint64_t>(acc[j]) + sum[j]), w);
 }
 
-}  // namespace
-
-extern "C" {
-
-int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t* lwe_key,
-                int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
+int keygen_mp(const ygpu_params* params, uint64_t seed, const Noise& nl, const Noise& ng, int32_t* lwe_key,
+              int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
     if (!params || !lwe_key || !glwe_key || !bsk_coeff || !ksk) return YGPU_ERR_INVALID_ARG;
     const ygpu_params p = *params;
     if (p.k < 1 || p.N < 2 || p.n < 1 || p.level < 1 || p.ks_level < 1) return YGPU_ERR_INVALID_ARG;
@@ -127,7 +153,6 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
             for (int j = 0; j < N; j++) zt[c][j] = static_cast<uint64_t>(glwe_key[c * N + j]);
             ntt.forward(zt[c].data());
         }
-        const double sg = std::pow(2.0, sigma_log2_glwe + w);
         const int nt = worker_threads(threads);
         // BSK rows: [i][lvl][row][col][N]
         par_for(n * L * K1, nt, [&](int idx) {
@@ -146,7 +171,7 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
                 ntt.inverse(tmp.data());
                 for (int j = 0; j < N; j++) body[j] = (body[j] + tmp[j]) % q;
             }
-            for (int j = 0; j < N; j++) body[j] = (body[j] + lift(r.gauss(sg), q)) % q;
+            for (int j = 0; j < N; j++) body[j] = (body[j] + lift(ng.draw(r), q)) % q;
             // mu_lvl = s_i * 2^(w - (lvl+1) b) as a torus value, lifted mod q
             const int64_t mu = sext(static_cast<int64_t>(lwe_key[i]) << (w - (lvl + 1) * p.radix_bits), w);
             if (row == K) {
@@ -157,15 +182,15 @@ int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe,
             }
             std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
         });
-        gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
+        gen_ksk(p, seed, nl, lwe_key, glwe_key, ksk, nt);
     } catch (...) {
         return YGPU_ERR_INVALID_ARG;
     }
     return YGPU_OK;
 }
 
-int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t s2_level,
-                         int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_half, uint64_t* s2, int32_t* ksk, int32_t threads) {
+int keygen_succinct(const ygpu_params* params, uint64_t seed, const Noise& nl, const Noise& ng, int32_t s2_level,
+                    int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_half, uint64_t* s2, int32_t* ksk, int32_t threads) {
     if (!params || !lwe_key || !glwe_key || !bsk_half || !s2 || !ksk) return YGPU_ERR_INVALID_ARG;
     const ygpu_params p = *params;
     if (p.k < 1 || p.N < 2 || p.n < 1 || p.level < 1 || p.ks_level < 1) return YGPU_ERR_INVALID_ARG;
@@ -180,7 +205,6 @@ int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_
             for (int j = 0; j < N; j++) zt[c][j] = static_cast<uint64_t>(glwe_key[c * N + j]);
             ntt.forward(zt[c].data());
         }
-        const double sg = std::pow(2.0, sigma_log2_glwe + w);
         const int nt = worker_threads(threads);
         // cPrime rows [i][lvl][col][N] (genBootstrappingKeyWWL24, bootstrapping.cpp:106-126):
         // symEncTrlweMultiSampleNtt of mu = s_i 2^(w - (lvl+1) b) at coefficient 0
@@ -200,7 +224,7 @@ int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_
             }
             const int64_t mu = sext(static_cast<int64_t>(lwe_key[i]) << (w - (lvl + 1) * p.radix_bits), w);
             for (int j = 0; j < N; j++) {
-                const int64_t bt = sext(r.gauss(sg) + (j == 0 ? mu : 0), w);
+                const int64_t bt = sext(ng.draw(r) + (j == 0 ? mu : 0), w);
                 body[j] = (body[j] + lift(bt, q)) % q;
             }
             std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
@@ -210,7 +234,7 @@ int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_
         par_for(s2_level, nt, [&](int l2) {
             Rng r(seed, S_S2 + static_cast<uint64_t>(l2));
             std::vector<int64_t> a(static_cast<size_t>(K) * N), b(N);
-            for (int j = 0; j < N; j++) b[j] = sext(r.gauss(sg), w);
+            for (int j = 0; j < N; j++) b[j] = sext(ng.draw(r), w);
             for (int c = 0; c < K; c++) {
                 int64_t* ac = &a[static_cast<size_t>(c) * N];
                 for (int j = 0; j < N; j++) ac[j] = sext(static_cast<int64_t>(r.next()), w);
@@ -226,15 +250,15 @@ int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_
             }
             for (int j = 0; j < N; j++) out[static_cast<size_t>(K) * N + j] = lift(b[j], q);
         });
-        gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
+        gen_ksk(p, seed, nl, lwe_key, glwe_key, ksk, nt);
     } catch (...) {
         return YGPU_ERR_INVALID_ARG;
     }
     return YGPU_OK;
 }
 
-int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t* lwe_key,
-                        int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
+int keygen_classic(const ygpu_params* params, uint64_t seed, const Noise& nl, const Noise& ng, int32_t* lwe_key,
+                   int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
     if (!params || !lwe_key || !glwe_key || !bsk_coeff || !ksk) return YGPU_ERR_INVALID_ARG;
     const ygpu_params p = *params;
     if (p.k < 1 || p.N < 2 || p.n < 1 || p.level < 1 || p.ks_level < 1) return YGPU_ERR_INVALID_ARG;
@@ -248,7 +272,6 @@ int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_l
             for (int j = 0; j < N; j++) zt[c][j] = static_cast<uint64_t>(glwe_key[c * N + j]);
             ntt.forward(zt[c].data());
         }
-        const double sg = std::pow(2.0, sigma_log2_glwe + w);
         const int nt = worker_threads(threads);
         const uint64_t qhalf = (q - 1) / 2;
         // rows [i][lvl][row][col][N]: encryptTrgswNtt (trgsw.cpp:281-284) =
@@ -264,7 +287,7 @@ int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_l
             uint64_t* out = bsk_coeff + static_cast<size_t>(idx) * K1 * N;
             std::vector<int64_t> at(static_cast<size_t>(K) * N);
             std::vector<uint64_t> body(N, 0), tmp(N);
-            for (int j = 0; j < N; j++) body[j] = lift(sext(r.gauss(sg), w), q);
+            for (int j = 0; j < N; j++) body[j] = lift(sext(ng.draw(r), w), q);
             for (int c = 0; c < K; c++) {
                 uint64_t* a = out + static_cast<size_t>(c) * N;
                 for (int j = 0; j < N; j++) {
@@ -291,17 +314,16 @@ int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_l
             }
             std::memcpy(out + static_cast<size_t>(K) * N, body.data(), sizeof(uint64_t) * N);
         });
-        gen_ksk(p, seed, sigma_log2_lwe, lwe_key, glwe_key, ksk, nt);
+        gen_ksk(p, seed, nl, lwe_key, glwe_key, ksk, nt);
     } catch (...) {
         return YGPU_ERR_INVALID_ARG;
     }
     return YGPU_OK;
 }
 
-int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count, const int32_t* mu,
-                     int32_t* out) {
+int lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, const Noise& nl, int32_t count, const int32_t* mu,
+                int32_t* out) {
     if (n < 1 || !lwe_key || count < 0 || (!mu && count) || (!out && count)) return YGPU_ERR_INVALID_ARG;
-    const double s = std::pow(2.0, sigma_log2 + 32);
     for (int c = 0; c < count; c++) {
         Rng r(seed, S_ENC + static_cast<uint64_t>(c));
         int32_t* o = out + static_cast<size_t>(c) * (n + 1);
@@ -310,10 +332,64 @@ int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double si
             o[u] = static_cast<int32_t>(static_cast<uint32_t>(r.next()));
             if (lwe_key[u]) dot += static_cast<uint32_t>(o[u]);
         }
-        const uint32_t e = static_cast<uint32_t>(static_cast<int64_t>(r.gauss(s)));
+        const uint32_t e = static_cast<uint32_t>(nl.draw(r));
         o[n] = static_cast<int32_t>(dot + e + static_cast<uint32_t>(mu[c]));
     }
     return YGPU_OK;
+}
+
+bool tuniform_ok(const ygpu_params* params, int lwe_b, int glwe_b) {
+    return params && lwe_b >= 0 && lwe_b < 31 && glwe_b >= 0 && glwe_b < params->torus_bits - 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ygpu_keygen(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t* lwe_key,
+                int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
+    if (!params) return YGPU_ERR_INVALID_ARG;
+    return keygen_mp(params, seed, Noise::gaussian_log2(sigma_log2_lwe, 32), Noise::gaussian_log2(sigma_log2_glwe, params->torus_bits),
+                     lwe_key, glwe_key, bsk_coeff, ksk, threads);
+}
+
+int ygpu_keygen_succinct(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t s2_level,
+                         int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_half, uint64_t* s2, int32_t* ksk, int32_t threads) {
+    if (!params) return YGPU_ERR_INVALID_ARG;
+    return keygen_succinct(params, seed, Noise::gaussian_log2(sigma_log2_lwe, 32),
+                           Noise::gaussian_log2(sigma_log2_glwe, params->torus_bits), s2_level, lwe_key, glwe_key, bsk_half, s2, ksk,
+                           threads);
+}
+
+int ygpu_keygen_classic(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t* lwe_key,
+                        int32_t* glwe_key, uint64_t* bsk_coeff, int32_t* ksk, int32_t threads) {
+    if (!params) return YGPU_ERR_INVALID_ARG;
+    return keygen_classic(params, seed, Noise::gaussian_log2(sigma_log2_lwe, 32),
+                          Noise::gaussian_log2(sigma_log2_glwe, params->torus_bits), lwe_key, glwe_key, bsk_coeff, ksk, threads);
+}
+
+int ygpu_keygen_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits, int32_t form,
+                         int32_t s2_level, int32_t* lwe_key, int32_t* glwe_key, uint64_t* bsk_out, uint64_t* s2_out, int32_t* ksk,
+                         int32_t threads) {
+    if (!tuniform_ok(params, lwe_bound_bits, glwe_bound_bits)) return YGPU_ERR_INVALID_ARG;
+    const Noise nl = Noise::tuniform_bound(lwe_bound_bits), ng = Noise::tuniform_bound(glwe_bound_bits);
+    switch (form) {
+        case YGPU_KEY_MP: return keygen_mp(params, seed, nl, ng, lwe_key, glwe_key, bsk_out, ksk, threads);
+        case YGPU_KEY_CLASSIC: return keygen_classic(params, seed, nl, ng, lwe_key, glwe_key, bsk_out, ksk, threads);
+        case YGPU_KEY_SUCCINCT: return keygen_succinct(params, seed, nl, ng, s2_level, lwe_key, glwe_key, bsk_out, s2_out, ksk, threads);
+        default: return YGPU_ERR_INVALID_ARG;
+    }
+}
+
+int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count, const int32_t* mu,
+                     int32_t* out) {
+    return lwe_encrypt(n, lwe_key, seed, Noise::gaussian_log2(sigma_log2, 32), count, mu, out);
+}
+
+int ygpu_lwe_encrypt_tuniform(int32_t n, const int32_t* lwe_key, uint64_t seed, int32_t bound_bits, int32_t count, const int32_t* mu,
+                              int32_t* out) {
+    if (bound_bits < 0 || bound_bits >= 31) return YGPU_ERR_INVALID_ARG;
+    return lwe_encrypt(n, lwe_key, seed, Noise::tuniform_bound(bound_bits), count, mu, out);
 }
 
 }  // extern "C"

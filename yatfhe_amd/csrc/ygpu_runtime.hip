@@ -147,8 +147,15 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
 int ygpu_destroy(ygpu_ctx* ctx) {
     if (!ctx) return YGPU_OK;
     {
-        std::lock_guard<std::mutex> lk(ctx->mu);
+        std::unique_lock<std::mutex> lk(ctx->mu);
         hipSetDevice(ctx->device);
+        ctx->cv.wait(lk, [&] {
+            for (auto& l : ctx->lanes)
+                if (l->busy) return false;
+            return true;
+        });
+        for (auto& l : ctx->lanes) free_lane(*l);
+        ctx->lanes.clear();
         hipStreamSynchronize(ctx->stream);
         for (auto& k : ctx->keys) {
             if (k.d_bsk) hipFree(k.d_bsk);
@@ -253,19 +260,25 @@ int ygpu_key_upload_succinct(ygpu_ctx* ctx, const uint64_t* bsk_half, const uint
 }
 
 int ygpu_key_release(ygpu_ctx* ctx, int32_t key_id) {
-    return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
-        if (rc) return rc;
-        for (size_t g = 0; g < ctx->graphs.size(); g++)
-            if (ctx->graphs[g].live && std::find(ctx->graphs[g].keys.begin(), ctx->graphs[g].keys.end(), key_id) != ctx->graphs[g].keys.end())
-                return fail(YGPU_ERR_BUSY, "key " + std::to_string(key_id) + " is used by graph " + std::to_string(g) + ": destroy it first");
-        YG_HIP(hipStreamSynchronize(ctx->stream));
-        KeyRec& k = ctx->keys[key_id];
-        hipFree(k.d_bsk);
-        hipFree(k.d_ksk);
-        k = KeyRec{};
-        return set_table_slot(ctx, key_id);
-    });
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    std::unique_lock<std::mutex> lk(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return fail(YGPU_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    int rc = key_live(ctx, key_id);
+    if (rc) return rc;
+    for (size_t g = 0; g < ctx->graphs.size(); g++)
+        if (ctx->graphs[g].live && std::find(ctx->graphs[g].keys.begin(), ctx->graphs[g].keys.end(), key_id) != ctx->graphs[g].keys.end())
+            return fail(YGPU_ERR_BUSY, "key " + std::to_string(key_id) + " is used by graph " + std::to_string(g) + ": destroy it first");
+    // synchronous calls on plain lanes that use the key finish first
+    ctx->cv.wait(lk, [&] { return ctx->keys[key_id].users == 0; });
+    rc = key_live(ctx, key_id);  // another thread may have released it meanwhile
+    if (rc) return rc;
+    YG_HIP(hipStreamSynchronize(ctx->stream));
+    KeyRec& k = ctx->keys[key_id];
+    hipFree(k.d_bsk);
+    hipFree(k.d_ksk);
+    k = KeyRec{};
+    return set_table_slot(ctx, key_id);
 }
 
 int ygpu_key_bytes(const ygpu_ctx* ctx, int32_t key_id, int64_t* bytes) {
@@ -377,95 +390,218 @@ int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+int lane_create(ygpu_ctx* ctx, std::unique_ptr<PlainLane>& out) {
+    auto L = std::make_unique<PlainLane>();
+    const ygpu_params& p = ctx->p;
+    const size_t tb = ctx->torus_bytes();
+    hipError_t e = hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L->d_lwe), sizeof(int32_t) * (p.n + 1));
+    if (e == hipSuccess) e = hipMalloc(&L->d_v, tb * p.N);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L->d_out), sizeof(int32_t) * (p.n + 1));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L->d_ext), sizeof(int32_t) * (p.k * p.N + 1));
+    if (e == hipSuccess) e = hipMalloc(&L->d_acc, tb * (p.k + 1) * p.N);
+    if (e == hipSuccess) e = hipMalloc(&L->d_accin, tb * (p.k + 1) * p.N);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&L->d_partial), ks_workspace_bytes(ctx, 1));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&L->h_status), 64, hipHostMallocDefault);
+    void* dp = nullptr;
+    if (e == hipSuccess) {
+        *L->h_status = 0u;
+        e = hipHostGetDevicePointer(&dp, L->h_status, 0);
+    }
+    if (e != hipSuccess) {
+        free_lane(*L);
+        return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("plain lane: ") + hipGetErrorString(e));
+    }
+    L->hd_status = static_cast<uint32_t*>(dp);
+    out = std::move(L);
+    return YGPU_OK;
+}
+
+// Runs f(lane) on a plain lane of its own, holding the context lock only to
+// take and return the lane (and to count the call as a user of key_id, so the
+// key cannot be released under it).  Concurrent callers run side by side.
+template <class F>
+int with_lane(ygpu_ctx* ctx, int32_t key_id, F&& f) {
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    PlainLane* L = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        hipError_t e = hipSetDevice(ctx->device);
+        if (e != hipSuccess) return fail(YGPU_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        if (key_id >= 0) {
+            int rc = key_live(ctx, key_id);
+            if (rc) return rc;
+        }
+        for (auto& l : ctx->lanes)
+            if (!l->busy) { L = l.get(); break; }
+        if (!L) {
+            std::unique_ptr<PlainLane> nl;
+            int rc = lane_create(ctx, nl);
+            if (rc) return rc;
+            L = nl.get();
+            ctx->lanes.push_back(std::move(nl));
+        }
+        L->busy = true;
+        L->tile = plain_tile(ctx);
+        L->extract_index = ctx->extract_index;
+        L->barrier_polls = ctx->barrier_polls;
+        if (key_id >= 0) ctx->keys[key_id].users++;
+    }
+    int rc;
+    try {
+        rc = f(*L);
+    } catch (const std::exception& ex) {
+        rc = fail(YGPU_ERR_INVALID_ARG, ex.what());
+    }
+    // nothing of this call may still run on the lane when another takes it
+    hipStreamSynchronize(L->stream);
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        L->busy = false;
+        if (key_id >= 0) ctx->keys[key_id].users--;
+    }
+    ctx->cv.notify_all();
+    return rc;
+}
+
+// Synchronises the lane's stream and reports its barrier-failure word.
+int lane_sync(PlainLane& L) {
+    YG_HIP(hipStreamSynchronize(L.stream));
+    volatile uint32_t* st = L.h_status;
+    if (*st == 0u) return YGPU_OK;
+    *st = 0u;
+    return fail(YGPU_ERR_DEVICE, "a blind-rotation workgroup's team barrier timed out: the call's results are invalid "
+                                 "(its accumulator reads the most negative torus value)");
+}
+
+BrArgs lane_br_args(ygpu_ctx* ctx, PlainLane& L, const int32_t* d_lwe, const void* d_v, void* d_acc, int32_t key_id) {
+    BrArgs a = br_args(ctx, 1, d_lwe, d_v, 0, d_acc, ctx->d_bsk_table + key_id, nullptr);
+    a.status = L.hd_status;
+    a.barrier_polls = L.barrier_polls;
+    return a;
+}
+
+// Host accumulator Torus[(k+1)N] -> the lane's device accumulator buffer `dst`.
+hipError_t lane_upload(ygpu_ctx* ctx, PlainLane& L, void* dst, const void* src, size_t bytes) {
+    (void)ctx;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, L.stream);
+}
+
+// Extract (at `index`) + keyswitch of the accumulator at L.d_acc into L.d_out
+// (and the extracted LWE into L.d_ext when want_ext).
+int lane_extract_keyswitch(ygpu_ctx* ctx, PlainLane& L, int32_t key_id, bool want_ext, int index) {
+    return run_ks(ctx, 1, L.d_acc, ctx->d_ksk_table + key_id, nullptr, L.d_out, want_ext ? L.d_ext : nullptr, L.d_partial, L.stream,
+                  index);
+}
+
+}  // namespace
+
+extern "C" {
+
 int ygpu_pbs(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v, int32_t* lwe_out, void* acc_out,
              int32_t* extract_out) {
     if (!lwe_in || !v || !lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
-    return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
-        if (rc) return rc;
-        rc = plain_buffers(ctx);
-        if (rc) return rc;
+    return with_lane(ctx, key_id, [&](PlainLane& L) -> int {
         const ygpu_params& p = ctx->p;
         const size_t tb = ctx->torus_bytes();
-        YG_HIP(hipMemcpyAsync(ctx->d_lwe1, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, ctx->stream));
-        YG_HIP(hipMemcpyAsync(ctx->d_v1, v, tb * p.N, hipMemcpyHostToDevice, ctx->stream));
-        BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, plain_tile(ctx), a);
+        YG_HIP(hipMemcpyAsync(L.d_lwe, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, L.stream));
+        YG_HIP(hipMemcpyAsync(L.d_v, v, tb * p.N, hipMemcpyHostToDevice, L.stream));
+        int rc = run_br(ctx, L.tile, lane_br_args(ctx, L, L.d_lwe, L.d_v, L.d_acc, key_id), L.stream);
         if (rc) return rc;
-        rc = run_ks(ctx, 1, ctx->d_acc1, ctx->d_ksk_table + key_id, nullptr, ctx->d_out1, extract_out ? ctx->d_ext1 : nullptr);
+        rc = lane_extract_keyswitch(ctx, L, key_id, extract_out != nullptr, L.extract_index);
         if (rc) return rc;
-        YG_HIP(hipMemcpyAsync(lwe_out, ctx->d_out1, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, ctx->stream));
-        if (acc_out) YG_HIP(hipMemcpyAsync(acc_out, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
-        if (extract_out) YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
-        return sync_and_check(ctx);
+        YG_HIP(hipMemcpyAsync(lwe_out, L.d_out, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, L.stream));
+        if (acc_out) YG_HIP(hipMemcpyAsync(acc_out, L.d_acc, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, L.stream));
+        if (extract_out) YG_HIP(hipMemcpyAsync(extract_out, L.d_ext, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, L.stream));
+        return lane_sync(L);
     });
 }
 
 int ygpu_blind_rotate(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v, void* acc_out) {
     if (!lwe_in || !v || !acc_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
-    return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
-        if (rc) return rc;
-        rc = plain_buffers(ctx);
-        if (rc) return rc;
+    return with_lane(ctx, key_id, [&](PlainLane& L) -> int {
         const ygpu_params& p = ctx->p;
         const size_t tb = ctx->torus_bytes();
-        YG_HIP(hipMemcpyAsync(ctx->d_lwe1, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, ctx->stream));
-        YG_HIP(hipMemcpyAsync(ctx->d_v1, v, tb * p.N, hipMemcpyHostToDevice, ctx->stream));
-        BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
-        rc = run_br(ctx, plain_tile(ctx), a);
+        YG_HIP(hipMemcpyAsync(L.d_lwe, lwe_in, sizeof(int32_t) * (p.n + 1), hipMemcpyHostToDevice, L.stream));
+        YG_HIP(hipMemcpyAsync(L.d_v, v, tb * p.N, hipMemcpyHostToDevice, L.stream));
+        int rc = run_br(ctx, L.tile, lane_br_args(ctx, L, L.d_lwe, L.d_v, L.d_acc, key_id), L.stream);
         if (rc) return rc;
-        YG_HIP(hipMemcpyAsync(acc_out, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
-        return sync_and_check(ctx);
+        YG_HIP(hipMemcpyAsync(acc_out, L.d_acc, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, L.stream));
+        return lane_sync(L);
     });
 }
 
 int ygpu_blind_rotate_scaled(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar, void* acc) {
     if (!abar || !acc) return fail(YGPU_ERR_INVALID_ARG, "null argument");
-    return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
-        if (rc) return rc;
-        rc = plain_buffers(ctx);
-        if (rc) return rc;
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    const int twoN = 2 * ctx->p.N;
+    for (int i = 0; i < ctx->p.n; i++)
+        if (abar[i] < -twoN || abar[i] >= twoN) return fail(YGPU_ERR_INVALID_ARG, "abar out of [-2N, 2N)");
+    return with_lane(ctx, key_id, [&](PlainLane& L) -> int {
         const ygpu_params& p = ctx->p;
-        const int twoN = 2 * p.N;
-        for (int i = 0; i < p.n; i++)
-            if (abar[i] < -twoN || abar[i] >= twoN) return fail(YGPU_ERR_INVALID_ARG, "abar out of [-2N, 2N)");
         const size_t tb = ctx->torus_bytes();
-        YG_HIP(hipMemcpyAsync(ctx->d_lwe1, abar, sizeof(int32_t) * p.n, hipMemcpyHostToDevice, ctx->stream));
+        YG_HIP(hipMemcpyAsync(L.d_lwe, abar, sizeof(int32_t) * p.n, hipMemcpyHostToDevice, L.stream));
         // the caller's initial accumulator, in a buffer of its own
-        YG_HIP(hipMemcpyAsync(ctx->d_accin1, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
-        BrArgs a = br_args(ctx, 1, ctx->d_lwe1, ctx->d_v1, 0, ctx->d_acc1, ctx->d_bsk_table + key_id, nullptr);
+        YG_HIP(lane_upload(ctx, L, L.d_accin, acc, tb * (p.k + 1) * p.N));
+        BrArgs a = lane_br_args(ctx, L, L.d_lwe, L.d_v, L.d_acc, key_id);
         a.prescaled = 1;
-        a.acc_in = ctx->d_accin1;
-        rc = run_br(ctx, plain_tile(ctx), a);
+        a.acc_in = L.d_accin;
+        int rc = run_br(ctx, L.tile, a, L.stream);
         if (rc) return rc;
-        YG_HIP(hipMemcpyAsync(acc, ctx->d_acc1, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, ctx->stream));
-        return sync_and_check(ctx);
+        YG_HIP(hipMemcpyAsync(acc, L.d_acc, tb * (p.k + 1) * p.N, hipMemcpyDeviceToHost, L.stream));
+        return lane_sync(L);
+    });
+}
+
+int ygpu_sample_extract_index(ygpu_ctx* ctx, const void* acc, int32_t index, int32_t* extract_out) {
+    if (!acc || !extract_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    if (index < 0 || index >= ctx->p.N) return fail(YGPU_ERR_INVALID_ARG, "extraction index must be in [0, N)");
+    return with_lane(ctx, -1, [&](PlainLane& L) -> int {
+        const ygpu_params& p = ctx->p;
+        const size_t tb = ctx->torus_bytes();
+        YG_HIP(lane_upload(ctx, L, L.d_acc, acc, tb * (p.k + 1) * p.N));
+        KsArgs a = ks_args(ctx, 1, L.d_acc, nullptr, nullptr, nullptr, nullptr);
+        a.extract_index = index;
+        YG_HIP(timed(ctx, "k_extract", L.stream, [&] { return launch_extract(ctx->tbits, a, L.d_ext, L.stream); }));
+        YG_HIP(hipMemcpyAsync(extract_out, L.d_ext, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, L.stream));
+        return lane_sync(L);
     });
 }
 
 int ygpu_sample_extract(ygpu_ctx* ctx, const void* acc, int32_t* extract_out) {
-    if (!acc || !extract_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
-    return with_ctx(ctx, [&]() -> int {
-        int rc = plain_buffers(ctx);
-        if (rc) return rc;
+    if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
+    int32_t index = 0;
+    int rc = ygpu_get_extract_index(ctx, &index);
+    if (rc) return rc;
+    return ygpu_sample_extract_index(ctx, acc, index, extract_out);
+}
+
+int ygpu_extract_keyswitch(ygpu_ctx* ctx, int32_t key_id, const void* acc, int32_t* extract_out, int32_t* lwe_out) {
+    if (!acc || !lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_lane(ctx, key_id, [&](PlainLane& L) -> int {
         const ygpu_params& p = ctx->p;
         const size_t tb = ctx->torus_bytes();
-        YG_HIP(hipMemcpyAsync(ctx->d_acc1, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
-        KsArgs a = ks_args(ctx, 1, ctx->d_acc1, nullptr, nullptr, nullptr, nullptr);
-        YG_HIP(timed(ctx, "k_extract", ctx->stream, [&] { return launch_extract(ctx->tbits, a, ctx->d_ext1, ctx->stream); }));
-        YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
-        return sync_and_check(ctx);
+        YG_HIP(lane_upload(ctx, L, L.d_acc, acc, tb * (p.k + 1) * p.N));
+        int rc = lane_extract_keyswitch(ctx, L, key_id, extract_out != nullptr, L.extract_index);
+        if (rc) return rc;
+        YG_HIP(hipMemcpyAsync(lwe_out, L.d_out, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, L.stream));
+        if (extract_out) YG_HIP(hipMemcpyAsync(extract_out, L.d_ext, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, L.stream));
+        return lane_sync(L);
     });
 }
 
 int ygpu_keyswitch(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, int32_t* lwe_out) {
     if (!lwe_in || !lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
     if (!ctx) return fail(YGPU_ERR_INVALID_ARG, "null context");
-    // Embed the LWE (dimension kN) as the accumulator whose index-0 extraction
-    // it is: acc_c[0] = a_{cN}, acc_c[N-j] = -a_{cN+j}, body[0] = b, scaled by
-    // 2^(w-32) (exact, and undone exactly by the extraction's modswitch).
+    // Embed the LWE (dimension kN) as the accumulator whose extraction at
+    // index 0 it is: acc_c[0] = a_{cN}, acc_c[N - j] = -a_{cN+j} (j > 0),
+    // body[0] = b, scaled by 2^(w-32) (exact, and undone exactly by the
+    // extraction's modswitch); the keyswitch then extracts at index 0.
     const ygpu_params& p = ctx->p;
     const int N = p.N, w = p.torus_bits;
     std::vector<int64_t> acc(static_cast<size_t>(p.k + 1) * N, 0);
@@ -474,29 +610,19 @@ int ygpu_keyswitch(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, int32_t
         for (int j = 1; j < N; j++) acc[static_cast<size_t>(c) * N + N - j] = sext(-(static_cast<int64_t>(lwe_in[c * N + j]) << (w - 32)), w);
     }
     acc[static_cast<size_t>(p.k) * N] = static_cast<int64_t>(lwe_in[p.k * N]) << (w - 32);
+    std::vector<int32_t> a32;
+    const void* src = acc.data();
     if (w == 32) {
-        std::vector<int32_t> a32(acc.size());
+        a32.resize(acc.size());
         for (size_t i = 0; i < acc.size(); i++) a32[i] = static_cast<int32_t>(acc[i]);
-        return ygpu_extract_keyswitch(ctx, key_id, a32.data(), nullptr, lwe_out);
+        src = a32.data();
     }
-    return ygpu_extract_keyswitch(ctx, key_id, acc.data(), nullptr, lwe_out);
-}
-
-int ygpu_extract_keyswitch(ygpu_ctx* ctx, int32_t key_id, const void* acc, int32_t* extract_out, int32_t* lwe_out) {
-    if (!acc || !lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
-    return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
+    return with_lane(ctx, key_id, [&](PlainLane& L) -> int {
+        YG_HIP(lane_upload(ctx, L, L.d_acc, src, ctx->torus_bytes() * (p.k + 1) * N));
+        int rc = lane_extract_keyswitch(ctx, L, key_id, false, 0);
         if (rc) return rc;
-        rc = plain_buffers(ctx);
-        if (rc) return rc;
-        const ygpu_params& p = ctx->p;
-        const size_t tb = ctx->torus_bytes();
-        YG_HIP(hipMemcpyAsync(ctx->d_acc1, acc, tb * (p.k + 1) * p.N, hipMemcpyHostToDevice, ctx->stream));
-        rc = run_ks(ctx, 1, ctx->d_acc1, ctx->d_ksk_table + key_id, nullptr, ctx->d_out1, extract_out ? ctx->d_ext1 : nullptr);
-        if (rc) return rc;
-        YG_HIP(hipMemcpyAsync(lwe_out, ctx->d_out1, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, ctx->stream));
-        if (extract_out) YG_HIP(hipMemcpyAsync(extract_out, ctx->d_ext1, sizeof(int32_t) * (p.k * p.N + 1), hipMemcpyDeviceToHost, ctx->stream));
-        return sync_and_check(ctx);
+        YG_HIP(hipMemcpyAsync(lwe_out, L.d_out, sizeof(int32_t) * (p.n + 1), hipMemcpyDeviceToHost, L.stream));
+        return lane_sync(L);
     });
 }
 
@@ -538,6 +664,22 @@ int ygpu_sync(ygpu_ctx* ctx) {
         const int rc = ctx->timing ? resolve_timing(ctx) : YGPU_OK;
         const int st = check_status(ctx);
         return st ? st : rc;
+    });
+}
+
+int ygpu_set_extract_index(ygpu_ctx* ctx, int32_t index) {
+    return with_ctx(ctx, [&]() -> int {
+        if (index < 0 || index >= ctx->p.N) return fail(YGPU_ERR_INVALID_ARG, "extraction index must be in [0, N)");
+        ctx->extract_index = index;
+        return YGPU_OK;
+    });
+}
+
+int ygpu_get_extract_index(ygpu_ctx* ctx, int32_t* index) {
+    if (!index) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_ctx(ctx, [&]() -> int {
+        *index = ctx->extract_index;
+        return YGPU_OK;
     });
 }
 
@@ -750,6 +892,7 @@ int ygpu_timing_get(ygpu_ctx* ctx, const char* kernel, int64_t* launches, double
     return with_ctx(ctx, [&]() -> int {
         int rc = resolve_timing(ctx);
         if (rc) return rc;
+        std::lock_guard<std::mutex> tl(ctx->timing_mu);
         auto it = ctx->stats.find(kernel);
         *launches = it == ctx->stats.end() ? 0 : it->second.first;
         *total_ms = it == ctx->stats.end() ? 0.0 : it->second.second;
@@ -760,6 +903,7 @@ int ygpu_timing_get(ygpu_ctx* ctx, const char* kernel, int64_t* launches, double
 int ygpu_timing_reset(ygpu_ctx* ctx) {
     return with_ctx(ctx, [&]() -> int {
         int rc = resolve_timing(ctx);
+        std::lock_guard<std::mutex> tl(ctx->timing_mu);
         ctx->stats.clear();
         return rc;
     });

@@ -792,6 +792,15 @@ int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl)
     });
 }
 
+int ygpu_server_set_extract_index(ygpu_server* s, int32_t pset, int32_t index) {
+    return with_server(s, [&] {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+        if (s->started) return fail(YGPU_ERR_INVALID_ARG, "the extraction index must be set before the first request");
+        return ygpu_set_extract_index(s->psets[pset]->ctx, index);
+    });
+}
+
 }  // extern "C"
 
 namespace {
