@@ -217,6 +217,17 @@ int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in
 int ygpu_get_tile(ygpu_ctx* ctx, int32_t* cpw, int32_t* twl);
 int ygpu_set_tile(ygpu_ctx* ctx, int32_t cpw, int32_t twl);
 int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, int32_t* twl, double* best_ms);
+/* The tile's third dimension where the gadget digits are signed bytes
+ * (TORUS32 N=1024 2^7 x 3): tab forward-transform stages (0, 1 or 2) taken
+ * from per-workgroup LDS tables of w * d mod p for every digit d instead of
+ * computed products, trading VALU work for LDS reads.  tab = -1 in
+ * ygpu_set_tile_ex (and ygpu_set_tile) keeps the current tab where that
+ * combination is instantiated, else takes the first. */
+int ygpu_get_tile_ex(ygpu_ctx* ctx, int32_t* cpw, int32_t* twl, int32_t* tab);
+int ygpu_set_tile_ex(ygpu_ctx* ctx, int32_t cpw, int32_t twl, int32_t tab);
+/* The candidates of the last ygpu_autotune and their best launch times (ms):
+ * writes min(max, count) of them and sets *count. */
+int ygpu_autotune_results(ygpu_ctx* ctx, int32_t max, int32_t* cpw, int32_t* twl, int32_t* tab, double* ms, int32_t* count);
 
 /* hipGraph of the key-grouped pipeline (blind rotate -> extract -> keyswitch)
  * over fixed device buffers, captured once and replayed with one launch.

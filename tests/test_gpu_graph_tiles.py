@@ -92,12 +92,14 @@ def test_every_tile_is_bit_identical(p32, keys32):
         kid = ctx.upload_key(keys32)
         for cpw in (4, 2, 1):
             for twl in (1, 0):
-                ctx.set_tile(cpw, twl)
-                out, acc, _ = ctx.pbs_batch(kid, cts, v, want_acc=True)
-                outs.append((out, acc))
+                for tab in (0, 1, 2):   # forward stages from digit tables
+                    ctx.set_tile_ex(cpw, twl, tab)
+                    out, acc, _ = ctx.pbs_batch(kid, cts, v, want_acc=True)
+                    outs.append((out, acc))
         cpw, twl, ms = ctx.autotune(kid, 512)
         assert (cpw, twl) in [(4, 1), (4, 0), (2, 1), (2, 0), (1, 1), (1, 0)] and ms > 0
         assert ctx.tile() == (cpw, twl)
+        assert len(ctx.autotune_results()) == 18
         out, acc, _ = ctx.pbs_batch(kid, cts, v, want_acc=True)
         outs.append((out, acc))
     for o, a in outs[1:]:

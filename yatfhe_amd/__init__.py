@@ -197,6 +197,9 @@ def load_library(path: str | None = None):
         "ygpu_get_tile": (c_int, [c_vp, P(c_i32), P(c_i32)]),
         "ygpu_set_tile": (c_int, [c_vp, c_i32, c_i32]),
         "ygpu_autotune": (c_int, [c_vp, c_i32, c_i32, P(c_i32), P(c_i32), P(c_dbl)]),
+        "ygpu_get_tile_ex": (c_int, [c_vp, P(c_i32), P(c_i32), P(c_i32)]),
+        "ygpu_set_tile_ex": (c_int, [c_vp, c_i32, c_i32, c_i32]),
+        "ygpu_autotune_results": (c_int, [c_vp, c_i32, P(c_i32), P(c_i32), P(c_i32), P(c_dbl), P(c_i32)]),
         "ygpu_graph_create": (c_int, [c_vp, P(_Job), c_i32, c_i64, c_vp, c_vp, c_i32, c_vp, c_vp, P(c_i32)]),
         "ygpu_graph_launch": (c_int, [c_vp, c_i32]),
         "ygpu_graph_destroy": (c_int, [c_vp, c_i32]),
@@ -261,7 +264,7 @@ def exported_symbols():
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
         "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls", "ygpu_pbs_batch_device_noalloc",
         "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index",
-        "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform",
+        "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform", "ygpu_get_tile_ex", "ygpu_set_tile_ex", "ygpu_autotune_results",
     ]
 
 
@@ -750,6 +753,22 @@ class Context:
         c, t, ms = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_double(0)
         _check(self.lib.ygpu_autotune(self.h, key_id, count, ctypes.byref(c), ctypes.byref(t), ctypes.byref(ms)))
         return c.value, t.value, ms.value
+
+    def tile_ex(self):
+        c, t, b = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        _check(self.lib.ygpu_get_tile_ex(self.h, ctypes.byref(c), ctypes.byref(t), ctypes.byref(b)))
+        return c.value, t.value, b.value
+
+    def set_tile_ex(self, cpw: int, twl: int, tab: int):
+        _check(self.lib.ygpu_set_tile_ex(self.h, cpw, twl, tab))
+
+    def autotune_results(self):
+        """[(cpw, twl, tab, best ms)] of the last autotune's candidates."""
+        m = 32
+        c, t, b = (ctypes.c_int32 * m)(), (ctypes.c_int32 * m)(), (ctypes.c_int32 * m)()
+        ms, n = (ctypes.c_double * m)(), ctypes.c_int32(0)
+        _check(self.lib.ygpu_autotune_results(self.h, m, c, t, b, ms, ctypes.byref(n)))
+        return [(c[i], t[i], b[i], ms[i]) for i in range(min(m, n.value))]
 
     def graph_create(self, jobs, total: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None) -> int:
         arr = (_Job * max(1, len(jobs)))()

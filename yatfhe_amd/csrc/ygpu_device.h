@@ -156,12 +156,12 @@ __device__ __forceinline__ void report_barrier_failure(bool failed, uint32_t* st
     if (failed && threadIdx.x == 0 && status) *reinterpret_cast<volatile uint32_t*>(status) = 1u;
 }
 
-// Cooley-Tukey pass (forward): U, V -> U + wV, U - wV.
-template <int LOGN, int P>
+// Cooley-Tukey pass (forward): U, V -> U + wV, U - wV; stages LS0.. of the pass.
+template <int LOGN, int P, int LS0 = 0>
 __device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* __restrict__ tw, const PrimeF64& pr) {
     using PS = Pass<LOGN, P>;
 #pragma unroll
-    for (int ls = 0; ls < PS::S; ls++) {
+    for (int ls = LS0; ls < PS::S; ls++) {
         const int st = PS::S0 + ls;
         const int half = 1 << (PS::S - 1 - ls);
 #pragma unroll
@@ -224,6 +224,91 @@ template <int LOGN, int TEAM_WAVES>
 __device__ __forceinline__ void ntt_forward(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr) {
     WorkgroupSync<TEAM_WAVES> sy;
     ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, sy);
+}
+
+// The first two forward stages on digit inputs by table lookup.  Both
+// stages of pass 0 use fixed twiddles: stage 0 pairs (r, r + 8) with w1 =
+// tw[1]; stage 1 pairs (r, r + 4) with w2 = tw[2] for r < 4 and w3 = tw[3] for
+// r in [8, 12).  For the digits a, b, c, d of registers g, g + 4, g + 8, g + 12
+// the two stages give
+//   x[g]      = a + w1 c + (w2 b + w2 w1 d)     x[g + 4]  = a + w1 c - (w2 b + w2 w1 d)
+//   x[g + 8]  = a - w1 c + (w3 b - w3 w1 d)     x[g + 12] = a - w1 c - (w3 b - w3 w1 d)
+// so with tables of w * d mod p for every digit d no product is computed.
+// FirstStageTables holds them: [w1, w2, w2 w1, w3, w3 w1][256], indexed by the
+// digit's byte (two's complement), each entry of magnitude <= p/2 + 1.  The
+// outputs are below 1.5p + 2^7; a stage takes a bound b p to (1.09375 b + 0.5) p
+// (the mulmod bound above with p < 2^49), so after log2(N) stages every value
+// is below 8.7p (N = 1024) or 10p (N = 2048), against 7.75p / 8.98p with
+// computed products: all < 16p, exact.
+// With stage 0 alone from a table (fwd_first_stage) only [w1] is used.
+constexpr int FIRST_TABLE_ENTRIES = 5 * 256;
+__host__ __device__ constexpr int first_table_entries(int stages) { return stages == 2 ? FIRST_TABLE_ENTRIES : stages == 1 ? 256 : 1; }
+
+// Fills `entries` table entries (all threads of the workgroup take part; the
+// caller synchronises before use).  tw: the prime's forward twiddles.
+__device__ __forceinline__ void build_first_stage_tables(double* tab, const double* tw, const PrimeF64& pr, int tid, int nthreads,
+                                                         int entries = FIRST_TABLE_ENTRIES) {
+    const double w1 = tw[1], w2 = tw[2], w3 = tw[3];
+    const double w21 = reduce(mulmod(w2, w1, pr), pr), w31 = reduce(mulmod(w3, w1, pr), pr);
+    for (int e = tid; e < entries; e += nthreads) {
+        const int t = e >> 8;
+        const double d = static_cast<double>(static_cast<int8_t>(static_cast<uint8_t>(e & 255)));
+        const double w = t == 0 ? w1 : t == 1 ? w2 : t == 2 ? w21 : t == 3 ? w3 : w31;
+        tab[e] = reduce(mulmod(d, w, pr), pr);
+    }
+}
+
+// x = stages 0 and 1 of the forward transform of the digit polynomial whose
+// signed-byte digits are byte `lvl` of dg[r] (pass-0 layout).
+__device__ __forceinline__ void fwd_first_two_stages(double (&x)[16], const uint32_t (&dg)[16], int lvl, const double* tab) {
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const uint32_t ba = (dg[g] >> (8 * lvl)) & 255u, bb = (dg[g + 4] >> (8 * lvl)) & 255u;
+        const uint32_t bc = (dg[g + 8] >> (8 * lvl)) & 255u, bd = (dg[g + 12] >> (8 * lvl)) & 255u;
+        const double a = static_cast<double>(static_cast<int8_t>(static_cast<uint8_t>(ba)));
+        const double t1c = tab[bc], t2b = tab[256 + bb], t21d = tab[512 + bd], t3b = tab[768 + bb], t31d = tab[1024 + bd];
+        const double P = __dadd_rn(a, t1c), Q = __dadd_rn(a, -t1c);
+        const double R = __dadd_rn(t2b, t21d), S = __dadd_rn(t3b, -t31d);
+        x[g] = __dadd_rn(P, R);
+        x[g + 4] = __dadd_rn(P, -R);
+        x[g + 8] = __dadd_rn(Q, S);
+        x[g + 12] = __dadd_rn(Q, -S);
+    }
+}
+
+// Stage 0 alone by lookup: x[r] = a + w1 c, x[r + 8] = a - w1 c for the
+// digits a, c of registers r, r + 8 (r < 8); tab[0..256) = w1 * d.
+__device__ __forceinline__ void fwd_first_stage(double (&x)[16], const uint32_t (&dg)[16], int lvl, const double* tab) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const uint32_t ba = (dg[r] >> (8 * lvl)) & 255u, bc = (dg[r + 8] >> (8 * lvl)) & 255u;
+        const double a = static_cast<double>(static_cast<int8_t>(static_cast<uint8_t>(ba)));
+        const double t = tab[bc];
+        x[r] = __dadd_rn(a, t);
+        x[r + 8] = __dadd_rn(a, -t);
+    }
+}
+
+// The forward transform after fwd_first_stage.
+template <int LOGN, class Sync>
+__device__ __forceinline__ void ntt_forward_from_stage1(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr,
+                                                        Sync& sy) {
+    fwd_pass<LOGN, 0, 1>(x, tl, tw, pr);
+    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
+    fwd_pass<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
+    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+}
+
+// The forward transform after fwd_first_two_stages.
+template <int LOGN, class Sync>
+__device__ __forceinline__ void ntt_forward_from_stage2(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr,
+                                                        Sync& sy) {
+    fwd_pass<LOGN, 0, 2>(x, tl, tw, pr);
+    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
+    fwd_pass<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
+    fwd_pass<LOGN, 2>(x, tl, tw, pr);
 }
 
 // Inverse negacyclic NTT without the 1/N factor (folded into the key):

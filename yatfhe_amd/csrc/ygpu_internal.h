@@ -113,6 +113,7 @@ struct ygpu_ctx {
     hipStream_t stream = nullptr;
     int logn = 0, np = 1, tbits = 32;
     BrTile tile{};          // batched-form tile (autotuned or default)
+    std::vector<std::pair<BrTile, double>> tune;  // the last autotune's candidates and their ms
     bool tile_fixed = false; // set by ygpu_set_tile: used for every batch size
     int cu_count = 0;       // compute units of the device
     bool capturing = false;

@@ -16,8 +16,10 @@ namespace ygpu {
 //   PACK8  digits packed 8 bits each (radix_bits <= 8), else 16 bits
 //   L, RB  gadget levels and base log2 when fixed at compile time (0: runtime)
 //   TWL    twiddles in LDS (1), global memory through L1 (0), or automatic (-1)
-template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0, int TWL_ = -1>
+//   TAB    forward stages taken from digit tables (0, 1 or 2; fixed 8-bit shapes)
+template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0, int TWL_ = -1, int TAB_ = 0>
 struct BrCfg {
+    static constexpr int TAB = TAB_;
     static constexpr int L = L_;
     static constexpr int RB = RB_;
     static constexpr int TWL = TWL_;
@@ -183,7 +185,9 @@ struct BrTile {
     int cpw = 4;
     int twl = 1;
     int lat = 0;  // the latency form: one wavefront per (row, gadget level)
+    int tab = 0;  // forward stages from digit tables (0: none; 1: stage 0; 2: stages 0 and 1)
 };
+constexpr int MAX_TILES = 32;
 hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& tile, int radix_bits, const BrArgs& a, hipStream_t s);
 // The latency form of the TORUS32 N = 1024 gadget-2^7x3 blind rotation (one
 // ciphertext per workgroup, a wavefront per row and level), in its own

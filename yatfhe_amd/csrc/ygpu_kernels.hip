@@ -41,6 +41,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 
     __shared__ double tws[TWN];
     __shared__ double scratch[CPW][2 * SCR];
+    // the specialised 8-bit-digit shapes may run the first one or two forward
+    // stages by table lookup (fwd_first_stage / fwd_first_two_stages): fewer
+    // products for more LDS reads; the tile chooses
+    constexpr int TAB = (C::L > 0 && C::RB <= 7 && NP == 1) ? C::TAB : 0;
+    constexpr int TAB_ENTRIES = first_table_entries(TAB);
+    __shared__ double firstTab[TAB_ENTRIES];
     // NP == 2: the first prime's residues of each team's column wait here while
     // the second prime is processed (registers are the binding resource)
     constexpr int RESN = NP > 1 ? 2 * N : 1;
@@ -62,7 +68,9 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     TorusT* stage = reinterpret_cast<TorusT*>(scr);
 
     const int n = a.n;
-    const int L = a.level;
+    // the gadget depth: compile-time for the specialised shapes (the level
+    // loop below is then unrolled)
+    const int L = C::L > 0 ? C::L : a.level;
     const int rb = a.radix_bits;
     const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
     const double* __restrict__ bsk = a.bsk_table[slot];
@@ -72,6 +80,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     if constexpr (C::TW_LDS) {
         for (int j = static_cast<int>(threadIdx.x); j < TWN; j += C::THREADS) tws[j] = a.tw[j];
     }
+    if constexpr (TAB > 0) build_first_stage_tables(firstTab, a.tw, a.prime[0], static_cast<int>(threadIdx.x), C::THREADS, TAB_ENTRIES);
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
     // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
     TorusT accr[16];
@@ -152,10 +161,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
             // own: output column tm (this team's); oth: column 1 - tm.  The
             // team index only moves addresses, never register-array indices.
             double own[16], oth[16];
+            if constexpr (C::L == 0) {
 #pragma unroll
-            for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
+                for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
+            }
 
-            for (int lvl = 0; lvl < L; lvl++) {
+            auto level = [&](const int lvl) {
                 const int d = lvl * K1 + tm;
                 // BSK values of this digit polynomial: the own column is issued
                 // before the transform so its latency overlaps it, the other
@@ -171,33 +182,51 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
                 }
                 double x[16];
+                if constexpr (TAB == 2) {
+                    WorkgroupSync<TW> sy;
+                    fwd_first_two_stages(x, dg0, lvl, firstTab);
+                    ntt_forward_from_stage2<LOGN>(x, scr, tl, tw, pr, sy);
+                } else if constexpr (TAB == 1) {
+                    WorkgroupSync<TW> sy;
+                    fwd_first_stage(x, dg0, lvl, firstTab);
+                    ntt_forward_from_stage1<LOGN>(x, scr, tl, tw, pr, sy);
+                } else {
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    int dv;
-                    if constexpr (C::L > 0 && C::RB <= 7) {
-                        dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
-                    } else if constexpr (C::L > 0) {
-                        const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
-                        dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
-                    } else {
-                        dv = digit_runtime(dg0[r], dg1[r], lvl, rb);
+                    for (int r = 0; r < 16; r++) {
+                        int dv;
+                        if constexpr (C::L > 0 && C::RB <= 7) {
+                            dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
+                        } else if constexpr (C::L > 0) {
+                            const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
+                            dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
+                        } else {
+                            dv = digit_runtime(dg0[r], dg1[r], lvl, rb);
+                        }
+                        x[r] = static_cast<double>(dv);
                     }
-                    x[r] = static_cast<double>(dv);
+                    ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
                 }
-                ntt_forward<LOGN, TW>(x, scr, tl, tw, pr);
                 if constexpr (NP > 1) {
 #pragma unroll
                     for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
                 }
                 // one column at a time: the scheduling barrier keeps the other
-                // column's loads from being hoisted over these products
+                // column's loads from being hoisted over these products; the
+                // first level of an unrolled loop assigns instead of adding
+                const bool first = C::L > 0 && lvl == 0;
 #pragma unroll
-                for (int r = 0; r < 16; r++) own[r] = __dadd_rn(own[r], mulmod(x[r], kbo[r], pr));
+                for (int r = 0; r < 16; r++) {
+                    const double pm = mulmod(x[r], kbo[r], pr);
+                    own[r] = first ? pm : __dadd_rn(own[r], pm);
+                }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
-                for (int r = 0; r < 16; r++) oth[r] = __dadd_rn(oth[r], mulmod(x[r], kbt[r], pr));
+                for (int r = 0; r < 16; r++) {
+                    const double pm = mulmod(x[r], kbt[r], pr);
+                    oth[r] = first ? pm : __dadd_rn(oth[r], pm);
+                }
                 if constexpr (C::L == 0) {
                     // more than four levels: reduce the partial sums once after
                     // four, so that with the partner's they stay below 16p
@@ -207,6 +236,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                         for (int r = 0; r < 16; r++) { own[r] = reduce(own[r], pr); oth[r] = reduce(oth[r], pr); }
                     }
                 }
+            };
+            if constexpr (C::L > 0) {
+#pragma unroll
+                for (int lvl = 0; lvl < C::L; lvl++) level(lvl);
+            } else {
+                for (int lvl = 0; lvl < L; lvl++) level(lvl);
             }
 
             // partner sums: hand column 1-tm to the other team, gather the full
@@ -557,16 +592,30 @@ hipError_t launch_br(const BrArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// TORUS32 N=1024 2^7 x 3: ciphertexts per workgroup x twiddle placement x
+// table-lookup forward stages.
+template <int CPW, int TWL>
+hipError_t launch_t32_b7x3_tab(int tab, const BrArgs& a, hipStream_t s) {
+    if (tab == 2) return launch_br<BrCfg<32, 10, 1, CPW, true, 3, 7, TWL, 2>>(a, s);
+    if (tab == 1) return launch_br<BrCfg<32, 10, 1, CPW, true, 3, 7, TWL, 1>>(a, s);
+    return launch_br<BrCfg<32, 10, 1, CPW, true, 3, 7, TWL, 0>>(a, s);
+}
+template <int CPW>
+hipError_t launch_t32_b7x3_twl(const BrTile& t, const BrArgs& a, hipStream_t s) {
+    return t.twl ? launch_t32_b7x3_tab<CPW, 1>(t.tab, a, s) : launch_t32_b7x3_tab<CPW, 0>(t.tab, a, s);
+}
+hipError_t launch_t32_b7x3(const BrTile& t, const BrArgs& a, hipStream_t s) {
+    if (t.cpw >= 4) return launch_t32_b7x3_twl<4>(t, a, s);
+    if (t.cpw == 2) return launch_t32_b7x3_twl<2>(t, a, s);
+    return launch_t32_b7x3_twl<1>(t, a, s);
+}
+
 // Specialised gadget shapes (the headline parameter sets) come in several
 // tiles; every other parameter set runs a runtime-gadget instantiation.
 hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& t, int radix_bits, const BrArgs& a, hipStream_t s) {
     if (t.lat && blind_rotate_lat_supported(tbits, logn, a.level, radix_bits)) return launch_blind_rotate_lat(a, s);
     const bool p8 = radix_bits <= 8;
-    if (tbits == 32 && logn == 10 && a.level == 3 && radix_bits == 7) {
-        if (t.cpw >= 4) return t.twl ? launch_br<BrCfg<32, 10, 1, 4, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, true, 3, 7, 0>>(a, s);
-        if (t.cpw == 2) return t.twl ? launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 0>>(a, s);
-        return t.twl ? launch_br<BrCfg<32, 10, 1, 1, true, 3, 7, 1>>(a, s) : launch_br<BrCfg<32, 10, 1, 1, true, 3, 7, 0>>(a, s);
-    }
+    if (tbits == 32 && logn == 10 && a.level == 3 && radix_bits == 7) return launch_t32_b7x3(t, a, s);
     if (tbits == 56) return launch_blind_rotate_crt(logn, t, radix_bits, a.level, a, s);
     if (tbits == 32 && logn == 10) {
         if (t.cpw >= 4) return p8 ? launch_br<BrCfg<32, 10, 1, 4, true>>(a, s) : launch_br<BrCfg<32, 10, 1, 4, false>>(a, s);
@@ -577,11 +626,12 @@ hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& t, int radix_b
 }
 
 int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* out, int max) {
-    BrTile c[8];
+    BrTile c[MAX_TILES];
     int n = 0;
     if (tbits == 32 && logn == 10 && level == 3 && radix_bits == 7) {
         for (int cpw : {4, 2, 1})
-            for (int twl : {1, 0}) c[n++] = BrTile{cpw, twl};
+            for (int twl : {1, 0})
+                for (int tab : {0, 1, 2}) c[n++] = BrTile{cpw, twl, 0, tab};
     } else if (tbits == 56 && logn == 11 && level == 3 && radix_bits == 10) {
         c[n++] = BrTile{1, 1};
         c[n++] = BrTile{1, 0};
