@@ -39,12 +39,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# The serving leg runs two parameter sets with three batch lanes and a key-copy
-# stream each; HIP multiplexes streams over GPU_MAX_HW_QUEUES hardware queues
-# (4 by default), and a short batch sharing a queue with a long one waits for
-# it.  The HIP runtime reads this when it initialises (first library call).
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 16:
-    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 METRIC = "PBS/sec at N=1024 bit-exact, batched vs plain on GPU; p50/p99 serving latency"
 BATCH = 4096
@@ -478,6 +472,7 @@ def leg_serving(yg, args, peak):
                           f"{r['locality']:.2f} spilled {r['spilled']} stolen {r['stolen']} shed {r['shed']} mismatches "
                           f"{r['mismatches']}", file=sys.stderr, flush=True)
             st = srv.stats()
+            lanes = [srv.lanes(ps32), srv.lanes(ps56)]
         ok = [i for i, x in enumerate(sweep)
               if x["shed"] == 0 and x["errors"] == 0 and x["throughput_rps"] >= 0.9 * x["offered_rps"]]
         best = max((sweep[i] for i in ok), key=lambda x: x["throughput_rps"]) if ok else None
@@ -488,6 +483,8 @@ def leg_serving(yg, args, peak):
             "key_budget_gib": budget_gb,
             "eviction": args.serving_eviction,
             "keys_prefetched": prefetched,
+            "lanes_t32_t56": lanes,
+            "hardware_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default)"),
             "saturation": None if top is None else {
                 "offered_rps": top["offered_rps"], "throughput_rps": top["throughput_rps"],
                 "roofline_frac": round(top["throughput_rps"] * mm_avg / MODMUL_CEILING, 4)},
@@ -600,7 +597,8 @@ def main():
     ap.add_argument("--serving-rates", default=",".join(str(r) for r in SERVING_RATES))
     ap.add_argument("--serving-eviction", default="freq_bytes", choices=["freq_bytes", "lru"])
     ap.add_argument("--keyload-reps", type=int, default=5)
-    ap.add_argument("--serving-lanes", type=int, default=3, help="in-flight batches per parameter set")
+    ap.add_argument("--serving-lanes", type=int, default=0,
+                    help="in-flight batches per parameter set (0: the server fits its streams to the hardware queues)")
     ap.add_argument("--serving-sched-modes", default="affinity,soft",
                     help="extra serving runs (all keys resident) with these scheduling modes ('' for none)")
     ap.add_argument("--serving-sched-rates", default="5000,20000")

@@ -374,7 +374,8 @@ typedef struct ygpu_server ygpu_server;
 
 typedef struct ygpu_server_config {
     int32_t device;
-    int32_t lanes;            /* in-flight batches per parameter set (0: 3) */
+    int32_t lanes;            /* in-flight batches per parameter set (0: fitted to the
+                                 hardware queues, see ygpu_server_lanes) */
     int32_t max_batch;        /* ciphertexts per launch (0: 4096) */
     int32_t queue_max;        /* queued requests per parameter set before submit sheds (0: 2^20) */
     int32_t max_tenants;      /* tenant table size (0: 4096) */
@@ -441,6 +442,16 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out);
 int ygpu_server_destroy(ygpu_server* s);
 /* Parameter sets (one engine context each); both torus widths may coexist. */
 int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* pset);
+/* Lanes of a parameter set (made at the first request or prefetch).  With
+ * lanes = 0 in the configuration the server's streams are fitted to the
+ * device's hardware queues (GPU_MAX_HW_QUEUES, 4 by default, read when the
+ * server starts): one copy stream for every key load plus lanes, one queue
+ * each, so no lane waits behind another's kernels; every parameter set has
+ * one lane and the remaining queues go to the sets of the cheapest blind
+ * rotation first (at most 3 a set).  The server assumes no other stream of
+ * the process is busy on the device while it serves.  (No counterpart in the
+ * reference, whose workers are CPU threads.) */
+int ygpu_server_lanes(ygpu_server* s, int32_t pset, int32_t* lanes);
 int ygpu_server_set_tile(ygpu_server* s, int32_t pset, int32_t cpw, int32_t twl);
 /* Extraction coefficient (param.driftPhase) of a parameter set's bootstraps;
  * before the first request.  See ygpu_set_extract_index. */

@@ -209,6 +209,7 @@ def load_library(path: str | None = None):
         "ygpu_get_extract_index": (c_int, [c_vp, P(c_i32)]),
         "ygpu_sample_extract_index": (c_int, [c_vp, c_vp, c_i32, c_vp]),
         "ygpu_server_set_extract_index": (c_int, [c_vp, c_i32, c_i32]),
+        "ygpu_server_lanes": (c_int, [c_vp, c_i32, P(c_i32)]),
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_classic": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_succinct": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
@@ -263,7 +264,7 @@ def exported_symbols():
         "ygpu_key_upload_succinct", "ygpu_keygen_succinct", "ygpu_server_add_tenant_succinct",
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
         "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls", "ygpu_pbs_batch_device_noalloc",
-        "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index",
+        "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index", "ygpu_server_lanes",
         "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform", "ygpu_get_tile_ex", "ygpu_set_tile_ex", "ygpu_autotune_results",
     ]
 
@@ -843,7 +844,8 @@ class Server:
         _check(self.lib.ygpu_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
         self.params = []
         self.tenant_params = []
-        self.config = {"lanes": lanes or 3, "max_batch": max_batch or 4096, "queue_max": queue_max or (1 << 20),
+        # lanes 0: the server fits its streams to the hardware queues (lanes())
+        self.config = {"lanes": lanes, "max_batch": max_batch or 4096, "queue_max": queue_max or (1 << 20),
                        "eviction": eviction, "sched": sched, "spill_threshold": spill_threshold or 2,
                        "steal_threshold": steal_threshold or 2, "key_budget_bytes": key_budget_bytes}
 
@@ -867,6 +869,11 @@ class Server:
 
     def set_tile(self, pset: int, cpw: int, twl: int):
         _check(self.lib.ygpu_server_set_tile(self.h, pset, cpw, twl))
+
+    def lanes(self, pset: int) -> int:
+        n = ctypes.c_int32(0)
+        _check(self.lib.ygpu_server_lanes(self.h, pset, ctypes.byref(n)))
+        return n.value
 
     def set_extract_index(self, pset: int, index: int):
         _check(self.lib.ygpu_server_set_extract_index(self.h, pset, index))

@@ -111,6 +111,7 @@ struct ygpu_ctx {
     ygpu_params p{};
     int device = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = true;  // false: borrowed from the owner (a server's copy stream)
     int logn = 0, np = 1, tbits = 32;
     BrTile tile{};          // batched-form tile (autotuned or default)
     std::vector<std::pair<BrTile, double>> tune;  // the last autotune's candidates and their ms
@@ -684,6 +685,11 @@ inline hipError_t expand_key(const ygpu_ctx* ctx, int s2_level, void* d_stage, d
     a.crt = ctx->crt;
     return launch_bsk_derive(ctx->logn, ctx->tbits, a, s);
 }
+
+// A context issuing its work on `stream`, which the caller owns and keeps
+// alive until ygpu_destroy; nullptr creates a stream of the context's own
+// (ygpu_init).  Initialisation issues no work on the null stream.
+int init_context(const ygpu_params* params, int32_t device, hipStream_t stream, ygpu_ctx** out);
 
 // Bytes a key occupies on the device, without building it.
 inline size_t key_device_bytes(const ygpu_ctx* ctx) {

@@ -69,7 +69,13 @@ int ygpu_device_count(int32_t* count) {
     return YGPU_OK;
 }
 
-int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
+}  // extern "C"
+
+namespace ygpu_rt {
+
+int build_context(ygpu_ctx* ctx, const ygpu_params* params, int32_t device, hipStream_t stream);
+
+int init_context(const ygpu_params* params, int32_t device, hipStream_t stream, ygpu_ctx** out) {
     if (!params || !out) return fail(YGPU_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     int rc = check_params(*params);
@@ -77,7 +83,18 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(YGPU_ERR_NO_DEVICE, "no HIP device");
     if (device < 0 || device >= ndev) return fail(YGPU_ERR_INVALID_ARG, "device index out of range");
-    std::unique_ptr<ygpu_ctx> ctx(new ygpu_ctx());
+    ygpu_ctx* ctx = new ygpu_ctx();
+    rc = build_context(ctx, params, device, stream);
+    if (rc) {
+        const std::string msg = ygpu_last_error();
+        ygpu_destroy(ctx);  // frees what was built
+        return fail(rc, msg);
+    }
+    *out = ctx;
+    return YGPU_OK;
+}
+
+int build_context(ygpu_ctx* ctx, const ygpu_params* params, int32_t device, hipStream_t stream) {
     ctx->p = *params;
     ctx->device = device;
     ctx->tbits = params->torus_bits;
@@ -100,7 +117,12 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
     ctx->tile = BrTile{t32 ? 2 : 1, (t32 || params->torus_bits == 56) ? 1 : 0, 0, t32 ? 2 : 0};
     if (const char* e = std::getenv("YGPU_CPW")) ctx->tile.cpw = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
-    YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    if (stream) {
+        ctx->stream = stream;
+        ctx->owns_stream = false;
+    } else {
+        YG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    }
     const int N = params->N;
     std::vector<double> tw(static_cast<size_t>(ctx->np) * 2 * N);
     for (int k = 0; k < ctx->np; k++) {
@@ -125,15 +147,16 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         ctx->crt.qhalf = static_cast<int64_t>((q - 1) / 2);
     }
     YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_tw), sizeof(double) * tw.size()));
-    YG_HIP(hipMemcpy(ctx->d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice));
+    // copies on the context's stream: no null-stream work, which would hold
+    // one of the device's hardware queues for the rest of the process
+    YG_HIP(hipMemcpyAsync(ctx->d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice, ctx->stream));
     // key tables of fixed capacity, all slots empty
-    {
-        std::vector<const void*> empty(KEY_SLOTS, nullptr);
-        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_bsk_table), sizeof(void*) * KEY_SLOTS));
-        YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ksk_table), sizeof(void*) * KEY_SLOTS));
-        YG_HIP(hipMemcpy(ctx->d_bsk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice));
-        YG_HIP(hipMemcpy(ctx->d_ksk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice));
-    }
+    std::vector<const void*> empty(KEY_SLOTS, nullptr);
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_bsk_table), sizeof(void*) * KEY_SLOTS));
+    YG_HIP(hipMalloc(reinterpret_cast<void**>(&ctx->d_ksk_table), sizeof(void*) * KEY_SLOTS));
+    YG_HIP(hipMemcpyAsync(ctx->d_bsk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice, ctx->stream));
+    YG_HIP(hipMemcpyAsync(ctx->d_ksk_table, empty.data(), sizeof(void*) * KEY_SLOTS, hipMemcpyHostToDevice, ctx->stream));
+    YG_HIP(hipStreamSynchronize(ctx->stream));
     ctx->d_static_scratch = static_scratch_address(device);
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->h_status), 64, hipHostMallocDefault));
     *ctx->h_status = 0u;
@@ -142,8 +165,15 @@ int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
         YG_HIP(hipHostGetDevicePointer(&dp, ctx->h_status, 0));
         ctx->hd_status = static_cast<uint32_t*>(dp);
     }
-    *out = ctx.release();
     return YGPU_OK;
+}
+
+}  // namespace ygpu_rt
+
+extern "C" {
+
+int ygpu_init(const ygpu_params* params, int32_t device, ygpu_ctx** out) {
+    return init_context(params, device, nullptr, out);
 }
 
 int ygpu_destroy(ygpu_ctx* ctx) {
@@ -158,7 +188,7 @@ int ygpu_destroy(ygpu_ctx* ctx) {
         });
         for (auto& l : ctx->lanes) free_lane(*l);
         ctx->lanes.clear();
-        hipStreamSynchronize(ctx->stream);
+        if (ctx->stream) hipStreamSynchronize(ctx->stream);
         for (auto& k : ctx->keys) {
             if (k.d_bsk) hipFree(k.d_bsk);
             if (k.d_ksk) hipFree(k.d_ksk);
@@ -172,7 +202,7 @@ int ygpu_destroy(ygpu_ctx* ctx) {
         if (ctx->h_status) hipHostFree(ctx->h_status);
         for (auto& t : ctx->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
         for (auto e : ctx->event_pool) hipEventDestroy(e);
-        hipStreamDestroy(ctx->stream);
+        if (ctx->owns_stream && ctx->stream) hipStreamDestroy(ctx->stream);
     }
     delete ctx;
     return YGPU_OK;
@@ -812,8 +842,9 @@ int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, in
         YG_HIP(hipMalloc(&sc.lwe, lwe.size() * sizeof(int32_t)));
         YG_HIP(hipMalloc(&sc.v, ctx->torus_bytes() * p.N));
         YG_HIP(hipMalloc(&sc.acc, ctx->torus_bytes() * (p.k + 1) * p.N * static_cast<size_t>(count)));
-        YG_HIP(hipMemcpy(sc.lwe, lwe.data(), lwe.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-        YG_HIP(hipMemset(sc.v, 0, ctx->torus_bytes() * p.N));
+        YG_HIP(hipMemcpyAsync(sc.lwe, lwe.data(), lwe.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+        YG_HIP(hipMemsetAsync(sc.v, 0, ctx->torus_bytes() * p.N, ctx->stream));
+        YG_HIP(hipStreamSynchronize(ctx->stream));
         YG_HIP(hipEventCreate(&sc.e0));
         YG_HIP(hipEventCreate(&sc.e1));
         void *d_lwe = sc.lwe, *d_v = sc.v, *d_acc = sc.acc;
@@ -879,7 +910,8 @@ int ygpu_graph_create(ygpu_ctx* ctx, const ygpu_job* jobs, int32_t n_jobs, int64
         GraphRec g;
         const ygpu_params& p = ctx->p;
         YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_slots), sizeof(int32_t) * slots.size()));
-        YG_HIP(hipMemcpy(g.d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice));
+        YG_HIP(hipMemcpyAsync(g.d_slots, slots.data(), sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice, ctx->stream));
+        YG_HIP(hipStreamSynchronize(ctx->stream));
         YG_HIP(hipMalloc(reinterpret_cast<void**>(&g.d_partial), ks_workspace_bytes(ctx, static_cast<int>(total))));
         void* acc = d_acc_out;
         if (!acc) {

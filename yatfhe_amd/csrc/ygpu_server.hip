@@ -128,7 +128,7 @@ struct PSet {
     std::deque<int> order;            // tenants with pending requests, oldest first
     int64_t queued = 0;               // pending requests
     std::vector<int64_t> lane_depth;  // pending requests queued on each lane (affinity, soft)
-    hipStream_t copy_stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // the server's (shared by the parameter sets)
     void* d_stage = nullptr;          // packed BSK of the load being expanded (either form)
     std::deque<int> loads;            // tenants being loaded, in copy order
     std::deque<int> inflight;         // lanes in launch order
@@ -157,6 +157,7 @@ struct ygpu_server {
     const int32_t** d_ksk_tab = nullptr;
     const double** h_bsk_tab = nullptr;   // pinned mirrors (copy sources)
     const int32_t** h_ksk_tab = nullptr;
+    hipStream_t copy_stream = nullptr;  // key loads of every parameter set; the contexts' stream
     bool started = false;
     bool stop = false;
     uint64_t tick = 0;
@@ -303,8 +304,8 @@ int start_load(ygpu_server* s, PSet& ps, int t) {
     s->h_bsk_tab[t] = reinterpret_cast<const double*>(base);
     s->h_ksk_tab[t] = reinterpret_cast<const int32_t*>(base + align256(f.bsk_bytes));
     hipStream_t cs = ps.copy_stream;
-    // packed residues -> staging -> engine form; loads of one parameter set
-    // run in order on its copy stream, so one staging buffer serves them all
+    // packed residues -> staging -> engine form; loads run in order on the
+    // server's copy stream, so one staging buffer per parameter set serves them all
     hipError_t e = hipMemcpyAsync(ps.d_stage, f.packed, f.packed_bytes, hipMemcpyHostToDevice, cs);
     if (e == hipSuccess) e = expand_key(ps.ctx, f.s2_level, ps.d_stage, reinterpret_cast<double*>(base), cs);
     if (e == hipSuccess) e = hipMemcpyAsync(base + align256(f.bsk_bytes), f.planes, f.planes_bytes, hipMemcpyHostToDevice, cs);
@@ -675,8 +676,55 @@ void completer_loop(ygpu_server* s, PSet* ps) {
     }
 }
 
+// Hardware queues HIP multiplexes the process's streams over
+// (GPU_MAX_HW_QUEUES, 4 by default).
+int hardware_queues() {
+    const char* e = std::getenv("GPU_MAX_HW_QUEUES");
+    const int q = e ? std::atoi(e) : 0;
+    return q > 0 ? q : 4;
+}
+
+// Lanes of each parameter set.  A lane count given in the configuration is
+// used for every set.  Otherwise the server's streams are fitted to the
+// hardware queues: one queue for the copy stream and one per lane, so that no
+// two lanes share a queue (a stream sharing a queue waits behind the other
+// stream's kernels, and a short TORUS32 batch behind a long TORUS56 batch
+// loses its latency).  Every set has one lane; the remaining queues go to the
+// sets of the cheapest blind rotation first (N times the number of primes),
+// where overlap of short batches pays most, at most 3 lanes a set.
+std::vector<int> lane_plan(const ygpu_server* s) {
+    const int P = static_cast<int>(s->psets.size());
+    std::vector<int> n(P, s->cfg.lanes > 0 ? s->cfg.lanes : 1);
+    if (s->cfg.lanes > 0 || P == 0) return n;
+    std::vector<int> order(P);
+    for (int i = 0; i < P; i++) order[i] = i;
+    auto cost = [&](int i) { return static_cast<int64_t>(s->psets[i]->p.N) * s->psets[i]->ctx->np; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) < cost(b); });
+    int spare = hardware_queues() - 1 - P;
+    for (int round = 0; round < 2 && spare > 0; round++)
+        for (int i : order)
+            if (spare > 0) { n[i]++; spare--; }
+    return n;
+}
+
 int start_threads(ygpu_server* s) {
     if (s->started) return YGPU_OK;
+    const std::vector<int> plan = lane_plan(s);
+    for (size_t i = 0; i < s->psets.size(); i++) {
+        PSet& ps = *s->psets[i];
+        if (!ps.lanes.empty()) continue;
+        ps.lanes.resize(plan[i]);
+        ps.lane_depth.assign(plan[i], 0);
+        for (Lane& L : ps.lanes) {
+            const int rc = lane_alloc(s, ps, L);
+            if (rc) {
+                for (Lane& l2 : ps.lanes) lane_free(l2);
+                ps.lanes.clear();
+                ps.lane_depth.clear();
+                return rc;
+            }
+        }
+    }
     for (auto& ps : s->psets) {
         ps->dispatcher = std::thread(dispatcher_loop, s, ps.get());
         ps->completer = std::thread(completer_loop, s, ps.get());
@@ -710,7 +758,7 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
     if (cfg->device < 0 || cfg->device >= ndev) return fail(YGPU_ERR_INVALID_ARG, "device index out of range");
     auto s = std::make_unique<ygpu_server>();
     s->cfg = *cfg;
-    if (s->cfg.lanes <= 0) s->cfg.lanes = 3;
+    if (s->cfg.lanes < 0) s->cfg.lanes = 0;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 4096;
     if (s->cfg.queue_max <= 0) s->cfg.queue_max = 1 << 20;
     if (s->cfg.max_tenants <= 0) s->cfg.max_tenants = 4096;
@@ -742,8 +790,10 @@ int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out) {
     YG_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_ksk_tab), tab, hipHostMallocDefault));
     std::memset(s->h_bsk_tab, 0, tab);
     std::memset(s->h_ksk_tab, 0, tab);
-    YG_HIP(hipMemcpy(s->d_bsk_tab, s->h_bsk_tab, tab, hipMemcpyHostToDevice));
-    YG_HIP(hipMemcpy(s->d_ksk_tab, s->h_ksk_tab, tab, hipMemcpyHostToDevice));
+    YG_HIP(hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking));
+    YG_HIP(hipMemcpyAsync(s->d_bsk_tab, s->h_bsk_tab, tab, hipMemcpyHostToDevice, s->copy_stream));
+    YG_HIP(hipMemcpyAsync(s->d_ksk_tab, s->h_ksk_tab, tab, hipMemcpyHostToDevice, s->copy_stream));
+    YG_HIP(hipStreamSynchronize(s->copy_stream));
     *out = s.release();
     return YGPU_OK;
 }
@@ -754,31 +804,30 @@ int ygpu_server_add_params(ygpu_server* s, const ygpu_params* params, int32_t* p
         std::lock_guard<std::mutex> g(s->mu);
         if (s->started) return fail(YGPU_ERR_INVALID_ARG, "parameter sets must be added before the first request");
         auto ps = std::make_unique<PSet>();
-        int rc = ygpu_init(params, s->cfg.device, &ps->ctx);
+        // the context's own work (set-up, autotuning) runs on the copy stream:
+        // the server's streams are the copy stream and the lanes, made at the
+        // first request
+        int rc = init_context(params, s->cfg.device, s->copy_stream, &ps->ctx);
         if (rc) return rc;
         ps->p = ps->ctx->p;
         ps->tbytes = ps->ctx->torus_bytes();
-        if (hipStreamCreateWithFlags(&ps->copy_stream, hipStreamNonBlocking) != hipSuccess) {
-            ygpu_destroy(ps->ctx);
-            return fail(YGPU_ERR_HIP, "copy stream");
-        }
+        ps->copy_stream = s->copy_stream;
         if (hipMalloc(&ps->d_stage, stage_bytes_any(ps->p, ps->ctx->np)) != hipSuccess) {
-            hipStreamDestroy(ps->copy_stream);
             ygpu_destroy(ps->ctx);
             return fail(YGPU_ERR_OUT_OF_MEMORY, "key staging buffer");
         }
-        ps->lanes.resize(s->cfg.lanes);
-        ps->lane_depth.assign(s->cfg.lanes, 0);
-        for (Lane& L : ps->lanes) {
-            rc = lane_alloc(s, *ps, L);
-            if (rc) {
-                for (Lane& l2 : ps->lanes) lane_free(l2);
-                ygpu_destroy(ps->ctx);
-                return rc;
-            }
-        }
         *pset = static_cast<int32_t>(s->psets.size());
         s->psets.push_back(std::move(ps));
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_lanes(ygpu_server* s, int32_t pset, int32_t* lanes) {
+    return with_server(s, [&] {
+        if (!lanes) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+        *lanes = s->started ? static_cast<int32_t>(s->psets[pset]->lanes.size()) : lane_plan(s)[pset];
         return YGPU_OK;
     });
 }
@@ -905,7 +954,10 @@ int ygpu_server_prefetch(ygpu_server* s, int32_t tenant) {
     return with_server(s, [&] {
         std::unique_lock<std::mutex> lk(s->mu);
         if (tenant < 0 || tenant >= static_cast<int32_t>(s->tenants.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown tenant");
-        if (!s->started) start_threads(s);
+        if (!s->started) {
+            const int rc = start_threads(s);
+            if (rc) return rc;
+        }
         Tenant& tn = s->tenants[tenant];
         if (!tn.resident) {
             const int rc = start_load(s, *s->psets[tn.pset], tenant);
@@ -942,7 +994,10 @@ int ygpu_server_submit(ygpu_server* s, ygpu_request* req) {
             req->status = YGPU_ERR_BUSY;
             return fail(YGPU_ERR_BUSY, "queue full: request shed");
         }
-        if (!s->started) start_threads(s);
+        if (!s->started) {
+            const int rc = start_threads(s);
+            if (rc) return rc;
+        }
         req->status = 1;
         if (req->arrival_ns == 0) req->arrival_ns = now_ns();
         req->seq = ++s->tick;
@@ -1026,15 +1081,13 @@ int ygpu_server_destroy(ygpu_server* s) {
         }
         s->cv_req.notify_all();
     }
+    if (s->copy_stream) hipStreamSynchronize(s->copy_stream);
     for (auto& ps : s->psets) {
-        if (ps->copy_stream) {
-            hipStreamSynchronize(ps->copy_stream);
-            hipStreamDestroy(ps->copy_stream);
-        }
         if (ps->d_stage) hipFree(ps->d_stage);
         for (Lane& L : ps->lanes) lane_free(L);
         ygpu_destroy(ps->ctx);
     }
+    if (s->copy_stream) hipStreamDestroy(s->copy_stream);
     for (Tenant& t : s->tenants)
         if (t.loaded) hipEventDestroy(t.loaded);
     for (PinnedKey& f : s->forms) {
@@ -1160,7 +1213,9 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
         service.reserve(c->requests);
         std::unordered_map<int, std::pair<double, int64_t>> perTenant;
         int64_t laneCount[16] = {};
-        const int lanes = s->cfg.lanes;
+        std::vector<int> laneBase(s->psets.size() + 1, 0);  // first global lane of each set
+        for (size_t k = 0; k < s->psets.size(); k++)
+            laneBase[k + 1] = laneBase[k] + static_cast<int>(s->psets[k]->lanes.size());
         const bool affine = s->cfg.sched != YGPU_SCHED_SHARED;
         // a run that has not drained 120 s after its last arrival is stuck
         const int64_t deadline = now_ns() + int64_t{120} * 1000000000;
@@ -1196,9 +1251,10 @@ int ygpu_server_loadgen(ygpu_server* s, const ygpu_loadgen_config* c, ygpu_loadg
             pt.first += lat.back();
             pt.second++;
             coalesced += reqs[i].coalesced;
-            const int gl = psetOf[reqs[i].tenant] * lanes + reqs[i].lane;
+            const int gl = laneBase[psetOf[reqs[i].tenant]] + reqs[i].lane;
             if (reqs[i].lane >= 0 && gl < 16) laneCount[gl]++;
-            owned += affine && reqs[i].lane == reqs[i].tenant % lanes;
+            const int nl = laneBase[psetOf[reqs[i].tenant] + 1] - laneBase[psetOf[reqs[i].tenant]];
+            owned += affine && nl > 0 && reqs[i].lane == reqs[i].tenant % nl;
             if (c->expected && c->expected[reqs[i].tenant] &&
                 std::memcmp(reqs[i].lwe_out, c->expected[reqs[i].tenant], sizeof(int32_t) * cols[reqs[i].tenant]) != 0)
                 mismatches++;
