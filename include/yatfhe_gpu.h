@@ -436,6 +436,7 @@ typedef struct ygpu_server_stats {
     int64_t coalesced, spilled, stolen, peak_queue_depth;
     int64_t loads_in_flight, peak_loads_in_flight;  /* key loads (copy + expansion) under way */
     int64_t load_inflight_ns;                       /* integral over time of loads in flight (ns) */
+    int64_t file_reads, file_read_bytes, file_read_ns;  /* key files read on a miss: count, transfer-form bytes, time */
 } ygpu_server_stats;
 
 int ygpu_server_create(const ygpu_server_config* cfg, ygpu_server** out);
@@ -468,6 +469,23 @@ int ygpu_server_add_tenant_succinct(ygpu_server* s, int32_t pset, const uint64_t
 /* A new tenant sharing another tenant's host key material but cached on the
  * device on its own (separate residency, loads and evictions). */
 int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* tenant);
+/* A tenant whose key stays in a key file (either form, ygpu_key_file_write*)
+ * until a load needs it: the lowest tier, below pinned host memory.  Replaces
+ * the reference's DiskReader behind its serving key cache
+ * (ya_benchmark/include/cache_manager.h:13-51; the loader reads the tenant's
+ * key from file on a miss, ya_benchmark/blindrotate_serving.cpp:210-219).
+ * Registration reads and checks the header only (parameters must equal the
+ * set's).  On a miss one reader thread reads the whole file (digests checked)
+ * into one of the set's pinned slots, packs it as ygpu_server_add_tenant
+ * would, and the device load follows; a slot keeps its form until it is
+ * reused (least recently used first, never while a load reads it).  A file
+ * that fails to read fails the tenant's pending requests with its error
+ * (YGPU_ERR_CORRUPT, YGPU_ERR_IO ...) and is read again on the next miss. */
+int ygpu_server_add_tenant_file(ygpu_server* s, int32_t pset, const char* path, int32_t* tenant);
+/* Pinned slots of a parameter set for file-backed keys (default 2, at most
+ * 64), each a full key's transfer form; allocated with the set's first
+ * file-backed tenant, so set before it. */
+int ygpu_server_set_file_slots(ygpu_server* s, int32_t pset, int32_t slots);
 int ygpu_server_tenant_params(ygpu_server* s, int32_t tenant, int32_t* pset);
 /* Loads a tenant's key into the device cache now (evicting idle keys if
  * needed) and waits for it; YGPU_ERR_BUSY when no memory can be freed. */

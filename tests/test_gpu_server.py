@@ -236,6 +236,66 @@ def test_succinct_tenants_alone_equal_mix(p32):
             assert np.array_equal(ref[c], alone[(t, c)]), f"tenant {t} ciphertext {c}: server differs from the oracle"
 
 
+def test_key_file_tier_vs_oracle(tmp_path, p32):
+    """Tenants whose keys stay in key files, the tier below pinned host memory
+    (the reference's DiskReader behind its serving key cache): one pinned slot
+    and device room for two keys, so every round reads the files again.  Every
+    served result equals the oracle's bootstrap under the tenant's key (full
+    keys, and a succinct key under its oracle-derived full key); a corrupt
+    file fails only its own tenant's requests."""
+    v32 = yg.sign_test_polynomial(p32)
+    keys = [yg.keygen(p32, seed=800 + t) for t in range(2)] + [yg.keygen_succinct(p32, seed=810, s2_level=4)]
+    paths = []
+    for i, k in enumerate(keys):
+        paths.append(str(tmp_path / f"tenant{i}.ygkey"))
+        yg.save_keys(paths[-1], k)
+    bad = str(tmp_path / "corrupt.ygkey")
+    yg.save_keys(bad, keys[0])
+    with open(bad, "r+b") as f:  # one bit of a BSK residue, past the 128-byte header
+        f.seek(4096)
+        b = f.read(1)
+        f.seek(4096)
+        f.write(bytes([b[0] ^ 1]))
+    work = [(k, t32_inputs(k, 4, 90 + i)) for i, k in enumerate(keys)]
+    got = {}
+    with yg.Server(0, key_budget_bytes=2 * 83 << 20, lanes=1, max_batch=64) as srv:
+        ps = srv.add_params(p32)
+        srv.set_file_slots(ps, 1)
+        tids = [srv.add_tenant_file(ps, path) for path in paths]
+        tbad = srv.add_tenant_file(ps, bad)
+        print("server: file tier prefetch", flush=True)
+        assert srv.prefetch(tids[0])
+        for rnd in range(2):
+            print(f"server: file tier round {rnd}", flush=True)
+            for t, (k, cts) in zip(tids, work):
+                for c in range(len(cts)):
+                    r = srv.request(t, cts[c], v32)
+                    srv.submit(r)
+                    got[(rnd, t, c)] = srv.wait(r, timeout=60).copy()
+        print("server: corrupt key file", flush=True)
+        r = srv.request(tbad, work[0][1][0], v32)
+        srv.submit(r)
+        with pytest.raises(yg.YgpuError) as err:
+            srv.wait(r, timeout=60)
+        assert err.value.code == yg.YGPU_ERR_CORRUPT
+        r = srv.request(tids[1], work[1][1][0], v32)  # the others are still served
+        srv.submit(r)
+        assert np.array_equal(srv.wait(r, timeout=60), got[(0, tids[1], 0)])
+        st = srv.stats()
+        print("server stats", st, flush=True)
+        # keys were evicted and loaded again, and with one slot reloads read the files again
+        assert st["failed"] == 1 and st["key_evictions"] > 0 and st["peak_resident_bytes"] <= 2 * 83 << 20
+        assert st["key_loads"] > len(tids) and st["file_reads"] > len(tids)
+        assert st["file_read_bytes"] <= st["bytes_loaded"]
+    print("server: file tier oracle", flush=True)
+    for t, (k, cts) in zip(tids, work):
+        okeys = _Derived(p32, k) if isinstance(k, yg.SuccinctKeySet) else k
+        ref = oracle_results(p32, okeys, cts, v32)
+        for rnd in range(2):
+            for c in range(len(cts)):
+                assert np.array_equal(ref[c], got[(rnd, t, c)]), f"tenant {t} round {rnd} ciphertext {c} differs from the oracle"
+
+
 def test_serving_256_tenants_mixed_widths_vs_oracle(p32, p56):
     """The serving workload's shape: 256 tenants with Zipf(1.1) popularity, 30 %
     of them TORUS56 N=2048 (16-entry LUT) and 70 % TORUS32 N=1024 (sign gate),

@@ -24,6 +24,8 @@ Q_61P_T56 = 1945555039024054273  # torus.h:157
 
 YGPU_OK = 0
 YGPU_ERR_BUSY = -7
+YGPU_ERR_IO = -8
+YGPU_ERR_CORRUPT = -9
 YGPU_ERR_DEVICE = -10
 
 
@@ -75,7 +77,8 @@ class _Request(ctypes.Structure):
 _STAT_FIELDS = ["submitted", "completed", "failed", "shed", "batches", "batched_cts", "request_hits", "key_loads",
                 "key_evictions", "bytes_loaded", "memory_waits", "resident_keys", "resident_bytes",
                 "peak_resident_bytes", "key_budget_bytes", "host_key_bytes", "coalesced", "spilled", "stolen",
-                "peak_queue_depth", "loads_in_flight", "peak_loads_in_flight", "load_inflight_ns"]
+                "peak_queue_depth", "loads_in_flight", "peak_loads_in_flight", "load_inflight_ns",
+                "file_reads", "file_read_bytes", "file_read_ns"]
 
 
 class _ServerStats(ctypes.Structure):
@@ -222,6 +225,8 @@ def load_library(path: str | None = None):
         "ygpu_server_set_tile": (c_int, [c_vp, c_i32, c_i32, c_i32]),
         "ygpu_server_add_tenant": (c_int, [c_vp, c_i32, c_vp, c_vp, P(c_i32)]),
         "ygpu_server_add_tenant_shared": (c_int, [c_vp, c_i32, P(c_i32)]),
+        "ygpu_server_add_tenant_file": (c_int, [c_vp, c_i32, ctypes.c_char_p, P(c_i32)]),
+        "ygpu_server_set_file_slots": (c_int, [c_vp, c_i32, c_i32]),
         "ygpu_server_add_tenant_succinct": (c_int, [c_vp, c_i32, c_vp, c_vp, c_i32, c_vp, P(c_i32)]),
         "ygpu_server_tenant_params": (c_int, [c_vp, c_i32, P(c_i32)]),
         "ygpu_server_prefetch": (c_int, [c_vp, c_i32]),
@@ -265,6 +270,7 @@ def exported_symbols():
         "ygpu_key_file_write_succinct", "ygpu_key_file_form", "ygpu_key_file_read_succinct", "ygpu_keygen_classic",
         "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls", "ygpu_pbs_batch_device_noalloc",
         "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index", "ygpu_server_lanes",
+        "ygpu_server_add_tenant_file", "ygpu_server_set_file_slots",
         "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform", "ygpu_get_tile_ex", "ygpu_set_tile_ex", "ygpu_autotune_results",
     ]
 
@@ -903,6 +909,16 @@ class Server:
         _check(self.lib.ygpu_server_add_tenant_shared(self.h, src, ctypes.byref(t)))
         self.tenant_params.append(self.tenant_params[src])
         return t.value
+
+    def add_tenant_file(self, pset: int, path: str) -> int:
+        """A tenant whose key stays in a key file until a load needs it (read on a miss)."""
+        t = ctypes.c_int32()
+        _check(self.lib.ygpu_server_add_tenant_file(self.h, pset, os.fsencode(path), ctypes.byref(t)))
+        self.tenant_params.append(pset)
+        return t.value
+
+    def set_file_slots(self, pset: int, slots: int):
+        _check(self.lib.ygpu_server_set_file_slots(self.h, pset, slots))
 
     def prefetch(self, tenant: int) -> bool:
         """Make a tenant's key resident now; False when no memory can be freed."""

@@ -691,6 +691,18 @@ inline hipError_t expand_key(const ygpu_ctx* ctx, int s2_level, void* d_stage, d
 // (ygpu_init).  Initialisation issues no work on the null stream.
 int init_context(const ygpu_params* params, int32_t device, hipStream_t stream, ygpu_ctx** out);
 
+inline bool params_equal(const ygpu_params& a, const ygpu_params& b) {
+    return a.torus_bits == b.torus_bits && a.n == b.n && a.k == b.k && a.N == b.N && a.radix_bits == b.radix_bits &&
+           a.level == b.level && a.ks_radix_bits == b.ks_radix_bits && a.ks_level == b.ks_level &&
+           a.ks_width_bits == b.ks_width_bits && a.lwe_q_bits == b.lwe_q_bits && a.q_ntt == b.q_ntt;
+}
+
+// Words of the KSK byte planes (ksk_planes) of a parameter set.
+inline size_t ksk_plane_words(const ygpu_params& p) {
+    const size_t rows = static_cast<size_t>(p.k) * p.N * p.ks_level;
+    return (rows + 3) / 4 * 4 * (static_cast<size_t>(p.n) + 1);
+}
+
 // Bytes a key occupies on the device, without building it.
 inline size_t key_device_bytes(const ygpu_ctx* ctx) {
     const ygpu_params& p = ctx->p;

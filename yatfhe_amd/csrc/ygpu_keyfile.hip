@@ -95,11 +95,7 @@ void sizes(const ygpu_params& p, int s2_level, uint64_t* bsk, uint64_t* ksk) {
     *ksk = static_cast<uint64_t>(p.k) * p.N * p.ks_level * (p.n + 1);
 }
 
-bool same_params(const ygpu_params& a, const ygpu_params& b) {
-    return a.torus_bits == b.torus_bits && a.n == b.n && a.k == b.k && a.N == b.N && a.radix_bits == b.radix_bits &&
-           a.level == b.level && a.ks_radix_bits == b.ks_radix_bits && a.ks_level == b.ks_level &&
-           a.ks_width_bits == b.ks_width_bits && a.lwe_q_bits == b.lwe_q_bits && a.q_ntt == b.q_ntt;
-}
+bool same_params(const ygpu_params& a, const ygpu_params& b) { return params_equal(a, b); }
 
 struct File {
     FILE* f = nullptr;

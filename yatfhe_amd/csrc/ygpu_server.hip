@@ -61,12 +61,33 @@ inline int64_t now_ns() {
 
 // A key's transfer form in pinned host memory: bit-packed BSK residues (full
 // key, or succinct key with s2_level > 0) and KSK planes.  bsk_bytes is the
-// engine-form BSK size on the device.
+// engine-form BSK size on the device.  A file-backed form (the lowest tier)
+// holds only its key file's path until a load needs it: it is then read into
+// one of its parameter set's pinned slots, which it keeps until the slot is
+// reused for another file.
 struct PinnedKey {
     uint64_t* packed = nullptr;
     uint32_t* planes = nullptr;
     size_t packed_bytes = 0, planes_bytes = 0, bsk_bytes = 0;
     int s2_level = 0;
+    std::string path;      // key file (file-backed forms only)
+    int pset = -1;
+    int slot = -1;         // pinned slot holding it (file-backed, read or being read)
+    bool present = true;   // packed and planes hold the form
+    bool reading = false;  // queued for the reader or being read
+    int copies = 0;        // device loads in flight that read packed and planes
+    bool unused = false;   // read, and no load has used it yet (its slot is kept)
+    int read_rc = 0;       // the last read's status, and its message
+    std::string read_err;
+};
+
+// A pinned buffer of a parameter set for file-backed forms (a full key's
+// transfer form; a succinct form is smaller).
+struct FileSlot {
+    uint64_t* packed = nullptr;
+    uint32_t* planes = nullptr;
+    int form = -1;  // the form it holds or is being read into (-1: free)
+    uint64_t last_use = 0;
 };
 
 struct Tenant {
@@ -137,6 +158,8 @@ struct PSet {
     std::condition_variable cv_load;  // a load was started
     std::thread dispatcher, completer, watcher;
     bool dispatcher_done = false;
+    std::vector<FileSlot> slots;  // pinned slots for file-backed forms (made with the first)
+    int slot_count = 2;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~size_t{255}; }
@@ -158,6 +181,9 @@ struct ygpu_server {
     const double** h_bsk_tab = nullptr;   // pinned mirrors (copy sources)
     const int32_t** h_ksk_tab = nullptr;
     hipStream_t copy_stream = nullptr;  // key loads of every parameter set; the contexts' stream
+    std::deque<int> read_queue;         // file-backed forms to read, in request order
+    std::condition_variable cv_read;
+    std::thread reader;
     bool started = false;
     bool stop = false;
     uint64_t tick = 0;
@@ -295,11 +321,123 @@ void load_count(ygpu_server* s, int delta) {
     s->st.peak_loads_in_flight = std::max(s->st.peak_loads_in_flight, s->st.loads_in_flight);
 }
 
+void fail_pending(ygpu_server* s, PSet& ps, int t, int rc);
+
+// Queues the read of file-backed form `fi` into a slot of `ps`: a free slot,
+// else the least recently used one whose form no load is reading.  Returns 2
+// when the read is queued or under way, 3 when every slot is busy now, <0 on
+// error.  Server lock held.
+int request_read(ygpu_server* s, PSet& ps, int fi) {
+    PinnedKey& f = s->forms[fi];
+    if (f.reading) return 2;
+    int best = -1;
+    for (int i = 0; i < static_cast<int>(ps.slots.size()); i++) {
+        const FileSlot& sl = ps.slots[i];
+        if (sl.form < 0) {
+            best = i;
+            break;
+        }
+        const PinnedKey& o = s->forms[sl.form];
+        if (o.reading || o.copies > 0 || o.unused) continue;
+        if (best < 0 || sl.last_use < ps.slots[best].last_use) best = i;
+    }
+    if (best < 0) return ps.slots.empty() ? fail(YGPU_ERR_INVALID_ARG, "no pinned slots for file-backed keys") : 3;
+    FileSlot& sl = ps.slots[best];
+    if (sl.form >= 0) {
+        PinnedKey& o = s->forms[sl.form];
+        o.present = false;
+        o.packed = nullptr;
+        o.planes = nullptr;
+        o.slot = -1;
+    }
+    sl.form = fi;
+    sl.last_use = ++s->tick;
+    f.slot = best;
+    f.reading = true;
+    s->read_queue.push_back(fi);
+    s->cv_read.notify_one();
+    return 2;
+}
+
+// Reads a key file into a slot: the file's residues are packed and its KSK
+// cut into byte planes exactly as for a key handed over in memory.
+int read_form(const ygpu_ctx* ctx, const std::string& path, int s2_level, size_t packed_bytes, size_t planes_bytes,
+              uint64_t* packed, uint32_t* planes) {
+    const ygpu_params& p = ctx->p;
+    int32_t form = 0;
+    int rc = ygpu_key_file_form(path.c_str(), &form);
+    if (rc) return rc;
+    if (form != s2_level) return fail(YGPU_ERR_CORRUPT, path + ": key form changed since the tenant was registered");
+    const size_t half = half_polys(p) * p.N;
+    const size_t bskWords = s2_level ? half + s2_polys(p, s2_level) * p.N : bsk_polys(p) * p.N;
+    std::vector<uint64_t> bsk(bskWords);
+    std::vector<int32_t> ksk(static_cast<size_t>(p.k) * p.N * p.ks_level * (p.n + 1));
+    HostKeyForm hk;
+    ygpu_ctx* c = const_cast<ygpu_ctx*>(ctx);  // the preparers read only its parameters
+    if (s2_level == 0) {
+        rc = ygpu_key_file_read(path.c_str(), &p, bsk.data(), ksk.data());
+        if (!rc) rc = prepare_key_host(c, bsk.data(), ksk.data(), hk);
+    } else {
+        rc = ygpu_key_file_read_succinct(path.c_str(), &p, bsk.data(), bsk.data() + half, ksk.data());
+        if (!rc) rc = prepare_key_host_succinct(c, bsk.data(), bsk.data() + half, s2_level, ksk.data(), hk);
+    }
+    if (rc) return rc;
+    if (hk.packed.size() * sizeof(uint64_t) != packed_bytes || hk.planes.size() * sizeof(uint32_t) != planes_bytes)
+        return fail(YGPU_ERR_CORRUPT, path + ": key size differs from the registered one");
+    std::memcpy(packed, hk.packed.data(), packed_bytes);
+    std::memcpy(planes, hk.planes.data(), planes_bytes);
+    return YGPU_OK;
+}
+
+// Reads file-backed forms as loads ask for them, one at a time, without the
+// server lock (the reference's DiskReader on a cache miss,
+// cache_manager.h:13-51, blindrotate_serving.cpp:210-219).  A failed read
+// fails the pending requests of every tenant of that form.
+void reader_loop(ygpu_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cv_read.wait(lk, [&] { return s->stop || !s->read_queue.empty(); });
+        if (s->read_queue.empty()) break;
+        const int fi = s->read_queue.front();
+        s->read_queue.pop_front();
+        const PinnedKey f = s->forms[fi];  // a copy: forms may grow meanwhile
+        PSet& ps = *s->psets[f.pset];
+        const FileSlot sl = ps.slots[f.slot];
+        lk.unlock();
+        const int64_t t0 = now_ns();
+        const int rc = read_form(ps.ctx, f.path, f.s2_level, f.packed_bytes, f.planes_bytes, sl.packed, sl.planes);
+        const std::string err = rc ? ygpu_last_error() : std::string();
+        const int64_t t1 = now_ns();
+        lk.lock();
+        PinnedKey& g = s->forms[fi];
+        g.reading = false;
+        g.read_rc = rc;
+        g.read_err = err;
+        s->st.file_read_ns += t1 - t0;
+        if (rc == 0) {
+            g.present = true;
+            g.unused = true;
+            g.packed = sl.packed;
+            g.planes = sl.planes;
+            s->st.file_reads++;
+            s->st.file_read_bytes += static_cast<int64_t>(f.packed_bytes + f.planes_bytes);
+        } else {
+            ps.slots[g.slot].form = -1;
+            g.slot = -1;
+            for (size_t t = 0; t < s->tenants.size(); t++)
+                if (s->tenants[t].form == fi && !s->tenants[t].resident) fail_pending(s, ps, static_cast<int>(t), fail(rc, err));
+        }
+        notify_all_psets(s);
+        s->cv_req.notify_all();
+    }
+}
+
 int start_load(ygpu_server* s, PSet& ps, int t) {
     Tenant& tn = s->tenants[t];
+    if (!s->forms[tn.form].present) return request_read(s, ps, tn.form);
     size_t off = 0;
     if (!arena_alloc(s, tn.bytes, t, &off)) return 1;
-    const PinnedKey& f = s->forms[tn.form];
+    PinnedKey& f = s->forms[tn.form];
     char* base = s->d_arena + off;
     s->h_bsk_tab[t] = reinterpret_cast<const double*>(base);
     s->h_ksk_tab[t] = reinterpret_cast<const int32_t*>(base + align256(f.bsk_bytes));
@@ -322,6 +460,9 @@ int start_load(ygpu_server* s, PSet& ps, int t) {
     tn.ready = false;
     tn.fresh = true;
     tn.off = off;
+    f.copies++;
+    f.unused = false;
+    if (f.slot >= 0) ps.slots[f.slot].last_use = ++s->tick;
     tn.last_use = ++s->tick;
     ps.loads.push_back(t);
     load_count(s, +1);
@@ -619,6 +760,7 @@ void watcher_loop(ygpu_server* s, PSet* ps) {
         ps->loads.pop_front();
         load_count(s, -1);
         Tenant& tn = s->tenants[t];
+        s->forms[tn.form].copies--;  // its pinned form may now be replaced
         if (e != hipSuccess) {
             // the key never arrived: release its memory and fail its requests
             tn.ready = true;
@@ -725,6 +867,7 @@ int start_threads(ygpu_server* s) {
             }
         }
     }
+    s->reader = std::thread(reader_loop, s);
     for (auto& ps : s->psets) {
         ps->dispatcher = std::thread(dispatcher_loop, s, ps.get());
         ps->completer = std::thread(completer_loop, s, ps.get());
@@ -931,6 +1074,79 @@ int ygpu_server_add_tenant_succinct(ygpu_server* s, int32_t pset, const uint64_t
     });
 }
 
+int ygpu_server_set_file_slots(ygpu_server* s, int32_t pset, int32_t slots) {
+    return with_server(s, [&] {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+        if (slots < 1 || slots > 64) return fail(YGPU_ERR_INVALID_ARG, "slots must be in [1, 64]");
+        if (!s->psets[pset]->slots.empty())
+            return fail(YGPU_ERR_INVALID_ARG, "slots must be set before the parameter set's first file-backed tenant");
+        s->psets[pset]->slot_count = slots;
+        return YGPU_OK;
+    });
+}
+
+int ygpu_server_add_tenant_file(ygpu_server* s, int32_t pset, const char* path, int32_t* tenant) {
+    return with_server(s, [&] {
+        if (!path || !tenant) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+        ygpu_params fp{};
+        int rc = ygpu_key_file_info(path, &fp);
+        if (rc) return rc;
+        int32_t s2_level = 0;
+        rc = ygpu_key_file_form(path, &s2_level);
+        if (rc) return rc;
+        hipSetDevice(s->cfg.device);
+        std::lock_guard<std::mutex> g(s->mu);
+        if (pset < 0 || pset >= static_cast<int32_t>(s->psets.size())) return fail(YGPU_ERR_NOT_FOUND, "unknown parameter set");
+        if (static_cast<int>(s->tenants.size()) >= s->cfg.max_tenants) return fail(YGPU_ERR_INVALID_ARG, "max_tenants reached");
+        PSet& ps = *s->psets[pset];
+        const ygpu_params& p = ps.p;
+        if (!params_equal(fp, p)) return fail(YGPU_ERR_INVALID_ARG, std::string(path) + ": parameters differ from the parameter set's");
+        if (s2_level > 0) {
+            rc = check_s2_level(p, s2_level);
+            if (rc) return rc;
+        }
+        PinnedKey f;
+        f.path = path;
+        f.pset = pset;
+        f.present = false;
+        f.s2_level = s2_level;
+        f.bsk_bytes = bsk_polys(p) * ps.ctx->np * p.N * sizeof(double);
+        f.packed_bytes = (s2_level ? packed_words_succinct(p, s2_level) : packed_words(p)) * sizeof(uint64_t);
+        f.planes_bytes = ksk_plane_words(p) * sizeof(uint32_t);
+        if (align256(f.bsk_bytes) + align256(f.planes_bytes) > s->arena_bytes)
+            return fail(YGPU_ERR_OUT_OF_MEMORY, "key larger than the key budget");
+        if (ps.slots.empty()) {
+            // the set's pinned slots, each sized for a full key's transfer form
+            std::vector<FileSlot> made(ps.slot_count);
+            hipError_t e = hipSuccess;
+            for (FileSlot& sl : made) {
+                if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&sl.packed), packed_words(p) * sizeof(uint64_t), hipHostMallocDefault);
+                if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&sl.planes), f.planes_bytes, hipHostMallocDefault);
+            }
+            if (e != hipSuccess) {
+                for (FileSlot& sl : made) {
+                    if (sl.packed) hipHostFree(sl.packed);
+                    if (sl.planes) hipHostFree(sl.planes);
+                }
+                return server_fail_hip(e, "pinned slots for file-backed keys");
+            }
+            ps.slots = std::move(made);
+            s->st.host_key_bytes += static_cast<int64_t>(ps.slots.size() * (packed_words(p) * sizeof(uint64_t) + f.planes_bytes));
+        }
+        Tenant t;
+        t.pset = pset;
+        t.bytes = align256(f.bsk_bytes) + align256(f.planes_bytes);
+        t.load_bytes = f.packed_bytes + f.planes_bytes;
+        YG_HIP(hipEventCreateWithFlags(&t.loaded, hipEventDisableTiming));
+        t.form = static_cast<int>(s->forms.size());
+        s->forms.push_back(f);
+        *tenant = static_cast<int32_t>(s->tenants.size());
+        s->tenants.push_back(t);
+        return YGPU_OK;
+    });
+}
+
 int ygpu_server_add_tenant_shared(ygpu_server* s, int32_t src_tenant, int32_t* tenant) {
     return with_server(s, [&] {
         if (!tenant) return fail(YGPU_ERR_INVALID_ARG, "null argument");
@@ -959,10 +1175,19 @@ int ygpu_server_prefetch(ygpu_server* s, int32_t tenant) {
             if (rc) return rc;
         }
         Tenant& tn = s->tenants[tenant];
-        if (!tn.resident) {
+        while (!tn.resident) {
             const int rc = start_load(s, *s->psets[tn.pset], tenant);
             if (rc == 1) return fail(YGPU_ERR_BUSY, "no evictable key memory for a prefetch");
-            if (rc) return rc;
+            if (rc == 3) return fail(YGPU_ERR_BUSY, "every pinned slot for file-backed keys is in use");
+            if (rc < 0) return rc;
+            if (rc == 2) {
+                // a file-backed form: wait for the reader, then load
+                const int fi = tn.form;
+                s->cv_req.wait(lk, [&] { return !s->forms[fi].reading || s->stop; });
+                if (s->stop) return fail(YGPU_ERR_INVALID_ARG, "server is shutting down");
+                if (!s->forms[fi].present) return fail(s->forms[fi].read_rc ? s->forms[fi].read_rc : YGPU_ERR_CORRUPT, s->forms[fi].read_err);
+                continue;
+            }
             tn.fresh = false;  // no request is waiting for it: an ordinary LRU entry
         }
         s->cv_req.wait(lk, [&] { return tn.ready || !tn.resident; });
@@ -1060,7 +1285,10 @@ int ygpu_server_destroy(ygpu_server* s) {
         std::lock_guard<std::mutex> g(s->mu);
         s->stop = true;
         for (auto& ps : s->psets) ps->cv_work.notify_all();
+        s->cv_read.notify_all();
+        s->cv_req.notify_all();
     }
+    if (s->reader.joinable()) s->reader.join();
     for (auto& ps : s->psets) {
         if (ps->dispatcher.joinable()) ps->dispatcher.join();
         if (ps->completer.joinable()) ps->completer.join();
@@ -1091,9 +1319,15 @@ int ygpu_server_destroy(ygpu_server* s) {
     for (Tenant& t : s->tenants)
         if (t.loaded) hipEventDestroy(t.loaded);
     for (PinnedKey& f : s->forms) {
+        if (!f.path.empty()) continue;  // file-backed: its buffers are its slot's
         if (f.packed) hipHostFree(f.packed);
         if (f.planes) hipHostFree(f.planes);
     }
+    for (auto& ps : s->psets)
+        for (FileSlot& sl : ps->slots) {
+            if (sl.packed) hipHostFree(sl.packed);
+            if (sl.planes) hipHostFree(sl.planes);
+        }
     if (s->d_arena) hipFree(s->d_arena);
     if (s->d_bsk_tab) hipFree(s->d_bsk_tab);
     if (s->d_ksk_tab) hipFree(s->d_ksk_tab);
