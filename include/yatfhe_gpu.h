@@ -356,6 +356,53 @@ int ygpu_key_file_read_succinct(const char* path, const ygpu_params* expect, uin
  * ygpu_key_upload_succinct). */
 int ygpu_key_upload_file(ygpu_ctx* ctx, const char* path, int32_t* key_id);
 
+/* ------------------------------------------------------- lazy-pipe keys
+ * The lazy-pipe ("OURS") key format, BootstrappingKeyMPLazyPipe
+ * (yatfhe/include/yatfhe/bootstrapping.h:250-282), TORUS32, binary keys.  Its
+ * rows have R = params->level gadget levels (param.lApprox, the accumulator's
+ * decomposition); each row is stored as D = digit_levels zero-mean digits
+ * (param.l; the reference's TORUS32 default 2^4 x 8 keeps all 32 bits).  For
+ * every LWE key bit s_{i+1} (i < n-1) and row level lvl the key holds the D
+ * digits of the half RGSW row cPrime[lvl] (bskDecompA / bskDecompB), bskFirst
+ * (a TRLWE of v s_0: the key belongs to one test polynomial v) and the TRLEV
+ * of s^2 with D levels (s2Dft).  Layouts:
+ *   first     int32 [k+1][N]            bskFirst[0]: a[0..k-1], b
+ *   digits_a  int8  [n-1][R][D][k][N]   digit l of cPrime[lvl].a[k] of s_{i+1}
+ *   digits_b  int8  [n-1][R][D][N]      digit l of cPrime[lvl].b
+ *   s2        uint64 residues mod q_ntt [D][k+1][N]
+ * ygpu_keygen_lazy replaces genBootstrappingKeyMPLazyPipe
+ * (bootstrapping.cpp:269-320): the half rows are ygpu_keygen_succinct's for the
+ * same seed (their torus values, cut into digits), the secret keys and KSK
+ * equal ygpu_keygen's.  The bootstrap replaces blindRotateLazyPipeNtt
+ * (blind_rotate.cpp:822-902) followed by the extraction and keyswitch of
+ * functionalBootstrapping: acc = X^{-b} (R(v) + (X^{a_0} - 1) bskFirst), then
+ * for every step with a_i != 0 the stored digits of s_i are rotated by
+ * (X^{a_i} - 1) (narrowed to int8, the gadget's 1 added on the diagonal digit),
+ * scheme-switched through s^2 into RGSW rows, and acc := rows (x) acc.  The
+ * rows are rebuilt for every ciphertext (in windows of steps on the device):
+ * 2 R D forward transforms per step where an MP key's step needs 2 R.
+ * Instantiated for N = 1024, k = 1, bases up to 2^7, R <= D <= 8, D b <= 32
+ * (YGPU_ERR_UNSUPPORTED otherwise; gadget 2^7 x 3 has its own kernel form);
+ * the engine-form entry points reject lazy-pipe key ids and these reject
+ * other keys. */
+int ygpu_keygen_lazy(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t digit_levels,
+                     const int32_t* v, int32_t* lwe_key, int32_t* glwe_key, int32_t* first, int8_t* digits_a, int8_t* digits_b,
+                     uint64_t* s2, int32_t* ksk, int32_t threads);
+int ygpu_key_upload_lazy(ygpu_ctx* ctx, int32_t digit_levels, const int32_t* first, const int8_t* digits_a, const int8_t* digits_b,
+                         const uint64_t* s2, const int32_t* ksk, int32_t* key_id);
+/* One ciphertext, host buffers (acc_out and extract_out optional). */
+int ygpu_pbs_lazy(ygpu_ctx* ctx, int32_t key_id, const int32_t* lwe_in, const void* v, int32_t* lwe_out, void* acc_out,
+                  int32_t* extract_out);
+/* blindRotateLazyPipeNtt itself: abar is the rescaled input (ScaledTlwe: a_0
+ * .. a_{n-1}, b, already mod 2N), acc_out the accumulator. */
+int ygpu_blind_rotate_lazy(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar, const void* v, void* acc_out);
+/* A batch on device buffers, as ygpu_pbs_batch_device. */
+int ygpu_pbs_lazy_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const int32_t* d_lwe_in, const void* d_v,
+                               int32_t v_per_ct, int32_t* d_lwe_out, void* d_acc_out, int32_t* d_extract_out);
+/* Steps per window of derived rows (0: the most that fit a 1 GiB workspace for
+ * the batch); results do not depend on it.  No counterpart in the reference. */
+int ygpu_set_lazy_window(ygpu_ctx* ctx, int32_t steps);
+
 /* ---------------------------------------------------------------- serving
  * Continuous-batching server on one GPU.  Replaces the reference's serving
  * loop (ya_benchmark/blindrotate_serving.cpp:121-353): its shared key cache

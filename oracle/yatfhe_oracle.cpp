@@ -28,6 +28,13 @@
 //                           mask-digit row) within the noise bound; blind rotation
 //                           with the derived key decrypts to X^{sum a s} v
 //   level-parallel PBS      equal to the single-threaded PBS, bit for bit
+//   lazy-pipe rows          schoolbook route vs NTT route, bit for bit; the rows
+//                           of component i rotated by a decrypt to the RGSW of
+//                           1 + (X^a - 1) s_{i+1} (gadget times the message, within
+//                           the digits' approximation); narrowing of rotated
+//                           digits pinned on planted +-128 values
+//   lazy-pipe rotation      schoolbook vs NTT route; decryption identity
+//                           (phase = X^{-b + sum a s} v) over whole PBSs
 #include <cstdint>
 #include <cstring>
 #include <cmath>
@@ -617,6 +624,162 @@ void blind_rotate_level_parallel(const Key& key, const int32_t* abar, int64_t* a
         }
     }
 }
+// ---------------------------------------------------------------- lazy pipe
+// BootstrappingKeyMPLazyPipe (bootstrapping.h:250-282) as genBootstrappingKeyMPLazyPipe
+// (bootstrapping.cpp:269-320) fills it, binary keys.  Components i < n-1 stand
+// for s_{i+1}; the rows have R = bsk.level = param.lApprox = p.level gadget
+// levels (the accumulator's decomposition), the stored digits D = param.l.
+//   first [k+1][N]        bskFirst[0], torus: TRLWE of v * s_0
+//   da [n-1][R][D][k][N]  bskDecompA[decompIndex(i, lvl)][l][k]: digit l of cPrime[lvl].a[k]
+//   db [n-1][R][D][N]     bskDecompB[decompIndex(i, lvl)][l]: digit l of cPrime[lvl].b
+//   s2 [D][k+1][N]        s2Dft (TrlevDft{p, p.l}) as coefficient residues mod q
+// Digits are the reference's Decomp type (torus.h:14-18): int8 at TORUS32, int32
+// at TORUS56, so rotated digits narrow to it.
+struct LazyKey {
+    or_params p;
+    int D;  // digit levels (param.l)
+    Ntt ntt;
+    std::vector<int64_t> first;
+    std::vector<int32_t> da, db;
+    std::vector<uint64_t> s2, s2ntt, g;
+    LazyKey(const or_params& p_, int digit_levels, const int64_t* first_, const int32_t* da_, const int32_t* db_, const uint64_t* s2_)
+        : p(p_), D(digit_levels), ntt(p_.q_ntt, p_.N) {
+        const size_t N = p.N, K = p.k, K1 = p.k + 1, R = p.level, m = static_cast<size_t>(p.n - 1);
+        first.assign(first_, first_ + K1 * N);
+        da.assign(da_, da_ + m * R * D * K * N);
+        db.assign(db_, db_ + m * R * D * N);
+        s2.assign(s2_, s2_ + D * K1 * N);
+        s2ntt = s2;
+        for (size_t o = 0; o < s2ntt.size(); o += N) ntt.forward(&s2ntt[o]);
+        // getNttGadgetRecomper(l) (ntt_hexl.cpp:122-149, over param.l levels): the
+        // transform of the constant polynomial 2^(w - (l+1) b), which is that
+        // constant at every point
+        for (int l = 0; l < D; l++) g.push_back(lift(sext(int64_t{1} << (p.torus_bits - (l + 1) * p.radix_bits), p.torus_bits), p.q_ntt));
+    }
+};
+
+// static_cast<Decomp> of an int (torus.h:14-18).
+inline int64_t narrow_decomp(int64_t x, int w) { return w == 32 ? static_cast<int64_t>(static_cast<int8_t>(x)) : static_cast<int64_t>(static_cast<int32_t>(x)); }
+
+// rotateDecompPolynomialMinusOne (polynomial.cpp:273-282): out = Decomp(X^a in - in).
+void rotate_decomp_minus_one(int w, int N, int a, const int32_t* in, int64_t* out) {
+    int aTrue, isWrap;
+    validate_rotator(aTrue, isWrap, a, N);
+    for (int i = 0; i < N; i++) {
+        const int64_t tmp = ((i < aTrue) ? -static_cast<int64_t>(in[i - aTrue + N]) : static_cast<int64_t>(in[i - aTrue])) * isWrap;
+        out[i] = narrow_decomp(tmp - in[i], w);
+    }
+}
+
+// rotateBskComponent (blind_rotate.cpp:490-507) of component `comp` by a, then
+// switchDecompTrlweToSecretEmbeddingNtt (trgsw.cpp:1069-1102) per level: the
+// rows of the per-ciphertext RGSW [L][k+1 rows][k+1 cols][N].  Row r < k is
+// c[lvl][r] (columns a[0..k-1], b), row k is cPrime[lvl]:
+//   cPrime.b    = sum_l D(b)_l g_l          cPrime.a[k1] = sum_l D(a_k1)_l g_l
+//   c[k1].a[k2] = sum_l D(a_k1)_l s2[l].a[k2] + cPrime.b     c[k1].b = sum_l D(a_k1)_l s2[l].b
+// (mod q), D the rotated digits with the gadget's +1 on the diagonal digit
+// (rotatedB[lvl][lvl] coefficient 0).  The NTT route leaves the rows in this
+// oracle's transform domain; the schoolbook route forms them in coefficient
+// form (negacyclic products by brute force).
+void lazy_rows(const LazyKey& key, int comp, int a, std::vector<uint64_t>& rows, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K = p.k, K1 = p.k + 1, L = p.level, D = key.D, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    rows.assign(static_cast<size_t>(L) * K1 * K1 * N, 0);
+    std::vector<int64_t> d(N);
+    std::vector<uint64_t> t(N), prod(N);
+    auto addto = [&](uint64_t* dst, const uint64_t* src) {
+        for (int j = 0; j < N; j++) dst[j] = (dst[j] + src[j]) % q;
+    };
+    for (int lvl = 0; lvl < L; lvl++) {
+        uint64_t* R = &rows[static_cast<size_t>(lvl) * K1 * K1 * N];
+        auto row = [&](int r, int c) { return R + (static_cast<size_t>(r) * K1 + c) * N; };
+        const size_t cell = static_cast<size_t>(comp) * L + lvl;
+        for (int l = 0; l < D; l++) {
+            // the b digits: cPrime.b
+            rotate_decomp_minus_one(w, N, a, &key.db[(cell * D + l) * N], d.data());
+            if (l == lvl) d[0] = narrow_decomp(d[0] + 1, w);
+            for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
+            if (!schoolbook) key.ntt.forward(t.data());
+            for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], key.g[l], q);
+            addto(row(K, K), prod.data());
+            // the a digits: cPrime.a[k1] and c[k1]
+            for (int k1 = 0; k1 < K; k1++) {
+                rotate_decomp_minus_one(w, N, a, &key.da[((cell * D + l) * K + k1) * N], d.data());
+                for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
+                if (!schoolbook) key.ntt.forward(t.data());
+                for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], key.g[l], q);
+                addto(row(K, k1), prod.data());
+                for (int c = 0; c < K1; c++) {
+                    const size_t so = (static_cast<size_t>(l) * K1 + c) * N;
+                    if (schoolbook) {
+                        schoolbook_modq(q, N, d.data(), &key.s2[so], prod.data());
+                    } else {
+                        for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], key.s2ntt[so + j], q);
+                    }
+                    addto(row(k1, c), prod.data());
+                }
+            }
+        }
+        // adding b: c[k1].a[k2] += cPrime.b
+        for (int k1 = 0; k1 < K; k1++)
+            for (int k2 = 0; k2 < K; k2++) addto(row(k1, k2), row(K, K));
+    }
+}
+
+// externalProductTrgswMPNttInPlace (trgsw.cpp:774-776 -> 712-767) with the
+// derived rows: acc := rows (x) acc, level digits of each accumulator row.
+void lazy_external_product(const LazyKey& key, const std::vector<uint64_t>& rows, int64_t* acc, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K1 = p.k + 1, L = p.level, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    std::vector<int64_t> dig(static_cast<size_t>(K1) * L * N);
+    for (int r = 0; r < K1; r++) digits_row(acc + static_cast<size_t>(r) * N, N, L, p.radix_bits, w, &dig[static_cast<size_t>(r) * L * N]);
+    std::vector<uint64_t> sum(static_cast<size_t>(K1) * N, 0), t(N), prod(N);
+    for (int lvl = 0; lvl < L; lvl++) {
+        for (int r = 0; r < K1; r++) {
+            const int64_t* d = &dig[(static_cast<size_t>(r) * L + lvl) * N];
+            if (!schoolbook) {
+                for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
+                key.ntt.forward(t.data());
+            }
+            for (int c = 0; c < K1; c++) {
+                const uint64_t* kr = &rows[((static_cast<size_t>(lvl) * K1 + r) * K1 + c) * N];
+                if (schoolbook) schoolbook_modq(q, N, d, kr, prod.data());
+                else for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], kr[j], q);
+                uint64_t* s = &sum[static_cast<size_t>(c) * N];
+                for (int j = 0; j < N; j++) s[j] = (s[j] + prod[j]) % q;
+            }
+        }
+    }
+    for (int c = 0; c < K1; c++) {
+        uint64_t* s = &sum[static_cast<size_t>(c) * N];
+        if (!schoolbook) key.ntt.inverse(s);
+        for (int j = 0; j < N; j++) acc[static_cast<size_t>(c) * N + j] = intt_exit(s[j], q, w);
+    }
+}
+
+// blindRotateLazyPipeNtt (blind_rotate.cpp:822-902), binary keys, as one
+// sequential loop (the reference's pipeline only overlaps the next
+// component's rotation and scheme switch with this one's product):
+//   acc = X^{-b} (R(v) + (X^{a_0} - 1) bskFirst)
+//   for i = 1 .. n-1, a_i != 0:  acc := RGSW_i (x) acc, RGSW_i the rows of
+//   component i-1 rotated by a_i.
+// abar: the rescaled input [n+1] (a_0 .. a_{n-1}, b).
+void blind_rotate_lazy(const LazyKey& key, const int32_t* abar, const int64_t* v, int64_t* acc, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K = p.k, K1 = p.k + 1, w = p.torus_bits;
+    std::vector<int64_t> tmp(static_cast<size_t>(K1) * N);
+    for (int r = 0; r < K1; r++) rotate_minus_one(w, N, abar[0], &key.first[static_cast<size_t>(r) * N], &tmp[static_cast<size_t>(r) * N]);
+    for (int j = 0; j < N; j++) tmp[static_cast<size_t>(K) * N + j] = sext(tmp[static_cast<size_t>(K) * N + j] + v[j], w);
+    for (int r = 0; r < K1; r++) rotate(w, N, -abar[p.n], &tmp[static_cast<size_t>(r) * N], acc + static_cast<size_t>(r) * N);
+    std::vector<uint64_t> rows;
+    for (int i = 1; i < p.n; i++) {
+        if (abar[i] == 0) continue;
+        lazy_rows(key, i - 1, abar[i], rows, schoolbook);
+        lazy_external_product(key, rows, acc, schoolbook);
+    }
+}
 
 void pbs_one(const Key& key, const int32_t* ksk, const int32_t* lwe, const int64_t* v,
              int64_t* acc, int32_t* ext, int32_t* out, bool schoolbook) {
@@ -843,6 +1006,55 @@ void or_pbs_level_parallel(void* key, const int32_t* ksk, const int32_t* lwe, co
         keyswitch(p, ksk, ext.data(), out);
         if (seconds) seconds[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+}
+
+// The lazy-pipe key (layouts at LazyKey) and its bootstrapping.
+void* or_lazy_key_new(const or_params* p, int32_t digit_levels, const int64_t* first, const int32_t* da, const int32_t* db,
+                      const uint64_t* s2) {
+    try {
+        return new LazyKey(*p, digit_levels, first, da, db, s2);
+    } catch (...) {
+        return nullptr;
+    }
+}
+void or_lazy_key_free(void* k) { delete static_cast<LazyKey*>(k); }
+// The per-ciphertext rows of component `comp` rotated by a, in coefficient
+// form [L][k+1][k+1][N] (the NTT route's rows inverse-transformed).
+void or_lazy_rows(void* key, int32_t comp, int32_t a, uint64_t* out, int32_t schoolbook) {
+    const LazyKey& k = *static_cast<LazyKey*>(key);
+    std::vector<uint64_t> rows;
+    lazy_rows(k, comp, a, rows, schoolbook != 0);
+    if (!schoolbook)
+        for (size_t o = 0; o < rows.size(); o += k.p.N) k.ntt.inverse(&rows[o]);
+    std::memcpy(out, rows.data(), sizeof(uint64_t) * rows.size());
+}
+void or_blind_rotate_lazy(void* key, const int32_t* abar, const int64_t* v, int64_t* acc, int32_t schoolbook) {
+    blind_rotate_lazy(*static_cast<LazyKey*>(key), abar, v, acc, schoolbook != 0);
+}
+// blindRotateLazyPipeNtt, extract and keyswitch of each ciphertext ([count][n+1]),
+// threaded over ciphertexts; acc_out, ext_out optional.
+void or_pbs_lazy_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v, int64_t* acc_out,
+                       int32_t* ext_out, int32_t* out, int32_t threads) {
+    const LazyKey& k = *static_cast<LazyKey*>(key);
+    const or_params& p = k.p;
+    const size_t accN = static_cast<size_t>(p.k + 1) * p.N, extN = static_cast<size_t>(p.k) * p.N + 1;
+    std::atomic<int32_t> next{0};
+    auto work = [&] {
+        std::vector<int32_t> scaled(p.n + 1), ext(extN);
+        std::vector<int64_t> acc(accN);
+        for (int32_t c; (c = next.fetch_add(1)) < count;) {
+            rescale(p, lwe + static_cast<size_t>(c) * (p.n + 1), scaled.data());
+            blind_rotate_lazy(k, scaled.data(), v, acc.data(), false);
+            extract(p, acc.data(), ext.data());
+            if (acc_out) std::memcpy(acc_out + c * accN, acc.data(), sizeof(int64_t) * accN);
+            if (ext_out) std::memcpy(ext_out + c * extN, ext.data(), sizeof(int32_t) * extN);
+            if (out) keyswitch(p, ksk, ext.data(), out + static_cast<size_t>(c) * (p.n + 1));
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < std::max(1, threads); t++) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
 }
 
 void or_pbs_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v,

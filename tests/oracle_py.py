@@ -69,6 +69,11 @@ def lib():
             "or_derive_succinct": (None, [P(_OrParams), i32, vp, vp, vp, i32]),
             "or_pbs_level_parallel": (None, [vp, vp, vp, vp, vp, i32, vp]),
             "or_derive_succinct_row": (None, [P(_OrParams), i32, vp, vp, vp, i32]),
+            "or_lazy_key_new": (vp, [P(_OrParams), i32, vp, vp, vp, vp]),
+            "or_lazy_key_free": (None, [vp]),
+            "or_lazy_rows": (None, [vp, i32, i32, vp, i32]),
+            "or_blind_rotate_lazy": (None, [vp, vp, vp, vp, i32]),
+            "or_pbs_lazy_batch": (None, [vp, vp, i32, vp, vp, vp, vp, vp, i32]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -204,6 +209,63 @@ class Oracle:
         f = self.L.or_keyswitch_u32 if u32 else self.L.or_keyswitch
         f(ctypes.byref(self.cp), P(self.ksk), P(np.ascontiguousarray(ext, dtype=np.int32)), P(out))
         return out
+
+
+class LazyOracle:
+    """The oracle's blindRotateLazyPipeNtt (+ extract and keyswitch) for one lazy-pipe key set."""
+
+    def __init__(self, params, keys, extract_index: int = 0):
+        self.L = lib()
+        self.p = params
+        self.cp = or_params(params, extract_index)
+        self.first = np.ascontiguousarray(np.asarray(keys.first).astype(np.int64))
+        self.da = np.ascontiguousarray(np.asarray(keys.digits_a).astype(np.int32))
+        self.db = np.ascontiguousarray(np.asarray(keys.digits_b).astype(np.int32))
+        self.s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+        self.ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        self.digit_levels = int(np.asarray(keys.digits_b).shape[2])
+        self.key = self.L.or_lazy_key_new(ctypes.byref(self.cp), self.digit_levels, P(self.first), P(self.da), P(self.db),
+                                          P(self.s2))
+        if not self.key:
+            raise RuntimeError("oracle lazy-pipe key creation failed")
+
+    def close(self):
+        if self.key:
+            self.L.or_lazy_key_free(self.key)
+            self.key = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rows(self, comp, a, schoolbook=False):
+        """Coefficient-form rows [l][k+1][k+1][N] of component comp (s_{comp+1}) rotated by a."""
+        p = self.p
+        out = np.zeros((p.level, p.k + 1, p.k + 1, p.N), np.uint64)
+        self.L.or_lazy_rows(self.key, int(comp), int(a), P(out), 1 if schoolbook else 0)
+        return out
+
+    def blind_rotate_scaled(self, abar, v, schoolbook=False):
+        p = self.p
+        acc = np.zeros((p.k + 1) * p.N, np.int64)
+        self.L.or_blind_rotate_lazy(self.key, P(np.ascontiguousarray(abar, dtype=np.int32)),
+                                    P(np.ascontiguousarray(np.asarray(v).astype(np.int64))), P(acc), 1 if schoolbook else 0)
+        return acc.reshape(p.k + 1, p.N).astype(p.torus_dtype)
+
+    def pbs_batch_all(self, lwes, v, threads=None):
+        """(accumulators, extracted LWEs, keyswitched LWEs) of many ciphertexts."""
+        p = self.p
+        lwes = np.ascontiguousarray(lwes, dtype=np.int32)
+        c = lwes.shape[0]
+        out = np.zeros((c, p.n + 1), np.int32)
+        acc = np.zeros((c, p.k + 1, p.N), np.int64)
+        ext = np.zeros((c, p.k * p.N + 1), np.int32)
+        t = threads or min(16, os.cpu_count() or 1)
+        self.L.or_pbs_lazy_batch(self.key, P(self.ksk), c, P(lwes), P(np.ascontiguousarray(np.asarray(v).astype(np.int64))),
+                                 P(acc), P(ext), P(out), t)
+        return acc.astype(p.torus_dtype), ext, out
 
 
 def glwe_phase(params, acc, glwe_key):

@@ -216,6 +216,12 @@ def load_library(path: str | None = None):
         "ygpu_keygen": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_classic": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_keygen_succinct": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
+        "ygpu_keygen_lazy": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
+        "ygpu_key_upload_lazy": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, P(c_i32)]),
+        "ygpu_pbs_lazy": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp]),
+        "ygpu_blind_rotate_lazy": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_pbs_lazy_batch_device": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
+        "ygpu_set_lazy_window": (c_int, [c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
         "ygpu_lwe_encrypt_tuniform": (c_int, [c_i32, c_vp, c_u64, c_i32, c_i32, c_vp, c_vp]),
         "ygpu_keygen_tuniform": (c_int, [P(_Params), c_u64, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
@@ -271,6 +277,8 @@ def exported_symbols():
         "ygpu_abi_struct_sizes", "ygpu_set_barrier_polls", "ygpu_pbs_batch_device_noalloc",
         "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index", "ygpu_server_lanes",
         "ygpu_server_add_tenant_file", "ygpu_server_set_file_slots",
+        "ygpu_keygen_lazy", "ygpu_key_upload_lazy", "ygpu_pbs_lazy", "ygpu_blind_rotate_lazy", "ygpu_pbs_lazy_batch_device",
+        "ygpu_set_lazy_window",
         "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform", "ygpu_get_tile_ex", "ygpu_set_tile_ex", "ygpu_autotune_results",
     ]
 
@@ -376,6 +384,51 @@ def keygen_succinct(params: Params, seed: int, s2_level: int | None = None, sigm
     _check(lib.ygpu_keygen_succinct(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe, s2_level,
                                     _ptr(lk), _ptr(gk), _ptr(half), _ptr(s2), _ptr(ksk), threads))
     return SuccinctKeySet(params, lk, gk, half, s2, ksk)
+
+
+@dataclass
+class LazyKeySet:
+    """Lazy-pipe key (BootstrappingKeyMPLazyPipe): digits of the half RGSW rows,
+    bskFirst (a TRLWE of v s_0, so the key belongs to the test polynomial v) and
+    the TRLEV of s^2."""
+    params: Params
+    lwe_key: np.ndarray   # int32[n]
+    glwe_key: np.ndarray  # int32[kN]
+    first: np.ndarray     # int32[k+1][N]
+    digits_a: np.ndarray  # int8[n-1][R][D][k][N]  (R = params.level, D = digit levels)
+    digits_b: np.ndarray  # int8[n-1][R][D][N]
+    s2: np.ndarray        # uint64[D][k+1][N], coefficient residues mod q
+    ksk: np.ndarray       # int32[kN][t][n+1]
+    v: np.ndarray         # the test polynomial bskFirst encrypts
+
+    @property
+    def digit_levels(self) -> int:
+        return int(self.digits_b.shape[2])
+
+
+def keygen_lazy(params: Params, seed: int, v: np.ndarray, digit_levels: int | None = None, sigma_log2_lwe: float | None = None,
+                sigma_log2_glwe: float | None = None, threads: int = 0) -> LazyKeySet:
+    """Seeded lazy-pipe keys (ygpu_keygen_lazy) for test polynomial v, rows of
+    params.level levels stored as digit_levels digits (default params.level);
+    secret keys and KSK equal keygen's for the seed."""
+    lib = load_library()
+    dl, dg = default_sigmas(params)
+    sigma_log2_lwe = dl if sigma_log2_lwe is None else sigma_log2_lwe
+    sigma_log2_glwe = dg if sigma_log2_glwe is None else sigma_log2_glwe
+    R, K, N = params.level, params.k, params.N
+    L = params.level if digit_levels is None else digit_levels
+    vv = np.ascontiguousarray(v, dtype=np.int32)
+    lk = np.zeros(params.n, np.int32)
+    gk = np.zeros(K * N, np.int32)
+    first = np.zeros((K + 1, N), np.int32)
+    da = np.zeros((params.n - 1, R, L, K, N), np.int8)
+    db = np.zeros((params.n - 1, R, L, N), np.int8)
+    s2 = np.zeros((L, K + 1, N), np.uint64)
+    ksk = np.zeros(params.ksk_shape, np.int32)
+    cp = params.c()
+    _check(lib.ygpu_keygen_lazy(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe, L, _ptr(vv), _ptr(lk), _ptr(gk),
+                                _ptr(first), _ptr(da), _ptr(db), _ptr(s2), _ptr(ksk), threads))
+    return LazyKeySet(params, lk, gk, first, da, db, s2, ksk, vv.copy())
 
 
 def keygen_classic(params: Params, seed: int, sigma_log2_lwe: float | None = None,
@@ -611,6 +664,69 @@ class Context:
         ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
         _check(self.lib.ygpu_key_upload_succinct(self.h, _ptr(half), _ptr(s2), keys.s2_level, _ptr(ksk), ctypes.byref(kid)))
         return kid.value
+
+    def upload_key_lazy(self, keys: LazyKeySet) -> int:
+        kid = ctypes.c_int32(-1)
+        arrs = [np.ascontiguousarray(keys.first, dtype=np.int32), np.ascontiguousarray(keys.digits_a, dtype=np.int8),
+                np.ascontiguousarray(keys.digits_b, dtype=np.int8), np.ascontiguousarray(keys.s2, dtype=np.uint64),
+                np.ascontiguousarray(keys.ksk, dtype=np.int32)]
+        _check(self.lib.ygpu_key_upload_lazy(self.h, keys.digit_levels, *[_ptr(a) for a in arrs], ctypes.byref(kid)))
+        return kid.value
+
+    def pbs_lazy(self, key_id: int, lwe_in: np.ndarray, v: np.ndarray, want_acc=False, want_extract=False):
+        p = self.params
+        lwe = np.ascontiguousarray(lwe_in, dtype=np.int32)
+        vv = np.ascontiguousarray(v, dtype=p.torus_dtype)
+        out = np.zeros(p.n + 1, np.int32)
+        acc = np.zeros((p.k + 1, p.N), p.torus_dtype) if want_acc else None
+        ext = np.zeros(p.k * p.N + 1, np.int32) if want_extract else None
+        _check(self.lib.ygpu_pbs_lazy(self.h, key_id, _ptr(lwe), _ptr(vv), _ptr(out),
+                                      _ptr(acc) if acc is not None else None, _ptr(ext) if ext is not None else None))
+        return out, acc, ext
+
+    def blind_rotate_lazy(self, key_id: int, abar: np.ndarray, v: np.ndarray):
+        """blindRotateLazyPipeNtt on a rescaled input (a_0 .. a_{n-1}, b mod 2N)."""
+        p = self.params
+        ab = np.ascontiguousarray(abar, dtype=np.int32)
+        vv = np.ascontiguousarray(v, dtype=p.torus_dtype)
+        acc = np.zeros((p.k + 1, p.N), p.torus_dtype)
+        _check(self.lib.ygpu_blind_rotate_lazy(self.h, key_id, _ptr(ab), _ptr(vv), _ptr(acc)))
+        return acc
+
+    def pbs_lazy_batch_device(self, key_id: int, count: int, d_lwe, d_v, v_per_ct: bool, d_out, d_acc=None, d_ext=None):
+        _check(self.lib.ygpu_pbs_lazy_batch_device(self.h, key_id, count, d_lwe.value, d_v.value, 1 if v_per_ct else 0,
+                                                   d_out.value, d_acc.value if d_acc else None,
+                                                   d_ext.value if d_ext else None))
+
+    def set_lazy_window(self, steps: int):
+        _check(self.lib.ygpu_set_lazy_window(self.h, steps))
+
+    def pbs_lazy_batch(self, key_id: int, lwe_in: np.ndarray, v: np.ndarray, want_acc=False, want_extract=False):
+        """Host-buffer convenience over ygpu_pbs_lazy_batch_device."""
+        p = self.params
+        lwe = np.ascontiguousarray(lwe_in, dtype=np.int32)
+        count = lwe.shape[0]
+        vv = np.ascontiguousarray(v, dtype=p.torus_dtype)
+        tb = np.dtype(p.torus_dtype).itemsize
+        bufs = []
+        try:
+            d_lwe = self.alloc(lwe.nbytes); bufs.append(d_lwe)
+            d_v = self.alloc(vv.nbytes); bufs.append(d_v)
+            d_out = self.alloc(count * (p.n + 1) * 4); bufs.append(d_out)
+            d_acc = self.alloc(count * (p.k + 1) * p.N * tb) if want_acc else None
+            d_ext = self.alloc(count * (p.k * p.N + 1) * 4) if want_extract else None
+            bufs += [b for b in (d_acc, d_ext) if b]
+            d_lwe.upload(lwe)
+            d_v.upload(vv)
+            self.pbs_lazy_batch_device(key_id, count, d_lwe, d_v, vv.ndim == 2, d_out, d_acc, d_ext)
+            self.sync()
+            out = d_out.download(np.zeros((count, p.n + 1), np.int32))
+            acc = d_acc.download(np.zeros((count, p.k + 1, p.N), p.torus_dtype)) if d_acc else None
+            ext = d_ext.download(np.zeros((count, p.k * p.N + 1), np.int32)) if d_ext else None
+            return out, acc, ext
+        finally:
+            for b in bufs:
+                b.free()
 
     def upload_key_file(self, path: str) -> int:
         kid = ctypes.c_int32()
@@ -1033,7 +1149,7 @@ class Server:
 __all__ = [
     "Params", "PARAMS_T32_N1024", "PARAMS_T56_N2048", "Q_49P_T32", "Q_61P_T56", "KeySet", "keygen",
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
-    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request",
+    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request", "LazyKeySet", "keygen_lazy",
     "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
     "key_file_s2_level", "load_succinct_arrays", "keygen_classic", "SCHED_MODES", "abi_struct_mismatches",
     "keygen_tuniform", "lwe_encrypt_tuniform", "KEY_FORMS",

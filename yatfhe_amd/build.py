@@ -45,7 +45,8 @@ def _run(cmd, verbose):
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    for src in ("ygpu_kernels.hip", "ygpu_kernels_crt.hip", "ygpu_kernels_lat.hip", "ygpu_keyexpand.hip", "ygpu_runtime.hip", "ygpu_server.hip", "ygpu_keyfile.hip"):
+    for src in ("ygpu_kernels.hip", "ygpu_kernels_crt.hip", "ygpu_kernels_lat.hip", "ygpu_kernels_lazy.hip", "ygpu_keyexpand.hip",
+                "ygpu_runtime.hip", "ygpu_lazy.hip", "ygpu_server.hip", "ygpu_keyfile.hip"):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
         objs.append(obj)

@@ -39,8 +39,11 @@ inline int fail(int code, const std::string& msg) {
 
 struct KeyRec {
     bool live = false;
-    double* d_bsk = nullptr;
+    double* d_bsk = nullptr;   // engine-form BSK (null for a lazy-pipe key)
     int32_t* d_ksk = nullptr;
+    void* d_lazy = nullptr;    // a lazy-pipe key's device block (lazy_layout)
+    bool lazy = false;
+    int lazy_digits = 0;       // its stored digit levels (param.l)
     size_t bytes = 0;
     int users = 0;  // synchronous calls running on a plain lane with this key
 };
@@ -153,6 +156,17 @@ struct ygpu_ctx {
     void* d_accin1 = nullptr;  // caller's initial accumulator (ygpu_blind_rotate_scaled)
     void* d_stage = nullptr;   // packed key staging for uploads
     size_t stage_cap = 0;
+    // lazy-pipe runs: derived-row workspace, rotation amounts, accumulators,
+    // per-ciphertext row tables; and the window (steps per derive launch, 0: auto)
+    void* d_lazy_ws = nullptr;
+    size_t lazy_ws_cap = 0;
+    void* d_lazy_av = nullptr;
+    size_t lazy_av_cap = 0;
+    void* d_lazy_acc = nullptr;
+    size_t lazy_acc_cap = 0;
+    void* d_lazy_tab = nullptr;
+    size_t lazy_tab_cap = 0;
+    int lazy_window = 0;
     std::mutex mu;
     std::condition_variable cv;   // a plain lane or a key's last user finished
     std::vector<std::unique_ptr<PlainLane>> lanes;
@@ -308,6 +322,21 @@ inline int key_live(const ygpu_ctx* ctx, int32_t key_id) {
         return fail(YGPU_ERR_NOT_FOUND, "unknown key id " + std::to_string(key_id));
     return YGPU_OK;
 }
+// A live key in the engine's BSK form (every entry point but the lazy-pipe ones).
+inline int engine_key_live(const ygpu_ctx* ctx, int32_t key_id) {
+    const int rc = key_live(ctx, key_id);
+    if (rc) return rc;
+    if (ctx->keys[key_id].lazy)
+        return fail(YGPU_ERR_UNSUPPORTED, "key " + std::to_string(key_id) + " is a lazy-pipe key: use the ygpu_*_lazy entry points");
+    return YGPU_OK;
+}
+// A live lazy-pipe key.
+inline int lazy_key_live(const ygpu_ctx* ctx, int32_t key_id) {
+    const int rc = key_live(ctx, key_id);
+    if (rc) return rc;
+    if (!ctx->keys[key_id].lazy) return fail(YGPU_ERR_UNSUPPORTED, "key " + std::to_string(key_id) + " is not a lazy-pipe key");
+    return YGPU_OK;
+}
 
 // Device position of transform output index k for lane tl, register r.
 inline int last_pass_index(int logn, int tl, int r) {
@@ -417,7 +446,7 @@ inline BrTile batch_tile(const ygpu_ctx* ctx, int count) {
 inline int job_slots(const ygpu_ctx* ctx, const ygpu_job* jobs, int n_jobs, int64_t total, std::vector<int32_t>& slots) {
     slots.assign(static_cast<size_t>(total), -1);
     for (int j = 0; j < n_jobs; j++) {
-        int rc = key_live(ctx, jobs[j].key_id);
+        int rc = engine_key_live(ctx, jobs[j].key_id);
         if (rc) return rc;
         if (jobs[j].count < 0 || jobs[j].first < 0 || jobs[j].first + jobs[j].count > total)
             return fail(YGPU_ERR_INVALID_ARG, "job " + std::to_string(j) + " out of range");
@@ -685,6 +714,8 @@ inline hipError_t expand_key(const ygpu_ctx* ctx, int s2_level, void* d_stage, d
     a.crt = ctx->crt;
     return launch_bsk_derive(ctx->logn, ctx->tbits, a, s);
 }
+
+int register_key(ygpu_ctx* ctx, const KeyRec& rec, int32_t* key_id);
 
 // A context issuing its work on `stream`, which the caller owns and keeps
 // alive until ygpu_destroy; nullptr creates a stream of the context's own

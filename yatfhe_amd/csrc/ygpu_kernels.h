@@ -17,9 +17,12 @@ namespace ygpu {
 //   L, RB  gadget levels and base log2 when fixed at compile time (0: runtime)
 //   TWL    twiddles in LDS (1), global memory through L1 (0), or automatic (-1)
 //   TAB    forward stages taken from digit tables (0, 1 or 2; fixed 8-bit shapes)
-template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0, int TWL_ = -1, int TAB_ = 0>
+template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0, int TWL_ = -1, int TAB_ = 0, bool LAZY_ = false>
 struct BrCfg {
     static constexpr int TAB = TAB_;
+    // the lazy-pipe step: digits of the accumulator itself (no rotation), the
+    // accumulator replaced by the product, steps with abar = 0 left unchanged
+    static constexpr bool LAZY = LAZY_;
     static constexpr int L = L_;
     static constexpr int RB = RB_;
     static constexpr int TWL = TWL_;
@@ -71,7 +74,39 @@ struct BrArgs {
     // fails at once without waiting, which exercises the report path).
     uint32_t* status;
     int barrier_polls;
+    int lwe_stride;  // ints between ciphertexts of lwe_in (0: n + 1)
 };
+
+// The lazy-pipe key's per-ciphertext work (k_lazy_prepare, k_lazy_derive).
+struct LazyArgs {
+    int count, n;
+    int level;               // R: gadget levels of the rows (param.lApprox)
+    int dlevel;              // D: stored digit levels (param.l), R <= D <= 8
+    int prescaled;           // lwe_in already holds abar (mod 2N) values
+    const int32_t* lwe_in;   // [count][n+1]
+    const int32_t* v;        // Torus[N] (v_stride 0) or Torus[count][N]
+    int v_stride;
+    const int32_t* first;    // bskFirst[0] [k+1][N]
+    int32_t* acc;            // [count][k+1][N]
+    int32_t* avals;          // [count][n-1]: the rotation amounts of steps 1 .. n-1
+    const int8_t* da;        // [n-1][R][D][k][N]
+    const int8_t* db;        // [n-1][R][D][N]
+    const double* s2e;       // s^2 TRLEV in the engine form [D][k+1][N]
+    double g[8];             // 2^(32 - (l+1) b) N^-1 mod p, centred, l < D
+    double* ws;              // rows [count][ws_steps][R][k+1][k+1][N]
+    int s0, steps, ws_steps; // the window: components s0 .. s0 + steps - 1
+    const double** row_tab;  // [count]: each ciphertext's rows in ws (k_lazy_prepare fills it)
+    int32_t* row_slot;       // [count]: c -> c
+    const double* tw;        // [2][N]
+    PrimeF64 prime;
+};
+bool lazy_supported(int tbits, int logn, int k);
+hipError_t launch_lazy_prepare(const LazyArgs& a, hipStream_t s);
+hipError_t launch_lazy_derive(const LazyArgs& a, hipStream_t s);
+// The batched blind rotation in its lazy form (no rotation, the accumulator
+// replaced by the product, steps with a rotation amount of 0 skipped), TORUS32
+// N = 1024: gadget 2^7 x 3 specialised, other gadgets up to 2^7 and 8 levels.
+hipError_t launch_blind_rotate_lazy(const BrArgs& a, hipStream_t s);
 
 constexpr int KS_THREADS = 256;
 constexpr int KS_TC = 64;    // ciphertexts per keyswitch tile

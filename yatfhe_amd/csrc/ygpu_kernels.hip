@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     const int rb = a.radix_bits;
     const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
     const double* __restrict__ bsk = a.bsk_table[slot];
-    const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
+    const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (a.lwe_stride > 0 ? a.lwe_stride : n + 1);
     const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
 
     if constexpr (C::TW_LDS) {
@@ -120,6 +120,17 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
         // (X^a - 1) acc_r at this lane's pass-0 positions (rotateTorusPolynomialMinusOne,
         // polynomial.cpp:213-238) and its digits, packed 8 or 16 bits per digit.
         uint32_t dg0[16], dg1[16];
+        if constexpr (C::LAZY) {
+            // the lazy-pipe step decomposes the accumulator itself
+            // (externalProductTrgswMPNttInPlace, trgsw.cpp:774-776)
+            static_assert(C::TBITS == 32, "lazy form: TORUS32");
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                if constexpr (C::L > 0) digits_packed<32, C::L, C::RB>(static_cast<int32_t>(accr[r]), dg0[r], dg1[r]);
+                else digits_runtime_packed<32>(static_cast<int32_t>(accr[r]), L, rb, dg0[r], dg1[r]);
+            }
+        }
+        if constexpr (!C::LAZY) {
         team_sync<TW>();
 #pragma unroll
         for (int r = 0; r < 16; r++) stage[tl + T * r] = accr[r];
@@ -149,6 +160,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     digits_runtime_packed<C::TBITS>(rot, L, rb, dg0[r], dg1[r]);
                 }
             }
+        }
         }
 
         const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
@@ -258,11 +270,15 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 
             ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
             if constexpr (NP == 1) {
-                // centre mod q, reduce to the torus, accumulate (mod 2^w)
+                // centre mod q, reduce to the torus, accumulate (mod 2^w); the
+                // lazy-pipe step replaces the accumulator, and leaves it where
+                // abar = 0 (blind_rotate.cpp:893-895)
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const double c = centre(x[r], pr);
-                    if constexpr (C::TBITS == 32) {
+                    if constexpr (C::LAZY) {
+                        accr[r] = abar == 0 ? accr[r] : static_cast<TorusT>(f64_low32(c));
+                    } else if constexpr (C::TBITS == 32) {
                         accr[r] = static_cast<TorusT>(static_cast<uint32_t>(accr[r]) + f64_low32(c));
                     } else {
                         accr[r] = static_cast<TorusT>(sext_t<C::TBITS>(static_cast<int64_t>(accr[r]) + f64_to_i64(c)));
@@ -623,6 +639,13 @@ hipError_t launch_blind_rotate(int tbits, int logn, const BrTile& t, int radix_b
     }
     if (tbits == 32 && logn == 11) return p8 ? launch_br<BrCfg<32, 11, 1, 1, true>>(a, s) : launch_br<BrCfg<32, 11, 1, 1, false>>(a, s);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_blind_rotate_lazy(const BrArgs& a, hipStream_t s) {
+    if (a.n < 1 || a.level < 1 || a.level > 8 || a.radix_bits < 1 || a.radix_bits > 7 || !a.acc_in || !a.ct_slot)
+        return hipErrorInvalidValue;
+    if (a.level == 3 && a.radix_bits == 7) return launch_br<BrCfg<32, 10, 1, 2, true, 3, 7, 1, 2, true>>(a, s);
+    return launch_br<BrCfg<32, 10, 1, 2, true, 0, 0, 1, 0, true>>(a, s);
 }
 
 int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* out, int max) {

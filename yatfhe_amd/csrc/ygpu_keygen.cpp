@@ -338,6 +338,90 @@ int lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, const Noise& n
     return YGPU_OK;
 }
 
+// zeroMeanDigits (gadget_decomposition.h:75-95): the l balanced digits of the
+// low w bits of x in base 2^b, rounding bias for the dropped bits, ties at
+// B/2 resolved by the bit above, sign applied last.  out[level], level 0 the
+// most significant.
+void zero_mean_digits(int64_t x, int l, int b, int w, int64_t* out) {
+    const uint64_t mask = w >= 64 ? ~0ULL : ((uint64_t{1} << w) - 1);
+    const uint64_t u = static_cast<uint64_t>(x) & mask;
+    const bool negative = (u >> (w - 1)) & 1;
+    const int dropped = w - l * b;
+    const uint64_t m = (negative ? (0ULL - u) & mask : u) + (dropped > 0 ? (uint64_t{1} << (dropped - 1)) : 0);
+    const int64_t B = int64_t{1} << b, halfB = B >> 1;
+    int64_t carry = 0;
+    for (int level = l - 1; level >= 0; --level) {
+        const int shift = w - (level + 1) * b;
+        const int64_t digit = static_cast<int64_t>((m >> shift) & static_cast<uint64_t>(B - 1)) + carry;
+        const int64_t tie = level > 0 ? static_cast<int64_t>((m >> (shift + 2 * b - 1)) & 1) : 0;
+        carry = (digit > halfB) | ((digit == halfB) & tie);
+        out[level] = (negative ? -1 : 1) * (carry ? digit - B : digit);
+    }
+}
+
+constexpr uint64_t S_FIRST = 7ULL << 40;
+
+// genBootstrappingKeyMPLazyPipe (bootstrapping.cpp:269-320), binary keys: rows
+// of R = bsk.level = lApprox = p.level gadget levels, each cut into D = param.l
+// digits.  The half RGSW rows cPrime[lvl] of s_1 .. s_{n-1} are those of the
+// succinct key for the same seed (encryptTrgswMP with isHalf, trgsw.cpp:18-34,
+// whose torus rows are symEncTrlweMultiSampleNtt's applyIntt of the mod-q
+// body: centred, low w bits); each torus coefficient is cut into D zero-mean
+// digits (signedGadgetDecomposition, gadget_decomposition.h:103-106) stored as
+// the reference's TORUS32 Decomp, int8.  bskFirst encrypts v when s_0 = 1
+// (symEncTrlweMultiSample) and 0 otherwise (symEncTrlweSingleSample), in torus
+// arithmetic; s2 is the TRLEV of s^2 with D levels (s2Dft, TrlevDft{p, p.l}).
+int keygen_lazy(const ygpu_params* params, uint64_t seed, const Noise& nl, const Noise& ng, int32_t digit_levels, const int32_t* v,
+                int32_t* lwe_key, int32_t* glwe_key, int32_t* first, int8_t* digits_a, int8_t* digits_b, uint64_t* s2, int32_t* ksk,
+                int32_t threads) {
+    if (!params || !v || !lwe_key || !glwe_key || !first || !digits_a || !digits_b || !s2 || !ksk) return YGPU_ERR_INVALID_ARG;
+    const ygpu_params p = *params;
+    if (p.torus_bits != 32) return YGPU_ERR_UNSUPPORTED;  // digits are int8 (TORUS32's Decomp)
+    if (p.n < 2 || p.k < 1 || p.level < 1 || p.radix_bits > 7) return YGPU_ERR_INVALID_ARG;
+    const int D = digit_levels;
+    if (D < 1 || D > 8 || D * p.radix_bits > p.torus_bits) return YGPU_ERR_INVALID_ARG;
+    const int N = p.N, K = p.k, K1 = p.k + 1, L = p.level, n = p.n, w = p.torus_bits;
+    const uint64_t q = p.q_ntt, qhalf = (q - 1) / 2;
+    try {
+        std::vector<uint64_t> half(static_cast<size_t>(n) * L * K1 * N);
+        int rc = keygen_succinct(params, seed, nl, ng, D, lwe_key, glwe_key, half.data(), s2, ksk, threads);
+        if (rc) return rc;
+        const int nt = worker_threads(threads);
+        par_for((n - 1) * L, nt, [&](int idx) {
+            const int i = idx / L + 1, lvl = idx % L;  // component idx / L holds s_i
+            const uint64_t* row = half.data() + (static_cast<size_t>(i) * L + lvl) * K1 * N;
+            std::vector<int64_t> d(D);
+            for (int c = 0; c < K1; c++) {
+                for (int j = 0; j < N; j++) {
+                    const uint64_t r = row[static_cast<size_t>(c) * N + j];
+                    const int64_t t = sext(r > qhalf ? static_cast<int64_t>(r) - static_cast<int64_t>(q) : static_cast<int64_t>(r), w);
+                    zero_mean_digits(t, D, p.radix_bits, w, d.data());
+                    for (int l = 0; l < D; l++) {
+                        const size_t cell = static_cast<size_t>(idx) * D + l;
+                        if (c < K) digits_a[(cell * K + c) * N + j] = static_cast<int8_t>(d[l]);
+                        else digits_b[cell * N + j] = static_cast<int8_t>(d[l]);
+                    }
+                }
+            }
+        });
+        // bskFirst[0]: mask uniform over the torus, body e + mu + a s in the torus
+        Rng r(seed, S_FIRST);
+        std::vector<int64_t> a(static_cast<size_t>(K) * N), b(N);
+        for (int j = 0; j < N; j++) b[j] = ng.draw(r) + (lwe_key[0] ? static_cast<int64_t>(v[j]) : 0);
+        for (int c = 0; c < K; c++) {
+            int64_t* ac = &a[static_cast<size_t>(c) * N];
+            for (int j = 0; j < N; j++) ac[j] = sext(static_cast<int64_t>(r.next()), w);
+            torus_mul_binary_acc(N, w, ac, glwe_key + static_cast<size_t>(c) * N, b.data());
+        }
+        for (int c = 0; c < K; c++)
+            for (int j = 0; j < N; j++) first[static_cast<size_t>(c) * N + j] = static_cast<int32_t>(a[static_cast<size_t>(c) * N + j]);
+        for (int j = 0; j < N; j++) first[static_cast<size_t>(K) * N + j] = static_cast<int32_t>(sext(b[j], w));
+    } catch (...) {
+        return YGPU_ERR_INVALID_ARG;
+    }
+    return YGPU_OK;
+}
+
 bool tuniform_ok(const ygpu_params* params, int lwe_b, int glwe_b) {
     return params && lwe_b >= 0 && lwe_b < 31 && glwe_b >= 0 && glwe_b < params->torus_bits - 1;
 }
@@ -379,6 +463,14 @@ int ygpu_keygen_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_b
         case YGPU_KEY_SUCCINCT: return keygen_succinct(params, seed, nl, ng, s2_level, lwe_key, glwe_key, bsk_out, s2_out, ksk, threads);
         default: return YGPU_ERR_INVALID_ARG;
     }
+}
+
+int ygpu_keygen_lazy(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t digit_levels,
+                     const int32_t* v, int32_t* lwe_key, int32_t* glwe_key, int32_t* first, int8_t* digits_a, int8_t* digits_b,
+                     uint64_t* s2, int32_t* ksk, int32_t threads) {
+    if (!params) return YGPU_ERR_INVALID_ARG;
+    return keygen_lazy(params, seed, Noise::gaussian_log2(sigma_log2_lwe, 32), Noise::gaussian_log2(sigma_log2_glwe, params->torus_bits),
+                       digit_levels, v, lwe_key, glwe_key, first, digits_a, digits_b, s2, ksk, threads);
 }
 
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count, const int32_t* mu,

@@ -189,15 +189,17 @@ int ygpu_destroy(ygpu_ctx* ctx) {
         for (auto& l : ctx->lanes) free_lane(*l);
         ctx->lanes.clear();
         if (ctx->stream) hipStreamSynchronize(ctx->stream);
-        for (auto& k : ctx->keys) {
+            for (auto& k : ctx->keys) {
             if (k.d_bsk) hipFree(k.d_bsk);
             if (k.d_ksk) hipFree(k.d_ksk);
+            if (k.d_lazy) hipFree(k.d_lazy);
         }
         for (auto& g : ctx->graphs) if (g.live) free_graph(g);
         for (void* p : {static_cast<void*>(ctx->d_tw), static_cast<void*>(ctx->d_bsk_table), static_cast<void*>(ctx->d_ksk_table),
                         ctx->d_acc, static_cast<void*>(ctx->d_partial), static_cast<void*>(ctx->d_slots),
                         static_cast<void*>(ctx->d_lwe1), ctx->d_v1, static_cast<void*>(ctx->d_out1),
-                        static_cast<void*>(ctx->d_ext1), ctx->d_acc1, ctx->d_accin1, ctx->d_stage})
+                        static_cast<void*>(ctx->d_ext1), ctx->d_acc1, ctx->d_accin1, ctx->d_stage, ctx->d_lazy_ws,
+                        ctx->d_lazy_av, ctx->d_lazy_acc, ctx->d_lazy_tab})
             if (p) hipFree(p);
         if (ctx->h_status) hipHostFree(ctx->h_status);
         for (auto& t : ctx->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
@@ -213,6 +215,44 @@ int ygpu_get_params(const ygpu_ctx* ctx, ygpu_params* out) {
     *out = ctx->p;
     return YGPU_OK;
 }
+
+}  // extern "C"
+
+namespace ygpu_rt {
+
+// Gives an uploaded key a slot of the fixed-capacity key tables; frees its
+// device memory on failure.
+int register_key(ygpu_ctx* ctx, const KeyRec& rec, int32_t* key_id) {
+    auto drop = [&](const KeyRec& k) {
+        if (k.d_bsk) hipFree(k.d_bsk);
+        if (k.d_ksk) hipFree(k.d_ksk);
+        if (k.d_lazy) hipFree(k.d_lazy);
+    };
+    int id = -1;
+    for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
+    if (id < 0) {
+        if (static_cast<int>(ctx->keys.size()) >= KEY_SLOTS) {
+            drop(rec);
+            return fail(YGPU_ERR_UNSUPPORTED, "a context holds at most " + std::to_string(KEY_SLOTS) + " keys");
+        }
+        id = static_cast<int>(ctx->keys.size());
+        ctx->keys.push_back(rec);
+    } else {
+        ctx->keys[id] = rec;
+    }
+    const int rc = set_table_slot(ctx, id);
+    if (rc) {
+        drop(ctx->keys[id]);
+        ctx->keys[id] = KeyRec{};
+        return rc;
+    }
+    *key_id = id;
+    return YGPU_OK;
+}
+
+}  // namespace ygpu_rt
+
+extern "C" {
 
 namespace {
 
@@ -244,28 +284,7 @@ int upload_host_form(ygpu_ctx* ctx, const HostKeyForm& hk, int32_t* key_id) {
         hipFree(rec.d_ksk);
         return fail(YGPU_ERR_HIP, std::string("key upload: ") + hipGetErrorString(e));
     }
-    int id = -1;
-    for (size_t i = 0; i < ctx->keys.size(); i++) if (!ctx->keys[i].live) { id = static_cast<int>(i); break; }
-    if (id < 0) {
-        if (static_cast<int>(ctx->keys.size()) >= KEY_SLOTS) {
-            hipFree(rec.d_bsk);
-            hipFree(rec.d_ksk);
-            return fail(YGPU_ERR_UNSUPPORTED, "a context holds at most " + std::to_string(KEY_SLOTS) + " keys");
-        }
-        id = static_cast<int>(ctx->keys.size());
-        ctx->keys.push_back(rec);
-    } else {
-        ctx->keys[id] = rec;
-    }
-    rc = set_table_slot(ctx, id);
-    if (rc) {
-        hipFree(ctx->keys[id].d_bsk);
-        hipFree(ctx->keys[id].d_ksk);
-        ctx->keys[id] = KeyRec{};
-        return rc;
-    }
-    *key_id = id;
-    return YGPU_OK;
+    return register_key(ctx, rec, key_id);
 }
 
 }  // namespace
@@ -307,8 +326,9 @@ int ygpu_key_release(ygpu_ctx* ctx, int32_t key_id) {
     if (rc) return rc;
     YG_HIP(hipStreamSynchronize(ctx->stream));
     KeyRec& k = ctx->keys[key_id];
-    hipFree(k.d_bsk);
-    hipFree(k.d_ksk);
+    if (k.d_bsk) hipFree(k.d_bsk);
+    if (k.d_ksk) hipFree(k.d_ksk);
+    if (k.d_lazy) hipFree(k.d_lazy);
     k = KeyRec{};
     return set_table_slot(ctx, key_id);
 }
@@ -327,7 +347,7 @@ int ygpu_pbs_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const in
     if (count == 0) return YGPU_OK;
     if (!d_lwe_in || !d_v || !d_lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null buffer");
     return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
+        int rc = engine_key_live(ctx, key_id);
         if (rc) return rc;
         void* acc = d_acc_out;
         if (!acc) {
@@ -348,7 +368,7 @@ int ygpu_pbs_batch_device_noalloc(ygpu_ctx* ctx, int32_t key_id, int32_t count, 
     if (count == 0) return YGPU_OK;
     if (!d_lwe_in || !d_v || !d_lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null buffer");
     return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
+        int rc = engine_key_live(ctx, key_id);
         if (rc) return rc;
         if (!ctx->d_static_scratch) return fail(YGPU_ERR_HIP, "the static scratch array has no device address on this device");
         size_t accBytes = 0;
@@ -410,7 +430,7 @@ int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in
                           void* d_acc_out) {
     if (!d_lwe_in || !d_v || !d_lwe_out) return fail(YGPU_ERR_INVALID_ARG, "null buffer");
     return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
+        int rc = engine_key_live(ctx, key_id);
         if (rc) return rc;
         rc = plain_buffers(ctx);
         if (rc) return rc;
@@ -465,7 +485,7 @@ int with_lane(ygpu_ctx* ctx, int32_t key_id, F&& f) {
         hipError_t e = hipSetDevice(ctx->device);
         if (e != hipSuccess) return fail(YGPU_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
         if (key_id >= 0) {
-            int rc = key_live(ctx, key_id);
+            int rc = engine_key_live(ctx, key_id);
             if (rc) return rc;
         }
         for (auto& l : ctx->lanes)
@@ -819,7 +839,7 @@ int ygpu_autotune_results(ygpu_ctx* ctx, int32_t max, int32_t* cpw, int32_t* twl
 int ygpu_autotune(ygpu_ctx* ctx, int32_t key_id, int32_t count, int32_t* cpw, int32_t* twl, double* best_ms) {
     if (count < 1) return fail(YGPU_ERR_INVALID_ARG, "count must be positive");
     return with_ctx(ctx, [&]() -> int {
-        int rc = key_live(ctx, key_id);
+        int rc = engine_key_live(ctx, key_id);
         if (rc) return rc;
         const ygpu_params& p = ctx->p;
         BrTile cand[MAX_TILES];
