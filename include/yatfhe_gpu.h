@@ -399,6 +399,25 @@ int ygpu_blind_rotate_lazy(ygpu_ctx* ctx, int32_t key_id, const int32_t* abar, c
 /* A batch on device buffers, as ygpu_pbs_batch_device. */
 int ygpu_pbs_lazy_batch_device(ygpu_ctx* ctx, int32_t key_id, int32_t count, const int32_t* d_lwe_in, const void* d_v,
                                int32_t v_per_ct, int32_t* d_lwe_out, void* d_acc_out, int32_t* d_extract_out);
+/* The Alt form, BootstrappingKeyMPLazyPipeAlt (bootstrapping.h:284-306): the
+ * lazy-pipe format the reference's serving benchmark runs as "OURS"
+ * (blindrotate_serving.cpp:480-488).  It stores the half RGSW rows of every s_i
+ * in torus form, rows int32 [n][R][k+1][N] (bskPrime[i].cPrime[lvl]), and the
+ * TRLEV of s^2 with D = digit_levels levels; it does not depend on v.  The
+ * bootstrap replaces blindRotateLazyPipeAltNtt (blind_rotate.cpp:996-1136):
+ * the first component GD(v) (x) RLEV(s_0) (deriveFirstComponentNtt with
+ * prepareGdV, trgsw.cpp:911-955, R digits of v) is formed once per test
+ * polynomial; every step rotates the torus rows by (X^{a_i} - 1), adds the
+ * gadget to the body (rotateTrlweMinusOneBPlusOne), cuts the mask into D
+ * digits and scheme-switches them through s^2 (the body is transformed whole),
+ * D + 1 forward transforms per row level.  The lazy-pipe entry points above
+ * run Alt keys too.  ygpu_keygen_lazy_alt replaces
+ * genBootstrappingKeyMPLazyPipeAlt (bootstrapping.cpp:322-339): the rows are
+ * ygpu_keygen_succinct's for the same seed in torus form. */
+int ygpu_keygen_lazy_alt(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t digit_levels,
+                         int32_t* lwe_key, int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk, int32_t threads);
+int ygpu_key_upload_lazy_alt(ygpu_ctx* ctx, int32_t digit_levels, const int32_t* rows, const uint64_t* s2, const int32_t* ksk,
+                             int32_t* key_id);
 /* Steps per window of derived rows (0: the most that fit a 1 GiB workspace for
  * the batch); results do not depend on it.  No counterpart in the reference. */
 int ygpu_set_lazy_window(ygpu_ctx* ctx, int32_t steps);

@@ -35,6 +35,9 @@
 //                           digits pinned on planted +-128 values
 //   lazy-pipe rotation      schoolbook vs NTT route; decryption identity
 //                           (phase = X^{-b + sum a s} v) over whole PBSs
+//   lazy-pipe Alt           rows and first component: schoolbook vs NTT route;
+//                           rows decrypt to the RGSW of 1 + (X^a - 1) s_i, the
+//                           first component to v s_0; whole PBSs decrypt
 #include <cstdint>
 #include <cstring>
 #include <cmath>
@@ -729,8 +732,7 @@ void lazy_rows(const LazyKey& key, int comp, int a, std::vector<uint64_t>& rows,
 
 // externalProductTrgswMPNttInPlace (trgsw.cpp:774-776 -> 712-767) with the
 // derived rows: acc := rows (x) acc, level digits of each accumulator row.
-void lazy_external_product(const LazyKey& key, const std::vector<uint64_t>& rows, int64_t* acc, bool schoolbook) {
-    const or_params& p = key.p;
+void lazy_external_product(const or_params& p, const Ntt& ntt, const std::vector<uint64_t>& rows, int64_t* acc, bool schoolbook) {
     const int N = p.N, K1 = p.k + 1, L = p.level, w = p.torus_bits;
     const uint64_t q = p.q_ntt;
     std::vector<int64_t> dig(static_cast<size_t>(K1) * L * N);
@@ -741,7 +743,7 @@ void lazy_external_product(const LazyKey& key, const std::vector<uint64_t>& rows
             const int64_t* d = &dig[(static_cast<size_t>(r) * L + lvl) * N];
             if (!schoolbook) {
                 for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
-                key.ntt.forward(t.data());
+                ntt.forward(t.data());
             }
             for (int c = 0; c < K1; c++) {
                 const uint64_t* kr = &rows[((static_cast<size_t>(lvl) * K1 + r) * K1 + c) * N];
@@ -754,7 +756,7 @@ void lazy_external_product(const LazyKey& key, const std::vector<uint64_t>& rows
     }
     for (int c = 0; c < K1; c++) {
         uint64_t* s = &sum[static_cast<size_t>(c) * N];
-        if (!schoolbook) key.ntt.inverse(s);
+        if (!schoolbook) ntt.inverse(s);
         for (int j = 0; j < N; j++) acc[static_cast<size_t>(c) * N + j] = intt_exit(s[j], q, w);
     }
 }
@@ -777,7 +779,132 @@ void blind_rotate_lazy(const LazyKey& key, const int32_t* abar, const int64_t* v
     for (int i = 1; i < p.n; i++) {
         if (abar[i] == 0) continue;
         lazy_rows(key, i - 1, abar[i], rows, schoolbook);
-        lazy_external_product(key, rows, acc, schoolbook);
+        lazy_external_product(p, key.ntt, rows, acc, schoolbook);
+    }
+}
+
+// ---------------------------------------------------------------- lazy pipe, Alt
+// BootstrappingKeyMPLazyPipeAlt (bootstrapping.h:284-306) as
+// genBootstrappingKeyMPLazyPipeAlt (bootstrapping.cpp:322-339) fills it: for
+// every s_i (i < n) the half RGSW rows cPrime[lvl] in torus form (TrgswMP,
+// lvl < R = lApprox), and the TRLEV of s^2 with D = param.l levels.
+//   rows [n][R][k+1][N]   torus: bskPrime[i].cPrime[lvl] (a[0..k-1], b)
+//   s2 [D][k+1][N]        s2Dft as coefficient residues mod q
+struct AltKey {
+    or_params p;
+    int D;
+    Ntt ntt;
+    std::vector<int64_t> rows;
+    std::vector<uint64_t> s2, s2ntt, g;
+    AltKey(const or_params& p_, int digit_levels, const int64_t* rows_, const uint64_t* s2_)
+        : p(p_), D(digit_levels), ntt(p_.q_ntt, p_.N) {
+        const size_t N = p.N, K1 = p.k + 1, R = p.level;
+        rows.assign(rows_, rows_ + static_cast<size_t>(p.n) * R * K1 * N);
+        s2.assign(s2_, s2_ + D * K1 * N);
+        s2ntt = s2;
+        for (size_t o = 0; o < s2ntt.size(); o += N) ntt.forward(&s2ntt[o]);
+        for (int l = 0; l < D; l++) g.push_back(lift(sext(int64_t{1} << (p.torus_bits - (l + 1) * p.radix_bits), p.torus_bits), p.q_ntt));
+    }
+};
+
+// The per-ciphertext rows of component i rotated by a (blindRotateLazyPipeAltNttSweep's
+// stageA and stageB, blind_rotate.cpp:1035-1087): per level, rotateTrlweMinusOneBPlusOne
+// (trlwe.cpp:291-297) of cPrime[lvl] with one = 2^(w - (lvl+1) b); the rotated
+// mask decomposed into D digits (gadgetDecomposeTrlweA, trlwe.cpp:200-210);
+//   cPrime.a[k1] = sum_l D_l g_l,  c[k1].a[k2] = sum_l D_l s2[l].a[k2] + cPrime.b,
+//   c[k1].b = sum_l D_l s2[l].b,  cPrime.b = the rotated body itself (mod q).
+// Layout and routes as lazy_rows.
+void alt_rows(const AltKey& key, int comp, int a, std::vector<uint64_t>& rows, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K = p.k, K1 = p.k + 1, R = p.level, D = key.D, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    rows.assign(static_cast<size_t>(R) * K1 * K1 * N, 0);
+    std::vector<int64_t> rot(N), dig(static_cast<size_t>(D) * N);
+    std::vector<uint64_t> t(N), prod(N);
+    auto addto = [&](uint64_t* dst, const uint64_t* src) {
+        for (int j = 0; j < N; j++) dst[j] = (dst[j] + src[j]) % q;
+    };
+    for (int lvl = 0; lvl < R; lvl++) {
+        uint64_t* Rw = &rows[static_cast<size_t>(lvl) * K1 * K1 * N];
+        auto row = [&](int r, int c) { return Rw + (static_cast<size_t>(r) * K1 + c) * N; };
+        const int64_t* cp = &key.rows[((static_cast<size_t>(comp) * R + lvl) * K1) * N];
+        // the rotated body plus the gadget: cPrime.b
+        rotate_minus_one(w, N, a, cp + static_cast<size_t>(K) * N, rot.data());
+        rot[0] = sext(rot[0] + (int64_t{1} << (w - (lvl + 1) * p.radix_bits)), w);
+        for (int j = 0; j < N; j++) t[j] = lift(rot[j], q);
+        if (!schoolbook) key.ntt.forward(t.data());
+        std::memcpy(row(K, K), t.data(), sizeof(uint64_t) * N);
+        for (int k1 = 0; k1 < K; k1++) {
+            rotate_minus_one(w, N, a, cp + static_cast<size_t>(k1) * N, rot.data());
+            digits_row(rot.data(), N, D, p.radix_bits, w, dig.data());
+            for (int l = 0; l < D; l++) {
+                const int64_t* d = &dig[static_cast<size_t>(l) * N];
+                for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
+                if (!schoolbook) key.ntt.forward(t.data());
+                for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], key.g[l], q);
+                addto(row(K, k1), prod.data());
+                for (int c = 0; c < K1; c++) {
+                    const size_t so = (static_cast<size_t>(l) * K1 + c) * N;
+                    if (schoolbook) schoolbook_modq(q, N, d, &key.s2[so], prod.data());
+                    else for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], key.s2ntt[so + j], q);
+                    addto(row(k1, c), prod.data());
+                }
+            }
+        }
+        for (int k1 = 0; k1 < K; k1++)
+            for (int k2 = 0; k2 < K; k2++) addto(row(k1, k2), row(K, K));
+    }
+}
+
+// deriveFirstComponentNtt (trgsw.cpp:929-955) with prepareGdV (trgsw.cpp:911-927):
+// out = sum_lvl GD(v)_lvl * cPrime_0[lvl] (mod q, R = lApprox digits of v),
+// the TRLWE of v s_0.
+void alt_first(const AltKey& key, const int64_t* v, int64_t* out, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K1 = p.k + 1, R = p.level, w = p.torus_bits;
+    const uint64_t q = p.q_ntt;
+    std::vector<int64_t> gd(static_cast<size_t>(R) * N);
+    digits_row(v, N, R, p.radix_bits, w, gd.data());
+    std::vector<uint64_t> acc(static_cast<size_t>(K1) * N, 0), t(N), r(N), prod(N);
+    for (int lvl = 0; lvl < R; lvl++) {
+        const int64_t* d = &gd[static_cast<size_t>(lvl) * N];
+        for (int j = 0; j < N; j++) t[j] = lift(d[j], q);
+        if (!schoolbook) key.ntt.forward(t.data());
+        for (int c = 0; c < K1; c++) {
+            const int64_t* cp = &key.rows[(static_cast<size_t>(lvl) * K1 + c) * N];  // component 0
+            for (int j = 0; j < N; j++) r[j] = lift(cp[j], q);
+            if (schoolbook) {
+                schoolbook_modq(q, N, d, r.data(), prod.data());
+            } else {
+                key.ntt.forward(r.data());
+                for (int j = 0; j < N; j++) prod[j] = mulmod(t[j], r[j], q);
+            }
+            for (int j = 0; j < N; j++) acc[static_cast<size_t>(c) * N + j] = (acc[static_cast<size_t>(c) * N + j] + prod[j]) % q;
+        }
+    }
+    for (int c = 0; c < K1; c++) {
+        uint64_t* a = &acc[static_cast<size_t>(c) * N];
+        if (!schoolbook) key.ntt.inverse(a);
+        for (int j = 0; j < N; j++) out[static_cast<size_t>(c) * N + j] = intt_exit(a[j], q, w);
+    }
+}
+
+// blindRotateLazyPipeAltNtt (blind_rotate.cpp:996-1000 -> the sweep, 1002-1136)
+// as one sequential loop: acc = X^{-b} (R(v) + (X^{a_0} - 1) derived), then for
+// i = 1 .. n-1 with a_i != 0, acc := rows_i (x) acc.
+void blind_rotate_alt(const AltKey& key, const int32_t* abar, const int64_t* v, int64_t* acc, bool schoolbook) {
+    const or_params& p = key.p;
+    const int N = p.N, K = p.k, K1 = p.k + 1, w = p.torus_bits;
+    std::vector<int64_t> first(static_cast<size_t>(K1) * N), tmp(static_cast<size_t>(K1) * N);
+    alt_first(key, v, first.data(), schoolbook);
+    for (int r = 0; r < K1; r++) rotate_minus_one(w, N, abar[0], &first[static_cast<size_t>(r) * N], &tmp[static_cast<size_t>(r) * N]);
+    for (int j = 0; j < N; j++) tmp[static_cast<size_t>(K) * N + j] = sext(tmp[static_cast<size_t>(K) * N + j] + v[j], w);
+    for (int r = 0; r < K1; r++) rotate(w, N, -abar[p.n], &tmp[static_cast<size_t>(r) * N], acc + static_cast<size_t>(r) * N);
+    std::vector<uint64_t> rows;
+    for (int i = 1; i < p.n; i++) {
+        if (abar[i] == 0) continue;
+        alt_rows(key, i, abar[i], rows, schoolbook);
+        lazy_external_product(p, key.ntt, rows, acc, schoolbook);
     }
 }
 
@@ -1045,6 +1172,53 @@ void or_pbs_lazy_batch(void* key, const int32_t* ksk, int32_t count, const int32
         for (int32_t c; (c = next.fetch_add(1)) < count;) {
             rescale(p, lwe + static_cast<size_t>(c) * (p.n + 1), scaled.data());
             blind_rotate_lazy(k, scaled.data(), v, acc.data(), false);
+            extract(p, acc.data(), ext.data());
+            if (acc_out) std::memcpy(acc_out + c * accN, acc.data(), sizeof(int64_t) * accN);
+            if (ext_out) std::memcpy(ext_out + c * extN, ext.data(), sizeof(int32_t) * extN);
+            if (out) keyswitch(p, ksk, ext.data(), out + static_cast<size_t>(c) * (p.n + 1));
+        }
+    };
+    std::vector<std::thread> ts;
+    for (int t = 1; t < std::max(1, threads); t++) ts.emplace_back(work);
+    work();
+    for (auto& t : ts) t.join();
+}
+
+// The Alt lazy-pipe key (layouts at AltKey) and its bootstrapping.
+void* or_alt_key_new(const or_params* p, int32_t digit_levels, const int64_t* rows, const uint64_t* s2) {
+    try {
+        return new AltKey(*p, digit_levels, rows, s2);
+    } catch (...) {
+        return nullptr;
+    }
+}
+void or_alt_key_free(void* k) { delete static_cast<AltKey*>(k); }
+void or_alt_rows(void* key, int32_t comp, int32_t a, uint64_t* out, int32_t schoolbook) {
+    const AltKey& k = *static_cast<AltKey*>(key);
+    std::vector<uint64_t> rows;
+    alt_rows(k, comp, a, rows, schoolbook != 0);
+    if (!schoolbook)
+        for (size_t o = 0; o < rows.size(); o += k.p.N) k.ntt.inverse(&rows[o]);
+    std::memcpy(out, rows.data(), sizeof(uint64_t) * rows.size());
+}
+void or_alt_first(void* key, const int64_t* v, int64_t* out, int32_t schoolbook) {
+    alt_first(*static_cast<AltKey*>(key), v, out, schoolbook != 0);
+}
+void or_blind_rotate_alt(void* key, const int32_t* abar, const int64_t* v, int64_t* acc, int32_t schoolbook) {
+    blind_rotate_alt(*static_cast<AltKey*>(key), abar, v, acc, schoolbook != 0);
+}
+void or_pbs_alt_batch(void* key, const int32_t* ksk, int32_t count, const int32_t* lwe, const int64_t* v, int64_t* acc_out,
+                      int32_t* ext_out, int32_t* out, int32_t threads) {
+    const AltKey& k = *static_cast<AltKey*>(key);
+    const or_params& p = k.p;
+    const size_t accN = static_cast<size_t>(p.k + 1) * p.N, extN = static_cast<size_t>(p.k) * p.N + 1;
+    std::atomic<int32_t> next{0};
+    auto work = [&] {
+        std::vector<int32_t> scaled(p.n + 1), ext(extN);
+        std::vector<int64_t> acc(accN);
+        for (int32_t c; (c = next.fetch_add(1)) < count;) {
+            rescale(p, lwe + static_cast<size_t>(c) * (p.n + 1), scaled.data());
+            blind_rotate_alt(k, scaled.data(), v, acc.data(), false);
             extract(p, acc.data(), ext.data());
             if (acc_out) std::memcpy(acc_out + c * accN, acc.data(), sizeof(int64_t) * accN);
             if (ext_out) std::memcpy(ext_out + c * extN, ext.data(), sizeof(int32_t) * extN);

@@ -74,6 +74,12 @@ def lib():
             "or_lazy_rows": (None, [vp, i32, i32, vp, i32]),
             "or_blind_rotate_lazy": (None, [vp, vp, vp, vp, i32]),
             "or_pbs_lazy_batch": (None, [vp, vp, i32, vp, vp, vp, vp, vp, i32]),
+            "or_alt_key_new": (vp, [P(_OrParams), i32, vp, vp]),
+            "or_alt_key_free": (None, [vp]),
+            "or_alt_rows": (None, [vp, i32, i32, vp, i32]),
+            "or_alt_first": (None, [vp, vp, vp, i32]),
+            "or_blind_rotate_alt": (None, [vp, vp, vp, vp, i32]),
+            "or_pbs_alt_batch": (None, [vp, vp, i32, vp, vp, vp, vp, vp, i32]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -265,6 +271,64 @@ class LazyOracle:
         t = threads or min(16, os.cpu_count() or 1)
         self.L.or_pbs_lazy_batch(self.key, P(self.ksk), c, P(lwes), P(np.ascontiguousarray(np.asarray(v).astype(np.int64))),
                                  P(acc), P(ext), P(out), t)
+        return acc.astype(p.torus_dtype), ext, out
+
+
+class AltOracle:
+    """The oracle's blindRotateLazyPipeAltNtt (+ extract and keyswitch) for one Alt key set."""
+
+    def __init__(self, params, keys, extract_index: int = 0):
+        self.L = lib()
+        self.p = params
+        self.cp = or_params(params, extract_index)
+        self.rows_t = np.ascontiguousarray(np.asarray(keys.rows).astype(np.int64))
+        self.s2 = np.ascontiguousarray(keys.s2, dtype=np.uint64)
+        self.ksk = np.ascontiguousarray(keys.ksk, dtype=np.int32)
+        self.key = self.L.or_alt_key_new(ctypes.byref(self.cp), int(self.s2.shape[0]), P(self.rows_t), P(self.s2))
+        if not self.key:
+            raise RuntimeError("oracle Alt key creation failed")
+
+    def close(self):
+        if self.key:
+            self.L.or_alt_key_free(self.key)
+            self.key = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rows(self, comp, a, schoolbook=False):
+        """Coefficient-form rows [R][k+1][k+1][N] of s_comp rotated by a."""
+        p = self.p
+        out = np.zeros((p.level, p.k + 1, p.k + 1, p.N), np.uint64)
+        self.L.or_alt_rows(self.key, int(comp), int(a), P(out), 1 if schoolbook else 0)
+        return out
+
+    def first(self, v, schoolbook=False):
+        p = self.p
+        out = np.zeros((p.k + 1) * p.N, np.int64)
+        self.L.or_alt_first(self.key, P(np.ascontiguousarray(np.asarray(v).astype(np.int64))), P(out), 1 if schoolbook else 0)
+        return out.reshape(p.k + 1, p.N)
+
+    def blind_rotate_scaled(self, abar, v, schoolbook=False):
+        p = self.p
+        acc = np.zeros((p.k + 1) * p.N, np.int64)
+        self.L.or_blind_rotate_alt(self.key, P(np.ascontiguousarray(abar, dtype=np.int32)),
+                                   P(np.ascontiguousarray(np.asarray(v).astype(np.int64))), P(acc), 1 if schoolbook else 0)
+        return acc.reshape(p.k + 1, p.N).astype(p.torus_dtype)
+
+    def pbs_batch_all(self, lwes, v, threads=None):
+        p = self.p
+        lwes = np.ascontiguousarray(lwes, dtype=np.int32)
+        c = lwes.shape[0]
+        out = np.zeros((c, p.n + 1), np.int32)
+        acc = np.zeros((c, p.k + 1, p.N), np.int64)
+        ext = np.zeros((c, p.k * p.N + 1), np.int32)
+        t = threads or min(16, os.cpu_count() or 1)
+        self.L.or_pbs_alt_batch(self.key, P(self.ksk), c, P(lwes), P(np.ascontiguousarray(np.asarray(v).astype(np.int64))),
+                                P(acc), P(ext), P(out), t)
         return acc.astype(p.torus_dtype), ext, out
 
 

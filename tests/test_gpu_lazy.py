@@ -114,3 +114,62 @@ def test_lazy_and_engine_keys_stay_apart():
         ctx.release_key(lazy_id)
         with pytest.raises(yg.YgpuError):
             ctx.pbs_lazy_batch(lazy_id, cts, lk.v)
+
+
+# ------------------------------------------------------------------ Alt form
+from tests.oracle_py import AltOracle  # noqa: E402
+
+
+@pytest.mark.parametrize("name", list(SHAPES))
+def test_lazy_alt_batch_bit_exact(name):
+    """BootstrappingKeyMPLazyPipeAlt (the serving benchmark's OURS) against the
+    oracle's blindRotateLazyPipeAltNtt, windows 0/1/5, the plain and scaled
+    forms, and a test polynomial per ciphertext (one first component each)."""
+    p, digits, sigma = SHAPES[name]
+    ak = yg.keygen_lazy_alt(p, seed=71, digit_levels=digits, sigma_log2_glwe=sigma)
+    v = yg.sign_test_polynomial(p)
+    bits, cts = inputs(p, ak, 24, 72)
+    orc = AltOracle(p, ak)
+    ref_acc, ref_ext, ref_out = orc.pbs_batch_all(cts, v)
+    with yg.Context(p, 0) as ctx:
+        kid = ctx.upload_key_lazy(ak)
+        for window in (0, 1, 5):
+            print(f"lazy alt {name}: window {window}", flush=True)
+            ctx.set_lazy_window(window)
+            out, acc, ext = ctx.pbs_lazy_batch(kid, cts, v, want_acc=True, want_extract=True)
+            assert np.array_equal(acc, ref_acc), f"{name} window {window}: accumulators differ"
+            assert np.array_equal(ext, ref_ext) and np.array_equal(out, ref_out), f"{name} window {window}"
+        o1, a1, _ = ctx.pbs_lazy(kid, cts[7], v, want_acc=True)
+        assert np.array_equal(o1, ref_out[7]) and np.array_equal(a1, ref_acc[7])
+        sc = np.zeros(p.n + 1, np.int32)
+        lib().or_rescale(ctypes.byref(or_params(p)), P(np.ascontiguousarray(cts[9])), P(sc))
+        assert np.array_equal(ctx.blind_rotate_lazy(kid, sc, v), orc.blind_rotate_scaled(sc, v))
+        # a different test polynomial per ciphertext
+        rng = np.random.default_rng(73)
+        vs = np.stack([yg.lut_test_polynomial(p, rng.integers(0, 4, 4), 4) for _ in range(6)])
+        out6, acc6, _ = ctx.pbs_lazy_batch(kid, cts[:6], vs, want_acc=True)
+        for c in range(6):
+            ra, _, ro = orc.pbs_batch_all(cts[c:c + 1], vs[c])
+            assert np.array_equal(acc6[c], ra[0]) and np.array_equal(out6[c], ro[0]), c
+    orc.close()
+    if name == "b4x4_d8":
+        ph = yg.lwe_decrypt_phase(out, ak.lwe_key)
+        assert np.array_equal((ph[5:] > 0).astype(int), bits[5:])
+
+
+def test_lazy_alt_full_dimension():
+    p = dataclasses.replace(yg.PARAMS_T32_N1024, radix_bits=4, level=4)
+    ak = yg.keygen_lazy_alt(p, seed=74, digit_levels=8, sigma_log2_glwe=2.21 - 32)
+    v = yg.sign_test_polynomial(p)
+    rng = np.random.default_rng(75)
+    bits = rng.integers(0, 2, 96)
+    cts = yg.lwe_encrypt(ak.lwe_key, ((2 * bits - 1) * (1 << 29)).astype(np.int32), seed=76)
+    print("lazy alt n=630: batch of 96", flush=True)
+    with yg.Context(p, 0) as ctx:
+        kid = ctx.upload_key_lazy(ak)
+        out, acc, _ = ctx.pbs_lazy_batch(kid, cts, v, want_acc=True)
+    assert np.array_equal((yg.lwe_decrypt_phase(out, ak.lwe_key) > 0).astype(int), bits)
+    orc = AltOracle(p, ak)
+    ref_acc, _, ref_out = orc.pbs_batch_all(cts[[0, 50, 95]], v)
+    orc.close()
+    assert np.array_equal(ref_out, out[[0, 50, 95]]) and np.array_equal(ref_acc, acc[[0, 50, 95]])

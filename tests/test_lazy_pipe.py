@@ -269,3 +269,101 @@ def test_lazy_pbs_phase_and_decrypt(exact_keys):
     ph = yg.lwe_decrypt_phase(out, lk.lwe_key)
     assert np.array_equal((ph > 0).astype(int), bits)
     orc.close()
+
+
+# ------------------------------------------------------------------ Alt form
+# BootstrappingKeyMPLazyPipeAlt / blindRotateLazyPipeAltNtt (blind_rotate.cpp:996-1136),
+# the format the reference's serving benchmark runs as OURS.
+from tests.oracle_py import AltOracle  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def alt_small():
+    return yg.keygen_lazy_alt(SMALL, seed=41)
+
+
+@pytest.fixture(scope="module")
+def alt_exact():
+    return EXACT, yg.keygen_lazy_alt(EXACT, seed=42, digit_levels=EXACT_DIGITS, sigma_log2_glwe=SIGMA_GLWE_EXACT)
+
+
+def test_alt_keygen_rows_are_succinct_rows_in_torus_form(alt_small):
+    sk = yg.keygen_succinct(SMALL, seed=41, s2_level=SMALL.level)
+    ak = alt_small
+    assert np.array_equal(ak.s2, sk.s2) and np.array_equal(ak.ksk, sk.ksk) and np.array_equal(ak.lwe_key, sk.lwe_key)
+    q = SMALL.q_ntt
+    torus = np.array([wrap(x) for x in centred(sk.bsk_half, q).reshape(-1)], np.int64).reshape(ak.rows.shape)
+    assert np.array_equal(torus, ak.rows.astype(np.int64))
+
+
+@pytest.mark.parametrize("a", [0, 1, 63, 64, 100])
+def test_alt_rows_and_first_schoolbook_equals_ntt(alt_small, a):
+    orc = AltOracle(SMALL, alt_small)
+    for comp in (1, SMALL.n - 1):
+        assert np.array_equal(orc.rows(comp, a), orc.rows(comp, a, schoolbook=True)), (comp, a)
+    v = yg.sign_test_polynomial(SMALL)
+    assert np.array_equal(orc.first(v), orc.first(v, schoolbook=True))
+    orc.close()
+
+
+@pytest.mark.parametrize("a", [0, 5, 1000])
+def test_alt_rows_decrypt_exact_digits(alt_exact, a):
+    """Row k of level lvl decrypts to m g_lvl and row 0 to -z m g_lvl, for
+    m = 1 + (X^a - 1) s_i; the body is exact (transformed whole) and the mask's
+    digits keep all 32 bits, so every level is checked within g_lvl / 4."""
+    p, ak = alt_exact
+    orc = AltOracle(p, ak)
+    q, N = p.q_ntt, p.N
+    z = ak.glwe_key.astype(np.int64)
+    for comp in (1, 30, p.n - 1):
+        s = int(ak.lwe_key[comp])
+        m = negacyclic_rotate(np.eye(1, N, 0, np.int64)[0], a) if s else np.eye(1, N, 0, np.int64)[0]
+        rows = orc.rows(comp, a)
+        for lvl in range(p.level):
+            g = 1 << (32 - (lvl + 1) * p.radix_bits)
+            ph = glwe_phase_modq(q, rows[lvl, 1], ak.glwe_key)
+            assert max(abs(wrap(int(x) - int(y) * g)) for x, y in zip(ph, m)) < g // 4, (comp, lvl, "body row")
+            ph0 = glwe_phase_modq(q, rows[lvl, 0], ak.glwe_key)
+            zm = negacyclic_mul(m, z)
+            assert max(abs(wrap(int(x) + int(y) * g)) for x, y in zip(ph0, zm)) < g // 4, (comp, lvl, "mask row")
+    orc.close()
+
+
+def test_alt_first_component_and_pbs_decrypt(alt_exact):
+    """deriveFirstComponentNtt gives a TRLWE of v s_0; whole Alt bootstraps
+    decrypt (phase X^{-b + sum a_i s_i} v, and the keyswitched sign)."""
+    p, ak = alt_exact
+    orc = AltOracle(p, ak)
+    N = p.N
+    v = yg.sign_test_polynomial(p)
+    first = orc.first(v)
+    ph = glwe_phase(p, first, ak.glwe_key)
+    want = v.astype(np.int64) * int(ak.lwe_key[0])
+    assert max(abs(wrap(a - b)) for a, b in zip(ph, want)) < 1 << 20
+    rng = np.random.default_rng(5)
+    bits = rng.integers(0, 2, 8)
+    cts = yg.lwe_encrypt(ak.lwe_key, ((2 * bits - 1) * (1 << 29)).astype(np.int32), seed=9)
+    acc, _, out = orc.pbs_batch_all(cts, v)
+    from tests.oracle_py import or_params
+    for c in range(len(cts)):
+        sc = np.zeros(p.n + 1, np.int32)
+        lib().or_rescale(ctypes.byref(or_params(p)), P(np.ascontiguousarray(cts[c])), P(sc))
+        rot = (-int(sc[-1]) + int((sc[:-1].astype(np.int64) * ak.lwe_key).sum())) % (2 * N)
+        want = np.zeros(N, np.int64)
+        lib().or_rotate(32, N, rot, P(v.astype(np.int64)), P(want))
+        ph = glwe_phase(p, acc[c].astype(np.int64), ak.glwe_key)
+        assert max(abs(wrap(a - b)) for a, b in zip(ph, want)) < 1 << 26, c
+    assert np.array_equal((yg.lwe_decrypt_phase(out, ak.lwe_key) > 0).astype(int), bits)
+    orc.close()
+
+
+def test_alt_blind_rotation_schoolbook_equals_ntt(alt_small):
+    orc = AltOracle(SMALL, alt_small)
+    rng = np.random.default_rng(6)
+    v = yg.sign_test_polynomial(SMALL)
+    for t in range(2):
+        abar = rng.integers(0, 2 * SMALL.N, SMALL.n + 1).astype(np.int32)
+        if t == 1:
+            abar[::2] = 0
+        assert np.array_equal(orc.blind_rotate_scaled(abar, v), orc.blind_rotate_scaled(abar, v, schoolbook=True))
+    orc.close()

@@ -222,6 +222,8 @@ def load_library(path: str | None = None):
         "ygpu_blind_rotate_lazy": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp]),
         "ygpu_pbs_lazy_batch_device": (c_int, [c_vp, c_i32, c_i32, c_vp, c_vp, c_i32, c_vp, c_vp, c_vp]),
         "ygpu_set_lazy_window": (c_int, [c_vp, c_i32]),
+        "ygpu_keygen_lazy_alt": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
+        "ygpu_key_upload_lazy_alt": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, P(c_i32)]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
         "ygpu_lwe_encrypt_tuniform": (c_int, [c_i32, c_vp, c_u64, c_i32, c_i32, c_vp, c_vp]),
         "ygpu_keygen_tuniform": (c_int, [P(_Params), c_u64, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
@@ -278,7 +280,7 @@ def exported_symbols():
         "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index", "ygpu_server_lanes",
         "ygpu_server_add_tenant_file", "ygpu_server_set_file_slots",
         "ygpu_keygen_lazy", "ygpu_key_upload_lazy", "ygpu_pbs_lazy", "ygpu_blind_rotate_lazy", "ygpu_pbs_lazy_batch_device",
-        "ygpu_set_lazy_window",
+        "ygpu_set_lazy_window", "ygpu_keygen_lazy_alt", "ygpu_key_upload_lazy_alt",
         "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform", "ygpu_get_tile_ex", "ygpu_set_tile_ex", "ygpu_autotune_results",
     ]
 
@@ -429,6 +431,41 @@ def keygen_lazy(params: Params, seed: int, v: np.ndarray, digit_levels: int | No
     _check(lib.ygpu_keygen_lazy(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe, L, _ptr(vv), _ptr(lk), _ptr(gk),
                                 _ptr(first), _ptr(da), _ptr(db), _ptr(s2), _ptr(ksk), threads))
     return LazyKeySet(params, lk, gk, first, da, db, s2, ksk, vv.copy())
+
+
+@dataclass
+class LazyAltKeySet:
+    """Alt lazy-pipe key (BootstrappingKeyMPLazyPipeAlt): torus half rows of every
+    s_i and the TRLEV of s^2; independent of the test polynomial."""
+    params: Params
+    lwe_key: np.ndarray   # int32[n]
+    glwe_key: np.ndarray  # int32[kN]
+    rows: np.ndarray      # int32[n][R][k+1][N]
+    s2: np.ndarray        # uint64[D][k+1][N]
+    ksk: np.ndarray       # int32[kN][t][n+1]
+
+    @property
+    def digit_levels(self) -> int:
+        return int(self.s2.shape[0])
+
+
+def keygen_lazy_alt(params: Params, seed: int, digit_levels: int | None = None, sigma_log2_lwe: float | None = None,
+                    sigma_log2_glwe: float | None = None, threads: int = 0) -> LazyAltKeySet:
+    """Seeded Alt lazy-pipe keys (ygpu_keygen_lazy_alt)."""
+    lib = load_library()
+    dl, dg = default_sigmas(params)
+    sigma_log2_lwe = dl if sigma_log2_lwe is None else sigma_log2_lwe
+    sigma_log2_glwe = dg if sigma_log2_glwe is None else sigma_log2_glwe
+    D = params.level if digit_levels is None else digit_levels
+    lk = np.zeros(params.n, np.int32)
+    gk = np.zeros(params.k * params.N, np.int32)
+    rows = np.zeros((params.n, params.level, params.k + 1, params.N), np.int32)
+    s2 = np.zeros((D, params.k + 1, params.N), np.uint64)
+    ksk = np.zeros(params.ksk_shape, np.int32)
+    cp = params.c()
+    _check(lib.ygpu_keygen_lazy_alt(ctypes.byref(cp), seed, sigma_log2_lwe, sigma_log2_glwe, D, _ptr(lk), _ptr(gk), _ptr(rows),
+                                    _ptr(s2), _ptr(ksk), threads))
+    return LazyAltKeySet(params, lk, gk, rows, s2, ksk)
 
 
 def keygen_classic(params: Params, seed: int, sigma_log2_lwe: float | None = None,
@@ -665,8 +702,14 @@ class Context:
         _check(self.lib.ygpu_key_upload_succinct(self.h, _ptr(half), _ptr(s2), keys.s2_level, _ptr(ksk), ctypes.byref(kid)))
         return kid.value
 
-    def upload_key_lazy(self, keys: LazyKeySet) -> int:
+    def upload_key_lazy(self, keys) -> int:
+        """A lazy-pipe key set (LazyKeySet or LazyAltKeySet)."""
         kid = ctypes.c_int32(-1)
+        if isinstance(keys, LazyAltKeySet):
+            arrs = [np.ascontiguousarray(keys.rows, dtype=np.int32), np.ascontiguousarray(keys.s2, dtype=np.uint64),
+                    np.ascontiguousarray(keys.ksk, dtype=np.int32)]
+            _check(self.lib.ygpu_key_upload_lazy_alt(self.h, keys.digit_levels, *[_ptr(a) for a in arrs], ctypes.byref(kid)))
+            return kid.value
         arrs = [np.ascontiguousarray(keys.first, dtype=np.int32), np.ascontiguousarray(keys.digits_a, dtype=np.int8),
                 np.ascontiguousarray(keys.digits_b, dtype=np.int8), np.ascontiguousarray(keys.s2, dtype=np.uint64),
                 np.ascontiguousarray(keys.ksk, dtype=np.int32)]
@@ -1149,7 +1192,7 @@ class Server:
 __all__ = [
     "Params", "PARAMS_T32_N1024", "PARAMS_T56_N2048", "Q_49P_T32", "Q_61P_T56", "KeySet", "keygen",
     "lwe_encrypt", "lwe_decrypt_phase", "sign_test_polynomial", "lut_test_polynomial", "Context",
-    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request", "LazyKeySet", "keygen_lazy",
+    "DeviceBuffer", "device_count", "load_library", "exported_symbols", "YgpuError", "replace", "Server", "Request", "LazyKeySet", "keygen_lazy", "LazyAltKeySet", "keygen_lazy_alt",
     "save_keys", "key_file_params", "load_key_arrays", "SuccinctKeySet", "keygen_succinct", "max_s2_level",
     "key_file_s2_level", "load_succinct_arrays", "keygen_classic", "SCHED_MODES", "abi_struct_mismatches",
     "keygen_tuniform", "lwe_encrypt_tuniform", "KEY_FORMS",

@@ -43,7 +43,8 @@ struct KeyRec {
     int32_t* d_ksk = nullptr;
     void* d_lazy = nullptr;    // a lazy-pipe key's device block (lazy_layout)
     bool lazy = false;
-    int lazy_digits = 0;       // its stored digit levels (param.l)
+    bool lazy_alt = false;     // the Alt form: torus rows instead of digits
+    int lazy_digits = 0;       // its digit levels (param.l)
     size_t bytes = 0;
     int users = 0;  // synchronous calls running on a plain lane with this key
 };
@@ -166,6 +167,8 @@ struct ygpu_ctx {
     size_t lazy_acc_cap = 0;
     void* d_lazy_tab = nullptr;
     size_t lazy_tab_cap = 0;
+    void* d_lazy_first = nullptr;  // the Alt form's derived first components
+    size_t lazy_first_cap = 0;
     int lazy_window = 0;
     std::mutex mu;
     std::condition_variable cv;   // a plain lane or a key's last user finished

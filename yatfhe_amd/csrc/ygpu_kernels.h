@@ -82,11 +82,13 @@ struct LazyArgs {
     int count, n;
     int level;               // R: gadget levels of the rows (param.lApprox)
     int dlevel;              // D: stored digit levels (param.l), R <= D <= 8
+    int radix_bits;
     int prescaled;           // lwe_in already holds abar (mod 2N) values
     const int32_t* lwe_in;   // [count][n+1]
     const int32_t* v;        // Torus[N] (v_stride 0) or Torus[count][N]
     int v_stride;
-    const int32_t* first;    // bskFirst[0] [k+1][N]
+    const int32_t* first;    // bskFirst[0] [k+1][N] (Alt: the derived first component, per v)
+    int first_stride;        // ints between ciphertexts' first components (0: one for all)
     int32_t* acc;            // [count][k+1][N]
     int32_t* avals;          // [count][n-1]: the rotation amounts of steps 1 .. n-1
     const int8_t* da;        // [n-1][R][D][k][N]
@@ -97,12 +99,19 @@ struct LazyArgs {
     int s0, steps, ws_steps; // the window: components s0 .. s0 + steps - 1
     const double** row_tab;  // [count]: each ciphertext's rows in ws (k_lazy_prepare fills it)
     int32_t* row_slot;       // [count]: c -> c
+    // the Alt form (BootstrappingKeyMPLazyPipeAlt): torus rows instead of digits
+    const int32_t* rows_t;   // [n][R][k+1][N]: cPrime[lvl] of every s_i, torus
+    const double* row0e;     // component 0's rows in the engine form [R][k+1][N]
+    int32_t* first_out;      // [count_v][k+1][N]: k_lazy_alt_first's output
+    double ninv;             // N^-1 mod p, centred
     const double* tw;        // [2][N]
     PrimeF64 prime;
 };
 bool lazy_supported(int tbits, int logn, int k);
 hipError_t launch_lazy_prepare(const LazyArgs& a, hipStream_t s);
 hipError_t launch_lazy_derive(const LazyArgs& a, hipStream_t s);
+hipError_t launch_lazy_alt_first(const LazyArgs& a, int count_v, hipStream_t s);
+hipError_t launch_lazy_alt_derive(const LazyArgs& a, hipStream_t s);
 // The batched blind rotation in its lazy form (no rotation, the accumulator
 // replaced by the product, steps with a rotation amount of 0 skipped), TORUS32
 // N = 1024: gadget 2^7 x 3 specialised, other gadgets up to 2^7 and 8 levels.

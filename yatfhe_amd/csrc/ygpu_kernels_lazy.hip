@@ -52,9 +52,10 @@ __global__ void __launch_bounds__(256) k_lazy_prepare(LazyArgs a) {
     validate_rotator(-abar(n), N, aT2, w2);
     const int32_t* __restrict__ v = a.v + static_cast<size_t>(a.v_stride) * ct;
     int32_t* __restrict__ acc = a.acc + static_cast<size_t>(ct) * K1 * N;
+    const int32_t* __restrict__ first = a.first + static_cast<size_t>(a.first_stride) * ct;
     for (int e = static_cast<int>(threadIdx.x); e < K1 * N; e += 256) {
         const int r = e / N, j = e % N;
-        const int32_t* __restrict__ f = a.first + static_cast<size_t>(r) * N;
+        const int32_t* __restrict__ f = first + static_cast<size_t>(r) * N;
         // t_r[m] = (X^{a0} f_r)[m] - f_r[m] (+ v[m] for the body row), at m = the source of j under X^{-b}
         const int m = (j - aT2) & (N - 1);
         const int src = (m - aT1) & (N - 1);
@@ -149,7 +150,157 @@ __global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_derive(LazyArgs a) {
     }
 }
 
+// The Alt form's first component, deriveFirstComponentNtt (trgsw.cpp:929-955)
+// with prepareGdV (911-927): sum_lvl GD(v)_lvl * cPrime_0[lvl] mod p (R =
+// lApprox digits of v), inverse-transformed, centred, reduced to the torus.
+// One wavefront per test polynomial.
+__global__ void __launch_bounds__(64) k_lazy_alt_first(LazyArgs a, int count_v) {
+    constexpr int N = LZ_N, T = LZ_T, K1 = LZ_K1, LOGN = LZ_LOGN;
+    __shared__ double scr[LZ_SCR];
+    const int vi = static_cast<int>(blockIdx.x);
+    if (vi >= count_v) return;
+    const int tl = static_cast<int>(threadIdx.x);
+    const int R = a.level;
+    const int rb = a.radix_bits;
+    const int32_t* __restrict__ v = a.v + static_cast<size_t>(a.v_stride) * vi;
+    const PrimeF64 pr = a.prime;
+    uint32_t dg0[16], dg1[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) digits_runtime_packed<32>(v[tl + T * r], R, rb, dg0[r], dg1[r]);
+    double ca[16], cb[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) { ca[r] = 0.0; cb[r] = 0.0; }
+    for (int lvl = 0; lvl < R; lvl++) {
+        double x[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) x[r] = static_cast<double>(digit_runtime(dg0[r], dg1[r], lvl, rb));
+        ntt_forward<LOGN, 1>(x, scr, tl, a.tw, pr);
+        const double* __restrict__ ra = a.row0e + (static_cast<size_t>(lvl) * K1 + 0) * N + tl;
+        const double* __restrict__ rbp = a.row0e + (static_cast<size_t>(lvl) * K1 + 1) * N + tl;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            ca[r] = __dadd_rn(ca[r], mulmod(x[r], ra[r * T], pr));
+            cb[r] = __dadd_rn(cb[r], mulmod(x[r], rbp[r * T], pr));
+        }
+    }
+    int32_t* __restrict__ out = a.first_out + static_cast<size_t>(vi) * K1 * N;
+    for (int c = 0; c < K1; c++) {
+        double x[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) x[r] = c == 0 ? ca[r] : cb[r];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        ntt_inverse<LOGN, 1>(x, scr, tl, a.tw + N, pr);
+#pragma unroll
+        for (int r = 0; r < 16; r++) out[static_cast<size_t>(c) * N + tl + T * r] = static_cast<int32_t>(f64_low32(centre(x[r], pr)));
+    }
+}
+
+// The Alt form's rows: one wavefront per (ciphertext, step, level lvl < R) as
+// k_lazy_derive, from the torus rows (blindRotateLazyPipeAltNttSweep's stageA
+// and stageB, blind_rotate.cpp:1035-1087): the mask of cPrime[lvl] of s_{comp}
+// times (X^a - 1), cut into D digits, transformed and multiplied into c and
+// cPrime.a; the body times (X^a - 1) plus 2^(32 - (lvl+1) b) at coefficient 0
+// (rotateTrlweMinusOneBPlusOne, trlwe.cpp:291-297), transformed whole: cPrime.b.
+__global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_alt_derive(LazyArgs a) {
+    constexpr int N = LZ_N, T = LZ_T, K1 = LZ_K1, LOGN = LZ_LOGN;
+    __shared__ double scratch[LZ_WAVES][LZ_SCR];
+    __shared__ int32_t stage[LZ_WAVES][N];
+    const int w = static_cast<int>(threadIdx.x) / 64;
+    const int tl = static_cast<int>(threadIdx.x) % 64;
+    const int L = a.level, D = a.dlevel, rb = a.radix_bits;
+    const int64_t item = static_cast<int64_t>(blockIdx.x) * LZ_WAVES + w;
+    if (item >= static_cast<int64_t>(a.count) * a.steps * L) return;  // whole wavefront: no workgroup barrier follows
+    const int lvl = static_cast<int>(item % L);
+    const int si = static_cast<int>((item / L) % a.steps);
+    const int ct = static_cast<int>(item / (static_cast<int64_t>(L) * a.steps));
+    const int step = a.s0 + si;  // steps 1 .. n-1 are 0 .. n-2 here
+    int aT, wrap;
+    validate_rotator(a.avals[static_cast<size_t>(ct) * (a.n - 1) + step], N, aT, wrap);
+    const bool flip = wrap < 0;
+    const PrimeF64 pr = a.prime;
+    double* scr = scratch[w];
+    int32_t* st = stage[w];
+    const int32_t* __restrict__ row = a.rows_t + ((static_cast<size_t>(step + 1) * L + lvl) * K1) * N;
+    // (X^a - 1) p at this lane's pass-0 positions, mod 2^32
+    auto rotated = [&](const int32_t* __restrict__ src, uint32_t (&out)[16]) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < 4; q++) reinterpret_cast<uint4*>(st)[tl + 64 * q] = reinterpret_cast<const uint4*>(src)[tl + 64 * q];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int j = tl + T * r;
+            const uint32_t vs = static_cast<uint32_t>(st[(j - aT) & (N - 1)]);
+            out[r] = (((j < aT) != flip) ? 0u - vs : vs) - static_cast<uint32_t>(st[j]);
+        }
+    };
+    double cpa[16], ca[16], cb[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) { cpa[r] = 0.0; ca[r] = 0.0; cb[r] = 0.0; }
+    uint32_t u[16];
+    // the mask: D digits of each rotated coefficient (gadgetDecomposeTrlweA)
+    rotated(row, u);
+    uint32_t dg0[16], dg1[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) digits_runtime_packed<32>(static_cast<int32_t>(u[r]), D, rb, dg0[r], dg1[r]);
+    for (int l = 0; l < D; l++) {
+        const double g = a.g[l];
+        double x[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) x[r] = static_cast<double>(digit_runtime(dg0[r], dg1[r], l, rb));
+        ntt_forward<LOGN, 1>(x, scr, tl, a.tw, pr);
+        const double* __restrict__ s2a = a.s2e + (static_cast<size_t>(l) * K1 + 0) * N + tl;
+        const double* __restrict__ s2b = a.s2e + (static_cast<size_t>(l) * K1 + 1) * N + tl;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            cpa[r] = __dadd_rn(cpa[r], mulmod(x[r], g, pr));
+            ca[r] = __dadd_rn(ca[r], mulmod(x[r], s2a[r * T], pr));
+            cb[r] = __dadd_rn(cb[r], mulmod(x[r], s2b[r * T], pr));
+        }
+    }
+    // the body, rotated, plus the gadget's 1 of this level, whole
+    rotated(row + N, u);
+    double cpb[16];
+    {
+        const uint32_t one = uint32_t{1} << (32 - (lvl + 1) * rb);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const uint32_t b = (tl == 0 && r == 0) ? u[r] + one : u[r];
+            cpb[r] = static_cast<double>(static_cast<int32_t>(b));
+        }
+        ntt_forward<LOGN, 1>(cpb, scr, tl, a.tw, pr);
+#pragma unroll
+        for (int r = 0; r < 16; r++) cpb[r] = mulmod(cpb[r], a.ninv, pr);
+    }
+    double* __restrict__ out = a.ws + ((static_cast<size_t>(ct) * a.ws_steps + si) * L + lvl) * K1 * K1 * N + tl;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const double b = centre(cpb[r], pr);
+        out[0 * N + r * T] = centre(__dadd_rn(reduce(ca[r], pr), b), pr);
+        out[1 * N + r * T] = centre(cb[r], pr);
+        out[2 * N + r * T] = centre(cpa[r], pr);
+        out[3 * N + r * T] = b;
+    }
+}
+
 bool lazy_supported(int tbits, int logn, int k) { return tbits == 32 && logn == LZ_LOGN && k == 1; }
+
+hipError_t launch_lazy_alt_first(const LazyArgs& a, int count_v, hipStream_t s) {
+    hipLaunchKernelGGL(k_lazy_alt_first, dim3(static_cast<unsigned>(count_v)), dim3(64), 0, s, a, count_v);
+    return hipGetLastError();
+}
+
+hipError_t launch_lazy_alt_derive(const LazyArgs& a, hipStream_t s) {
+    const int64_t items = static_cast<int64_t>(a.count) * a.steps * a.level;
+    const int64_t groups = (items + LZ_WAVES - 1) / LZ_WAVES;
+    if (groups < 1 || groups > (int64_t{1} << 31) - 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_lazy_alt_derive, dim3(static_cast<unsigned>(groups)), dim3(64 * LZ_WAVES), 0, s, a);
+    return hipGetLastError();
+}
 
 hipError_t launch_lazy_prepare(const LazyArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_lazy_prepare, dim3(static_cast<unsigned>(a.count)), dim3(256), 0, s, a);

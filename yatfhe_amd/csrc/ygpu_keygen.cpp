@@ -422,6 +422,32 @@ int keygen_lazy(const ygpu_params* params, uint64_t seed, const Noise& nl, const
     return YGPU_OK;
 }
 
+// genBootstrappingKeyMPLazyPipeAlt (bootstrapping.cpp:322-339), binary keys:
+// encryptTrgswMP(bskPrime[i], s_i) with isHalf for every i < n (the succinct
+// key's half rows for the same seed, in torus form) and the TRLEV of s^2 with
+// D = param.l levels.  rows: int32 [n][R][k+1][N].
+int keygen_lazy_alt(const ygpu_params* params, uint64_t seed, const Noise& nl, const Noise& ng, int32_t digit_levels, int32_t* lwe_key,
+                    int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk, int32_t threads) {
+    if (!params || !lwe_key || !glwe_key || !rows || !s2 || !ksk) return YGPU_ERR_INVALID_ARG;
+    const ygpu_params p = *params;
+    if (p.torus_bits != 32) return YGPU_ERR_UNSUPPORTED;
+    if (digit_levels < 1 || digit_levels > 8 || digit_levels * p.radix_bits > p.torus_bits) return YGPU_ERR_INVALID_ARG;
+    const uint64_t q = p.q_ntt, qhalf = (q - 1) / 2;
+    try {
+        const size_t total = static_cast<size_t>(p.n) * p.level * (p.k + 1) * p.N;
+        std::vector<uint64_t> half(total);
+        int rc = keygen_succinct(params, seed, nl, ng, digit_levels, lwe_key, glwe_key, half.data(), s2, ksk, threads);
+        if (rc) return rc;
+        for (size_t i = 0; i < total; i++) {
+            const uint64_t r = half[i];
+            rows[i] = static_cast<int32_t>(sext(r > qhalf ? static_cast<int64_t>(r) - static_cast<int64_t>(q) : static_cast<int64_t>(r), 32));
+        }
+    } catch (...) {
+        return YGPU_ERR_INVALID_ARG;
+    }
+    return YGPU_OK;
+}
+
 bool tuniform_ok(const ygpu_params* params, int lwe_b, int glwe_b) {
     return params && lwe_b >= 0 && lwe_b < 31 && glwe_b >= 0 && glwe_b < params->torus_bits - 1;
 }
@@ -471,6 +497,13 @@ int ygpu_keygen_lazy(const ygpu_params* params, uint64_t seed, double sigma_log2
     if (!params) return YGPU_ERR_INVALID_ARG;
     return keygen_lazy(params, seed, Noise::gaussian_log2(sigma_log2_lwe, 32), Noise::gaussian_log2(sigma_log2_glwe, params->torus_bits),
                        digit_levels, v, lwe_key, glwe_key, first, digits_a, digits_b, s2, ksk, threads);
+}
+
+int ygpu_keygen_lazy_alt(const ygpu_params* params, uint64_t seed, double sigma_log2_lwe, double sigma_log2_glwe, int32_t digit_levels,
+                         int32_t* lwe_key, int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk, int32_t threads) {
+    if (!params) return YGPU_ERR_INVALID_ARG;
+    return keygen_lazy_alt(params, seed, Noise::gaussian_log2(sigma_log2_lwe, 32), Noise::gaussian_log2(sigma_log2_glwe, params->torus_bits),
+                           digit_levels, lwe_key, glwe_key, rows, s2, ksk, threads);
 }
 
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count, const int32_t* mu,

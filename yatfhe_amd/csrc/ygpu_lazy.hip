@@ -40,6 +40,27 @@ LazyLayout lazy_layout(const ygpu_params& p, int digits) {
     return l;
 }
 
+// The device block of an Alt key: the torus rows of every s_i, component 0's
+// rows in the engine form (the first-component derivation), s^2 TRLEV in the
+// engine form.
+struct AltLayout {
+    size_t rows = 0, row0e = 0, s2e = 0, total = 0;
+    size_t rows_bytes = 0, row0_bytes = 0, s2_bytes = 0;
+};
+
+AltLayout alt_layout(const ygpu_params& p, int digits) {
+    AltLayout l;
+    const size_t R = p.level, D = digits, K1 = p.k + 1, N = p.N;
+    l.rows_bytes = static_cast<size_t>(p.n) * R * K1 * N * sizeof(int32_t);
+    l.row0_bytes = R * K1 * N * sizeof(double);
+    l.s2_bytes = D * K1 * N * sizeof(double);
+    l.rows = 0;
+    l.row0e = align_up(l.rows + l.rows_bytes);
+    l.s2e = align_up(l.row0e + l.row0_bytes);
+    l.total = align_up(l.s2e + l.s2_bytes);
+    return l;
+}
+
 int check_lazy_params(const ygpu_ctx* ctx, int digits) {
     const ygpu_params& p = ctx->p;
     if (!lazy_supported(ctx->tbits, ctx->logn, p.k) || p.radix_bits > 7 || p.level > 8 || p.n < 2 || p.n - 1 > KS_MAX_N)
@@ -59,6 +80,7 @@ int run_lazy(ygpu_ctx* ctx, int32_t key_id, int count, const int32_t* d_lwe, boo
     const KeyRec& k = ctx->keys[key_id];
     const int D = k.lazy_digits;
     const LazyLayout lay = lazy_layout(p, D);
+    const AltLayout alay = alt_layout(p, D);
     const int n = p.n, L = p.level, K1 = p.k + 1, N = p.N;
     const size_t accBytes = sizeof(int32_t) * K1 * N * static_cast<size_t>(count);
     void* acc = d_acc_out;
@@ -85,16 +107,23 @@ int run_lazy(ygpu_ctx* ctx, int32_t key_id, int count, const int32_t* d_lwe, boo
     la.n = n;
     la.level = L;
     la.dlevel = D;
+    la.radix_bits = p.radix_bits;
     la.prescaled = prescaled ? 1 : 0;
     la.lwe_in = d_lwe;
     la.v = static_cast<const int32_t*>(d_v);
     la.v_stride = v_stride;
-    la.first = reinterpret_cast<const int32_t*>(block + lay.first);
     la.acc = static_cast<int32_t*>(acc);
     la.avals = static_cast<int32_t*>(ctx->d_lazy_av);
-    la.da = reinterpret_cast<const int8_t*>(block + lay.da);
-    la.db = reinterpret_cast<const int8_t*>(block + lay.db);
-    la.s2e = reinterpret_cast<const double*>(block + lay.s2e);
+    if (k.lazy_alt) {
+        la.rows_t = reinterpret_cast<const int32_t*>(block + alay.rows);
+        la.row0e = reinterpret_cast<const double*>(block + alay.row0e);
+        la.s2e = reinterpret_cast<const double*>(block + alay.s2e);
+    } else {
+        la.first = reinterpret_cast<const int32_t*>(block + lay.first);
+        la.da = reinterpret_cast<const int8_t*>(block + lay.da);
+        la.db = reinterpret_cast<const int8_t*>(block + lay.db);
+        la.s2e = reinterpret_cast<const double*>(block + lay.s2e);
+    }
     // getNttGadgetRecomper(l) (ntt_hexl.cpp:122-149) times N^-1, centred
     const HostNtt& t = ctx->ntt[0];
     for (int l = 0; l < D; l++) {
@@ -107,12 +136,24 @@ int run_lazy(ygpu_ctx* ctx, int32_t key_id, int count, const int32_t* d_lwe, boo
     la.row_slot = reinterpret_cast<int32_t*>(static_cast<char*>(ctx->d_lazy_tab) + sizeof(double*) * count);
     la.tw = ctx->d_tw;
     la.prime = ctx->prime[0];
+    la.ninv = centred_double(t.ninv, t.p);
     hipStream_t st = ctx->stream;
+    if (k.lazy_alt) {
+        // the first component, GD(v) (x) RLEV(s_0), once per test polynomial
+        const int countV = v_stride ? count : 1;
+        rc = ensure(&ctx->d_lazy_first, &ctx->lazy_first_cap, sizeof(int32_t) * K1 * N * static_cast<size_t>(countV));
+        if (rc) return rc;
+        la.first_out = static_cast<int32_t*>(ctx->d_lazy_first);
+        la.first = la.first_out;
+        la.first_stride = v_stride ? K1 * N : 0;
+        YG_HIP(timed(ctx, "k_lazy_alt_first", st, [&] { return launch_lazy_alt_first(la, countV, st); }));
+    }
     YG_HIP(timed(ctx, "k_lazy_prepare", st, [&] { return launch_lazy_prepare(la, st); }));
     for (int s0 = 0; s0 < n - 1; s0 += S) {
         la.s0 = s0;
         la.steps = std::min(S, n - 1 - s0);
-        YG_HIP(timed(ctx, "k_lazy_derive", st, [&] { return launch_lazy_derive(la, st); }));
+        if (k.lazy_alt) YG_HIP(timed(ctx, "k_lazy_alt_derive", st, [&] { return launch_lazy_alt_derive(la, st); }));
+        else YG_HIP(timed(ctx, "k_lazy_derive", st, [&] { return launch_lazy_derive(la, st); }));
         BrArgs a = br_args(ctx, count, la.avals + s0, d_v, v_stride, acc, la.row_tab, la.row_slot);
         a.n = la.steps;
         a.prescaled = 1;
@@ -169,6 +210,68 @@ int ygpu_key_upload_lazy(ygpu_ctx* ctx, int32_t digit_levels, const int32_t* fir
         if (e == hipSuccess)
             e = launch_bsk_expand(ctx->logn, expand_args(ctx, static_cast<const uint64_t*>(ctx->d_stage), static_cast<int>(s2Polys),
                                                          reinterpret_cast<double*>(block + lay.s2e)), st);
+        if (e == hipSuccess) e = hipMemcpyAsync(rec.d_ksk, planes.data(), planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            hipStreamSynchronize(st);
+            hipFree(rec.d_lazy);
+            hipFree(rec.d_ksk);
+            return fail(YGPU_ERR_HIP, std::string("lazy-pipe key upload: ") + hipGetErrorString(e));
+        }
+        return register_key(ctx, rec, key_id);
+    });
+}
+
+int ygpu_key_upload_lazy_alt(ygpu_ctx* ctx, int32_t digit_levels, const int32_t* rows, const uint64_t* s2, const int32_t* ksk,
+                             int32_t* key_id) {
+    if (!rows || !s2 || !ksk || !key_id) return fail(YGPU_ERR_INVALID_ARG, "null argument");
+    return with_ctx(ctx, [&]() -> int {
+        int rc = check_lazy_params(ctx, digit_levels);
+        if (rc) return rc;
+        const ygpu_params& p = ctx->p;
+        const AltLayout lay = alt_layout(p, digit_levels);
+        const size_t K1 = p.k + 1, N = p.N;
+        const size_t s2Polys = static_cast<size_t>(digit_levels) * K1, row0Polys = static_cast<size_t>(p.level) * K1;
+        // component 0's rows as residues (torus values lifted mod q) beside s^2
+        std::vector<uint64_t> res(row0Polys * N);
+        for (size_t i = 0; i < res.size(); i++) res[i] = rows[i] < 0 ? static_cast<uint64_t>(static_cast<int64_t>(rows[i]) + static_cast<int64_t>(p.q_ntt))
+                                                                     : static_cast<uint64_t>(rows[i]);
+        const size_t pw = poly_words(p);
+        std::vector<uint64_t> packed((s2Polys + row0Polys) * pw + 1, 0);
+        int64_t bad = pack_residues(s2, s2Polys, p, packed.data());
+        if (bad >= 0) return fail(YGPU_ERR_INVALID_ARG, "s2 coefficient " + std::to_string(bad) + " is not below q_ntt");
+        pack_residues(res.data(), row0Polys, p, packed.data() + s2Polys * pw);
+        std::vector<uint32_t> planes;
+        ksk_planes(p, ksk, planes);
+        KeyRec rec;
+        rec.live = true;
+        rec.lazy = true;
+        rec.lazy_alt = true;
+        rec.lazy_digits = digit_levels;
+        rec.bytes = lay.total + planes.size() * sizeof(uint32_t);
+        hipError_t e = hipMalloc(&rec.d_lazy, lay.total);
+        if (e != hipSuccess) return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc lazy-pipe key: ") + hipGetErrorString(e));
+        e = hipMalloc(reinterpret_cast<void**>(&rec.d_ksk), planes.size() * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            hipFree(rec.d_lazy);
+            return fail(YGPU_ERR_OUT_OF_MEMORY, std::string("hipMalloc KSK: ") + hipGetErrorString(e));
+        }
+        rc = ensure(&ctx->d_stage, &ctx->stage_cap, packed.size() * sizeof(uint64_t));
+        if (rc) {
+            hipFree(rec.d_lazy);
+            hipFree(rec.d_ksk);
+            return rc;
+        }
+        char* block = static_cast<char*>(rec.d_lazy);
+        hipStream_t st = ctx->stream;
+        const uint64_t* stage = static_cast<const uint64_t*>(ctx->d_stage);
+        e = hipMemcpyAsync(block + lay.rows, rows, lay.rows_bytes, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_stage, packed.data(), packed.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess)
+            e = launch_bsk_expand(ctx->logn, expand_args(ctx, stage, static_cast<int>(s2Polys), reinterpret_cast<double*>(block + lay.s2e)), st);
+        if (e == hipSuccess)
+            e = launch_bsk_expand(ctx->logn, expand_args(ctx, stage + s2Polys * pw, static_cast<int>(row0Polys),
+                                                         reinterpret_cast<double*>(block + lay.row0e)), st);
         if (e == hipSuccess) e = hipMemcpyAsync(rec.d_ksk, planes.data(), planes.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {

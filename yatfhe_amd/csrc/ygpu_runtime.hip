@@ -199,7 +199,7 @@ int ygpu_destroy(ygpu_ctx* ctx) {
                         ctx->d_acc, static_cast<void*>(ctx->d_partial), static_cast<void*>(ctx->d_slots),
                         static_cast<void*>(ctx->d_lwe1), ctx->d_v1, static_cast<void*>(ctx->d_out1),
                         static_cast<void*>(ctx->d_ext1), ctx->d_acc1, ctx->d_accin1, ctx->d_stage, ctx->d_lazy_ws,
-                        ctx->d_lazy_av, ctx->d_lazy_acc, ctx->d_lazy_tab})
+                        ctx->d_lazy_av, ctx->d_lazy_acc, ctx->d_lazy_tab, ctx->d_lazy_first})
             if (p) hipFree(p);
         if (ctx->h_status) hipHostFree(ctx->h_status);
         for (auto& t : ctx->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
