@@ -418,6 +418,14 @@ int ygpu_keygen_lazy_alt(const ygpu_params* params, uint64_t seed, double sigma_
                          int32_t* lwe_key, int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk, int32_t threads);
 int ygpu_key_upload_lazy_alt(ygpu_ctx* ctx, int32_t digit_levels, const int32_t* rows, const uint64_t* s2, const int32_t* ksk,
                              int32_t* key_id);
+/* Both lazy-pipe forms with the reference's TUniform noise (bounds as
+ * ygpu_keygen_tuniform: lweNoiseB, rlweNoiseB). */
+int ygpu_keygen_lazy_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits,
+                              int32_t digit_levels, const int32_t* v, int32_t* lwe_key, int32_t* glwe_key, int32_t* first,
+                              int8_t* digits_a, int8_t* digits_b, uint64_t* s2, int32_t* ksk, int32_t threads);
+int ygpu_keygen_lazy_alt_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits,
+                                  int32_t digit_levels, int32_t* lwe_key, int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk,
+                                  int32_t threads);
 /* Steps per window of derived rows (0: the most that fit a 1 GiB workspace for
  * the batch); results do not depend on it.  No counterpart in the reference. */
 int ygpu_set_lazy_window(ygpu_ctx* ctx, int32_t steps);

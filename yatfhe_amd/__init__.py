@@ -224,6 +224,9 @@ def load_library(path: str | None = None):
         "ygpu_set_lazy_window": (c_int, [c_vp, c_i32]),
         "ygpu_keygen_lazy_alt": (c_int, [P(_Params), c_u64, c_dbl, c_dbl, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_key_upload_lazy_alt": (c_int, [c_vp, c_i32, c_vp, c_vp, c_vp, P(c_i32)]),
+        "ygpu_keygen_lazy_tuniform": (c_int, [P(_Params), c_u64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                              c_vp, c_i32]),
+        "ygpu_keygen_lazy_alt_tuniform": (c_int, [P(_Params), c_u64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
         "ygpu_lwe_encrypt": (c_int, [c_i32, c_vp, c_u64, c_dbl, c_i32, c_vp, c_vp]),
         "ygpu_lwe_encrypt_tuniform": (c_int, [c_i32, c_vp, c_u64, c_i32, c_i32, c_vp, c_vp]),
         "ygpu_keygen_tuniform": (c_int, [P(_Params), c_u64, c_i32, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32]),
@@ -280,7 +283,8 @@ def exported_symbols():
         "ygpu_set_extract_index", "ygpu_get_extract_index", "ygpu_sample_extract_index", "ygpu_server_set_extract_index", "ygpu_server_lanes",
         "ygpu_server_add_tenant_file", "ygpu_server_set_file_slots",
         "ygpu_keygen_lazy", "ygpu_key_upload_lazy", "ygpu_pbs_lazy", "ygpu_blind_rotate_lazy", "ygpu_pbs_lazy_batch_device",
-        "ygpu_set_lazy_window", "ygpu_keygen_lazy_alt", "ygpu_key_upload_lazy_alt",
+        "ygpu_set_lazy_window", "ygpu_keygen_lazy_alt", "ygpu_key_upload_lazy_alt", "ygpu_keygen_lazy_tuniform",
+        "ygpu_keygen_lazy_alt_tuniform",
         "ygpu_keygen_tuniform", "ygpu_lwe_encrypt_tuniform", "ygpu_get_tile_ex", "ygpu_set_tile_ex", "ygpu_autotune_results",
     ]
 

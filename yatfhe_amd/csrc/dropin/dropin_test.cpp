@@ -248,6 +248,55 @@ static void wwl24_fr(const std::string& which) {
     }
 }
 
+// blindRotateLazyPipeNtt and blindRotateLazyPipeAltNtt (the serving
+// benchmark's OURS keys) with the reference's TORUS32 lazy-pipe gadget (2^4,
+// l = 8, lApprox = 4) at N = 1024: every slot decrypts after extraction and
+// keyswitch, with the reference call forms (blind_rotate.h:38-46).
+static void lazy_fr() {
+#if defined(TORUS32)
+    YatfheParameters param{};
+    param.n = 630; param.N = 1024; param.setRadixBits(4); param.l = 8; param.lApprox = 4;
+    param.torusBase = 4;
+    initYatfhe(param);
+    const int p = param.torusBase;
+    TlweKey tlweKey{param};
+    TrgswKey trgswKey{param};
+    TrlweKey& trlweKey = trgswKey.trlweKey;
+    TlweKeySwitchingKey ksKey{param};
+    genTlweKey(tlweKey);
+    genTrlweKey(trlweKey);
+    genTlweKeySwitchingKey(ksKey, trlweKey, tlweKey, param);
+    TorusPolynomial v{param.N};
+    generateTestPolynomialFR(v, p, 2 * param.N);
+    BootstrappingKeyMPLazyPipe pipe{param, param.lApprox, true, false};
+    genBootstrappingKeyMPLazyPipe(pipe, trgswKey, tlweKey, v, param);
+    BootstrappingKeyMPLazyPipeAlt alt{param, param.lApprox, true};
+    genBootstrappingKeyMPLazyPipeAlt(alt, trgswKey, tlweKey, param);
+    std::vector<NttPolynomial> gdVntt;
+    prepareGdV(gdVntt, v, param);
+    std::printf("lazy pipe: n:%d, N:%d, b:%d, l:%d, lApprox:%d\n", param.n, param.N, param.radixBits, param.l, param.lApprox);
+    for (int plain = -p / 2; plain < p / 2; plain++) {
+        const int slot = ((plain % p) + p) % p;
+        Tlwe input{param.n};
+        symEncTlwe(input, modSwitchToTorusGeneral(slot, 2 * p, param.qLwe), tlweKey);
+        ScaledTlwe scaled{2 * param.N, param.n};
+        rescaleTlweToNewMod(scaled, input);
+        for (int form = 0; form < 2; form++) {
+            Trlwe accum{param};
+            if (form == 0) blindRotateLazyPipeNtt(accum, pipe, scaled, v, param);
+            else blindRotateLazyPipeAltNtt(accum, alt, scaled, v, gdVntt, param);
+            Tlwe tmp{ksKey.nCurrKey};
+            Tlwe output{param.n};
+            extractTlweFromTrlwe(tmp, accum, 0);
+            switchKeyForTlwe(output, ksKey, tmp, param);
+            const int dec = symDecTlweToInt(output, tlweKey, p);
+            std::printf("%s m: %d, slot: %d, dec: %d\n", form ? "alt" : "pipe", plain, slot, dec);
+            EXPECT_EQ(plain, dec, form ? "LazyPipeAlt decrypted slot" : "LazyPipe decrypted slot");
+        }
+    }
+#endif
+}
+
 int main(int argc, char** argv) {
     setYatfheSeed(2024);
     try {
@@ -258,6 +307,7 @@ int main(int argc, char** argv) {
             // the reference's TORUS32 defaults: n=680, N=2048, B=2^4, l=8, lApprox=4, KS 2^2 x 16
             mp21_fr("default");
             wwl24_fr("default");
+            lazy_fr();
 #endif
         }
     } catch (const std::exception& e) {

@@ -216,6 +216,12 @@ struct TrgswMP {
     int l{};
     int k{};
     bool isHalf{false};
+    TrgswMP() = default;
+    // (trgsw.h:89-118): the lazy-pipe Alt key holds its torus half rows here
+    TrgswMP(const YatfheParameters& p, const int level, const bool half = false)
+        : cPrime(level, Trlwe(p.k, p.N)), l(level), k(p.k), isHalf(half) {
+        if (!isHalf) c.assign(level, std::vector<Trlwe>(p.k, Trlwe(p.k, p.N)));
+    }
 };
 struct Trgsw {
     std::vector<std::vector<Trlwe>> trlweSamples{};
@@ -306,6 +312,62 @@ struct BootstrappingKeyWWL24 {
         if (group != 1) throw std::invalid_argument("BootstrappingKeyWWL24: only group == 1 keys are supported");
         n = p.n;
         bskDft.assign(n, TrgswMPDft(p, level, true));
+        s2Dft = TrlevDft(p, p.l);
+    }
+};
+
+// DecompPolynomial (polynomial.h:79-92): one digit level of a polynomial, in
+// the Decomp type (int8 at TORUS32).
+struct DecompPolynomial {
+    std::vector<Decomp> coeffs{};
+    int N{};
+    DecompPolynomial() = default;
+    explicit DecompPolynomial(int n) : coeffs(n, 0), N(n) {}
+    DecompPolynomial(int n, Decomp value) : coeffs(n, value), N(n) {}
+};
+
+// BootstrappingKeyMPLazyPipe (bootstrapping.h:250-282), group 1, binary keys:
+// bskFirst[0] a TRLWE of v s_0; bskDecompA[decompIndex(i, lvl)][l][k] and
+// bskDecompB[decompIndex(i, lvl)][l] the param.l digits of cPrime[lvl] of s_{i+1}.
+// bskB (the reference's transform of cPrime.b) is not read by
+// blindRotateLazyPipeNtt and is left zero here.
+struct BootstrappingKeyMPLazyPipe {
+    std::vector<Trlwe> bskFirst{};
+    std::vector<std::vector<std::vector<DecompPolynomial>>> bskDecompA{};
+    std::vector<NttPolynomial> bskB{};
+    std::vector<std::vector<DecompPolynomial>> bskDecompB{};
+    TrlevDft s2Dft;
+    int n{};
+    int level{};
+    int group{};
+    bool initialized{false};
+    BootstrappingKeyMPLazyPipe() = default;
+    BootstrappingKeyMPLazyPipe(const YatfheParameters& p, const int level_, const bool, const bool) : level(level_), group(p.group) {
+        if (group != 1) throw std::invalid_argument("BootstrappingKeyMPLazyPipe: only group == 1 keys are supported");
+        n = p.n;
+        const size_t m = static_cast<size_t>(n - 1) * level;
+        bskFirst.assign(1, Trlwe(p.k, p.N));
+        bskDecompA.assign(m, std::vector<std::vector<DecompPolynomial>>(p.l, std::vector<DecompPolynomial>(p.k, DecompPolynomial(p.N))));
+        bskB.assign(m, NttPolynomial(p.N));
+        bskDecompB.assign(m, std::vector<DecompPolynomial>(p.l, DecompPolynomial(p.N)));
+        s2Dft = TrlevDft(p, p.l);
+    }
+    size_t decompIndex(int i, int lvl) const { return static_cast<size_t>(i) * level + lvl; }
+};
+
+// BootstrappingKeyMPLazyPipeAlt (bootstrapping.h:284-306): the half rows of
+// every s_i in torus form and the TRLEV of s^2 with param.l levels.
+struct BootstrappingKeyMPLazyPipeAlt {
+    std::vector<TrgswMP> bskPrime;
+    TrlevDft s2Dft;
+    int n{};
+    int level{};
+    int group{};
+    BootstrappingKeyMPLazyPipeAlt() = default;
+    BootstrappingKeyMPLazyPipeAlt(const YatfheParameters& p, const int level_, const bool isHalf) : level(level_), group(p.group) {
+        if (group != 1) throw std::invalid_argument("BootstrappingKeyMPLazyPipeAlt: only group == 1 keys are supported");
+        n = p.n;
+        bskPrime.assign(p.n, TrgswMP(p, level_, isHalf));
         s2Dft = TrlevDft(p, p.l);
     }
 };
@@ -882,6 +944,73 @@ inline void genBootstrappingKeyWWL24(BootstrappingKeyWWL24& bsk, TrgswKey&, cons
         }
 }
 
+// genBootstrappingKeyMPLazyPipe (bootstrapping.cpp:269-320) for test polynomial
+// v, TORUS32: rows of bsk.level levels, param.l digits each.
+inline void genBootstrappingKeyMPLazyPipe(BootstrappingKeyMPLazyPipe& bsk, const TrgswKey&, const TlweKey&, const TorusPolynomial& v,
+                                          const YatfheParameters& p) {
+#if defined(TORUS32)
+    using namespace yatfhe_dropin;
+    if (bsk.bskDecompB.empty()) throw std::invalid_argument("genBootstrappingKeyMPLazyPipe: empty key");
+    ensure_draw(false);
+    const ygpu_params q = to_params(p, bsk.level);
+    const size_t N = p.N, K = p.k, R = bsk.level, D = p.l, m = static_cast<size_t>(p.n - 1);
+    std::vector<int32_t> lwe(p.n), glwe(K * N), first((K + 1) * N), ksk(K * N * p.ksLevel * (p.n + 1));
+    std::vector<int8_t> da(m * R * D * K * N), db(m * R * D * N);
+    std::vector<uint64_t> s2(D * (K + 1) * N);
+    std::vector<int32_t> vv(v.coeffs.begin(), v.coeffs.end());
+    check(ygpu_keygen_lazy_tuniform(&q, draw().seed, p.lweNoiseB, p.rlweNoiseB, static_cast<int32_t>(D), vv.data(), lwe.data(), glwe.data(),
+                                    first.data(), da.data(), db.data(), s2.data(), ksk.data(), 0));
+    for (size_t c = 0; c <= K; c++) {
+        TorusPolynomial& t = c < K ? bsk.bskFirst[0].a[c] : bsk.bskFirst[0].b;
+        for (size_t j = 0; j < N; j++) t.coeffs[j] = first[c * N + j];
+    }
+    for (size_t cell = 0; cell < m * R; cell++)
+        for (size_t l = 0; l < D; l++) {
+            for (size_t c = 0; c < K; c++)
+                std::memcpy(bsk.bskDecompA[cell][l][c].coeffs.data(), da.data() + ((cell * D + l) * K + c) * N, N);
+            std::memcpy(bsk.bskDecompB[cell][l].coeffs.data(), db.data() + (cell * D + l) * N, N);
+        }
+    for (size_t l = 0; l < D; l++)
+        for (size_t c = 0; c <= K; c++) {
+            NttPolynomial& poly = c < K ? bsk.s2Dft.trlweDfts[l].a[c] : bsk.s2Dft.trlweDfts[l].b;
+            std::memcpy(poly.coeffs.data(), s2.data() + (l * (K + 1) + c) * N, sizeof(uint64_t) * N);
+        }
+#else
+    (void)bsk; (void)v; (void)p;
+    throw std::invalid_argument("genBootstrappingKeyMPLazyPipe: the lazy-pipe forms are built for TORUS32");
+#endif
+}
+
+// genBootstrappingKeyMPLazyPipeAlt (bootstrapping.cpp:322-339), TORUS32.
+inline void genBootstrappingKeyMPLazyPipeAlt(BootstrappingKeyMPLazyPipeAlt& bsk, const TrgswKey&, const TlweKey&, const YatfheParameters& p) {
+#if defined(TORUS32)
+    using namespace yatfhe_dropin;
+    if (bsk.bskPrime.empty()) throw std::invalid_argument("genBootstrappingKeyMPLazyPipeAlt: empty key");
+    ensure_draw(false);
+    const ygpu_params q = to_params(p, bsk.level);
+    const size_t N = p.N, K = p.k, R = bsk.level, D = p.l;
+    std::vector<int32_t> lwe(p.n), glwe(K * N), rows(static_cast<size_t>(p.n) * R * (K + 1) * N), ksk(K * N * p.ksLevel * (p.n + 1));
+    std::vector<uint64_t> s2(D * (K + 1) * N);
+    check(ygpu_keygen_lazy_alt_tuniform(&q, draw().seed, p.lweNoiseB, p.rlweNoiseB, static_cast<int32_t>(D), lwe.data(), glwe.data(),
+                                        rows.data(), s2.data(), ksk.data(), 0));
+    size_t o = 0;
+    for (auto& g : bsk.bskPrime)
+        for (size_t lvl = 0; lvl < R; lvl++)
+            for (size_t c = 0; c <= K; c++) {
+                TorusPolynomial& t = c < K ? g.cPrime[lvl].a[c] : g.cPrime[lvl].b;
+                for (size_t j = 0; j < N; j++) t.coeffs[j] = rows[o++];
+            }
+    for (size_t l = 0; l < D; l++)
+        for (size_t c = 0; c <= K; c++) {
+            NttPolynomial& poly = c < K ? bsk.s2Dft.trlweDfts[l].a[c] : bsk.s2Dft.trlweDfts[l].b;
+            std::memcpy(poly.coeffs.data(), s2.data() + (l * (K + 1) + c) * N, sizeof(uint64_t) * N);
+        }
+#else
+    (void)bsk; (void)p;
+    throw std::invalid_argument("genBootstrappingKeyMPLazyPipeAlt: the lazy-pipe forms are built for TORUS32");
+#endif
+}
+
 // symEncTlwe (tlwe.cpp:20-31): TUniform noise with the key's errorB, seeded
 inline void symEncTlwe(Tlwe& tlweSample, const Torus message, const TlweKey& key) {
     std::vector<int32_t> s(key.s.begin(), key.s.end());
@@ -990,6 +1119,148 @@ inline void blindRotateWWL24Ntt(Trlwe& accum, const BootstrappingKeyWWL24& bsk, 
         }
     }
     blind_rotate_on(key, accum, input);
+}
+
+// prepareGdV (trgsw.cpp:911-927): the lApprox gadget digits of v, as
+// coefficient-form residues (the NTT-form containers hold coefficient form
+// here).  blindRotateLazyPipeAltNtt takes it for source compatibility; the
+// engine derives the first component from v itself.
+inline void prepareGdV(std::vector<NttPolynomial>& gdVntt, const TorusPolynomial& v, const YatfheParameters& param) {
+    const int L = param.lApprox, N = param.N, b = param.radixBits, w = param.torusBits;
+    gdVntt.assign(L, NttPolynomial(N));
+    const int64_t B = int64_t{1} << b, halfB = B >> 1;
+    const uint64_t mask = w >= 64 ? ~0ULL : ((uint64_t{1} << w) - 1);
+    const int dropped = w - L * b;
+    for (int j = 0; j < N; j++) {
+        // zeroMeanDigits (gadget_decomposition.h:75-95)
+        const uint64_t u = static_cast<uint64_t>(static_cast<int64_t>(v.coeffs[j])) & mask;
+        const bool neg = (u >> (w - 1)) & 1;
+        const uint64_t mm = (neg ? (0ULL - u) & mask : u) + (dropped > 0 ? (uint64_t{1} << (dropped - 1)) : 0);
+        int64_t carry = 0;
+        for (int lvl = L - 1; lvl >= 0; --lvl) {
+            const int shift = w - (lvl + 1) * b;
+            const int64_t digit = static_cast<int64_t>((mm >> shift) & static_cast<uint64_t>(B - 1)) + carry;
+            const int64_t tie = lvl > 0 ? static_cast<int64_t>((mm >> (shift + 2 * b - 1)) & 1) : 0;
+            carry = (digit > halfB) | ((digit == halfB) & tie);
+            const int64_t d = (neg ? -1 : 1) * (carry ? digit - B : digit);
+            gdVntt[lvl].coeffs[j] = d < 0 ? static_cast<NttType>(static_cast<int64_t>(param.qNtt) + d) : static_cast<NttType>(d);
+        }
+    }
+}
+
+namespace yatfhe_dropin {
+inline void lazy_rotate_on(std::pair<ygpu_ctx*, int32_t> key, Trlwe& accum, const ScaledTlwe& input, const TorusPolynomial& v) {
+    std::vector<int32_t> abar(input.a.begin(), input.a.end());
+    abar.push_back(input.b);
+    std::vector<Torus> acc(static_cast<size_t>(accum.k + 1) * accum.N);
+    check(ygpu_blind_rotate_lazy(key.first, key.second, abar.data(), v.coeffs.data(), acc.data()));
+    from_flat(acc, accum);
+}
+}  // namespace yatfhe_dropin
+
+// blindRotateLazyPipeNtt (blind_rotate.cpp:822-902): accum := the lazy-pipe
+// blind rotation of input under bsk (whose bskFirst encrypts v s_0).
+inline void blindRotateLazyPipeNtt(Trlwe& accum, const BootstrappingKeyMPLazyPipe& bsk, const ScaledTlwe& input, const TorusPolynomial& v,
+                                   const YatfheParameters& param) {
+    using namespace yatfhe_dropin;
+    std::pair<ygpu_ctx*, int32_t> key;
+    {
+        State& s = state();
+        std::lock_guard<std::mutex> lk(s.mu);
+        require_init();
+        const YatfheParameters& p = s.param;
+        if (bsk.n != p.n || bsk.bskDecompB.size() != static_cast<size_t>(p.n - 1) * bsk.level)
+            throw std::invalid_argument("lazy-pipe key has the wrong number of components");
+        const size_t N = p.N, K = p.k, R = bsk.level, D = bsk.bskDecompB.at(0).size(), m = static_cast<size_t>(p.n - 1);
+        const size_t per = R * D * N;
+        const uint64_t fp = fingerprint(m * per + 1, [&](size_t idx) -> uint64_t {
+            if (idx == m * per) return static_cast<uint64_t>(static_cast<uint32_t>(bsk.bskFirst.at(0).b.coeffs.at(0)));
+            const size_t cell = idx / (D * N), l = (idx / N) % D, j = idx % N;
+            return static_cast<uint64_t>(static_cast<uint8_t>(bsk.bskDecompB.at(cell).at(l).coeffs.at(j))) |
+                   (static_cast<uint64_t>(static_cast<uint8_t>(bsk.bskDecompA.at(cell).at(l).at(0).coeffs.at(j))) << 8);
+        });
+        ygpu_ctx* ctx = context_locked(bsk.level, param.driftPhase);
+        const auto ck = std::make_tuple(ctx, static_cast<const void*>(&bsk), fp, static_cast<const void*>(nullptr), uint64_t{2});
+        auto it = s.keys.find(ck);
+        if (it != s.keys.end()) {
+            key = {ctx, it->second};
+        } else {
+            std::vector<int32_t> first((K + 1) * N);
+            for (size_t c = 0; c <= K; c++) {
+                const TorusPolynomial& t = c < K ? bsk.bskFirst.at(0).a.at(c) : bsk.bskFirst.at(0).b;
+                for (size_t j = 0; j < N; j++) first[c * N + j] = static_cast<int32_t>(t.coeffs.at(j));
+            }
+            std::vector<int8_t> da(m * R * D * K * N), db(m * R * D * N);
+            for (size_t cell = 0; cell < m * R; cell++)
+                for (size_t l = 0; l < D; l++) {
+                    for (size_t c = 0; c < K; c++)
+                        for (size_t j = 0; j < N; j++)
+                            da[((cell * D + l) * K + c) * N + j] = static_cast<int8_t>(bsk.bskDecompA.at(cell).at(l).at(c).coeffs.at(j));
+                    for (size_t j = 0; j < N; j++) db[(cell * D + l) * N + j] = static_cast<int8_t>(bsk.bskDecompB.at(cell).at(l).coeffs.at(j));
+                }
+            std::vector<uint64_t> s2(D * (K + 1) * N);
+            for (size_t l = 0; l < D; l++)
+                for (size_t c = 0; c <= K; c++) {
+                    const TrlweDft& t = bsk.s2Dft.trlweDfts.at(l);
+                    std::memcpy(s2.data() + (l * (K + 1) + c) * N, (c < K ? t.a.at(c) : t.b).coeffs.data(), sizeof(uint64_t) * N);
+                }
+            std::vector<int32_t> zero(K * N * p.ksLevel * (p.n + 1), 0);
+            int32_t id = -1;
+            check(ygpu_key_upload_lazy(ctx, static_cast<int32_t>(D), first.data(), da.data(), db.data(), s2.data(), zero.data(), &id));
+            s.keys[ck] = id;
+            key = {ctx, id};
+        }
+    }
+    lazy_rotate_on(key, accum, input, v);
+}
+
+// blindRotateLazyPipeAltNtt (blind_rotate.cpp:996-1000, the sweep at 1002-1136).
+inline void blindRotateLazyPipeAltNtt(Trlwe& accum, const BootstrappingKeyMPLazyPipeAlt& bsk, const ScaledTlwe& input,
+                                      const TorusPolynomial& v, const std::vector<NttPolynomial>& gdVntt, const YatfheParameters& param) {
+    using namespace yatfhe_dropin;
+    if (static_cast<int>(gdVntt.size()) != param.lApprox) throw std::invalid_argument("blindRotateLazyPipeAltNtt: gdVntt needs lApprox levels");
+    std::pair<ygpu_ctx*, int32_t> key;
+    {
+        State& s = state();
+        std::lock_guard<std::mutex> lk(s.mu);
+        require_init();
+        const YatfheParameters& p = s.param;
+        if (bsk.n != p.n || bsk.bskPrime.size() != static_cast<size_t>(p.n)) throw std::invalid_argument("Alt key has the wrong number of components");
+        const size_t N = p.N, K = p.k, R = bsk.level, D = bsk.s2Dft.l;
+        const size_t per = R * (K + 1) * N;
+        const uint64_t fp = fingerprint(bsk.bskPrime.size() * per, [&](size_t idx) -> uint64_t {
+            const size_t i = idx / per, lvl = (idx / ((K + 1) * N)) % R, c = (idx / N) % (K + 1), j = idx % N;
+            const Trlwe& t = bsk.bskPrime.at(i).cPrime.at(lvl);
+            return static_cast<uint64_t>(static_cast<uint32_t>((c < K ? t.a.at(c) : t.b).coeffs.at(j)));
+        });
+        ygpu_ctx* ctx = context_locked(bsk.level, param.driftPhase);
+        const auto ck = std::make_tuple(ctx, static_cast<const void*>(&bsk), fp, static_cast<const void*>(nullptr), uint64_t{3});
+        auto it = s.keys.find(ck);
+        if (it != s.keys.end()) {
+            key = {ctx, it->second};
+        } else {
+            std::vector<int32_t> rows(static_cast<size_t>(p.n) * per);
+            size_t o = 0;
+            for (const auto& g : bsk.bskPrime)
+                for (size_t lvl = 0; lvl < R; lvl++)
+                    for (size_t c = 0; c <= K; c++) {
+                        const TorusPolynomial& t = c < K ? g.cPrime.at(lvl).a.at(c) : g.cPrime.at(lvl).b;
+                        for (size_t j = 0; j < N; j++) rows[o++] = static_cast<int32_t>(t.coeffs.at(j));
+                    }
+            std::vector<uint64_t> s2(D * (K + 1) * N);
+            for (size_t l = 0; l < D; l++)
+                for (size_t c = 0; c <= K; c++) {
+                    const TrlweDft& t = bsk.s2Dft.trlweDfts.at(l);
+                    std::memcpy(s2.data() + (l * (K + 1) + c) * N, (c < K ? t.a.at(c) : t.b).coeffs.data(), sizeof(uint64_t) * N);
+                }
+            std::vector<int32_t> zero(K * N * p.ksLevel * (p.n + 1), 0);
+            int32_t id = -1;
+            check(ygpu_key_upload_lazy_alt(ctx, static_cast<int32_t>(D), rows.data(), s2.data(), zero.data(), &id));
+            s.keys[ck] = id;
+            key = {ctx, id};
+        }
+    }
+    lazy_rotate_on(key, accum, input, v);
 }
 
 // extractTlweFromTrlwe (trlwe.cpp:241-250) at `index`, on the GPU

@@ -506,6 +506,22 @@ int ygpu_keygen_lazy_alt(const ygpu_params* params, uint64_t seed, double sigma_
                            digit_levels, lwe_key, glwe_key, rows, s2, ksk, threads);
 }
 
+int ygpu_keygen_lazy_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits,
+                              int32_t digit_levels, const int32_t* v, int32_t* lwe_key, int32_t* glwe_key, int32_t* first,
+                              int8_t* digits_a, int8_t* digits_b, uint64_t* s2, int32_t* ksk, int32_t threads) {
+    if (!tuniform_ok(params, lwe_bound_bits, glwe_bound_bits)) return YGPU_ERR_INVALID_ARG;
+    return keygen_lazy(params, seed, Noise::tuniform_bound(lwe_bound_bits), Noise::tuniform_bound(glwe_bound_bits), digit_levels, v,
+                       lwe_key, glwe_key, first, digits_a, digits_b, s2, ksk, threads);
+}
+
+int ygpu_keygen_lazy_alt_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits,
+                                  int32_t digit_levels, int32_t* lwe_key, int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk,
+                                  int32_t threads) {
+    if (!tuniform_ok(params, lwe_bound_bits, glwe_bound_bits)) return YGPU_ERR_INVALID_ARG;
+    return keygen_lazy_alt(params, seed, Noise::tuniform_bound(lwe_bound_bits), Noise::tuniform_bound(glwe_bound_bits), digit_levels,
+                           lwe_key, glwe_key, rows, s2, ksk, threads);
+}
+
 int ygpu_lwe_encrypt(int32_t n, const int32_t* lwe_key, uint64_t seed, double sigma_log2, int32_t count, const int32_t* mu,
                      int32_t* out) {
     return lwe_encrypt(n, lwe_key, seed, Noise::gaussian_log2(sigma_log2, 32), count, mu, out);
