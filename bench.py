@@ -52,7 +52,7 @@ FP64_PEAK_TFLOPS = 78.6  # MI355X published dense FP64 (vector and matrix)
 INSTR_PER_MODMUL = 6
 MODMUL_CEILING = FP64_PEAK_TFLOPS * 1e12 / 2 / INSTR_PER_MODMUL  # 6.55e12 modmul/s
 DETAIL_PATH = os.path.join(ROOT, "gpurun_out", "bench_detail.json")
-LEGS = ("t32_batch", "t56_batch", "multitenant", "serving", "keyload")
+LEGS = ("t32_batch", "t56_batch", "multitenant", "serving", "keyload", "lazy")
 
 
 def modmul_per_pbs(p) -> int:
@@ -309,6 +309,63 @@ def leg_t56_batch(yg, args, peak):
                   "host_p50_ms": round(pct(lat_host, 50) * 1e3, 2)},
         "cpu_baseline": None if args.no_cpu else cpu_baseline(p, keys, cts, v, per_thread=2, lat_runs=20),
     }
+
+
+# ------------------------------------------------------ lazy-pipe key formats
+BATCH_LAZY = 4096
+
+
+def leg_lazy(yg, args):
+    """The lazy-pipe key formats at n=630, N=1024, batch 4096 on device buffers:
+    the Alt form (the reference's serving benchmark's OURS keys) and the plain
+    pipe, both with the reference's lazy-pipe gadget 2^4, l=8, lApprox=4 (its
+    digits keep all 32 bits; at 2^7 x 3 the format's rows carry the 11 dropped
+    bits through the key and do not decrypt).  Per-ciphertext row derivation
+    and the blind rotation are timed apart."""
+    import dataclasses
+    out = {}
+    exact = dataclasses.replace(yg.PARAMS_T32_N1024, radix_bits=4, level=4)
+    forms = {"alt_b4x4_d8": (exact, 8, "alt"), "pipe_b4x4_d8": (exact, 8, "pipe")}
+    for name, (p, digits, form) in forms.items():
+        v = yg.sign_test_polynomial(p)
+        if form == "alt":
+            keys = yg.keygen_lazy_alt(p, seed=0, digit_levels=digits, sigma_log2_glwe=2.21 - 32)
+            derive = "k_lazy_alt_derive"
+        else:
+            keys = yg.keygen_lazy(p, seed=0, v=v, digit_levels=digits, sigma_log2_glwe=2.21 - 32)
+            derive = "k_lazy_derive"
+        bits, cts = sign_inputs(yg, keys, BATCH_LAZY, 1)
+        with yg.Context(p, 0) as ctx:
+            kid = ctx.upload_key_lazy(keys)
+            d_lwe = ctx.alloc(cts.nbytes)
+            d_lwe.upload(cts)
+            d_v = ctx.alloc(v.nbytes)
+            d_v.upload(v)
+            d_out = ctx.alloc(BATCH_LAZY * (p.n + 1) * 4)
+            ctx.pbs_lazy_batch_device(kid, BATCH_LAZY, d_lwe, d_v, False, d_out)
+            ctx.sync()
+            ctx.timing(True)
+            ctx.timing_reset()
+            steps = 2
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.pbs_lazy_batch_device(kid, BATCH_LAZY, d_lwe, d_v, False, d_out)
+            ctx.sync()
+            dt = time.perf_counter() - t0
+            times = {k: ctx.timing_get(k) for k in (derive, "k_blind_rotate_lazy", "k_extract_ks")}
+            ctx.timing(False)
+            res = d_out.download(np.zeros((BATCH_LAZY, p.n + 1), np.int32))
+            for b in (d_out, d_v, d_lwe):
+                b.free()
+        ph = yg.lwe_decrypt_phase(res, keys.lwe_key)
+        out[name] = {
+            "workload": f"TORUS32 n={p.n} N={p.N} gadget 2^{p.radix_bits}x{p.level}, {digits} digit levels, {form} form, "
+                        f"{BATCH_LAZY} sign ciphertexts (seed 1), one key (seed 0)",
+            "value": round(steps * BATCH_LAZY / dt, 1), "unit": "PBS/s", "ms_per_step": round(dt / steps * 1e3, 2),
+            "kernels_ms_per_step": {k: round(ms / steps, 2) for k, (n_, ms) in times.items()},
+            "decrypt_check": bool(np.array_equal((ph > 0).astype(int), bits)),
+        }
+    return out
 
 
 # ------------------------------------------------------ 64 tenants, one graph
@@ -710,6 +767,12 @@ def main():
         r = leg_keyload(yg, args)
         detail["key_load_forms"] = r
         summaries["key_load_ms"] = {k: [v["full"]["device_expand_ms"], v["succinct"]["device_expand_ms"]] for k, v in r.items()}
+    if "lazy" in legs:
+        log("lazy-pipe key formats")
+        r = leg_lazy(yg, args)
+        detail["lazy_pipe_t32_b4096"] = r
+        summaries["lazy_pipe_t32_b4096"] = {k: {"value": x["value"], "ms_per_step": x["ms_per_step"],
+                                                "decrypt_check": x["decrypt_check"]} for k, x in r.items()}
     if "serving" in legs:
         log("serving sweep")
         r = leg_serving(yg, args, peak)

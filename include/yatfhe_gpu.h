@@ -426,7 +426,7 @@ int ygpu_keygen_lazy_tuniform(const ygpu_params* params, uint64_t seed, int32_t 
 int ygpu_keygen_lazy_alt_tuniform(const ygpu_params* params, uint64_t seed, int32_t lwe_bound_bits, int32_t glwe_bound_bits,
                                   int32_t digit_levels, int32_t* lwe_key, int32_t* glwe_key, int32_t* rows, uint64_t* s2, int32_t* ksk,
                                   int32_t threads);
-/* Steps per window of derived rows (0: the most that fit a 1 GiB workspace for
+/* Steps per window of derived rows (0: the most that fit a 4 GiB workspace for
  * the batch); results do not depend on it.  No counterpart in the reference. */
 int ygpu_set_lazy_window(ygpu_ctx* ctx, int32_t steps);
 

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_lazy_prepare(LazyArgs a) {
 //   c.a = sum_l D(a)_l s2[l].a + cPrime.b,  c.b = sum_l D(a)_l s2[l].b
 // mod p, with g_l = 2^(32 - (l+1) b) N^-1 and s2 in the engine form (N^-1
 // included), so the rows carry N^-1 as the engine BSK does.
-__global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_derive(LazyArgs a) {
+__global__ void __launch_bounds__(64 * LZ_WAVES, 2) k_lazy_derive(LazyArgs a) {
     constexpr int N = LZ_N, T = LZ_T, K1 = LZ_K1, LOGN = LZ_LOGN;
     __shared__ double scratch[LZ_WAVES][LZ_SCR];
     __shared__ __align__(16) int8_t stage[LZ_WAVES][N];
@@ -95,9 +95,8 @@ __global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_derive(LazyArgs a) {
     int8_t* st = stage[w];
     const size_t cell = static_cast<size_t>(comp) * L + lvl;
 
-    double cpa[16], cpb[16], ca[16], cb[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) { cpa[r] = 0.0; cpb[r] = 0.0; ca[r] = 0.0; cb[r] = 0.0; }
+    double* __restrict__ out = a.ws + ((static_cast<size_t>(ct) * a.ws_steps + si) * L + lvl) * K1 * K1 * N + tl;
+    double cpa[16], ca[16], cb[16];
     // rotateDecompPolynomialMinusOne (polynomial.cpp:273-282) of a stored digit
     // polynomial, narrowed to int8, at this lane's pass-0 positions
     auto rotated = [&](const int8_t* __restrict__ src, bool diag, double (&x)[16]) {
@@ -118,13 +117,32 @@ __global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_derive(LazyArgs a) {
             x[r] = static_cast<double>(d);
         }
     };
+    // the body row first (cPrime.b, stored at once), which also starts c.a;
+    // each sum then holds at most 8 products of magnitude <= 0.51 p
+    {
+        double cpb[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) cpb[r] = 0.0;
+        for (int l = 0; l < D; l++) {
+            double x[16];
+            rotated(a.db + (cell * D + l) * N, l == lvl, x);
+            ntt_forward<LOGN, 1>(x, scr, tl, a.tw, pr);
+            const double g = a.g[l];
+#pragma unroll
+            for (int r = 0; r < 16; r++) cpb[r] = __dadd_rn(cpb[r], mulmod(x[r], g, pr));
+        }
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const double b = centre(cpb[r], pr);
+            out[3 * N + r * T] = b;
+            ca[r] = b;
+            cpa[r] = 0.0;
+            cb[r] = 0.0;
+        }
+    }
     for (int l = 0; l < D; l++) {
         const double g = a.g[l];
         double x[16];
-        rotated(a.db + (cell * D + l) * N, l == lvl, x);
-        ntt_forward<LOGN, 1>(x, scr, tl, a.tw, pr);
-#pragma unroll
-        for (int r = 0; r < 16; r++) cpb[r] = __dadd_rn(cpb[r], mulmod(x[r], g, pr));
         rotated(a.da + (cell * D + l) * N, false, x);
         ntt_forward<LOGN, 1>(x, scr, tl, a.tw, pr);
         const double* __restrict__ s2a = a.s2e + (static_cast<size_t>(l) * K1 + 0) * N + tl;
@@ -138,15 +156,11 @@ __global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_derive(LazyArgs a) {
     }
     // rows of this level: input row 0 (the accumulator's mask digits) is c[0],
     // input row 1 (its body digits) is cPrime; columns a, b
-    double* __restrict__ out = a.ws + ((static_cast<size_t>(ct) * a.ws_steps + si) * L + lvl) * K1 * K1 * N + tl;
-    // each sum holds at most 8 products of magnitude <= 0.51 p
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const double b = reduce(cpb[r], pr);
-        out[0 * N + r * T] = centre(__dadd_rn(reduce(ca[r], pr), b), pr);
+        out[0 * N + r * T] = centre(reduce(ca[r], pr), pr);
         out[1 * N + r * T] = centre(cb[r], pr);
         out[2 * N + r * T] = centre(cpa[r], pr);
-        out[3 * N + r * T] = centre(b, pr);
     }
 }
 
@@ -202,7 +216,7 @@ __global__ void __launch_bounds__(64) k_lazy_alt_first(LazyArgs a, int count_v) 
 // times (X^a - 1), cut into D digits, transformed and multiplied into c and
 // cPrime.a; the body times (X^a - 1) plus 2^(32 - (lvl+1) b) at coefficient 0
 // (rotateTrlweMinusOneBPlusOne, trlwe.cpp:291-297), transformed whole: cPrime.b.
-__global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_alt_derive(LazyArgs a) {
+__global__ void __launch_bounds__(64 * LZ_WAVES, 2) k_lazy_alt_derive(LazyArgs a) {
     constexpr int N = LZ_N, T = LZ_T, K1 = LZ_K1, LOGN = LZ_LOGN;
     __shared__ double scratch[LZ_WAVES][LZ_SCR];
     __shared__ int32_t stage[LZ_WAVES][N];
@@ -238,15 +252,32 @@ __global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_alt_derive(LazyArgs a) {
             out[r] = (((j < aT) != flip) ? 0u - vs : vs) - static_cast<uint32_t>(st[j]);
         }
     };
+    double* __restrict__ out = a.ws + ((static_cast<size_t>(ct) * a.ws_steps + si) * L + lvl) * K1 * K1 * N + tl;
     double cpa[16], ca[16], cb[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) { cpa[r] = 0.0; ca[r] = 0.0; cb[r] = 0.0; }
-    uint32_t u[16];
-    // the mask: D digits of each rotated coefficient (gadgetDecomposeTrlweA)
-    rotated(row, u);
     uint32_t dg0[16], dg1[16];
+    {
+        // the body, rotated, plus the gadget's 1 of this level, whole: cPrime.b,
+        // stored at once; it also starts c.a
+        uint32_t u[16];
+        rotated(row + N, u);
+        const uint32_t one = uint32_t{1} << (32 - (lvl + 1) * rb);
+        double x[16];
 #pragma unroll
-    for (int r = 0; r < 16; r++) digits_runtime_packed<32>(static_cast<int32_t>(u[r]), D, rb, dg0[r], dg1[r]);
+        for (int r = 0; r < 16; r++) x[r] = static_cast<double>(static_cast<int32_t>((tl == 0 && r == 0) ? u[r] + one : u[r]));
+        ntt_forward<LOGN, 1>(x, scr, tl, a.tw, pr);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const double b = centre(mulmod(x[r], a.ninv, pr), pr);
+            out[3 * N + r * T] = b;
+            ca[r] = b;
+            cpa[r] = 0.0;
+            cb[r] = 0.0;
+        }
+        // the mask: D digits of each rotated coefficient (gadgetDecomposeTrlweA)
+        rotated(row, u);
+#pragma unroll
+        for (int r = 0; r < 16; r++) digits_runtime_packed<32>(static_cast<int32_t>(u[r]), D, rb, dg0[r], dg1[r]);
+    }
     for (int l = 0; l < D; l++) {
         const double g = a.g[l];
         double x[16];
@@ -262,28 +293,11 @@ __global__ void __launch_bounds__(64 * LZ_WAVES) k_lazy_alt_derive(LazyArgs a) {
             cb[r] = __dadd_rn(cb[r], mulmod(x[r], s2b[r * T], pr));
         }
     }
-    // the body, rotated, plus the gadget's 1 of this level, whole
-    rotated(row + N, u);
-    double cpb[16];
-    {
-        const uint32_t one = uint32_t{1} << (32 - (lvl + 1) * rb);
-#pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const uint32_t b = (tl == 0 && r == 0) ? u[r] + one : u[r];
-            cpb[r] = static_cast<double>(static_cast<int32_t>(b));
-        }
-        ntt_forward<LOGN, 1>(cpb, scr, tl, a.tw, pr);
-#pragma unroll
-        for (int r = 0; r < 16; r++) cpb[r] = mulmod(cpb[r], a.ninv, pr);
-    }
-    double* __restrict__ out = a.ws + ((static_cast<size_t>(ct) * a.ws_steps + si) * L + lvl) * K1 * K1 * N + tl;
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const double b = centre(cpb[r], pr);
-        out[0 * N + r * T] = centre(__dadd_rn(reduce(ca[r], pr), b), pr);
+        out[0 * N + r * T] = centre(reduce(ca[r], pr), pr);
         out[1 * N + r * T] = centre(cb[r], pr);
         out[2 * N + r * T] = centre(cpa[r], pr);
-        out[3 * N + r * T] = b;
     }
 }
 

@@ -12,9 +12,9 @@
 
 namespace {
 
-// 1 GiB of derived rows per batch window: 32 KiB per ciphertext, step and row
-// level at TORUS32 N = 1024.
-constexpr size_t LAZY_WS_BYTES = size_t{1} << 30;
+// 4 GiB of derived rows per batch window (32 KiB per ciphertext, step and row
+// level at TORUS32 N = 1024): 8 steps for 4096 ciphertexts at 4 levels.
+constexpr size_t LAZY_WS_BYTES = size_t{4} << 30;
 
 inline size_t align_up(size_t x) { return (x + 255) & ~size_t{255}; }
 
