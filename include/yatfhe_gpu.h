@@ -210,7 +210,9 @@ int ygpu_pbs_plain_device(ygpu_ctx* ctx, int32_t key_id, const int32_t* d_lwe_in
                           int32_t* d_lwe_out, void* d_acc_out);
 
 /* Batched-form tile: ciphertexts per workgroup (cpw) and whether the transform
- * twiddles live in LDS (twl = 1) or are read through L1 (twl = 0).
+ * twiddles live in LDS (twl = 1) or are read through L1 (twl = 0); twl = 2
+ * (TORUS32 N=1024 2^7 x 3, with tab = 2) keeps lane-major LDS copies of the
+ * twiddles of transform passes 1 and 2, read without bank conflicts.
  * ygpu_autotune times every tile instantiated for the context's parameter set
  * on a synthetic batch of `count` ciphertexts under `key_id` and keeps the
  * fastest (no counterpart in the reference, which has no tiles). */

@@ -39,12 +39,28 @@ __device__ __forceinline__ double reduce(double x, const PrimeF64& P) {
     return __fma_rn(-t, P.p, x);
 }
 
+// x[r] = reduce(x[r]) for all 16 values, four at a time (one instruction of
+// each of four independent reductions in turn).
+__device__ __forceinline__ void reduce16(double (&x)[16], const PrimeF64& P) {
+#pragma unroll
+    for (int g = 0; g < 16; g += 4) {
+        double t[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[b] = __dmul_rn(x[g + b], P.pinv);
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[b] = __builtin_rint(t[b]);
+#pragma unroll
+        for (int b = 0; b < 4; b++) x[g + b] = __fma_rn(-t[b], P.p, x[g + b]);
+    }
+}
 // Exact centred representative in [-(p-1)/2, (p-1)/2] of a value |x| < 8p.
+// The first reduction leaves |r| <= (p+1)/2 (its quotient estimate is off by
+// at most 2^-49 of p); the second maps exactly the two out-of-range values
+// +-(p+1)/2 to -+(p-1)/2: r * fl(1/p) stays below 1/2 - 2^-51 in magnitude for
+// |r| <= (p-1)/2 and above 1/2 + 2^-51 for |r| = (p+1)/2, so its rint is 0 or
+// +-1 as required.  Six instructions, against ten with two compare-selects.
 __device__ __forceinline__ double centre(double x, const PrimeF64& P) {
-    double r = reduce(x, P);
-    r = (r > P.half) ? __dadd_rn(r, -P.p) : r;
-    r = (r < -P.half) ? __dadd_rn(r, P.p) : r;
-    return r;
+    return reduce(reduce(x, P), P);
 }
 
 // ---------------------------------------------------------------------------
@@ -156,9 +172,79 @@ __device__ __forceinline__ void report_barrier_failure(bool failed, uint32_t* st
     if (failed && threadIdx.x == 0 && status) *reinterpret_cast<volatile uint32_t*>(status) = 1u;
 }
 
+// Where a pass finds its twiddles.  TwFlat: the bit-reversed table of psi
+// powers (forward [N], inverse [N] after it), index (1 << st) + (j >> (LOGN - st)).
+// TwLane (N = 1024): passes 1 and 2 read lane-major tables instead, in which
+// the twiddles a lane needs for one stage sit at k * 64 + tl for its k-th
+// distinct twiddle, so the 64 lanes of a read touch 64 consecutive doubles:
+// no bank conflicts (the flat table's reads in these passes are strided, up to
+// four-way conflicting).  Pass 0's twiddles are uniform across lanes and stay
+// in the flat table.
+struct TwFlat {
+    const double* tw;
+    template <int LOGN, int P>
+    __device__ __forceinline__ double get(int st, int tl, int r) const {
+        const int j = Pass<LOGN, P>::idx(tl, r);
+        return tw[(1 << st) + (j >> (LOGN - st))];
+    }
+};
+// Lane-table geometry (LOGN = 10, T = 64): pass 1 has 1 + 2 + 4 + 8 = 15
+// distinct twiddles per lane over its four stages, pass 2 has 4 + 8 = 12.
+constexpr int TW_LANE_P1 = 15, TW_LANE_P2 = 12, TW_LANE_PER_DIR = (TW_LANE_P1 + TW_LANE_P2) * 64;
+// Offset (in units of 64 doubles) of stage ls's block, and the lane's k for
+// register r: pass 1 (j = (tl & 3) | r << 2 | (tl >> 2) << 6): the twiddle of
+// stage 4 + ls is indexed by (tl >> 2) << ls | r >> (4 - ls), so k = r >> (4 - ls);
+// pass 2 (r = 4g + e, j = e | (tl + 64 g) << 2): stage 8 + ls is indexed by
+// (tl + 64 g) << ls | e >> (2 - ls), so k = g (ls = 0) or 2g + (e >> 1) (ls = 1).
+__host__ __device__ constexpr int tw_lane_block(int P, int ls) {
+    return P == 1 ? (1 << ls) - 1 : TW_LANE_P1 + (ls == 0 ? 0 : 4);
+}
+__host__ __device__ constexpr int tw_lane_k(int P, int ls, int r) {
+    return P == 1 ? (r >> (4 - ls)) : (ls == 0 ? (r >> 2) : 2 * (r >> 2) + ((r & 3) >> 1));
+}
+struct TwLane {
+    const double* flat;  // the flat table of this direction (pass 0)
+    const double* lane;  // this direction's lane tables
+    template <int LOGN, int P>
+    __device__ __forceinline__ double get(int st, int tl, int r) const {
+        static_assert(LOGN == 10, "lane tables: N = 1024");
+        if constexpr (P == 0) {
+            const int j = Pass<LOGN, 0>::idx(tl, r);
+            return flat[(1 << st) + (j >> (LOGN - st))];
+        } else {
+            const int ls = st - Pass<LOGN, P>::S0;
+            return lane[(tw_lane_block(P, ls) + tw_lane_k(P, ls, r)) * 64 + tl];
+        }
+    }
+};
+// Fills one direction's lane tables from its flat table (all threads of the
+// workgroup take part; the caller synchronises before use).
+__device__ __forceinline__ void build_tw_lane(double* lane, const double* flat, int tid, int nthreads) {
+    for (int e = tid; e < TW_LANE_PER_DIR; e += nthreads) {
+        const int tl = e & 63, blk = e >> 6;
+        int P, ls, k;
+        if (blk < TW_LANE_P1) {
+            P = 1;
+            ls = blk < 1 ? 0 : blk < 3 ? 1 : blk < 7 ? 2 : 3;
+            k = blk - ((1 << ls) - 1);
+        } else {
+            P = 2;
+            ls = blk - TW_LANE_P1 < 4 ? 0 : 1;
+            k = blk - TW_LANE_P1 - (ls == 0 ? 0 : 4);
+        }
+        // a register r with this k (the lowest one)
+        int r = 0;
+        if (P == 1) r = k << (4 - ls);
+        else r = ls == 0 ? 4 * k : 4 * (k >> 1) + 2 * (k & 1);
+        const int st = (P == 1 ? 4 : 8) + ls;
+        const int j = P == 1 ? Pass<10, 1>::idx(tl, r) : Pass<10, 2>::idx(tl, r);
+        lane[e] = flat[(1 << st) + (j >> (10 - st))];
+    }
+}
+
 // Cooley-Tukey pass (forward): U, V -> U + wV, U - wV; stages LS0.. of the pass.
-template <int LOGN, int P, int LS0 = 0>
-__device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* __restrict__ tw, const PrimeF64& pr) {
+template <int LOGN, int P, int LS0 = 0, class TW>
+__device__ __forceinline__ void fwd_pass_t(double (&x)[16], int tl, const TW& tw, const PrimeF64& pr) {
     using PS = Pass<LOGN, P>;
 #pragma unroll
     for (int ls = LS0; ls < PS::S; ls++) {
@@ -167,8 +253,7 @@ __device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* 
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             if (r & half) continue;
-            const int j = PS::idx(tl, r);
-            const double w = tw[(1 << st) + (j >> (LOGN - st))];
+            const double w = tw.template get<LOGN, P>(st, tl, r);
             const double U = x[r];
             const double V = mulmod(x[r + half], w, pr);
             x[r] = __dadd_rn(U, V);
@@ -176,10 +261,13 @@ __device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* 
         }
     }
 }
-
+template <int LOGN, int P, int LS0 = 0>
+__device__ __forceinline__ void fwd_pass(double (&x)[16], int tl, const double* __restrict__ tw, const PrimeF64& pr) {
+    fwd_pass_t<LOGN, P, LS0>(x, tl, TwFlat{tw}, pr);
+}
 // Gentleman-Sande pass (inverse): U, V -> U + V, (U - V) w.
-template <int LOGN, int P>
-__device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* __restrict__ itw, const PrimeF64& pr) {
+template <int LOGN, int P, class TW>
+__device__ __forceinline__ void inv_pass_t(double (&x)[16], int tl, const TW& itw, const PrimeF64& pr) {
     using PS = Pass<LOGN, P>;
 #pragma unroll
     for (int ls = PS::S - 1; ls >= 0; ls--) {
@@ -188,14 +276,17 @@ __device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* 
 #pragma unroll
         for (int r = 0; r < 16; r++) {
             if (r & half) continue;
-            const int j = PS::idx(tl, r);
-            const double w = itw[(1 << st) + (j >> (LOGN - st))];
+            const double w = itw.template get<LOGN, P>(st, tl, r);
             const double U = x[r];
             const double V = x[r + half];
             x[r] = __dadd_rn(U, V);
             x[r + half] = mulmod(__dadd_rn(U, -V), w, pr);
         }
     }
+}
+template <int LOGN, int P>
+__device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* __restrict__ itw, const PrimeF64& pr) {
+    inv_pass_t<LOGN, P>(x, tl, TwFlat{itw}, pr);
 }
 
 template <int LOGN, int PW, int PR, int X, class Sync>
@@ -301,32 +392,37 @@ __device__ __forceinline__ void ntt_forward_from_stage1(double (&x)[16], double*
 }
 
 // The forward transform after fwd_first_two_stages.
+template <int LOGN, class Sync, class TW>
+__device__ __forceinline__ void ntt_forward_from_stage2_t(double (&x)[16], double* scr, int tl, const TW& tw, const PrimeF64& pr, Sync& sy) {
+    fwd_pass_t<LOGN, 0, 2>(x, tl, tw, pr);
+    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
+    fwd_pass_t<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
+    fwd_pass_t<LOGN, 2>(x, tl, tw, pr);
+}
 template <int LOGN, class Sync>
 __device__ __forceinline__ void ntt_forward_from_stage2(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr,
                                                         Sync& sy) {
-    fwd_pass<LOGN, 0, 2>(x, tl, tw, pr);
-    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
-    fwd_pass<LOGN, 1>(x, tl, tw, pr);
-    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
-    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+    ntt_forward_from_stage2_t<LOGN>(x, scr, tl, TwFlat{tw}, pr, sy);
 }
 
 // Inverse negacyclic NTT without the 1/N factor (folded into the key):
 // pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Values are reduced
 // at the start of every pass, which bounds them by 8.8p throughout.
+template <int LOGN, class Sync, class TW>
+__device__ __forceinline__ void ntt_inverse_sync_t(double (&x)[16], double* scr, int tl, const TW& itw, const PrimeF64& pr, Sync& sy) {
+    reduce16(x, pr);
+    inv_pass_t<LOGN, 2>(x, tl, itw, pr);
+    exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
+    reduce16(x, pr);
+    inv_pass_t<LOGN, 1>(x, tl, itw, pr);
+    exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
+    reduce16(x, pr);
+    inv_pass_t<LOGN, 0>(x, tl, itw, pr);
+}
 template <int LOGN, class Sync>
 __device__ __forceinline__ void ntt_inverse_sync(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr, Sync& sy) {
-#pragma unroll
-    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
-    inv_pass<LOGN, 2>(x, tl, itw, pr);
-    exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
-#pragma unroll
-    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
-    inv_pass<LOGN, 1>(x, tl, itw, pr);
-    exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
-#pragma unroll
-    for (int r = 0; r < 16; r++) x[r] = reduce(x[r], pr);
-    inv_pass<LOGN, 0>(x, tl, itw, pr);
+    ntt_inverse_sync_t<LOGN>(x, scr, tl, TwFlat{itw}, pr, sy);
 }
 template <int LOGN, int TEAM_WAVES>
 __device__ __forceinline__ void ntt_inverse(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr) {

@@ -15,7 +15,9 @@ namespace ygpu {
 //   CPW    ciphertexts per workgroup
 //   PACK8  digits packed 8 bits each (radix_bits <= 8), else 16 bits
 //   L, RB  gadget levels and base log2 when fixed at compile time (0: runtime)
-//   TWL    twiddles in LDS (1), global memory through L1 (0), or automatic (-1)
+//   TWL    twiddles in LDS (1), global memory through L1 (0), or automatic (-1);
+//          2: lane-major LDS tables for transform passes 1 and 2 (TwLane;
+//          TORUS32 N = 1024 with TAB = 2)
 //   TAB    forward stages taken from digit tables (0, 1 or 2; fixed 8-bit shapes)
 template <int TBITS_, int LOGN_, int NP_, int CPW_, bool PACK8_, int L_ = 0, int RB_ = 0, int TWL_ = -1, int TAB_ = 0, bool LAZY_ = false>
 struct BrCfg {
@@ -41,7 +43,8 @@ struct BrCfg {
     // Waves per SIMD the register allocation must allow: two (256 VGPRs).
     static constexpr int MIN_WAVES = 2;
     // twiddles in LDS when they fit beside the per-ciphertext state
-    static constexpr bool TW_LDS = TWL_ < 0 ? (NP * 2 * N * 8 <= 16384) : (TWL_ == 1);
+    static constexpr bool TW_LDS = TWL_ < 0 ? (NP * 2 * N * 8 <= 16384) : (TWL_ >= 1);
+    static constexpr bool TW_LANE = TWL_ == 2;
 };
 
 // Garner reconstruction constants for NP == 2.

@@ -37,9 +37,14 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = C::NP, TW = C::TEAM_WAVES, CPW = C::CPW;
     constexpr int SCR = Scratch<LOGN>::DOUBLES;
     constexpr int CT_THREADS = 2 * T;
-    constexpr int TWN = C::TW_LDS ? NP * 2 * N : 1;
+    // lane-major twiddle tables (TwLane): passes 1 and 2 read them; the flat
+    // table then only holds pass 0's sixteen twiddles of each direction
+    constexpr bool LANE = C::TW_LANE;
+    static_assert(!LANE || (LOGN == 10 && NP == 1 && C::TAB == 2), "lane tables: TORUS32 N = 1024, TAB = 2");
+    constexpr int TWN = LANE ? 32 : C::TW_LDS ? NP * 2 * N : 1;
 
     __shared__ double tws[TWN];
+    __shared__ double twLane[LANE ? 2 * TW_LANE_PER_DIR : 1];
     __shared__ double scratch[CPW][2 * SCR];
     // the specialised 8-bit-digit shapes may run the first one or two forward
     // stages by table lookup (fwd_first_stage / fwd_first_two_stages): fewer
@@ -77,7 +82,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (a.lwe_stride > 0 ? a.lwe_stride : n + 1);
     const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
 
-    if constexpr (C::TW_LDS) {
+    if constexpr (LANE) {
+        for (int j = static_cast<int>(threadIdx.x); j < 32; j += C::THREADS) tws[j] = a.tw[(j >> 4) * N + (j & 15)];
+        build_tw_lane(twLane, a.tw, static_cast<int>(threadIdx.x), C::THREADS);
+        build_tw_lane(twLane + TW_LANE_PER_DIR, a.tw + N, static_cast<int>(threadIdx.x), C::THREADS);
+    } else if constexpr (C::TW_LDS) {
         for (int j = static_cast<int>(threadIdx.x); j < TWN; j += C::THREADS) tws[j] = a.tw[j];
     }
     if constexpr (TAB > 0) build_first_stage_tables(firstTab, a.tw, a.prime[0], static_cast<int>(threadIdx.x), C::THREADS, TAB_ENTRIES);
@@ -187,8 +196,13 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 // in L1 after the first ciphertext of the workgroup)
                 double kbo[16], kbt[16];
                 const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + np) * N + tl;
-                const double* __restrict__ kpo = kp + static_cast<size_t>(tm) * NP * N;
-                const double* __restrict__ kpt = kp + static_cast<size_t>(1 - tm) * NP * N;
+                // global address space: through the key table the compiler
+                // cannot infer it and issues flat loads, which also count
+                // against the LDS wait counter, so every transform exchange
+                // would wait for these loads to land
+                using GD = const __attribute__((address_space(1))) double;
+                GD* __restrict__ kpo = (GD*)(kp + static_cast<size_t>(tm) * NP * N);
+                GD* __restrict__ kpt = (GD*)(kp + static_cast<size_t>(1 - tm) * NP * N);
                 if constexpr (NP == 1) {
 #pragma unroll
                     for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
@@ -197,7 +211,8 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 if constexpr (TAB == 2) {
                     WorkgroupSync<TW> sy;
                     fwd_first_two_stages(x, dg0, lvl, firstTab);
-                    ntt_forward_from_stage2<LOGN>(x, scr, tl, tw, pr, sy);
+                    if constexpr (LANE) ntt_forward_from_stage2_t<LOGN>(x, scr, tl, TwLane{tws, twLane}, pr, sy);
+                    else ntt_forward_from_stage2<LOGN>(x, scr, tl, tw, pr, sy);
                 } else if constexpr (TAB == 1) {
                     WorkgroupSync<TW> sy;
                     fwd_first_stage(x, dg0, lvl, firstTab);
@@ -222,18 +237,21 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
 #pragma unroll
                     for (int r = 0; r < 16; r++) kbo[r] = kpo[r * T];
                 }
-                // one column at a time: the scheduling barrier keeps the other
-                // column's loads from being hoisted over these products; the
-                // first level of an unrolled loop assigns instead of adding
+                // the other column's loads are issued before the own column's
+                // products so their latency overlaps them (1.9 % per launch
+                // over issuing them after); the scheduling barrier keeps the
+                // two columns' products apart, which bounds the live
+                // registers; the first level of an unrolled loop assigns
+                // instead of adding
                 const bool first = C::L > 0 && lvl == 0;
+#pragma unroll
+                for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const double pm = mulmod(x[r], kbo[r], pr);
                     own[r] = first ? pm : __dadd_rn(own[r], pm);
                 }
                 __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                for (int r = 0; r < 16; r++) kbt[r] = kpt[r * T];
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const double pm = mulmod(x[r], kbt[r], pr);
@@ -268,14 +286,22 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
             for (int r = 0; r < 16; r++) x[r] = __dadd_rn(own[r], other[r * T + tl]);
             __syncthreads();
 
-            ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+            if constexpr (LANE) {
+                WorkgroupSync<TW> sy;
+                ntt_inverse_sync_t<LOGN>(x, scr, tl, TwLane{tws + 16, twLane + TW_LANE_PER_DIR}, pr, sy);
+            } else {
+                ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+            }
             if constexpr (NP == 1) {
                 // centre mod q, reduce to the torus, accumulate (mod 2^w); the
                 // lazy-pipe step replaces the accumulator, and leaves it where
-                // abar = 0 (blind_rotate.cpp:893-895)
+                // abar = 0 (blind_rotate.cpp:893-895); centre() as two
+                // interleaved reductions
+                reduce16(x, pr);
+                reduce16(x, pr);
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    const double c = centre(x[r], pr);
+                    const double c = x[r];
                     if constexpr (C::LAZY) {
                         accr[r] = abar == 0 ? accr[r] : static_cast<TorusT>(f64_low32(c));
                     } else if constexpr (C::TBITS == 32) {
@@ -618,6 +644,7 @@ hipError_t launch_t32_b7x3_tab(int tab, const BrArgs& a, hipStream_t s) {
 }
 template <int CPW>
 hipError_t launch_t32_b7x3_twl(const BrTile& t, const BrArgs& a, hipStream_t s) {
+    if (t.twl == 2) return launch_br<BrCfg<32, 10, 1, CPW, true, 3, 7, 2, 2>>(a, s);  // lane tables: TAB = 2 only
     return t.twl ? launch_t32_b7x3_tab<CPW, 1>(t.tab, a, s) : launch_t32_b7x3_tab<CPW, 0>(t.tab, a, s);
 }
 hipError_t launch_t32_b7x3(const BrTile& t, const BrArgs& a, hipStream_t s) {
@@ -652,9 +679,11 @@ int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* o
     BrTile c[MAX_TILES];
     int n = 0;
     if (tbits == 32 && logn == 10 && level == 3 && radix_bits == 7) {
-        for (int cpw : {4, 2, 1})
+        for (int cpw : {4, 2, 1}) {
+            c[n++] = BrTile{cpw, 2, 0, 2};
             for (int twl : {1, 0})
                 for (int tab : {0, 1, 2}) c[n++] = BrTile{cpw, twl, 0, tab};
+        }
     } else if (tbits == 56 && logn == 11 && level == 3 && radix_bits == 10) {
         c[n++] = BrTile{1, 1};
         c[n++] = BrTile{1, 0};
