@@ -57,3 +57,24 @@ def test_batched_call_reports_barrier_failure_t56(p56, keys56):
         ctx.set_barrier_polls(DEFAULT_POLLS)
         out, _, _ = ctx.pbs_batch(kid, cts, v)
     assert np.array_equal(out, want)
+
+
+def test_batched_call_reports_handover_failure_t32(p32, keys32):
+    # the TORUS32 batched tile hands its partial sums between a ciphertext's
+    # two one-wave teams with a bounded wait on a count in LDS
+    bits = np.array([1, 0, 1, 1, 0])
+    cts = yg.lwe_encrypt(keys32.lwe_key, ((2 * bits - 1) * (1 << 29)).astype(np.int32), seed=37)
+    v = yg.sign_test_polynomial(p32)
+    orc = Oracle(p32, keys32)
+    want = orc.pbs_batch(cts, v)
+    orc.close()
+    with yg.Context(p32, 0) as ctx:
+        kid = ctx.upload_key(keys32)
+        ctx.set_tile_ex(2, 1, 2)   # the throughput tile even for a short batch
+        ctx.set_barrier_polls(0)
+        with pytest.raises(yg.YgpuError) as e:
+            ctx.pbs_batch(kid, cts, v)
+        assert e.value.code == yg.YGPU_ERR_DEVICE
+        ctx.set_barrier_polls(DEFAULT_POLLS)
+        out, _, _ = ctx.pbs_batch(kid, cts, v)
+    assert np.array_equal(out, want)

@@ -165,6 +165,50 @@ struct TeamBarrier {
     }
 };
 
+// Bounded wait of a wavefront until the LDS word *ctr reaches `target`
+// (another wave of the workgroup raises it).  `polls` bounds the wait as in
+// TeamBarrier: with no polls left the wave stores 1 into *fail and stops
+// waiting, and a set *fail ends every later wait at once (polls = 0 fails
+// without waiting).  The loop is one asm statement: as compiler-visible
+// control flow in the middle of the blind-rotation step it cost that kernel
+// a large register spill.  Its LDS reads complete inside the statement
+// (s_waitcnt lgkmcnt(0) before any use), and it writes only LDS.
+__device__ __forceinline__ void lds_wait_at_least(const uint32_t* ctr, uint32_t target, int polls, uint32_t* fail) {
+    using L3 = __attribute__((address_space(3))) const uint32_t;
+    using L3w = __attribute__((address_space(3))) uint32_t;
+    const uint32_t ca = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((L3*)ctr));
+    const uint32_t fa = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((L3w*)fail));
+    uint32_t v, sv, cnt;
+    asm volatile(
+        "s_mov_b32 %[cnt], %[polls]\n"
+        "1:\n"
+        "s_cmp_le_i32 %[cnt], 0\n"
+        "s_cbranch_scc1 2f\n"
+        "ds_read_b32 %[v], %[ca]\n"
+        "s_waitcnt lgkmcnt(0)\n"
+        "v_readfirstlane_b32 %[sv], %[v]\n"
+        "s_nop 4\n"
+        "s_cmp_ge_u32 %[sv], %[target]\n"
+        "s_cbranch_scc1 3f\n"
+        "ds_read_b32 %[v], %[fa]\n"
+        "s_waitcnt lgkmcnt(0)\n"
+        "v_readfirstlane_b32 %[sv], %[v]\n"
+        "s_nop 4\n"
+        "s_cmp_lg_u32 %[sv], 0\n"
+        "s_cbranch_scc1 3f\n"
+        "s_sub_i32 %[cnt], %[cnt], 1\n"
+        "s_sleep 1\n"
+        "s_branch 1b\n"
+        "2:\n"
+        "v_mov_b32 %[v], 1\n"
+        "ds_write_b32 %[fa], %[v]\n"
+        "s_waitcnt lgkmcnt(0)\n"
+        "3:\n"
+        : [v] "=&v"(v), [sv] "=&s"(sv), [cnt] "=&s"(cnt)
+        : [ca] "v"(ca), [fa] "v"(fa), [target] "s"(target), [polls] "s"(polls)
+        : "memory", "scc");
+}
+
 // A workgroup whose team barrier timed out reports it to the host: one lane
 // stores 1 into the launch's status word (host-visible memory; every failing
 // workgroup stores the same value, so no atomic is needed).

@@ -10,6 +10,7 @@
 //  k_extract        the extracted LWE alone (for callers that want it).
 //  k_ks_finalize    negation / body of the sliced keyswitch.
 #include "ygpu_kernels.h"
+#include <limits>
 
 namespace ygpu {
 
@@ -56,6 +57,13 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     // the second prime is processed (registers are the binding resource)
     constexpr int RESN = NP > 1 ? 2 * N : 1;
     __shared__ double resbuf[CPW][RESN];
+    // One-wave teams (N = 1024, one prime) hand their partial sums over with
+    // one workgroup barrier per step instead of two: a team counts its reads
+    // of the partner's values (readDone), and the partner waits for that
+    // count only where it next writes its scratch, an inverse transform pass
+    // later, by which time the reads have long completed.
+    constexpr bool HANDOFF = TW == 1 && NP == 1;
+    __shared__ uint32_t readDone[HANDOFF ? CPW * 2 + 1 : 1];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int first = grp * CPW;
@@ -90,6 +98,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
         for (int j = static_cast<int>(threadIdx.x); j < TWN; j += C::THREADS) tws[j] = a.tw[j];
     }
     if constexpr (TAB > 0) build_first_stage_tables(firstTab, a.tw, a.prime[0], static_cast<int>(threadIdx.x), C::THREADS, TAB_ENTRIES);
+    // the hand-over counts, then the failure flag of their bounded waits
+    if constexpr (HANDOFF) {
+        if (threadIdx.x < CPW * 2 + 1) readDone[threadIdx.x] = 0u;
+    }
+    uint32_t* const handFail = &readDone[HANDOFF ? CPW * 2 : 0];
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
     // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
     TorusT accr[16];
@@ -135,7 +148,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
             static_assert(C::TBITS == 32, "lazy form: TORUS32");
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                if constexpr (C::L > 0) digits_packed<32, C::L, C::RB>(static_cast<int32_t>(accr[r]), dg0[r], dg1[r]);
+                if constexpr (C::L > 0) digits_packed_add32<C::L, C::RB>(static_cast<uint32_t>(accr[r]), dg0[r], dg1[r]);
                 else digits_runtime_packed<32>(static_cast<int32_t>(accr[r]), L, rb, dg0[r], dg1[r]);
             }
         }
@@ -156,7 +169,9 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 const uint32_t av = static_cast<uint32_t>(stage[src]);
                 const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(accr[r]);
                 if constexpr (C::L > 0) {
-                    digits_packed<32, C::L, C::RB>(static_cast<int32_t>(rot), dg0[r], dg1[r]);
+                    // the carry chain as one 32-bit addition (2.3 % per
+                    // launch over the level loop of digits_packed)
+                    digits_packed_add32<C::L, C::RB>(rot, dg0[r], dg1[r]);
                 } else {
                     digits_runtime_packed<32>(static_cast<int32_t>(rot), L, rb, dg0[r], dg1[r]);
                 }
@@ -284,13 +299,39 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
             double x[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) x[r] = __dadd_rn(own[r], other[r * T + tl]);
-            __syncthreads();
-
-            if constexpr (LANE) {
+            if constexpr (HANDOFF) {
+                // the reads of the partner's scratch are complete: publish the
+                // step count (one writer per word: this team's wave)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                __hip_atomic_store(&readDone[lc * 2 + tm], static_cast<uint32_t>(i + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                // the inverse transform; before its first write to this
+                // team's scratch, wait until the partner has read it (bounded:
+                // a wait that runs out sets the failure flag, and no later
+                // wait blocks)
                 WorkgroupSync<TW> sy;
-                ntt_inverse_sync_t<LOGN>(x, scr, tl, TwLane{tws + 16, twLane + TW_LANE_PER_DIR}, pr, sy);
+                auto inverse = [&](const auto& twsrc) {
+                    reduce16(x, pr);
+                    inv_pass_t<LOGN, 2>(x, tl, twsrc, pr);
+                    lds_wait_at_least(&readDone[lc * 2 + 1 - tm], static_cast<uint32_t>(i + 1), a.barrier_polls, handFail);
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                    exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
+                    reduce16(x, pr);
+                    inv_pass_t<LOGN, 1>(x, tl, twsrc, pr);
+                    exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
+                    reduce16(x, pr);
+                    inv_pass_t<LOGN, 0>(x, tl, twsrc, pr);
+                };
+                if constexpr (LANE) inverse(TwLane{tws + 16, twLane + TW_LANE_PER_DIR});
+                else inverse(TwFlat{itw});
             } else {
-                ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+                __syncthreads();
+                if constexpr (LANE) {
+                    WorkgroupSync<TW> sy;
+                    ntt_inverse_sync_t<LOGN>(x, scr, tl, TwLane{tws + 16, twLane + TW_LANE_PER_DIR}, pr, sy);
+                } else {
+                    ntt_inverse<LOGN, TW>(x, scr, tl, itw, pr);
+                }
             }
             if constexpr (NP == 1) {
                 // centre mod q, reduce to the torus, accumulate (mod 2^w); the
@@ -329,10 +370,19 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
         }
     }
 
+    // a hand-over wait that ran out leaves results that are not proven: the
+    // accumulators read the most negative torus value instead, and the
+    // launch's status word reports it
+    bool failed = false;
+    if constexpr (HANDOFF) {
+        __syncthreads();
+        failed = __hip_atomic_load(handFail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+        report_barrier_failure(failed, a.status);
+    }
     if (live) {
         TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + tm * N;
 #pragma unroll
-        for (int r = 0; r < 16; r++) outAcc[tl + T * r] = accr[r];
+        for (int r = 0; r < 16; r++) outAcc[tl + T * r] = failed ? std::numeric_limits<TorusT>::min() : accr[r];
     }
 }
 
