@@ -131,40 +131,6 @@ struct WorkgroupSync {
     __device__ __forceinline__ void sync() { team_sync<TEAM_WAVES>(); }
 };
 
-// Barrier of the W wavefronts of one team through a counter in LDS, for teams
-// smaller than the workgroup: other teams do not wait.  Each wave's first lane
-// adds one after the wave's LDS writes have completed; the wave then waits
-// until the counter reaches W times the number of barriers it has passed.
-// Every team member passes the same sequence of barriers, so the counts agree.
-// The wait is bounded: a wave that has polled `polls` times without the count
-// arriving sets *fail, and from then on no barrier waits, so a miscount ends
-// in wrong results and a set flag, never in a hang (polls = 0 fails every
-// barrier at once).  The kernel reports the flag (report_barrier_failure).
-constexpr int TEAM_BARRIER_POLLS = 1 << 16;
-template <int W>
-struct TeamBarrier {
-    uint32_t* ctr;
-    uint32_t* fail;
-    uint32_t passed;
-    int polls;
-    __device__ __forceinline__ void sync() {
-        passed += W;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        if ((threadIdx.x & 63) == 0) __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        for (int poll = 0;; poll++) {
-            if (poll >= polls) {
-                __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                break;
-            }
-            if (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) >= passed) break;
-            if (__hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
-            __builtin_amdgcn_s_sleep(1);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
-};
-
 // Bounded wait of a wavefront until the LDS word *ctr reaches `target`
 // (another wave of the workgroup raises it).  `polls` bounds the wait as in
 // TeamBarrier: with no polls left the wave stores 1 into *fail and stops
@@ -208,6 +174,38 @@ __device__ __forceinline__ void lds_wait_at_least(const uint32_t* ctr, uint32_t 
         : [ca] "v"(ca), [fa] "v"(fa), [target] "s"(target), [polls] "s"(polls)
         : "memory", "scc");
 }
+
+// Barrier of the W wavefronts of one team through LDS, for teams smaller than
+// the workgroup: other teams do not wait.  Each wave of the team owns one
+// word, the number of barriers it has passed: it stores the new count (all
+// lanes store the same value, so no atomic is needed) once its earlier LDS
+// writes have completed, then waits until every other wave's word has
+// reached it.  Every team member passes the same sequence of barriers, so the
+// counts agree.  The waits are bounded (lds_wait_at_least): a wave that has
+// polled `polls` times without the counts arriving sets *fail, and from then
+// on no wait blocks, so a miscount ends in wrong results and a set flag, never
+// in a hang (polls = 0 fails every barrier at once).  The kernel reports the
+// flag (report_barrier_failure).
+constexpr int TEAM_BARRIER_POLLS = 1 << 16;
+template <int W>
+struct TeamBarrier {
+    uint32_t* words;  // the team's W words, one per wave
+    uint32_t* fail;
+    uint32_t passed;
+    int polls;
+    int me;           // this wave's index in the team (wave-uniform)
+    __device__ __forceinline__ void sync() {
+        passed += 1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __hip_atomic_store(&words[me], passed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int o = 1; o < W; o++) {
+            const int other = me + o < W ? me + o : me + o - W;
+            lds_wait_at_least(&words[other], passed, polls, fail);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
 
 // A workgroup whose team barrier timed out reports it to the host: one lane
 // stores 1 into the launch's status word (host-visible memory; every failing
@@ -601,16 +599,14 @@ struct DigitsAdd32 {
     }
 };
 
+// The carry-chain core of digits_packed_add32: m holds the biased magnitude
+// with the gadget's L windows at bits 32 - (l + 1) RB, s is all ones for a
+// negative value.
 template <int L, int RB>
-__device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, uint32_t& w1) {
-    constexpr int DROP = 32 - L * RB;
-    static_assert(DROP >= 0, "gadget wider than the torus");
-    constexpr uint32_t BIAS = DROP > 0 ? (1u << (DROP - 1)) : 0u;
+__device__ __forceinline__ void digits_add32_core(uint32_t m, uint32_t s, uint32_t& w0, uint32_t& w1) {
     constexpr int B = 1 << RB, HB = B >> 1;
     constexpr uint32_t K = DigitsAdd32<L, RB>::window_constant();
     constexpr uint32_t TM = DigitsAdd32<L, RB>::tie_mask();
-    const uint32_t s = static_cast<uint32_t>(static_cast<int32_t>(u) >> 31);  // all ones when negative
-    const uint32_t m = ((u ^ s) - s) + BIAS;
     const uint32_t t = (m >> (2 * RB - 1)) & TM;
     const uint32_t sum = m + K + t;
     const int sneg = static_cast<int>(s);
@@ -623,6 +619,29 @@ __device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, ui
         const int bal = static_cast<int>((sum >> shift) & static_cast<uint32_t>(B - 1)) - (HB - 1) - tl;
         pack_digit<RB>(lvl, (bal ^ sneg) - sneg, w0, w1);
     }
+}
+
+template <int L, int RB>
+__device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, uint32_t& w1) {
+    constexpr int DROP = 32 - L * RB;
+    static_assert(DROP >= 0, "gadget wider than the torus");
+    constexpr uint32_t BIAS = DROP > 0 ? (1u << (DROP - 1)) : 0u;
+    const uint32_t s = static_cast<uint32_t>(static_cast<int32_t>(u) >> 31);  // all ones when negative
+    digits_add32_core<L, RB>(((u ^ s) - s) + BIAS, s, w0, w1);
+}
+
+// The same digits of a TBITS-bit torus value (TBITS > 32, L RB <= 32): the
+// bias is added to the magnitude at full width, and only its top 32 bits,
+// which hold every window and tie bit, go through the 32-bit carry chain
+// (same digits as digits_packed<TBITS, L, RB>).
+template <int TBITS, int L, int RB>
+__device__ __forceinline__ void digits_packed_wide(int64_t x, uint32_t& w0, uint32_t& w1) {
+    constexpr int DROP = TBITS - L * RB;
+    static_assert(TBITS > 32 && TBITS <= 63 && L * RB <= 32 && DROP > 0, "wide torus, gadget within 32 bits");
+    const int64_t xs = sext_t<TBITS>(x);
+    const uint32_t s = static_cast<uint32_t>(static_cast<int32_t>(static_cast<uint64_t>(xs) >> 32) >> 31);
+    const uint64_t mag = static_cast<uint64_t>(xs < 0 ? -xs : xs) + (uint64_t{1} << (DROP - 1));
+    digits_add32_core<L, RB>(static_cast<uint32_t>(mag >> (TBITS - 32)), s, w0, w1);
 }
 
 // All L balanced digits (gadget_decomposition.h:75-95) of a value of a

@@ -40,9 +40,9 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // inverse twiddles are always read through L1
     constexpr bool FTW_LDS = C::TWL != 0;
     __shared__ double ftw[FTW_LDS ? NP : 1][FTW_LDS ? N : 1];
-    // barrier counters of the four two-wave teams of each ciphertext, and the
-    // flag a timed-out team barrier sets
-    __shared__ uint32_t teamCount[CPW * 4 + 1];
+    // barrier words of the four two-wave teams of each ciphertext (one per
+    // wave), and the flag a timed-out team barrier sets
+    __shared__ uint32_t teamCount[CPW * 4 * 2 + 1];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int first = grp * CPW;
@@ -59,14 +59,16 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     int64_t* acc = accbuf[lc][tm];
 
     const int n = a.n;
-    const int L = a.level;
+    // the gadget depth: compile-time for the specialised shapes (the level
+    // loop is then unrolled, and the digit fields have constant offsets)
+    const int L = C::L > 0 ? C::L : a.level;
     const int rb = a.radix_bits;
     const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
     const double* __restrict__ bsk = a.bsk_table[slot];
     const int32_t* __restrict__ lwe = a.lwe_in + static_cast<size_t>(ct) * (n + 1);
     const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
     const PrimeF64 pr = a.prime[tp];
-    if (threadIdx.x < CPW * 4 + 1) teamCount[threadIdx.x] = 0u;
+    if (threadIdx.x < CPW * 4 * 2 + 1) teamCount[threadIdx.x] = 0u;
     if constexpr (FTW_LDS) {
         for (int e = static_cast<int>(threadIdx.x); e < NP * N; e += C::THREADS)
             ftw[e / N][e % N] = a.tw[static_cast<size_t>(e / N) * 2 * N + e % N];
@@ -99,7 +101,9 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // team's two waves; hand-overs between teams keep workgroup barriers
     using TeamSync = typename std::conditional<TW == 2, TeamBarrier<2>, WorkgroupSync<TW>>::type;
     TeamSync ts{};
-    if constexpr (TW == 2) ts = TeamSync{&teamCount[lc * 4 + tt], &teamCount[CPW * 4], 0u, a.barrier_polls};
+    if constexpr (TW == 2)
+        ts = TeamSync{&teamCount[(lc * 4 + tt) * 2], &teamCount[CPW * 4 * 2], 0u, a.barrier_polls,
+                      __builtin_amdgcn_readfirstlane(tl / 64)};
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
     // the next step's rotation amount is loaded a step ahead, so no step
@@ -125,7 +129,8 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             const int64_t av = acc[src];
             const int64_t rot = sext_t<C::TBITS>((negs ? -av : av) - acc[j]);
             if constexpr (C::L > 0) {
-                digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
+                if constexpr (C::L * C::RB <= 32) digits_packed_wide<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
+                else digits_packed<C::TBITS, C::L, C::RB>(rot, dg0[r], dg1[r]);
             } else {
                 digits_runtime_packed<C::TBITS>(rot, L, rb, dg0[r], dg1[r]);
             }
@@ -168,9 +173,11 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
 
         const double* __restrict__ bskRow = bsk + static_cast<size_t>(i) * stepStride;
         double own[16], oth[16];
+        if constexpr (C::L == 0) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
-        for (int lvl = 0; lvl < L; lvl++) {
+            for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
+        }
+        auto level = [&](const int lvl) {
             const int d = lvl * K1 + tm;
             const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + tp) * N + tl;
             // global address space: the compiler cannot infer it through the
@@ -196,17 +203,25 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, ts);
             // one column at a time: the scheduling barrier keeps the second
             // column's loads from being hoisted over the first column's
-            // products, which would hold 32 more registers live
+            // products, which would hold 32 more registers live; the first
+            // level of an unrolled loop assigns instead of adding
+            const bool firstLvl = C::L > 0 && lvl == 0;
             double kb[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) kb[r] = kpo[r * T];
 #pragma unroll
-            for (int r = 0; r < 16; r++) own[r] = __dadd_rn(own[r], mulmod(x[r], kb[r], pr));
+            for (int r = 0; r < 16; r++) {
+                const double pm = mulmod(x[r], kb[r], pr);
+                own[r] = firstLvl ? pm : __dadd_rn(own[r], pm);
+            }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int r = 0; r < 16; r++) kb[r] = kpt[r * T];
 #pragma unroll
-            for (int r = 0; r < 16; r++) oth[r] = __dadd_rn(oth[r], mulmod(x[r], kb[r], pr));
+            for (int r = 0; r < 16; r++) {
+                const double pm = mulmod(x[r], kb[r], pr);
+                oth[r] = firstLvl ? pm : __dadd_rn(oth[r], pm);
+            }
             if constexpr (C::L == 0) {
                 // more than four levels: reduce the partial sums once after
                 // four, so that with the partner's they stay below 16p
@@ -215,6 +230,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                     for (int r = 0; r < 16; r++) { own[r] = reduce(own[r], pr); oth[r] = reduce(oth[r], pr); }
                 }
             }
+        };
+        if constexpr (C::L > 0) {
+#pragma unroll
+            for (int lvl = 0; lvl < C::L; lvl++) level(lvl);
+        } else {
+            for (int lvl = 0; lvl < L; lvl++) level(lvl);
         }
 
         // partner sums (same prime): hand column 1-tm to team (1-tm, tp)
@@ -265,7 +286,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // unproven: every accumulator of the workgroup reads INT64_MIN instead and
     // the launch's status word reports it (the step loop ends with a workgroup
     // barrier, so the flag is final here)
-    const bool failed = teamCount[CPW * 4] != 0u;
+    const bool failed = teamCount[CPW * 4 * 2] != 0u;
     report_barrier_failure(failed, a.status);
     if (live && tp == 0) {
         TorusT* __restrict__ outAcc = static_cast<TorusT*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + tm * N;

@@ -46,7 +46,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     __shared__ int32_t acc[K1][N];
     __shared__ int32_t abars[KS_MAX_N];  // the ciphertext's rotation amounts, mod-switched once
     __shared__ uint32_t digs[K1][N];      // packed digits of (X^a - 1) row r, all levels
-    __shared__ uint32_t rowCount[K1 + 1];  // digit-share barriers of each row's waves, then the failure flag
+    __shared__ uint32_t rowCount[K1 * L + 1];  // digit-share barrier words of each row's waves, then the failure flag
 
     const int ct = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     if (ct >= a.count) return;  // whole workgroup
@@ -55,8 +55,8 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     const int lvl = w % L;    // gadget level
     const int tl = static_cast<int>(threadIdx.x) % 64;
     double* scr = scratch[w];
-    if (threadIdx.x < K1 + 1) rowCount[threadIdx.x] = 0u;
-    TeamBarrier<L> rowSync{&rowCount[r], &rowCount[K1], 0u, a.barrier_polls};
+    if (threadIdx.x < K1 * L + 1) rowCount[threadIdx.x] = 0u;
+    TeamBarrier<L> rowSync{&rowCount[r * L], &rowCount[K1 * L], 0u, a.barrier_polls, lvl};
 
     const int n = a.n;
     const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     // unproven: the accumulators read INT32_MIN instead and the launch's
     // status word reports it (the step loop ends with a workgroup barrier, so
     // the flag is final here)
-    const bool failed = rowCount[K1] != 0u;
+    const bool failed = rowCount[K1 * L] != 0u;
     report_barrier_failure(failed, a.status);
     if (lvl == 0) {
         int32_t* __restrict__ outAcc = static_cast<int32_t*>(a.acc_out) + static_cast<size_t>(ct) * K1 * N + r * N;
