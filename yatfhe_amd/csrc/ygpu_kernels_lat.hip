@@ -46,6 +46,9 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     __shared__ int32_t acc[K1][N];
     __shared__ int32_t abars[KS_MAX_N];  // the ciphertext's rotation amounts, mod-switched once
     __shared__ uint32_t digs[K1][N];      // packed digits of (X^a - 1) row r, all levels
+    // the first two forward stages of a digit polynomial by table lookup
+    // (fwd_first_two_stages): w * d mod p for every signed-byte digit d
+    __shared__ double firstTab[FIRST_TABLE_ENTRIES];
     __shared__ uint32_t rowCount[K1 * L + 1];  // digit-share barrier words of each row's waves, then the failure flag
 
     const int ct = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
@@ -66,6 +69,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     const PrimeF64 pr = a.prime[0];
 
     for (int j = static_cast<int>(threadIdx.x); j < 2 * N; j += LAT_THREADS) tws[j] = a.tw[j];
+    build_first_stage_tables(firstTab, a.tw, pr, static_cast<int>(threadIdx.x), LAT_THREADS);
     for (int j = static_cast<int>(threadIdx.x); j < n; j += LAT_THREADS) abars[j] = a.prescaled ? lwe[j] : mod_switch_2n(lwe[j], LOGN + 1);
     // accumulator: caller's, or genNoiselessTrlweSample (trlwe.cpp:175-182)
     // after rescaleTlweToNewMod (tlwe.cpp:75-81): (0, X^{-bbar} v)
@@ -125,15 +129,17 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
         }
         // only the three waves of row r read its digits
         rowSync.sync();
+        // the first two forward stages of level lvl's digit polynomial by
+        // table lookup (1 % per plain call over computing them)
         double x[16];
+        {
+            uint32_t dg[16];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            // the signed byte of level lvl.  Not __builtin_amdgcn_sbfe: with
-            // this wave-uniform (non-constant) offset, hipcc converts its
-            // result to double as an unsigned value (v_cvt_f64_u32).
-            x[q] = static_cast<double>(static_cast<int8_t>(digs[r][tl + T * q] >> (8 * lvl)));
+            for (int q = 0; q < 16; q++) dg[q] = digs[r][tl + T * q];
+            fwd_first_two_stages(x, dg, lvl, firstTab);
+            WorkgroupSync<1> sy;
+            ntt_forward_from_stage2<LOGN>(x, scr, tl, tws, pr, sy);
         }
-        ntt_forward<LOGN, 1>(x, scr, tl, tws, pr);
         // products with BSK row (level lvl, input row r): column r and 1 - r
         double po[16], pt[16];
 #pragma unroll
@@ -172,11 +178,13 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             ntt_inverse<LOGN, 1>(s, scr, tl, tws + N, pr);
-            // centre mod q, reduce to the torus, accumulate (mod 2^32)
+            // centre mod q (two reductions), reduce to the torus, accumulate (mod 2^32)
+            reduce16(s, pr);
+            reduce16(s, pr);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int j = tl + T * q;
-                acc[r][j] = static_cast<int32_t>(static_cast<uint32_t>(acc[r][j]) + f64_low32(centre(s[q], pr)));
+                acc[r][j] = static_cast<int32_t>(static_cast<uint32_t>(acc[r][j]) + f64_low32(s[q]));
             }
         }
         __syncthreads();
