@@ -331,6 +331,44 @@ __device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* 
     inv_pass_t<LOGN, P>(x, tl, TwFlat{itw}, pr);
 }
 
+// The inverse transform's first pass for N = 1024 (stages 9 and 8) on inputs
+// |x| <= 8p, without the reduction of all sixteen values that a general input
+// needs first: stage 9's sums stay below 16p <= 2^53 (exact) and its
+// differences (below 16p) go through mulmod (below 2p); only the eight sums
+// are reduced (to p/2 + 1) before stage 8, after which every value is below 4p.
+// The external product's sums of six products of at most 1.32p (centred key
+// entries, forward outputs below 8.7p) are below 8p.
+template <class TW>
+__device__ __forceinline__ void inv_pass2_from_8p(double (&x)[16], int tl, const TW& itw, const PrimeF64& pr) {
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) {
+        const double w = itw.template get<10, 2>(9, tl, r);
+        const double U = x[r];
+        const double V = x[r + 1];
+        x[r] = __dadd_rn(U, V);
+        x[r + 1] = mulmod(__dadd_rn(U, -V), w, pr);
+    }
+#pragma unroll
+    for (int g = 0; g < 16; g += 8) {
+        double t[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[b] = __dmul_rn(x[g + 2 * b], pr.pinv);
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[b] = __builtin_rint(t[b]);
+#pragma unroll
+        for (int b = 0; b < 4; b++) x[g + 2 * b] = __fma_rn(-t[b], pr.p, x[g + 2 * b]);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        if (r & 2) continue;
+        const double w = itw.template get<10, 2>(8, tl, r);
+        const double U = x[r];
+        const double V = x[r + 2];
+        x[r] = __dadd_rn(U, V);
+        x[r + 2] = mulmod(__dadd_rn(U, -V), w, pr);
+    }
+}
+
 template <int LOGN, int PW, int PR, int X, class Sync>
 __device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl, Sync& sy) {
     sy.sync();

@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     using TorusT = typename C::TorusT;
     constexpr int LOGN = C::LOGN, N = C::N, T = C::T, K1 = 2, NP = C::NP, TW = C::TEAM_WAVES, CPW = C::CPW;
     constexpr int SCR = Scratch<LOGN>::DOUBLES;
+    static_assert(C::TBITS != 32 || 2 * N * sizeof(int32_t) <= SCR * sizeof(double), "the rotation stage [acc, -acc] fits the scratch");
     constexpr int CT_THREADS = 2 * T;
     // lane-major twiddle tables (TwLane): passes 1 and 2 read them; the flat
     // table then only holds pass 0's sixteen twiddles of each direction
@@ -152,7 +153,31 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 else digits_runtime_packed<32>(static_cast<int32_t>(accr[r]), L, rb, dg0[r], dg1[r]);
             }
         }
-        if constexpr (!C::LAZY) {
+        if constexpr (!C::LAZY && C::TBITS == 32) {
+        // the row and its negation side by side, G = [acc, -acc]: then
+        // (X^t acc)[j] = G[(j - t) mod 2N] for the full rotation amount
+        // t = abar mod 2N, with no per-position sign or wrap test
+        team_sync<TW>();
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            stage[tl + T * r] = accr[r];
+            stage[N + tl + T * r] = static_cast<TorusT>(0u - static_cast<uint32_t>(accr[r]));
+        }
+        team_sync<TW>();
+        const int base = tl - (abar & (2 * N - 1));
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const uint32_t av = static_cast<uint32_t>(stage[(base + T * r) & (2 * N - 1)]);
+            const uint32_t rot = av - static_cast<uint32_t>(accr[r]);
+            if constexpr (C::L > 0) {
+                // the carry chain as one 32-bit addition (2.3 % per
+                // launch over the level loop of digits_packed)
+                digits_packed_add32<C::L, C::RB>(rot, dg0[r], dg1[r]);
+            } else {
+                digits_runtime_packed<32>(static_cast<int32_t>(rot), L, rb, dg0[r], dg1[r]);
+            }
+        }
+        } else if constexpr (!C::LAZY) {
         team_sync<TW>();
 #pragma unroll
         for (int r = 0; r < 16; r++) stage[tl + T * r] = accr[r];
@@ -165,17 +190,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
             const int j = tl + T * r;
             const int src = (j - aT) & (N - 1);
             const bool negs = (j < aT) != flip;
-            if constexpr (C::TBITS == 32) {
-                const uint32_t av = static_cast<uint32_t>(stage[src]);
-                const uint32_t rot = (negs ? 0u - av : av) - static_cast<uint32_t>(accr[r]);
-                if constexpr (C::L > 0) {
-                    // the carry chain as one 32-bit addition (2.3 % per
-                    // launch over the level loop of digits_packed)
-                    digits_packed_add32<C::L, C::RB>(rot, dg0[r], dg1[r]);
-                } else {
-                    digits_runtime_packed<32>(static_cast<int32_t>(rot), L, rb, dg0[r], dg1[r]);
-                }
-            } else {
+            {
                 const int64_t av = static_cast<int64_t>(stage[src]);
                 const int64_t rot = sext_t<C::TBITS>((negs ? -av : av) - static_cast<int64_t>(accr[r]));
                 if constexpr (C::L > 0) {
@@ -311,8 +326,13 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 // wait blocks)
                 WorkgroupSync<TW> sy;
                 auto inverse = [&](const auto& twsrc) {
-                    reduce16(x, pr);
-                    inv_pass_t<LOGN, 2>(x, tl, twsrc, pr);
+                    if constexpr (LOGN == 10 && C::L > 0 && C::L <= 3) {
+                        // sums of at most six products: below 8p
+                        inv_pass2_from_8p(x, tl, twsrc, pr);
+                    } else {
+                        reduce16(x, pr);
+                        inv_pass_t<LOGN, 2>(x, tl, twsrc, pr);
+                    }
                     lds_wait_at_least(&readDone[lc * 2 + 1 - tm], static_cast<uint32_t>(i + 1), a.barrier_polls, handFail);
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                     exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
