@@ -513,7 +513,7 @@ __device__ __forceinline__ void ntt_inverse(double (&x)[16], double* scr, int tl
 // ---------------------------------------------------------------------------
 // Torus helpers.
 template <int TBITS>
-__device__ __forceinline__ int64_t sext_t(int64_t x) {
+__host__ __device__ __forceinline__ int64_t sext_t(int64_t x) {
     return static_cast<int64_t>(static_cast<uint64_t>(x) << (64 - TBITS)) >> (64 - TBITS);
 }
 
@@ -604,7 +604,7 @@ __device__ __forceinline__ uint32_t f64_low32(double v) {  // |v| < 2^51: v mod 
 // digit in [-B/2, B/2] fits a signed byte) into w0, or 16-bit fields into w0
 // (levels 0, 1) and w1 (levels 2, 3).
 template <int RB>
-__device__ __forceinline__ void pack_digit(int lvl, int sd, uint32_t& w0, uint32_t& w1) {
+__host__ __device__ __forceinline__ void pack_digit(int lvl, int sd, uint32_t& w0, uint32_t& w1) {
     if constexpr (RB <= 7) {
         w0 |= (static_cast<uint32_t>(sd) & 0xFFu) << (8 * lvl);
     } else {
@@ -637,11 +637,13 @@ struct DigitsAdd32 {
     }
 };
 
+// (The digit functions from here to digits_packed_wide are also compiled for
+// the host, where tests compare them with the oracle's decomposition.)
 // The carry-chain core of digits_packed_add32: m holds the biased magnitude
 // with the gadget's L windows at bits 32 - (l + 1) RB, s is all ones for a
 // negative value.
 template <int L, int RB>
-__device__ __forceinline__ void digits_add32_core(uint32_t m, uint32_t s, uint32_t& w0, uint32_t& w1) {
+__host__ __device__ __forceinline__ void digits_add32_core(uint32_t m, uint32_t s, uint32_t& w0, uint32_t& w1) {
     constexpr int B = 1 << RB, HB = B >> 1;
     constexpr uint32_t K = DigitsAdd32<L, RB>::window_constant();
     constexpr uint32_t TM = DigitsAdd32<L, RB>::tie_mask();
@@ -660,7 +662,7 @@ __device__ __forceinline__ void digits_add32_core(uint32_t m, uint32_t s, uint32
 }
 
 template <int L, int RB>
-__device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, uint32_t& w1) {
+__host__ __device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, uint32_t& w1) {
     constexpr int DROP = 32 - L * RB;
     static_assert(DROP >= 0, "gadget wider than the torus");
     constexpr uint32_t BIAS = DROP > 0 ? (1u << (DROP - 1)) : 0u;
@@ -673,7 +675,7 @@ __device__ __forceinline__ void digits_packed_add32(uint32_t u, uint32_t& w0, ui
 // which hold every window and tie bit, go through the 32-bit carry chain
 // (same digits as digits_packed<TBITS, L, RB>).
 template <int TBITS, int L, int RB>
-__device__ __forceinline__ void digits_packed_wide(int64_t x, uint32_t& w0, uint32_t& w1) {
+__host__ __device__ __forceinline__ void digits_packed_wide(int64_t x, uint32_t& w0, uint32_t& w1) {
     constexpr int DROP = TBITS - L * RB;
     static_assert(TBITS > 32 && TBITS <= 63 && L * RB <= 32 && DROP > 0, "wide torus, gadget within 32 bits");
     const int64_t xs = sext_t<TBITS>(x);
