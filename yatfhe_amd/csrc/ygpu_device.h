@@ -331,6 +331,65 @@ __device__ __forceinline__ void inv_pass(double (&x)[16], int tl, const double* 
     inv_pass_t<LOGN, P>(x, tl, TwFlat{itw}, pr);
 }
 
+// A four-stage Gentleman-Sande pass on inputs |x| <= 4p (+ a few units: the
+// outputs of a pass that began reduced) without first reducing all sixteen:
+// the first stage's sums (below 8p) are reduced after it, its differences go
+// through mulmod (below 1.25p), and the worst case over the registers after
+// the three later stages is 10p (register 1), so every value stays below
+// 16p <= 2^53.  Eight reductions instead of sixteen.
+template <int LOGN, int P, class TW>
+__device__ __forceinline__ void inv_pass_from_4p(double (&x)[16], int tl, const TW& itw, const PrimeF64& pr) {
+    using PS = Pass<LOGN, P>;
+    static_assert(PS::S == 4, "a four-stage pass");
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) {
+        const double w = itw.template get<LOGN, P>(PS::S0 + PS::S - 1, tl, r);
+        const double U = x[r];
+        const double V = x[r + 1];
+        x[r] = __dadd_rn(U, V);
+        x[r + 1] = mulmod(__dadd_rn(U, -V), w, pr);
+    }
+#pragma unroll
+    for (int g = 0; g < 16; g += 8) {
+        double t[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[b] = __dmul_rn(x[g + 2 * b], pr.pinv);
+#pragma unroll
+        for (int b = 0; b < 4; b++) t[b] = __builtin_rint(t[b]);
+#pragma unroll
+        for (int b = 0; b < 4; b++) x[g + 2 * b] = __fma_rn(-t[b], pr.p, x[g + 2 * b]);
+    }
+#pragma unroll
+    for (int ls = PS::S - 2; ls >= 0; ls--) {
+        const int st = PS::S0 + ls;
+        const int half = 1 << (PS::S - 1 - ls);
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            if (r & half) continue;
+            const double w = itw.template get<LOGN, P>(st, tl, r);
+            const double U = x[r];
+            const double V = x[r + half];
+            x[r] = __dadd_rn(U, V);
+            x[r + half] = mulmod(__dadd_rn(U, -V), w, pr);
+        }
+    }
+}
+
+// Exact centred representatives of the outputs of an inverse transform whose
+// last pass (four stages) began with every value reduced (|x| <= p/2 + 2).
+// Register r's worst case after that pass is 8p for r = 0, 4.75p for r = 1
+// and at most 2.75p for the others.  One reduction is exact below 4p: with
+// p < 2^49, the computed x * fl(1/p) is within 2^-51 (from fl(1/p)) plus half
+// an ulp of a value below 4, 2^-52, of x/p, which is less than x/p's distance
+// 1/(2p) > 2^-50 from the nearest half-integer (x is an integer, p odd), so
+// its rint is the nearest integer to x/p.  Registers 0 and 1 get centre()'s
+// second reduction.
+__device__ __forceinline__ void centre16_after_inverse(double (&x)[16], const PrimeF64& pr) {
+    reduce16(x, pr);
+    x[0] = reduce(x[0], pr);
+    x[1] = reduce(x[1], pr);
+}
+
 // The inverse transform's first pass for N = 1024 (stages 9 and 8) on inputs
 // |x| <= 8p, without the reduction of all sixteen values that a general input
 // needs first: stage 9's sums stay below 16p <= 2^53 (exact) and its
@@ -487,15 +546,16 @@ __device__ __forceinline__ void ntt_forward_from_stage2(double (&x)[16], double*
 }
 
 // Inverse negacyclic NTT without the 1/N factor (folded into the key):
-// pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Values are reduced
-// at the start of every pass, which bounds them by 8.8p throughout.
+// pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Passes 2 and 0
+// begin with every value reduced; pass 2's outputs are then below 4p (at most
+// three stages), so pass 1 reduces only its first stage's sums
+// (inv_pass_from_4p).  Every value stays below 10p; the outputs are below 8p.
 template <int LOGN, class Sync, class TW>
 __device__ __forceinline__ void ntt_inverse_sync_t(double (&x)[16], double* scr, int tl, const TW& itw, const PrimeF64& pr, Sync& sy) {
     reduce16(x, pr);
     inv_pass_t<LOGN, 2>(x, tl, itw, pr);
     exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
-    reduce16(x, pr);
-    inv_pass_t<LOGN, 1>(x, tl, itw, pr);
+    inv_pass_from_4p<LOGN, 1>(x, tl, itw, pr);
     exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
     reduce16(x, pr);
     inv_pass_t<LOGN, 0>(x, tl, itw, pr);

@@ -336,8 +336,8 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     lds_wait_at_least(&readDone[lc * 2 + 1 - tm], static_cast<uint32_t>(i + 1), a.barrier_polls, handFail);
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                     exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
-                    reduce16(x, pr);
-                    inv_pass_t<LOGN, 1>(x, tl, twsrc, pr);
+                    // pass 2's outputs are below 4p
+                    inv_pass_from_4p<LOGN, 1>(x, tl, twsrc, pr);
                     exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
                     reduce16(x, pr);
                     inv_pass_t<LOGN, 0>(x, tl, twsrc, pr);
@@ -356,10 +356,8 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
             if constexpr (NP == 1) {
                 // centre mod q, reduce to the torus, accumulate (mod 2^w); the
                 // lazy-pipe step replaces the accumulator, and leaves it where
-                // abar = 0 (blind_rotate.cpp:893-895); centre() as two
-                // interleaved reductions
-                reduce16(x, pr);
-                reduce16(x, pr);
+                // abar = 0 (blind_rotate.cpp:893-895)
+                centre16_after_inverse(x, pr);
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const double c = x[r];

@@ -258,8 +258,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         // residues of column tm mod both primes: the two prime teams of row tm
         // split the reconstruction, team tp taking registers [8 tp, 8 tp + 8);
         // each hands the other team the half it needs
-#pragma unroll
-        for (int r = 0; r < 16; r++) x[r] = centre(x[r], pr);
+        centre16_after_inverse(x, pr);
         ts.sync();
 #pragma unroll
         for (int r = 0; r < 8; r++) {

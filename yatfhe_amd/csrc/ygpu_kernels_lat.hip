@@ -178,9 +178,8 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             ntt_inverse<LOGN, 1>(s, scr, tl, tws + N, pr);
-            // centre mod q (two reductions), reduce to the torus, accumulate (mod 2^32)
-            reduce16(s, pr);
-            reduce16(s, pr);
+            // centre mod q, reduce to the torus, accumulate (mod 2^32)
+            centre16_after_inverse(s, pr);
 #pragma unroll
             for (int q = 0; q < 16; q++) {
                 const int j = tl + T * q;
