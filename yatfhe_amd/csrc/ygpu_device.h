@@ -259,6 +259,21 @@ struct TwLane {
         }
     }
 };
+// Pass 0's twiddles held in registers (wave-uniform values, so scalar
+// registers), loaded once per launch; the other passes read `base`.  In pass 0
+// lane tl's register r holds position j = tl + T r with T = N/16 > tl, so the
+// stage-st twiddle index (1 << st) + (j >> (LOGN - st)) is (1 << st) + (r >> (4 - st)).
+template <class BASE>
+struct TwPass0Reg {
+    double f[16];  // the direction's flat table entries 0..15
+    BASE base;
+    template <int LOGN, int P>
+    __device__ __forceinline__ double get(int st, int tl, int r) const {
+        if constexpr (P == 0) return f[(1 << st) + (r >> (4 - st))];
+        else return base.template get<LOGN, P>(st, tl, r);
+    }
+};
+
 // Fills one direction's lane tables from its flat table (all threads of the
 // workgroup take part; the caller synchronises before use).
 __device__ __forceinline__ void build_tw_lane(double* lane, const double* flat, int tid, int nthreads) {
@@ -442,13 +457,17 @@ __device__ __forceinline__ void exchange(double (&x)[16], double* scr, int tl, S
 // in pass-2 layout.  Inputs are digits (|x| <= 2^15); no reduction is needed:
 // a stage adds at most p(0.5 + 1.5|x| 2^-53) to the bound (mulmod above), so
 // after 10 (11) stages every value is below 7.75p (8.98p) < 16p <= 2^53.
+template <int LOGN, class Sync, class TW>
+__device__ __forceinline__ void ntt_forward_sync_t(double (&x)[16], double* scr, int tl, const TW& tw, const PrimeF64& pr, Sync& sy) {
+    fwd_pass_t<LOGN, 0>(x, tl, tw, pr);
+    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
+    fwd_pass_t<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
+    fwd_pass_t<LOGN, 2>(x, tl, tw, pr);
+}
 template <int LOGN, class Sync>
 __device__ __forceinline__ void ntt_forward_sync(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr, Sync& sy) {
-    fwd_pass<LOGN, 0>(x, tl, tw, pr);
-    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
-    fwd_pass<LOGN, 1>(x, tl, tw, pr);
-    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
-    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+    ntt_forward_sync_t<LOGN>(x, scr, tl, TwFlat{tw}, pr, sy);
 }
 template <int LOGN, int TEAM_WAVES>
 __device__ __forceinline__ void ntt_forward(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr) {

@@ -126,6 +126,20 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     }
     __syncthreads();
     const double* twBase = C::TW_LDS ? static_cast<const double*>(tws) : a.tw;
+    // pass 0's twiddles of both directions stay in (scalar) registers for the
+    // whole launch instead of being read from LDS at every step
+    constexpr bool P0REG = !LANE && NP == 1 && C::TW_LDS;
+    TwPass0Reg<TwFlat> fw0{}, iw0{};
+    if constexpr (P0REG) {
+        const double* __restrict__ g = a.tw;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            fw0.f[e] = g[e];
+            iw0.f[e] = g[N + e];
+        }
+        fw0.base = TwFlat{twBase};
+        iw0.base = TwFlat{twBase + N};
+    }
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
     // the next step's rotation amount is loaded a step ahead, so no step
@@ -242,6 +256,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     WorkgroupSync<TW> sy;
                     fwd_first_two_stages(x, dg0, lvl, firstTab);
                     if constexpr (LANE) ntt_forward_from_stage2_t<LOGN>(x, scr, tl, TwLane{tws, twLane}, pr, sy);
+                    else if constexpr (P0REG) ntt_forward_from_stage2_t<LOGN>(x, scr, tl, fw0, pr, sy);
                     else ntt_forward_from_stage2<LOGN>(x, scr, tl, tw, pr, sy);
                 } else if constexpr (TAB == 1) {
                     WorkgroupSync<TW> sy;
@@ -343,6 +358,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     inv_pass_t<LOGN, 0>(x, tl, twsrc, pr);
                 };
                 if constexpr (LANE) inverse(TwLane{tws + 16, twLane + TW_LANE_PER_DIR});
+                else if constexpr (P0REG) inverse(iw0);
                 else inverse(TwFlat{itw});
             } else {
                 __syncthreads();

@@ -75,6 +75,21 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     }
     const double* tw = FTW_LDS ? static_cast<const double*>(ftw[tp]) : a.tw + static_cast<size_t>(tp) * 2 * N;
     const double* __restrict__ itw = a.tw + static_cast<size_t>(tp) * 2 * N + N;
+    // pass 0's twiddles of this team's prime, both directions, stay in
+    // (scalar) registers for the whole launch (the specialised shapes; the
+    // run-time shapes have no scalar registers to spare); the prime index is
+    // wave-uniform, which readfirstlane makes visible to the compiler
+    TwPass0Reg<TwFlat> fw0{}, iw0{};
+    {
+        const double* __restrict__ g = a.tw + static_cast<size_t>(__builtin_amdgcn_readfirstlane(tp)) * 2 * N;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            fw0.f[e] = g[e];
+            iw0.f[e] = g[N + e];
+        }
+        fw0.base = TwFlat{tw};
+        iw0.base = TwFlat{itw};
+    }
 
     // rescaleTlweToNewMod (tlwe.cpp:75-81) of the body, then
     // genNoiselessTrlweSample (trlwe.cpp:175-182): acc = (0, X^{-bbar} v).
@@ -200,7 +215,8 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                 }
                 x[r] = static_cast<double>(dv);
             }
-            ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, ts);
+            if constexpr (C::L > 0) ntt_forward_sync_t<LOGN>(x, scr, tl, fw0, pr, ts);
+            else ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, ts);
             // one column at a time: the scheduling barrier keeps the second
             // column's loads from being hoisted over the first column's
             // products, which would hold 32 more registers live; the first
@@ -253,7 +269,8 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         // rewritten (the transform's own barriers order only this team)
         __syncthreads();
 
-        ntt_inverse_sync<LOGN>(x, scr, tl, itw, pr, ts);
+        if constexpr (C::L > 0) ntt_inverse_sync_t<LOGN>(x, scr, tl, iw0, pr, ts);
+        else ntt_inverse_sync<LOGN>(x, scr, tl, itw, pr, ts);
 
         // residues of column tm mod both primes: the two prime teams of row tm
         // split the reconstruction, team tp taking registers [8 tp, 8 tp + 8);

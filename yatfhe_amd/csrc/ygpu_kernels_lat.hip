@@ -69,6 +69,19 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     const PrimeF64 pr = a.prime[0];
 
     for (int j = static_cast<int>(threadIdx.x); j < 2 * N; j += LAT_THREADS) tws[j] = a.tw[j];
+    // pass 0's twiddles of both directions stay in (scalar) registers for the
+    // whole launch
+    TwPass0Reg<TwFlat> fw0{}, iw0{};
+    {
+        const double* __restrict__ g = a.tw;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            fw0.f[e] = g[e];
+            iw0.f[e] = g[N + e];
+        }
+        fw0.base = TwFlat{tws};
+        iw0.base = TwFlat{tws + N};
+    }
     build_first_stage_tables(firstTab, a.tw, pr, static_cast<int>(threadIdx.x), LAT_THREADS);
     for (int j = static_cast<int>(threadIdx.x); j < n; j += LAT_THREADS) abars[j] = a.prescaled ? lwe[j] : mod_switch_2n(lwe[j], LOGN + 1);
     // accumulator: caller's, or genNoiselessTrlweSample (trlwe.cpp:175-182)
@@ -138,7 +151,7 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             for (int q = 0; q < 16; q++) dg[q] = digs[r][tl + T * q];
             fwd_first_two_stages(x, dg, lvl, firstTab);
             WorkgroupSync<1> sy;
-            ntt_forward_from_stage2<LOGN>(x, scr, tl, tws, pr, sy);
+            ntt_forward_from_stage2_t<LOGN>(x, scr, tl, fw0, pr, sy);
         }
         // products with BSK row (level lvl, input row r): column r and 1 - r
         double po[16], pt[16];
@@ -177,7 +190,10 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            ntt_inverse<LOGN, 1>(s, scr, tl, tws + N, pr);
+            {
+                WorkgroupSync<1> isy;
+                ntt_inverse_sync_t<LOGN>(s, scr, tl, iw0, pr, isy);
+            }
             // centre mod q, reduce to the torus, accumulate (mod 2^32)
             centre16_after_inverse(s, pr);
 #pragma unroll
