@@ -128,8 +128,9 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
     const double* twBase = C::TW_LDS ? static_cast<const double*>(tws) : a.tw;
     // pass 0's twiddles of both directions stay in (scalar) registers for the
     // whole launch instead of being read from LDS at every step
-    constexpr bool P0REG = !LANE && NP == 1 && C::TW_LDS;
+    constexpr bool P0REG = NP == 1 && C::TW_LDS;
     TwPass0Reg<TwFlat> fw0{}, iw0{};
+    TwPass0Reg<TwLane> fl0{}, il0{};
     if constexpr (P0REG) {
         const double* __restrict__ g = a.tw;
 #pragma unroll
@@ -139,6 +140,15 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
         }
         fw0.base = TwFlat{twBase};
         iw0.base = TwFlat{twBase + N};
+        if constexpr (LANE) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                fl0.f[e] = fw0.f[e];
+                il0.f[e] = iw0.f[e];
+            }
+            fl0.base = TwLane{tws, twLane};
+            il0.base = TwLane{tws + 16, twLane + TW_LANE_PER_DIR};
+        }
     }
 
     const size_t stepStride = static_cast<size_t>(L) * K1 * K1 * NP * N;  // doubles per BSK row
@@ -255,7 +265,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                 if constexpr (TAB == 2) {
                     WorkgroupSync<TW> sy;
                     fwd_first_two_stages(x, dg0, lvl, firstTab);
-                    if constexpr (LANE) ntt_forward_from_stage2_t<LOGN>(x, scr, tl, TwLane{tws, twLane}, pr, sy);
+                    if constexpr (LANE) ntt_forward_from_stage2_t<LOGN>(x, scr, tl, fl0, pr, sy);
                     else if constexpr (P0REG) ntt_forward_from_stage2_t<LOGN>(x, scr, tl, fw0, pr, sy);
                     else ntt_forward_from_stage2<LOGN>(x, scr, tl, tw, pr, sy);
                 } else if constexpr (TAB == 1) {
@@ -357,7 +367,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     reduce16(x, pr);
                     inv_pass_t<LOGN, 0>(x, tl, twsrc, pr);
                 };
-                if constexpr (LANE) inverse(TwLane{tws + 16, twLane + TW_LANE_PER_DIR});
+                if constexpr (LANE) inverse(il0);
                 else if constexpr (P0REG) inverse(iw0);
                 else inverse(TwFlat{itw});
             } else {
