@@ -249,6 +249,7 @@ def leg_t56_batch(yg, args, peak):
     with yg.Context(p, 0) as ctx:
         kid = ctx.upload_key(keys)
         cpw, twl, tune_ms = ctx.autotune(kid, BATCH_T56)
+        tab = ctx.tile_ex()[2]
         d_lwe = ctx.alloc(cts.nbytes)
         d_lwe.upload(cts)
         d_v = ctx.alloc(v.nbytes)
@@ -295,7 +296,7 @@ def leg_t56_batch(yg, args, peak):
         "workload": "TORUS56 n=742 k=1 N=2048 gadget 2^10x3 KS 2^2x8, 8192 ciphertexts of uniform 4-bit messages "
                     "(seed 3), random 16-entry LUT (seed 2), one key (seed 0)",
         "value": round(value, 1), "unit": "PBS/s", "steps": steps, "ms_per_step": round(dt / steps * 1e3, 2),
-        "tile": {"cpw": cpw, "twl": twl, "autotune_best_ms": round(tune_ms, 2)},
+        "tile": {"cpw": cpw, "twl": twl, "tab": tab, "autotune_best_ms": round(tune_ms, 2)},
         "roofline": {"bound": "fp64-valu", "achieved": round(mm * BATCH_T56 / br_s / 1e9, 2),
                      "peak": round(MODMUL_CEILING / 1e9, 2), "unit": "Gmodmul/s",
                      "frac": round(mm * BATCH_T56 / br_s / MODMUL_CEILING, 4),
@@ -752,7 +753,7 @@ def main():
         detail["t56_n2048_b8192"] = r
         summaries["t56_n2048_b8192"] = {"value": r["value"], "ms_per_step": r["ms_per_step"],
                                         "frac": r["roofline"]["frac"], "avg_launch_ms": r["roofline"]["avg_launch_ms"],
-                                        "tile": [r["tile"]["cpw"], r["tile"]["twl"]], "lut_decrypt_check": r["lut_decrypt_check"],
+                                        "tile": [r["tile"]["cpw"], r["tile"]["twl"], r["tile"]["tab"]], "lut_decrypt_check": r["lut_decrypt_check"],
                                         "plain_p50_ms": r["plain"]["p50_ms"], "plain_host_p50_ms": r["plain"]["host_p50_ms"],
                                         "cpu_value": None if r["cpu_baseline"] is None else r["cpu_baseline"]["value"]}
     if "multitenant" in legs:

@@ -118,18 +118,26 @@ def test_t56_tiles_bit_identical(p56, keys56):
     with yg.Context(p56, 0) as ctx:
         kid = ctx.upload_key(keys56)
         assert ctx.tile() == (1, 1)
-        # forward twiddles in LDS (1) or read through L1 (0)
-        ctx.set_tile(1, 1)
+        assert ctx.tile_ex() == (1, 1, 1)
+        # forward twiddles in LDS (1) or read through L1 (0), stage 0 from the
+        # digit table (1) or computed (0)
+        ctx.set_tile_ex(1, 1, 0)
         a, _, _ = ctx.pbs_batch(kid, cts, v)
-        ctx.set_tile(1, 0)
-        a0, _, _ = ctx.pbs_batch(kid, cts, v)
-        c0 = np.stack([ctx.pbs(kid, cts[i], v)[0] for i in range(len(cts))])
+        others = []
+        for twl, tab in [(1, 1), (0, 1), (0, 0)]:
+            ctx.set_tile_ex(1, twl, tab)
+            others.append(ctx.pbs_batch(kid, cts, v)[0])
+            # a plain call per ciphertext runs the same kernel with a one-ciphertext grid
+            others.append(np.stack([ctx.pbs(kid, cts[i], v)[0] for i in range(len(cts))]))
         cpw, twl, ms = ctx.autotune(kid, 64)
         assert (cpw, twl) in [(1, 1), (1, 0)] and ms > 0
-        b, _, _ = ctx.pbs_batch(kid, cts, v)
-        # a plain call per ciphertext runs the same kernel with a one-ciphertext grid
-        c = np.stack([ctx.pbs(kid, cts[i], v)[0] for i in range(len(cts))])
-    for other in (a0, c0, b, c):
+        assert len(ctx.autotune_results()) == 4
+        others.append(ctx.pbs_batch(kid, cts, v)[0])
+        others.append(np.stack([ctx.pbs(kid, cts[i], v)[0] for i in range(len(cts))]))
+    for other in others:
         assert np.array_equal(a, other)
+    orc = Oracle(p56, keys56)
+    assert np.array_equal(a[:2], orc.pbs_batch(cts[:2], v))
+    orc.close()
     ph = yg.lwe_decrypt_phase(a, keys56.lwe_key).astype(np.int64)
     assert np.array_equal(np.round(ph / (1 << 32) * 16).astype(np.int64) % 16, lut[msgs] % 16)

@@ -539,14 +539,28 @@ __device__ __forceinline__ void fwd_first_stage(double (&x)[16], const uint32_t 
 }
 
 // The forward transform after fwd_first_stage.
+template <int LOGN, class Sync, class TW>
+__device__ __forceinline__ void ntt_forward_from_stage1_t(double (&x)[16], double* scr, int tl, const TW& tw, const PrimeF64& pr, Sync& sy) {
+    fwd_pass_t<LOGN, 0, 1>(x, tl, tw, pr);
+    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
+    fwd_pass_t<LOGN, 1>(x, tl, tw, pr);
+    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
+    fwd_pass_t<LOGN, 2>(x, tl, tw, pr);
+}
 template <int LOGN, class Sync>
 __device__ __forceinline__ void ntt_forward_from_stage1(double (&x)[16], double* scr, int tl, const double* __restrict__ tw, const PrimeF64& pr,
                                                         Sync& sy) {
-    fwd_pass<LOGN, 0, 1>(x, tl, tw, pr);
-    exchange<LOGN, 0, 1, 0>(x, scr, tl, sy);
-    fwd_pass<LOGN, 1>(x, tl, tw, pr);
-    exchange<LOGN, 1, 2, 1>(x, scr, tl, sy);
-    fwd_pass<LOGN, 2>(x, tl, tw, pr);
+    ntt_forward_from_stage1_t<LOGN>(x, scr, tl, TwFlat{tw}, pr, sy);
+}
+
+// Stage 0 by lookup for wide digits (|d| <= H, H = B/2 up to 2^15): tab[d + H]
+// = w1 d mod p, centred (|entry| <= p/2 + 1), w1 = tw[1] the stage-0 twiddle.
+// x[r] = a + tab[c + H], x[r + 8] = a - tab[c + H] for the digits a, c of
+// registers r, r + 8 (r < 8).  The values differ from the computed stage only
+// by multiples of p, and stay within the same bounds (|x| <= p/2 + 1 + H).
+__device__ __forceinline__ void build_stage0_table_wide(double* tab, int entries, int half, double w1, const PrimeF64& pr, int tid,
+                                                        int nthreads) {
+    for (int e = tid; e < entries; e += nthreads) tab[e] = reduce(mulmod(static_cast<double>(e - half), w1, pr), pr);
 }
 
 // The forward transform after fwd_first_two_stages.

@@ -431,9 +431,11 @@ inline int run_br(ygpu_ctx* ctx, const BrTile& tile, const BrArgs& a, hipStream_
     return YGPU_OK;
 }
 
-// One ciphertext per workgroup, on the latency form where it is instantiated.
+// One ciphertext per workgroup, on the latency form where it is instantiated
+// (elsewhere with the current tile's twiddle and digit-table choice).
 inline BrTile plain_tile(const ygpu_ctx* ctx) {
-    return BrTile{1, ctx->tile.twl, blind_rotate_lat_supported(ctx->tbits, ctx->logn, ctx->p.level, ctx->p.radix_bits) ? 1 : 0};
+    const int lat = blind_rotate_lat_supported(ctx->tbits, ctx->logn, ctx->p.level, ctx->p.radix_bits) ? 1 : 0;
+    return BrTile{1, ctx->tile.twl, lat, lat ? 0 : ctx->tile.tab};
 }
 // Tile of a batch of `count` ciphertexts: a batch no larger than the number of
 // compute units gives every ciphertext a workgroup of its own on the plain

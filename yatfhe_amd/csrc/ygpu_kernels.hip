@@ -779,8 +779,8 @@ int blind_rotate_tiles(int tbits, int logn, int level, int radix_bits, BrTile* o
                 for (int tab : {0, 1, 2}) c[n++] = BrTile{cpw, twl, 0, tab};
         }
     } else if (tbits == 56 && logn == 11 && level == 3 && radix_bits == 10) {
-        c[n++] = BrTile{1, 1};
-        c[n++] = BrTile{1, 0};
+        for (int tab : {1, 0})
+            for (int twl : {1, 0}) c[n++] = BrTile{1, twl, 0, tab};
     } else if (tbits == 56) {
         c[n++] = BrTile{1, 1};
     } else if (tbits == 32 && logn == 10) {

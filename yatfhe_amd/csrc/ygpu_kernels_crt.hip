@@ -43,6 +43,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     // barrier words of the four two-wave teams of each ciphertext (one per
     // wave), and the flag a timed-out team barrier sets
     __shared__ uint32_t teamCount[CPW * 4 * 2 + 1];
+    // the specialised shapes may take the forward transform's stage 0 from a
+    // table of w1 d mod p for every digit d, per prime (the tile chooses)
+    constexpr int TAB = (C::L > 0 && C::TAB == 1) ? 1 : 0;
+    constexpr int DH = TAB ? (1 << (C::RB - 1)) : 0;
+    constexpr int DTE = TAB ? ((2 * DH + 1 + 7) & ~7) : 1;
+    __shared__ double dtab[TAB ? NP : 1][DTE];
 
     const int grp = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     const int first = grp * CPW;
@@ -69,6 +75,12 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
     const TorusT* __restrict__ v = static_cast<const TorusT*>(a.v) + static_cast<size_t>(a.v_stride) * ct;
     const PrimeF64 pr = a.prime[tp];
     if (threadIdx.x < CPW * 4 * 2 + 1) teamCount[threadIdx.x] = 0u;
+    if constexpr (TAB) {
+#pragma unroll
+        for (int q = 0; q < NP; q++)
+            build_stage0_table_wide(dtab[q], 2 * DH + 1, DH, a.tw[static_cast<size_t>(q) * 2 * N + 1], a.prime[q], static_cast<int>(threadIdx.x),
+                                    C::THREADS);
+    }
     if constexpr (FTW_LDS) {
         for (int e = static_cast<int>(threadIdx.x); e < NP * N; e += C::THREADS)
             ftw[e / N][e % N] = a.tw[static_cast<size_t>(e / N) * 2 * N + e % N];
@@ -202,21 +214,34 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             GD* __restrict__ kpo = (GD*)(kp + static_cast<size_t>(tm) * NP * N);
             GD* __restrict__ kpt = (GD*)(kp + static_cast<size_t>(1 - tm) * NP * N);
             double x[16];
+            int dv[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                int dv;
                 if constexpr (C::L > 0 && C::RB <= 7) {
-                    dv = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
+                    dv[r] = __builtin_amdgcn_sbfe(static_cast<int>(dg0[r]), 8 * lvl, 8);
                 } else if constexpr (C::L > 0) {
                     const uint32_t w = lvl < 2 ? dg0[r] : dg1[r];
-                    dv = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
+                    dv[r] = __builtin_amdgcn_sbfe(static_cast<int>(w), 16 * (lvl & 1), 16);
                 } else {
-                    dv = digit_runtime(dg0[r], dg1[r], lvl, rb);
+                    dv[r] = digit_runtime(dg0[r], dg1[r], lvl, rb);
                 }
-                x[r] = static_cast<double>(dv);
             }
-            if constexpr (C::L > 0) ntt_forward_sync_t<LOGN>(x, scr, tl, fw0, pr, ts);
-            else ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, ts);
+            if constexpr (TAB) {
+                const double* tb = dtab[tp];
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const double av = static_cast<double>(dv[r]);
+                    const double t = tb[dv[r + 8] + DH];
+                    x[r] = __dadd_rn(av, t);
+                    x[r + 8] = __dadd_rn(av, -t);
+                }
+                ntt_forward_from_stage1_t<LOGN>(x, scr, tl, fw0, pr, ts);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 16; r++) x[r] = static_cast<double>(dv[r]);
+                if constexpr (C::L > 0) ntt_forward_sync_t<LOGN>(x, scr, tl, fw0, pr, ts);
+                else ntt_forward_sync<LOGN>(x, scr, tl, tw, pr, ts);
+            }
             // one column at a time: the scheduling barrier keeps the second
             // column's loads from being hoisted over the first column's
             // products, which would hold 32 more registers live; the first
@@ -320,8 +345,11 @@ hipError_t launch_br_crt(const BrArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_blind_rotate_crt(int logn, const BrTile& t, int radix_bits, int level, const BrArgs& a, hipStream_t s) {
-    if (logn == 11 && level == 3 && radix_bits == 10)
+    if (logn == 11 && level == 3 && radix_bits == 10) {
+        if (t.tab == 1)
+            return t.twl ? launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 1, 1>>(a, s) : launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 0, 1>>(a, s);
         return t.twl ? launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 1>>(a, s) : launch_br_crt<BrCfg<56, 11, 2, 1, false, 3, 10, 0>>(a, s);
+    }
     const bool p8 = radix_bits <= 8;
     if (logn == 10) return p8 ? launch_br_crt<BrCfg<56, 10, 2, 1, true>>(a, s) : launch_br_crt<BrCfg<56, 10, 2, 1, false>>(a, s);
     if (logn == 11) return p8 ? launch_br_crt<BrCfg<56, 11, 2, 1, true>>(a, s) : launch_br_crt<BrCfg<56, 11, 2, 1, false>>(a, s);

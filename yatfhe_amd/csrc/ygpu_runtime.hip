@@ -112,9 +112,12 @@ int build_context(ygpu_ctx* ctx, const ygpu_params* params, int32_t device, hipS
     // default tile: small workgroups (a barrier couples few waves), twiddles in
     // LDS where they fit; TORUS32 N = 1024: two ciphertexts per workgroup and
     // the first two forward stages read from digit tables (45.9 ms against
-    // 48.7 ms without tables for 4096 ciphertexts at n = 630, 2^7 x 3)
+    // 48.7 ms without tables for 4096 ciphertexts at n = 630, 2^7 x 3);
+    // TORUS56 N = 2048, 2^10 x 3: stage 0 from a digit table (562.9 against
+    // 569.4 ms for 8192 ciphertexts at n = 742)
     const bool t32 = params->torus_bits == 32 && params->N == 1024;
-    ctx->tile = BrTile{t32 ? 2 : 1, (t32 || params->torus_bits == 56) ? 1 : 0, 0, t32 ? 2 : 0};
+    const bool t56s = params->torus_bits == 56 && params->N == 2048 && params->level == 3 && params->radix_bits == 10;
+    ctx->tile = BrTile{t32 ? 2 : 1, (t32 || params->torus_bits == 56) ? 1 : 0, 0, t32 ? 2 : t56s ? 1 : 0};
     if (const char* e = std::getenv("YGPU_CPW")) ctx->tile.cpw = std::max(1, std::atoi(e));
     YG_HIP(hipSetDevice(device));
     if (stream) {
