@@ -181,9 +181,10 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             double s[16];
 #pragma unroll
             for (int q = 0; q < 16; q++) {
-                double t = 0.0;
+                double t = scratch[r * L][q * T + tl];
+#pragma unroll
                 for (int l2 = 0; l2 < L; l2++) {
-                    t = __dadd_rn(t, scratch[r * L + l2][q * T + tl]);
+                    if (l2 > 0) t = __dadd_rn(t, scratch[r * L + l2][q * T + tl]);
                     t = __dadd_rn(t, other[(1 - r) * L + l2][q * T + tl]);
                 }
                 s[q] = t;
@@ -191,8 +192,16 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             {
+                // sums of six products (each at most 1.32p): below 8p, so
+                // the first pass reduces only its first stage's sums; pass 2's
+                // outputs are then below 4p (inv_pass_from_4p)
                 WorkgroupSync<1> isy;
-                ntt_inverse_sync_t<LOGN>(s, scr, tl, iw0, pr, isy);
+                inv_pass2_from_8p(s, tl, iw0, pr);
+                exchange<LOGN, 2, 1, 1>(s, scr, tl, isy);
+                inv_pass_from_4p<LOGN, 1>(s, tl, iw0, pr);
+                exchange<LOGN, 1, 0, 0>(s, scr, tl, isy);
+                reduce16(s, pr);
+                inv_pass_t<LOGN, 0>(s, tl, iw0, pr);
             }
             // centre mod q, reduce to the torus, accumulate (mod 2^32)
             centre16_after_inverse(s, pr);
