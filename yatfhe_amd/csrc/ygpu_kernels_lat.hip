@@ -11,9 +11,11 @@
 //               products with the BSK entries of (level l, row r) for both
 //               output columns;
 //   barrier;
-//   wave (c, 0): sums the 2 L partial products of column c, inverse
-//               transforms it, centres, reduces to the torus and accumulates
-//               row c (held in LDS);
+//   waves (c, 0) and (c, 2): each sums the 2 L partial products of column c
+//               over one half of the bit-reversed array, runs stages 9 .. 1
+//               of the inverse transform on it, and the two swap half their
+//               values for stage 0; then each centres, reduces to the torus
+//               and accumulates its half of row c (held in LDS);
 //   barrier.
 // This leaves most waves idle during the inverse transform, so it is the form
 // for batches too small to fill the GPU (the plain drop-in call, short serving
@@ -34,6 +36,56 @@ constexpr int LAT_WAVES = LAT_K1 * LAT_L;
 constexpr int LAT_THREADS = 64 * LAT_WAVES;
 constexpr int LAT_SCR = Scratch<LAT_LOGN>::DOUBLES;
 
+// The inverse transform of one output column, split between two wavefronts.
+// Gentleman-Sande stages 9 .. 1 pair positions within one half of the
+// bit-reversed array (j and j + 2^(9 - st), both below or both above 512);
+// only stage 0 pairs j with j + 512.  Wave h takes half h: 512 values, 8 per
+// lane, in three passes of three stages, then the two waves swap half their
+// values for stage 0.  Index j' within the half of lane L, register e:
+//   pass 1 (stages 9, 8, 7 on bits 0-2):  j' = L << 3 | e
+//   pass 2 (stages 6, 5, 4 on bits 3-5):  j' = (L & 7) | e << 3 | (L >> 3) << 6
+//   pass 3 (stages 3, 2, 1 on bits 6-8):  j' = L | e << 6
+// Register bit k is the bit of stage (first stage of the pass) - k, so the
+// stages run at register distance 1, 2, 4 as in inv_pass_t.
+__device__ __forceinline__ int half_idx(int pass, int lane, int e) {
+    return pass == 1 ? (lane << 3 | e) : pass == 2 ? ((lane & 7) | e << 3 | (lane >> 3) << 6) : (lane | e << 6);
+}
+// Padded LDS addresses of the two exchanges: each 16-lane group of a 64-bit
+// access touches 16 distinct 8-byte slots (mod 32) on the writing and the
+// reading side (575 and 568 doubles, inside a wave's scratch).
+__device__ __forceinline__ int half_xa(int j) { return j + (j >> 3); }
+__device__ __forceinline__ int half_xb(int j) { return j + 8 * (j >> 6); }
+
+// One pass of three Gentleman-Sande stages on a half; `itw` is the flat table
+// of inverse twiddles (index (1 << st) + (j >> (10 - st)) for the pair's lower
+// position j).  With reduce_sums the first stage's four sums are reduced after
+// it (inputs below 8p: those sums are below 16p); the later stages' bounds
+// are then at most 8p (pass 1 from 8p inputs), 4p (pass 2 from reduced
+// inputs) and 5p (pass 3 from 4p inputs), all below 16p <= 2^53.
+template <int PASS>
+__device__ __forceinline__ void half_inv_pass(double (&x)[8], int lane, int h, const double* itw, const PrimeF64& pr, bool reduce_sums) {
+    constexpr int ST0 = 9 - 3 * (PASS - 1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int st = ST0 - k;
+        const int half = 1 << k;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            if (e & half) continue;
+            const int j = 512 * h + half_idx(PASS, lane, e);
+            const double w = itw[(1 << st) + (j >> (10 - st))];
+            const double U = x[e];
+            const double V = x[e + half];
+            x[e] = __dadd_rn(U, V);
+            x[e + half] = mulmod(__dadd_rn(U, -V), w, pr);
+        }
+        if (k == 0 && reduce_sums) {
+#pragma unroll
+            for (int e = 0; e < 8; e += 2) x[e] = reduce(x[e], pr);
+        }
+    }
+}
+
 }  // namespace
 
 __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
@@ -49,7 +101,9 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     // the first two forward stages of a digit polynomial by table lookup
     // (fwd_first_two_stages): w * d mod p for every signed-byte digit d
     __shared__ double firstTab[FIRST_TABLE_ENTRIES];
-    __shared__ uint32_t rowCount[K1 * L + 1];  // digit-share barrier words of each row's waves, then the failure flag
+    // digit-share barrier words of each row's waves, then the failure flag,
+    // then the words of the two inverse waves of each column
+    __shared__ uint32_t rowCount[K1 * L + 1 + K1 * 2];
 
     const int ct = map_workgroup(static_cast<int>(blockIdx.x), static_cast<int>(gridDim.x));
     if (ct >= a.count) return;  // whole workgroup
@@ -58,8 +112,12 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
     const int lvl = w % L;    // gadget level
     const int tl = static_cast<int>(threadIdx.x) % 64;
     double* scr = scratch[w];
-    if (threadIdx.x < K1 * L + 1) rowCount[threadIdx.x] = 0u;
+    if (threadIdx.x < K1 * L + 1 + K1 * 2) rowCount[threadIdx.x] = 0u;
     TeamBarrier<L> rowSync{&rowCount[r * L], &rowCount[K1 * L], 0u, a.barrier_polls, lvl};
+    // waves (r, 0) and (r, 2) share column r's inverse transform: half h = lvl / 2
+    // (on four different SIMDs with the round-robin wave placement)
+    const int ih = lvl >> 1;
+    TeamBarrier<2> pairSync{&rowCount[K1 * L + 1 + r * 2], &rowCount[K1 * L], 0u, a.barrier_polls, ih};
 
     const int n = a.n;
     const int slot = a.ct_slot ? a.ct_slot[ct] : 0;
@@ -175,40 +233,83 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
             other[w][q * T + tl] = pt[q];
         }
         __syncthreads();
-        if (lvl == 0) {
-            // column r: the own products of the waves of row r and the other
-            // products of the waves of row 1 - r
-            double s[16];
+        if (lvl != 1) {
+            // column r, half ih of the bit-reversed array: the own products
+            // of the waves of row r and the other products of the waves of
+            // row 1 - r, read from the forward transform's last-pass layout
+            // (position j at register (j & 3) | (j >> 8) << 2, lane (j >> 2) & 63);
+            // sums of six products (each at most 1.32p): below 8p
+            double s[8];
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                double t = scratch[r * L][q * T + tl];
+            for (int e = 0; e < 8; e++) {
+                const int j = 512 * ih + half_idx(1, tl, e);
+                const int ad = ((j & 3) | (j >> 8) << 2) * T + ((j >> 2) & 63);
+                double t = scratch[r * L][ad];
 #pragma unroll
                 for (int l2 = 0; l2 < L; l2++) {
-                    if (l2 > 0) t = __dadd_rn(t, scratch[r * L + l2][q * T + tl]);
-                    t = __dadd_rn(t, other[(1 - r) * L + l2][q * T + tl]);
+                    if (l2 > 0) t = __dadd_rn(t, scratch[r * L + l2][ad]);
+                    t = __dadd_rn(t, other[(1 - r) * L + l2][ad]);
                 }
-                s[q] = t;
+                s[e] = t;
             }
+            const double* itw = tws + N;
+            half_inv_pass<1>(s, tl, ih, itw, pr, true);
+            // the partner wave has read its share of this wave's scratch
+            // before the exchanges rewrite it
+            pairSync.sync();
+#pragma unroll
+            for (int e = 0; e < 8; e++) scr[half_xa(half_idx(1, tl, e))] = s[e];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            {
-                // sums of six products (each at most 1.32p): below 8p, so
-                // the first pass reduces only its first stage's sums; pass 2's
-                // outputs are then below 4p (inv_pass_from_4p)
-                WorkgroupSync<1> isy;
-                inv_pass2_from_8p(s, tl, iw0, pr);
-                exchange<LOGN, 2, 1, 1>(s, scr, tl, isy);
-                inv_pass_from_4p<LOGN, 1>(s, tl, iw0, pr);
-                exchange<LOGN, 1, 0, 0>(s, scr, tl, isy);
-                reduce16(s, pr);
-                inv_pass_t<LOGN, 0>(s, tl, iw0, pr);
-            }
-            // centre mod q, reduce to the torus, accumulate (mod 2^32)
-            centre16_after_inverse(s, pr);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int j = tl + T * q;
-                acc[r][j] = static_cast<int32_t>(static_cast<uint32_t>(acc[r][j]) + f64_low32(s[q]));
+            for (int e = 0; e < 8; e++) s[e] = reduce(scr[half_xa(half_idx(2, tl, e))], pr);
+            half_inv_pass<2>(s, tl, ih, itw, pr, false);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int e = 0; e < 8; e++) scr[half_xb(half_idx(2, tl, e))] = s[e];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int e = 0; e < 8; e++) s[e] = scr[half_xb(half_idx(3, tl, e))];
+            half_inv_pass<3>(s, tl, ih, itw, pr, true);
+            // stage 0 pairs j' of half 0 (U) with j' of half 1 (V) at the same
+            // lane and register (j' = tl + 64 e): wave ih keeps e in
+            // [4 ih, 4 ih + 4) and hands the other four values to its partner
+            // through the start of its own scratch (its exchange reads are
+            // complete: wave order)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int k = 0; k < 4; k++) scr[k * T + tl] = ih ? s[k] : s[4 + k];
+            pairSync.sync();
+            const double* theirs = scratch[r * L + 2 - lvl];
+            const double w1 = iw0.f[1];
+            double o[8];  // o[k]: position tl + 64 (4 ih + k); o[4 + k]: that position + 512
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double mine = ih ? s[4 + k] : s[k];
+                const double got = theirs[k * T + tl];
+                const double U = ih ? got : mine;
+                const double V = ih ? mine : got;
+                o[k] = __dadd_rn(U, V);
+                o[4 + k] = mulmod(__dadd_rn(U, -V), w1, pr);
+            }
+            // centre mod q: the sums of registers e = 0, 1 (wave 0) can reach
+            // 4p and 10p and take a second reduction; every other output is
+            // below 3p (one reduction is exact below 4p, centre16_after_inverse)
+#pragma unroll
+            for (int k = 0; k < 8; k++) o[k] = reduce(o[k], pr);
+            o[0] = reduce(o[0], pr);
+            o[1] = reduce(o[1], pr);
+            // reduce to the torus, accumulate (mod 2^32)
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int j = tl + 64 * (4 * ih + (k & 3)) + 512 * (k >> 2);
+                acc[r][j] = static_cast<int32_t>(static_cast<uint32_t>(acc[r][j]) + f64_low32(o[k]));
             }
         }
         __syncthreads();
