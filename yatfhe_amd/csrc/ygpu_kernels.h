@@ -214,9 +214,11 @@ __device__ __forceinline__ void ks_digits(int32_t x, int t, int b, int width, in
 // the quotient t = C / q is estimated in binary64 (off by at most one), so the
 // remainder R = C - t q lies in (-3q/2, 3q/2), inside int64 for q < 2^61: it is
 // formed exactly by wrapping 64-bit arithmetic (C itself may exceed 64 bits;
-// its low 64 bits suffice) and corrected into the centred range.
+// its low 64 bits suffice) and corrected into the centred range.  y takes
+// one reduction: a product from mulmod is below 2 p1 in magnitude (inputs
+// below 16p), and one reduction is exact below 4p (centre16_after_inverse).
 __device__ __forceinline__ int64_t crt_mod_q(double c0, double c1, const PrimeF64& P1, const CrtConst& k) {
-    const double y = centre(mulmod(__dadd_rn(c1, -c0), k.inv01, P1), P1);
+    const double y = reduce(mulmod(__dadd_rn(c1, -c0), k.inv01, P1), P1);
     const double Cd = __fma_rn(k.p0, y, c0);
     const int64_t qt = static_cast<int64_t>(__builtin_rint(__dmul_rn(Cd, k.qinv)));
     const uint64_t lowC = static_cast<uint64_t>(k.p0i) * static_cast<uint64_t>(f64_to_i64(y)) + static_cast<uint64_t>(f64_to_i64(c0));
