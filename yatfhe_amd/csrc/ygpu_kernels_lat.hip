@@ -181,12 +181,15 @@ __global__ void __launch_bounds__(LAT_THREADS) k_blind_rotate_lat(BrArgs a) {
         // (X^a - 1) row r (rotateTorusPolynomialMinusOne, polynomial.cpp:213-238)
         // and its digits (gadget_decomposition.h:75-95): the three waves of
         // row r each decompose a share of the positions into LDS, registers
-        // q in [0, 6), [6, 11) and [11, 16) of the lane layout for levels 0,
-        // 1 and 2
+        // q of the lane layout in [0, 4), [4, 8), [8, 16) for levels 0, 1, 2
+        // of row 0 and [0, 8), [8, 12), [12, 16) of row 1.  Waves w and
+        // w + 4 share a SIMD (wave w = 3 r + level), so waves 0 and 4, 1 and
+        // 5 take four registers each and the waves alone on their SIMDs
+        // (2 and 3) take eight: every SIMD decomposes eight.
         const bool flip = wrap < 0;
         {
-            const int q0 = lvl == 0 ? 0 : (lvl == 1 ? 6 : 11);
-            const int q1 = lvl == 0 ? 6 : (lvl == 1 ? 11 : 16);
+            const int q0 = r == 0 ? (lvl == 0 ? 0 : lvl == 1 ? 4 : 8) : (lvl == 0 ? 0 : lvl == 1 ? 8 : 12);
+            const int q1 = r == 0 ? (lvl == 0 ? 4 : lvl == 1 ? 8 : 16) : (lvl == 0 ? 8 : lvl == 1 ? 12 : 16);
             for (int q = q0; q < q1; q++) {
                 const int j = tl + T * q;
                 const int src = (j - aT) & (N - 1);
