@@ -53,9 +53,10 @@ __device__ __forceinline__ void reduce16(double (&x)[16], const PrimeF64& P) {
         for (int b = 0; b < 4; b++) x[g + b] = __fma_rn(-t[b], P.p, x[g + b]);
     }
 }
-// Exact centred representative in [-(p-1)/2, (p-1)/2] of a value |x| < 8p.
-// The first reduction leaves |r| <= (p+1)/2 (its quotient estimate is off by
-// at most 2^-49 of p); the second maps exactly the two out-of-range values
+// Exact centred representative in [-(p-1)/2, (p-1)/2] of a value |x| <= 9p.
+// The first reduction leaves |r| <= (p+1)/2: x * fl(1/p) is within
+// 9p 2^-102 + 2^-50 <= 17 2^-53 of x/p (p < 2^49), so |r| <= p/2 + 1.07; the
+// second maps exactly the two out-of-range values
 // +-(p+1)/2 to -+(p-1)/2: r * fl(1/p) stays below 1/2 - 2^-51 in magnitude for
 // |r| <= (p-1)/2 and above 1/2 + 2^-51 for |r| = (p+1)/2, so its rint is 0 or
 // +-1 as required.  Six instructions, against ten with two compare-selects.
@@ -390,10 +391,12 @@ __device__ __forceinline__ void inv_pass_from_4p(double (&x)[16], int tl, const 
     }
 }
 
-// Exact centred representatives of the outputs of an inverse transform whose
-// last pass (four stages) began with every value reduced (|x| <= p/2 + 2).
-// Register r's worst case after that pass is 8p for r = 0, 4.75p for r = 1
-// and at most 2.75p for the others.  One reduction is exact below 4p: with
+// Exact centred representatives of the outputs of an inverse transform
+// (ntt_inverse_sync_t's reduction schedule: pass 0 from inputs below 3p with
+// its first stage's sums reduced).  Register r's worst case after the last
+// pass is 4p for r = 0, 8.41p for r = 1 and at most 2.79p for the others
+// (with a last pass that began reduced: 8p, 4.75p and 2.75p).  One reduction
+// is exact below 4p, two below 9p (centre): with
 // p < 2^49, the computed x * fl(1/p) is within 2^-51 (from fl(1/p)) plus half
 // an ulp of a value below 4, 2^-52, of x/p, which is less than x/p's distance
 // 1/(2p) > 2^-50 from the nearest half-integer (x is an integer, p odd), so
@@ -579,19 +582,23 @@ __device__ __forceinline__ void ntt_forward_from_stage2(double (&x)[16], double*
 }
 
 // Inverse negacyclic NTT without the 1/N factor (folded into the key):
-// pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Passes 2 and 0
-// begin with every value reduced; pass 2's outputs are then below 4p (at most
-// three stages), so pass 1 reduces only its first stage's sums
-// (inv_pass_from_4p).  Every value stays below 10p; the outputs are below 8p.
+// pass-2 layout (bit-reversed) -> pass-0 layout (natural).  Pass 2 begins
+// with every value reduced, so its outputs are below 4p (at most three
+// stages); pass 1 reduces only its first stage's sums (inv_pass_from_4p), and
+// then its registers 0 and 1, the two whose worst case (4p and 10p) is not
+// below 2.94p; pass 0 starts below 2.94p and again reduces only its first
+// stage's sums.  Every value stays below 10p; the outputs are below 8.41p
+// (centre16_after_inverse gives their exact centred values).
 template <int LOGN, class Sync, class TW>
 __device__ __forceinline__ void ntt_inverse_sync_t(double (&x)[16], double* scr, int tl, const TW& itw, const PrimeF64& pr, Sync& sy) {
     reduce16(x, pr);
     inv_pass_t<LOGN, 2>(x, tl, itw, pr);
     exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
     inv_pass_from_4p<LOGN, 1>(x, tl, itw, pr);
+    x[0] = reduce(x[0], pr);
+    x[1] = reduce(x[1], pr);
     exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
-    reduce16(x, pr);
-    inv_pass_t<LOGN, 0>(x, tl, itw, pr);
+    inv_pass_from_4p<LOGN, 0>(x, tl, itw, pr);
 }
 template <int LOGN, class Sync>
 __device__ __forceinline__ void ntt_inverse_sync(double (&x)[16], double* scr, int tl, const double* __restrict__ itw, const PrimeF64& pr, Sync& sy) {

@@ -361,11 +361,13 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate(BrArg
                     lds_wait_at_least(&readDone[lc * 2 + 1 - tm], static_cast<uint32_t>(i + 1), a.barrier_polls, handFail);
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                     exchange<LOGN, 2, 1, 1>(x, scr, tl, sy);
-                    // pass 2's outputs are below 4p
+                    // pass 2's outputs are below 4p; the reduction schedule
+                    // of ntt_inverse_sync_t from here on
                     inv_pass_from_4p<LOGN, 1>(x, tl, twsrc, pr);
+                    x[0] = reduce(x[0], pr);
+                    x[1] = reduce(x[1], pr);
                     exchange<LOGN, 1, 0, 0>(x, scr, tl, sy);
-                    reduce16(x, pr);
-                    inv_pass_t<LOGN, 0>(x, tl, twsrc, pr);
+                    inv_pass_from_4p<LOGN, 0>(x, tl, twsrc, pr);
                 };
                 if constexpr (LANE) inverse(il0);
                 else if constexpr (P0REG) inverse(iw0);
