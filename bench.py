@@ -664,7 +664,7 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="progress of every serving rate on stderr")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the bootstrapping path on a single MI355X "
+        print(f"bench.py measures one GPU: the engine runs the bootstrapping path on a single MI355X "
               f"(--gpus {args.gpus} requested). Tenant-affine sharding across GPUs is a design note only.",
               file=sys.stderr)
         sys.exit(2)
