@@ -204,7 +204,19 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
 #pragma unroll
             for (int r = 0; r < 16; r++) { own[r] = 0.0; oth[r] = 0.0; }
         }
+        // Issue priority falls as a step advances, so that of the two waves
+        // on one SIMD the one behind issues first.  Waves w and w + 4 of the
+        // workgroup share a SIMD: a prime-0 wave and a prime-1 wave, which
+        // meet again at the next workgroup barrier.  At equal priority the
+        // arbiter favours the older wave, and the prime-0 waves ran ahead
+        // and then idled at the hand-over while the prime-1 waves finished
+        // alone on their SIMDs.  Between two workgroup barriers the priority
+        // only decreases: 3, 2, 1 over the levels, then 0 to the hand-over;
+        // 2 for the inverse, 1 for centring and reconstruction, 0 after.
         auto level = [&](const int lvl) {
+            if (lvl == 0) __builtin_amdgcn_s_setprio(3);
+            else if (lvl == 1) __builtin_amdgcn_s_setprio(2);
+            else __builtin_amdgcn_s_setprio(1);
             const int d = lvl * K1 + tm;
             const double* __restrict__ kp = bskRow + (static_cast<size_t>(d) * K1 * NP + tp) * N + tl;
             // global address space: the compiler cannot infer it through the
@@ -279,6 +291,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
             for (int lvl = 0; lvl < L; lvl++) level(lvl);
         }
 
+        __builtin_amdgcn_s_setprio(0);
         // partner sums (same prime): hand column 1-tm to team (1-tm, tp)
         ts.sync();
 #pragma unroll
@@ -294,12 +307,14 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
         // rewritten (the transform's own barriers order only this team)
         __syncthreads();
 
+        __builtin_amdgcn_s_setprio(2);
         if constexpr (C::L > 0) ntt_inverse_sync_t<LOGN>(x, scr, tl, iw0, pr, ts);
         else ntt_inverse_sync<LOGN>(x, scr, tl, itw, pr, ts);
 
         // residues of column tm mod both primes: the two prime teams of row tm
         // split the reconstruction, team tp taking registers [8 tp, 8 tp + 8);
         // each hands the other team the half it needs
+        __builtin_amdgcn_s_setprio(1);
         centre16_after_inverse(x, pr);
         ts.sync();
 #pragma unroll
@@ -320,6 +335,7 @@ __global__ void __launch_bounds__(C::THREADS, C::MIN_WAVES) k_blind_rotate_crt(B
                 acc[j] = sext_t<C::TBITS>(acc[j] + val);
             }
         }
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
     }
 
